@@ -1,0 +1,167 @@
+"""ctypes bindings for the checker libraries (test infrastructure only).
+
+``liboracle.so`` is the oracle (CPU restatement of the reference hot path);
+``_ref/libsaqref.so`` is the reference's own glog-free headers compiled with its
+release flags.  Both are built by ``oracle/Makefile``.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libsaqref.so")
+
+MAX_SEG = 64
+f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+u16p = np.ctypeslib.ndpointer(np.uint16, flags="C_CONTIGUOUS")
+u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+i64 = C.c_int64
+
+
+class Plan(C.Structure):
+    _fields_ = [("num_seg", C.c_int32), ("num_dim", C.c_int32), ("use_compact_layout", C.c_int32),
+                ("seg_dim", C.c_int32 * MAX_SEG), ("seg_bits", C.c_int32 * MAX_SEG)]
+
+
+class Cluster(C.Structure):
+    _fields_ = [("num_vec", C.c_int64), ("short_factors", C.c_void_p), ("short_code", C.c_void_p),
+                ("long_code", C.c_void_p), ("long_factors", C.c_void_p), ("ids", C.c_void_p),
+                ("centroid_rot", C.c_void_p)]
+
+
+class Sizes(C.Structure):
+    _fields_ = [("num_blocks", C.c_int64), ("short_factor_floats", C.c_int64),
+                ("short_code_bytes", C.c_int64), ("long_code_bytes", C.c_int64),
+                ("long_stride", C.c_int64), ("long_factor_floats", C.c_int64),
+                ("centroid_floats", C.c_int64)]
+
+
+class LutScalars(C.Structure):
+    _fields_ = [("delta", C.c_float), ("sum_q", C.c_float), ("sum_vl_lut", C.c_float),
+                ("q_l2sqr", C.c_float), ("q_l2norm", C.c_float)]
+
+
+class CaqCfg(C.Structure):
+    _fields_ = [("rd_lmt", C.c_int32), ("eps", C.c_float), ("rotate", C.c_int32)]
+
+
+class CodeInfo(C.Structure):
+    _fields_ = [("o_l2norm", C.c_float), ("ip_cent_oa", C.c_float), ("fac_rescale", C.c_float),
+                ("fac_error", C.c_float), ("ip_o_oa", C.c_double), ("oa_l2sqr", C.c_double),
+                ("delta", C.c_double), ("v_mi", C.c_double), ("rounds", C.c_int32), ("status", C.c_int32)]
+
+
+class Metrics(C.Structure):
+    _fields_ = [("fast_bitsum", C.c_uint64), ("acc_bitsum", C.c_uint64), ("total_comp_cnt", C.c_uint64)]
+
+
+_orc = None
+_ref = None
+
+
+def oracle():
+    global _orc
+    if _orc is None:
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError("oracle not built: run make -C oracle")
+        L = C.CDLL(ORACLE_SO)
+        L.orc_eigen_sum.restype = C.c_float
+        L.orc_eigen_sum.argtypes = [f32p, i64]
+        L.orc_eigen_sqnorm.restype = C.c_float
+        L.orc_eigen_sqnorm.argtypes = [f32p, i64]
+        L.orc_eigen_dot.restype = C.c_float
+        L.orc_eigen_dot.argtypes = [f32p, f32p, i64]
+        L.orc_eigen_diff_sqnorm.restype = C.c_float
+        L.orc_eigen_diff_sqnorm.argtypes = [f32p, f32p, i64]
+        L.orc_eigen_var_q2.restype = C.c_float
+        L.orc_eigen_var_q2.argtypes = [f32p, f32p, i64]
+        L.orc_gemv_rowvec.argtypes = [f32p, f32p, i64, f32p]
+        L.orc_tree16.restype = C.c_float
+        L.orc_tree16.argtypes = [f32p]
+        L.orc_pack_lut.argtypes = [i64, f32p, f32p]
+        L.orc_pack_codes.argtypes = [i64, u8p, i64, u8p]
+        L.orc_lut_prepare.argtypes = [i64, f32p, u16p, C.POINTER(LutScalars)]
+        L.orc_accumulate.argtypes = [u8p, u16p, i64, u32p]
+        L.orc_cvt_u16.restype = C.c_uint16
+        L.orc_cvt_u16.argtypes = [C.c_float]
+        L.orc_compute_ip.restype = C.c_float
+        L.orc_compute_ip.argtypes = [C.c_int, f32p, u8p, i64]
+        L.orc_compact_code.argtypes = [C.c_int, u16p, i64, u8p]
+        L.orc_code_at.restype = C.c_int
+        L.orc_code_at.argtypes = [C.c_int, u8p, i64, i64]
+        L.orc_caq_encode.restype = C.c_int
+        L.orc_caq_encode.argtypes = [f32p, i64, C.c_int, C.POINTER(CaqCfg), C.c_void_p, u16p,
+                                     C.POINTER(CodeInfo)]
+        L.orc_rotate_rows.argtypes = [f32p, i64, i64, C.c_void_p, f32p, f32p]
+        L.orc_quantize_segment.restype = C.c_int
+        L.orc_quantize_segment.argtypes = [C.POINTER(Plan), C.c_int, f32p, i64, f32p, C.c_void_p,
+                                           C.POINTER(CaqCfg), C.c_void_p, C.c_void_p]
+        L.orc_cluster_sizes.argtypes = [C.POINTER(Plan), i64, C.POINTER(Sizes)]
+        L.orc_plan_dp.restype = C.c_int
+        L.orc_plan_dp.argtypes = [f32p, i64, C.c_float, i32p, i32p]
+        L.orc_plan_eqseg.restype = C.c_int
+        L.orc_plan_eqseg.argtypes = [i64, C.c_float, C.c_int, i32p, i32p]
+        L.orc_probe.argtypes = [f32p, i64, i64, f32p, i64, C.c_int, u32p, C.c_void_p]
+        L.orc_search.restype = C.c_int
+        L.orc_search.argtypes = [C.POINTER(Plan), f32p, C.c_void_p, C.c_void_p, u32p, i64, f32p, i64,
+                                 C.c_float, C.c_int, u32p, C.c_void_p, C.c_void_p]
+        L.orc_search_batch.restype = C.c_int
+        L.orc_search_batch.argtypes = [C.POINTER(Plan), f32p, C.c_void_p, C.c_void_p, u32p, i64, f32p,
+                                       i64, i64, C.c_float, C.c_int, C.c_int, u32p, C.c_void_p,
+                                       C.c_void_p]
+        L.orc_estimate.restype = C.c_int
+        L.orc_estimate.argtypes = [C.POINTER(Plan), f32p, C.c_void_p, C.c_void_p, u32p, i64, f32p,
+                                   C.c_float, C.c_int, u32p, f32p, C.c_void_p, C.c_void_p, C.c_void_p]
+        _orc = L
+    return _orc
+
+
+def ref():
+    """The reference's own kernels, or None where they are not built."""
+    global _ref
+    if _ref is None:
+        if not os.path.exists(REF_SO):
+            return None
+        try:
+            L = C.CDLL(REF_SO)
+        except OSError:
+            return None
+        L.ref_pack_codes.argtypes = [i64, u8p, i64, u8p]
+        L.ref_pack_lut.argtypes = [i64, f32p, f32p]
+        L.ref_accumulate.argtypes = [u8p, u16p, i64, i32p]
+        L.ref_compute_ip.restype = C.c_float
+        L.ref_compute_ip.argtypes = [C.c_int, f32p, u8p, i64]
+        L.ref_compact_code.argtypes = [C.c_int, u16p, i64, u8p]
+        L.ref_eigen_sum.restype = C.c_float
+        L.ref_eigen_sum.argtypes = [f32p, i64]
+        L.ref_eigen_sqnorm.restype = C.c_float
+        L.ref_eigen_sqnorm.argtypes = [f32p, i64]
+        L.ref_eigen_dot_rowcol.restype = C.c_float
+        L.ref_eigen_dot_rowcol.argtypes = [f32p, f32p, i64]
+        L.ref_eigen_diff_sqnorm.restype = C.c_float
+        L.ref_eigen_diff_sqnorm.argtypes = [f32p, i64, i64, f32p, i64]
+        L.ref_eigen_var_q2.restype = C.c_float
+        L.ref_eigen_var_q2.argtypes = [f32p, i64, i64, f32p, i64]
+        L.ref_eigen_gemv.argtypes = [f32p, f32p, i64, f32p]
+        L.ref_eigen_gemm.argtypes = [f32p, i64, i64, f32p, f32p]
+        L.ref_lut_quantize.argtypes = [f32p, i64, C.c_float, C.c_float, u16p]
+        L.ref_lut_delta.restype = C.c_float
+        L.ref_lut_delta.argtypes = [C.c_float, C.c_float]
+        L.ref_code_floor.argtypes = [f32p, i64, C.c_double, C.c_double, C.c_int, f32p]
+        _ref = L
+    return _ref
+
+
+def make_plan(dims, bits, num_dim, compact=False):
+    p = Plan()
+    p.num_seg = len(dims)
+    p.num_dim = num_dim
+    p.use_compact_layout = 1 if compact else 0
+    for i, (d, b) in enumerate(zip(dims, bits)):
+        p.seg_dim[i] = int(d)
+        p.seg_bits[i] = int(b)
+    return p
