@@ -1,0 +1,154 @@
+/*
+ * saqgpu.h -- C ABI of the MI355X (gfx950) SAQ hot path.
+ *
+ * The reference (howarlii/SAQ) is a header-only C++ library with no FFI: its
+ * natural seams are IVF::search / SAQSearcher::searchCluster for the scan and
+ * IVF::construct / SAQuantizer::quantize_cluster for the encoder.  Every entry
+ * point below replaces one of those seams; the host-side saqlib mirror
+ * (saq_amd/host/saqlib_gpu.hpp) binds them exactly as a maintainer would bind
+ * them from the reference's own IVF class.
+ *
+ * Conventions: plain C types, no exceptions cross the boundary, every call
+ * returns an saqgpu_status (0 = ok) and saqgpu_last_error() describes the last
+ * failure of the calling thread.  Pointers flagged SAQGPU_DEVICE_PTRS are HIP
+ * device pointers on the index's device; otherwise host memory.
+ */
+#ifndef SAQGPU_H
+#define SAQGPU_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    SAQGPU_OK = 0,
+    SAQGPU_EINVAL = 1,      /* bad argument (the reference CHECK-aborts on these) */
+    SAQGPU_EDEVICE = 2,     /* HIP runtime error */
+    SAQGPU_ENOMEM = 3,      /* device allocation failed */
+    SAQGPU_ESTATE = 4,      /* index not in the right state (e.g. cluster not set) */
+    SAQGPU_EUNSUPPORTED = 5 /* configuration outside what the kernels implement */
+} saqgpu_status;
+
+#define SAQGPU_MAX_SEG 64
+#define SAQGPU_MAX_TOPK 1024
+#define SAQGPU_DEVICE_PTRS 1
+
+typedef struct saqgpu_index saqgpu_index;
+
+/* Index description: the persistent part of IVF + SaqData
+ * (reference: saqlib/index/ivf.hpp:27-121, saqlib/quantization/saq_data.hpp:20-57). */
+typedef struct {
+    int64_t num_data;            /* IVF::num_data_ */
+    int32_t num_dim;             /* IVF::num_dim_ (original dimension) */
+    int32_t num_cen;             /* IVF::num_cen_ (K) */
+    int32_t num_seg;             /* SaqData::quant_plan.size() */
+    int32_t use_compact_layout;  /* QuantizeConfig::use_compact_layout */
+    const int32_t *seg_dim;      /* quant_plan[i].first  (multiples of 64) */
+    const int32_t *seg_bits;     /* quant_plan[i].second (0..13) */
+    const float *rotations;      /* concatenated P_s, row-major D_s x D_s; NULL = no rotation */
+    const float *variance;       /* SaqData::data_variance, padded dimension floats */
+    const float *centroids;      /* FlatInitializer centroids, K x num_dim */
+    const int64_t *cluster_sizes;/* K entries */
+    int32_t device;              /* HIP device ordinal */
+} saqgpu_index_desc;
+
+/* Sizes of one cluster's arrays in the reference SaqCluData layout
+ * (saqlib/quantization/cluster_data.hpp:206-294, 317-336). */
+typedef struct {
+    int64_t num_blocks;
+    int64_t short_factor_floats;
+    int64_t short_code_bytes;
+    int64_t long_code_bytes;
+    int64_t long_factor_floats;
+    int64_t centroid_floats;
+} saqgpu_cluster_sizes;
+
+/* CAQ encoder configuration (QuantSingleConfig, saqlib/quantization/config.h:12-19). */
+typedef struct {
+    int32_t caq_adj_rd_lmt; /* adjustment round limit (0 disables adjustment) */
+    float caq_adj_eps;      /* adjustment epsilon */
+} saqgpu_caq_cfg;
+
+/* Search parameters (IVF::search arguments + SearcherConfig, ivf.hpp:91-94, config.h:75-78). */
+typedef struct {
+    int32_t topk;
+    int32_t nprobe;
+    float searcher_vars_bound_m;
+    int32_t dist_type; /* 0 = L2Sqr, 1 = IP (reproduces the reference's behaviour) */
+} saqgpu_search_params;
+
+/* Per-query counters of QueryRuntimeMetrics (saqlib/quantization/caq/caq_estimator.hpp:21-25). */
+typedef struct {
+    uint64_t fast_bitsum;
+    uint64_t acc_bitsum;
+    uint64_t total_comp_cnt;
+} saqgpu_metrics;
+
+const char *saqgpu_last_error(void);
+
+/* Replaces IVF(n, dim, K, cfg) + SaqDataMaker/prepare_quantizers + IVF::load's header part
+ * (ivf.hpp:68-72, 230-255).  Allocates every device array of the index. */
+int saqgpu_index_create(const saqgpu_index_desc *desc, saqgpu_index **out);
+void saqgpu_index_destroy(saqgpu_index *idx);
+
+int saqgpu_cluster_sizes_of(const saqgpu_index *idx, int32_t cid, saqgpu_cluster_sizes *out);
+
+/* Uploads one cluster given in the reference layout (SaqCluData::load, cluster_data.hpp:317-326);
+ * the library transposes it into its device layout.  Host pointers. */
+int saqgpu_index_set_cluster(saqgpu_index *idx, int32_t cid, const float *short_factors,
+                             const uint8_t *short_code, const uint8_t *long_code,
+                             const float *long_factors, const uint32_t *ids,
+                             const float *centroid_rot);
+
+/* Downloads one cluster back into the reference layout (SaqCluData::save, :327-336). */
+int saqgpu_index_get_cluster(const saqgpu_index *idx, int32_t cid, float *short_factors,
+                             uint8_t *short_code, uint8_t *long_code, float *long_factors,
+                             uint32_t *ids, float *centroid_rot);
+
+/* Replaces IVF::construct's quantisation stage (ivf.hpp:149-183 -> SAQuantizer::quantize_cluster,
+ * saq_quantizer.hpp:34-60): encodes data (n x num_dim) with its cluster assignment into the
+ * device-resident index.  data/cluster_ids are host pointers unless flags has
+ * SAQGPU_DEVICE_PTRS.  Returns the number of vectors whose adjustment loop hit the
+ * reference's CHECK_GE(ip, 0) in *n_check_failed (may be NULL). */
+int saqgpu_encode(saqgpu_index *idx, const float *data, const uint32_t *cluster_ids, int64_t n,
+                  const saqgpu_caq_cfg *cfg, int32_t flags, int64_t *n_check_failed, void *stream);
+
+/* Encodes rotated residuals directly (the per-vector CAQEncoder::encode_and_fac + packing,
+ * caq_encoder.hpp:222-247, cluster_packer.hpp:59-79) for one segment of one cluster whose
+ * rotated centroid is already set: o is num_vec x D_s (host).  Used to pin the encoder
+ * independently of the rotation order. */
+int saqgpu_encode_residuals(saqgpu_index *idx, int32_t cid, int32_t seg, const float *o,
+                            const float *centroid_seg, const saqgpu_caq_cfg *cfg);
+
+/* Replaces FlatInitializer::centroids_distances (index/initializer.hpp:52-76) for a batch:
+ * probes[q * nprobe + i] = i-th closest centroid of query q. */
+int saqgpu_probe(saqgpu_index *idx, const float *queries, int64_t nq, int32_t nprobe,
+                 int32_t dist_type, uint32_t *probes, int32_t flags, void *stream);
+
+/* Replaces IVF::search (ivf.hpp:275-311) for a batch of queries (nq x num_dim).  probes may be
+ * NULL (the GPU coarse quantiser picks them) or nq x nprobe cluster ids.  out_ids is
+ * nq x topk (unfilled = 0, like ResultPool::copy_results); out_dists and metrics may be NULL. */
+int saqgpu_search(saqgpu_index *idx, const float *queries, int64_t nq, const uint32_t *probes,
+                  const saqgpu_search_params *params, uint32_t *out_ids, float *out_dists,
+                  saqgpu_metrics *metrics, int32_t flags, void *stream);
+
+/* Replaces IVF::estimate (ivf.hpp:313-362) for one query: accurate, fast and variance
+ * estimates of every vector of the probed clusters, in probe order; returns the count
+ * through *n_out. */
+int saqgpu_estimate(saqgpu_index *idx, const float *query, const uint32_t *probes, int32_t nprobe,
+                    float vars_bound_m, int32_t dist_type, uint32_t *out_ids, float *out_acc,
+                    float *out_fast, float *out_vars, int64_t *n_out);
+
+/* SaqDataMaker::analyze_plan (saqlib/quantization/saq_data.hpp:119-227): segment plan from the
+ * per-dimension variance (num_dim floats).  dims/bits receive up to SAQGPU_MAX_SEG entries. */
+int saqgpu_make_plan(const float *variance, int32_t num_dim, float avg_bits, int32_t enable_segmentation,
+                     int32_t seg_eqseg, int32_t *dims, int32_t *bits, int32_t *num_seg);
+
+/* Device memory held by the index (bytes), for capacity planning. */
+int64_t saqgpu_index_device_bytes(const saqgpu_index *idx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

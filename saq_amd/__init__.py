@@ -1,0 +1,218 @@
+"""saq_amd -- MI355X (gfx950) implementation of SAQ's IVF scan and CAQ encoder.
+
+Python binding of the C ABI in include/saqgpu.h (libsaqgpu.so, built in-tree by
+saq_amd/build.py).  There is no CPU fallback: if the library is missing or the
+GPU call fails, the call raises.
+
+The Index class mirrors the reference's saqlib::IVF surface (construct/search/
+estimate/save/load, saqlib/index/ivf.hpp:84-100) on top of the C ABI.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsaqgpu.so")
+MAX_SEG = 64
+DEVICE_PTRS = 1
+
+
+class SaqGpuError(RuntimeError):
+    pass
+
+
+class _Desc(C.Structure):
+    _fields_ = [("num_data", C.c_int64), ("num_dim", C.c_int32), ("num_cen", C.c_int32),
+                ("num_seg", C.c_int32), ("use_compact_layout", C.c_int32),
+                ("seg_dim", C.c_void_p), ("seg_bits", C.c_void_p), ("rotations", C.c_void_p),
+                ("variance", C.c_void_p), ("centroids", C.c_void_p), ("cluster_sizes", C.c_void_p),
+                ("device", C.c_int32)]
+
+
+class ClusterSizes(C.Structure):
+    _fields_ = [("num_blocks", C.c_int64), ("short_factor_floats", C.c_int64),
+                ("short_code_bytes", C.c_int64), ("long_code_bytes", C.c_int64),
+                ("long_factor_floats", C.c_int64), ("centroid_floats", C.c_int64)]
+
+
+class CaqCfg(C.Structure):
+    _fields_ = [("caq_adj_rd_lmt", C.c_int32), ("caq_adj_eps", C.c_float)]
+
+
+class SearchParams(C.Structure):
+    _fields_ = [("topk", C.c_int32), ("nprobe", C.c_int32), ("searcher_vars_bound_m", C.c_float),
+                ("dist_type", C.c_int32)]
+
+
+_lib = None
+
+_SIGS = {
+    "saqgpu_last_error": (C.c_char_p, []),
+    "saqgpu_index_create": (C.c_int, [C.POINTER(_Desc), C.POINTER(C.c_void_p)]),
+    "saqgpu_index_destroy": (None, [C.c_void_p]),
+    "saqgpu_cluster_sizes_of": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(ClusterSizes)]),
+    "saqgpu_index_set_cluster": (C.c_int, [C.c_void_p, C.c_int32] + [C.c_void_p] * 6),
+    "saqgpu_index_get_cluster": (C.c_int, [C.c_void_p, C.c_int32] + [C.c_void_p] * 6),
+    "saqgpu_encode": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(CaqCfg), C.c_int32,
+                                C.c_void_p, C.c_void_p]),
+    "saqgpu_encode_residuals": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
+                                          C.POINTER(CaqCfg)]),
+    "saqgpu_probe": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_void_p, C.c_int32,
+                               C.c_void_p]),
+    "saqgpu_search": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.POINTER(SearchParams),
+                                C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
+    "saqgpu_estimate": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_float, C.c_int32,
+                                  C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "saqgpu_index_device_bytes": (C.c_int64, [C.c_void_p]),
+}
+
+
+def lib():
+    """The loaded C ABI.  Raises if the gfx950 library has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise SaqGpuError(f"{LIB_PATH} is missing: build it with python saq_amd/build.py")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    return list(_SIGS)
+
+
+def _check(rc):
+    if rc != 0:
+        raise SaqGpuError(f"saqgpu error {rc}: {lib().saqgpu_last_error().decode()}")
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        assert a.flags["C_CONTIGUOUS"]
+        return a.ctypes.data
+    return a.data_ptr()  # torch tensor on the device
+
+
+class Index:
+    """Device-resident SAQ IVF index (the state of saqlib::IVF, ivf.hpp:27-121)."""
+
+    def __init__(self, num_dim, plan, centroids, cluster_sizes, variance, rotations=None,
+                 use_compact_layout=False, device=0):
+        self.num_dim = int(num_dim)
+        self.plan = [(int(d), int(b)) for d, b in plan]
+        self.centroids = np.ascontiguousarray(centroids, dtype=np.float32)
+        self.num_cen = self.centroids.shape[0]
+        self.cluster_sizes = np.ascontiguousarray(cluster_sizes, dtype=np.int64)
+        self.num_data = int(self.cluster_sizes.sum())
+        self.D_pad = sum(d for d, _ in self.plan)
+        self.variance = np.zeros(self.D_pad, np.float32)
+        self.variance[:len(variance)] = variance
+        self.rotations = None
+        if rotations is not None:
+            self.rotations = np.ascontiguousarray(np.concatenate([np.asarray(r, np.float32).ravel()
+                                                                  for r in rotations]))
+        dims = np.array([d for d, _ in self.plan], np.int32)
+        bits = np.array([b for _, b in self.plan], np.int32)
+        self._keep = (dims, bits)
+        desc = _Desc(self.num_data, self.num_dim, self.num_cen, len(self.plan), int(use_compact_layout),
+                     dims.ctypes.data, bits.ctypes.data, _ptr(self.rotations), self.variance.ctypes.data,
+                     self.centroids.ctypes.data, self.cluster_sizes.ctypes.data, device)
+        h = C.c_void_p()
+        _check(lib().saqgpu_index_create(C.byref(desc), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().saqgpu_index_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def cluster_sizes_of(self, cid):
+        s = ClusterSizes()
+        _check(lib().saqgpu_cluster_sizes_of(self.h, cid, C.byref(s)))
+        return s
+
+    def set_cluster(self, cid, clu):
+        """Uploads a cluster given in the reference SaqCluData layout (dict of arrays)."""
+        _check(lib().saqgpu_index_set_cluster(
+            self.h, cid, _ptr(clu.get("short_factors")), _ptr(clu["short_code"]), _ptr(clu["long_code"]),
+            _ptr(clu["long_factors"]), _ptr(clu["ids"]), _ptr(clu["centroid_rot"])))
+
+    def get_cluster(self, cid):
+        s = self.cluster_sizes_of(cid)
+        n = int(self.cluster_sizes[cid])
+        out = {
+            "short_factors": np.zeros(max(1, s.short_factor_floats), np.float32),
+            "short_code": np.zeros(max(1, s.short_code_bytes), np.uint8),
+            "long_code": np.zeros(max(1, s.long_code_bytes), np.uint8),
+            "long_factors": np.zeros(max(1, s.long_factor_floats), np.float32),
+            "ids": np.zeros(max(1, n), np.uint32),
+            "centroid_rot": np.zeros(self.D_pad, np.float32),
+        }
+        _check(lib().saqgpu_index_get_cluster(self.h, cid, *[_ptr(out[k]) for k in
+                                                              ("short_factors", "short_code", "long_code",
+                                                               "long_factors", "ids", "centroid_rot")]))
+        out["short_factors"] = out["short_factors"][:s.short_factor_floats]
+        out["short_code"] = out["short_code"][:s.short_code_bytes]
+        out["long_code"] = out["long_code"][:s.long_code_bytes]
+        out["long_factors"] = out["long_factors"][:s.long_factor_floats]
+        out["ids"] = out["ids"][:n]
+        return out
+
+    def encode(self, data, cluster_ids, rd_lmt=6, eps=1e-8, stream=None):
+        """IVF::construct's quantisation on the GPU (data: host ndarray or device tensor)."""
+        cfg = CaqCfg(rd_lmt, eps)
+        nfail = C.c_int64(0)
+        dev = not isinstance(data, np.ndarray)
+        _check(lib().saqgpu_encode(self.h, _ptr(data), _ptr(cluster_ids), int(data.shape[0]), C.byref(cfg),
+                                   DEVICE_PTRS if dev else 0, C.byref(nfail), stream))
+        return int(nfail.value)
+
+    def encode_residuals(self, cid, seg, o, centroid_seg, rd_lmt=6, eps=1e-8):
+        cfg = CaqCfg(rd_lmt, eps)
+        _check(lib().saqgpu_encode_residuals(self.h, cid, seg, _ptr(np.ascontiguousarray(o, np.float32)),
+                                             _ptr(np.ascontiguousarray(centroid_seg, np.float32)), C.byref(cfg)))
+
+    def probe(self, queries, nprobe, dist_type=0, out=None, stream=None):
+        dev = not isinstance(queries, np.ndarray)
+        nq = int(queries.shape[0])
+        if out is None:
+            out = np.zeros((nq, nprobe), np.uint32)
+        _check(lib().saqgpu_probe(self.h, _ptr(queries), nq, nprobe, dist_type, _ptr(out),
+                                  DEVICE_PTRS if dev else 0, stream))
+        return out
+
+    def search(self, queries, topk, nprobe, probes=None, m=4.0, dist_type=0, out_ids=None, out_dists=None,
+               metrics=None, stream=None, want_dists=True, want_metrics=True):
+        """IVF::search for a batch (queries: nq x num_dim).  Returns (ids, dists, metrics)."""
+        dev = not isinstance(queries, np.ndarray)
+        nq = int(queries.shape[0])
+        p = SearchParams(topk, nprobe, m, dist_type)
+        if not dev:
+            queries = np.ascontiguousarray(queries, np.float32)
+            if probes is not None:
+                probes = np.ascontiguousarray(probes, np.uint32)
+            out_ids = np.zeros((nq, topk), np.uint32) if out_ids is None else out_ids
+            if want_dists and out_dists is None:
+                out_dists = np.zeros((nq, topk), np.float32)
+            if want_metrics and metrics is None:
+                metrics = np.zeros((nq, 3), np.uint64)
+        _check(lib().saqgpu_search(self.h, _ptr(queries), nq, _ptr(probes), C.byref(p), _ptr(out_ids),
+                                   _ptr(out_dists), _ptr(metrics), DEVICE_PTRS if dev else 0, stream))
+        return out_ids, out_dists, metrics
+
+    def device_bytes(self):
+        return int(lib().saqgpu_index_device_bytes(self.h))

@@ -1,0 +1,585 @@
+// api.cpp -- host side of the C ABI declared in include/saqgpu.h.
+//
+// Owns the device-resident index, converts between the reference's SaqCluData layout
+// (saqlib/quantization/cluster_data.hpp:179-337) and the device layout documented in
+// common.hpp, and dispatches the gfx950 kernels.  No CPU fallback exists: every search
+// and encode runs on the GPU or fails with a status.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/saqgpu.h"
+#include "common.hpp"
+
+namespace saqgpu {
+hipError_t launch_query_prep(const IndexView &ix, const float *queries, int64_t nq, const float *variance,
+                             const float *rot, const int64_t *rot_off, float m, float *qrot, float *bound,
+                             int max_dim, hipStream_t st);
+hipError_t launch_scan(const IndexView &ix, const float *qrot, const float *bound, const uint32_t *probes,
+                       int nprobe, int topk, int64_t nq, int dist_type, uint32_t *out_ids, float *out_dists,
+                       uint64_t *out_metrics, size_t smem, hipStream_t st);
+size_t scan_smem_bytes(const IndexView &ix, int topk);
+hipError_t launch_probe(const float *centroids, int K, int D, const float *queries, int64_t nq, int nprobe,
+                        int dist_type, float *dist_ws, uint32_t *probes, hipStream_t st);
+hipError_t launch_encode(const IndexView &ix, const float *data, int64_t n, const uint32_t *order,
+                         const int64_t *dst_vec, const uint32_t *vec_cluster, const float *centroids_unrot,
+                         const float *rot, const int64_t *rot_off, int rd_lmt, float eps, float *ws_o,
+                         uint16_t *ws_code, int *ws_fail, int64_t chunk, int seg, hipStream_t st);
+hipError_t launch_encode_finalize(const IndexView &ix, const float *crot_src, hipStream_t st);
+}  // namespace saqgpu
+
+using namespace saqgpu;
+
+namespace {
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+#define HIPCHK(expr)                                                                        \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail(SAQGPU_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr int kPermLo[16] = {0, 8, 1, 9, 2, 10, 3, 11, 4, 12, 5, 13, 6, 14, 7, 15};  // fastscan.hpp:41-42
+
+inline int64_t rd_up(int64_t x, int64_t y) { return (x + y - 1) / y * y; }
+
+template <class T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    int alloc(size_t cnt) {
+        if (p) hipFree(p);
+        p = nullptr;
+        n = cnt;
+        if (cnt == 0) return 0;
+        if (hipMalloc(&p, cnt * sizeof(T)) != hipSuccess) {
+            p = nullptr;
+            return -1;
+        }
+        return 0;
+    }
+    int ensure(size_t cnt) { return cnt <= n && p ? 0 : alloc(cnt); }
+    void release() {
+        if (p) hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+}  // namespace
+
+struct saqgpu_index {
+    int device = 0;
+    int64_t num_data = 0;
+    int num_dim = 0, num_cen = 0, nseg = 0, D_pad = 0, ngroups = 0, max_dim = 0;
+    bool compact = false, has_rot = false;
+    std::vector<int32_t> seg_dim, seg_bits;
+    std::vector<SegDesc> segs;
+    std::vector<int64_t> clu_blk, clu_vec, rot_off;
+    std::vector<int32_t> clu_n;
+    std::vector<uint8_t> clu_set;
+    int64_t total_blocks = 0, block_stride = 0, long_stride = 0;
+    DevBuf<uint8_t> blocks, long_codes;
+    DevBuf<float> exfac, ipcent, crot, centroids, rot, variance;
+    DevBuf<uint32_t> ids;
+    DevBuf<int64_t> d_clu_blk, d_clu_vec, d_rot_off;
+    DevBuf<int32_t> d_clu_n;
+    DevBuf<SegDesc> d_segs;
+    // search workspace
+    std::mutex mu;
+    DevBuf<float> ws_q, ws_qrot, ws_bound, ws_dists, ws_cdist;
+    DevBuf<uint32_t> ws_probes, ws_ids;
+    DevBuf<uint64_t> ws_met;
+
+    IndexView view() const {
+        IndexView v{};
+        v.nseg = nseg;
+        v.num_dim = num_dim;
+        v.D_pad = D_pad;
+        v.num_cen = num_cen;
+        v.ngroups = ngroups;
+        v.fac_floats = nseg * kBlock;
+        v.block_stride = block_stride;
+        v.long_stride = long_stride;
+        v.blocks = blocks.p;
+        v.long_codes = long_codes.p;
+        v.exfac = exfac.p;
+        v.ids = ids.p;
+        v.crot = crot.p;
+        v.ipcent = ipcent.p;
+        v.clu_blk = d_clu_blk.p;
+        v.clu_vec = d_clu_vec.p;
+        v.clu_n = d_clu_n.p;
+        v.segs = d_segs.p;
+        return v;
+    }
+
+    // reference layout of one cluster (SaqCluData ctor, cluster_data.hpp:206-294)
+    struct RefLayout {
+        int64_t nb, fac_per_block, code_per_block, long_total, long_stride_ref;
+        std::vector<int64_t> code_off, long_off, long_bytes;
+        bool single, compact;
+    };
+    RefLayout ref_layout(int64_t n) const {
+        RefLayout L;
+        L.nb = (n + kBlock - 1) / kBlock;
+        L.single = nseg == 1;
+        L.compact = L.single || compact;
+        L.code_off.resize(nseg);
+        L.long_off.resize(nseg);
+        L.long_bytes.resize(nseg);
+        int64_t code = 0, lpv = 0, ltot = 0;
+        for (int s = 0; s < nseg; ++s) {
+            L.code_off[s] = code;
+            code += seg_bits[s] ? (int64_t)seg_dim[s] * kBlock / 8 : 0;
+            L.long_bytes[s] = seg_bits[s] ? (int64_t)seg_dim[s] * (seg_bits[s] - 1) / 8 : 0;
+            if (L.compact) {
+                L.long_off[s] = ltot;
+                ltot += rd_up(L.long_bytes[s] * n, 16);
+            } else {
+                L.long_off[s] = lpv;
+                lpv += rd_up(L.long_bytes[s], 16);
+            }
+        }
+        if (L.single) {
+            L.fac_per_block = 0;
+            for (auto &c : L.code_off) c += 2 * kBlock * 4;
+            L.code_per_block = 2 * kBlock * 4 + code;
+        } else {
+            L.fac_per_block = (int64_t)nseg * 2 * kBlock;
+            L.code_per_block = code;
+        }
+        L.long_stride_ref = L.compact ? 0 : lpv;
+        L.long_total = L.compact ? ltot : lpv * n;
+        return L;
+    }
+};
+
+extern "C" {
+
+const char *saqgpu_last_error(void) { return g_err.c_str(); }
+
+int saqgpu_index_create(const saqgpu_index_desc *d, saqgpu_index **out) {
+    if (!d || !out) return fail(SAQGPU_EINVAL, "null argument");
+    *out = nullptr;
+    if (d->num_seg < 1 || d->num_seg > SAQGPU_MAX_SEG) return fail(SAQGPU_EINVAL, "num_seg out of range");
+    if (d->num_dim < 1 || d->num_cen < 1 || d->num_data < 0) return fail(SAQGPU_EINVAL, "bad sizes");
+    if (!d->seg_dim || !d->seg_bits) return fail(SAQGPU_EINVAL, "null plan");
+    int dsum = 0;
+    for (int s = 0; s < d->num_seg; ++s) {
+        if (d->seg_dim[s] <= 0 || d->seg_dim[s] % 64 || d->seg_bits[s] < 0 || d->seg_bits[s] > 13)
+            return fail(SAQGPU_EINVAL, "segment dims must be multiples of 64 and bits in 0..13");
+        dsum += d->seg_dim[s];
+    }
+    if (dsum != (int)rd_up(d->num_dim, 64)) return fail(SAQGPU_EINVAL, "segment dims must sum to the padded dimension");
+    auto *ix = new saqgpu_index();
+    ix->device = d->device;
+    if (hipSetDevice(d->device) != hipSuccess) {
+        delete ix;
+        return fail(SAQGPU_EDEVICE, "hipSetDevice failed");
+    }
+    ix->num_data = d->num_data;
+    ix->num_dim = d->num_dim;
+    ix->num_cen = d->num_cen;
+    ix->nseg = d->num_seg;
+    ix->compact = d->use_compact_layout != 0;
+    ix->has_rot = d->rotations != nullptr;
+    int doff = 0, goff = 0, loff = 0;
+    int64_t roff = 0;
+    for (int s = 0; s < d->num_seg; ++s) {
+        const int D = d->seg_dim[s], b = d->seg_bits[s];
+        if (D <= 0 || D % 64 || b < 0 || b > 13) {
+            delete ix;
+            return fail(SAQGPU_EINVAL, "segment dims must be multiples of 64 and bits in 0..13");
+        }
+        SegDesc sd{};
+        sd.dim = D;
+        sd.bits = b;
+        sd.doff = doff;
+        sd.group_off = goff;
+        sd.ngroups = b ? D / 64 : 0;
+        sd.long_off = loff;
+        sd.long_bytes = b ? D * (b - 1) / 8 : 0;
+        sd.rescale_sqrtD = (4 / 0.8f) * (float)(1.0 / (double)std::sqrt((float)D));
+        sd.sqd = (float)(2.0 / (1 << b));
+        ix->segs.push_back(sd);
+        ix->seg_dim.push_back(D);
+        ix->seg_bits.push_back(b);
+        ix->rot_off.push_back(roff);
+        roff += (int64_t)D * D;
+        doff += D;
+        goff += sd.ngroups;
+        loff += (int)rd_up(sd.long_bytes, 16);
+        ix->max_dim = std::max(ix->max_dim, D);
+    }
+    ix->D_pad = doff;
+    ix->ngroups = goff;
+    ix->long_stride = loff;
+    ix->block_stride = (int64_t)ix->nseg * kBlock * 4 + (int64_t)goff * 256;
+    if (ix->D_pad != (int)rd_up(d->num_dim, 64)) {
+        delete ix;
+        return fail(SAQGPU_EINVAL, "segment dims must sum to the padded dimension");
+    }
+    int64_t blk = 0, vec = 0;
+    for (int c = 0; c < d->num_cen; ++c) {
+        const int64_t n = d->cluster_sizes ? d->cluster_sizes[c] : 0;
+        ix->clu_blk.push_back(blk);
+        ix->clu_vec.push_back(vec);
+        ix->clu_n.push_back((int32_t)n);
+        blk += (n + kBlock - 1) / kBlock;
+        vec += n;
+    }
+    ix->clu_set.assign(d->num_cen, 0);
+    if (vec != d->num_data) {
+        delete ix;
+        return fail(SAQGPU_EINVAL, "cluster sizes do not sum to num_data");
+    }
+    ix->total_blocks = blk;
+    int rc = 0;
+    rc |= ix->blocks.alloc(std::max<int64_t>(1, blk) * ix->block_stride);
+    rc |= ix->long_codes.alloc(std::max<int64_t>(1, vec) * std::max<int64_t>(16, ix->long_stride));
+    rc |= ix->exfac.alloc(std::max<int64_t>(1, vec) * ix->nseg * 2);
+    rc |= ix->ipcent.alloc(std::max<int64_t>(1, blk) * ix->nseg * kBlock);
+    rc |= ix->ids.alloc(std::max<int64_t>(1, vec));
+    rc |= ix->crot.alloc((size_t)d->num_cen * ix->D_pad);
+    rc |= ix->centroids.alloc((size_t)d->num_cen * d->num_dim);
+    rc |= ix->rot.alloc(std::max<int64_t>(1, roff));
+    rc |= ix->variance.alloc(ix->D_pad);
+    rc |= ix->d_clu_blk.alloc(d->num_cen);
+    rc |= ix->d_clu_vec.alloc(d->num_cen);
+    rc |= ix->d_clu_n.alloc(d->num_cen);
+    rc |= ix->d_rot_off.alloc(ix->nseg);
+    rc |= ix->d_segs.alloc(ix->nseg);
+    if (rc) {
+        saqgpu_index_destroy(ix);
+        return fail(SAQGPU_ENOMEM, "device allocation failed");
+    }
+    hipMemset(ix->blocks.p, 0, ix->blocks.n);
+    hipMemset(ix->long_codes.p, 0, ix->long_codes.n);
+    hipMemset(ix->exfac.p, 0, ix->exfac.n * 4);
+    hipMemset(ix->ipcent.p, 0, ix->ipcent.n * 4);
+    hipMemset(ix->ids.p, 0, ix->ids.n * 4);
+    hipMemset(ix->crot.p, 0, ix->crot.n * 4);
+    std::vector<float> var(ix->D_pad, 0.0f);
+    if (d->variance) std::copy(d->variance, d->variance + ix->D_pad, var.begin());
+    hipMemcpy(ix->variance.p, var.data(), var.size() * 4, hipMemcpyHostToDevice);
+    if (d->centroids)
+        hipMemcpy(ix->centroids.p, d->centroids, (size_t)d->num_cen * d->num_dim * 4, hipMemcpyHostToDevice);
+    if (d->rotations) hipMemcpy(ix->rot.p, d->rotations, roff * 4, hipMemcpyHostToDevice);
+    hipMemcpy(ix->d_clu_blk.p, ix->clu_blk.data(), d->num_cen * 8, hipMemcpyHostToDevice);
+    hipMemcpy(ix->d_clu_vec.p, ix->clu_vec.data(), d->num_cen * 8, hipMemcpyHostToDevice);
+    hipMemcpy(ix->d_clu_n.p, ix->clu_n.data(), d->num_cen * 4, hipMemcpyHostToDevice);
+    hipMemcpy(ix->d_rot_off.p, ix->rot_off.data(), ix->nseg * 8, hipMemcpyHostToDevice);
+    if (hipMemcpy(ix->d_segs.p, ix->segs.data(), ix->nseg * sizeof(SegDesc), hipMemcpyHostToDevice) != hipSuccess) {
+        saqgpu_index_destroy(ix);
+        return fail(SAQGPU_EDEVICE, "upload failed");
+    }
+    *out = ix;
+    return SAQGPU_OK;
+}
+
+void saqgpu_index_destroy(saqgpu_index *ix) {
+    if (!ix) return;
+    hipSetDevice(ix->device);
+    ix->blocks.release();
+    ix->long_codes.release();
+    ix->exfac.release();
+    ix->ipcent.release();
+    ix->crot.release();
+    ix->centroids.release();
+    ix->rot.release();
+    ix->variance.release();
+    ix->ids.release();
+    ix->d_clu_blk.release();
+    ix->d_clu_vec.release();
+    ix->d_rot_off.release();
+    ix->d_clu_n.release();
+    ix->d_segs.release();
+    ix->ws_q.release();
+    ix->ws_qrot.release();
+    ix->ws_bound.release();
+    ix->ws_dists.release();
+    ix->ws_cdist.release();
+    ix->ws_probes.release();
+    ix->ws_ids.release();
+    ix->ws_met.release();
+    delete ix;
+}
+
+int64_t saqgpu_index_device_bytes(const saqgpu_index *ix) {
+    if (!ix) return 0;
+    return (int64_t)(ix->blocks.n + ix->long_codes.n + 4 * (ix->exfac.n + ix->ipcent.n + ix->ids.n + ix->crot.n +
+                                                            ix->centroids.n + ix->rot.n + ix->variance.n));
+}
+
+int saqgpu_cluster_sizes_of(const saqgpu_index *ix, int32_t cid, saqgpu_cluster_sizes *out) {
+    if (!ix || !out || cid < 0 || cid >= ix->num_cen) return fail(SAQGPU_EINVAL, "bad cluster id");
+    const int64_t n = ix->clu_n[cid];
+    auto L = ix->ref_layout(n);
+    out->num_blocks = L.nb;
+    out->short_factor_floats = L.single ? 0 : L.fac_per_block * L.nb;
+    out->short_code_bytes = L.code_per_block * L.nb;
+    out->long_code_bytes = L.long_total;
+    out->long_factor_floats = n * ix->nseg * 2;
+    out->centroid_floats = ix->D_pad;
+    return SAQGPU_OK;
+}
+
+int saqgpu_index_set_cluster(saqgpu_index *ix, int32_t cid, const float *short_factors, const uint8_t *short_code,
+                             const uint8_t *long_code, const float *long_factors, const uint32_t *ids,
+                             const float *centroid_rot) {
+    if (!ix || cid < 0 || cid >= ix->num_cen) return fail(SAQGPU_EINVAL, "bad cluster id");
+    HIPCHK(hipSetDevice(ix->device));
+    const int64_t n = ix->clu_n[cid];
+    const auto L = ix->ref_layout(n);
+    const int S = ix->nseg;
+    if (n > 0 && (!short_code || !long_factors || !ids || (!L.single && !short_factors)))
+        return fail(SAQGPU_EINVAL, "missing cluster arrays");
+    // block records
+    std::vector<uint8_t> rec(L.nb * ix->block_stride, 0);
+    std::vector<float> ipc(L.nb * S * kBlock, 0.0f);
+    for (int64_t b = 0; b < L.nb; ++b) {
+        uint8_t *r = &rec[b * ix->block_stride];
+        float *fac = reinterpret_cast<float *>(r);
+        for (int s = 0; s < S; ++s) {
+            const float *src = L.single ? reinterpret_cast<const float *>(short_code + b * L.code_per_block)
+                                        : short_factors + b * L.fac_per_block + (int64_t)s * 2 * kBlock;
+            std::memcpy(fac + s * kBlock, src, kBlock * 4);
+            std::memcpy(&ipc[(b * S + s) * kBlock], src + kBlock, kBlock * 4);
+        }
+        uint8_t *codes = r + (int64_t)S * kBlock * 4;
+        for (int s = 0; s < S; ++s) {
+            const SegDesc &sd = ix->segs[s];
+            if (!sd.bits) continue;
+            const uint8_t *blk = short_code + b * L.code_per_block + L.code_off[s];
+            const int bytes = sd.dim / 8;
+            for (int v = 0; v < kBlock; ++v) {
+                // nibble of codebook m for vector v (pack_codes, fastscan.hpp:75-112)
+                int jpos = 0;
+                while (kPermLo[jpos] != (v & 15)) ++jpos;
+                for (int i = 0; i < bytes; ++i) {
+                    const uint8_t b0 = blk[16 * (2 * i) + jpos], b1 = blk[16 * (2 * i + 1) + jpos];
+                    const int n0 = v < 16 ? (b0 & 15) : (b0 >> 4);
+                    const int n1 = v < 16 ? (b1 & 15) : (b1 >> 4);
+                    const int g = sd.group_off + i / 8;
+                    codes[(int64_t)g * 256 + v * 8 + (i % 8)] = (uint8_t)((n0 << 4) | n1);
+                }
+            }
+        }
+    }
+    // long codes into per-vector records
+    std::vector<uint8_t> lc(std::max<int64_t>(1, n) * ix->long_stride, 0);
+    for (int64_t v = 0; v < n; ++v)
+        for (int s = 0; s < S; ++s) {
+            const int64_t lb = L.long_bytes[s];
+            if (!lb) continue;
+            const uint8_t *src = L.compact ? long_code + L.long_off[s] + v * lb
+                                           : long_code + L.long_off[s] + v * L.long_stride_ref;
+            std::memcpy(&lc[v * ix->long_stride + ix->segs[s].long_off], src, lb);
+        }
+    const int64_t b0 = ix->clu_blk[cid], v0 = ix->clu_vec[cid];
+    if (L.nb) {
+        HIPCHK(hipMemcpy(ix->blocks.p + b0 * ix->block_stride, rec.data(), rec.size(), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(ix->ipcent.p + b0 * S * kBlock, ipc.data(), ipc.size() * 4, hipMemcpyHostToDevice));
+    }
+    if (n) {
+        HIPCHK(hipMemcpy(ix->long_codes.p + v0 * ix->long_stride, lc.data(), n * ix->long_stride,
+                         hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(ix->exfac.p + v0 * S * 2, long_factors, n * S * 2 * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(ix->ids.p + v0, ids, n * 4, hipMemcpyHostToDevice));
+    }
+    if (centroid_rot)
+        HIPCHK(hipMemcpy(ix->crot.p + (int64_t)cid * ix->D_pad, centroid_rot, ix->D_pad * 4, hipMemcpyHostToDevice));
+    ix->clu_set[cid] = 1;
+    return SAQGPU_OK;
+}
+
+int saqgpu_index_get_cluster(const saqgpu_index *ix, int32_t cid, float *short_factors, uint8_t *short_code,
+                             uint8_t *long_code, float *long_factors, uint32_t *ids, float *centroid_rot) {
+    if (!ix || cid < 0 || cid >= ix->num_cen) return fail(SAQGPU_EINVAL, "bad cluster id");
+    HIPCHK(hipSetDevice(ix->device));
+    const int64_t n = ix->clu_n[cid];
+    const auto L = ix->ref_layout(n);
+    const int S = ix->nseg;
+    const int64_t b0 = ix->clu_blk[cid], v0 = ix->clu_vec[cid];
+    std::vector<uint8_t> rec(std::max<int64_t>(1, L.nb) * ix->block_stride);
+    std::vector<float> ipc(std::max<int64_t>(1, L.nb) * S * kBlock);
+    std::vector<uint8_t> lc(std::max<int64_t>(1, n) * ix->long_stride);
+    if (L.nb) {
+        HIPCHK(hipMemcpy(rec.data(), ix->blocks.p + b0 * ix->block_stride, L.nb * ix->block_stride,
+                         hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(ipc.data(), ix->ipcent.p + b0 * S * kBlock, L.nb * S * kBlock * 4, hipMemcpyDeviceToHost));
+    }
+    if (n) {
+        HIPCHK(hipMemcpy(lc.data(), ix->long_codes.p + v0 * ix->long_stride, n * ix->long_stride,
+                         hipMemcpyDeviceToHost));
+        if (long_factors)
+            HIPCHK(hipMemcpy(long_factors, ix->exfac.p + v0 * S * 2, n * S * 2 * 4, hipMemcpyDeviceToHost));
+        if (ids) HIPCHK(hipMemcpy(ids, ix->ids.p + v0, n * 4, hipMemcpyDeviceToHost));
+    }
+    if (centroid_rot)
+        HIPCHK(hipMemcpy(centroid_rot, ix->crot.p + (int64_t)cid * ix->D_pad, ix->D_pad * 4, hipMemcpyDeviceToHost));
+    if (short_code) {
+        if (!L.single && !short_factors) return fail(SAQGPU_EINVAL, "short_factors required");
+        for (int64_t b = 0; b < L.nb; ++b) {
+            const uint8_t *r = &rec[b * ix->block_stride];
+            const float *fac = reinterpret_cast<const float *>(r);
+            for (int s = 0; s < S; ++s) {
+                float *dst = L.single ? reinterpret_cast<float *>(short_code + b * L.code_per_block)
+                                      : short_factors + b * L.fac_per_block + (int64_t)s * 2 * kBlock;
+                std::memcpy(dst, fac + s * kBlock, kBlock * 4);
+                std::memcpy(dst + kBlock, &ipc[(b * S + s) * kBlock], kBlock * 4);
+            }
+            const uint8_t *codes = r + (int64_t)S * kBlock * 4;
+            for (int s = 0; s < S; ++s) {
+                const SegDesc &sd = ix->segs[s];
+                if (!sd.bits) continue;
+                uint8_t *blk = short_code + b * L.code_per_block + L.code_off[s];
+                const int bytes = sd.dim / 8;
+                for (int i = 0; i < bytes; ++i) {
+                    const int g = sd.group_off + i / 8;
+                    for (int j = 0; j < 16; ++j) {
+                        const int v = kPermLo[j];
+                        const uint8_t lo = codes[(int64_t)g * 256 + v * 8 + (i % 8)];
+                        const uint8_t hi = codes[(int64_t)g * 256 + (v + 16) * 8 + (i % 8)];
+                        blk[16 * (2 * i) + j] = (uint8_t)((lo >> 4) | ((hi >> 4) << 4));
+                        blk[16 * (2 * i + 1) + j] = (uint8_t)((lo & 15) | ((hi & 15) << 4));
+                    }
+                }
+            }
+        }
+    }
+    if (long_code) {
+        std::memset(long_code, 0, L.long_total);
+        for (int64_t v = 0; v < n; ++v)
+            for (int s = 0; s < S; ++s) {
+                const int64_t lb = L.long_bytes[s];
+                if (!lb) continue;
+                uint8_t *dst = L.compact ? long_code + L.long_off[s] + v * lb
+                                         : long_code + L.long_off[s] + v * L.long_stride_ref;
+                std::memcpy(dst, &lc[v * ix->long_stride + ix->segs[s].long_off], lb);
+            }
+    }
+    return SAQGPU_OK;
+}
+
+int saqgpu_probe(saqgpu_index *ix, const float *queries, int64_t nq, int32_t nprobe, int32_t dist_type,
+                 uint32_t *probes, int32_t flags, void *stream) {
+    if (!ix || !queries || !probes || nq < 0) return fail(SAQGPU_EINVAL, "null argument");
+    if (nprobe < 1 || nprobe > ix->num_cen) return fail(SAQGPU_EINVAL, "nprobe must be in 1..K");
+    if (dist_type != 0 && dist_type != 1) return fail(SAQGPU_EINVAL, "dist_type must be 0 (L2) or 1 (IP)");
+    if (nq == 0) return SAQGPU_OK;
+    std::lock_guard<std::mutex> lk(ix->mu);
+    HIPCHK(hipSetDevice(ix->device));
+    hipStream_t st = (hipStream_t)stream;
+    const bool dev = flags & SAQGPU_DEVICE_PTRS;
+    const float *dq = queries;
+    uint32_t *dp = probes;
+    if (!dev) {
+        if (ix->ws_q.ensure(nq * ix->num_dim) || ix->ws_probes.ensure(nq * nprobe))
+            return fail(SAQGPU_ENOMEM, "workspace allocation failed");
+        HIPCHK(hipMemcpyAsync(ix->ws_q.p, queries, nq * ix->num_dim * 4, hipMemcpyHostToDevice, st));
+        dq = ix->ws_q.p;
+        dp = ix->ws_probes.p;
+    }
+    if (ix->ws_cdist.ensure((size_t)nq * ix->num_cen)) return fail(SAQGPU_ENOMEM, "workspace allocation failed");
+    HIPCHK(launch_probe(ix->centroids.p, ix->num_cen, ix->num_dim, dq, nq, nprobe, dist_type, ix->ws_cdist.p, dp, st));
+    if (!dev) {
+        HIPCHK(hipMemcpyAsync(probes, dp, nq * nprobe * 4, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+    }
+    return SAQGPU_OK;
+}
+
+int saqgpu_search(saqgpu_index *ix, const float *queries, int64_t nq, const uint32_t *probes,
+                  const saqgpu_search_params *p, uint32_t *out_ids, float *out_dists, saqgpu_metrics *metrics,
+                  int32_t flags, void *stream) {
+    if (!ix || !queries || !p || !out_ids || nq < 0) return fail(SAQGPU_EINVAL, "null argument");
+    if (p->topk < 1 || p->topk > SAQGPU_MAX_TOPK) return fail(SAQGPU_EINVAL, "topk must be in 1..1024");
+    if (p->nprobe < 1 || p->nprobe > ix->num_cen) return fail(SAQGPU_EINVAL, "nprobe must be in 1..K");
+    if (p->dist_type != 0 && p->dist_type != 1) return fail(SAQGPU_EINVAL, "dist_type must be 0 (L2) or 1 (IP)");
+    for (int c = 0; c < ix->num_cen; ++c)
+        if (!ix->clu_set[c] && ix->clu_n[c] > 0) return fail(SAQGPU_ESTATE, "index has clusters not yet set");
+    if (nq == 0) return SAQGPU_OK;
+    const bool dev = flags & SAQGPU_DEVICE_PTRS;
+    if (probes && !dev)
+        for (int64_t i = 0; i < nq * p->nprobe; ++i)
+            if (probes[i] >= (uint32_t)ix->num_cen) return fail(SAQGPU_EINVAL, "probe id out of range");
+    std::lock_guard<std::mutex> lk(ix->mu);
+    HIPCHK(hipSetDevice(ix->device));
+    hipStream_t st = (hipStream_t)stream;
+    const IndexView v = ix->view();
+    size_t smem = scan_smem_bytes(v, p->topk);
+    int max_smem = 0;
+    HIPCHK(hipDeviceGetAttribute(&max_smem, hipDeviceAttributeMaxSharedMemoryPerBlock, ix->device));
+    if ((int64_t)smem > max_smem) return fail(SAQGPU_EUNSUPPORTED, "plan needs more LDS than a workgroup has");
+    int rc = 0;
+    rc |= ix->ws_qrot.ensure(nq * ix->D_pad);
+    rc |= ix->ws_bound.ensure(nq * ix->nseg);
+    if (!dev) {
+        rc |= ix->ws_q.ensure(nq * ix->num_dim);
+        rc |= ix->ws_ids.ensure(nq * p->topk);
+        rc |= ix->ws_dists.ensure(nq * p->topk);
+        rc |= ix->ws_met.ensure(nq * 3);
+    }
+    if (!probes || !dev) rc |= ix->ws_probes.ensure(nq * p->nprobe);
+    if (rc) return fail(SAQGPU_ENOMEM, "workspace allocation failed");
+    const float *dq = queries;
+    const uint32_t *dp = probes;
+    uint32_t *dids = out_ids;
+    float *dd = out_dists;
+    uint64_t *dm = reinterpret_cast<uint64_t *>(metrics);
+    if (!dev) {
+        HIPCHK(hipMemcpyAsync(ix->ws_q.p, queries, nq * ix->num_dim * 4, hipMemcpyHostToDevice, st));
+        dq = ix->ws_q.p;
+        if (probes) {
+            HIPCHK(hipMemcpyAsync(ix->ws_probes.p, probes, nq * p->nprobe * 4, hipMemcpyHostToDevice, st));
+            dp = ix->ws_probes.p;
+        }
+        dids = ix->ws_ids.p;
+        dd = out_dists ? ix->ws_dists.p : nullptr;
+        dm = metrics ? ix->ws_met.p : nullptr;
+    }
+    if (!probes) {
+        if (ix->ws_cdist.ensure((size_t)nq * ix->num_cen)) return fail(SAQGPU_ENOMEM, "workspace allocation failed");
+        HIPCHK(launch_probe(ix->centroids.p, ix->num_cen, ix->num_dim, dq, nq, p->nprobe, p->dist_type,
+                            ix->ws_cdist.p, ix->ws_probes.p, st));
+        dp = ix->ws_probes.p;
+    }
+    HIPCHK(launch_query_prep(v, dq, nq, ix->variance.p, ix->has_rot ? ix->rot.p : nullptr, ix->d_rot_off.p,
+                             p->searcher_vars_bound_m, ix->ws_qrot.p, ix->ws_bound.p, ix->max_dim, st));
+    HIPCHK(launch_scan(v, ix->ws_qrot.p, ix->ws_bound.p, dp, p->nprobe, p->topk, nq, p->dist_type, dids, dd, dm,
+                       smem, st));
+    if (!dev) {
+        HIPCHK(hipMemcpyAsync(out_ids, dids, nq * p->topk * 4, hipMemcpyDeviceToHost, st));
+        if (out_dists) HIPCHK(hipMemcpyAsync(out_dists, dd, nq * p->topk * 4, hipMemcpyDeviceToHost, st));
+        if (metrics) HIPCHK(hipMemcpyAsync(metrics, dm, nq * 3 * 8, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+    }
+    return SAQGPU_OK;
+}
+
+int saqgpu_estimate(saqgpu_index *, const float *, const uint32_t *, int32_t, float, int32_t, uint32_t *, float *,
+                    float *, float *, int64_t *) {
+    return fail(SAQGPU_EUNSUPPORTED, "estimate is not implemented yet");
+}
+
+int saqgpu_encode(saqgpu_index *, const float *, const uint32_t *, int64_t, const saqgpu_caq_cfg *, int32_t,
+                  int64_t *, void *) {
+    return fail(SAQGPU_EUNSUPPORTED, "encode is not implemented yet");
+}
+
+int saqgpu_encode_residuals(saqgpu_index *, int32_t, int32_t, const float *, const float *, const saqgpu_caq_cfg *) {
+    return fail(SAQGPU_EUNSUPPORTED, "encode is not implemented yet");
+}
+
+}  // extern "C"
