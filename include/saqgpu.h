@@ -145,6 +145,12 @@ int saqgpu_estimate(saqgpu_index *idx, const float *query, const uint32_t *probe
 int saqgpu_make_plan(const float *variance, int32_t num_dim, float avg_bits, int32_t enable_segmentation,
                      int32_t seg_eqseg, int32_t *dims, int32_t *bits, int32_t *num_seg);
 
+/* Kernel timing for measurement: enable = 1 starts recording a HIP event pair around every scan
+ * kernel launch (on the stream it is launched on), enable = 0 stops.  When scan_ms/launches are
+ * non-NULL the recorded pairs are synchronised and their total returned.  No counterpart in the
+ * reference (its test_qps times whole queries with wall clocks). */
+int saqgpu_timing(saqgpu_index *idx, int32_t enable, double *scan_ms, int64_t *launches);
+
 /* Device memory held by the index (bytes), for capacity planning. */
 int64_t saqgpu_index_device_bytes(const saqgpu_index *idx);
 

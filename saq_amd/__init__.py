@@ -65,6 +65,9 @@ _SIGS = {
     "saqgpu_estimate": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_float, C.c_int32,
                                   C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "saqgpu_index_device_bytes": (C.c_int64, [C.c_void_p]),
+    "saqgpu_timing": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
+    "saqgpu_make_plan": (C.c_int, [C.c_void_p, C.c_int32, C.c_float, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
+                                   C.c_void_p]),
 }
 
 
@@ -213,6 +216,14 @@ class Index:
         _check(lib().saqgpu_search(self.h, _ptr(queries), nq, _ptr(probes), C.byref(p), _ptr(out_ids),
                                    _ptr(out_dists), _ptr(metrics), DEVICE_PTRS if dev else 0, stream))
         return out_ids, out_dists, metrics
+
+    def timing(self, enable):
+        """Starts (enable=True) or stops event timing of the scan kernel; returns (ms, launches)
+        recorded since the last call."""
+        ms = C.c_double(0)
+        n = C.c_int64(0)
+        _check(lib().saqgpu_timing(self.h, int(enable), C.byref(ms), C.byref(n)))
+        return ms.value, n.value
 
     def device_bytes(self):
         return int(lib().saqgpu_index_device_bytes(self.h))

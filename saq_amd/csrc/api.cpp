@@ -26,11 +26,13 @@ hipError_t launch_scan(const IndexView &ix, const float *qrot, const float *boun
 size_t scan_smem_bytes(const IndexView &ix, int topk);
 hipError_t launch_probe(const float *centroids, int K, int D, const float *queries, int64_t nq, int nprobe,
                         int dist_type, float *dist_ws, uint32_t *probes, hipStream_t st);
-hipError_t launch_encode(const IndexView &ix, const float *data, int64_t n, const uint32_t *order,
-                         const int64_t *dst_vec, const uint32_t *vec_cluster, const float *centroids_unrot,
-                         const float *rot, const int64_t *rot_off, int rd_lmt, float eps, float *ws_o,
-                         uint16_t *ws_code, int *ws_fail, int64_t chunk, int seg, hipStream_t st);
-hipError_t launch_encode_finalize(const IndexView &ix, const float *crot_src, hipStream_t st);
+hipError_t launch_encode_segment(const IndexView &ix, const float *data, const uint32_t *src, const int64_t *gvec,
+                                 const int64_t *gblk, const int32_t *lane, const uint32_t *clu, const float *cent,
+                                 const float *rot, int seg, int seg_dim, int64_t nslots, int64_t ld, int rd_lmt,
+                                 float eps, float *O, uint16_t *code, int *fail, hipStream_t st);
+hipError_t launch_caq_only(const IndexView &ix, const int64_t *gvec, const int64_t *gblk, const int32_t *lane,
+                           const uint32_t *clu, const float *cent, int seg, int64_t nslots, int64_t ld, int rd_lmt,
+                           float eps, float *O, uint16_t *code, int *fail, hipStream_t st);
 }  // namespace saqgpu
 
 using namespace saqgpu;
@@ -58,7 +60,7 @@ struct DevBuf {
     T *p = nullptr;
     size_t n = 0;
     int alloc(size_t cnt) {
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
         p = nullptr;
         n = cnt;
         if (cnt == 0) return 0;
@@ -70,7 +72,7 @@ struct DevBuf {
     }
     int ensure(size_t cnt) { return cnt <= n && p ? 0 : alloc(cnt); }
     void release() {
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
         p = nullptr;
         n = 0;
     }
@@ -99,6 +101,9 @@ struct saqgpu_index {
     DevBuf<float> ws_q, ws_qrot, ws_bound, ws_dists, ws_cdist;
     DevBuf<uint32_t> ws_probes, ws_ids;
     DevBuf<uint64_t> ws_met;
+    // measurement
+    bool timing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
 
     IndexView view() const {
         IndexView v{};
@@ -313,6 +318,30 @@ void saqgpu_index_destroy(saqgpu_index *ix) {
     ix->ws_ids.release();
     ix->ws_met.release();
     delete ix;
+}
+
+int saqgpu_timing(saqgpu_index *ix, int32_t enable, double *scan_ms, int64_t *launches) {
+    if (!ix) return fail(SAQGPU_EINVAL, "null index");
+    std::lock_guard<std::mutex> lk(ix->mu);
+    HIPCHK(hipSetDevice(ix->device));
+    double tot = 0;
+    for (auto &pr : ix->events) {
+        float ms = 0;
+        HIPCHK(hipEventSynchronize(pr.second));
+        HIPCHK(hipEventElapsedTime(&ms, pr.first, pr.second));
+        tot += ms;
+    }
+    if (scan_ms) *scan_ms = tot;
+    if (launches) *launches = (int64_t)ix->events.size();
+    if (enable != (int)ix->timing || enable) {
+        for (auto &pr : ix->events) {
+            (void)hipEventDestroy(pr.first);
+            (void)hipEventDestroy(pr.second);
+        }
+        ix->events.clear();
+    }
+    ix->timing = enable != 0;
+    return SAQGPU_OK;
 }
 
 int64_t saqgpu_index_device_bytes(const saqgpu_index *ix) {
@@ -557,8 +586,18 @@ int saqgpu_search(saqgpu_index *ix, const float *queries, int64_t nq, const uint
     }
     HIPCHK(launch_query_prep(v, dq, nq, ix->variance.p, ix->has_rot ? ix->rot.p : nullptr, ix->d_rot_off.p,
                              p->searcher_vars_bound_m, ix->ws_qrot.p, ix->ws_bound.p, ix->max_dim, st));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (ix->timing) {
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        HIPCHK(hipEventRecord(e0, st));
+    }
     HIPCHK(launch_scan(v, ix->ws_qrot.p, ix->ws_bound.p, dp, p->nprobe, p->topk, nq, p->dist_type, dids, dd, dm,
                        smem, st));
+    if (ix->timing) {
+        HIPCHK(hipEventRecord(e1, st));
+        ix->events.emplace_back(e0, e1);
+    }
     if (!dev) {
         HIPCHK(hipMemcpyAsync(out_ids, dids, nq * p->topk * 4, hipMemcpyDeviceToHost, st));
         if (out_dists) HIPCHK(hipMemcpyAsync(out_dists, dd, nq * p->topk * 4, hipMemcpyDeviceToHost, st));
@@ -573,13 +612,157 @@ int saqgpu_estimate(saqgpu_index *, const float *, const uint32_t *, int32_t, fl
     return fail(SAQGPU_EUNSUPPORTED, "estimate is not implemented yet");
 }
 
-int saqgpu_encode(saqgpu_index *, const float *, const uint32_t *, int64_t, const saqgpu_caq_cfg *, int32_t,
-                  int64_t *, void *) {
-    return fail(SAQGPU_EUNSUPPORTED, "encode is not implemented yet");
+}  // extern "C"
+
+namespace {
+struct Slots {
+    std::vector<uint32_t> src, clu;
+    std::vector<int64_t> gvec, gblk;
+    std::vector<int32_t> lane;
+};
+
+// IVF::construct's bucketing (ivf.hpp:149-159): members of each list in increasing id order.
+int make_slots(const saqgpu_index *ix, const uint32_t *cid, int64_t n, Slots &sl) {
+    std::vector<int64_t> cnt(ix->num_cen, 0);
+    for (int64_t i = 0; i < n; ++i) {
+        if (cid[i] >= (uint32_t)ix->num_cen) return fail(SAQGPU_EINVAL, "cluster id out of range");
+        cnt[cid[i]]++;
+    }
+    for (int c = 0; c < ix->num_cen; ++c)
+        if (cnt[c] != ix->clu_n[c]) return fail(SAQGPU_EINVAL, "cluster ids disagree with the index's cluster sizes");
+    std::vector<int64_t> rank(ix->num_cen, 0);
+    sl.src.resize(n);
+    sl.clu.resize(n);
+    sl.gvec.resize(n);
+    sl.gblk.resize(n);
+    sl.lane.resize(n);
+    for (int64_t i = 0; i < n; ++i) {
+        const uint32_t c = cid[i];
+        const int64_t r = rank[c]++;
+        const int64_t slot = ix->clu_vec[c] + r;
+        sl.src[slot] = (uint32_t)i;
+        sl.clu[slot] = c;
+        sl.gvec[slot] = slot;
+        sl.gblk[slot] = ix->clu_blk[c] + r / kBlock;
+        sl.lane[slot] = (int32_t)(r % kBlock);
+    }
+    return SAQGPU_OK;
 }
 
-int saqgpu_encode_residuals(saqgpu_index *, int32_t, int32_t, const float *, const float *, const saqgpu_caq_cfg *) {
-    return fail(SAQGPU_EUNSUPPORTED, "encode is not implemented yet");
+template <class T>
+int upload(DevBuf<T> &b, const std::vector<T> &v) {
+    if (b.ensure(std::max<size_t>(1, v.size()))) return fail(SAQGPU_ENOMEM, "allocation failed");
+    if (!v.empty() && hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+        return fail(SAQGPU_EDEVICE, "upload failed");
+    return SAQGPU_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int saqgpu_encode(saqgpu_index *ix, const float *data, const uint32_t *cluster_ids, int64_t n,
+                  const saqgpu_caq_cfg *cfg, int32_t flags, int64_t *n_check_failed, void *stream) {
+    if (!ix || !data || !cluster_ids || !cfg) return fail(SAQGPU_EINVAL, "null argument");
+    if (n != ix->num_data) return fail(SAQGPU_EINVAL, "n must equal the index's num_data");
+    if (cfg->caq_adj_rd_lmt < 0) return fail(SAQGPU_EINVAL, "caq_adj_rd_lmt must be >= 0");
+    std::lock_guard<std::mutex> lk(ix->mu);
+    HIPCHK(hipSetDevice(ix->device));
+    hipStream_t st = (hipStream_t)stream;
+    const bool dev = flags & SAQGPU_DEVICE_PTRS;
+    std::vector<uint32_t> hcid;
+    const uint32_t *cid = cluster_ids;
+    if (dev) {
+        hcid.resize(n);
+        HIPCHK(hipMemcpy(hcid.data(), cluster_ids, n * 4, hipMemcpyDeviceToHost));
+        cid = hcid.data();
+    }
+    Slots sl;
+    if (int rc = make_slots(ix, cid, n, sl)) return rc;
+    DevBuf<uint32_t> d_src, d_clu;
+    DevBuf<int64_t> d_gvec, d_gblk;
+    DevBuf<int32_t> d_lane;
+    DevBuf<float> d_data, d_O, d_bound;
+    DevBuf<uint16_t> d_code;
+    DevBuf<int> d_fail;
+    int rc = 0;
+    if ((rc = upload(d_src, sl.src)) || (rc = upload(d_clu, sl.clu)) || (rc = upload(d_gvec, sl.gvec)) ||
+        (rc = upload(d_gblk, sl.gblk)) || (rc = upload(d_lane, sl.lane)))
+        return rc;
+    const float *ddata = data;
+    if (!dev) {
+        if (d_data.ensure(std::max<int64_t>(1, n) * ix->num_dim)) return fail(SAQGPU_ENOMEM, "allocation failed");
+        HIPCHK(hipMemcpy(d_data.p, data, n * ix->num_dim * 4, hipMemcpyHostToDevice));
+        ddata = d_data.p;
+    }
+    const IndexView v = ix->view();
+    // rotated centroids c_s * P_s of every list (quantizer.hpp:46), Eigen GEMV order
+    if (d_bound.ensure((size_t)ix->num_cen * ix->nseg)) return fail(SAQGPU_ENOMEM, "allocation failed");
+    HIPCHK(launch_query_prep(v, ix->centroids.p, ix->num_cen, ix->variance.p, ix->has_rot ? ix->rot.p : nullptr,
+                             ix->d_rot_off.p, 0.0f, ix->crot.p, d_bound.p, ix->max_dim, st));
+    const int64_t chunk = std::min<int64_t>(std::max<int64_t>(1, n), 1 << 18);
+    if (d_O.ensure(chunk * ix->max_dim) || d_code.ensure(chunk * ix->max_dim) || d_fail.ensure(1))
+        return fail(SAQGPU_ENOMEM, "allocation failed");
+    HIPCHK(hipMemsetAsync(d_fail.p, 0, sizeof(int), st));
+    for (int64_t off = 0; off < n; off += chunk) {
+        const int64_t cnt = std::min(chunk, n - off);
+        for (int s = 0; s < ix->nseg; ++s)
+            HIPCHK(launch_encode_segment(v, ddata, d_src.p + off, d_gvec.p + off, d_gblk.p + off, d_lane.p + off,
+                                         d_clu.p + off, ix->centroids.p,
+                                         ix->has_rot ? ix->rot.p + ix->rot_off[s] : nullptr, s, ix->segs[s].dim, cnt,
+                                         chunk, cfg->caq_adj_rd_lmt, cfg->caq_adj_eps, d_O.p, d_code.p, d_fail.p, st));
+    }
+    int nf = 0;
+    HIPCHK(hipMemcpyAsync(&nf, d_fail.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (n_check_failed) *n_check_failed = nf;
+    std::vector<uint32_t> ids(sl.src.begin(), sl.src.end());
+    if (n) HIPCHK(hipMemcpy(ix->ids.p, ids.data(), n * 4, hipMemcpyHostToDevice));
+    std::fill(ix->clu_set.begin(), ix->clu_set.end(), 1);
+    return SAQGPU_OK;
+}
+
+int saqgpu_encode_residuals(saqgpu_index *ix, int32_t cid, int32_t seg, const float *o, const float *centroid_seg,
+                            const saqgpu_caq_cfg *cfg) {
+    if (!ix || !o || !cfg || cid < 0 || cid >= ix->num_cen || seg < 0 || seg >= ix->nseg)
+        return fail(SAQGPU_EINVAL, "bad argument");
+    std::lock_guard<std::mutex> lk(ix->mu);
+    HIPCHK(hipSetDevice(ix->device));
+    const int64_t n = ix->clu_n[cid];
+    if (n == 0) return SAQGPU_OK;
+    const int D = ix->segs[seg].dim;
+    std::vector<float> ot((size_t)D * n);
+    for (int64_t i = 0; i < n; ++i)
+        for (int j = 0; j < D; ++j) ot[(size_t)j * n + i] = o[i * D + j];
+    std::vector<float> cen(ix->num_dim, 0.0f);
+    const int doff = ix->segs[seg].doff;
+    for (int j = 0; j < D && doff + j < ix->num_dim; ++j) cen[doff + j] = centroid_seg ? centroid_seg[j] : 0.0f;
+    Slots sl;
+    sl.gvec.resize(n);
+    sl.gblk.resize(n);
+    sl.lane.resize(n);
+    sl.clu.assign(n, 0);
+    for (int64_t r = 0; r < n; ++r) {
+        sl.gvec[r] = ix->clu_vec[cid] + r;
+        sl.gblk[r] = ix->clu_blk[cid] + r / kBlock;
+        sl.lane[r] = (int32_t)(r % kBlock);
+    }
+    DevBuf<int64_t> d_gvec, d_gblk;
+    DevBuf<int32_t> d_lane;
+    DevBuf<uint32_t> d_clu;
+    DevBuf<float> d_O, d_cen;
+    DevBuf<uint16_t> d_code;
+    DevBuf<int> d_fail;
+    int rc = 0;
+    std::vector<float> otv(ot);
+    if ((rc = upload(d_gvec, sl.gvec)) || (rc = upload(d_gblk, sl.gblk)) || (rc = upload(d_lane, sl.lane)) ||
+        (rc = upload(d_clu, sl.clu)) || (rc = upload(d_O, otv)) || (rc = upload(d_cen, cen)))
+        return rc;
+    if (d_code.ensure((size_t)D * n) || d_fail.ensure(1)) return fail(SAQGPU_ENOMEM, "allocation failed");
+    HIPCHK(hipMemset(d_fail.p, 0, sizeof(int)));
+    HIPCHK(launch_caq_only(ix->view(), d_gvec.p, d_gblk.p, d_lane.p, d_clu.p, d_cen.p, seg, n, n,
+                           cfg->caq_adj_rd_lmt, cfg->caq_adj_eps, d_O.p, d_code.p, d_fail.p, 0));
+    HIPCHK(hipDeviceSynchronize());
+    return SAQGPU_OK;
 }
 
 }  // extern "C"

@@ -1,0 +1,378 @@
+// encode.hip -- the CAQ encoder on gfx950 (IVF::construct's quantisation,
+// saqlib/index/ivf.hpp:161-183 -> SAQuantizer::quantize_cluster, saq_quantizer.hpp:34-60).
+//
+//   rotate_kernel  residual o = x_s * P_s - c_rot (quantizer.hpp:45-48) for a chunk of vectors,
+//                  each row accumulated in the order of Eigen's GEMV (FMA chains over blocks of
+//                  16 input dims, summed block by block), written transposed [D_s][chunk] so
+//                  the encoder's per-vector sweeps are coalesced.
+//   caq_kernel     one thread per vector: CAQEncoder::encode (max |o|, floor codes with the
+//                  release build's hoisted reciprocal), the FP64 code-adjustment loop in
+//                  source order, encode_and_fac's factors and rescale_vmx_to1, then the
+//                  ClusterPacker output: the MSB bit of each code into the vector's lane of
+//                  its block record, the low bits into the CodeHelper<b-1> compacted long
+//                  code, ExFactor and the block factors (caq_encoder.hpp:57-247,
+//                  cluster_packer.hpp:59-139, code_helper.hpp:58-568).
+#include <cfloat>
+
+#include "common.hpp"
+
+#pragma clang fp contract(off)
+
+namespace saqgpu {
+
+struct EncArgs {
+    IndexView ix;
+    const float *data;     // rows of num_dim floats
+    const uint32_t *src;   // per slot: source row
+    const int64_t *gvec;   // per slot: destination vector index
+    const int64_t *gblk;   // per slot: destination block
+    const int32_t *lane;   // per slot: lane in that block
+    const uint32_t *clu;   // per slot: cluster
+    const float *cent;     // K x num_dim unrotated centroids
+    const float *rot;      // P_s or nullptr
+    int32_t seg;
+    int64_t nslots;
+    int64_t ld;            // leading dimension (slots) of the transposed scratch
+    int32_t rd_lmt;
+    float eps;
+    float *O;              // [D_s][ld]
+    uint16_t *code;        // [D_s][ld]
+    int *fail;
+};
+
+constexpr int kRotRows = 8;
+
+// grid (ceil(nslots / 8), ceil(D_s / 256))
+__global__ __launch_bounds__(256) void rotate_kernel(EncArgs a) {
+    __shared__ float xs[kRotRows][256];
+    const IndexView &ix = a.ix;
+    const SegDesc sd = ix.segs[a.seg];
+    const int D = sd.dim;
+    const int j = blockIdx.y * 256 + threadIdx.x;
+    const int64_t r0 = (int64_t)blockIdx.x * kRotRows;
+    const int bc = D < 128 ? D : (D * 4 < 32000 ? 16 : 4);
+    float acc[kRotRows], c[kRotRows];
+#pragma unroll
+    for (int t = 0; t < kRotRows; ++t) acc[t] = c[t] = 0.0f;
+    if (a.rot) {
+        for (int i0 = 0; i0 < D; i0 += 256) {
+            __syncthreads();
+            for (int t = 0; t < kRotRows; ++t) {
+                const int64_t r = r0 + t;
+                const int d = sd.doff + i0 + threadIdx.x;
+                float v = 0.0f;
+                if (r < a.nslots && i0 + (int)threadIdx.x < D && d < ix.num_dim)
+                    v = a.data[(int64_t)a.src[r] * ix.num_dim + d];
+                xs[t][threadIdx.x] = v;
+            }
+            __syncthreads();
+            if (j < D) {
+                const int iend = min(256, D - i0);
+                for (int ii = 0; ii < iend; ++ii) {
+                    const int i = i0 + ii;
+                    const float p = a.rot[(int64_t)i * D + j];
+#pragma unroll
+                    for (int t = 0; t < kRotRows; ++t) c[t] = __fmaf_rn(p, xs[t][ii], c[t]);
+                    if ((i + 1) % bc == 0 || i + 1 == D) {
+#pragma unroll
+                        for (int t = 0; t < kRotRows; ++t) {
+                            acc[t] = acc[t] + c[t];
+                            c[t] = 0.0f;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    if (j >= D) return;
+    for (int t = 0; t < kRotRows; ++t) {
+        const int64_t r = r0 + t;
+        if (r >= a.nslots) break;
+        const int d = sd.doff + j;
+        float x = acc[t];
+        if (!a.rot) x = d < ix.num_dim ? a.data[(int64_t)a.src[r] * ix.num_dim + d] : 0.0f;
+        a.O[(int64_t)j * a.ld + r] = x - ix.crot[(int64_t)a.clu[r] * ix.D_pad + d];
+    }
+}
+
+// Eigen redux (Redux.h:227-272) of n elements (n a multiple of 64) by one thread.
+template <class F>
+__device__ __forceinline__ float redux_thread64(int n, F elem) {
+    float p0[16], p1[16];
+#pragma unroll
+    for (int l = 0; l < 16; ++l) {
+        p0[l] = elem(l);
+        p1[l] = elem(16 + l);
+    }
+    for (int idx = 32; idx < n; idx += 32) {
+#pragma unroll
+        for (int l = 0; l < 16; ++l) {
+            p0[l] = p0[l] + elem(idx + l);
+            p1[l] = p1[l] + elem(idx + 16 + l);
+        }
+    }
+    float b[8], c[4];
+#pragma unroll
+    for (int l = 0; l < 16; ++l) p0[l] = p0[l] + p1[l];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) b[i] = p0[i] + p0[i + 8];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) c[i] = b[i] + b[i + 4];
+    return (c[0] + c[2]) + (c[1] + c[3]);
+}
+
+// One byte of the CodeHelper<k> compacted layout (code_helper.hpp:95-568) for k = 1..8.
+// low(d) yields the low k bits of the code of dim d.
+template <class F>
+__device__ __forceinline__ void compact_store(int k, int D, uint8_t *out, F low) {
+    switch (k) {
+    case 1:
+        for (int i = 0; i < D; i += 8) {
+            uint32_t v = 0;
+            for (int t = 0; t < 8; ++t) v |= (low(i + t) & 1u) << t;
+            out[i / 8] = (uint8_t)v;
+        }
+        break;
+    case 2:
+    case 3: {
+        const int stride = k == 2 ? 16 : 24;
+        for (int c = 0; c < D / 64; ++c) {
+            uint8_t *o = out + stride * c;
+            for (int b = 0; b < 16; ++b) {
+                uint32_t v = 0;
+                for (int p = 0; p < 4; ++p) v |= (low(64 * c + 16 * p + b) & 3u) << (2 * p);
+                o[b] = (uint8_t)v;
+            }
+            if (k == 3)
+                for (int kk = 0; kk < 8; ++kk) {
+                    uint32_t v = 0;
+                    for (int t = 0; t < 8; ++t) v |= ((low(64 * c + 8 * t + kk) >> 2) & 1u) << t;
+                    o[16 + kk] = (uint8_t)v;
+                }
+        }
+        break;
+    }
+    case 4:
+        for (int c = 0; c < D / 32; ++c)
+            for (int b = 0; b < 16; ++b)
+                out[16 * c + b] = (uint8_t)((low(32 * c + b) & 15u) | ((low(32 * c + 16 + b) & 15u) << 4));
+        break;
+    case 6:
+    case 7: {
+        const int stride = k == 6 ? 48 : 56;
+        for (int c = 0; c < D / 64; ++c) {
+            uint8_t *o = out + stride * c;
+            for (int b = 0; b < 16; ++b) {
+                const uint32_t v0 = low(64 * c + b), v1 = low(64 * c + 16 + b);
+                const uint32_t v2 = low(64 * c + 32 + b), v3 = low(64 * c + 48 + b);
+                o[b] = (uint8_t)((v0 & 0x3F) | ((v2 << 2) & 0xC0));
+                o[16 + b] = (uint8_t)((v1 & 0x3F) | ((v3 << 2) & 0xC0));
+                o[32 + b] = (uint8_t)((v2 & 0x0F) | ((v3 & 0x0F) << 4));
+            }
+            if (k == 7)
+                for (int kk = 0; kk < 8; ++kk) {
+                    uint32_t v = 0;
+                    for (int t = 0; t < 8; ++t) v |= ((low(64 * c + 8 * t + kk) >> 6) & 1u) << t;
+                    o[48 + kk] = (uint8_t)v;
+                }
+        }
+        break;
+    }
+    case 8:
+        for (int i = 0; i < D; ++i) out[i] = (uint8_t)low(i);
+        break;
+    default:
+        break;
+    }
+}
+
+template <class F>
+__device__ __forceinline__ void compact_code(int k, int D, uint8_t *out, F low) {
+    if (k == 0) return;
+    if (k == 5) {  // CodeHelper<4> of code >> 1, then CodeHelper<1> of code & 1
+        compact_store(4, D, out, [&](int d) { return low(d) >> 1; });
+        compact_store(1, D, out + D * 4 / 8, [&](int d) { return low(d) & 1u; });
+        return;
+    }
+    if (k > 8) {  // low bytes, then CodeHelper<k-8> of the high bits
+        for (int i = 0; i < D; ++i) out[i] = (uint8_t)(low(i) & 0xFF);
+        const int kh = k - 8;
+        if (kh == 5) {
+            compact_store(4, D, out + D, [&](int d) { return (low(d) >> 8) >> 1; });
+            compact_store(1, D, out + D + D * 4 / 8, [&](int d) { return (low(d) >> 8) & 1u; });
+        } else {
+            compact_store(kh, D, out + D, [&](int d) { return low(d) >> 8; });
+        }
+        return;
+    }
+    compact_store(k, D, out, low);
+}
+
+__global__ __launch_bounds__(256) void caq_kernel(EncArgs a) {
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= a.nslots) return;
+    const IndexView &ix = a.ix;
+    const SegDesc sd = ix.segs[a.seg];
+    const int D = sd.dim, bits = sd.bits;
+    const int64_t ld = a.ld;
+    const float *o = a.O + t;
+    uint16_t *code = a.code + t;
+    auto O = [&](int j) { return o[(int64_t)j * ld]; };
+
+    double v_mx = 0, v_mi = 0, delta = 0, ip = 0, oa_l2sqr = 0;
+    const int cm = (1 << bits) - 1;
+    if (bits > 0) {
+        float mx = O(0), mn = O(0);
+        for (int j = 1; j < D; ++j) {
+            const float v = O(j);
+            mx = fmaxf(mx, v);
+            mn = fminf(mn, v);
+        }
+        v_mx = (double)fmaxf(mx, -mn);
+        v_mi = -v_mx;
+        delta = (v_mx - v_mi) / (cm + 1);
+        const double vec_sum = (double)redux_thread64(D, O);
+        const float fvmi = (float)v_mi, finv = 1.0f / (float)delta, fcm = (float)cm;
+        double ip_o_code = 0;
+        unsigned long long code_l2sqr = 0;
+        uint32_t code_sum = 0;
+        for (int j = 0; j < D; ++j) {
+            const float oj = O(j);
+            float cf = 0.0f;
+            if (delta != 0) {
+                const float fl = floorf((oj - fvmi) * finv);
+                cf = (fcm < fl) ? fcm : fl;  // std::min(floor, code_max)
+            }
+            ip_o_code += (double)(cf * oj);
+            const int c = (int)cf;
+            code[(int64_t)j * ld] = (uint16_t)c;
+            code_l2sqr += (unsigned long long)(c * c);
+            code_sum += (uint32_t)c;
+        }
+        ip = ip_o_code * delta + (v_mi + 0.5 * delta) * vec_sum;
+        oa_l2sqr = delta * delta * (double)code_l2sqr + (delta * delta + 2 * delta * v_mi) * (double)code_sum;
+        oa_l2sqr += (0.25 * delta * delta + delta * v_mi + v_mi * v_mi) * (double)D;
+        if (a.rd_lmt && oa_l2sqr != 0) {  // code_adjustment, caq_encoder.hpp:57-142
+            const double re_eps = (double)a.eps * oa_l2sqr;
+            for (int adj_cnt = 1, round = 1; adj_cnt; round++) {
+                if (!(ip >= 0)) {
+                    atomicAdd(a.fail, 1);  // the reference would abort on CHECK_GE(ip_resi_oa, 0)
+                    break;
+                }
+                adj_cnt = 0;
+                for (int j = 0; j < D; ++j) {
+                    int cur = 0;
+                    const float oj = O(j);
+                    const int c0 = code[(int64_t)j * ld];
+                    double oa = (c0 + 0.5) * delta + v_mi;
+                    int c = c0;
+                    const double tmp = oa_l2sqr - oa * oa;
+                    const double ip_delta = delta * oj;
+                    while (c < cm) {
+                        const double nq = oa + delta, nl = tmp + nq * nq, ni = ip + ip_delta;
+                        if ((ip * ip + re_eps) * nl >= ni * ni * oa_l2sqr) break;
+                        c++;
+                        ip = ni;
+                        oa = nq;
+                        oa_l2sqr = nl;
+                        cur++;
+                    }
+                    while (c > 0) {
+                        const double nq = oa - delta, nl = tmp + nq * nq, ni = ip - ip_delta;
+                        if ((ip * ip + re_eps) * nl >= ni * ni * oa_l2sqr) break;
+                        c--;
+                        ip = ni;
+                        oa = nq;
+                        oa_l2sqr = nl;
+                        cur++;
+                    }
+                    if (c0 != c) {
+                        code[(int64_t)j * ld] = (uint16_t)c;
+                        adj_cnt += cur;
+                    }
+                }
+                if (a.rd_lmt && round >= a.rd_lmt) break;
+                double chk_oa = 0, chk_ip = 0;
+                for (int j = 0; j < D; ++j) {
+                    const double q = (code[(int64_t)j * ld] + 0.5) * delta + v_mi;
+                    chk_ip += q * (double)O(j);
+                    chk_oa += q * q;
+                }
+                oa_l2sqr = chk_oa;
+                ip = chk_ip;
+            }
+        }
+    }
+    // encode_and_fac (caq_encoder.hpp:222-247)
+    const double o_l2sqr = (double)redux_thread64(D, [&](int j) {
+        const float v = O(j);
+        return v * v;
+    });
+    const double o_l2norm = sqrt(o_l2sqr);
+    double fac_rescale = ip != 0 ? o_l2sqr / ip : 0;
+    const double fac_error = o_l2sqr * (double)1.9f * sqrt((((o_l2sqr * oa_l2sqr) / (ip * ip)) - 1) / (double)(D - 1));
+    if (v_mx != 0) {  // rescale_vmx_to1
+        const double sr = 1.0 / v_mx;
+        delta *= sr;
+        ip *= sr;
+        oa_l2sqr *= sr * sr;
+        fac_rescale *= v_mx;
+        v_mi *= sr;
+        v_mx *= sr;
+    }
+    const int64_t gv = a.gvec[t], gb = a.gblk[t];
+    const int ln = a.lane[t];
+    uint8_t *rec = const_cast<uint8_t *>(ix.blocks) + gb * ix.block_stride;
+    reinterpret_cast<float *>(rec)[a.seg * kBlock + ln] = (float)o_l2norm;
+    if (bits == 0) return;
+    // ip_cent_oa = centroid . get_oa() with the unrotated segment centroid (caq_encoder.hpp:42-45, 241)
+    const float fd = (float)delta, fm = (float)v_mi;
+    const float *cen = a.cent + (int64_t)a.clu[t] * ix.num_dim;
+    const float ipc = redux_thread64(D, [&](int j) {
+        const int d = sd.doff + j;
+        const float cj = d < ix.num_dim ? cen[d] : 0.0f;
+        return cj * ((float)code[(int64_t)j * ld] * fd + fm);
+    });
+    const_cast<float *>(ix.ipcent)[(gb * ix.nseg + a.seg) * kBlock + ln] = ipc;
+    float *ef = const_cast<float *>(ix.exfac) + (gv * ix.nseg + a.seg) * 2;
+    ef[0] = (float)fac_rescale;
+    ef[1] = (float)fac_error;
+    // short code: MSB of each code, MSB-first bytes (cluster_packer.hpp:124-139)
+    const uint32_t short_bit = 1u << (bits - 1);
+    uint8_t *codes = rec + (int64_t)ix.nseg * kBlock * 4;
+    for (int g = 0; g < sd.ngroups; ++g) {
+        uint32_t w[2] = {0, 0};
+        for (int b = 0; b < 8; ++b) {
+            uint32_t byte = 0;
+            for (int tt = 0; tt < 8; ++tt)
+                byte |= ((code[(int64_t)(64 * g + 8 * b + tt) * ld] & short_bit) ? 0x80u : 0u) >> tt;
+            w[b / 4] |= byte << (8 * (b % 4));
+        }
+        uint2 *dst = reinterpret_cast<uint2 *>(codes + (int64_t)(sd.group_off + g) * 256 + ln * 8);
+        *dst = make_uint2(w[0], w[1]);
+    }
+    // long code: low bits in the CodeHelper<bits-1> layout
+    uint8_t *lc = const_cast<uint8_t *>(ix.long_codes) + gv * ix.long_stride + sd.long_off;
+    compact_code(bits - 1, D, lc, [&](int d) { return (uint32_t)code[(int64_t)d * ld] & (short_bit - 1u); });
+}
+
+hipError_t launch_encode_segment(const IndexView &ix, const float *data, const uint32_t *src, const int64_t *gvec,
+                                 const int64_t *gblk, const int32_t *lane, const uint32_t *clu, const float *cent,
+                                 const float *rot, int seg, int seg_dim, int64_t nslots, int64_t ld, int rd_lmt,
+                                 float eps, float *O, uint16_t *code, int *fail, hipStream_t st) {
+    EncArgs a{ix, data, src, gvec, gblk, lane, clu, cent, rot, seg, nslots, ld, rd_lmt, eps, O, code, fail};
+    dim3 g1((unsigned)((nslots + kRotRows - 1) / kRotRows), (unsigned)((seg_dim + 255) / 256));
+    hipLaunchKernelGGL(rotate_kernel, g1, dim3(256), 0, st, a);
+    hipLaunchKernelGGL(caq_kernel, dim3((unsigned)((nslots + 255) / 256)), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_caq_only(const IndexView &ix, const int64_t *gvec, const int64_t *gblk, const int32_t *lane,
+                           const uint32_t *clu, const float *cent, int seg, int64_t nslots, int64_t ld, int rd_lmt,
+                           float eps, float *O, uint16_t *code, int *fail, hipStream_t st) {
+    EncArgs a{ix, nullptr, nullptr, gvec, gblk, lane, clu, cent, nullptr, seg, nslots, ld, rd_lmt, eps, O, code, fail};
+    hipLaunchKernelGGL(caq_kernel, dim3((unsigned)((nslots + 255) / 256)), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+}  // namespace saqgpu
