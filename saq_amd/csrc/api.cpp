@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -22,8 +23,8 @@ hipError_t launch_query_prep(const IndexView &ix, const float *queries, int64_t 
                              int max_dim, hipStream_t st);
 hipError_t launch_scan(const IndexView &ix, const float *qrot, const float *bound, const uint32_t *probes,
                        int nprobe, int topk, int64_t nq, int dist_type, uint32_t *out_ids, float *out_dists,
-                       uint64_t *out_metrics, size_t smem, hipStream_t st);
-size_t scan_smem_bytes(const IndexView &ix, int topk);
+                       uint64_t *out_metrics, size_t smem, hipStream_t st, unsigned long long *prof, int cmax);
+size_t scan_smem_bytes(const IndexView &ix, int topk, int cmax);
 hipError_t launch_probe(const float *centroids, int K, int D, const float *queries, int64_t nq, int nprobe,
                         int dist_type, float *dist_ws, uint32_t *probes, hipStream_t st);
 hipError_t launch_encode_segment(const IndexView &ix, const float *data, const uint32_t *src, const int64_t *gvec,
@@ -547,7 +548,11 @@ int saqgpu_search(saqgpu_index *ix, const float *queries, int64_t nq, const uint
     HIPCHK(hipSetDevice(ix->device));
     hipStream_t st = (hipStream_t)stream;
     const IndexView v = ix->view();
-    size_t smem = scan_smem_bytes(v, p->topk);
+    // candidates per refine round: their long-code records are staged in LDS
+    // up to 64 per round while a workgroup's LDS stays within 80 KB (two workgroups per CU)
+    int cmax = 64;
+    while (cmax > 8 && scan_smem_bytes(v, p->topk, cmax) > 80 * 1024) cmax /= 2;
+    size_t smem = scan_smem_bytes(v, p->topk, cmax);
     int max_smem = 0;
     HIPCHK(hipDeviceGetAttribute(&max_smem, hipDeviceAttributeMaxSharedMemoryPerBlock, ix->device));
     if ((int64_t)smem > max_smem) return fail(SAQGPU_EUNSUPPORTED, "plan needs more LDS than a workgroup has");
@@ -592,8 +597,24 @@ int saqgpu_search(saqgpu_index *ix, const float *queries, int64_t nq, const uint
         HIPCHK(hipEventCreate(&e1));
         HIPCHK(hipEventRecord(e0, st));
     }
+    // SAQGPU_PROFILE=1: per-phase shader clocks of the scan kernel, printed to stderr (measurement only)
+    static const bool prof_on = getenv("SAQGPU_PROFILE") != nullptr;
+    DevBuf<unsigned long long> prof;
+    if (prof_on && prof.ensure(nq * 8)) return fail(SAQGPU_ENOMEM, "allocation failed");
     HIPCHK(launch_scan(v, ix->ws_qrot.p, ix->ws_bound.p, dp, p->nprobe, p->topk, nq, p->dist_type, dids, dd, dm,
-                       smem, st));
+                       smem, st, prof_on ? prof.p : nullptr, cmax));
+    if (prof_on) {
+        std::vector<unsigned long long> h(nq * 8);
+        HIPCHK(hipStreamSynchronize(st));
+        HIPCHK(hipMemcpy(h.data(), prof.p, nq * 8 * 8, hipMemcpyDeviceToHost));
+        double tot[8] = {0};
+        for (int64_t i = 0; i < nq; ++i)
+            for (int k = 0; k < 8; ++k) tot[k] += (double)h[i * 8 + k];
+        fprintf(stderr, "[saqgpu profile] per-query wall-clock ticks (100 MHz):");
+        for (int k = 0; k < 7; ++k) fprintf(stderr, " ph%d=%.0f", k, tot[k] / nq);
+        fprintf(stderr, "\n");
+        prof.release();
+    }
     if (ix->timing) {
         HIPCHK(hipEventRecord(e1, st));
         ix->events.emplace_back(e0, e1);

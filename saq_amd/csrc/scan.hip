@@ -13,7 +13,11 @@
 //                        2. speculative refine: every vector that can still be refined under
 //                           the current k-th distance gets its extra-bit inner product
 //                           (CodeHelper<b-1>::compute_ip) from 16 lanes, in the reference's
-//                           16-lane order and reduction tree
+//                           16-lane order and reduction tree, and its stage-3 running sum
+//                      The block records of the next step (or of the next probed list), the next
+//                      list's rotated centroid and the candidates' long codes move from HBM to LDS
+//                      by LDS DMA (global_load_lds_dwordx4), issued ahead of use so that the
+//                      transfers overlap the replay of the current step.
 //                        3. exact-order replay by one wavefront: the block/segment pruning of
 //                           scanCluster / searchCluster (saq_searcher.hpp:68-214) and
 //                           ResultPool insertion (utils/pool.hpp:12-61) in reference order.
@@ -30,7 +34,6 @@ namespace saqgpu {
 
 constexpr int kThreads = 256;
 constexpr int kStepBlocks = 8;            // 256 vectors per step
-constexpr int kCandMax = 64;              // candidates refined per replay round
 constexpr int kPos[16] = {3, 3, 2, 3, 1, 3, 2, 3, 0, 3, 2, 3, 1, 3, 2, 3};  // fastscan.hpp:22-39
 
 struct PrepArgs {
@@ -127,6 +130,8 @@ struct ScanArgs {
     uint32_t *out_ids;
     float *out_dists;
     uint64_t *out_metrics;
+    unsigned long long *prof;  // optional per-query phase clocks (measurement only)
+    int32_t cmax;              // candidates per refine round (bounded by LDS for their records)
 };
 
 // Per-segment scalars kept in LDS for the current (query, cluster).
@@ -134,11 +139,20 @@ enum {
     SC_DELTA = 0, SC_SUMVL, SC_SUMQCB, SC_QL2, SC_SUMQ, SC_VFAC, SC_IPQC, SC_VL, SC_INV, SC_N = 12
 };
 
+// control block indices
+enum { C_PSIZE = 0, C_POS, C_MASK, C_CURSOR, C_N = 8 };
+
 struct Smem {
-    float *r;
+    float *r, *qr, *crbuf;
     uint16_t *lut;
-    float *cbmin, *cbmax, *sc, *xs, *ipq, *curr, *binfo, *accd;
+    float *cbmin, *cbmax, *sc, *xs, *ipq, *curr, *binfo, *accd, *exr, *exv;
     int *cand;
+    uint32_t *cid;
+    int *wcnt;
+    int8_t *gseg, *candidx, *bseq;
+    int *sbits;
+    float *sresc, *ssqd;
+    uint8_t *lbuf, *blkbuf;
     float *pd;
     uint32_t *pid;
     int *ctl;
@@ -146,115 +160,140 @@ struct Smem {
     SegDesc *segs;
 };
 
-// control block indices
-enum { C_PSIZE = 0, C_NCAND, C_POS, C_LASTV, C_MORE, C_MASK, C_CURSOR, C_DISTK, C_N = 16 };
-
-__device__ __forceinline__ void carve(unsigned char *base, const IndexView &ix, int topk, Smem &m) {
+__device__ __host__ inline size_t carve_layout(unsigned char *base, const IndexView &ix, int topk, int cmax,
+                                                Smem *mp) {
     const int S = ix.nseg;
-    float *p = reinterpret_cast<float *>(base);
-    m.r = p;
-    p += ix.D_pad;
-    m.lut = reinterpret_cast<uint16_t *>(p);
-    p += ix.ngroups * 16 * 16 / 2;
-    m.cbmin = p;
-    p += ix.ngroups * 16;
-    m.cbmax = p;
-    p += ix.ngroups * 16;
-    m.sc = p;
-    p += S * SC_N;
-    m.xs = p;
-    p += S * kThreads;
-    m.ipq = p;
-    p += S * kThreads;
-    m.curr = p;
-    p += kThreads;
-    m.binfo = p;
-    p += kStepBlocks * (S + 2);
-    m.accd = p;
-    p += kCandMax * S;
-    m.cand = reinterpret_cast<int *>(p);
-    p += kCandMax;
-    m.pd = p;
-    p += topk + 1;
-    m.pid = reinterpret_cast<uint32_t *>(p);
-    p += topk + 1;
-    m.ctl = reinterpret_cast<int *>(p);
-    p += C_N;
-    // 8-byte alignment for the counters
-    uintptr_t u = reinterpret_cast<uintptr_t>(p);
-    u = (u + 15) & ~uintptr_t(15);
-    m.met = reinterpret_cast<unsigned long long *>(u);
-    m.segs = reinterpret_cast<SegDesc *>(m.met + 4);
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        off = (off + 15) & ~size_t(15);
+        unsigned char *p = base + off;
+        off += bytes;
+        return p;
+    };
+    Smem m{};
+    m.blkbuf = take((size_t)kStepBlocks * ix.block_stride);
+    m.lbuf = take((size_t)cmax * ix.long_stride);
+    m.r = reinterpret_cast<float *>(take(ix.D_pad * 4));
+    m.qr = reinterpret_cast<float *>(take(ix.D_pad * 4));
+    m.crbuf = reinterpret_cast<float *>(take(ix.D_pad * 4));
+    m.lut = reinterpret_cast<uint16_t *>(take((size_t)ix.ngroups * 256 * 2));
+    m.cbmin = reinterpret_cast<float *>(take((size_t)ix.ngroups * 16 * 4));
+    m.cbmax = reinterpret_cast<float *>(take((size_t)ix.ngroups * 16 * 4));
+    m.sc = reinterpret_cast<float *>(take((size_t)S * SC_N * 4));
+    m.xs = reinterpret_cast<float *>(take((size_t)S * kThreads * 4));
+    m.ipq = reinterpret_cast<float *>(take((size_t)S * kThreads * 4));
+    m.curr = reinterpret_cast<float *>(take(kThreads * 4));
+    m.binfo = reinterpret_cast<float *>(take((size_t)kStepBlocks * (S + 2) * 4));
+    m.accd = reinterpret_cast<float *>(take((size_t)cmax * S * 4));
+    m.exr = reinterpret_cast<float *>(take((size_t)cmax * S * 4));
+    m.exv = reinterpret_cast<float *>(take((size_t)cmax * S * 4));
+    m.cand = reinterpret_cast<int *>(take((size_t)cmax * 4));
+    m.cid = reinterpret_cast<uint32_t *>(take((size_t)cmax * 4));
+    m.wcnt = reinterpret_cast<int *>(take(16));
+    m.gseg = reinterpret_cast<int8_t *>(take(ix.ngroups));
+    m.candidx = reinterpret_cast<int8_t *>(take(kThreads));
+    m.bseq = reinterpret_cast<int8_t *>(take(S));
+    m.sbits = reinterpret_cast<int *>(take((size_t)S * 4));
+    m.sresc = reinterpret_cast<float *>(take((size_t)S * 4));
+    m.ssqd = reinterpret_cast<float *>(take((size_t)S * 4));
+    m.pd = reinterpret_cast<float *>(take((size_t)(topk + 1) * 4));
+    m.pid = reinterpret_cast<uint32_t *>(take((size_t)(topk + 1) * 4));
+    m.ctl = reinterpret_cast<int *>(take(C_N * 4));
+    m.met = reinterpret_cast<unsigned long long *>(take(4 * 8));
+    m.segs = reinterpret_cast<SegDesc *>(take((size_t)S * sizeof(SegDesc)));
+    if (mp) *mp = m;
+    return off + 16;
 }
 
-size_t scan_smem_bytes(const IndexView &ix, int topk) {
-    const int S = ix.nseg;
-    size_t f = ix.D_pad + ix.ngroups * 16 * 16 / 2 + 2 * ix.ngroups * 16 + S * SC_N + 2 * S * kThreads + kThreads +
-               kStepBlocks * (S + 2) + kCandMax * S + kCandMax + 2 * (topk + 1) + C_N;
-    return f * 4 + 16 + 4 * 8 + S * sizeof(SegDesc) + 16;
+size_t scan_smem_bytes(const IndexView &ix, int topk, int cmax) {
+    return carve_layout(nullptr, ix, topk, cmax, nullptr);
 }
 
-// Lane accumulator of CodeHelper<k>::compute_ip (code_helper.hpp:105-591) for lane gl of a
-// 16-lane group: dims gl, gl+16, gl+32, ... in increasing order, decoded from the
-// reference's compacted layout k (1..8).
-__device__ __forceinline__ float lane_acc(int k, const float *r, const uint8_t *y, int D, int gl) {
+// LDS DMA of `bytes` (a multiple of 16) from global memory to LDS by the whole workgroup:
+// every wavefront moves 1 KB per global_load_lds_dwordx4 (lane l -> dst + 16 l).
+__device__ __forceinline__ void dma_to_lds(const uint8_t *src, uint8_t *dst, int bytes, int wave, int lane) {
+    for (int off = wave * 1024; off < bytes; off += 4 * 1024) {
+        const int i = off + lane * 16;
+        if (i < bytes) __builtin_amdgcn_global_load_lds(src + i, dst + off, 16, 0, 0);
+    }
+}
+
+// Waits for this wavefront's outstanding vector-memory operations (LDS DMA included).
+__device__ __forceinline__ void dma_wait() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
+// Code of dimension 16 i + gl in the reference's compacted layout K (1..8)
+// (code_helper.hpp:95-591); each layout keeps the dims = gl (mod 16) of lane gl together.
+template <int K>
+__device__ __forceinline__ int lane_code(const uint8_t *y, int i, int gl) {
+    if (K == 1) return (y[2 * i + (gl >> 3)] >> (gl & 7)) & 1;
+    if (K == 2) return (y[16 * (i >> 2) + gl] >> (2 * (i & 3))) & 3;
+    if (K == 3) {
+        const int c = i >> 2, p = i & 3;
+        return ((y[24 * c + gl] >> (2 * p)) & 3) | (((y[24 * c + 16 + (gl & 7)] >> (2 * p + (gl >> 3))) & 1) << 2);
+    }
+    if (K == 4) {
+        const int b = y[16 * (i >> 1) + gl];
+        return (i & 1) ? (b >> 4) : (b & 15);
+    }
+    if (K == 6 || K == 7) {
+        const int c = i >> 2, p = i & 3;
+        const int base = (K == 6 ? 48 : 56) * c;
+        int code;
+        if (p == 0) code = y[base + gl] & 0x3F;
+        else if (p == 1) code = y[base + 16 + gl] & 0x3F;
+        else if (p == 2) code = ((y[base + gl] >> 2) & 0x30) | (y[base + 32 + gl] & 0x0F);
+        else code = ((y[base + 16 + gl] >> 2) & 0x30) | ((y[base + 32 + gl] >> 4) & 0x0F);
+        if (K == 7) code |= ((y[base + 48 + (gl & 7)] >> (2 * p + (gl >> 3))) & 1) << 6;
+        return code;
+    }
+    return y[16 * i + gl];  // K == 8
+}
+
+// Lane accumulator of CodeHelper<K>::compute_ip (code_helper.hpp:105-591) for lane gl of a
+// 16-lane group: dims gl, gl+16, gl+32, ... in increasing order.  Codes and query values are
+// read eight dims ahead so the LDS reads overlap; the FMA chain keeps the reference's order
+// (a masked add for K = 1).
+template <int K>
+__device__ __forceinline__ float lane_acc_k(const float *r, const uint8_t *y, int D, int gl) {
     float acc = 0.0f;
-    switch (k) {
-    case 1:
-        for (int i = 0; i < D / 16; ++i) {
-            const int bit = (y[2 * i + (gl >> 3)] >> (gl & 7)) & 1;
-            acc = acc + (bit ? r[16 * i + gl] : 0.0f);
-        }
-        break;
-    case 2:
-    case 3: {
-        const int stride = k == 2 ? 16 : 24;
-        for (int c = 0; c < D / 64; ++c) {
-            const int low = y[stride * c + gl];
-            const int top = k == 3 ? y[stride * c + 16 + (gl & 7)] : 0;
+    const int n = D / 16;  // a multiple of 4
+    int i0 = 0;
+    for (; i0 + 8 <= n; i0 += 8) {
+        int c[8];
+        float x[8];
 #pragma unroll
-            for (int p = 0; p < 4; ++p) {
-                int code = (low >> (2 * p)) & 3;
-                if (k == 3) code |= ((top >> (2 * p + (gl >> 3))) & 1) << 2;
-                acc = __fmaf_rn(r[64 * c + 16 * p + gl], (float)code, acc);
-            }
+        for (int u = 0; u < 8; ++u) {
+            c[u] = lane_code<K>(y, i0 + u, gl);
+            x[u] = r[16 * (i0 + u) + gl];
         }
-        break;
-    }
-    case 4:
-        for (int c = 0; c < D / 32; ++c) {
-            const int b = y[16 * c + gl];
-            acc = __fmaf_rn((float)(b & 15), r[32 * c + gl], acc);
-            acc = __fmaf_rn((float)(b >> 4), r[32 * c + 16 + gl], acc);
-        }
-        break;
-    case 6:
-    case 7: {
-        const int stride = k == 6 ? 48 : 56;
-        for (int c = 0; c < D / 64; ++c) {
-            const int base = stride * c;
-            const int c1 = y[base + gl], c2 = y[base + 16 + gl], c3 = y[base + 32 + gl];
-            const int top = k == 7 ? y[base + 48 + (gl & 7)] : 0;
 #pragma unroll
-            for (int p = 0; p < 4; ++p) {
-                int code;
-                if (p == 0) code = c1 & 0x3F;
-                else if (p == 1) code = c2 & 0x3F;
-                else if (p == 2) code = ((c1 >> 2) & 0x30) | (c3 & 0x0F);
-                else code = ((c2 >> 2) & 0x30) | ((c3 >> 4) & 0x0F);
-                if (k == 7) code |= ((top >> (2 * p + (gl >> 3))) & 1) << 6;
-                acc = __fmaf_rn(r[64 * c + 16 * p + gl], (float)code, acc);
-            }
-        }
-        break;
+        for (int u = 0; u < 8; ++u) acc = K == 1 ? acc + (c[u] ? x[u] : 0.0f) : __fmaf_rn(x[u], (float)c[u], acc);
     }
-    case 8:
-        for (int i = 0; i < D / 16; ++i) acc = __fmaf_rn(r[16 * i + gl], (float)y[16 * i + gl], acc);
-        break;
-    default:
-        break;
+    for (; i0 < n; i0 += 4) {
+        int c[4];
+        float x[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            c[u] = lane_code<K>(y, i0 + u, gl);
+            x[u] = r[16 * (i0 + u) + gl];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc = K == 1 ? acc + (c[u] ? x[u] : 0.0f) : __fmaf_rn(x[u], (float)c[u], acc);
     }
     return acc;
+}
+
+__device__ __forceinline__ float lane_acc(int k, const float *r, const uint8_t *y, int D, int gl) {
+    switch (k) {
+    case 1: return lane_acc_k<1>(r, y, D, gl);
+    case 2: return lane_acc_k<2>(r, y, D, gl);
+    case 3: return lane_acc_k<3>(r, y, D, gl);
+    case 4: return lane_acc_k<4>(r, y, D, gl);
+    case 6: return lane_acc_k<6>(r, y, D, gl);
+    case 7: return lane_acc_k<7>(r, y, D, gl);
+    case 8: return lane_acc_k<8>(r, y, D, gl);
+    default: return 0.0f;
+    }
 }
 
 // CodeHelper<k>::compute_ip for k = 1..12 by a 16-lane group; valid in every lane of it.
@@ -279,42 +318,6 @@ __device__ __forceinline__ float ex_ip_group(int k, const float *r, const uint8_
     return tree16_xor(lane_acc(k, r, y, D, gl));
 }
 
-// ResultPool::insert by one wavefront (utils/pool.hpp:233-246).  u, d uniform.
-template <int DT>
-__device__ __forceinline__ void pool_insert(Smem &m, int K, uint32_t u, float d, int lane) {
-    if (DT == 1) d = -d;
-    const int size = m.ctl[C_PSIZE];
-    if (size == K && d > m.pd[size - 1]) return;
-    int lo = 0;
-    for (int base = 0; base < size; base += 64) {
-        const int i = base + lane;
-        const bool p = (i < size) && (m.pd[i] < d);
-        lo += __popcll(__ballot(p));
-    }
-    for (int hi = size; hi > lo; hi -= 64) {
-        const int idx = hi - lane;
-        float dv = 0.0f;
-        uint32_t iv = 0;
-        const bool act = idx > lo;
-        if (act) {
-            dv = m.pd[idx - 1];
-            iv = m.pid[idx - 1];
-        }
-        __builtin_amdgcn_wave_barrier();
-        if (act) {
-            m.pd[idx] = dv;
-            m.pid[idx] = iv;
-        }
-        __builtin_amdgcn_wave_barrier();
-    }
-    if (lane == 0) {
-        m.pd[lo] = d;
-        m.pid[lo] = u;
-        m.ctl[C_PSIZE] = size + (size < K);
-    }
-    __builtin_amdgcn_wave_barrier();
-}
-
 template <int DT>
 __device__ __forceinline__ float pool_distk(const Smem &m, int K) {
     const int size = m.ctl[C_PSIZE];
@@ -328,41 +331,105 @@ __global__ __launch_bounds__(kThreads) void scan_kernel(ScanArgs a) {
     const IndexView &ix = a.ix;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t q = blockIdx.x;
-    const int S = ix.nseg, K = a.topk;
-    Smem m;
-    carve(smem, ix, K, m);
-    for (int i = tid; i < S; i += kThreads) m.segs[i] = ix.segs[i];
+    const int S = ix.nseg, K = a.topk, NG = ix.ngroups, CM = a.cmax;
+    Smem mm;
+    carve_layout(smem, ix, K, CM, &mm);
+    const Smem m = mm;
+    for (int i = tid; i < S; i += kThreads) {
+        const SegDesc sd = ix.segs[i];
+        m.segs[i] = sd;
+        m.sbits[i] = sd.bits;
+        m.sresc[i] = sd.rescale_sqrtD;
+        m.ssqd[i] = sd.sqd;
+    }
     for (int i = tid; i <= K; i += kThreads) {
         m.pd[i] = 0.0f;
         m.pid[i] = 0;
     }
     if (tid < C_N) m.ctl[tid] = 0;
     if (tid < 4) m.met[tid] = 0;
-    __syncthreads();
-    const bool single = (S == 1);
     const float *qrot = a.qrot + q * ix.D_pad;
+    for (int i = tid; i < ix.D_pad; i += kThreads) m.qr[i] = qrot[i];
+    // group -> segment table (groups of one segment are contiguous), segment ends as a mask,
+    // the quantised segments in order, and the prefix sums of refine bits (one per lane)
+    uint32_t endmask = 0;
+    int nbq = 0;
+    for (int s = 0; s < S; ++s) {
+        const SegDesc sd = ix.segs[s];
+        if (sd.bits == 0) continue;
+        endmask |= 1u << (sd.group_off + sd.ngroups - 1);
+        for (int g = tid; g < sd.ngroups; g += kThreads) m.gseg[sd.group_off + g] = (int8_t)s;
+        if (tid == 0) m.bseq[nbq] = (int8_t)s;
+        ++nbq;
+    }
+    unsigned long long cumbits = 0, fastcum = 0;
+    for (int s = 0; s <= min(lane, S - 1); ++s) {
+        const SegDesc sd = ix.segs[s];
+        if (sd.bits > 0) {
+            cumbits += (unsigned long long)sd.dim * (sd.bits - 1);
+            fastcum += (unsigned long long)kBlock * sd.dim;
+        }
+    }
+    unsigned long long tph[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    unsigned long long tlast = a.prof ? wall_clock64() : 0;
+    auto mark = [&](int ph) {
+        if (a.prof && tid == 0) {
+            const unsigned long long t = wall_clock64();
+            tph[ph] += t - tlast;
+            tlast = t;
+        }
+    };
+    const bool single = (S == 1);
     const float *bound = a.bound + q * S;
-    const int ncb = ix.ngroups * 16;
+    const int ncb = NG * 16;
+    const int v = tid, bl = v >> 5, ln = v & 31;
 
-    for (int pi = 0; pi < a.nprobe; ++pi) {
+    auto next_valid = [&](int i) {
+        for (; i < a.nprobe; ++i) {
+            const uint32_t c = a.probes[q * a.nprobe + i];
+            if (c < (uint32_t)ix.num_cen && ix.clu_n[c] > 0) break;  // bad ids are rejected on the host
+        }
+        return i;
+    };
+    // LDS DMA of the block records of one step, and of a list's rotated centroid
+    auto fetch_step = [&, m](uint32_t cid, int st) {
+        const int nbn = (ix.clu_n[cid] + kBlock - 1) / kBlock;
+        const int nbs = min(kStepBlocks, nbn - st);
+        dma_to_lds(ix.blocks + (ix.clu_blk[cid] + st) * ix.block_stride, m.blkbuf, (int)(nbs * ix.block_stride),
+                   wave, lane);
+    };
+    auto fetch_crot = [&, m](uint32_t cid) {
+        dma_to_lds(reinterpret_cast<const uint8_t *>(ix.crot + (int64_t)cid * ix.D_pad),
+                   reinterpret_cast<uint8_t *>(m.crbuf), ix.D_pad * 4, wave, lane);
+    };
+
+    int pi = next_valid(0);
+    if (pi < a.nprobe) {
+        const uint32_t c0 = a.probes[q * a.nprobe + pi];
+        fetch_crot(c0);
+        fetch_step(c0, 0);
+    }
+    while (pi < a.nprobe) {
         const uint32_t cid = a.probes[q * a.nprobe + pi];
-        if (cid >= (uint32_t)ix.num_cen) continue;  // invalid ids are rejected on the host
         const int n = ix.clu_n[cid];
-        if (n == 0) continue;
         const int nb = (n + kBlock - 1) / kBlock;
-        const int64_t blk0 = ix.clu_blk[cid], vec0 = ix.clu_vec[cid];
-        const float *crot = ix.crot + (int64_t)cid * ix.D_pad;
+        const int64_t vec0 = ix.clu_vec[cid];
+        const int pn = next_valid(pi + 1);
+        dma_wait();
+        __syncthreads();
+        mark(0);
 
         // ---- CaqCluEstimator::prepare for every segment (caq_estimator.hpp:94-106)
-        for (int i = tid; i < ix.D_pad; i += kThreads) m.r[i] = DT == 1 ? qrot[i] : qrot[i] - crot[i];
+        for (int d = tid; d < ix.D_pad; d += kThreads) m.r[d] = DT == 1 ? m.qr[d] : m.qr[d] - m.crbuf[d];
         __syncthreads();
         for (int s = wave; s < S; s += 4) {
             const SegDesc sd = m.segs[s];
             const float *rs = m.r + sd.doff;
-            const float sum_q = eigen_redux_wave(sd.dim, [&](int i) { return rs[i]; });
-            const float ql2 = eigen_redux_wave(sd.dim, [&](int i) { return rs[i] * rs[i]; });
+            const float *cs = m.crbuf + sd.doff;
+            const float sum_q = eigen_redux_wave(sd.dim, [rs](int i) { return rs[i]; });
+            const float ql2 = eigen_redux_wave(sd.dim, [rs](int i) { return rs[i] * rs[i]; });
             float ipqc = 0.0f;
-            if (DT == 1) ipqc = eigen_redux_wave(sd.dim, [&](int i) { return qrot[sd.doff + i] * crot[sd.doff + i]; });
+            if (DT == 1) ipqc = eigen_redux_wave(sd.dim, [rs, cs](int i) { return rs[i] * cs[i]; });
             if (lane == 0) {
                 float *sc = m.sc + s * SC_N;
                 const float ql2n = sqrtf(ql2);
@@ -376,9 +443,8 @@ __global__ __launch_bounds__(kThreads) void scan_kernel(ScanArgs a) {
         // LUT pass 1: per-codebook min / max of the fp32 table (fastscan.hpp:236-248)
         for (int c = tid; c < ncb; c += kThreads) {
             const int g = c >> 4;
-            int s = 0;
-            while (!(g >= m.segs[s].group_off && g < m.segs[s].group_off + m.segs[s].ngroups)) ++s;
-            const float *rr = m.r + m.segs[s].doff + (g - m.segs[s].group_off) * 64 + (c & 15) * 4;
+            const SegDesc &sd = m.segs[m.gseg[g]];
+            const float *rr = m.r + sd.doff + (g - sd.group_off) * 64 + (c & 15) * 4;
             float l[16];
             l[0] = 0;
             float mn = 0.0f, mx = 0.0f;
@@ -392,6 +458,7 @@ __global__ __launch_bounds__(kThreads) void scan_kernel(ScanArgs a) {
             m.cbmax[c] = mx;
         }
         __syncthreads();
+        mark(1);
         for (int s = wave; s < S; s += 4) {
             const SegDesc sd = m.segs[s];
             if (sd.bits == 0) continue;
@@ -417,9 +484,9 @@ __global__ __launch_bounds__(kThreads) void scan_kernel(ScanArgs a) {
         // LUT pass 2: quantise to u16 (lut.hpp:461-465)
         for (int c = tid; c < ncb; c += kThreads) {
             const int g = c >> 4;
-            int s = 0;
-            while (!(g >= m.segs[s].group_off && g < m.segs[s].group_off + m.segs[s].ngroups)) ++s;
-            const float *rr = m.r + m.segs[s].doff + (g - m.segs[s].group_off) * 64 + (c & 15) * 4;
+            const int s = m.gseg[g];
+            const SegDesc &sd = m.segs[s];
+            const float *rr = m.r + sd.doff + (g - sd.group_off) * 64 + (c & 15) * 4;
             const float vl = m.sc[s * SC_N + SC_VL], inv = m.sc[s * SC_N + SC_INV];
             float l[16];
             l[0] = 0;
@@ -431,57 +498,63 @@ __global__ __launch_bounds__(kThreads) void scan_kernel(ScanArgs a) {
                 dst[j / 2] = cvt_u16((l[j] - vl) * inv) | (cvt_u16((l[j + 1] - vl) * inv) << 16);
         }
         __syncthreads();
+        mark(2);
         if (tid == 0) m.met[2] += (unsigned long long)nb * kBlock;  // total_comp_cnt
 
-        // ---- blocks, 8 at a time
+        // ---- blocks, 8 at a time; the block records are already in LDS (blkbuf)
         for (int st = 0; st < nb; st += kStepBlocks) {
             const int nbs = min(kStepBlocks, nb - st);
-            const int v = tid, bl = v >> 5, ln = v & 31;
             const bool active = bl < nbs;
-            const uint8_t *rec = ix.blocks + (blk0 + st + bl) * ix.block_stride;
+            if (st > 0) {
+                dma_wait();
+                __syncthreads();
+            }
             if (active) {
+                const uint8_t *rec = m.blkbuf + bl * ix.block_stride;
                 const float *fac = reinterpret_cast<const float *>(rec);
                 for (int s = 0; s < S; ++s) m.xs[s * kThreads + v] = fac[s * kBlock + ln];
-                const uint8_t *codes = rec + ix.fac_floats * 4;
-                for (int s = 0; s < S; ++s) {
-                    const SegDesc sd = m.segs[s];
-                    if (sd.bits == 0) continue;
-                    uint32_t acc = 0;
-                    for (int g = sd.group_off; g < sd.group_off + sd.ngroups; ++g) {
-                        const uint2 w = *reinterpret_cast<const uint2 *>(codes + (int64_t)g * 256 + ln * 8);
-                        const uint16_t *t = m.lut + g * 256;
+                const uint8_t *codes = rec + ix.fac_floats * 4 + ln * 8;
+                // exact integer LUT sums (accumulate_hacc, fastscan_highacc.hpp:66-139), one segment
+                // ending at every set bit of endmask
+                uint32_t acc = 0;
+                int bk = 0;
+#pragma unroll 2
+                for (int g = 0; g < NG; ++g) {
+                    const uint2 w = *reinterpret_cast<const uint2 *>(codes + g * 256);
+                    const uint16_t *t = m.lut + g * 256;
 #pragma unroll
-                        for (int b = 0; b < 4; ++b) {
-                            const uint32_t byte = (w.x >> (8 * b)) & 0xFF;
-                            acc += t[(2 * b) * 16 + (byte >> 4)];
-                            acc += t[(2 * b + 1) * 16 + (byte & 15)];
-                        }
-#pragma unroll
-                        for (int b = 0; b < 4; ++b) {
-                            const uint32_t byte = (w.y >> (8 * b)) & 0xFF;
-                            acc += t[(2 * b + 8) * 16 + (byte >> 4)];
-                            acc += t[(2 * b + 9) * 16 + (byte & 15)];
-                        }
+                    for (int b = 0; b < 4; ++b) {
+                        const uint32_t byte = (w.x >> (8 * b)) & 0xFF;
+                        acc += t[(2 * b) * 16 + (byte >> 4)];
+                        acc += t[(2 * b + 1) * 16 + (byte & 15)];
                     }
-                    const float *sc = m.sc + s * SC_N;
-                    float t = (float)(int32_t)acc * sc[SC_DELTA];  // lut.hpp:495-497
-                    t = t + sc[SC_SUMVL];
-                    m.ipq[s * kThreads + v] = t;
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) {
+                        const uint32_t byte = (w.y >> (8 * b)) & 0xFF;
+                        acc += t[(2 * b + 8) * 16 + (byte >> 4)];
+                        acc += t[(2 * b + 9) * 16 + (byte & 15)];
+                    }
+                    if ((endmask >> g) & 1u) {
+                        const int s = m.bseq[bk++];
+                        const float *sc = m.sc + s * SC_N;
+                        float t2 = (float)(int32_t)acc * sc[SC_DELTA];  // lut.hpp:495-497
+                        t2 = t2 + sc[SC_SUMVL];
+                        m.ipq[s * kThreads + v] = t2;
+                        acc = 0;
+                    }
                 }
             }
             // per-vector estimates (distk independent) and per-block minima
             {
                 float cur = 0.0f;
-                float m1 = FLT_MAX;
                 if (single) {
-                    const SegDesc sd = m.segs[0];
                     const float *sc = m.sc;
                     const float x = active ? m.xs[v] : 0.0f;
-                    if (sd.bits == 0) {
+                    if (m.sbits[0] == 0) {
                         cur = DT == 1 ? sc[SC_VFAC] : max0(x * x + sc[SC_VFAC]);
                     } else {
                         const float ipq = active ? m.ipq[v] : 0.0f;
-                        float f = (ipq - sc[SC_SUMQCB]) * sd.rescale_sqrtD;
+                        float f = (ipq - sc[SC_SUMQCB]) * m.sresc[0];
                         f = f * x;
                         cur = DT == 1 ? f * 0.5f + sc[SC_IPQC] : max0(x * x + (sc[SC_QL2] - f));
                     }
@@ -492,17 +565,16 @@ __global__ __launch_bounds__(kThreads) void scan_kernel(ScanArgs a) {
                         const float vd = DT == 1 ? m.sc[s * SC_N + SC_VFAC] : max0(x * x + m.sc[s * SC_N + SC_VFAC]);
                         cur = cur + vd;
                     }
-                    m1 = cur;
+                    float m1 = cur;
                     for (int o = 16; o >= 1; o >>= 1) m1 = fminf(m1, __shfl_xor(m1, o, 32));
                     float M = m1;
                     for (int s = 0; s < S; ++s) {
-                        const SegDesc sd = m.segs[s];
-                        if (sd.bits == 0) continue;
+                        if (m.sbits[s] == 0) continue;
                         const float *sc = m.sc + s * SC_N;
                         const float x = active ? m.xs[s * kThreads + v] : 0.0f;
                         const float vd = DT == 1 ? sc[SC_VFAC] : max0(x * x + sc[SC_VFAC]);
                         const float ipq = active ? m.ipq[s * kThreads + v] : 0.0f;
-                        float f = (ipq - sc[SC_SUMQCB]) * sd.rescale_sqrtD;
+                        float f = (ipq - sc[SC_SUMQCB]) * m.sresc[s];
                         f = f * x;
                         const float fd = DT == 1 ? f * 0.5f + sc[SC_IPQC] : max0(x * x + (sc[SC_QL2] - f));
                         cur = cur - vd;
@@ -527,6 +599,15 @@ __global__ __launch_bounds__(kThreads) void scan_kernel(ScanArgs a) {
                 m.ctl[C_CURSOR] = 0;  // no block in progress
             }
             __syncthreads();
+            mark(3);
+            // the step's block records are consumed: fetch the next work unit's
+            if (st + kStepBlocks < nb) {
+                fetch_step(cid, st + kStepBlocks);
+            } else if (pn < a.nprobe) {
+                const uint32_t nc = a.probes[q * a.nprobe + pn];
+                fetch_crot(nc);
+                fetch_step(nc, 0);
+            }
 
             // ---- refine / replay rounds
             for (;;) {
@@ -550,66 +631,157 @@ __global__ __launch_bounds__(kThreads) void scan_kernel(ScanArgs a) {
                         flag = blk_ok && idx < n && (m.curr[v] < distk);
                     }
                 }
-                // ordered compaction of flagged vectors (wave-major, lane order)
+                // ordered compaction of the speculative candidates (wave-major, lane order)
                 const unsigned long long bal = __ballot(flag);
-                __shared__ int wcnt[4];
-                if (lane == 0) wcnt[wave] = __popcll(bal);
+                if (lane == 0) m.wcnt[wave] = __popcll(bal);
                 __syncthreads();
                 int before = 0, total = 0;
                 for (int w = 0; w < 4; ++w) {
-                    if (w < wave) before += wcnt[w];
-                    total += wcnt[w];
+                    if (w < wave) before += m.wcnt[w];
+                    total += m.wcnt[w];
                 }
                 const int rank = before + __popcll(bal & ((1ull << lane) - 1));
-                if (flag && rank < kCandMax) m.cand[rank] = v;
-                const int ncand = min(total, kCandMax);
+                const bool take = flag && rank < CM;
+                if (take) m.cand[rank] = v;
+                if (active) m.candidx[v] = take ? (int8_t)rank : (int8_t)-1;
+                const int ncand = min(total, CM);
                 __syncthreads();
-                // extra-bit inner products, 16 lanes per (candidate, segment)
-                {
+                mark(4);
+                if (ncand) {
+                    // long-code records of the candidates by LDS DMA; ids and rescale factors
+                    const int chunks = (int)(ix.long_stride / 16);
+                    const int nchunk = ncand * chunks;
+                    for (int off = wave * 64; off < nchunk; off += kThreads) {
+                        const int i = off + lane;
+                        if (i < nchunk) {
+                            const int c = i / chunks, ch = i - c * chunks;
+                            const int64_t gvec = vec0 + (int64_t)(st * kBlock + m.cand[c]);
+                            __builtin_amdgcn_global_load_lds(ix.long_codes + gvec * ix.long_stride + ch * 16,
+                                                             m.lbuf + (size_t)off * 16, 16, 0, 0);
+                        }
+                    }
+                    uint32_t my_id = 0;
+                    if (tid < ncand) my_id = ix.ids[vec0 + (int64_t)(st * kBlock + m.cand[tid])];
+                    const int nr = ncand * S;
+                    float rs = 0.0f;
+                    if (tid < nr) {
+                        const int c = tid / S, s = tid - c * S;
+                        const int64_t gvec = vec0 + (int64_t)(st * kBlock + m.cand[c]);
+                        rs = ix.exfac[(gvec * S + s) * 2];
+                    }
+                    for (int i = tid + kThreads; i < nr; i += kThreads) {
+                        const int c = i / S, s = i - c * S;
+                        const int64_t gvec = vec0 + (int64_t)(st * kBlock + m.cand[c]);
+                        m.exr[i] = ix.exfac[(gvec * S + s) * 2];
+                    }
+                    if (tid < nr) m.exr[tid] = rs;
+                    if (tid < ncand) m.cid[tid] = my_id;
+                    dma_wait();
+                    __syncthreads();
+                    // extra-bit inner products (CodeHelper<b-1>::compute_ip), 16 lanes per item
                     const int grp = tid >> 4, gl = tid & 15;
                     for (int item = grp; item < ncand * S; item += kThreads / 16) {
                         const int c = item / S, s = item - c * S;
-                        const SegDesc sd = m.segs[s];
-                        if (sd.bits < 2) continue;
-                        const int vv = m.cand[c];
-                        const int64_t gvec = vec0 + (int64_t)(st * kBlock + vv);
-                        const uint8_t *y = ix.long_codes + gvec * ix.long_stride + sd.long_off;
-                        const float ex = ex_ip_group(sd.bits - 1, m.r + sd.doff, y, sd.dim, gl);
-                        if (gl == 0) {
+                        const int bits = m.sbits[s];
+                        if (bits < 2) continue;
+                        const SegDesc &sd = m.segs[s];
+                        const uint8_t *y = m.lbuf + c * ix.long_stride + sd.long_off;
+                        const float ex = ex_ip_group(bits - 1, m.r + sd.doff, y, sd.dim, gl);
+                        if (gl == 0) m.exv[item] = ex;
+                    }
+                    __syncthreads();
+                    // per candidate: the accurate distance of every segment (compAccurateDist,
+                    // caq_estimator.hpp:190-217) and the running sum of searchCluster's stage 3
+                    // (saq_searcher.hpp:149-156), which does not depend on the k-th distance --
+                    // only where the replay stops it does
+                    if (tid < ncand) {
+                        const int c = tid, vv = m.cand[c];
+                        float acc = m.curr[vv];
+                        for (int s = 0; s < S; ++s) {
+                            const int bits = m.sbits[s];
                             const float *sc = m.sc + s * SC_N;
-                            const float ipq = m.ipq[s * kThreads + vv];
                             const float x = m.xs[s * kThreads + vv];
-                            // Lut::getExtIP (lut.hpp:508-513) and compAccurateDist (caq_estimator.hpp:190-217)
-                            const float t = (float)(((double)ipq + (double)ex * (double)sd.sqd) +
-                                                    (-1.0 + (double)(sd.sqd / 2)) * (double)sc[SC_SUMQ]);
-                            const float resc = ix.exfac[(gvec * S + s) * 2];
-                            const float ipoq = resc * t;
-                            m.accd[c * S + s] = DT == 1 ? ipoq + sc[SC_IPQC] : x * x + sc[SC_QL2] - 2 * ipoq;
+                            float av, cd;
+                            if (bits == 0) {
+                                av = DT == 1 ? sc[SC_IPQC] : x * x + sc[SC_QL2];
+                                cd = DT == 1 ? sc[SC_VFAC] : max0(x * x + sc[SC_VFAC]);
+                            } else {
+                                const float ipq = m.ipq[s * kThreads + vv];
+                                float f = (ipq - sc[SC_SUMQCB]) * m.sresc[s];
+                                f = f * x;
+                                cd = DT == 1 ? f * 0.5f + sc[SC_IPQC] : max0(x * x + (sc[SC_QL2] - f));
+                                const float sqd = m.ssqd[s];
+                                const float ex = bits >= 2 ? m.exv[c * S + s] : 0.0f;
+                                // Lut::getExtIP (lut.hpp:508-513)
+                                const float t = (float)(((double)ipq + (double)ex * (double)sqd) +
+                                                        (-1.0 + (double)(sqd / 2)) * (double)sc[SC_SUMQ]);
+                                const float ipoq = m.exr[c * S + s] * t;
+                                av = DT == 1 ? ipoq + sc[SC_IPQC] : x * x + sc[SC_QL2] - 2 * ipoq;
+                            }
+                            if (single)
+                                acc = av;
+                            else
+                                acc += av - cd;
+                            m.accd[c * S + s] = acc;
                         }
                     }
                 }
                 __syncthreads();
-                // exact-order replay by wavefront 0
+                mark(5);
+                // exact-order replay by wavefront 0 (LDS and registers only)
                 if (wave == 0) {
-                    const int lastv = total > kCandMax ? m.cand[kCandMax - 1] : kThreads;
+                    const int lastv = total > CM ? m.cand[CM - 1] : kThreads;
+                    int size = m.ctl[C_PSIZE];
+                    float tail = size == K ? m.pd[K - 1] : FLT_MAX;  // raw (negated for IP)
+                    float dk = DT == 1 ? -tail : tail;
                     int cur_pos = pos;
                     bool prog = inprog;
                     uint32_t mask_state = inmask;
-                    float dk = distk;
-                    int ci = 0;
                     bool need_more = false;
                     unsigned long long fast_bits = 0, acc_bits = 0;
+                    auto insert = [&, m](uint32_t u, float d) {  // ResultPool::insert (pool.hpp:233-246)
+                        if (DT == 1) d = -d;
+                        if (size == K && d > tail) return;
+                        int lo = 0;
+                        for (int base = 0; base < size; base += 64) {
+                            const int i = base + lane;
+                            lo += __popcll(__ballot((i < size) && (m.pd[i] < d)));
+                        }
+                        for (int hi = size; hi > lo; hi -= 64) {
+                            const int idx = hi - lane;
+                            float dv = 0.0f;
+                            uint32_t iv = 0;
+                            const bool act = idx > lo;
+                            if (act) {
+                                dv = m.pd[idx - 1];
+                                iv = m.pid[idx - 1];
+                            }
+                            __builtin_amdgcn_wave_barrier();
+                            if (act) {
+                                m.pd[idx] = dv;
+                                m.pid[idx] = iv;
+                            }
+                            __builtin_amdgcn_wave_barrier();
+                        }
+                        if (lane == 0) {
+                            m.pd[lo] = d;
+                            m.pid[lo] = u;
+                        }
+                        __builtin_amdgcn_wave_barrier();
+                        size += (size < K);
+                        tail = size == K ? m.pd[K - 1] : FLT_MAX;
+                        dk = DT == 1 ? -tail : tail;
+                    };
                     for (int b = pblk; b < nbs && !need_more; ++b) {
                         const int j0 = (b == pblk) ? (pos & 31) : 0;
                         const bool fresh = !(b == pblk && prog);
+                        const float my_curr = lane < 32 ? m.curr[b * 32 + lane] : FLT_MAX;
                         if (single) {
                             uint32_t msk;
-                            const SegDesc sd = m.segs[0];
                             if (fresh) {
-                                fast_bits += (unsigned long long)kBlock * sd.dim * (sd.bits ? 1 : 0);
+                                fast_bits += (unsigned long long)kBlock * m.segs[0].dim * (m.sbits[0] ? 1 : 0);
                                 const int cnt = (st + b == nb - 1) ? n % kBlock : kBlock;
-                                const bool lt = (lane < 32) && (m.curr[b * 32 + (lane & 31)] < dk);
-                                msk = (uint32_t)__ballot(lt) & (uint32_t)((1ull << cnt) - 1);
+                                msk = (uint32_t)__ballot(lane < 32 && my_curr < dk) & (uint32_t)((1ull << cnt) - 1);
                             } else {
                                 msk = mask_state;
                             }
@@ -624,78 +796,46 @@ __global__ __launch_bounds__(kThreads) void scan_kernel(ScanArgs a) {
                                     break;
                                 }
                                 msk &= msk - 1;
-                                const int64_t gvec = vec0 + (int64_t)(st * kBlock + vv);
-                                float d;
-                                const float x = m.xs[vv];
-                                const float *sc = m.sc;
-                                if (sd.bits == 0) {
-                                    d = DT == 1 ? sc[SC_IPQC] : x * x + sc[SC_QL2];
-                                } else if (sd.bits == 1) {
-                                    const float t = (float)(((double)m.ipq[vv] + 0.0 * (double)sd.sqd) +
-                                                            (-1.0 + (double)(sd.sqd / 2)) * (double)sc[SC_SUMQ]);
-                                    const float ipoq = ix.exfac[gvec * 2] * t;
-                                    d = DT == 1 ? ipoq + sc[SC_IPQC] : x * x + sc[SC_QL2] - 2 * ipoq;
-                                } else {
-                                    while (ci < ncand - 1 && m.cand[ci] < vv) ++ci;
-                                    d = m.accd[ci * S];
-                                    acc_bits += (unsigned long long)sd.dim * (sd.bits - 1);
-                                }
-                                pool_insert<DT>(m, K, ix.ids[gvec], d, lane);
-                                dk = pool_distk<DT>(m, K);
+                                const int c = m.candidx[vv];
+                                if (m.sbits[0]) acc_bits += (unsigned long long)m.segs[0].dim * (m.sbits[0] - 1);
+                                insert(m.cid[c], m.accd[c * S]);
                             }
                             if (!need_more) prog = false;
                         } else {
                             if (fresh) {
+                                // stage 1 (skip on the variance minimum), then stage 2: the fast
+                                // estimate of each quantised segment in turn, stopping at the first
+                                // whose block minimum exceeds dk (saq_searcher.hpp:95-132)
                                 const float *bi = m.binfo + b * (S + 2);
-                                if (bi[0] > dk) continue;  // stage 1 skip
-                                for (int s = 0; s < S; ++s) {
-                                    if (m.segs[s].bits == 0) continue;
-                                    fast_bits += (unsigned long long)kBlock * m.segs[s].dim;
-                                    if (bi[2 + s] > dk) break;
-                                }
-                                if (bi[1] > dk) continue;  // stage 2 broke: no refine
+                                const float bv = lane < S + 2 ? bi[lane] : 0.0f;
+                                if (__shfl(bv, 0) > dk) continue;
+                                const bool brk = lane >= 2 && lane < S + 2 && m.sbits[lane - 2] > 0 && bv > dk;
+                                const unsigned long long bm = __ballot(brk);
+                                const int sb = bm ? __builtin_ctzll(bm) - 2 : S - 1;
+                                fast_bits += (unsigned long long)__shfl(fastcum, sb);
+                                if (__shfl(bv, 1) > dk) continue;  // stage 2 broke: no refine
                             }
-                            for (int j = j0; j < 32; ++j) {
+                            const int nvalid = min(kBlock, n - (st + b) * kBlock);
+                            uint32_t rem = (uint32_t)((1ull << nvalid) - 1) & ~((1u << j0) - 1u);
+                            for (;;) {
+                                const uint32_t hit = (uint32_t)__ballot(lane < 32 && my_curr < dk) & rem;
+                                if (!hit) break;
+                                const int j = __builtin_ctz(hit);
                                 const int vv = b * 32 + j;
-                                if (!(m.curr[vv] < dk)) continue;
-                                if ((st + b) * kBlock + j >= n) break;
                                 if (vv > lastv) {
                                     need_more = true;
                                     cur_pos = vv;
                                     prog = true;
                                     break;
                                 }
-                                while (ci < ncand - 1 && m.cand[ci] < vv) ++ci;
-                                const int64_t gvec = vec0 + (int64_t)(st * kBlock + vv);
-                                float acc = m.curr[vv];
-                                for (int s = 0; s < S; ++s) {
-                                    const SegDesc sd = m.segs[s];
-                                    const float *sc = m.sc + s * SC_N;
-                                    const float x = m.xs[s * kThreads + vv];
-                                    float av, cd;
-                                    if (sd.bits == 0) {
-                                        av = DT == 1 ? sc[SC_IPQC] : x * x + sc[SC_QL2];
-                                        cd = DT == 1 ? sc[SC_VFAC] : max0(x * x + sc[SC_VFAC]);
-                                    } else {
-                                        const float ipq = m.ipq[s * kThreads + vv];
-                                        float f = (ipq - sc[SC_SUMQCB]) * sd.rescale_sqrtD;
-                                        f = f * x;
-                                        cd = DT == 1 ? f * 0.5f + sc[SC_IPQC] : max0(x * x + (sc[SC_QL2] - f));
-                                        if (sd.bits == 1) {
-                                            const float t = (float)(((double)ipq + 0.0 * (double)sd.sqd) +
-                                                                    (-1.0 + (double)(sd.sqd / 2)) * (double)sc[SC_SUMQ]);
-                                            const float ipoq = ix.exfac[(gvec * S + s) * 2] * t;
-                                            av = DT == 1 ? ipoq + sc[SC_IPQC] : x * x + sc[SC_QL2] - 2 * ipoq;
-                                        } else {
-                                            av = m.accd[ci * S + s];
-                                            acc_bits += (unsigned long long)sd.dim * (sd.bits - 1);
-                                        }
-                                    }
-                                    acc += av - cd;
-                                    if (acc >= dk) break;
-                                }
-                                pool_insert<DT>(m, K, ix.ids[gvec], acc, lane);
-                                dk = pool_distk<DT>(m, K);
+                                rem &= ~((2u << j) - 1u);
+                                const int c = m.candidx[vv];
+                                // the first segment at which the running sum reaches dk ends the loop
+                                const float pv = lane < S ? m.accd[c * S + lane] : 0.0f;
+                                const unsigned long long ge = __ballot(lane < S && pv >= dk);
+                                const int sb = ge ? __builtin_ctzll(ge) : S - 1;
+                                acc_bits += (unsigned long long)__shfl(cumbits, sb);
+                                insert(m.cid[c], __shfl(pv, sb));
                             }
                             if (!need_more) prog = false;
                         }
@@ -703,16 +843,19 @@ __global__ __launch_bounds__(kThreads) void scan_kernel(ScanArgs a) {
                     if (lane == 0) {
                         m.met[0] += fast_bits;
                         m.met[1] += acc_bits;
+                        m.ctl[C_PSIZE] = size;
                         m.ctl[C_POS] = need_more ? cur_pos : kThreads;
                         m.ctl[C_MASK] = need_more ? (int)mask_state : 0;
                         m.ctl[C_CURSOR] = need_more && prog ? 1 : 0;
                     }
                 }
                 __syncthreads();
+                mark(6);
                 if (m.ctl[C_POS] >= kThreads) break;
                 __syncthreads();
             }
         }
+        pi = pn;
     }
     __syncthreads();
     for (int i = tid; i < K; i += kThreads) {
@@ -720,6 +863,8 @@ __global__ __launch_bounds__(kThreads) void scan_kernel(ScanArgs a) {
         if (a.out_dists) a.out_dists[q * K + i] = DT == 1 ? -m.pd[i] : m.pd[i];
     }
     if (a.out_metrics && tid < 3) a.out_metrics[q * 3 + tid] = m.met[tid];
+    if (a.prof && tid == 0)
+        for (int i = 0; i < 8; ++i) a.prof[q * 8 + i] = tph[i];
 }
 
 }  // namespace saqgpu
@@ -738,15 +883,16 @@ hipError_t launch_query_prep(const IndexView &ix, const float *queries, int64_t 
 
 hipError_t launch_scan(const IndexView &ix, const float *qrot, const float *bound, const uint32_t *probes,
                        int nprobe, int topk, int64_t nq, int dist_type, uint32_t *out_ids, float *out_dists,
-                       uint64_t *out_metrics, size_t smem, hipStream_t st) {
-    ScanArgs a{ix, qrot, bound, probes, nprobe, topk, nq, out_ids, out_dists, out_metrics};
-    if (dist_type == 1) {
-        hipFuncSetAttribute((const void *)scan_kernel<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+                       uint64_t *out_metrics, size_t smem, hipStream_t st, unsigned long long *prof, int cmax) {
+    ScanArgs a{ix, qrot, bound, probes, nprobe, topk, nq, out_ids, out_dists, out_metrics, prof, cmax};
+    if (ix.ngroups > 32 || ix.nseg > 32 || cmax > 64 || cmax < 1) return hipErrorInvalidValue;
+    const void *fn = dist_type == 1 ? (const void *)scan_kernel<1> : (const void *)scan_kernel<0>;
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+    if (e != hipSuccess) return e;
+    if (dist_type == 1)
         hipLaunchKernelGGL(scan_kernel<1>, dim3((unsigned)nq), dim3(kThreads), smem, st, a);
-    } else {
-        hipFuncSetAttribute((const void *)scan_kernel<0>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+    else
         hipLaunchKernelGGL(scan_kernel<0>, dim3((unsigned)nq), dim3(kThreads), smem, st, a);
-    }
     return hipGetLastError();
 }
 
