@@ -46,7 +46,24 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    build_host_tools(verbose)
     return LIB
+
+
+def build_host_tools(verbose=False):
+    """Host programs over the C ABI (saq_amd/host): plain g++, linked against libsaqgpu.so."""
+    host = os.path.join(HERE, "host")
+    inc = os.path.join(os.path.dirname(HERE), "include")
+    for src in ("ivf_run.cpp",):
+        out = os.path.join(host, src[:-4])
+        cmd = [os.environ.get("CXX", "g++"), "-std=c++17", "-O2", "-Wall", "-I" + inc, "-I" + host,
+               os.path.join(host, src), "-o", out + ".tmp", "-L" + HERE, "-lsaqgpu", "-Wl,-rpath,$ORIGIN/.."]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"g++ failed on {src}:\n{r.stderr}")
+        os.replace(out + ".tmp", out)
+        if verbose:
+            print("built", out)
 
 
 if __name__ == "__main__":

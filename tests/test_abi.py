@@ -63,3 +63,16 @@ def test_plan_equal_segmentation_and_disabled():
     var = np.ones(300, np.float32)
     assert make_plan(var, 300, 4.0, seg_eqseg=2) == [(192, 4), (128, 4)]
     assert make_plan(var, 300, 3.0, enable_segmentation=False) == [(320, 3)]
+
+
+def test_host_tool_links_against_the_library():
+    """saq_amd/host/ivf_run (the saqlib::IVF host API over the C ABI) is built and resolves
+    libsaqgpu.so; without arguments it prints its usage and touches no device."""
+    import os
+    import subprocess
+    tool = os.path.join(os.path.dirname(saq_amd.LIB_PATH), "host", "ivf_run")
+    assert os.path.exists(tool)
+    ldd = subprocess.run(["ldd", tool], capture_output=True, text=True).stdout
+    assert "libsaqgpu.so" in ldd and "not found" not in ldd.split("libsaqgpu.so", 1)[1].split("\n", 1)[0]
+    r = subprocess.run([tool], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "usage" in r.stderr
