@@ -145,6 +145,41 @@ int saqgpu_estimate(saqgpu_index *idx, const float *query, const uint32_t *probe
 int saqgpu_make_plan(const float *variance, int32_t num_dim, float avg_bits, int32_t enable_segmentation,
                      int32_t seg_eqseg, int32_t *dims, int32_t *bits, int32_t *num_seg);
 
+/* ---- batched scan (no single counterpart in the reference) ----------------------------
+ * Serves a batch of queries as one grouped job instead of nq independent IVF::search calls:
+ * (query, list) pairs are grouped by list and each list's codes are contracted against a tile
+ * of its queries on int8 MFMA.  Every vector of every probed list gets the accurate estimate
+ * of caq_estimator.hpp:190-217 (no fast-scan pruning), computed from the full codes with the
+ * residual query quantised per (query, list, segment) to int8 (one term) or int8x2 (two
+ * terms); the per-query top-k is exact for those estimates, ties broken by id.  IDs therefore
+ * differ from IVF::search's only where the estimates' quantisation reorders near-ties. */
+#define SAQGPU_PREC_INT8 1
+#define SAQGPU_PREC_INT8X2 2
+
+typedef struct {
+    int32_t topk;                     /* 1..256 */
+    int32_t nprobe;                   /* lists per query (the maximum when nprobe_per_query is set) */
+    const int32_t *nprobe_per_query;  /* NULL, or nq entries in 1..nprobe, in the queries' memory kind */
+    int32_t precision;                /* SAQGPU_PREC_INT8 or SAQGPU_PREC_INT8X2 */
+    int32_t query_tile;               /* queries per work tile: 16, 32 or 64; 0 = the autotuned choice */
+} saqgpu_batched_params;
+
+/* Builds the batched code layout (full codes in MFMA operand order plus per-vector factors)
+ * from the index's clusters; search_batched calls it when needed.  Re-run after clusters change. */
+int saqgpu_prepare_batched(saqgpu_index *idx);
+
+/* Batched search: probes as in saqgpu_search (NULL = GPU coarse quantiser, nprobe lists per
+ * query); out_ids nq x topk (unfilled = 0), out_dists optional. */
+int saqgpu_search_batched(saqgpu_index *idx, const float *queries, int64_t nq, const uint32_t *probes,
+                          const saqgpu_batched_params *params, uint32_t *out_ids, float *out_dists,
+                          int32_t flags, void *stream);
+
+/* The project's autotuner for the batched scan: times every query tile (16, 32, 64) on this
+ * batch and keeps the fastest as the index's choice for query_tile = 0; returns it. */
+int saqgpu_autotune_batched(saqgpu_index *idx, const float *queries, int64_t nq, const uint32_t *probes,
+                            const saqgpu_batched_params *params, int32_t flags, void *stream,
+                            int32_t *best_query_tile);
+
 /* Kernel timing for measurement: enable = 1 starts recording a HIP event pair around every scan
  * kernel launch (on the stream it is launched on), enable = 0 stops.  When scan_ms/launches are
  * non-NULL the recorded pairs are synchronised and their total returned.  No counterpart in the

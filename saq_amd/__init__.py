@@ -45,6 +45,14 @@ class SearchParams(C.Structure):
                 ("dist_type", C.c_int32)]
 
 
+class BatchedParams(C.Structure):
+    _fields_ = [("topk", C.c_int32), ("nprobe", C.c_int32), ("nprobe_per_query", C.c_void_p),
+                ("precision", C.c_int32), ("query_tile", C.c_int32)]
+
+
+PREC_INT8 = 1
+PREC_INT8X2 = 2
+
 _lib = None
 
 _SIGS = {
@@ -64,6 +72,11 @@ _SIGS = {
                                 C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
     "saqgpu_estimate": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_float, C.c_int32,
                                   C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "saqgpu_prepare_batched": (C.c_int, [C.c_void_p]),
+    "saqgpu_search_batched": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.POINTER(BatchedParams),
+                                        C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
+    "saqgpu_autotune_batched": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.POINTER(BatchedParams),
+                                          C.c_int32, C.c_void_p, C.POINTER(C.c_int32)]),
     "saqgpu_index_device_bytes": (C.c_int64, [C.c_void_p]),
     "saqgpu_timing": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
     "saqgpu_make_plan": (C.c_int, [C.c_void_p, C.c_int32, C.c_float, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
@@ -216,6 +229,51 @@ class Index:
         _check(lib().saqgpu_search(self.h, _ptr(queries), nq, _ptr(probes), C.byref(p), _ptr(out_ids),
                                    _ptr(out_dists), _ptr(metrics), DEVICE_PTRS if dev else 0, stream))
         return out_ids, out_dists, metrics
+
+    def prepare_batched(self):
+        _check(lib().saqgpu_prepare_batched(self.h))
+
+    def _bparams(self, topk, nprobe, nprobe_per_query, precision, query_tile, dev):
+        npq = None
+        if nprobe_per_query is not None and not dev:
+            npq = np.ascontiguousarray(nprobe_per_query, np.int32)
+        elif nprobe_per_query is not None:
+            npq = nprobe_per_query
+        p = BatchedParams(topk, nprobe, _ptr(npq), precision, query_tile)
+        return p, npq
+
+    def search_batched(self, queries, topk, nprobe, probes=None, nprobe_per_query=None, precision=PREC_INT8,
+                       query_tile=0, out_ids=None, out_dists=None, stream=None, want_dists=True):
+        """Batched int8-MFMA search (saqgpu_search_batched).  Returns (ids, dists)."""
+        dev = not isinstance(queries, np.ndarray)
+        nq = int(queries.shape[0])
+        p, keep = self._bparams(topk, nprobe, nprobe_per_query, precision, query_tile, dev)
+        if not dev:
+            queries = np.ascontiguousarray(queries, np.float32)
+            if probes is not None:
+                probes = np.ascontiguousarray(probes, np.uint32)
+            out_ids = np.zeros((nq, topk), np.uint32) if out_ids is None else out_ids
+            if want_dists and out_dists is None:
+                out_dists = np.zeros((nq, topk), np.float32)
+        _check(lib().saqgpu_search_batched(self.h, _ptr(queries), nq, _ptr(probes), C.byref(p), _ptr(out_ids),
+                                           _ptr(out_dists), DEVICE_PTRS if dev else 0, stream))
+        del keep
+        return out_ids, out_dists
+
+    def autotune_batched(self, queries, topk, nprobe, probes=None, nprobe_per_query=None, precision=PREC_INT8,
+                         stream=None):
+        """Times every query tile on this batch; the fastest becomes the default.  Returns it."""
+        dev = not isinstance(queries, np.ndarray)
+        if not dev:
+            queries = np.ascontiguousarray(queries, np.float32)
+            if probes is not None:
+                probes = np.ascontiguousarray(probes, np.uint32)
+        p, keep = self._bparams(topk, nprobe, nprobe_per_query, precision, 0, dev)
+        best = C.c_int32(0)
+        _check(lib().saqgpu_autotune_batched(self.h, _ptr(queries), int(queries.shape[0]), _ptr(probes), C.byref(p),
+                                             DEVICE_PTRS if dev else 0, stream, C.byref(best)))
+        del keep
+        return best.value
 
     def timing(self, enable):
         """Starts (enable=True) or stops event timing of the scan kernel; returns (ms, launches)

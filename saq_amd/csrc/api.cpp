@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "../../include/saqgpu.h"
+#include "batched.hpp"
 #include "common.hpp"
 
 namespace saqgpu {
@@ -102,6 +103,19 @@ struct saqgpu_index {
     DevBuf<float> ws_q, ws_qrot, ws_bound, ws_dists, ws_cdist;
     DevBuf<uint32_t> ws_probes, ws_ids;
     DevBuf<uint64_t> ws_met;
+    // batched layout (batched.hip): built from the clusters on first use
+    bool bq_ready = false;
+    int bq_tile = 0;  // autotuned queries per work tile (0 = not tuned)
+    int bq_nqs = 0, bq_kq = 0, bq_fac_off = 0;
+    int64_t bq_stride = 0, bq_total_mb = 0;
+    std::vector<BSeg> bsegs;
+    std::vector<int64_t> clu_mb;
+    DevBuf<uint8_t> bq_mb;
+    DevBuf<int64_t> d_clu_mb;
+    DevBuf<BSeg> d_bsegs;
+    DevBuf<int32_t> bq_ws, bq_npq;
+    DevBuf<float> bq_cand_d;
+    DevBuf<uint32_t> bq_cand_id;
     // measurement
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
@@ -318,6 +332,13 @@ void saqgpu_index_destroy(saqgpu_index *ix) {
     ix->ws_probes.release();
     ix->ws_ids.release();
     ix->ws_met.release();
+    ix->bq_mb.release();
+    ix->d_clu_mb.release();
+    ix->d_bsegs.release();
+    ix->bq_ws.release();
+    ix->bq_npq.release();
+    ix->bq_cand_d.release();
+    ix->bq_cand_id.release();
     delete ix;
 }
 
@@ -430,6 +451,7 @@ int saqgpu_index_set_cluster(saqgpu_index *ix, int32_t cid, const float *short_f
     if (centroid_rot)
         HIPCHK(hipMemcpy(ix->crot.p + (int64_t)cid * ix->D_pad, centroid_rot, ix->D_pad * 4, hipMemcpyHostToDevice));
     ix->clu_set[cid] = 1;
+    ix->bq_ready = false;
     return SAQGPU_OK;
 }
 
@@ -739,6 +761,7 @@ int saqgpu_encode(saqgpu_index *ix, const float *data, const uint32_t *cluster_i
     std::vector<uint32_t> ids(sl.src.begin(), sl.src.end());
     if (n) HIPCHK(hipMemcpy(ix->ids.p, ids.data(), n * 4, hipMemcpyHostToDevice));
     std::fill(ix->clu_set.begin(), ix->clu_set.end(), 1);
+    ix->bq_ready = false;
     return SAQGPU_OK;
 }
 
@@ -783,6 +806,294 @@ int saqgpu_encode_residuals(saqgpu_index *ix, int32_t cid, int32_t seg, const fl
     HIPCHK(launch_caq_only(ix->view(), d_gvec.p, d_gblk.p, d_lane.p, d_clu.p, d_cen.p, seg, n, n,
                            cfg->caq_adj_rd_lmt, cfg->caq_adj_eps, d_O.p, d_code.p, d_fail.p, 0));
     HIPCHK(hipDeviceSynchronize());
+    ix->bq_ready = false;
+    return SAQGPU_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ batched scan
+namespace {
+
+BatchedView batched_view(const saqgpu_index *ix) {
+    BatchedView b{};
+    b.nseg = ix->nseg;
+    b.nqs = ix->bq_nqs;
+    b.D_pad = ix->D_pad;
+    b.kq = ix->bq_kq;
+    b.fac_off = ix->bq_fac_off;
+    b.kq_stride = ix->bq_kq + 16;
+    b.mb_stride = ix->bq_stride;
+    b.mb = ix->bq_mb.p;
+    b.clu_mb = ix->d_clu_mb.p;
+    b.clu_n = ix->d_clu_n.p;
+    b.crot = ix->crot.p;
+    b.segs = ix->d_bsegs.p;
+    return b;
+}
+
+// Layout of one 16-vector record: the code chunks of every quantised segment, then
+// |o|^2[16], rescale[nqs][16], id[16].
+int prepare_batched_locked(saqgpu_index *ix) {
+    if (ix->bq_ready) return SAQGPU_OK;
+    for (int c = 0; c < ix->num_cen; ++c)
+        if (!ix->clu_set[c] && ix->clu_n[c] > 0) return fail(SAQGPU_ESTATE, "index has clusters not yet set");
+    ix->bsegs.assign(ix->nseg, BSeg{});
+    int kq = 0, nqs = 0;
+    int64_t code = 0;
+    std::vector<int32_t> chunk_seg, chunk_kc;
+    for (int s = 0; s < ix->nseg; ++s) {
+        const SegDesc &sd = ix->segs[s];
+        BSeg &b = ix->bsegs[s];
+        b.dim = sd.dim;
+        b.bits = sd.bits;
+        b.doff = sd.doff;
+        b.sqd = sd.bits ? (float)(2.0 / (double)(1 << sd.bits)) : 0.0f;
+        b.qs = -1;
+        if (!sd.bits) continue;
+        b.qs = nqs++;
+        b.kq_off = kq;
+        kq += sd.dim;
+        b.chunk_bytes = sd.bits <= 4 ? 512 : (sd.bits <= 8 ? 1024 : 2048);
+        b.code_off = (int32_t)code;
+        code += (int64_t)b.chunk_bytes * (sd.dim / 64);
+        for (int kc = 0; kc < sd.dim / 64; ++kc) {
+            chunk_seg.push_back(s);
+            chunk_kc.push_back(kc);
+        }
+    }
+    ix->bq_nqs = nqs;
+    ix->bq_kq = kq;
+    ix->bq_fac_off = (int)code;
+    ix->bq_stride = rd_up(code + (int64_t)kMB * 4 * (2 + nqs), 256);
+    ix->clu_mb.assign(ix->num_cen, 0);
+    int64_t tot = 0;
+    std::vector<int32_t> mb_clu;
+    for (int c = 0; c < ix->num_cen; ++c) {
+        ix->clu_mb[c] = tot;
+        const int64_t nmb = (ix->clu_n[c] + kMB - 1) / kMB;
+        for (int64_t i = 0; i < nmb; ++i) mb_clu.push_back(c);
+        tot += nmb;
+    }
+    ix->bq_total_mb = tot;
+    if (ix->bq_mb.ensure(std::max<int64_t>(1, tot) * ix->bq_stride)) return fail(SAQGPU_ENOMEM, "allocation failed");
+    int rc;
+    if ((rc = upload(ix->d_clu_mb, ix->clu_mb)) || (rc = upload(ix->d_bsegs, ix->bsegs))) return rc;
+    DevBuf<int32_t> d_mb_clu, d_cs, d_ck;
+    if ((rc = upload(d_mb_clu, mb_clu)) || (rc = upload(d_cs, chunk_seg)) || (rc = upload(d_ck, chunk_kc))) return rc;
+    HIPCHK(launch_bq_build(ix->view(), batched_view(ix), d_mb_clu.p, tot, d_cs.p, d_ck.p, (int)chunk_seg.size(),
+                           ix->bq_mb.p, 0));
+    HIPCHK(hipDeviceSynchronize());
+    d_mb_clu.release();
+    d_cs.release();
+    d_ck.release();
+    ix->bq_ready = true;
+    return SAQGPU_OK;
+}
+
+int check_batched_params(const saqgpu_index *ix, const saqgpu_batched_params *p, int64_t nq, int32_t flags) {
+    if (p->topk < 1 || p->topk > 256) return fail(SAQGPU_EINVAL, "batched topk must be in 1..256");
+    if (p->nprobe < 1 || p->nprobe > ix->num_cen) return fail(SAQGPU_EINVAL, "nprobe must be in 1..K");
+    if (p->precision != SAQGPU_PREC_INT8 && p->precision != SAQGPU_PREC_INT8X2)
+        return fail(SAQGPU_EINVAL, "precision must be SAQGPU_PREC_INT8 or SAQGPU_PREC_INT8X2");
+    if (p->query_tile != 0 && p->query_tile != 16 && p->query_tile != 32 && p->query_tile != 64)
+        return fail(SAQGPU_EINVAL, "query_tile must be 0, 16, 32 or 64");
+    if (p->nprobe_per_query && !(flags & SAQGPU_DEVICE_PTRS))
+        for (int64_t i = 0; i < nq; ++i)
+            if (p->nprobe_per_query[i] < 1 || p->nprobe_per_query[i] > p->nprobe)
+                return fail(SAQGPU_EINVAL, "nprobe_per_query entries must be in 1..nprobe");
+    return SAQGPU_OK;
+}
+
+// Queries per work tile when the autotuner has not run: the expected queries per probed list.
+int default_query_tile(const saqgpu_index *ix, int64_t nq, int nprobe) {
+    const double per_list = (double)nq * nprobe / std::max(1, ix->num_cen);
+    return per_list >= 48 ? 64 : (per_list >= 24 ? 32 : 16);
+}
+
+int search_batched_locked(saqgpu_index *ix, const float *queries, int64_t nq, const uint32_t *probes,
+                          const saqgpu_batched_params *p, int query_tile, uint32_t *out_ids, float *out_dists,
+                          int32_t flags, hipStream_t st, bool time_it) {
+    int rc = prepare_batched_locked(ix);
+    if (rc) return rc;
+    const bool dev = flags & SAQGPU_DEVICE_PTRS;
+    const int NP = p->nprobe, K = ix->num_cen, topk = p->topk;
+    const int qtm = query_tile / 16;
+    const int qs = p->precision == SAQGPU_PREC_INT8X2 ? 2 : 1;
+    const BatchedView bv = batched_view(ix);
+    int max_smem = 0;
+    HIPCHK(hipDeviceGetAttribute(&max_smem, hipDeviceAttributeMaxSharedMemoryPerBlock, ix->device));
+    if ((int64_t)bscan_smem_bytes(bv, topk, qtm, qs) > max_smem)
+        return fail(SAQGPU_EUNSUPPORTED, "query tile needs more LDS than a workgroup has; use a smaller query_tile");
+    const int64_t npairs = nq * NP;
+    const int64_t max_tiles = npairs + K;
+    rc = 0;
+    rc |= ix->ws_qrot.ensure(nq * ix->D_pad);
+    rc |= ix->ws_bound.ensure(nq * ix->nseg);
+    rc |= ix->bq_ws.ensure(3 * (size_t)K + 2 * npairs + 3 * max_tiles + 2);
+    rc |= ix->bq_cand_d.ensure(npairs * topk);
+    rc |= ix->bq_cand_id.ensure(npairs * topk);
+    if (!dev) {
+        rc |= ix->ws_q.ensure(nq * ix->num_dim);
+        rc |= ix->ws_ids.ensure(nq * topk);
+        rc |= ix->ws_dists.ensure(nq * topk);
+        if (p->nprobe_per_query) rc |= ix->bq_npq.ensure(nq);
+    }
+    if (!probes || !dev) rc |= ix->ws_probes.ensure(npairs);
+    if (rc) return fail(SAQGPU_ENOMEM, "workspace allocation failed");
+    const float *dq = queries;
+    const uint32_t *dp = probes;
+    const int32_t *dnpq = p->nprobe_per_query;
+    uint32_t *dids = out_ids;
+    float *dd = out_dists;
+    if (!dev) {
+        HIPCHK(hipMemcpyAsync(ix->ws_q.p, queries, nq * ix->num_dim * 4, hipMemcpyHostToDevice, st));
+        dq = ix->ws_q.p;
+        if (probes) {
+            HIPCHK(hipMemcpyAsync(ix->ws_probes.p, probes, npairs * 4, hipMemcpyHostToDevice, st));
+            dp = ix->ws_probes.p;
+        }
+        if (dnpq) {
+            HIPCHK(hipMemcpyAsync(ix->bq_npq.p, p->nprobe_per_query, nq * 4, hipMemcpyHostToDevice, st));
+            dnpq = ix->bq_npq.p;
+        }
+        dids = ix->ws_ids.p;
+        dd = out_dists ? ix->ws_dists.p : nullptr;
+    }
+    if (!probes) {
+        if (ix->ws_cdist.ensure((size_t)nq * K)) return fail(SAQGPU_ENOMEM, "workspace allocation failed");
+        HIPCHK(launch_probe(ix->centroids.p, K, ix->num_dim, dq, nq, NP, 0, ix->ws_cdist.p, ix->ws_probes.p, st));
+        dp = ix->ws_probes.p;
+    }
+    HIPCHK(launch_query_prep(ix->view(), dq, nq, ix->variance.p, ix->has_rot ? ix->rot.p : nullptr, ix->d_rot_off.p,
+                             4.0f, ix->ws_qrot.p, ix->ws_bound.p, ix->max_dim, st));
+    int32_t *wsp = ix->bq_ws.p;
+    BqWork w;
+    w.counts = wsp;
+    w.cstart = wsp + K;
+    w.tstart = wsp + 2 * K;
+    w.slot = wsp + 3 * K;
+    w.pairs = w.slot + npairs;
+    w.tile_clu = w.pairs + npairs;
+    w.tile_first = w.tile_clu + max_tiles;
+    w.tile_cnt = w.tile_first + max_tiles;
+    w.ntiles = w.tile_cnt + max_tiles;
+    w.work = w.ntiles + 1;
+    HIPCHK(launch_bq_group(dp, dnpq, nq, NP, K, query_tile, w, st));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (time_it) {
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        HIPCHK(hipEventRecord(e0, st));
+    }
+    const int grid = (int)std::min<int64_t>(max_tiles, 1 << 20);
+    HIPCHK(launch_bscan_tiles(bv, ix->ws_qrot.p, w.pairs, w, NP, topk, ix->bq_cand_d.p, ix->bq_cand_id.p, qtm, qs, grid,
+                              st));
+    if (time_it) {
+        HIPCHK(hipEventRecord(e1, st));
+        ix->events.emplace_back(e0, e1);
+    }
+    HIPCHK(launch_bq_merge(ix->bq_cand_d.p, ix->bq_cand_id.p, dnpq, nq, NP, topk, dids, dd, st));
+    if (!dev) {
+        HIPCHK(hipMemcpyAsync(out_ids, dids, nq * topk * 4, hipMemcpyDeviceToHost, st));
+        if (out_dists) HIPCHK(hipMemcpyAsync(out_dists, dd, nq * topk * 4, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+    }
+    return SAQGPU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int saqgpu_prepare_batched(saqgpu_index *ix) {
+    if (!ix) return fail(SAQGPU_EINVAL, "null index");
+    std::lock_guard<std::mutex> lk(ix->mu);
+    HIPCHK(hipSetDevice(ix->device));
+    return prepare_batched_locked(ix);
+}
+
+int saqgpu_search_batched(saqgpu_index *ix, const float *queries, int64_t nq, const uint32_t *probes,
+                          const saqgpu_batched_params *p, uint32_t *out_ids, float *out_dists, int32_t flags,
+                          void *stream) {
+    if (!ix || !queries || !p || !out_ids || nq < 0) return fail(SAQGPU_EINVAL, "null argument");
+    int rc = check_batched_params(ix, p, nq, flags);
+    if (rc) return rc;
+    if (nq == 0) return SAQGPU_OK;
+    if (probes && !(flags & SAQGPU_DEVICE_PTRS))
+        for (int64_t i = 0; i < nq * p->nprobe; ++i)
+            if (probes[i] >= (uint32_t)ix->num_cen) return fail(SAQGPU_EINVAL, "probe id out of range");
+    std::lock_guard<std::mutex> lk(ix->mu);
+    HIPCHK(hipSetDevice(ix->device));
+    const int qt = p->query_tile ? p->query_tile : (ix->bq_tile ? ix->bq_tile : default_query_tile(ix, nq, p->nprobe));
+    return search_batched_locked(ix, queries, nq, probes, p, qt, out_ids, out_dists, flags, (hipStream_t)stream,
+                                 ix->timing);
+}
+
+int saqgpu_autotune_batched(saqgpu_index *ix, const float *queries, int64_t nq, const uint32_t *probes,
+                            const saqgpu_batched_params *p, int32_t flags, void *stream, int32_t *best_query_tile) {
+    if (!ix || !queries || !p || nq < 1) return fail(SAQGPU_EINVAL, "null argument");
+    int rc = check_batched_params(ix, p, nq, flags);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(ix->mu);
+    HIPCHK(hipSetDevice(ix->device));
+    hipStream_t st = (hipStream_t)stream;
+    const bool dev = flags & SAQGPU_DEVICE_PTRS;
+    DevBuf<uint32_t> ids;
+    if (ids.ensure(nq * p->topk)) return fail(SAQGPU_ENOMEM, "allocation failed");
+    // the timed calls read device-resident copies so that only the search is measured
+    DevBuf<float> q;
+    DevBuf<uint32_t> pr;
+    DevBuf<int32_t> npq;
+    const float *dq = queries;
+    const uint32_t *dp = probes;
+    saqgpu_batched_params pp = *p;
+    if (!dev) {
+        if (q.ensure(nq * ix->num_dim)) return fail(SAQGPU_ENOMEM, "allocation failed");
+        HIPCHK(hipMemcpy(q.p, queries, nq * ix->num_dim * 4, hipMemcpyHostToDevice));
+        dq = q.p;
+        if (probes) {
+            if (pr.ensure(nq * p->nprobe)) return fail(SAQGPU_ENOMEM, "allocation failed");
+            HIPCHK(hipMemcpy(pr.p, probes, nq * p->nprobe * 4, hipMemcpyHostToDevice));
+            dp = pr.p;
+        }
+        if (p->nprobe_per_query) {
+            if (npq.ensure(nq)) return fail(SAQGPU_ENOMEM, "allocation failed");
+            HIPCHK(hipMemcpy(npq.p, p->nprobe_per_query, nq * 4, hipMemcpyHostToDevice));
+            pp.nprobe_per_query = npq.p;
+        }
+    }
+    int max_smem = 0;
+    HIPCHK(hipDeviceGetAttribute(&max_smem, hipDeviceAttributeMaxSharedMemoryPerBlock, ix->device));
+    if ((rc = prepare_batched_locked(ix))) return rc;
+    const BatchedView bv = batched_view(ix);
+    const int qs = p->precision == SAQGPU_PREC_INT8X2 ? 2 : 1;
+    float best = 1e30f;
+    int best_qt = 16;
+    for (int qt : {16, 32, 64}) {
+        if ((int64_t)bscan_smem_bytes(bv, p->topk, qt / 16, qs) > max_smem) continue;
+        hipEvent_t a, b;
+        HIPCHK(hipEventCreate(&a));
+        HIPCHK(hipEventCreate(&b));
+        // one untimed call, then two timed ones
+        if ((rc = search_batched_locked(ix, dq, nq, dp, &pp, qt, ids.p, nullptr, SAQGPU_DEVICE_PTRS, st, false))) return rc;
+        HIPCHK(hipEventRecord(a, st));
+        for (int r = 0; r < 2; ++r)
+            if ((rc = search_batched_locked(ix, dq, nq, dp, &pp, qt, ids.p, nullptr, SAQGPU_DEVICE_PTRS, st, false)))
+                return rc;
+        HIPCHK(hipEventRecord(b, st));
+        HIPCHK(hipEventSynchronize(b));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, a, b));
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        if (ms < best) {
+            best = ms;
+            best_qt = qt;
+        }
+    }
+    ix->bq_tile = best_qt;
+    if (best_query_tile) *best_query_tile = best_qt;
     return SAQGPU_OK;
 }
 

@@ -1,0 +1,772 @@
+// batched.hip -- the batched, cluster-grouped SAQ scan on int8 MFMA (gfx950).
+//
+// The plain scan (scan.hip) walks every query's probe list on its own and reproduces the
+// reference bit for bit.  This path serves many queries at once instead: the (query, list)
+// pairs of a batch are grouped by list, and each work tile -- one list and up to 16*QTM of
+// the queries that probe it -- turns the estimate of every vector of the list into a dense
+// query x code contraction on v_mfma_i32_16x16x64_i8.  Each list's codes are then read from
+// HBM once per query tile instead of once per query.
+//
+// Estimator.  For a segment with b > 0 bits the reference's accurate estimate is
+// (caq_estimator.hpp:190-217, with c = msb * 2^(b-1) + low the full b-bit code):
+//     ip_s = rescale * (sq_delta * sum_i r_i c_i + (-1 + sq_delta / 2) * sum_i r_i),
+//     dist = sum_s (|o_s|^2 + |r_s|^2) - 2 * sum_{s: b>0} ip_s,        r = q_rot - c_rot.
+// Here sum_i r_i c_i comes from the MFMA: r is quantised per (query, list, segment) to int8
+// with an fp32 scale (one term, or two terms for int8x2: r ~ s1*q1 + s2*q2), codes are int8
+// (b <= 7 as is, b = 8 shifted by -128, b >= 9 split into a high plane and a shifted low
+// plane), and the shift is corrected through the per-query sum of the quantised r.  The
+// fast-scan LUT stage is not needed: the full code gives the accurate estimate directly.
+//
+// Per-vector factors |o|^2 = sum_s o_l2norm_s^2, the ExFactor rescale of each quantised
+// segment and the ids sit beside the codes in the same 16-vector record, so one tile streams
+// one contiguous region per 16 vectors.
+//
+// Top-k.  Each tile keeps, per query, the k smallest (distance, id) pairs of the list in a
+// wave-held sorted list (ties broken by id, so the result does not depend on how pairs are
+// grouped into tiles); merge_kernel then merges each query's per-list lists.
+#include <algorithm>
+#include <cfloat>
+
+#include "batched.hpp"
+
+namespace saqgpu {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+// ------------------------------------------------------------------ layout conversion
+// Low bits of the CodeHelper<k> compacted layouts (utils/code_helper.hpp:95-591) for
+// k <= 8, one code.
+__device__ int code_at_le8(int k, const uint8_t *y, int D, int d) {
+    const int c64 = d / 64, r = d % 64, b = r % 16, part = r / 16;
+    switch (k) {
+    case 0: return 0;
+    case 1: return (y[d / 8] >> (d % 8)) & 1;
+    case 2: return (y[16 * c64 + b] >> (2 * part)) & 3;
+    case 3: return ((y[24 * c64 + b] >> (2 * part)) & 3) | (((y[24 * c64 + 16 + r % 8] >> (r / 8)) & 1) << 2);
+    case 4: {
+        const uint8_t v = y[16 * (d / 32) + (d % 32) % 16];
+        return (d % 32) < 16 ? (v & 15) : (v >> 4);
+    }
+    case 5: {  // 4-bit part, then a 1-bit part (code_helper.hpp:372-374)
+        const uint8_t v = y[16 * (d / 32) + (d % 32) % 16];
+        const int c4 = (d % 32) < 16 ? (v & 15) : (v >> 4);
+        return 2 * c4 + ((y[D * 4 / 8 + d / 8] >> (d % 8)) & 1);
+    }
+    case 6:
+    case 7: {
+        const int base = (k == 6 ? 48 : 56) * c64;
+        const uint8_t c1 = y[base + b], c2 = y[base + 16 + b], c3 = y[base + 32 + b];
+        int v;
+        if (part == 0) v = c1 & 0x3F;
+        else if (part == 1) v = c2 & 0x3F;
+        else if (part == 2) v = ((c1 >> 2) & 0x30) | (c3 & 0x0F);
+        else v = ((c2 >> 2) & 0x30) | ((c3 >> 4) & 0x0F);
+        if (k == 7) v |= ((y[base + 48 + r % 8] >> (r / 8)) & 1) << 6;
+        return v;
+    }
+    default: return y[d];  // k == 8
+    }
+}
+
+// k > 8: a plane of the low bytes, then the k-8 high bits (code_helper.hpp:85-88).
+__device__ int code_at(int k, const uint8_t *y, int D, int d) {
+    if (k > 8) return y[d] | (code_at_le8(k - 8, y + D, D, d) << 8);
+    return code_at_le8(k, y, D, d);
+}
+
+// Full b-bit code of dimension d of segment s of vector vi (within list c), from the plain
+// layout: the MSB from the block record's short code, the low bits from the long code.
+__device__ int full_code(const IndexView &ix, const SegDesc &sd, int s_group_off, int c, int vi, int d) {
+    const int64_t blk = ix.clu_blk[c] + vi / kBlock;
+    const uint8_t *rec = ix.blocks + blk * ix.block_stride + (int64_t)ix.nseg * kBlock * 4;
+    const uint8_t byte = rec[(int64_t)(s_group_off + d / 64) * 256 + (vi % kBlock) * 8 + (d / 8) % 8];
+    const int msb = (byte >> (7 - d % 8)) & 1;
+    if (sd.bits == 1) return msb;
+    const uint8_t *lc = ix.long_codes + (ix.clu_vec[c] + vi) * ix.long_stride + sd.long_off;
+    return (msb << (sd.bits - 1)) | code_at(sd.bits - 1, lc, sd.dim, d);
+}
+
+// grid (total mblocks, total 64-dim chunks of quantised segments), 64 threads: one lane's
+// bytes of one chunk of one mblock.  Lane l holds vector l % 16, dims 16 * (l / 16) .. +16 of
+// the chunk, in the order the MFMA's B operand takes them.
+__global__ __launch_bounds__(64) void bq_build_codes_kernel(IndexView ix, BatchedView bv, const int32_t *mb_clu,
+                                                            const int32_t *chunk_seg, const int32_t *chunk_kc,
+                                                            uint8_t *out) {
+    const int64_t mb = blockIdx.x;
+    const int ch = blockIdx.y;
+    const int l = threadIdx.x;
+    const int c = mb_clu[mb];
+    const int s = chunk_seg[ch], kc = chunk_kc[ch];
+    const SegDesc sd = ix.segs[s];
+    const BSeg bs = bv.segs[s];
+    const int vi = (int)(mb - bv.clu_mb[c]) * kMB + l % kMB;
+    const bool valid = vi < bv.clu_n[c];
+    int code[16];
+#pragma unroll
+    for (int t = 0; t < 16; ++t) code[t] = valid ? full_code(ix, sd, sd.group_off, c, vi, kc * 64 + 16 * (l / 16) + t) : 0;
+    uint8_t *dst = out + mb * bv.mb_stride + bs.code_off + (int64_t)kc * bs.chunk_bytes;
+    if (sd.bits <= 4) {
+        // packed byte p: low nibble = dim (p < 4 ? p : p + 4), high nibble = that dim + 4
+        uint8_t o[8];
+#pragma unroll
+        for (int p = 0; p < 8; ++p) {
+            const int t = p < 4 ? p : p + 4;
+            o[p] = (uint8_t)(code[t] | (code[t + 4] << 4));
+        }
+#pragma unroll
+        for (int p = 0; p < 8; ++p) dst[l * 8 + p] = o[p];
+    } else if (sd.bits <= 8) {
+        const int off = sd.bits == 8 ? 128 : 0;
+#pragma unroll
+        for (int t = 0; t < 16; ++t) dst[l * 16 + t] = (uint8_t)(int8_t)(code[t] - off);
+    } else {
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+            dst[l * 16 + t] = (uint8_t)(int8_t)((code[t] & 255) - 128);
+            dst[1024 + l * 16 + t] = (uint8_t)(code[t] >> 8);
+        }
+    }
+}
+
+// grid (total mblocks), 16 threads: |o|^2, the rescale of every quantised segment and the id.
+__global__ __launch_bounds__(64) void bq_build_factors_kernel(IndexView ix, BatchedView bv, const int32_t *mb_clu,
+                                                              uint8_t *out) {
+    const int64_t mb = blockIdx.x;
+    const int l = threadIdx.x;
+    if (l >= kMB) return;
+    const int c = mb_clu[mb];
+    const int vi = (int)(mb - bv.clu_mb[c]) * kMB + l;
+    const bool valid = vi < bv.clu_n[c];
+    float *f = reinterpret_cast<float *>(out + mb * bv.mb_stride + bv.fac_off);
+    uint32_t *idp = reinterpret_cast<uint32_t *>(f + kMB * (1 + bv.nqs));
+    if (!valid) {
+        f[l] = INFINITY;
+        for (int q = 0; q < bv.nqs; ++q) f[kMB * (1 + q) + l] = 0.0f;
+        idp[l] = 0xFFFFFFFFu;
+        return;
+    }
+    const int64_t blk = ix.clu_blk[c] + vi / kBlock;
+    const float *norms = reinterpret_cast<const float *>(ix.blocks + blk * ix.block_stride);
+    const int64_t gv = ix.clu_vec[c] + vi;
+    float o2 = 0.0f;
+    for (int s = 0; s < bv.nseg; ++s) {
+        const float x = norms[s * kBlock + vi % kBlock];
+        o2 = o2 + x * x;
+        const int q = bv.segs[s].qs;
+        if (q >= 0) f[kMB * (1 + q) + l] = ix.exfac[(gv * bv.nseg + s) * 2];
+    }
+    f[l] = o2;
+    idp[l] = ix.ids[gv];
+}
+
+// ------------------------------------------------------------------ grouping
+__global__ void bq_fill_kernel(int32_t *p, int64_t n, int32_t v) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// (q, p) -> slot within its list's pair group
+__global__ void bq_count_kernel(const uint32_t *probes, const int32_t *npq, int64_t nq, int NP, int32_t *counts,
+                                int32_t *slot) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= nq * NP) return;
+    const int64_t q = i / NP;
+    const int p = (int)(i % NP);
+    const int lim = npq ? npq[q] : NP;
+    if (p >= lim) {
+        slot[i] = -1;
+        return;
+    }
+    slot[i] = atomicAdd(&counts[probes[i]], 1);
+}
+
+// One workgroup: exclusive prefix sums of the pair counts and of the tile counts per list.
+__global__ __launch_bounds__(1024) void bq_scan_kernel(const int32_t *counts, int K, int QT, int32_t *cstart,
+                                                       int32_t *tstart, int32_t *ntiles) {
+    __shared__ int32_t part_c[1024], part_t[1024];
+    const int t = threadIdx.x;
+    const int per = (K + 1023) / 1024;
+    const int b = t * per, e = min(K, b + per);
+    int32_t sc = 0, st = 0;
+    for (int i = b; i < e; ++i) {
+        sc += counts[i];
+        st += (counts[i] + QT - 1) / QT;
+    }
+    part_c[t] = sc;
+    part_t[t] = st;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int32_t a = t >= off ? part_c[t - off] : 0, bt = t >= off ? part_t[t - off] : 0;
+        __syncthreads();
+        part_c[t] += a;
+        part_t[t] += bt;
+        __syncthreads();
+    }
+    int32_t rc = part_c[t] - sc, rt = part_t[t] - st;
+    for (int i = b; i < e; ++i) {
+        cstart[i] = rc;
+        tstart[i] = rt;
+        rc += counts[i];
+        rt += (counts[i] + QT - 1) / QT;
+    }
+    if (t == 1023) *ntiles = part_t[1023];
+}
+
+__global__ void bq_scatter_kernel(const uint32_t *probes, const int32_t *slot, int64_t n, const int32_t *cstart,
+                                  int32_t *pairs) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n || slot[i] < 0) return;
+    pairs[cstart[probes[i]] + slot[i]] = (int32_t)i;
+}
+
+__global__ void bq_tiles_kernel(const int32_t *counts, const int32_t *cstart, const int32_t *tstart, int K, int QT,
+                                int32_t *tile_clu, int32_t *tile_first, int32_t *tile_cnt) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= K) return;
+    const int n = counts[c];
+    for (int t = 0, i = 0; i < n; ++t, i += QT) {
+        tile_clu[tstart[c] + t] = c;
+        tile_first[tstart[c] + t] = cstart[c] + i;
+        tile_cnt[tstart[c] + t] = min(QT, n - i);
+    }
+}
+
+// ------------------------------------------------------------------ wave-held top-k
+__device__ __forceinline__ bool dless(float a, uint32_t ai, float b, uint32_t bi) {
+    return a < b || (a == b && ai < bi);
+}
+
+// Sorted list of up to 64 * KR (distance, id) pairs; element i sits in register i / 64 of
+// lane i % 64.
+template <int KR>
+struct WaveTopK {
+    float d[KR];
+    uint32_t id[KR];
+    int n, k;
+    float kd;
+    uint32_t kid;
+
+    __device__ void refresh() {
+        if (n < k) {
+            kd = INFINITY;
+            kid = 0xFFFFFFFFu;
+        } else {
+            const int r = (k - 1) >> 6, ln = (k - 1) & 63;
+            float x = d[0];
+            uint32_t xi = id[0];
+#pragma unroll
+            for (int j = 1; j < KR; ++j)
+                if (j == r) {
+                    x = d[j];
+                    xi = id[j];
+                }
+            kd = __shfl(x, ln);
+            kid = __shfl(xi, ln);
+        }
+    }
+    __device__ void load(const float *sd, const uint32_t *si, int cnt, int kk) {
+        const int lane = threadIdx.x & 63;
+        k = kk;
+        n = cnt;
+#pragma unroll
+        for (int j = 0; j < KR; ++j) {
+            const int i = j * 64 + lane;
+            d[j] = i < n ? sd[i] : INFINITY;
+            id[j] = i < n ? si[i] : 0xFFFFFFFFu;
+        }
+        refresh();
+    }
+    __device__ void store(float *sd, uint32_t *si) const {
+        const int lane = threadIdx.x & 63;
+#pragma unroll
+        for (int j = 0; j < KR; ++j) {
+            const int i = j * 64 + lane;
+            if (i < n) {
+                sd[i] = d[j];
+                si[i] = id[j];
+            }
+        }
+    }
+    // x must be less than the current k-th pair
+    __device__ void insert(float x, uint32_t xi) {
+        const int lane = threadIdx.x & 63;
+        int pos = 0;
+#pragma unroll
+        for (int j = 0; j < KR; ++j) {
+            const int i = j * 64 + lane;
+            pos += __popcll(__ballot(i < n && dless(d[j], id[j], x, xi)));
+        }
+        float nd[KR];
+        uint32_t ni[KR];
+#pragma unroll
+        for (int j = 0; j < KR; ++j) {
+            float up = __shfl_up(d[j], 1);
+            uint32_t upi = __shfl_up(id[j], 1);
+            if (j > 0) {
+                const float w = __shfl(d[j - 1], 63);
+                const uint32_t wi = __shfl(id[j - 1], 63);
+                if (lane == 0) {
+                    up = w;
+                    upi = wi;
+                }
+            }
+            const int i = j * 64 + lane;
+            nd[j] = i > pos ? up : (i == pos ? x : d[j]);
+            ni[j] = i > pos ? upi : (i == pos ? xi : id[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < KR; ++j) {
+            d[j] = nd[j];
+            id[j] = ni[j];
+        }
+        n = min(n + 1, k);
+        refresh();
+    }
+    // offers one candidate per lane
+    __device__ void offer(float x, uint32_t xi) {
+        unsigned long long m = __ballot(dless(x, xi, kd, kid));
+        while (m) {
+            const int j = __builtin_ctzll(m);
+            m &= m - 1;
+            const float bx = __shfl(x, j);
+            const uint32_t bi = __shfl(xi, j);
+            if (dless(bx, bi, kd, kid)) insert(bx, bi);
+        }
+    }
+};
+
+// ------------------------------------------------------------------ tile kernel
+struct BScanArgs {
+    BatchedView bv;
+    const float *qrot;  // nq x D_pad
+    const int32_t *pairs;
+    const int32_t *tile_clu, *tile_first, *tile_cnt;
+    const int32_t *ntiles;
+    int32_t *work;  // tile counter, zero at launch
+    int NP, topk;
+    float *cand_d;  // (nq * NP) x topk
+    uint32_t *cand_id;
+};
+
+struct BSmem {
+    int8_t *a1, *a2;   // [QT][kq_stride]
+    float *qc;         // [QT][nqs][3]: sqd*s1, sqd*s2, (-1 + sqd/2) * sum r
+    int32_t *qcorr;    // [QT][nqs][2]: shift corrections of the two terms
+    float *q2;         // [QT]
+    float *dt;         // [QT][CV]
+    uint32_t *dids;    // [CV]
+    float *tkd;        // [QT][topk]
+    uint32_t *tki;     // [QT][topk]
+    int32_t *tkn;      // [QT]
+    int32_t *ctl;      // tile, cluster, first, cnt
+};
+
+template <int QTM, int QS>
+__host__ __device__ inline size_t bscan_smem(const BatchedView &bv, int topk, BSmem *m, uint8_t *base) {
+    constexpr int QT = 16 * QTM, CV = 64 * kBWaves / QTM;
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        uint8_t *p = base ? base + off : nullptr;
+        off += (bytes + 15) / 16 * 16;
+        return p;
+    };
+    BSmem s;
+    s.a1 = (int8_t *)take((size_t)QT * bv.kq_stride);
+    s.a2 = (int8_t *)take(QS == 2 ? (size_t)QT * bv.kq_stride : 0);
+    s.qc = (float *)take((size_t)QT * bv.nqs * 3 * 4);
+    s.qcorr = (int32_t *)take((size_t)QT * bv.nqs * 2 * 4);
+    s.q2 = (float *)take(QT * 4);
+    s.dt = (float *)take((size_t)QT * CV * 4);
+    s.dids = (uint32_t *)take(CV * 4);
+    s.tkd = (float *)take((size_t)QT * topk * 4);
+    s.tki = (uint32_t *)take((size_t)QT * topk * 4);
+    s.tkn = (int32_t *)take(QT * 4);
+    s.ctl = (int32_t *)take(16);
+    if (m) *m = s;
+    return off;
+}
+
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+__device__ __forceinline__ int wave_isum(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+__device__ __forceinline__ v4i unpack_nibbles(uint2 x) {
+    v4i r;
+    r.x = (int)(x.x & 0x0F0F0F0Fu);
+    r.y = (int)((x.x >> 4) & 0x0F0F0F0Fu);
+    r.z = (int)(x.y & 0x0F0F0F0Fu);
+    r.w = (int)((x.y >> 4) & 0x0F0F0F0Fu);
+    return r;
+}
+
+template <int QTM, int QS, int KR>
+__global__ __launch_bounds__(kBThreads) void bscan_kernel(BScanArgs a) {
+    constexpr int QT = 16 * QTM;
+    constexpr int NVW = 4 / QTM;          // mblocks per wave per chunk
+    constexpr int CV = 64 * kBWaves / QTM;  // vectors per chunk
+    extern __shared__ __align__(16) uint8_t smem_raw[];
+    BSmem m;
+    const BatchedView &bv = a.bv;
+    bscan_smem<QTM, QS>(bv, a.topk, &m, smem_raw);
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int ntiles = *a.ntiles;
+    for (;;) {
+        if (tid == 0) {
+            const int t = atomicAdd(a.work, 1);
+            m.ctl[0] = t;
+            if (t < ntiles) {
+                m.ctl[1] = a.tile_clu[t];
+                m.ctl[2] = a.tile_first[t];
+                m.ctl[3] = a.tile_cnt[t];
+            }
+        }
+        __syncthreads();
+        const int tile = m.ctl[0];
+        if (tile >= ntiles) break;
+        const int c = m.ctl[1], first = m.ctl[2], cnt = m.ctl[3];
+        __syncthreads();  // ctl read by every thread before the next tile overwrites it
+
+        // ---- quantise r = q_rot - c_rot of every query of the tile, per segment
+        const float *cr = bv.crot + (int64_t)c * bv.D_pad;
+        for (int i = w; i < QT; i += kBWaves) {
+            const int q = i < cnt ? a.pairs[first + i] / a.NP : -1;
+            const float *qr = q >= 0 ? a.qrot + (int64_t)q * bv.D_pad : nullptr;
+            float q2 = 0.0f;
+            for (int s = 0; s < bv.nseg; ++s) {
+                const BSeg bs = bv.segs[s];
+                float amax = 0.0f, sr = 0.0f, sq = 0.0f;
+                if (qr)
+                    for (int d = lane; d < bs.dim; d += 64) {
+                        const float r = qr[bs.doff + d] - cr[bs.doff + d];
+                        amax = fmaxf(amax, fabsf(r));
+                        sr += r;
+                        sq += r * r;
+                    }
+                q2 += wave_sum(sq);
+                if (bs.qs < 0) continue;
+                amax = wave_max(amax);
+                sr = wave_sum(sr);
+                const float s1 = amax / 127.0f, inv1 = amax > 0.0f ? 127.0f / amax : 0.0f;
+                const float s2 = s1 / 254.0f, inv2 = amax > 0.0f ? 254.0f * 127.0f / amax : 0.0f;
+                int sum1 = 0, sum2 = 0;
+                int8_t *row1 = m.a1 + i * bv.kq_stride + bs.kq_off;
+                int8_t *row2 = m.a2 + i * bv.kq_stride + bs.kq_off;
+                for (int d = lane; d < bs.dim; d += 64) {
+                    const float r = qr ? qr[bs.doff + d] - cr[bs.doff + d] : 0.0f;
+                    const int v1 = (int)fminf(127.0f, fmaxf(-127.0f, rintf(r * inv1)));
+                    row1[d] = (int8_t)v1;
+                    sum1 += v1;
+                    if (QS == 2) {
+                        const float rem = r - (float)v1 * s1;
+                        const int v2 = (int)fminf(127.0f, fmaxf(-127.0f, rintf(rem * inv2)));
+                        row2[d] = (int8_t)v2;
+                        sum2 += v2;
+                    }
+                }
+                sum1 = wave_isum(sum1);
+                if (QS == 2) sum2 = wave_isum(sum2);
+                if (lane == 0) {
+                    float *qc = m.qc + (i * bv.nqs + bs.qs) * 3;
+                    qc[0] = bs.sqd * s1;
+                    qc[1] = bs.sqd * s2;
+                    qc[2] = (-1.0f + bs.sqd / 2) * sr;
+                    int32_t *cc = m.qcorr + (i * bv.nqs + bs.qs) * 2;
+                    cc[0] = bs.bits >= 8 ? 128 * sum1 : 0;
+                    cc[1] = bs.bits >= 8 ? 128 * sum2 : 0;
+                }
+            }
+            if (lane == 0) {
+                m.q2[i] = q >= 0 ? q2 : INFINITY;
+                m.tkn[i] = 0;
+            }
+        }
+        __syncthreads();
+
+        // ---- stream the list in chunks of CV vectors
+        const int n = bv.clu_n[c];
+        const int nmb = (n + kMB - 1) / kMB;
+        const int64_t mb0 = bv.clu_mb[c];
+        for (int cb = 0; cb < nmb; cb += kBWaves * NVW) {
+            float dist[QTM][NVW][4];
+            const uint8_t *rec[NVW];
+#pragma unroll
+            for (int j = 0; j < NVW; ++j) {
+                const int mbj = min(cb + w * NVW + j, nmb - 1);  // clamped: a duplicate record, discarded below
+                rec[j] = bv.mb + (mb0 + mbj) * bv.mb_stride;
+                const float o2 = reinterpret_cast<const float *>(rec[j] + bv.fac_off)[lane % kMB];
+#pragma unroll
+                for (int t = 0; t < QTM; ++t)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) dist[t][j][r] = o2 + m.q2[t * 16 + 4 * (lane / 16) + r];
+            }
+            for (int s = 0; s < bv.nseg; ++s) {
+                const BSeg bs = bv.segs[s];
+                if (bs.qs < 0) continue;
+                v4i acc1[QTM][NVW], acc2[QTM][NVW], ach1[QTM][NVW], ach2[QTM][NVW];
+#pragma unroll
+                for (int t = 0; t < QTM; ++t)
+#pragma unroll
+                    for (int j = 0; j < NVW; ++j) {
+                        acc1[t][j] = v4i{0, 0, 0, 0};
+                        acc2[t][j] = v4i{0, 0, 0, 0};
+                        ach1[t][j] = v4i{0, 0, 0, 0};
+                        ach2[t][j] = v4i{0, 0, 0, 0};
+                    }
+                const int nk = bs.dim / 64;
+                const int8_t *arow1 = m.a1 + (lane % 16) * bv.kq_stride + bs.kq_off + 16 * (lane / 16);
+                const int8_t *arow2 = m.a2 + (lane % 16) * bv.kq_stride + bs.kq_off + 16 * (lane / 16);
+                for (int kc = 0; kc < nk; ++kc) {
+                    v4i A1[QTM], A2[QTM];
+#pragma unroll
+                    for (int t = 0; t < QTM; ++t) {
+                        A1[t] = *reinterpret_cast<const v4i *>(arow1 + t * 16 * bv.kq_stride + kc * 64);
+                        if (QS == 2) A2[t] = *reinterpret_cast<const v4i *>(arow2 + t * 16 * bv.kq_stride + kc * 64);
+                    }
+                    if (bs.bits <= 4) {
+                        v4i B[NVW];
+#pragma unroll
+                        for (int j = 0; j < NVW; ++j)
+                            B[j] = unpack_nibbles(*reinterpret_cast<const uint2 *>(
+                                rec[j] + bs.code_off + (int64_t)kc * 512 + lane * 8));
+#pragma unroll
+                        for (int t = 0; t < QTM; ++t)
+#pragma unroll
+                            for (int j = 0; j < NVW; ++j) {
+                                acc1[t][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A1[t], B[j], acc1[t][j], 0, 0, 0);
+                                if (QS == 2)
+                                    acc2[t][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A2[t], B[j], acc2[t][j], 0, 0, 0);
+                            }
+                    } else {
+                        v4i B[NVW], BH[NVW];
+                        const bool two = bs.bits > 8;
+#pragma unroll
+                        for (int j = 0; j < NVW; ++j) {
+                            const uint8_t *p = rec[j] + bs.code_off + (int64_t)kc * bs.chunk_bytes + lane * 16;
+                            B[j] = *reinterpret_cast<const v4i *>(p);
+                            if (two) BH[j] = *reinterpret_cast<const v4i *>(p + 1024);
+                        }
+#pragma unroll
+                        for (int t = 0; t < QTM; ++t)
+#pragma unroll
+                            for (int j = 0; j < NVW; ++j) {
+                                acc1[t][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A1[t], B[j], acc1[t][j], 0, 0, 0);
+                                if (QS == 2)
+                                    acc2[t][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A2[t], B[j], acc2[t][j], 0, 0, 0);
+                                if (two) {
+                                    ach1[t][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A1[t], BH[j], ach1[t][j], 0, 0, 0);
+                                    if (QS == 2)
+                                        ach2[t][j] =
+                                            __builtin_amdgcn_mfma_i32_16x16x64_i8(A2[t], BH[j], ach2[t][j], 0, 0, 0);
+                                }
+                            }
+                    }
+                }
+                // epilogue of the segment
+#pragma unroll
+                for (int j = 0; j < NVW; ++j) {
+                    const float resc =
+                        reinterpret_cast<const float *>(rec[j] + bv.fac_off)[kMB * (1 + bs.qs) + lane % kMB];
+#pragma unroll
+                    for (int t = 0; t < QTM; ++t)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            const int qi = t * 16 + 4 * (lane / 16) + r;
+                            const float *qc = m.qc + (qi * bv.nqs + bs.qs) * 3;
+                            const int32_t *cc = m.qcorr + (qi * bv.nqs + bs.qs) * 2;
+                            const float i1 = (float)(acc1[t][j][r] + cc[0]) + 256.0f * (float)ach1[t][j][r];
+                            float ip = qc[0] * i1;
+                            if (QS == 2) {
+                                const float i2 = (float)(acc2[t][j][r] + cc[1]) + 256.0f * (float)ach2[t][j][r];
+                                ip = ip + qc[1] * i2;
+                            }
+                            ip = resc * (ip + qc[2]);
+                            dist[t][j][r] = dist[t][j][r] - 2.0f * ip;
+                        }
+                }
+            }
+            // ---- distances of the chunk to LDS, then per-query selection
+#pragma unroll
+            for (int j = 0; j < NVW; ++j) {
+                const int lv = (w * NVW + j) * kMB + lane % kMB;  // vector of the chunk
+                const bool valid = cb + w * NVW + j < nmb;
+#pragma unroll
+                for (int t = 0; t < QTM; ++t)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        m.dt[(t * 16 + 4 * (lane / 16) + r) * CV + lv] = valid ? dist[t][j][r] : INFINITY;
+                if (lane < kMB)
+                    m.dids[lv] = valid ? reinterpret_cast<const uint32_t *>(rec[j] + bv.fac_off)[kMB * (1 + bv.nqs) + lane]
+                                       : 0xFFFFFFFFu;
+            }
+            __syncthreads();
+            for (int i = w; i < cnt; i += kBWaves) {
+                WaveTopK<KR> tk;
+                tk.load(m.tkd + i * a.topk, m.tki + i * a.topk, m.tkn[i], a.topk);
+#pragma unroll
+                for (int u = 0; u < CV / 64; ++u) tk.offer(m.dt[i * CV + u * 64 + lane], m.dids[u * 64 + lane]);
+                tk.store(m.tkd + i * a.topk, m.tki + i * a.topk);
+                if (lane == 0) m.tkn[i] = tk.n;
+            }
+            __syncthreads();
+        }
+        // ---- the tile's per-query lists to global memory
+        for (int i = w; i < cnt; i += kBWaves) {
+            const int64_t pr = a.pairs[first + i];
+            const int nn = m.tkn[i];
+            for (int e = lane; e < a.topk; e += 64) {
+                a.cand_d[pr * a.topk + e] = e < nn ? m.tkd[i * a.topk + e] : INFINITY;
+                a.cand_id[pr * a.topk + e] = e < nn ? m.tki[i * a.topk + e] : 0xFFFFFFFFu;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------ merge
+// One wave per query: the k smallest (distance, id) pairs over its lists' candidate lists.
+template <int KR>
+__global__ __launch_bounds__(256) void bq_merge_kernel(const float *cand_d, const uint32_t *cand_id,
+                                                      const int32_t *npq, int64_t nq, int NP, int topk,
+                                                      uint32_t *out_ids, float *out_d) {
+    const int lane = threadIdx.x & 63;
+    const int64_t q = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (q >= nq) return;
+    WaveTopK<KR> tk;
+    tk.k = topk;
+    tk.n = 0;
+#pragma unroll
+    for (int j = 0; j < KR; ++j) {
+        tk.d[j] = INFINITY;
+        tk.id[j] = 0xFFFFFFFFu;
+    }
+    tk.refresh();
+    const int lim = npq ? npq[q] : NP;
+    for (int p = 0; p < lim; ++p) {
+        const float *cd = cand_d + (q * NP + p) * topk;
+        const uint32_t *ci = cand_id + (q * NP + p) * topk;
+        for (int e0 = 0; e0 < topk; e0 += 64) {
+            const int e = e0 + lane;
+            const float x = e < topk ? cd[e] : INFINITY;
+            const uint32_t xi = e < topk ? ci[e] : 0xFFFFFFFFu;
+            // each list is sorted: once its head is not below the k-th pair, nothing is
+            if (!dless(__shfl(x, 0), __shfl(xi, 0), tk.kd, tk.kid)) break;
+            tk.offer(x, xi);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < KR; ++j) {
+        const int i = j * 64 + lane;
+        if (i < topk) {
+            out_ids[q * topk + i] = i < tk.n ? tk.id[j] : 0u;
+            if (out_d) out_d[q * topk + i] = i < tk.n ? tk.d[j] : 0.0f;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ host-side launchers
+hipError_t launch_bq_build(const IndexView &ix, const BatchedView &bv, const int32_t *mb_clu, int64_t total_mb,
+                           const int32_t *chunk_seg, const int32_t *chunk_kc, int nchunks, uint8_t *out,
+                           hipStream_t st) {
+    if (total_mb == 0) return hipSuccess;
+    if (nchunks > 0) {
+        hipLaunchKernelGGL(bq_build_codes_kernel, dim3((unsigned)total_mb, nchunks), dim3(64), 0, st, ix, bv, mb_clu,
+                           chunk_seg, chunk_kc, out);
+    }
+    hipLaunchKernelGGL(bq_build_factors_kernel, dim3((unsigned)total_mb), dim3(64), 0, st, ix, bv, mb_clu, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_bq_group(const uint32_t *probes, const int32_t *npq, int64_t nq, int NP, int K, int QT,
+                           const BqWork &w, hipStream_t st) {
+    hipLaunchKernelGGL(bq_fill_kernel, dim3(64), dim3(256), 0, st, w.counts, (int64_t)K, 0);
+    hipLaunchKernelGGL(bq_fill_kernel, dim3(1), dim3(64), 0, st, w.work, (int64_t)1, 0);
+    const int64_t n = nq * NP;
+    const unsigned g = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(bq_count_kernel, dim3(g), dim3(256), 0, st, probes, npq, nq, NP, w.counts, w.slot);
+    hipLaunchKernelGGL(bq_scan_kernel, dim3(1), dim3(1024), 0, st, w.counts, K, QT, w.cstart, w.tstart, w.ntiles);
+    hipLaunchKernelGGL(bq_scatter_kernel, dim3(g), dim3(256), 0, st, probes, w.slot, n, w.cstart, w.pairs);
+    hipLaunchKernelGGL(bq_tiles_kernel, dim3((K + 255) / 256), dim3(256), 0, st, w.counts, w.cstart, w.tstart, K, QT,
+                       w.tile_clu, w.tile_first, w.tile_cnt);
+    return hipGetLastError();
+}
+
+template <int QTM, int QS, int KR>
+static hipError_t launch_bscan_t(const BScanArgs &a, int grid, hipStream_t st) {
+    const size_t smem = bscan_smem<QTM, QS>(a.bv, a.topk, nullptr, nullptr);
+    auto kern = bscan_kernel<QTM, QS, KR>;
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+    if (e != hipSuccess) return e;
+    int per_cu = 0;
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kBThreads, smem);
+    if (e != hipSuccess) return e;
+    int dev = 0, ncu = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    const int g = std::max(1, std::min(grid, std::max(1, per_cu) * ncu));
+    hipLaunchKernelGGL(kern, dim3(g), dim3(kBThreads), smem, st, a);
+    return hipGetLastError();
+}
+
+size_t bscan_smem_bytes(const BatchedView &bv, int topk, int qtm, int qs) {
+    if (qtm == 1) return qs == 2 ? bscan_smem<1, 2>(bv, topk, nullptr, nullptr) : bscan_smem<1, 1>(bv, topk, nullptr, nullptr);
+    if (qtm == 2) return qs == 2 ? bscan_smem<2, 2>(bv, topk, nullptr, nullptr) : bscan_smem<2, 1>(bv, topk, nullptr, nullptr);
+    return qs == 2 ? bscan_smem<4, 2>(bv, topk, nullptr, nullptr) : bscan_smem<4, 1>(bv, topk, nullptr, nullptr);
+}
+
+template <int QTM, int QS>
+static hipError_t launch_bscan_q(const BScanArgs &a, int grid, hipStream_t st) {
+    if (a.topk <= 64) return launch_bscan_t<QTM, QS, 1>(a, grid, st);
+    if (a.topk <= 128) return launch_bscan_t<QTM, QS, 2>(a, grid, st);
+    return launch_bscan_t<QTM, QS, 4>(a, grid, st);
+}
+
+hipError_t launch_bscan_tiles(const BatchedView &bv, const float *qrot, const int32_t *pairs, const BqWork &w,
+                              int NP, int topk, float *cand_d, uint32_t *cand_id, int qtm, int qs, int grid,
+                              hipStream_t st) {
+    BScanArgs a;
+    a.bv = bv;
+    a.qrot = qrot;
+    a.pairs = pairs;
+    a.tile_clu = w.tile_clu;
+    a.tile_first = w.tile_first;
+    a.tile_cnt = w.tile_cnt;
+    a.ntiles = w.ntiles;
+    a.work = w.work;
+    a.NP = NP;
+    a.topk = topk;
+    a.cand_d = cand_d;
+    a.cand_id = cand_id;
+    if (qtm == 1) return qs == 2 ? launch_bscan_q<1, 2>(a, grid, st) : launch_bscan_q<1, 1>(a, grid, st);
+    if (qtm == 2) return qs == 2 ? launch_bscan_q<2, 2>(a, grid, st) : launch_bscan_q<2, 1>(a, grid, st);
+    return qs == 2 ? launch_bscan_q<4, 2>(a, grid, st) : launch_bscan_q<4, 1>(a, grid, st);
+}
+
+hipError_t launch_bq_merge(const float *cand_d, const uint32_t *cand_id, const int32_t *npq, int64_t nq, int NP,
+                           int topk, uint32_t *out_ids, float *out_d, hipStream_t st) {
+    const unsigned g = (unsigned)((nq + 3) / 4);
+    if (topk <= 64)
+        hipLaunchKernelGGL(bq_merge_kernel<1>, dim3(g), dim3(256), 0, st, cand_d, cand_id, npq, nq, NP, topk, out_ids,
+                           out_d);
+    else if (topk <= 128)
+        hipLaunchKernelGGL(bq_merge_kernel<2>, dim3(g), dim3(256), 0, st, cand_d, cand_id, npq, nq, NP, topk, out_ids,
+                           out_d);
+    else
+        hipLaunchKernelGGL(bq_merge_kernel<4>, dim3(g), dim3(256), 0, st, cand_d, cand_id, npq, nq, NP, topk, out_ids,
+                           out_d);
+    return hipGetLastError();
+}
+
+}  // namespace saqgpu

@@ -1,0 +1,51 @@
+// batched.hpp -- device layout and launchers of the batched int8-MFMA scan (batched.hip).
+#pragma once
+#include "common.hpp"
+
+namespace saqgpu {
+
+constexpr int kMB = 16;     // vectors per MFMA column block (one mblock record)
+constexpr int kBWaves = 4;  // waves per tile workgroup
+constexpr int kBThreads = 64 * kBWaves;
+
+struct BSeg {
+    int32_t dim, bits, doff;
+    int32_t kq_off;       // offset of the segment in a quantised query row (bits > 0)
+    int32_t code_off;     // byte offset of the segment's code chunks in an mblock record
+    int32_t chunk_bytes;  // bytes of one 64-dim chunk: 512 (b <= 4), 1024 (b <= 8), 2048 (b >= 9)
+    float sqd;            // 2 / 2^b
+    int32_t qs;           // index among the quantised segments, -1 for b = 0
+};
+
+struct BatchedView {
+    int32_t nseg, nqs, D_pad, kq;  // segments, quantised segments, padded dim, sum of quantised dims
+    int32_t fac_off;               // byte offset of the factors in an mblock record
+    int32_t kq_stride;             // LDS row stride of a quantised query (kq + 16)
+    int64_t mb_stride;             // bytes per mblock record
+    const uint8_t *mb;             // records
+    const int64_t *clu_mb;         // first mblock of list c
+    const int32_t *clu_n;          // vectors in list c
+    const float *crot;             // K x D_pad rotated centroids
+    const BSeg *segs;
+};
+
+struct BqWork {
+    int32_t *counts, *slot, *cstart, *tstart, *ntiles, *pairs, *tile_clu, *tile_first, *tile_cnt, *work;
+};
+
+size_t bscan_smem_bytes(const BatchedView &bv, int topk, int qtm, int qs);
+hipError_t launch_bq_build(const IndexView &ix, const BatchedView &bv, const int32_t *mb_clu, int64_t total_mb,
+                           const int32_t *chunk_seg, const int32_t *chunk_kc, int nchunks, uint8_t *out,
+                           hipStream_t st);
+hipError_t launch_bq_group(const uint32_t *probes, const int32_t *npq, int64_t nq, int NP, int K, int QT,
+                           const BqWork &w, hipStream_t st);
+hipError_t launch_bq_merge(const float *cand_d, const uint32_t *cand_id, const int32_t *npq, int64_t nq, int NP,
+                           int topk, uint32_t *out_ids, float *out_d, hipStream_t st);
+
+// One launch of the tile kernel: qtm = query tiles of 16 per work tile (1, 2, 4), qs = terms
+// of the quantised query (1 = int8, 2 = int8x2).  grid is an upper bound on the workgroups.
+hipError_t launch_bscan_tiles(const BatchedView &bv, const float *qrot, const int32_t *pairs, const BqWork &w,
+                              int NP, int topk, float *cand_d, uint32_t *cand_id, int qtm, int qs, int grid,
+                              hipStream_t st);
+
+}  // namespace saqgpu

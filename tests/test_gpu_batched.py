@@ -1,0 +1,130 @@
+"""The batched int8-MFMA scan (saqgpu_search_batched) against the numpy model of the same
+estimator (tests/batched_model.py), and against the plain scan's recall.
+
+Bar: every returned distance equals the model's distance of that vector to 1e-5 relative;
+the returned (distance, id) list is sorted and its IDs are the model's top-k except where the
+model's own distances tie within that tolerance at the k-th place; results do not depend on
+the query tile; recall@10 against exact FP64 neighbours is at least the plain scan's (int8x2).
+"""
+import numpy as np
+import pytest
+
+import batched_model as BM
+import saq_fixtures as F
+from saq_amd import PREC_INT8, PREC_INT8X2
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def model(case, probes, topk, qs, npq=None):
+    tabs = BM.vector_tables(case)
+    qr = BM.rotated_queries(case)
+    return BM.model_search(case, tabs, qr, probes, topk, qs=qs, npq=npq)
+
+
+def check(case, res, alld, ids, dists, topk):
+    for i in range(ids.shape[0]):
+        want = res[i]
+        n = len(want)
+        got_ids = ids[i, :n].tolist()
+        got_d = dists[i, :n]
+        assert np.all(ids[i, n:] == 0) and np.all(dists[i, n:] == 0), f"query {i}: unfilled slots"
+        for u, dv in zip(got_ids, got_d.tolist()):
+            assert u in alld[i], f"query {i}: id {u} not in a probed list"
+            ref = alld[i][u]
+            assert abs(dv - ref) <= TOL * max(1.0, abs(ref)), f"query {i} id {u}: {dv} vs model {ref}"
+        pairs = list(zip(got_d.tolist(), got_ids))
+        assert pairs == sorted(pairs), f"query {i}: not sorted by (distance, id)"
+        if n:
+            kth = want[-1][1]
+            for u in set(got_ids) ^ {u for u, _ in want}:
+                assert abs(alld[i][u] - kth) <= 2 * TOL * max(1.0, abs(kth)), f"query {i}: id {u} differs off a tie"
+
+
+def run(case, topk, nprobe, qs, npq=None, query_tile=0):
+    probes = F.orc_probes(case, nprobe)
+    res, alld = model(case, probes, topk, qs, npq)
+    ix = F.gpu_index(case)
+    print(f"batched case plan={case.plan} k={topk} nprobe={nprobe} prec={qs} tile={query_tile}", flush=True)
+    ids, d = ix.search_batched(case.queries, topk, nprobe, probes=probes, nprobe_per_query=npq,
+                               precision=qs, query_tile=query_tile)
+    check(case, res, alld, ids, d, topk)
+    ix.close()
+    return ids, d
+
+
+@pytest.mark.parametrize("plan", [[(128, 4)], [(128, 7)], [(128, 8)], [(128, 12)], [(64, 1), (64, 3)],
+                                  [(64, 9), (64, 5), (64, 2), (64, 0)]])
+@pytest.mark.parametrize("qs", [PREC_INT8, PREC_INT8X2])
+def test_batched_matches_model(plan, qs):
+    d = sum(x for x, _ in plan)
+    case = F.make_case(n=2500, d=d, K=10, nq=24, plan=plan, seed=51, tau=32)
+    run(case, 10, 4, qs)
+
+
+def test_dp_plan_and_padded_dim():
+    case = F.make_case(n=3000, d=200, K=12, nq=20, avg_bits=4.0, seed=53, tau=32)
+    run(case, 10, 5, PREC_INT8X2)
+
+
+@pytest.mark.parametrize("topk", [1, 100, 200])
+def test_topk_sizes_and_unfilled(topk):
+    case = F.make_case(n=1200, d=128, K=8, nq=12, plan=[(64, 6), (64, 3)], seed=55)
+    run(case, topk, 2, PREC_INT8X2)
+
+
+def test_query_tiles_give_identical_results():
+    # 4 lists and 200 queries: several full tiles per list at every tile size
+    case = F.make_case(n=3000, d=128, K=4, nq=200, plan=[(128, 5)], seed=57)
+    out = [run(case, 10, 2, PREC_INT8, query_tile=t) for t in (16, 32, 64)]
+    for ids, d in out[1:]:
+        assert np.array_equal(ids, out[0][0])
+        assert np.array_equal(d.view(np.uint32), out[0][1].view(np.uint32))
+
+
+def test_uneven_nprobe_per_query():
+    case = F.make_case(n=2000, d=128, K=16, nq=40, plan=[(64, 4), (64, 4)], seed=59)
+    npq = np.random.default_rng(3).integers(1, 9, 40).astype(np.int32)
+    run(case, 10, 8, PREC_INT8X2, npq=npq)
+
+
+def test_empty_lists_and_gpu_probe():
+    ids_in = np.zeros(900, np.uint32)
+    ids_in[300:600] = 2
+    ids_in[600:] = 4  # lists 1 and 3 are empty
+    case = F.make_case(n=900, d=128, K=5, nq=10, plan=[(128, 4)], seed=61, cluster_ids=ids_in)
+    ids_a, d_a = run(case, 20, 5, PREC_INT8X2)
+    ix = F.gpu_index(case)
+    ids_b, d_b = ix.search_batched(case.queries, 20, 5, precision=PREC_INT8X2)  # GPU coarse quantiser
+    assert np.array_equal(ids_a, ids_b) and np.array_equal(d_a.view(np.uint32), d_b.view(np.uint32))
+    ix.close()
+
+
+def test_autotune_keeps_results():
+    case = F.make_case(n=3000, d=128, K=6, nq=100, plan=[(128, 4)], seed=63)
+    probes = F.orc_probes(case, 3)
+    ix = F.gpu_index(case)
+    a, _ = ix.search_batched(case.queries, 10, 3, probes=probes)
+    best = ix.autotune_batched(case.queries, 10, 3, probes=probes)
+    assert best in (16, 32, 64)
+    b, _ = ix.search_batched(case.queries, 10, 3, probes=probes)
+    assert np.array_equal(a, b)
+    ix.close()
+
+
+def test_recall_not_below_plain_scan():
+    case = F.make_case(n=20000, d=128, K=64, nq=100, plan=[(128, 4)], seed=65, comps=32, tau=16)
+    X = case.base.astype(np.float64)
+    Q = case.queries.astype(np.float64)
+    dd = (Q * Q).sum(1)[:, None] + (X * X).sum(1)[None, :] - 2 * Q @ X.T
+    gt = np.argsort(dd, axis=1)[:, :10]
+    ix = F.gpu_index(case)
+    plain, _, _ = ix.search(case.queries, 10, 16)
+    bat, _ = ix.search_batched(case.queries, 10, 16, precision=PREC_INT8X2)
+    r_plain = np.mean([len(set(plain[i]) & set(gt[i])) / 10 for i in range(100)])
+    r_bat = np.mean([len(set(bat[i]) & set(gt[i])) / 10 for i in range(100)])
+    print(f"recall@10 plain {r_plain:.3f} batched {r_bat:.3f}")
+    assert r_bat >= r_plain - 0.005
+    ix.close()
