@@ -1,4 +1,5 @@
-"""Benchmark of the plain SAQ IVF scan on one MI355X.
+"""Benchmark of the SAQ IVF scan on one MI355X: the batched int8-MFMA scan against the plain
+(reference-exact) GPU scan, at equal recall.
 
 Workload (the single-GPU configuration the headline metric is quoted on):
 N = 1,000,000 base vectors, D = 960 fp32, from a 256-component Gaussian mixture
@@ -10,12 +11,16 @@ per dimension are the smallest of 4..8 whose recall@10 against exact FP64 ground
 truth reaches 0.95 at nprobe 64, and nprobe is then the smallest value of a sweep
 that keeps it.  The recall reached by every bits value tried is in the JSON line.
 
-A step = one batched search of the 1,000 queries (GPU coarse quantiser + query
-rotation + scan), inputs resident in HBM.  The JSON line also carries the scan
-kernel's roofline (algorithmic bytes / HIP-event kernel time vs 8 TB/s), the GPU
-encoder's throughput, and a CPU baseline: the oracle (CPU restatement of the
-reference) timed on this host's cores on the same queries, whose IDs are also
-compared with the GPU's.
+A step = one search of the 1,000 queries (GPU coarse quantiser + query rotation +
+scan), inputs resident in HBM.  The plain scan is timed first; the batched scan is
+then run at the smallest nprobe (and the cheaper of its int8 / int8x2 precisions)
+whose recall@10 is at least the plain scan's, with the query tile its autotuner
+picks.  `value` is the batched scan's throughput; the plain scan's is in
+config.plain.  The JSON line also carries each scan kernel's roofline (algorithmic
+bytes / HIP-event kernel time vs 8 TB/s; int8 MFMA ops vs 5 POP/s for the batched
+kernel), the GPU encoder's throughput, and a CPU baseline: the oracle (CPU
+restatement of the reference) timed on this host's cores on the same queries,
+whose IDs are also compared with the plain GPU scan's.
 """
 import argparse
 import json
@@ -30,6 +35,20 @@ sys.path.insert(0, ROOT)
 
 METRIC = "QPS @ recall@10=0.95 batched vs plain GPU; encode Mvec/s; p99 ms"
 HBM_PEAK_GBS = 8000.0
+INT8_PEAK_TOPS = 5000.0  # dense int8 MFMA, MI355X
+
+
+def read_traffic(name, n, bits, nprobe):
+    """HBM bytes per launch of a kernel from its committed FETCH_SIZE/WRITE_SIZE summary, when
+    that summary was taken on this workload."""
+    path = os.path.join(ROOT, "profiles", name)
+    try:
+        t = json.load(open(path))
+        if t.get("nprobe") == nprobe and t.get("bits") == bits and t.get("n") == n:
+            return t.get("hbm_bytes_per_launch")
+    except (OSError, ValueError):
+        pass
+    return None
 
 
 def log(msg):
@@ -250,16 +269,21 @@ def main():
         if rec >= args.target_recall:
             break
 
-    for _ in range(args.warmup):
-        search(ix, nprobe)
-    torch.cuda.synchronize()
-    ix.timing(True)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        search(ix, nprobe)
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    scan_ms, launches = ix.timing(False)
+    def timed(fn):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        ix.timing(True)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            fn()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        kms, launches = ix.timing(False)
+        return dt, kms / max(1, launches)
+
+    # ---- plain scan (bit-exact with the reference)
+    dt, kern_ms = timed(lambda: search(ix, nprobe))
     qps = args.nq * args.steps / dt
     ids_gpu = out_ids.cpu().numpy().astype(np.uint32)
     m = met.cpu().numpy().astype(np.int64)
@@ -274,17 +298,77 @@ def main():
     d_pad = sum(d for d, _ in plan)
     alg_bytes = blocks * block_stride + m[:, 1].sum() // 8 + refines * (8 * len(plan) + 4) + \
         args.nq * nprobe * d_pad * 4
-    kern_s = scan_ms / 1e3 / max(1, launches)
+    kern_s = kern_ms / 1e3
     achieved = alg_bytes / kern_s / 1e9
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "scan_traffic.json")
-    if os.path.exists(tpath):
-        try:
-            t = json.load(open(tpath))
-            if t.get("nprobe") == nprobe and t.get("bits") == bits and t.get("n") == args.n:
-                traffic = t.get("hbm_bytes_per_launch")
-        except Exception:
-            traffic = None
+    traffic = read_traffic("scan_traffic.json", args.n, bits, nprobe)
+    plain = {"qps": round(qps, 1), "recall_at_10": round(rec, 4), "nprobe": nprobe,
+             "ms_per_step": round(dt / args.steps * 1e3, 4),
+             "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                          "alg_bytes_per_launch": int(alg_bytes), "kernel_ms": round(kern_ms, 4)}}
+    log(f"plain: {qps:.0f} QPS, scan kernel {kern_ms:.3f} ms, {achieved:.0f} GB/s")
+
+    # ---- batched int8-MFMA scan at a recall no lower than the plain scan's
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ix.prepare_batched()
+    torch.cuda.synchronize()
+    t_prep = time.perf_counter() - t0
+    bids = torch.zeros(args.nq, args.topk, dtype=torch.int32, device=dev)
+    bd = torch.zeros(args.nq, args.topk, dtype=torch.float32, device=dev)
+
+    def bsearch(npb, prec, qt=0):
+        ix.search_batched(queries, args.topk, npb, precision=prec, query_tile=qt, out_ids=bids, out_dists=bd,
+                          stream=stream)
+
+    def brecall(npb, prec):
+        bsearch(npb, prec)
+        torch.cuda.synchronize()
+        return recall(bids.cpu().numpy(), gt, args.topk)
+
+    choices = []
+    for prec in (saq_amd.PREC_INT8, saq_amd.PREC_INT8X2):
+        for npb in [x for x in sweep if x <= max(nprobe * 2, nprobe)]:
+            r = brecall(npb, prec)
+            log(f"batched prec={prec} nprobe {npb}: recall@{args.topk} {r:.4f}")
+            if r >= rec:
+                qt = ix.autotune_batched(queries, args.topk, npb, precision=prec, stream=stream)
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                for _ in range(3):
+                    bsearch(npb, prec)
+                torch.cuda.synchronize()
+                choices.append(((time.perf_counter() - t0) / 3, prec, npb, qt, r))
+                break
+    if not choices:
+        raise SystemExit("batched scan did not reach the plain scan's recall")
+    _, bprec, bnp, bqt, brec = min(choices)
+    bdt, bkern_ms = timed(lambda: bsearch(bnp, bprec, bqt))
+    bqps = args.nq * args.steps / bdt
+    # algorithmic work of one tile-kernel launch: per work tile (a list and up to bqt of its
+    # queries) one pass over the list's 16-vector records; the residual queries (q_rot, c_rot)
+    # and the per-pair candidate lists; MFMA ops = 2 * K per (query, vector) pair and term
+    pr = ix.probe(queries.cpu().numpy(), bnp).astype(np.int64)
+    cnt = np.bincount(pr.ravel(), minlength=args.nlist)
+    nmb = (sizes + 15) // 16
+    kq_bytes = sum((512 if b <= 4 else (1024 if b <= 8 else 2048)) * (d // 64) for d, b in plan if b > 0)
+    nqs = sum(1 for _, b in plan if b > 0)
+    rstride = (kq_bytes + 64 * (2 + nqs) + 255) // 256 * 256
+    tiles = (cnt + bqt - 1) // bqt
+    b_bytes = int((tiles * nmb).sum() * rstride + pr.size * (2 * d_pad * 4 + args.topk * 8))
+    k_eff = sum(d * (2 if b > 8 else 1) for d, b in plan if b > 0)
+    b_ops = float((cnt * sizes).sum()) * 2 * k_eff * (2 if bprec == saq_amd.PREC_INT8X2 else 1)
+    b_gbs = b_bytes / (bkern_ms / 1e3) / 1e9
+    b_tops = b_ops / (bkern_ms / 1e3) / 1e12
+    batched = {"qps": round(bqps, 1), "recall_at_10": round(brec, 4), "nprobe": bnp,
+               "precision": "int8x2" if bprec == saq_amd.PREC_INT8X2 else "int8", "query_tile": bqt,
+               "ms_per_step": round(bdt / args.steps * 1e3, 4), "layout_build_ms": round(t_prep * 1e3, 1),
+               "kernel_ms": round(bkern_ms, 4), "alg_bytes_per_launch": b_bytes,
+               "hbm_frac": round(b_gbs / HBM_PEAK_GBS, 4), "int8_mfma_tops": round(b_tops, 2),
+               "int8_mfma_frac": round(b_tops / INT8_PEAK_TOPS, 5)}
+    log(f"batched: {bqps:.0f} QPS ({bqps / qps:.2f}x plain), tile kernel {bkern_ms:.3f} ms, {b_gbs:.0f} GB/s, "
+        f"{b_tops:.1f} TOPS")
+    btraffic = read_traffic("bscan_traffic.json", args.n, bits, bnp)
 
     cpu = None
     if not args.no_cpu:
@@ -302,25 +386,26 @@ def main():
 
     out = {
         "metric": METRIC,
-        "value": round(qps, 1),
+        "value": round(bqps, 1),
         "unit": "queries/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(dt / args.steps * 1e3, 4),
+        "ms_per_step": round(bdt / args.steps * 1e3, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "u16-lut/i32/f32",
+        "dtype": "i8-mfma/i32/f32",
         "data": "synthetic",
-        "config": {"workload": f"plain GPU SAQ scan: N={args.n} D={args.dim} nlist={args.nlist} B={bits} "
-                               f"nq={args.nq} k={args.topk} nprobe={nprobe}",
-                   "recall_at_10": round(rec, 4), "nprobe": nprobe, "bits": bits, "plan": plan,
-                   "recall_at_10_by_bits_nprobe64": bits_tried,
+        "config": {"workload": f"batched SAQ IVF scan vs plain GPU scan: N={args.n} D={args.dim} "
+                               f"nlist={args.nlist} B={bits} nq={args.nq} k={args.topk}",
+                   "bits": bits, "plan": plan, "recall_at_10_by_bits_nprobe64": bits_tried,
+                   "batched": batched, "plain": plain, "batched_vs_plain": round(bqps / qps, 3),
                    "encode_mvec_per_s": round(args.n / t_enc / 1e6, 3)},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "alg_bytes_per_launch": int(alg_bytes), "kernel_ms": round(kern_s * 1e3, 4)},
+        "roofline": {"bound": "hbm", "achieved": round(b_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(b_gbs / HBM_PEAK_GBS, 4), "traffic": btraffic,
+                     "kernel": "bscan_kernel", "alg_bytes_per_launch": b_bytes,
+                     "kernel_ms": round(bkern_ms, 4)},
         "cpu_baseline": cpu,
     }
     log(f"total {time.time() - t_all:.0f}s")

@@ -206,7 +206,11 @@ class Index:
         dev = not isinstance(queries, np.ndarray)
         nq = int(queries.shape[0])
         if out is None:
-            out = np.zeros((nq, nprobe), np.uint32)
+            if dev:
+                import torch
+                out = torch.empty((nq, nprobe), dtype=torch.int32, device=queries.device)
+            else:
+                out = np.zeros((nq, nprobe), np.uint32)
         _check(lib().saqgpu_probe(self.h, _ptr(queries), nq, nprobe, dist_type, _ptr(out),
                                   DEVICE_PTRS if dev else 0, stream))
         return out

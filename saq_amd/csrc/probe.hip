@@ -78,6 +78,97 @@ __global__ __launch_bounds__(256) void probe_dist_kernel(const float *__restrict
     out[q * K + c] = d;
 }
 
+// Register-tiled form of probe_dist_kernel for D >= 32.  Eigen's redux keeps 32 running sums
+// (two 16-lane packets) over the first floor(D/32)*32 elements, sum j taking the elements
+// i = j (mod 32) in increasing order; here lane j of a half-wave holds exactly that sum, for
+// 8 queries x 16 centroids at once, with the rows staged through LDS in chunks of 128 dims.
+// The combine (p0 + p1), the trailing 16-element packet, the halving tree and the scalar tail
+// then follow Eigen's order across lanes.  A workgroup covers 32 queries x 32 centroids.
+constexpr int kPT = 32;     // queries (and centroids) per workgroup tile
+constexpr int kPCh = 128;   // dims per LDS chunk
+template <int DT>
+__global__ __launch_bounds__(256) void probe_tile_kernel(const float *__restrict__ cent, int K, int D,
+                                                         const float *__restrict__ queries, int64_t nq,
+                                                         float *__restrict__ out) {
+    __shared__ float qs[kPT][kPCh];
+    __shared__ float cs[kPT][kPCh];
+    const int tid = threadIdx.x, hw = tid >> 5, j = tid & 31;
+    const int64_t q0 = (int64_t)blockIdx.y * kPT;
+    const int c0 = blockIdx.x * kPT;
+    const int qa = (hw >> 1) * 8;   // this half-wave's 8 queries within the tile
+    const int cb = (hw & 1) * 16;   // and 16 centroids
+    const int aligned = D / 16 * 16, aligned2 = D / 32 * 32;
+    float acc[8][16];
+    for (int base = 0; base < aligned2; base += kPCh) {
+        const int len = min(kPCh, aligned2 - base);
+        __syncthreads();
+        for (int e = tid; e < kPT * kPCh; e += 256) {
+            const int r = e / kPCh, i = e % kPCh;
+            if (i < len) {
+                const int64_t q = q0 + r;
+                const int c = c0 + r;
+                qs[r][i] = q < nq ? queries[q * D + base + i] : 0.0f;
+                cs[r][i] = c < K ? cent[(int64_t)c * D + base + i] : 0.0f;
+            }
+        }
+        __syncthreads();
+        for (int t = 0; t < len / 32; ++t) {
+            const int i = 32 * t + j;
+            float qv[8], cv[16];
+#pragma unroll
+            for (int a = 0; a < 8; ++a) qv[a] = qs[qa + a][i];
+#pragma unroll
+            for (int b = 0; b < 16; ++b) cv[b] = cs[cb + b][i];
+            const bool first = base == 0 && t == 0;
+#pragma unroll
+            for (int a = 0; a < 8; ++a)
+#pragma unroll
+                for (int b = 0; b < 16; ++b) {
+                    float e;
+                    if (DT == 1) {
+                        e = qv[a] * cv[b];
+                    } else {
+                        const float d = cv[b] - qv[a];
+                        e = d * d;
+                    }
+                    acc[a][b] = first ? e : acc[a][b] + e;
+                }
+        }
+    }
+    // per pair: combine, trailing packet, halving tree, tail (lane 0 ends with the result)
+#pragma unroll
+    for (int a = 0; a < 8; ++a)
+#pragma unroll
+        for (int b = 0; b < 16; ++b) {
+            const int64_t q = q0 + qa + a;
+            const int c = c0 + cb + b;
+            const bool ok = q < nq && c < K;
+            const float *qr = queries + (ok ? q : 0) * D;
+            const float *cr = cent + (int64_t)(ok ? c : 0) * D;
+            auto elem = [&](int i) {
+                if (DT == 1) return qr[i] * cr[i];
+                const float d = cr[i] - qr[i];
+                return d * d;
+            };
+            float p0;
+            if (aligned > 16) {
+                const float hi = __shfl_down(acc[a][b], 16, 32);
+                p0 = acc[a][b] + hi;
+                if (aligned > aligned2 && j < 16) p0 = p0 + elem(aligned2 + j);
+            } else {
+                p0 = j < 16 ? elem(j) : 0.0f;
+            }
+            const float b8 = p0 + __shfl_down(p0, 8, 32);
+            const float c4 = b8 + __shfl_down(b8, 4, 32);
+            const float c2 = __shfl_down(c4, 2, 32), c1 = __shfl_down(c4, 1, 32), c3 = __shfl_down(c4, 3, 32);
+            float r = (c4 + c2) + (c1 + c3);
+            if (j == 0 && ok) {
+                for (int i = aligned; i < D; ++i) r = r + elem(i);
+                out[q * K + c] = r;
+            }
+        }
+}
+
 constexpr int kSelMax = 4096;
 
 // 64-bit key: ascending order = probe order.
@@ -164,13 +255,22 @@ __global__ __launch_bounds__(256) void probe_select_kernel(const float *__restri
 hipError_t launch_probe(const float *centroids, int K, int D, const float *queries, int64_t nq, int nprobe,
                         int dist_type, float *dist_ws, uint32_t *probes, hipStream_t st) {
     if (nprobe > kSelMax) return hipErrorInvalidValue;
+    const bool tiled = D >= 32 && (nq + kPT - 1) / kPT <= 65535;
     dim3 g1((unsigned)((K + 255) / 256), (unsigned)nq);
+    dim3 gt((unsigned)((K + kPT - 1) / kPT), (unsigned)((nq + kPT - 1) / kPT));
     const size_t sm = (size_t)D * 4;
+    if (!tiled && nq > 65535) return hipErrorInvalidValue;
     if (dist_type == 1) {
-        hipLaunchKernelGGL(probe_dist_kernel<1>, g1, dim3(256), sm, st, centroids, K, D, queries, dist_ws);
+        if (tiled)
+            hipLaunchKernelGGL(probe_tile_kernel<1>, gt, dim3(256), 0, st, centroids, K, D, queries, nq, dist_ws);
+        else
+            hipLaunchKernelGGL(probe_dist_kernel<1>, g1, dim3(256), sm, st, centroids, K, D, queries, dist_ws);
         hipLaunchKernelGGL(probe_select_kernel<1>, dim3((unsigned)nq), dim3(256), 0, st, dist_ws, K, nprobe, probes);
     } else {
-        hipLaunchKernelGGL(probe_dist_kernel<0>, g1, dim3(256), sm, st, centroids, K, D, queries, dist_ws);
+        if (tiled)
+            hipLaunchKernelGGL(probe_tile_kernel<0>, gt, dim3(256), 0, st, centroids, K, D, queries, nq, dist_ws);
+        else
+            hipLaunchKernelGGL(probe_dist_kernel<0>, g1, dim3(256), sm, st, centroids, K, D, queries, dist_ws);
         hipLaunchKernelGGL(probe_select_kernel<0>, dim3((unsigned)nq), dim3(256), 0, st, dist_ws, K, nprobe, probes);
     }
     return hipGetLastError();
