@@ -113,7 +113,9 @@ struct saqgpu_index {
     DevBuf<uint8_t> bq_mb;
     DevBuf<int64_t> d_clu_mb;
     DevBuf<BSeg> d_bsegs;
-    DevBuf<int32_t> bq_ws, bq_npq;
+    DevBuf<int32_t> bq_ws, bq_npq, bq_pcorr;
+    DevBuf<int8_t> bq_qq;
+    DevBuf<float> bq_pc;
     DevBuf<float> bq_cand_d;
     DevBuf<uint32_t> bq_cand_id;
     // measurement
@@ -336,6 +338,9 @@ void saqgpu_index_destroy(saqgpu_index *ix) {
     ix->d_clu_mb.release();
     ix->d_bsegs.release();
     ix->bq_ws.release();
+    ix->bq_pcorr.release();
+    ix->bq_qq.release();
+    ix->bq_pc.release();
     ix->bq_npq.release();
     ix->bq_cand_d.release();
     ix->bq_cand_id.release();
@@ -930,7 +935,10 @@ int search_batched_locked(saqgpu_index *ix, const float *queries, int64_t nq, co
     rc = 0;
     rc |= ix->ws_qrot.ensure(nq * ix->D_pad);
     rc |= ix->ws_bound.ensure(nq * ix->nseg);
-    rc |= ix->bq_ws.ensure(3 * (size_t)K + 2 * npairs + 3 * max_tiles + 2);
+    rc |= ix->bq_ws.ensure(3 * (size_t)K + 2 * npairs + 3 * max_tiles + 3);
+    rc |= ix->bq_qq.ensure((size_t)qs * npairs * ix->bq_kq);
+    rc |= ix->bq_pc.ensure((size_t)npairs * (3 * ix->bq_nqs + 1));
+    rc |= ix->bq_pcorr.ensure((size_t)npairs * 2 * std::max(1, ix->bq_nqs));
     rc |= ix->bq_cand_d.ensure(npairs * topk);
     rc |= ix->bq_cand_id.ensure(npairs * topk);
     if (!dev) {
@@ -978,8 +986,16 @@ int search_batched_locked(saqgpu_index *ix, const float *queries, int64_t nq, co
     w.tile_first = w.tile_clu + max_tiles;
     w.tile_cnt = w.tile_first + max_tiles;
     w.ntiles = w.tile_cnt + max_tiles;
-    w.work = w.ntiles + 1;
+    w.npairs = w.ntiles + 1;
+    w.work = w.npairs + 1;
+    BqPairBufs pb;
+    pb.qq1 = ix->bq_qq.p;
+    pb.qq2 = qs == 2 ? ix->bq_qq.p + npairs * ix->bq_kq : nullptr;
+    pb.pc = ix->bq_pc.p;
+    pb.pq2 = ix->bq_pc.p + npairs * 3 * ix->bq_nqs;
+    pb.pcorr = ix->bq_pcorr.p;
     HIPCHK(launch_bq_group(dp, dnpq, nq, NP, K, query_tile, w, st));
+    HIPCHK(launch_bq_pairprep(bv, ix->ws_qrot.p, dp, w, pb, NP, npairs, qs, st));
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (time_it) {
         HIPCHK(hipEventCreate(&e0));
@@ -987,8 +1003,7 @@ int search_batched_locked(saqgpu_index *ix, const float *queries, int64_t nq, co
         HIPCHK(hipEventRecord(e0, st));
     }
     const int grid = (int)std::min<int64_t>(max_tiles, 1 << 20);
-    HIPCHK(launch_bscan_tiles(bv, ix->ws_qrot.p, w.pairs, w, NP, topk, ix->bq_cand_d.p, ix->bq_cand_id.p, qtm, qs, grid,
-                              st));
+    HIPCHK(launch_bscan_tiles(bv, w, pb, topk, ix->bq_cand_d.p, ix->bq_cand_id.p, qtm, qs, grid, st));
     if (time_it) {
         HIPCHK(hipEventRecord(e1, st));
         ix->events.emplace_back(e0, e1);

@@ -182,7 +182,7 @@ __global__ void bq_count_kernel(const uint32_t *probes, const int32_t *npq, int6
 
 // One workgroup: exclusive prefix sums of the pair counts and of the tile counts per list.
 __global__ __launch_bounds__(1024) void bq_scan_kernel(const int32_t *counts, int K, int QT, int32_t *cstart,
-                                                       int32_t *tstart, int32_t *ntiles) {
+                                                       int32_t *tstart, int32_t *ntiles, int32_t *npairs) {
     __shared__ int32_t part_c[1024], part_t[1024];
     const int t = threadIdx.x;
     const int per = (K + 1023) / 1024;
@@ -209,7 +209,10 @@ __global__ __launch_bounds__(1024) void bq_scan_kernel(const int32_t *counts, in
         rc += counts[i];
         rt += (counts[i] + QT - 1) / QT;
     }
-    if (t == 1023) *ntiles = part_t[1023];
+    if (t == 1023) {
+        *ntiles = part_t[1023];
+        *npairs = part_c[1023];
+    }
 }
 
 __global__ void bq_scatter_kernel(const uint32_t *probes, const int32_t *slot, int64_t n, const int32_t *cstart,
@@ -335,56 +338,23 @@ struct WaveTopK {
     }
 };
 
-// ------------------------------------------------------------------ tile kernel
-struct BScanArgs {
+// ------------------------------------------------------------------ pair preparation
+// One wave per grouped (query, list) pair: the residual r = q_rot - c_rot of every segment,
+// quantised to int8 (plus the second int8 term for int8x2) as the pair's row of the MFMA A
+// operand, and the pair's constants.  Rows are stored in grouped order, so a work tile's
+// queries are consecutive rows.
+struct PairPrepArgs {
     BatchedView bv;
     const float *qrot;  // nq x D_pad
+    const uint32_t *probes;
     const int32_t *pairs;
-    const int32_t *tile_clu, *tile_first, *tile_cnt;
-    const int32_t *ntiles;
-    int32_t *work;  // tile counter, zero at launch
-    int NP, topk;
-    float *cand_d;  // (nq * NP) x topk
-    uint32_t *cand_id;
+    const int32_t *npairs;  // active pairs (device scalar)
+    int NP, qs;
+    int8_t *qq1, *qq2;  // [pairs][kq]
+    float *pc;          // [pairs][nqs][3]: sqd * s1, sqd * s2, (-1 + sqd / 2) * sum r
+    int32_t *pcorr;     // [pairs][nqs][2]: shift corrections of the two terms
+    float *pq2;         // [pairs]: |r|^2 over all segments
 };
-
-struct BSmem {
-    int8_t *a1, *a2;   // [QT][kq_stride]
-    float *qc;         // [QT][nqs][3]: sqd*s1, sqd*s2, (-1 + sqd/2) * sum r
-    int32_t *qcorr;    // [QT][nqs][2]: shift corrections of the two terms
-    float *q2;         // [QT]
-    float *dt;         // [QT][CV]
-    uint32_t *dids;    // [CV]
-    float *tkd;        // [QT][topk]
-    uint32_t *tki;     // [QT][topk]
-    int32_t *tkn;      // [QT]
-    int32_t *ctl;      // tile, cluster, first, cnt
-};
-
-template <int QTM, int QS>
-__host__ __device__ inline size_t bscan_smem(const BatchedView &bv, int topk, BSmem *m, uint8_t *base) {
-    constexpr int QT = 16 * QTM, CV = 64 * kBWaves / QTM;
-    size_t off = 0;
-    auto take = [&](size_t bytes) {
-        uint8_t *p = base ? base + off : nullptr;
-        off += (bytes + 15) / 16 * 16;
-        return p;
-    };
-    BSmem s;
-    s.a1 = (int8_t *)take((size_t)QT * bv.kq_stride);
-    s.a2 = (int8_t *)take(QS == 2 ? (size_t)QT * bv.kq_stride : 0);
-    s.qc = (float *)take((size_t)QT * bv.nqs * 3 * 4);
-    s.qcorr = (int32_t *)take((size_t)QT * bv.nqs * 2 * 4);
-    s.q2 = (float *)take(QT * 4);
-    s.dt = (float *)take((size_t)QT * CV * 4);
-    s.dids = (uint32_t *)take(CV * 4);
-    s.tkd = (float *)take((size_t)QT * topk * 4);
-    s.tki = (uint32_t *)take((size_t)QT * topk * 4);
-    s.tkn = (int32_t *)take(QT * 4);
-    s.ctl = (int32_t *)take(16);
-    if (m) *m = s;
-    return off;
-}
 
 __device__ __forceinline__ float wave_max(float v) {
 #pragma unroll
@@ -402,6 +372,127 @@ __device__ __forceinline__ int wave_isum(int v) {
     return v;
 }
 
+__global__ __launch_bounds__(256) void bq_pairprep_kernel(PairPrepArgs a) {
+    const int lane = threadIdx.x & 63;
+    const int64_t g = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (g >= *a.npairs) return;
+    const BatchedView &bv = a.bv;
+    const int pair = a.pairs[g];
+    const int64_t q = pair / a.NP;
+    const int c = (int)a.probes[pair];
+    const float *qr = a.qrot + q * bv.D_pad;
+    const float *cr = bv.crot + (int64_t)c * bv.D_pad;
+    float q2 = 0.0f;
+    for (int s = 0; s < bv.nseg; ++s) {
+        const BSeg bs = bv.segs[s];
+        float amax = 0.0f, sr = 0.0f, sq = 0.0f;
+        for (int d = lane; d < bs.dim; d += 64) {
+            const float r = qr[bs.doff + d] - cr[bs.doff + d];
+            amax = fmaxf(amax, fabsf(r));
+            sr += r;
+            sq += r * r;
+        }
+        q2 += wave_sum(sq);
+        if (bs.qs < 0) continue;
+        amax = wave_max(amax);
+        sr = wave_sum(sr);
+        const float s1 = amax / 127.0f, inv1 = amax > 0.0f ? 127.0f / amax : 0.0f;
+        const float s2 = s1 / 254.0f, inv2 = amax > 0.0f ? 254.0f * 127.0f / amax : 0.0f;
+        int sum1 = 0, sum2 = 0;
+        int8_t *row1 = a.qq1 + g * bv.kq + bs.kq_off;
+        int8_t *row2 = a.qs == 2 ? a.qq2 + g * bv.kq + bs.kq_off : nullptr;
+        for (int d = lane; d < bs.dim; d += 64) {
+            const float r = qr[bs.doff + d] - cr[bs.doff + d];
+            const int v1 = (int)fminf(127.0f, fmaxf(-127.0f, rintf(r * inv1)));
+            row1[d] = (int8_t)v1;
+            sum1 += v1;
+            if (row2) {
+                const float rem = r - (float)v1 * s1;
+                const int v2 = (int)fminf(127.0f, fmaxf(-127.0f, rintf(rem * inv2)));
+                row2[d] = (int8_t)v2;
+                sum2 += v2;
+            }
+        }
+        sum1 = wave_isum(sum1);
+        sum2 = wave_isum(sum2);
+        if (lane == 0) {
+            float *pc = a.pc + (g * bv.nqs + bs.qs) * 3;
+            pc[0] = bs.sqd * s1;
+            pc[1] = bs.sqd * s2;
+            pc[2] = (-1.0f + bs.sqd / 2) * sr;
+            int32_t *cc = a.pcorr + (g * bv.nqs + bs.qs) * 2;
+            cc[0] = bs.bits >= 8 ? 128 * sum1 : 0;
+            cc[1] = bs.bits >= 8 ? 128 * sum2 : 0;
+        }
+    }
+    if (lane == 0) a.pq2[g] = q2;
+}
+
+// ------------------------------------------------------------------ tile kernel
+// Persistent; every wave takes its own work tiles (one list and up to 16*QTM of its queries)
+// from a shared counter.  Per chunk of 16*NVW vectors and per quantised segment, the B
+// operands of two 64-dim chunks for all NVW mblocks are loaded together, then the MFMAs run;
+// after the segments the chunk's distances go through the wave's LDS to the per-query top-k,
+// which a query only touches when one of the chunk's candidates beats its k-th pair.
+struct BScanArgs {
+    BatchedView bv;
+    const int8_t *qq1, *qq2;
+    const float *pc;
+    const int32_t *pcorr;
+    const float *pq2;
+    const int32_t *pairs;
+    const int32_t *tile_clu, *tile_first, *tile_cnt;
+    const int32_t *ntiles;
+    int32_t *work;  // tile counter, zero at launch
+    int topk;
+    float *cand_d;  // (nq * NP) x topk
+    uint32_t *cand_id;
+};
+
+struct WSmem {
+    float *dt;       // [QT][CVW]
+    uint32_t *dids;  // [CVW]
+    float *qc;       // [QT][nqs][3]
+    int32_t *qcorr;  // [QT][nqs][2]
+    float *q2;       // [QT]
+    float *kd;       // [QT] k-th distance
+    uint32_t *ki;    // [QT] k-th id
+    int32_t *tkn;    // [QT]
+    float *tkd;      // [QT][topk]
+    uint32_t *tki;   // [QT][topk]
+};
+
+template <int QTM>
+__host__ __device__ inline size_t wave_smem(int nqs, int topk, WSmem *m, uint8_t *base) {
+    constexpr int QT = 16 * QTM, CVW = 64 / QTM;
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        uint8_t *p = base ? base + off : nullptr;
+        off += (bytes + 15) / 16 * 16;
+        return p;
+    };
+    WSmem s;
+    s.dt = (float *)take((size_t)QT * CVW * 4);
+    s.dids = (uint32_t *)take(CVW * 4);
+    s.qc = (float *)take((size_t)QT * nqs * 3 * 4);
+    s.qcorr = (int32_t *)take((size_t)QT * nqs * 2 * 4);
+    s.q2 = (float *)take(QT * 4);
+    s.kd = (float *)take(QT * 4);
+    s.ki = (uint32_t *)take(QT * 4);
+    s.tkn = (int32_t *)take(QT * 4);
+    s.tkd = (float *)take((size_t)QT * topk * 4);
+    s.tki = (uint32_t *)take((size_t)QT * topk * 4);
+    if (m) *m = s;
+    return off;
+}
+
+// LDS traffic between the lanes of one wave: keep the compiler from moving accesses across
+// this point (a wave's LDS operations complete in order).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
 __device__ __forceinline__ v4i unpack_nibbles(uint2 x) {
     v4i r;
     r.x = (int)(x.x & 0x0F0F0F0Fu);
@@ -411,100 +502,65 @@ __device__ __forceinline__ v4i unpack_nibbles(uint2 x) {
     return r;
 }
 
+__device__ __forceinline__ v4i mfma(v4i a, v4i b, v4i c) {
+    return __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, c, 0, 0, 0);
+}
+
+constexpr int kKB = 2;  // 64-dim chunks whose operands are loaded together
+
 template <int QTM, int QS, int KR>
 __global__ __launch_bounds__(kBThreads) void bscan_kernel(BScanArgs a) {
     constexpr int QT = 16 * QTM;
-    constexpr int NVW = 4 / QTM;          // mblocks per wave per chunk
-    constexpr int CV = 64 * kBWaves / QTM;  // vectors per chunk
+    constexpr int NVW = 4 / QTM;   // mblocks per chunk
+    constexpr int CVW = 16 * NVW;  // vectors per chunk
     extern __shared__ __align__(16) uint8_t smem_raw[];
-    BSmem m;
     const BatchedView &bv = a.bv;
-    bscan_smem<QTM, QS>(bv, a.topk, &m, smem_raw);
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const size_t per_wave = wave_smem<QTM>(bv.nqs, a.topk, nullptr, nullptr);
+    WSmem m;
+    wave_smem<QTM>(bv.nqs, a.topk, &m, smem_raw + w * per_wave);
     const int ntiles = *a.ntiles;
     for (;;) {
-        if (tid == 0) {
-            const int t = atomicAdd(a.work, 1);
-            m.ctl[0] = t;
-            if (t < ntiles) {
-                m.ctl[1] = a.tile_clu[t];
-                m.ctl[2] = a.tile_first[t];
-                m.ctl[3] = a.tile_cnt[t];
-            }
-        }
-        __syncthreads();
-        const int tile = m.ctl[0];
+        int tile = 0;
+        if (lane == 0) tile = atomicAdd(a.work, 1);
+        tile = __shfl(tile, 0);
         if (tile >= ntiles) break;
-        const int c = m.ctl[1], first = m.ctl[2], cnt = m.ctl[3];
-        __syncthreads();  // ctl read by every thread before the next tile overwrites it
-
-        // ---- quantise r = q_rot - c_rot of every query of the tile, per segment
-        const float *cr = bv.crot + (int64_t)c * bv.D_pad;
-        for (int i = w; i < QT; i += kBWaves) {
-            const int q = i < cnt ? a.pairs[first + i] / a.NP : -1;
-            const float *qr = q >= 0 ? a.qrot + (int64_t)q * bv.D_pad : nullptr;
-            float q2 = 0.0f;
-            for (int s = 0; s < bv.nseg; ++s) {
-                const BSeg bs = bv.segs[s];
-                float amax = 0.0f, sr = 0.0f, sq = 0.0f;
-                if (qr)
-                    for (int d = lane; d < bs.dim; d += 64) {
-                        const float r = qr[bs.doff + d] - cr[bs.doff + d];
-                        amax = fmaxf(amax, fabsf(r));
-                        sr += r;
-                        sq += r * r;
-                    }
-                q2 += wave_sum(sq);
-                if (bs.qs < 0) continue;
-                amax = wave_max(amax);
-                sr = wave_sum(sr);
-                const float s1 = amax / 127.0f, inv1 = amax > 0.0f ? 127.0f / amax : 0.0f;
-                const float s2 = s1 / 254.0f, inv2 = amax > 0.0f ? 254.0f * 127.0f / amax : 0.0f;
-                int sum1 = 0, sum2 = 0;
-                int8_t *row1 = m.a1 + i * bv.kq_stride + bs.kq_off;
-                int8_t *row2 = m.a2 + i * bv.kq_stride + bs.kq_off;
-                for (int d = lane; d < bs.dim; d += 64) {
-                    const float r = qr ? qr[bs.doff + d] - cr[bs.doff + d] : 0.0f;
-                    const int v1 = (int)fminf(127.0f, fmaxf(-127.0f, rintf(r * inv1)));
-                    row1[d] = (int8_t)v1;
-                    sum1 += v1;
-                    if (QS == 2) {
-                        const float rem = r - (float)v1 * s1;
-                        const int v2 = (int)fminf(127.0f, fmaxf(-127.0f, rintf(rem * inv2)));
-                        row2[d] = (int8_t)v2;
-                        sum2 += v2;
-                    }
-                }
-                sum1 = wave_isum(sum1);
-                if (QS == 2) sum2 = wave_isum(sum2);
-                if (lane == 0) {
-                    float *qc = m.qc + (i * bv.nqs + bs.qs) * 3;
-                    qc[0] = bs.sqd * s1;
-                    qc[1] = bs.sqd * s2;
-                    qc[2] = (-1.0f + bs.sqd / 2) * sr;
-                    int32_t *cc = m.qcorr + (i * bv.nqs + bs.qs) * 2;
-                    cc[0] = bs.bits >= 8 ? 128 * sum1 : 0;
-                    cc[1] = bs.bits >= 8 ? 128 * sum2 : 0;
-                }
-            }
-            if (lane == 0) {
-                m.q2[i] = q >= 0 ? q2 : INFINITY;
-                m.tkn[i] = 0;
-            }
+        const int c = a.tile_clu[tile], first = a.tile_first[tile], cnt = a.tile_cnt[tile];
+        // ---- the tile's pair constants to LDS; rows past cnt repeat the last pair
+        for (int e = lane; e < QT * bv.nqs; e += 64) {
+            const int row = e / bv.nqs, s = e % bv.nqs;
+            const int64_t g = first + min(row, cnt - 1);
+            const float *pc = a.pc + (g * bv.nqs + s) * 3;
+            m.qc[e * 3 + 0] = pc[0];
+            m.qc[e * 3 + 1] = pc[1];
+            m.qc[e * 3 + 2] = pc[2];
+            m.qcorr[e * 2 + 0] = a.pcorr[(g * bv.nqs + s) * 2];
+            m.qcorr[e * 2 + 1] = a.pcorr[(g * bv.nqs + s) * 2 + 1];
         }
-        __syncthreads();
-
-        // ---- stream the list in chunks of CV vectors
+        for (int row = lane; row < QT; row += 64) {
+            m.q2[row] = a.pq2[first + min(row, cnt - 1)];
+            m.kd[row] = INFINITY;
+            m.ki[row] = 0xFFFFFFFFu;
+            m.tkn[row] = 0;
+        }
+        wave_sync();
+        // A operand rows of this lane: query row t*16 + lane%16 of the tile
+        const int8_t *arow1[QTM], *arow2[QTM];
+#pragma unroll
+        for (int t = 0; t < QTM; ++t) {
+            const int64_t g = first + min(t * 16 + lane % 16, cnt - 1);
+            arow1[t] = a.qq1 + g * bv.kq + 16 * (lane / 16);
+            arow2[t] = QS == 2 ? a.qq2 + g * bv.kq + 16 * (lane / 16) : nullptr;
+        }
         const int n = bv.clu_n[c];
         const int nmb = (n + kMB - 1) / kMB;
         const int64_t mb0 = bv.clu_mb[c];
-        for (int cb = 0; cb < nmb; cb += kBWaves * NVW) {
-            float dist[QTM][NVW][4];
+        for (int cb = 0; cb < nmb; cb += NVW) {
             const uint8_t *rec[NVW];
+            float dist[QTM][NVW][4];
 #pragma unroll
             for (int j = 0; j < NVW; ++j) {
-                const int mbj = min(cb + w * NVW + j, nmb - 1);  // clamped: a duplicate record, discarded below
-                rec[j] = bv.mb + (mb0 + mbj) * bv.mb_stride;
+                rec[j] = bv.mb + (mb0 + min(cb + j, nmb - 1)) * bv.mb_stride;  // clamped duplicates are discarded
                 const float o2 = reinterpret_cast<const float *>(rec[j] + bv.fac_off)[lane % kMB];
 #pragma unroll
                 for (int t = 0; t < QTM; ++t)
@@ -518,57 +574,42 @@ __global__ __launch_bounds__(kBThreads) void bscan_kernel(BScanArgs a) {
 #pragma unroll
                 for (int t = 0; t < QTM; ++t)
 #pragma unroll
-                    for (int j = 0; j < NVW; ++j) {
-                        acc1[t][j] = v4i{0, 0, 0, 0};
-                        acc2[t][j] = v4i{0, 0, 0, 0};
-                        ach1[t][j] = v4i{0, 0, 0, 0};
-                        ach2[t][j] = v4i{0, 0, 0, 0};
-                    }
+                    for (int j = 0; j < NVW; ++j) acc1[t][j] = acc2[t][j] = ach1[t][j] = ach2[t][j] = v4i{0, 0, 0, 0};
                 const int nk = bs.dim / 64;
-                const int8_t *arow1 = m.a1 + (lane % 16) * bv.kq_stride + bs.kq_off + 16 * (lane / 16);
-                const int8_t *arow2 = m.a2 + (lane % 16) * bv.kq_stride + bs.kq_off + 16 * (lane / 16);
-                for (int kc = 0; kc < nk; ++kc) {
-                    v4i A1[QTM], A2[QTM];
+                const int kind = bs.bits <= 4 ? 0 : (bs.bits <= 8 ? 1 : 2);
+                for (int kc0 = 0; kc0 < nk; kc0 += kKB) {
+                    v4i A1[kKB][QTM], A2[kKB][QTM], B[kKB][NVW], BH[kKB][NVW];
 #pragma unroll
-                    for (int t = 0; t < QTM; ++t) {
-                        A1[t] = *reinterpret_cast<const v4i *>(arow1 + t * 16 * bv.kq_stride + kc * 64);
-                        if (QS == 2) A2[t] = *reinterpret_cast<const v4i *>(arow2 + t * 16 * bv.kq_stride + kc * 64);
-                    }
-                    if (bs.bits <= 4) {
-                        v4i B[NVW];
+                    for (int u = 0; u < kKB; ++u) {
+                        const int kc = min(kc0 + u, nk - 1);  // a repeated chunk past nk is not accumulated
 #pragma unroll
-                        for (int j = 0; j < NVW; ++j)
-                            B[j] = unpack_nibbles(*reinterpret_cast<const uint2 *>(
-                                rec[j] + bs.code_off + (int64_t)kc * 512 + lane * 8));
-#pragma unroll
-                        for (int t = 0; t < QTM; ++t)
-#pragma unroll
-                            for (int j = 0; j < NVW; ++j) {
-                                acc1[t][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A1[t], B[j], acc1[t][j], 0, 0, 0);
-                                if (QS == 2)
-                                    acc2[t][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A2[t], B[j], acc2[t][j], 0, 0, 0);
-                            }
-                    } else {
-                        v4i B[NVW], BH[NVW];
-                        const bool two = bs.bits > 8;
-#pragma unroll
-                        for (int j = 0; j < NVW; ++j) {
-                            const uint8_t *p = rec[j] + bs.code_off + (int64_t)kc * bs.chunk_bytes + lane * 16;
-                            B[j] = *reinterpret_cast<const v4i *>(p);
-                            if (two) BH[j] = *reinterpret_cast<const v4i *>(p + 1024);
+                        for (int t = 0; t < QTM; ++t) {
+                            A1[u][t] = *reinterpret_cast<const v4i *>(arow1[t] + bs.kq_off + kc * 64);
+                            if (QS == 2) A2[u][t] = *reinterpret_cast<const v4i *>(arow2[t] + bs.kq_off + kc * 64);
                         }
 #pragma unroll
+                        for (int j = 0; j < NVW; ++j) {
+                            const uint8_t *p = rec[j] + bs.code_off + (int64_t)kc * bs.chunk_bytes;
+                            if (kind == 0) {
+                                B[u][j] = unpack_nibbles(*reinterpret_cast<const uint2 *>(p + lane * 8));
+                            } else {
+                                B[u][j] = *reinterpret_cast<const v4i *>(p + lane * 16);
+                                if (kind == 2) BH[u][j] = *reinterpret_cast<const v4i *>(p + 1024 + lane * 16);
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < kKB; ++u) {
+                        if (kc0 + u >= nk) break;
+#pragma unroll
                         for (int t = 0; t < QTM; ++t)
 #pragma unroll
                             for (int j = 0; j < NVW; ++j) {
-                                acc1[t][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A1[t], B[j], acc1[t][j], 0, 0, 0);
-                                if (QS == 2)
-                                    acc2[t][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A2[t], B[j], acc2[t][j], 0, 0, 0);
-                                if (two) {
-                                    ach1[t][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A1[t], BH[j], ach1[t][j], 0, 0, 0);
-                                    if (QS == 2)
-                                        ach2[t][j] =
-                                            __builtin_amdgcn_mfma_i32_16x16x64_i8(A2[t], BH[j], ach2[t][j], 0, 0, 0);
+                                acc1[t][j] = mfma(A1[u][t], B[u][j], acc1[t][j]);
+                                if (QS == 2) acc2[t][j] = mfma(A2[u][t], B[u][j], acc2[t][j]);
+                                if (kind == 2) {
+                                    ach1[t][j] = mfma(A1[u][t], BH[u][j], ach1[t][j]);
+                                    if (QS == 2) ach2[t][j] = mfma(A2[u][t], BH[u][j], ach2[t][j]);
                                 }
                             }
                     }
@@ -596,33 +637,42 @@ __global__ __launch_bounds__(kBThreads) void bscan_kernel(BScanArgs a) {
                         }
                 }
             }
-            // ---- distances of the chunk to LDS, then per-query selection
+            // ---- the chunk's distances to LDS, then the per-query lists
 #pragma unroll
             for (int j = 0; j < NVW; ++j) {
-                const int lv = (w * NVW + j) * kMB + lane % kMB;  // vector of the chunk
-                const bool valid = cb + w * NVW + j < nmb;
+                const int lv = j * kMB + lane % kMB;
+                const bool valid = cb + j < nmb;
 #pragma unroll
                 for (int t = 0; t < QTM; ++t)
 #pragma unroll
                     for (int r = 0; r < 4; ++r)
-                        m.dt[(t * 16 + 4 * (lane / 16) + r) * CV + lv] = valid ? dist[t][j][r] : INFINITY;
+                        m.dt[(t * 16 + 4 * (lane / 16) + r) * CVW + lv] = valid ? dist[t][j][r] : INFINITY;
                 if (lane < kMB)
                     m.dids[lv] = valid ? reinterpret_cast<const uint32_t *>(rec[j] + bv.fac_off)[kMB * (1 + bv.nqs) + lane]
                                        : 0xFFFFFFFFu;
             }
-            __syncthreads();
-            for (int i = w; i < cnt; i += kBWaves) {
+            wave_sync();
+            const float x_id_pad = INFINITY;
+            for (int i = 0; i < cnt; ++i) {
+                const float x = lane < CVW ? m.dt[i * CVW + lane] : x_id_pad;
+                const uint32_t xi = lane < CVW ? m.dids[lane] : 0xFFFFFFFFu;
+                if (!__ballot(dless(x, xi, m.kd[i], m.ki[i]))) continue;
                 WaveTopK<KR> tk;
                 tk.load(m.tkd + i * a.topk, m.tki + i * a.topk, m.tkn[i], a.topk);
-#pragma unroll
-                for (int u = 0; u < CV / 64; ++u) tk.offer(m.dt[i * CV + u * 64 + lane], m.dids[u * 64 + lane]);
+                tk.offer(x, xi);
+                wave_sync();
                 tk.store(m.tkd + i * a.topk, m.tki + i * a.topk);
-                if (lane == 0) m.tkn[i] = tk.n;
+                if (lane == 0) {
+                    m.tkn[i] = tk.n;
+                    m.kd[i] = tk.kd;
+                    m.ki[i] = tk.kid;
+                }
+                wave_sync();
             }
-            __syncthreads();
+            wave_sync();  // the next chunk rewrites dt and dids
         }
         // ---- the tile's per-query lists to global memory
-        for (int i = w; i < cnt; i += kBWaves) {
+        for (int i = 0; i < cnt; ++i) {
             const int64_t pr = a.pairs[first + i];
             const int nn = m.tkn[i];
             for (int e = lane; e < a.topk; e += 64) {
@@ -630,7 +680,7 @@ __global__ __launch_bounds__(kBThreads) void bscan_kernel(BScanArgs a) {
                 a.cand_id[pr * a.topk + e] = e < nn ? m.tki[i * a.topk + e] : 0xFFFFFFFFu;
             }
         }
-        __syncthreads();
+        wave_sync();
     }
 }
 
@@ -695,7 +745,8 @@ hipError_t launch_bq_group(const uint32_t *probes, const int32_t *npq, int64_t n
     const int64_t n = nq * NP;
     const unsigned g = (unsigned)((n + 255) / 256);
     hipLaunchKernelGGL(bq_count_kernel, dim3(g), dim3(256), 0, st, probes, npq, nq, NP, w.counts, w.slot);
-    hipLaunchKernelGGL(bq_scan_kernel, dim3(1), dim3(1024), 0, st, w.counts, K, QT, w.cstart, w.tstart, w.ntiles);
+    hipLaunchKernelGGL(bq_scan_kernel, dim3(1), dim3(1024), 0, st, w.counts, K, QT, w.cstart, w.tstart, w.ntiles,
+                       w.npairs);
     hipLaunchKernelGGL(bq_scatter_kernel, dim3(g), dim3(256), 0, st, probes, w.slot, n, w.cstart, w.pairs);
     hipLaunchKernelGGL(bq_tiles_kernel, dim3((K + 255) / 256), dim3(256), 0, st, w.counts, w.cstart, w.tstart, K, QT,
                        w.tile_clu, w.tile_first, w.tile_cnt);
@@ -704,7 +755,7 @@ hipError_t launch_bq_group(const uint32_t *probes, const int32_t *npq, int64_t n
 
 template <int QTM, int QS, int KR>
 static hipError_t launch_bscan_t(const BScanArgs &a, int grid, hipStream_t st) {
-    const size_t smem = bscan_smem<QTM, QS>(a.bv, a.topk, nullptr, nullptr);
+    const size_t smem = kBWaves * wave_smem<QTM>(a.bv.nqs, a.topk, nullptr, nullptr);
     auto kern = bscan_kernel<QTM, QS, KR>;
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
@@ -721,9 +772,10 @@ static hipError_t launch_bscan_t(const BScanArgs &a, int grid, hipStream_t st) {
 }
 
 size_t bscan_smem_bytes(const BatchedView &bv, int topk, int qtm, int qs) {
-    if (qtm == 1) return qs == 2 ? bscan_smem<1, 2>(bv, topk, nullptr, nullptr) : bscan_smem<1, 1>(bv, topk, nullptr, nullptr);
-    if (qtm == 2) return qs == 2 ? bscan_smem<2, 2>(bv, topk, nullptr, nullptr) : bscan_smem<2, 1>(bv, topk, nullptr, nullptr);
-    return qs == 2 ? bscan_smem<4, 2>(bv, topk, nullptr, nullptr) : bscan_smem<4, 1>(bv, topk, nullptr, nullptr);
+    (void)qs;
+    if (qtm == 1) return kBWaves * wave_smem<1>(bv.nqs, topk, nullptr, nullptr);
+    if (qtm == 2) return kBWaves * wave_smem<2>(bv.nqs, topk, nullptr, nullptr);
+    return kBWaves * wave_smem<4>(bv.nqs, topk, nullptr, nullptr);
 }
 
 template <int QTM, int QS>
@@ -733,19 +785,40 @@ static hipError_t launch_bscan_q(const BScanArgs &a, int grid, hipStream_t st) {
     return launch_bscan_t<QTM, QS, 4>(a, grid, st);
 }
 
-hipError_t launch_bscan_tiles(const BatchedView &bv, const float *qrot, const int32_t *pairs, const BqWork &w,
-                              int NP, int topk, float *cand_d, uint32_t *cand_id, int qtm, int qs, int grid,
-                              hipStream_t st) {
-    BScanArgs a;
+hipError_t launch_bq_pairprep(const BatchedView &bv, const float *qrot, const uint32_t *probes, const BqWork &w,
+                              const BqPairBufs &pb, int NP, int64_t max_pairs, int qs, hipStream_t st) {
+    PairPrepArgs a;
     a.bv = bv;
     a.qrot = qrot;
-    a.pairs = pairs;
+    a.probes = probes;
+    a.pairs = w.pairs;
+    a.npairs = w.npairs;
+    a.NP = NP;
+    a.qs = qs;
+    a.qq1 = pb.qq1;
+    a.qq2 = pb.qq2;
+    a.pc = pb.pc;
+    a.pcorr = pb.pcorr;
+    a.pq2 = pb.pq2;
+    hipLaunchKernelGGL(bq_pairprep_kernel, dim3((unsigned)((max_pairs + 3) / 4)), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_bscan_tiles(const BatchedView &bv, const BqWork &w, const BqPairBufs &pb, int topk,
+                              float *cand_d, uint32_t *cand_id, int qtm, int qs, int grid, hipStream_t st) {
+    BScanArgs a;
+    a.bv = bv;
+    a.qq1 = pb.qq1;
+    a.qq2 = pb.qq2;
+    a.pc = pb.pc;
+    a.pcorr = pb.pcorr;
+    a.pq2 = pb.pq2;
+    a.pairs = w.pairs;
     a.tile_clu = w.tile_clu;
     a.tile_first = w.tile_first;
     a.tile_cnt = w.tile_cnt;
     a.ntiles = w.ntiles;
     a.work = w.work;
-    a.NP = NP;
     a.topk = topk;
     a.cand_d = cand_d;
     a.cand_id = cand_id;

@@ -29,8 +29,17 @@ struct BatchedView {
     const BSeg *segs;
 };
 
+// Grouping workspace: (query, list) pairs grouped by list, and the work tiles over them.
 struct BqWork {
-    int32_t *counts, *slot, *cstart, *tstart, *ntiles, *pairs, *tile_clu, *tile_first, *tile_cnt, *work;
+    int32_t *counts, *slot, *cstart, *tstart, *ntiles, *npairs, *pairs, *tile_clu, *tile_first, *tile_cnt, *work;
+};
+
+// Per grouped pair: the quantised residual rows (A operand) and the pair's constants.
+struct BqPairBufs {
+    int8_t *qq1, *qq2;  // [pairs][kq]
+    float *pc;          // [pairs][nqs][3]
+    int32_t *pcorr;     // [pairs][nqs][2]
+    float *pq2;         // [pairs]
 };
 
 size_t bscan_smem_bytes(const BatchedView &bv, int topk, int qtm, int qs);
@@ -42,10 +51,13 @@ hipError_t launch_bq_group(const uint32_t *probes, const int32_t *npq, int64_t n
 hipError_t launch_bq_merge(const float *cand_d, const uint32_t *cand_id, const int32_t *npq, int64_t nq, int NP,
                            int topk, uint32_t *out_ids, float *out_d, hipStream_t st);
 
+// Quantised residual rows and constants of every grouped pair (max_pairs bounds the grid).
+hipError_t launch_bq_pairprep(const BatchedView &bv, const float *qrot, const uint32_t *probes, const BqWork &w,
+                              const BqPairBufs &pb, int NP, int64_t max_pairs, int qs, hipStream_t st);
+
 // One launch of the tile kernel: qtm = query tiles of 16 per work tile (1, 2, 4), qs = terms
 // of the quantised query (1 = int8, 2 = int8x2).  grid is an upper bound on the workgroups.
-hipError_t launch_bscan_tiles(const BatchedView &bv, const float *qrot, const int32_t *pairs, const BqWork &w,
-                              int NP, int topk, float *cand_d, uint32_t *cand_id, int qtm, int qs, int grid,
-                              hipStream_t st);
+hipError_t launch_bscan_tiles(const BatchedView &bv, const BqWork &w, const BqPairBufs &pb, int topk,
+                              float *cand_d, uint32_t *cand_id, int qtm, int qs, int grid, hipStream_t st);
 
 }  // namespace saqgpu

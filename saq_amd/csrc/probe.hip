@@ -86,7 +86,32 @@ __global__ __launch_bounds__(256) void probe_dist_kernel(const float *__restrict
 // then follow Eigen's order across lanes.  A workgroup covers 32 queries x 32 centroids.
 constexpr int kPT = 32;     // queries (and centroids) per workgroup tile
 constexpr int kPCh = 128;   // dims per LDS chunk
-template <int DT>
+
+// One step of the 8 x 16 register tile: element i of 8 query rows and 16 centroid rows.
+template <int DT, bool kInit>
+__device__ __forceinline__ void probe_step(float (&acc)[8][16], const float (*qs)[kPCh], const float (*cs)[kPCh],
+                                           int qa, int cb, int i) {
+    float qv[8], cv[16];
+#pragma unroll
+    for (int a = 0; a < 8; ++a) qv[a] = qs[qa + a][i];
+#pragma unroll
+    for (int b = 0; b < 16; ++b) cv[b] = cs[cb + b][i];
+#pragma unroll
+    for (int a = 0; a < 8; ++a)
+#pragma unroll
+        for (int b = 0; b < 16; ++b) {
+            float e;
+            if (DT == 1) {
+                e = qv[a] * cv[b];
+            } else {
+                const float d = cv[b] - qv[a];
+                e = d * d;
+            }
+            acc[a][b] = kInit ? e : acc[a][b] + e;
+        }
+}
+
+template <int DT, bool kVec>
 __global__ __launch_bounds__(256) void probe_tile_kernel(const float *__restrict__ cent, int K, int D,
                                                          const float *__restrict__ queries, int64_t nq,
                                                          float *__restrict__ out) {
@@ -99,41 +124,43 @@ __global__ __launch_bounds__(256) void probe_tile_kernel(const float *__restrict
     const int cb = (hw & 1) * 16;   // and 16 centroids
     const int aligned = D / 16 * 16, aligned2 = D / 32 * 32;
     float acc[8][16];
+    // chunk staging: thread tid moves float4 number tid + 256u (u < 4) of each 32 x 128 tile;
+    // rows past nq or K repeat the last valid row (their results are not stored)
+    float4 pq[4], pc[4];
+    auto fetch = [&](int base) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int e4 = tid + 256 * u, r = e4 >> 5, i = base + 4 * (e4 & 31);
+            const int64_t q = min<int64_t>(q0 + r, nq - 1);
+            const int c = min(c0 + r, K - 1);
+            if (kVec) {
+                pq[u] = i < aligned2 ? *reinterpret_cast<const float4 *>(queries + q * D + i) : float4{};
+                pc[u] = i < aligned2 ? *reinterpret_cast<const float4 *>(cent + (int64_t)c * D + i) : float4{};
+            } else {
+                const float *qr = queries + q * D, *cr = cent + (int64_t)c * D;
+                pq[u] = i < aligned2 ? float4{qr[i], qr[i + 1], qr[i + 2], qr[i + 3]} : float4{};
+                pc[u] = i < aligned2 ? float4{cr[i], cr[i + 1], cr[i + 2], cr[i + 3]} : float4{};
+            }
+        }
+    };
+    fetch(0);
     for (int base = 0; base < aligned2; base += kPCh) {
         const int len = min(kPCh, aligned2 - base);
         __syncthreads();
-        for (int e = tid; e < kPT * kPCh; e += 256) {
-            const int r = e / kPCh, i = e % kPCh;
-            if (i < len) {
-                const int64_t q = q0 + r;
-                const int c = c0 + r;
-                qs[r][i] = q < nq ? queries[q * D + base + i] : 0.0f;
-                cs[r][i] = c < K ? cent[(int64_t)c * D + base + i] : 0.0f;
-            }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int e4 = tid + 256 * u, r = e4 >> 5, c4 = e4 & 31;
+            *reinterpret_cast<float4 *>(&qs[r][4 * c4]) = pq[u];
+            *reinterpret_cast<float4 *>(&cs[r][4 * c4]) = pc[u];
         }
         __syncthreads();
-        for (int t = 0; t < len / 32; ++t) {
-            const int i = 32 * t + j;
-            float qv[8], cv[16];
-#pragma unroll
-            for (int a = 0; a < 8; ++a) qv[a] = qs[qa + a][i];
-#pragma unroll
-            for (int b = 0; b < 16; ++b) cv[b] = cs[cb + b][i];
-            const bool first = base == 0 && t == 0;
-#pragma unroll
-            for (int a = 0; a < 8; ++a)
-#pragma unroll
-                for (int b = 0; b < 16; ++b) {
-                    float e;
-                    if (DT == 1) {
-                        e = qv[a] * cv[b];
-                    } else {
-                        const float d = cv[b] - qv[a];
-                        e = d * d;
-                    }
-                    acc[a][b] = first ? e : acc[a][b] + e;
-                }
+        if (base + kPCh < aligned2) fetch(base + kPCh);  // in flight while this chunk is reduced
+        int t = 0;
+        if (base == 0) {
+            probe_step<DT, true>(acc, qs, cs, qa, cb, j);
+            t = 1;
         }
+        for (; t < len / 32; ++t) probe_step<DT, false>(acc, qs, cs, qa, cb, 32 * t + j);
     }
     // per pair: combine, trailing packet, halving tree, tail (lane 0 ends with the result)
 #pragma unroll
@@ -252,23 +279,33 @@ __global__ __launch_bounds__(256) void probe_select_kernel(const float *__restri
     for (int i = tid; i < nprobe && i < n; i += 256) probes[q * nprobe + i] = (uint32_t)(sel[i] & 0xFFFFFFFFu);
 }
 
+template <int DT>
+static void probe_tile(bool vec, dim3 g, hipStream_t st, const float *centroids, int K, int D, const float *queries,
+                       int64_t nq, float *dist_ws) {
+    if (vec)
+        hipLaunchKernelGGL((probe_tile_kernel<DT, true>), g, dim3(256), 0, st, centroids, K, D, queries, nq, dist_ws);
+    else
+        hipLaunchKernelGGL((probe_tile_kernel<DT, false>), g, dim3(256), 0, st, centroids, K, D, queries, nq, dist_ws);
+}
+
 hipError_t launch_probe(const float *centroids, int K, int D, const float *queries, int64_t nq, int nprobe,
                         int dist_type, float *dist_ws, uint32_t *probes, hipStream_t st) {
     if (nprobe > kSelMax) return hipErrorInvalidValue;
     const bool tiled = D >= 32 && (nq + kPT - 1) / kPT <= 65535;
+    const bool vec = D % 4 == 0 && ((uintptr_t)centroids % 16) == 0 && ((uintptr_t)queries % 16) == 0;
     dim3 g1((unsigned)((K + 255) / 256), (unsigned)nq);
     dim3 gt((unsigned)((K + kPT - 1) / kPT), (unsigned)((nq + kPT - 1) / kPT));
     const size_t sm = (size_t)D * 4;
     if (!tiled && nq > 65535) return hipErrorInvalidValue;
     if (dist_type == 1) {
         if (tiled)
-            hipLaunchKernelGGL(probe_tile_kernel<1>, gt, dim3(256), 0, st, centroids, K, D, queries, nq, dist_ws);
+            probe_tile<1>(vec, gt, st, centroids, K, D, queries, nq, dist_ws);
         else
             hipLaunchKernelGGL(probe_dist_kernel<1>, g1, dim3(256), sm, st, centroids, K, D, queries, dist_ws);
         hipLaunchKernelGGL(probe_select_kernel<1>, dim3((unsigned)nq), dim3(256), 0, st, dist_ws, K, nprobe, probes);
     } else {
         if (tiled)
-            hipLaunchKernelGGL(probe_tile_kernel<0>, gt, dim3(256), 0, st, centroids, K, D, queries, nq, dist_ws);
+            probe_tile<0>(vec, gt, st, centroids, K, D, queries, nq, dist_ws);
         else
             hipLaunchKernelGGL(probe_dist_kernel<0>, g1, dim3(256), sm, st, centroids, K, D, queries, dist_ws);
         hipLaunchKernelGGL(probe_select_kernel<0>, dim3((unsigned)nq), dim3(256), 0, st, dist_ws, K, nprobe, probes);
