@@ -112,6 +112,7 @@ struct saqgpu_index {
     std::vector<int64_t> clu_mb;
     DevBuf<uint8_t> bq_mb;
     DevBuf<int64_t> d_clu_mb;
+    DevBuf<int32_t> d_bq_order;  // lists by decreasing size
     DevBuf<BSeg> d_bsegs;
     DevBuf<int32_t> bq_ws, bq_npq, bq_pcorr;
     DevBuf<int8_t> bq_qq;
@@ -336,6 +337,7 @@ void saqgpu_index_destroy(saqgpu_index *ix) {
     ix->ws_met.release();
     ix->bq_mb.release();
     ix->d_clu_mb.release();
+    ix->d_bq_order.release();
     ix->d_bsegs.release();
     ix->bq_ws.release();
     ix->bq_pcorr.release();
@@ -883,7 +885,12 @@ int prepare_batched_locked(saqgpu_index *ix) {
     ix->bq_total_mb = tot;
     if (ix->bq_mb.ensure(std::max<int64_t>(1, tot) * ix->bq_stride)) return fail(SAQGPU_ENOMEM, "allocation failed");
     int rc;
-    if ((rc = upload(ix->d_clu_mb, ix->clu_mb)) || (rc = upload(ix->d_bsegs, ix->bsegs))) return rc;
+    std::vector<int32_t> order(ix->num_cen);
+    for (int c = 0; c < ix->num_cen; ++c) order[c] = c;
+    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return ix->clu_n[a] > ix->clu_n[b]; });
+    if ((rc = upload(ix->d_clu_mb, ix->clu_mb)) || (rc = upload(ix->d_bsegs, ix->bsegs)) ||
+        (rc = upload(ix->d_bq_order, order)))
+        return rc;
     DevBuf<int32_t> d_mb_clu, d_cs, d_ck;
     if ((rc = upload(d_mb_clu, mb_clu)) || (rc = upload(d_cs, chunk_seg)) || (rc = upload(d_ck, chunk_kc))) return rc;
     HIPCHK(launch_bq_build(ix->view(), batched_view(ix), d_mb_clu.p, tot, d_cs.p, d_ck.p, (int)chunk_seg.size(),
@@ -994,7 +1001,7 @@ int search_batched_locked(saqgpu_index *ix, const float *queries, int64_t nq, co
     pb.pc = ix->bq_pc.p;
     pb.pq2 = ix->bq_pc.p + npairs * 3 * ix->bq_nqs;
     pb.pcorr = ix->bq_pcorr.p;
-    HIPCHK(launch_bq_group(dp, dnpq, nq, NP, K, query_tile, w, st));
+    HIPCHK(launch_bq_group(dp, dnpq, nq, NP, K, query_tile, ix->d_bq_order.p, w, st));
     HIPCHK(launch_bq_pairprep(bv, ix->ws_qrot.p, dp, w, pb, NP, npairs, qs, st));
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (time_it) {

@@ -180,17 +180,21 @@ __global__ void bq_count_kernel(const uint32_t *probes, const int32_t *npq, int6
     slot[i] = atomicAdd(&counts[probes[i]], 1);
 }
 
-// One workgroup: exclusive prefix sums of the pair counts and of the tile counts per list.
-__global__ __launch_bounds__(1024) void bq_scan_kernel(const int32_t *counts, int K, int QT, int32_t *cstart,
-                                                       int32_t *tstart, int32_t *ntiles, int32_t *npairs) {
+// One workgroup: exclusive prefix sums of the pair counts and of the tile counts per list,
+// taken over the lists in the given order (largest lists first, so the longest tiles start
+// first and the persistent kernel's tail stays short).
+__global__ __launch_bounds__(1024) void bq_scan_kernel(const int32_t *counts, const int32_t *order, int K, int QT,
+                                                       int32_t *cstart, int32_t *tstart, int32_t *ntiles,
+                                                       int32_t *npairs) {
     __shared__ int32_t part_c[1024], part_t[1024];
     const int t = threadIdx.x;
     const int per = (K + 1023) / 1024;
     const int b = t * per, e = min(K, b + per);
     int32_t sc = 0, st = 0;
     for (int i = b; i < e; ++i) {
-        sc += counts[i];
-        st += (counts[i] + QT - 1) / QT;
+        const int c = order[i];
+        sc += counts[c];
+        st += (counts[c] + QT - 1) / QT;
     }
     part_c[t] = sc;
     part_t[t] = st;
@@ -204,10 +208,11 @@ __global__ __launch_bounds__(1024) void bq_scan_kernel(const int32_t *counts, in
     }
     int32_t rc = part_c[t] - sc, rt = part_t[t] - st;
     for (int i = b; i < e; ++i) {
-        cstart[i] = rc;
-        tstart[i] = rt;
-        rc += counts[i];
-        rt += (counts[i] + QT - 1) / QT;
+        const int c = order[i];
+        cstart[c] = rc;
+        tstart[c] = rt;
+        rc += counts[c];
+        rt += (counts[c] + QT - 1) / QT;
     }
     if (t == 1023) {
         *ntiles = part_t[1023];
@@ -739,14 +744,14 @@ hipError_t launch_bq_build(const IndexView &ix, const BatchedView &bv, const int
 }
 
 hipError_t launch_bq_group(const uint32_t *probes, const int32_t *npq, int64_t nq, int NP, int K, int QT,
-                           const BqWork &w, hipStream_t st) {
+                           const int32_t *order, const BqWork &w, hipStream_t st) {
     hipLaunchKernelGGL(bq_fill_kernel, dim3(64), dim3(256), 0, st, w.counts, (int64_t)K, 0);
     hipLaunchKernelGGL(bq_fill_kernel, dim3(1), dim3(64), 0, st, w.work, (int64_t)1, 0);
     const int64_t n = nq * NP;
     const unsigned g = (unsigned)((n + 255) / 256);
     hipLaunchKernelGGL(bq_count_kernel, dim3(g), dim3(256), 0, st, probes, npq, nq, NP, w.counts, w.slot);
-    hipLaunchKernelGGL(bq_scan_kernel, dim3(1), dim3(1024), 0, st, w.counts, K, QT, w.cstart, w.tstart, w.ntiles,
-                       w.npairs);
+    hipLaunchKernelGGL(bq_scan_kernel, dim3(1), dim3(1024), 0, st, w.counts, order, K, QT, w.cstart, w.tstart,
+                       w.ntiles, w.npairs);
     hipLaunchKernelGGL(bq_scatter_kernel, dim3(g), dim3(256), 0, st, probes, w.slot, n, w.cstart, w.pairs);
     hipLaunchKernelGGL(bq_tiles_kernel, dim3((K + 255) / 256), dim3(256), 0, st, w.counts, w.cstart, w.tstart, K, QT,
                        w.tile_clu, w.tile_first, w.tile_cnt);

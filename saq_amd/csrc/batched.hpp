@@ -46,8 +46,9 @@ size_t bscan_smem_bytes(const BatchedView &bv, int topk, int qtm, int qs);
 hipError_t launch_bq_build(const IndexView &ix, const BatchedView &bv, const int32_t *mb_clu, int64_t total_mb,
                            const int32_t *chunk_seg, const int32_t *chunk_kc, int nchunks, uint8_t *out,
                            hipStream_t st);
+// order: the lists by decreasing size (the order work tiles are handed out in).
 hipError_t launch_bq_group(const uint32_t *probes, const int32_t *npq, int64_t nq, int NP, int K, int QT,
-                           const BqWork &w, hipStream_t st);
+                           const int32_t *order, const BqWork &w, hipStream_t st);
 hipError_t launch_bq_merge(const float *cand_d, const uint32_t *cand_id, const int32_t *npq, int64_t nq, int NP,
                            int topk, uint32_t *out_ids, float *out_d, hipStream_t st);
 
