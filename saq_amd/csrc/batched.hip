@@ -467,9 +467,13 @@ struct WSmem {
     uint32_t *tki;   // [QT][topk]
 };
 
-template <int QTM>
+// mblocks per wave chunk: 64 vectors' accumulators per query tile for int8, half that for
+// int8x2 (whose second term doubles the accumulators)
+__host__ __device__ constexpr int nvw_of(int qtm, int qs) { return qs == 2 ? (qtm >= 2 ? 1 : 2) : 4 / qtm; }
+
+template <int QTM, int QS>
 __host__ __device__ inline size_t wave_smem(int nqs, int topk, WSmem *m, uint8_t *base) {
-    constexpr int QT = 16 * QTM, CVW = 64 / QTM;
+    constexpr int QT = 16 * QTM, CVW = 16 * nvw_of(QTM, QS);
     size_t off = 0;
     auto take = [&](size_t bytes) {
         uint8_t *p = base ? base + off : nullptr;
@@ -516,14 +520,14 @@ constexpr int kKB = 2;  // 64-dim chunks whose operands are loaded together
 template <int QTM, int QS, int KR>
 __global__ __launch_bounds__(kBThreads) void bscan_kernel(BScanArgs a) {
     constexpr int QT = 16 * QTM;
-    constexpr int NVW = 4 / QTM;   // mblocks per chunk
-    constexpr int CVW = 16 * NVW;  // vectors per chunk
+    constexpr int NVW = nvw_of(QTM, QS);  // mblocks per chunk
+    constexpr int CVW = 16 * NVW;         // vectors per chunk
     extern __shared__ __align__(16) uint8_t smem_raw[];
     const BatchedView &bv = a.bv;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const size_t per_wave = wave_smem<QTM>(bv.nqs, a.topk, nullptr, nullptr);
+    const size_t per_wave = wave_smem<QTM, QS>(bv.nqs, a.topk, nullptr, nullptr);
     WSmem m;
-    wave_smem<QTM>(bv.nqs, a.topk, &m, smem_raw + w * per_wave);
+    wave_smem<QTM, QS>(bv.nqs, a.topk, &m, smem_raw + w * per_wave);
     const int ntiles = *a.ntiles;
     for (;;) {
         int tile = 0;
@@ -760,7 +764,7 @@ hipError_t launch_bq_group(const uint32_t *probes, const int32_t *npq, int64_t n
 
 template <int QTM, int QS, int KR>
 static hipError_t launch_bscan_t(const BScanArgs &a, int grid, hipStream_t st) {
-    const size_t smem = kBWaves * wave_smem<QTM>(a.bv.nqs, a.topk, nullptr, nullptr);
+    const size_t smem = kBWaves * wave_smem<QTM, QS>(a.bv.nqs, a.topk, nullptr, nullptr);
     auto kern = bscan_kernel<QTM, QS, KR>;
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
@@ -777,10 +781,14 @@ static hipError_t launch_bscan_t(const BScanArgs &a, int grid, hipStream_t st) {
 }
 
 size_t bscan_smem_bytes(const BatchedView &bv, int topk, int qtm, int qs) {
-    (void)qs;
-    if (qtm == 1) return kBWaves * wave_smem<1>(bv.nqs, topk, nullptr, nullptr);
-    if (qtm == 2) return kBWaves * wave_smem<2>(bv.nqs, topk, nullptr, nullptr);
-    return kBWaves * wave_smem<4>(bv.nqs, topk, nullptr, nullptr);
+    if (qs == 2) {
+        if (qtm == 1) return kBWaves * wave_smem<1, 2>(bv.nqs, topk, nullptr, nullptr);
+        if (qtm == 2) return kBWaves * wave_smem<2, 2>(bv.nqs, topk, nullptr, nullptr);
+        return kBWaves * wave_smem<4, 2>(bv.nqs, topk, nullptr, nullptr);
+    }
+    if (qtm == 1) return kBWaves * wave_smem<1, 1>(bv.nqs, topk, nullptr, nullptr);
+    if (qtm == 2) return kBWaves * wave_smem<2, 1>(bv.nqs, topk, nullptr, nullptr);
+    return kBWaves * wave_smem<4, 1>(bv.nqs, topk, nullptr, nullptr);
 }
 
 template <int QTM, int QS>
