@@ -48,9 +48,9 @@ struct PrepArgs {
     float *bound;  // nq x nseg
 };
 
-constexpr int kPrepQ = 8;
+constexpr int kPrepQ = 16;
 
-// grid (ceil(nq/8), nseg, ceil(D_s/256)): 256 outputs x 8 queries per workgroup.
+// grid (ceil(nq/16), nseg, ceil(D_s/256)): 256 outputs x 16 queries per workgroup.
 __global__ __launch_bounds__(256) void query_prep_kernel(PrepArgs a) {
     __shared__ float qs[kPrepQ][256];
     const IndexView &ix = a.ix;
@@ -76,7 +76,25 @@ __global__ __launch_bounds__(256) void query_prep_kernel(PrepArgs a) {
             qs[t][threadIdx.x] = v;
         }
         __syncthreads();
-        if (j < D) {
+        if (j < D && bc == 16 && P) {
+            // blocks of 16 input dims (256-dim chunks hold whole blocks): 16 independent row
+            // loads, then the block's FMA chain per query, then res += c
+            const int iend = min(256, D - i0);
+            for (int ib = 0; ib < iend; ib += 16) {
+                float pr[16];
+#pragma unroll
+                for (int u = 0; u < 16; ++u) pr[u] = P[(int64_t)(i0 + ib + u) * D + j];
+#pragma unroll
+                for (int u = 0; u < 16; ++u)
+#pragma unroll
+                    for (int t = 0; t < kPrepQ; ++t) c[t] = __fmaf_rn(pr[u], qs[t][ib + u], c[t]);
+#pragma unroll
+                for (int t = 0; t < kPrepQ; ++t) {
+                    acc[t] = acc[t] + c[t];  // pmadd(c, alpha = 1, res)
+                    c[t] = 0.0f;
+                }
+            }
+        } else if (j < D) {
             const int iend = min(256, D - i0);
             for (int ii = 0; ii < iend; ++ii) {
                 const int i = i0 + ii;
