@@ -183,6 +183,8 @@ def main():
     ap.add_argument("--kmeans-iters", type=int, default=8)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--exact-probe", action="store_true",
+                    help="batched scan with the reference-order coarse quantiser instead of the fp32-MFMA one")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 5 times (the known-size kernel that scales "
                          "FETCH_SIZE/WRITE_SIZE counters to bytes) and exit")
@@ -319,7 +321,7 @@ def main():
 
     def bsearch(npb, prec, qt=0):
         ix.search_batched(queries, args.topk, npb, precision=prec, query_tile=qt, out_ids=bids, out_dists=bd,
-                          stream=stream)
+                          stream=stream, exact_probe=args.exact_probe)
 
     def brecall(npb, prec):
         bsearch(npb, prec)
@@ -332,7 +334,8 @@ def main():
             r = brecall(npb, prec)
             log(f"batched prec={prec} nprobe {npb}: recall@{args.topk} {r:.4f}")
             if r >= rec:
-                qt = ix.autotune_batched(queries, args.topk, npb, precision=prec, stream=stream)
+                qt = ix.autotune_batched(queries, args.topk, npb, precision=prec, stream=stream,
+                                         exact_probe=args.exact_probe)
                 torch.cuda.synchronize()
                 t0 = time.perf_counter()
                 for _ in range(3):
@@ -348,7 +351,7 @@ def main():
     # algorithmic work of one tile-kernel launch: per work tile (a list and up to bqt of its
     # queries) one pass over the list's 16-vector records; the residual queries (q_rot, c_rot)
     # and the per-pair candidate lists; MFMA ops = 2 * K per (query, vector) pair and term
-    pr = ix.probe(queries.cpu().numpy(), bnp).astype(np.int64)
+    pr = ix.probe(queries.cpu().numpy(), bnp).astype(np.int64)  # exact lists; the fast ones differ only at ties
     cnt = np.bincount(pr.ravel(), minlength=args.nlist)
     nmb = (sizes + 15) // 16
     kq_bytes = sum((512 if b <= 4 else (1024 if b <= 8 else 2048)) * (d // 64) for d, b in plan if b > 0)
@@ -362,6 +365,7 @@ def main():
     b_tops = b_ops / (bkern_ms / 1e3) / 1e12
     batched = {"qps": round(bqps, 1), "recall_at_10": round(brec, 4), "nprobe": bnp,
                "precision": "int8x2" if bprec == saq_amd.PREC_INT8X2 else "int8", "query_tile": bqt,
+               "coarse_quantiser": "exact (reference order)" if args.exact_probe else "fp32 MFMA",
                "ms_per_step": round(bdt / args.steps * 1e3, 4), "layout_build_ms": round(t_prep * 1e3, 1),
                "kernel_ms": round(bkern_ms, 4), "alg_bytes_per_launch": b_bytes,
                "hbm_frac": round(b_gbs / HBM_PEAK_GBS, 4), "int8_mfma_tops": round(b_tops, 2),

@@ -162,6 +162,9 @@ typedef struct {
     const int32_t *nprobe_per_query;  /* NULL, or nq entries in 1..nprobe, in the queries' memory kind */
     int32_t precision;                /* SAQGPU_PREC_INT8 or SAQGPU_PREC_INT8X2 */
     int32_t query_tile;               /* queries per work tile: 16, 32 or 64; 0 = the autotuned choice */
+    int32_t exact_probe;              /* with probes == NULL: 1 = the reference's coarse quantiser order
+                                         (as saqgpu_probe), 0 = fp32-MFMA distances (probe lists may
+                                         differ where two centroid distances tie within fp32 rounding) */
 } saqgpu_batched_params;
 
 /* Builds the batched code layout (full codes in MFMA operand order plus per-vector factors)

@@ -47,7 +47,7 @@ class SearchParams(C.Structure):
 
 class BatchedParams(C.Structure):
     _fields_ = [("topk", C.c_int32), ("nprobe", C.c_int32), ("nprobe_per_query", C.c_void_p),
-                ("precision", C.c_int32), ("query_tile", C.c_int32)]
+                ("precision", C.c_int32), ("query_tile", C.c_int32), ("exact_probe", C.c_int32)]
 
 
 PREC_INT8 = 1
@@ -237,21 +237,21 @@ class Index:
     def prepare_batched(self):
         _check(lib().saqgpu_prepare_batched(self.h))
 
-    def _bparams(self, topk, nprobe, nprobe_per_query, precision, query_tile, dev):
+    def _bparams(self, topk, nprobe, nprobe_per_query, precision, query_tile, dev, exact_probe=False):
         npq = None
         if nprobe_per_query is not None and not dev:
             npq = np.ascontiguousarray(nprobe_per_query, np.int32)
         elif nprobe_per_query is not None:
             npq = nprobe_per_query
-        p = BatchedParams(topk, nprobe, _ptr(npq), precision, query_tile)
+        p = BatchedParams(topk, nprobe, _ptr(npq), precision, query_tile, int(bool(exact_probe)))
         return p, npq
 
     def search_batched(self, queries, topk, nprobe, probes=None, nprobe_per_query=None, precision=PREC_INT8,
-                       query_tile=0, out_ids=None, out_dists=None, stream=None, want_dists=True):
+                       query_tile=0, out_ids=None, out_dists=None, stream=None, want_dists=True, exact_probe=False):
         """Batched int8-MFMA search (saqgpu_search_batched).  Returns (ids, dists)."""
         dev = not isinstance(queries, np.ndarray)
         nq = int(queries.shape[0])
-        p, keep = self._bparams(topk, nprobe, nprobe_per_query, precision, query_tile, dev)
+        p, keep = self._bparams(topk, nprobe, nprobe_per_query, precision, query_tile, dev, exact_probe)
         if not dev:
             queries = np.ascontiguousarray(queries, np.float32)
             if probes is not None:
@@ -265,14 +265,14 @@ class Index:
         return out_ids, out_dists
 
     def autotune_batched(self, queries, topk, nprobe, probes=None, nprobe_per_query=None, precision=PREC_INT8,
-                         stream=None):
+                         stream=None, exact_probe=False):
         """Times every query tile on this batch; the fastest becomes the default.  Returns it."""
         dev = not isinstance(queries, np.ndarray)
         if not dev:
             queries = np.ascontiguousarray(queries, np.float32)
             if probes is not None:
                 probes = np.ascontiguousarray(probes, np.uint32)
-        p, keep = self._bparams(topk, nprobe, nprobe_per_query, precision, 0, dev)
+        p, keep = self._bparams(topk, nprobe, nprobe_per_query, precision, 0, dev, exact_probe)
         best = C.c_int32(0)
         _check(lib().saqgpu_autotune_batched(self.h, _ptr(queries), int(queries.shape[0]), _ptr(probes), C.byref(p),
                                              DEVICE_PTRS if dev else 0, stream, C.byref(best)))

@@ -28,6 +28,10 @@ hipError_t launch_scan(const IndexView &ix, const float *qrot, const float *boun
 size_t scan_smem_bytes(const IndexView &ix, int topk, int cmax);
 hipError_t launch_probe(const float *centroids, int K, int D, const float *queries, int64_t nq, int nprobe,
                         int dist_type, float *dist_ws, uint32_t *probes, hipStream_t st);
+hipError_t launch_sqnorm_rows(const float *x, int64_t rows, int D, float *out, hipStream_t st);
+hipError_t launch_probe_fast(const float *centroids, const float *cnorm, int K, int D, const float *queries,
+                             int64_t nq, int nprobe, int dist_type, float *qnorm, float *dist_ws, uint32_t *probes,
+                             hipStream_t st);
 hipError_t launch_encode_segment(const IndexView &ix, const float *data, const uint32_t *src, const int64_t *gvec,
                                  const int64_t *gblk, const int32_t *lane, const uint32_t *clu, const float *cent,
                                  const float *rot, int seg, int seg_dim, int64_t nslots, int64_t ld, int rd_lmt,
@@ -116,7 +120,7 @@ struct saqgpu_index {
     DevBuf<BSeg> d_bsegs;
     DevBuf<int32_t> bq_ws, bq_npq, bq_pcorr;
     DevBuf<int8_t> bq_qq;
-    DevBuf<float> bq_pc;
+    DevBuf<float> bq_pc, bq_cnorm, bq_qnorm;
     DevBuf<float> bq_cand_d;
     DevBuf<uint32_t> bq_cand_id;
     // measurement
@@ -343,6 +347,8 @@ void saqgpu_index_destroy(saqgpu_index *ix) {
     ix->bq_pcorr.release();
     ix->bq_qq.release();
     ix->bq_pc.release();
+    ix->bq_cnorm.release();
+    ix->bq_qnorm.release();
     ix->bq_npq.release();
     ix->bq_cand_d.release();
     ix->bq_cand_id.release();
@@ -895,6 +901,8 @@ int prepare_batched_locked(saqgpu_index *ix) {
     if ((rc = upload(d_mb_clu, mb_clu)) || (rc = upload(d_cs, chunk_seg)) || (rc = upload(d_ck, chunk_kc))) return rc;
     HIPCHK(launch_bq_build(ix->view(), batched_view(ix), d_mb_clu.p, tot, d_cs.p, d_ck.p, (int)chunk_seg.size(),
                            ix->bq_mb.p, 0));
+    if (ix->bq_cnorm.ensure(ix->num_cen)) return fail(SAQGPU_ENOMEM, "allocation failed");
+    HIPCHK(launch_sqnorm_rows(ix->centroids.p, ix->num_cen, ix->num_dim, ix->bq_cnorm.p, 0));
     HIPCHK(hipDeviceSynchronize());
     d_mb_clu.release();
     d_cs.release();
@@ -976,8 +984,13 @@ int search_batched_locked(saqgpu_index *ix, const float *queries, int64_t nq, co
         dd = out_dists ? ix->ws_dists.p : nullptr;
     }
     if (!probes) {
-        if (ix->ws_cdist.ensure((size_t)nq * K)) return fail(SAQGPU_ENOMEM, "workspace allocation failed");
-        HIPCHK(launch_probe(ix->centroids.p, K, ix->num_dim, dq, nq, NP, 0, ix->ws_cdist.p, ix->ws_probes.p, st));
+        if (ix->ws_cdist.ensure((size_t)nq * K) || ix->bq_qnorm.ensure(nq))
+            return fail(SAQGPU_ENOMEM, "workspace allocation failed");
+        if (p->exact_probe)
+            HIPCHK(launch_probe(ix->centroids.p, K, ix->num_dim, dq, nq, NP, 0, ix->ws_cdist.p, ix->ws_probes.p, st));
+        else
+            HIPCHK(launch_probe_fast(ix->centroids.p, ix->bq_cnorm.p, K, ix->num_dim, dq, nq, NP, 0, ix->bq_qnorm.p,
+                                     ix->ws_cdist.p, ix->ws_probes.p, st));
         dp = ix->ws_probes.p;
     }
     HIPCHK(launch_query_prep(ix->view(), dq, nq, ix->variance.p, ix->has_rot ? ix->rot.p : nullptr, ix->d_rot_off.p,

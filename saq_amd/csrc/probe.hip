@@ -196,6 +196,81 @@ __global__ __launch_bounds__(256) void probe_tile_kernel(const float *__restrict
         }
 }
 
+// ---- fp32-MFMA coarse quantiser (batched mode)
+// dist[q][c] = |q|^2 + |c|^2 - 2 q.c (or q.c for IP) with q.c on v_mfma_f32_16x16x4_f32.  This is
+// not FlatInitializer's reduction order, so a probe list can differ from the reference's where
+// two centroid distances lie within fp32 rounding of each other; the batched scan, which is
+// approximate by design, may use it.  A workgroup covers 64 queries x 64 centroids, each wave
+// 32 x 32, with 32-dim chunks of both staged through LDS.
+typedef float v4f __attribute__((ext_vector_type(4)));
+constexpr int kGT = 64, kGK = 32;
+
+__global__ __launch_bounds__(256) void sqnorm_rows_kernel(const float *__restrict__ x, int64_t rows, int D,
+                                                          float *__restrict__ out) {
+    const int lane = threadIdx.x & 63;
+    const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= rows) return;
+    float s = 0.0f;
+    for (int i = lane; i < D; i += 64) s += x[r * D + i] * x[r * D + i];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (lane == 0) out[r] = s;
+}
+
+template <int DT>
+__global__ __launch_bounds__(256) void probe_gemm_kernel(const float *__restrict__ cent, const float *__restrict__ cnorm,
+                                                         int K, int D, const float *__restrict__ queries,
+                                                         const float *__restrict__ qnorm, int64_t nq,
+                                                         float *__restrict__ out) {
+    __shared__ float As[kGT][kGK + 1];
+    __shared__ float Bs[kGT][kGK + 1];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int64_t q0 = (int64_t)blockIdx.y * kGT;
+    const int c0 = blockIdx.x * kGT;
+    const int qb = (w >> 1) * 32, cbb = (w & 1) * 32;
+    v4f acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int k0 = 0; k0 < D; k0 += kGK) {
+        __syncthreads();
+        for (int e = tid; e < kGT * kGK; e += 256) {
+            const int r = e / kGK, k = e % kGK;
+            const int64_t q = q0 + r;
+            const int c = c0 + r;
+            As[r][k] = (q < nq && k0 + k < D) ? queries[q * D + k0 + k] : 0.0f;
+            Bs[r][k] = (c < K && k0 + k < D) ? cent[(int64_t)c * D + k0 + k] : 0.0f;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < kGK; kk += 4) {
+            float av[2], bv[2];
+#pragma unroll
+            for (int a = 0; a < 2; ++a) av[a] = As[qb + a * 16 + lane % 16][kk + lane / 16];
+#pragma unroll
+            for (int b = 0; b < 2; ++b) bv[b] = Bs[cbb + b * 16 + lane % 16][kk + lane / 16];
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[a], bv[b], acc[a][b], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t q = q0 + qb + a * 16 + 4 * (lane / 16) + r;
+                const int c = c0 + cbb + b * 16 + lane % 16;
+                if (q < nq && c < K) {
+                    const float ip = acc[a][b][r];
+                    out[q * K + c] = DT == 1 ? ip : (qnorm[q] + cnorm[c]) - 2.0f * ip;
+                }
+            }
+}
+
 constexpr int kSelMax = 4096;
 
 // 64-bit key: ascending order = probe order.
@@ -277,6 +352,31 @@ __global__ __launch_bounds__(256) void probe_select_kernel(const float *__restri
         }
     }
     for (int i = tid; i < nprobe && i < n; i += 256) probes[q * nprobe + i] = (uint32_t)(sel[i] & 0xFFFFFFFFu);
+}
+
+hipError_t launch_sqnorm_rows(const float *x, int64_t rows, int D, float *out, hipStream_t st) {
+    if (rows == 0) return hipSuccess;
+    hipLaunchKernelGGL(sqnorm_rows_kernel, dim3((unsigned)((rows + 3) / 4)), dim3(256), 0, st, x, rows, D, out);
+    return hipGetLastError();
+}
+
+// The batched scan's coarse quantiser: fp32-MFMA distances (cnorm = |c|^2 per centroid, qnorm
+// workspace of nq floats), then the same exact top-nprobe selection.
+hipError_t launch_probe_fast(const float *centroids, const float *cnorm, int K, int D, const float *queries,
+                             int64_t nq, int nprobe, int dist_type, float *qnorm, float *dist_ws, uint32_t *probes,
+                             hipStream_t st) {
+    if (nprobe > kSelMax || (nq + kGT - 1) / kGT > 65535) return hipErrorInvalidValue;
+    hipError_t e = launch_sqnorm_rows(queries, nq, D, qnorm, st);
+    if (e != hipSuccess) return e;
+    dim3 g((unsigned)((K + kGT - 1) / kGT), (unsigned)((nq + kGT - 1) / kGT));
+    if (dist_type == 1) {
+        hipLaunchKernelGGL(probe_gemm_kernel<1>, g, dim3(256), 0, st, centroids, cnorm, K, D, queries, qnorm, nq, dist_ws);
+        hipLaunchKernelGGL(probe_select_kernel<1>, dim3((unsigned)nq), dim3(256), 0, st, dist_ws, K, nprobe, probes);
+    } else {
+        hipLaunchKernelGGL(probe_gemm_kernel<0>, g, dim3(256), 0, st, centroids, cnorm, K, D, queries, qnorm, nq, dist_ws);
+        hipLaunchKernelGGL(probe_select_kernel<0>, dim3((unsigned)nq), dim3(256), 0, st, dist_ws, K, nprobe, probes);
+    }
+    return hipGetLastError();
 }
 
 template <int DT>

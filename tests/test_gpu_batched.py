@@ -97,8 +97,23 @@ def test_empty_lists_and_gpu_probe():
     case = F.make_case(n=900, d=128, K=5, nq=10, plan=[(128, 4)], seed=61, cluster_ids=ids_in)
     ids_a, d_a = run(case, 20, 5, PREC_INT8X2)
     ix = F.gpu_index(case)
-    ids_b, d_b = ix.search_batched(case.queries, 20, 5, precision=PREC_INT8X2)  # GPU coarse quantiser
+    # GPU coarse quantiser in the reference's order: the same lists, the same results
+    ids_b, d_b = ix.search_batched(case.queries, 20, 5, precision=PREC_INT8X2, exact_probe=True)
     assert np.array_equal(ids_a, ids_b) and np.array_equal(d_a.view(np.uint32), d_b.view(np.uint32))
+    ix.close()
+
+
+def test_fast_probe_selects_the_same_lists_without_near_ties():
+    """The fp32-MFMA coarse quantiser differs from the exact one only where two centroid
+    distances tie within fp32 rounding; on this data no query has such a tie at its nprobe-th
+    list, so every result equals the exact-probe result."""
+    case = F.make_case(n=4000, d=200, K=24, nq=64, plan=[(128, 5), (128, 3)], seed=67)
+    ix = F.gpu_index(case)
+    for nprobe in (1, 4, 11):
+        a, da = ix.search_batched(case.queries, 10, nprobe, exact_probe=True)
+        b, db = ix.search_batched(case.queries, 10, nprobe, exact_probe=False)
+        assert np.array_equal(a, b), f"nprobe={nprobe}"
+        assert np.array_equal(da.view(np.uint32), db.view(np.uint32))
     ix.close()
 
 
