@@ -38,13 +38,14 @@ HBM_PEAK_GBS = 8000.0
 INT8_PEAK_TOPS = 5000.0  # dense int8 MFMA, MI355X
 
 
-def read_traffic(name, n, bits, nprobe):
+def read_traffic(name, n, bits, nprobe, **extra):
     """HBM bytes per launch of a kernel from its committed FETCH_SIZE/WRITE_SIZE summary, when
-    that summary was taken on this workload."""
+    that summary was taken on this workload (extra: further keys that must match)."""
     path = os.path.join(ROOT, "profiles", name)
     try:
         t = json.load(open(path))
-        if t.get("nprobe") == nprobe and t.get("bits") == bits and t.get("n") == n:
+        if t.get("nprobe") == nprobe and t.get("bits") == bits and t.get("n") == n and \
+                all(t.get(k) == v for k, v in extra.items()):
             return t.get("hbm_bytes_per_launch")
     except (OSError, ValueError):
         pass
@@ -372,7 +373,7 @@ def main():
                "int8_mfma_frac": round(b_tops / INT8_PEAK_TOPS, 5)}
     log(f"batched: {bqps:.0f} QPS ({bqps / qps:.2f}x plain), tile kernel {bkern_ms:.3f} ms, {b_gbs:.0f} GB/s, "
         f"{b_tops:.1f} TOPS")
-    btraffic = read_traffic("bscan_traffic.json", args.n, bits, bnp)
+    btraffic = read_traffic("bscan_traffic.json", args.n, bits, bnp, precision=batched["precision"], query_tile=bqt)
 
     cpu = None
     if not args.no_cpu:
