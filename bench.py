@@ -1,26 +1,31 @@
 """Benchmark of the SAQ IVF scan on one MI355X: the batched int8-MFMA scan against the plain
 (reference-exact) GPU scan, at equal recall.
 
-Workload (the single-GPU configuration the headline metric is quoted on):
-N = 1,000,000 base vectors, D = 960 fp32, from a 256-component Gaussian mixture
-(centres ~ N(0, 4I), per-dim std 1/sqrt(1 + i/32)), PCA-rotated with an FP64
-covariance, nlist = 4096 k-means lists, SAQ with segmentation, 1,000 held-out
-queries, k = 10.  On this mixture the estimator's quantisation error, not probing,
-limits recall (B = 4 levels off near 0.81 recall@10 for any nprobe), so the bits
-per dimension are the smallest of 4..8 whose recall@10 against exact FP64 ground
-truth reaches 0.95 at nprobe 64, and nprobe is then the smallest value of a sweep
-that keeps it.  The recall reached by every bits value tried is in the JSON line.
+Workloads (synthetic, seeded; PCA-rotated with an FP64 covariance; k-means lists, seed 0):
 
-A step = one search of the 1,000 queries (GPU coarse quantiser + query rotation +
-scan), inputs resident in HBM.  The plain scan is timed first; the batched scan is
-then run at the smallest nprobe (and the cheaper of its int8 / int8x2 precisions)
-whose recall@10 is at least the plain scan's, with the query tile its autotuner
-picks.  `value` is the batched scan's throughput; the plain scan's is in
-config.plain.  The JSON line also carries each scan kernel's roofline (algorithmic
-bytes / HIP-event kernel time vs 8 TB/s; int8 MFMA ops vs 5 POP/s for the batched
-kernel), the GPU encoder's throughput, and a CPU baseline: the oracle (CPU
-restatement of the reference) timed on this host's cores on the same queries,
-whose IDs are also compared with the plain GPU scan's.
+mix1m-d960 (default; the single-GPU configuration the headline metric is quoted on)
+    N = 1,000,000 base vectors, D = 960 fp32, from a 256-component Gaussian mixture
+    (centres ~ N(0, 4I), per-dim std 1/sqrt(1 + i/32)), nlist = 4096, 1,000 held-out
+    queries, k = 10.  On this mixture the estimator's quantisation error, not probing,
+    limits recall (B = 4 levels off near 0.81 recall@10 for any nprobe), so the bits per
+    dimension are the smallest of 4..8 whose recall@10 against exact FP64 ground truth
+    reaches 0.95 at nprobe 64, and nprobe is then the smallest value of a sweep that keeps
+    it.  The recall reached by every bits value tried is in the JSON line.
+zipf10m-d1024
+    N = 10,000,000, D = 1024, a 2,048-component mixture with Zipf(1.1) component weights
+    (std decay 1/sqrt(1 + i/32)), nlist = 16384, B = 4 (--bits), 10,000 queries whose
+    nprobe is drawn per query from U[32, 256] (seed 3), k = 10.
+
+A step = one search of the whole query batch (GPU coarse quantiser + query rotation +
+scan), inputs resident in HBM.  The plain scan is timed first; the batched scan is then
+run at the smallest nprobe (and the cheaper of its int8 / int8x2 precisions) whose
+recall@10 is at least the plain scan's (with per-query nprobe: the same nprobe, the
+cheaper precision that reaches it), with the query tile its autotuner picks.  `value` is
+the batched scan's throughput; the plain scan's is in config.plain.  The JSON line also
+carries each scan kernel's roofline (algorithmic bytes / HIP-event kernel time vs 8 TB/s;
+int8 MFMA ops vs 5 POP/s for the batched kernel), the GPU encoder's throughput, and a CPU
+baseline: the oracle (CPU restatement of the reference) timed on this host's cores on a
+bounded sample of the same queries, whose IDs are also compared with the plain GPU scan's.
 """
 import argparse
 import json
@@ -36,6 +41,13 @@ sys.path.insert(0, ROOT)
 METRIC = "QPS @ recall@10=0.95 batched vs plain GPU; encode Mvec/s; p99 ms"
 HBM_PEAK_GBS = 8000.0
 INT8_PEAK_TOPS = 5000.0  # dense int8 MFMA, MI355X
+
+WORKLOADS = {
+    "mix1m-d960": dict(n=1_000_000, dim=960, comps=256, tau=32, zipf=0.0, nlist=4096, nq=1000, bits=0.0,
+                       npq_range=None, steps=20, kmeans_iters=8, cpu_queries=0),
+    "zipf10m-d1024": dict(n=10_000_000, dim=1024, comps=2048, tau=32, zipf=1.1, nlist=16384, nq=10000, bits=4.0,
+                          npq_range=(32, 256), steps=3, kmeans_iters=6, cpu_queries=16),
+}
 
 
 def read_traffic(name, n, bits, nprobe, **extra):
@@ -56,30 +68,47 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def make_data(torch, n, d, comps, tau, nq, seed, dev):
+def make_data(torch, n, d, comps, tau, zipf, nq, seed, dev):
     g = torch.Generator(device=dev)
     g.manual_seed(seed)
     centers = torch.randn(comps, d, generator=g, device=dev) * 2.0
     std = 1.0 / torch.sqrt(1.0 + torch.arange(d, device=dev, dtype=torch.float32) / tau)
+    if zipf > 0:
+        w = 1.0 / torch.arange(1, comps + 1, device=dev, dtype=torch.float64) ** zipf
+    else:
+        w = torch.ones(comps, device=dev, dtype=torch.float64)
+    w = (w / w.sum()).float()
+
+    def draw(m):
+        if zipf > 0:
+            comp = torch.multinomial(w, m, replacement=True, generator=g)
+        else:
+            comp = torch.randint(0, comps, (m,), generator=g, device=dev)
+        return centers[comp] + torch.randn(m, d, generator=g, device=dev) * std
+
     base = torch.empty(n, d, device=dev)
     step = 1 << 18
     for s in range(0, n, step):
         e = min(n, s + step)
-        comp = torch.randint(0, comps, (e - s,), generator=g, device=dev)
-        base[s:e] = centers[comp] + torch.randn(e - s, d, generator=g, device=dev) * std
-    comp = torch.randint(0, comps, (nq,), generator=g, device=dev)
-    queries = centers[comp] + torch.randn(nq, d, generator=g, device=dev) * std
+        base[s:e] = draw(e - s)
+    queries = draw(nq)
     return base, queries
 
 
 def pca(torch, base):
-    x = base.double()
-    mean = x.mean(0)
-    cov = torch.zeros(x.shape[1], x.shape[1], dtype=torch.float64, device=x.device)
-    for s in range(0, x.shape[0], 1 << 17):
-        xc = x[s:s + (1 << 17)] - mean
+    """Mean and covariance accumulated in FP64, chunk by chunk; eigenvectors by decreasing
+    variance."""
+    n, d = base.shape
+    step = 1 << 17
+    mean = torch.zeros(d, dtype=torch.float64, device=base.device)
+    for s in range(0, n, step):
+        mean += base[s:s + step].double().sum(0)
+    mean /= n
+    cov = torch.zeros(d, d, dtype=torch.float64, device=base.device)
+    for s in range(0, n, step):
+        xc = base[s:s + step].double() - mean
         cov += xc.T @ xc
-    cov /= x.shape[0]
+    cov /= n
     w, v = torch.linalg.eigh(cov.cpu())
     order = torch.argsort(w, descending=True)
     comps = v[:, order].float().to(base.device)
@@ -87,14 +116,23 @@ def pca(torch, base):
     return mean.float(), comps, var
 
 
+def rotate_inplace(torch, x, mean, comps):
+    step = 1 << 18
+    for s in range(0, x.shape[0], step):
+        x[s:s + step] = (x[s:s + step] - mean) @ comps
+    return x
+
+
 def kmeans(torch, x, k, iters, seed):
     g = torch.Generator(device=x.device)
     g.manual_seed(seed)
     cent = x[torch.randperm(x.shape[0], generator=g, device=x.device)[:k]].clone()
+    step = 1 << 18
     for _ in range(iters):
         ids = assign(torch, x, cent)
         sums = torch.zeros(k, x.shape[1], dtype=torch.float64, device=x.device)
-        sums.index_add_(0, ids, x.double())
+        for s in range(0, x.shape[0], step):
+            sums.index_add_(0, ids[s:s + step], x[s:s + step].double())
         cnt = torch.bincount(ids, minlength=k)
         nz = cnt > 0
         cent[nz] = (sums[nz] / cnt[nz, None]).float()
@@ -104,21 +142,27 @@ def kmeans(torch, x, k, iters, seed):
 def assign(torch, x, cent):
     cn = (cent * cent).sum(1)
     out = torch.empty(x.shape[0], dtype=torch.int64, device=x.device)
-    for s in range(0, x.shape[0], 1 << 16):
-        d = cn[None, :] - 2.0 * (x[s:s + (1 << 16)] @ cent.T)
-        out[s:s + (1 << 16)] = d.argmin(1)
+    step = max(1, min(1 << 16, (1 << 30) // cent.shape[0]))
+    for s in range(0, x.shape[0], step):
+        d = cn[None, :] - 2.0 * (x[s:s + step] @ cent.T)
+        out[s:s + step] = d.argmin(1)
     return out
 
 
-def exact_topk(torch, base, queries, k):
-    b = base.double()
-    bn = (b * b).sum(1)
-    ids = []
-    for s in range(0, queries.shape[0], 256):
-        q = queries[s:s + 256].double()
-        d = bn[None, :] - 2.0 * (q @ b.T)
-        ids.append(d.topk(k, dim=1, largest=False).indices)
-    return torch.cat(ids).cpu().numpy()
+def exact_topk(torch, base, queries, k, cand=64):
+    """Exact FP64 top-k: FP32 distances pick `cand` candidates per query, FP64 re-ranks them."""
+    bn = (base * base).sum(1)
+    out = []
+    qstep = max(1, min(256, (1 << 31) // base.shape[0]))
+    for s in range(0, queries.shape[0], qstep):
+        q = queries[s:s + qstep]
+        d = bn[None, :] - 2.0 * (q @ base.T)
+        c = d.topk(cand, dim=1, largest=False).indices
+        xb = base[c].double()  # qstep x cand x D
+        dd = ((xb - q.double()[:, None, :]) ** 2).sum(2)
+        o = dd.argsort(dim=1)[:, :k]
+        out.append(torch.gather(c, 1, o))
+    return torch.cat(out).cpu().numpy()
 
 
 def recall(found, gt, k):
@@ -128,9 +172,9 @@ def recall(found, gt, k):
     return hit / (len(gt) * k)
 
 
-def cpu_baseline(ix, plan, queries_np, variance, rots, probes, topk, nprobe, m, gpu_ids, threads,
-                 min_seconds=10.0):
-    """The oracle (CPU restatement of the reference, scalar C++) on this host's cores."""
+def cpu_baseline(ix, plan, queries_np, variance, rots, probes, npq, topk, m, gpu_ids, threads, min_seconds=10.0):
+    """The oracle (CPU restatement of the reference, scalar C++) on this host's cores.  Only the
+    lists the sample probes are downloaded from the GPU index."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import ctypes as C
     from oracle_lib import Cluster, Metrics, make_plan, oracle
@@ -138,7 +182,10 @@ def cpu_baseline(ix, plan, queries_np, variance, rots, probes, topk, nprobe, m, 
     K = ix.num_cen
     clus = (Cluster * K)()
     keep = []
-    for c in range(K):
+    need = set()
+    for i in range(probes.shape[0]):
+        need.update(probes[i, :(npq[i] if npq is not None else probes.shape[1])].tolist())
+    for c in need:
         a = ix.get_cluster(c)
         keep.append(a)
         clus[c] = Cluster(int(ix.cluster_sizes[c]), a["short_factors"].ctypes.data if a["short_factors"].size else None,
@@ -153,13 +200,18 @@ def cpu_baseline(ix, plan, queries_np, variance, rots, probes, topk, nprobe, m, 
     nq = queries_np.shape[0]
     ids = np.zeros((nq, topk), np.uint32)
     mets = (Metrics * nq)()
-    probes = np.ascontiguousarray(probes, np.uint32)
     queries_np = np.ascontiguousarray(queries_np)
+    groups = [(np.arange(nq), probes.shape[1])] if npq is None else \
+        [(np.nonzero(npq == v)[0], int(v)) for v in np.unique(npq)]
     passes, dt, mism = 0, 0.0, 0
     while dt < min_seconds and passes < 200:  # repeat whole passes to a bounded sample of ~10 s
         t0 = time.perf_counter()
-        O.orc_search_batch(C.byref(p), vpad, rp, clus, probes, nprobe, queries_np, nq, topk, m, 0, threads, ids,
-                           None, C.cast(mets, C.c_void_p))
+        for rows, npb in groups:
+            sub_ids = np.zeros((len(rows), topk), np.uint32)
+            O.orc_search_batch(C.byref(p), vpad, rp, clus, np.ascontiguousarray(probes[rows, :npb], np.uint32), npb,
+                               np.ascontiguousarray(queries_np[rows]), len(rows), topk, m, 0, threads, sub_ids, None,
+                               C.cast(mets, C.c_void_p))
+            ids[rows] = sub_ids
         dt += time.perf_counter() - t0
         passes += 1
         if passes == 1:
@@ -170,18 +222,16 @@ def cpu_baseline(ix, plan, queries_np, variance, rots, probes, topk, nprobe, m, 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="mix1m-d960")
+    ap.add_argument("--steps", type=int, default=0, help="timed steps (0 = the workload's default)")
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--n", type=int, default=1_000_000)
-    ap.add_argument("--dim", type=int, default=960)
-    ap.add_argument("--nlist", type=int, default=4096)
-    ap.add_argument("--bits", type=float, default=0.0,
+    ap.add_argument("--n", type=int, default=0)
+    ap.add_argument("--nq", type=int, default=0)
+    ap.add_argument("--bits", type=float, default=-1.0,
                     help="SAQ bits per dimension (0 = the smallest of 4..8 reaching the target recall)")
-    ap.add_argument("--nq", type=int, default=1000)
     ap.add_argument("--topk", type=int, default=10)
     ap.add_argument("--target-recall", type=float, default=0.95)
     ap.add_argument("--nprobe", type=int, default=0, help="fixed nprobe (0 = smallest reaching the target)")
-    ap.add_argument("--kmeans-iters", type=int, default=8)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--exact-probe", action="store_true",
@@ -194,16 +244,21 @@ def main():
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+    W = dict(WORKLOADS[args.workload])
+    n = args.n or W["n"]
+    nq = args.nq or W["nq"]
+    steps = args.steps or W["steps"]
+    bits_arg = W["bits"] if args.bits < 0 else args.bits
 
     import torch
     if args.copy_calibration:
-        n = args.copy_calibration << 20
-        src = torch.ones(n // 4, dtype=torch.float32, device="cuda:0")
+        nb = args.copy_calibration << 20
+        src = torch.ones(nb // 4, dtype=torch.float32, device="cuda:0")
         dst = torch.empty_like(src)
         for _ in range(5):
             dst.copy_(src)
         torch.cuda.synchronize()
-        print(json.dumps({"copy_bytes_per_launch": n, "launches": 5}))
+        print(json.dumps({"copy_bytes_per_launch": nb, "launches": 5}))
         return
     import saq_amd
     from saq_amd.plan import make_plan
@@ -211,66 +266,82 @@ def main():
 
     dev = torch.device("cuda:0")
     t_all = time.time()
-    log(f"data: N={args.n} D={args.dim} queries={args.nq}")
-    base, queries = make_data(torch, args.n, args.dim, 256, 32, args.nq, 1, dev)
+    D, nlist = W["dim"], W["nlist"]
+    log(f"workload {args.workload}: N={n} D={D} nlist={nlist} queries={nq}")
+    base, queries = make_data(torch, n, D, W["comps"], W["tau"], W["zipf"], nq, 1, dev)
     mean, comps, var = pca(torch, base)
-    base = ((base - mean) @ comps).contiguous()
+    base = rotate_inplace(torch, base, mean, comps)
     queries = ((queries - mean) @ comps).contiguous()
     log("k-means")
-    cent, cid = kmeans(torch, base, args.nlist, args.kmeans_iters, 0)
-    sizes = torch.bincount(cid, minlength=args.nlist).cpu().numpy().astype(np.int64)
+    cent, cid = kmeans(torch, base, nlist, W["kmeans_iters"], 0)
+    sizes = torch.bincount(cid, minlength=nlist).cpu().numpy().astype(np.int64)
+    log(f"list sizes: mean {sizes.mean():.0f} max {sizes.max()} empty {(sizes == 0).sum()}")
+    log("ground truth")
     gt = exact_topk(torch, base, queries, args.topk)
     cid32 = cid.to(torch.int32).contiguous()
-    out_ids = torch.zeros(args.nq, args.topk, dtype=torch.int32, device=dev)
-    out_d = torch.zeros(args.nq, args.topk, dtype=torch.float32, device=dev)
-    met = torch.zeros(args.nq, 3, dtype=torch.int64, device=dev)
+    del cid
+    out_ids = torch.zeros(nq, args.topk, dtype=torch.int32, device=dev)
+    out_d = torch.zeros(nq, args.topk, dtype=torch.float32, device=dev)
+    met = torch.zeros(nq, 3, dtype=torch.int64, device=dev)
     stream = torch.cuda.current_stream().cuda_stream
+    npq_np = npq = None
+    if W["npq_range"]:
+        lo, hi = W["npq_range"]
+        npq_np = np.random.default_rng(3).integers(lo, hi + 1, nq).astype(np.int32)
+        npq = torch.from_numpy(npq_np).to(dev)
 
     def build(bits):
-        plan = make_plan(var, args.dim, bits)
+        plan = make_plan(var, D, bits)
         rots = [sdata.random_rotation(d, 1000 + i) for i, (d, _) in enumerate(plan)]
-        ix = saq_amd.Index(args.dim, plan, cent.cpu().numpy(), sizes, var, rotations=rots)
+        ix = saq_amd.Index(D, plan, cent.cpu().numpy(), sizes, var, rotations=rots)
         ix.encode(base, cid32)
         return ix, plan, rots
 
     def search(ix, nprobe):
-        ix.search(queries, args.topk, nprobe, out_ids=out_ids, out_dists=out_d, metrics=met, stream=stream)
+        ix.search(queries, args.topk, nprobe, out_ids=out_ids, out_dists=out_d, metrics=met, stream=stream,
+                  nprobe_per_query=npq)
 
     def recall_at(ix, nprobe):
         search(ix, nprobe)
         torch.cuda.synchronize()
         return recall(out_ids.cpu().numpy(), gt, args.topk)
 
-    # quantisation, not probing, limits recall on this mixture: pick the bits first (at a generous
-    # nprobe), then the smallest nprobe that keeps the target
+    # ---- bits: quantisation, not probing, limits recall on these mixtures
     bits_tried = {}
-    candidates = [args.bits] if args.bits else [4.0, 5.0, 6.0, 7.0, 8.0]
+    probe_check = W["npq_range"][1] if W["npq_range"] else 64
+    candidates = [bits_arg] if bits_arg else [4.0, 5.0, 6.0, 7.0, 8.0]
     ix = plan = rots = None
     for bits in candidates:
         if ix is not None:
             ix.close()
         ix, plan, rots = build(bits)
-        r = recall_at(ix, 64)
+        r = recall_at(ix, probe_check)
         bits_tried[bits] = round(r, 4)
-        log(f"B={bits} plan {plan}: recall@{args.topk} at nprobe 64 = {r:.4f}")
+        log(f"B={bits} plan {plan}: recall@{args.topk} at nprobe {'U' if npq is not None else probe_check} = {r:.4f}")
         if r >= args.target_recall:
             break
-    log(f"plan {plan}")
+    log(f"plan {plan}; index {ix.device_bytes() / 1e9:.2f} GB")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     nfail = ix.encode(base, cid32)
     torch.cuda.synchronize()
     t_enc = time.perf_counter() - t0
-    log(f"encode {args.n / t_enc / 1e6:.2f} Mvec/s ({t_enc * 1e3:.0f} ms), check failures {nfail}")
+    log(f"encode {n / t_enc / 1e6:.2f} Mvec/s ({t_enc * 1e3:.0f} ms), check failures {nfail}")
 
-    nprobe, rec = args.nprobe, None
-    sweep = [args.nprobe] if args.nprobe else [4, 6, 8, 12, 16, 24, 32, 48, 64, 96, 128]
-    for npb in sweep:
-        rec = recall_at(ix, npb)
-        log(f"nprobe {npb}: recall@{args.topk} {rec:.4f}")
-        nprobe = npb
-        if rec >= args.target_recall:
-            break
+    # ---- plain nprobe
+    if npq is not None:
+        nprobe = int(W["npq_range"][1])
+        sweep = [nprobe]
+        rec = bits_tried[bits]
+    else:
+        nprobe, rec = args.nprobe, None
+        sweep = [args.nprobe] if args.nprobe else [4, 6, 8, 12, 16, 24, 32, 48, 64, 96, 128]
+        for npb in sweep:
+            rec = recall_at(ix, npb)
+            log(f"nprobe {npb}: recall@{args.topk} {rec:.4f}")
+            nprobe = npb
+            if rec >= args.target_recall:
+                break
 
     def timed(fn):
         for _ in range(args.warmup):
@@ -278,7 +349,7 @@ def main():
         torch.cuda.synchronize()
         ix.timing(True)
         t0 = time.perf_counter()
-        for _ in range(args.steps):
+        for _ in range(steps):
             fn()
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
@@ -287,7 +358,7 @@ def main():
 
     # ---- plain scan (bit-exact with the reference)
     dt, kern_ms = timed(lambda: search(ix, nprobe))
-    qps = args.nq * args.steps / dt
+    qps = nq * steps / dt
     ids_gpu = out_ids.cpu().numpy().astype(np.uint32)
     m = met.cpu().numpy().astype(np.int64)
 
@@ -299,13 +370,12 @@ def main():
     per_refine = sum(d * (b - 1) // 8 for d, b in plan if b > 1)
     refines = m[:, 1].sum() // (per_refine * 8) if per_refine else 0  # full-refine equivalents
     d_pad = sum(d for d, _ in plan)
-    alg_bytes = blocks * block_stride + m[:, 1].sum() // 8 + refines * (8 * len(plan) + 4) + \
-        args.nq * nprobe * d_pad * 4
-    kern_s = kern_ms / 1e3
-    achieved = alg_bytes / kern_s / 1e9
-    traffic = read_traffic("scan_traffic.json", args.n, bits, nprobe)
-    plain = {"qps": round(qps, 1), "recall_at_10": round(rec, 4), "nprobe": nprobe,
-             "ms_per_step": round(dt / args.steps * 1e3, 4),
+    probed = int(npq_np.sum()) if npq_np is not None else nq * nprobe
+    alg_bytes = blocks * block_stride + m[:, 1].sum() // 8 + refines * (8 * len(plan) + 4) + probed * d_pad * 4
+    achieved = alg_bytes / (kern_ms / 1e3) / 1e9
+    traffic = read_traffic("scan_traffic.json", n, bits, nprobe)
+    plain = {"qps": round(qps, 1), "recall_at_10": round(rec, 4), "nprobe": nprobe if npq is None else "U" + str(
+        W["npq_range"]), "ms_per_step": round(dt / steps * 1e3, 4),
              "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                           "alg_bytes_per_launch": int(alg_bytes), "kernel_ms": round(kern_ms, 4)}}
@@ -317,26 +387,28 @@ def main():
     ix.prepare_batched()
     torch.cuda.synchronize()
     t_prep = time.perf_counter() - t0
-    bids = torch.zeros(args.nq, args.topk, dtype=torch.int32, device=dev)
-    bd = torch.zeros(args.nq, args.topk, dtype=torch.float32, device=dev)
+    bids = torch.zeros(nq, args.topk, dtype=torch.int32, device=dev)
+    bd = torch.zeros(nq, args.topk, dtype=torch.float32, device=dev)
 
     def bsearch(npb, prec, qt=0):
         ix.search_batched(queries, args.topk, npb, precision=prec, query_tile=qt, out_ids=bids, out_dists=bd,
-                          stream=stream, exact_probe=args.exact_probe)
+                          stream=stream, exact_probe=args.exact_probe, nprobe_per_query=npq)
 
     def brecall(npb, prec):
         bsearch(npb, prec)
         torch.cuda.synchronize()
         return recall(bids.cpu().numpy(), gt, args.topk)
 
-    choices = []
+    choices, best_seen = [], None
     for prec in (saq_amd.PREC_INT8, saq_amd.PREC_INT8X2):
         for npb in [x for x in sweep if x <= max(nprobe * 2, nprobe)]:
             r = brecall(npb, prec)
             log(f"batched prec={prec} nprobe {npb}: recall@{args.topk} {r:.4f}")
+            if best_seen is None or r > best_seen[0]:
+                best_seen = (r, prec, npb)
             if r >= rec:
                 qt = ix.autotune_batched(queries, args.topk, npb, precision=prec, stream=stream,
-                                         exact_probe=args.exact_probe)
+                                         exact_probe=args.exact_probe, nprobe_per_query=npq)
                 torch.cuda.synchronize()
                 t0 = time.perf_counter()
                 for _ in range(3):
@@ -344,16 +416,22 @@ def main():
                 torch.cuda.synchronize()
                 choices.append(((time.perf_counter() - t0) / 3, prec, npb, qt, r))
                 break
-    if not choices:
-        raise SystemExit("batched scan did not reach the plain scan's recall")
+    below = not choices
+    if below:  # report the most accurate configuration, flagged
+        r, prec, npb = best_seen
+        qt = ix.autotune_batched(queries, args.topk, npb, precision=prec, stream=stream,
+                                 exact_probe=args.exact_probe, nprobe_per_query=npq)
+        choices.append((0.0, prec, npb, qt, r))
     _, bprec, bnp, bqt, brec = min(choices)
     bdt, bkern_ms = timed(lambda: bsearch(bnp, bprec, bqt))
-    bqps = args.nq * args.steps / bdt
+    bqps = nq * steps / bdt
     # algorithmic work of one tile-kernel launch: per work tile (a list and up to bqt of its
     # queries) one pass over the list's 16-vector records; the residual queries (q_rot, c_rot)
     # and the per-pair candidate lists; MFMA ops = 2 * K per (query, vector) pair and term
     pr = ix.probe(queries.cpu().numpy(), bnp).astype(np.int64)  # exact lists; the fast ones differ only at ties
-    cnt = np.bincount(pr.ravel(), minlength=args.nlist)
+    if npq_np is not None:
+        pr = np.concatenate([pr[i, :npq_np[i]] for i in range(nq)])
+    cnt = np.bincount(pr.ravel(), minlength=nlist)
     nmb = (sizes + 15) // 16
     kq_bytes = sum((512 if b <= 4 else (1024 if b <= 8 else 2048)) * (d // 64) for d, b in plan if b > 0)
     nqs = sum(1 for _, b in plan if b > 0)
@@ -364,28 +442,33 @@ def main():
     b_ops = float((cnt * sizes).sum()) * 2 * k_eff * (2 if bprec == saq_amd.PREC_INT8X2 else 1)
     b_gbs = b_bytes / (bkern_ms / 1e3) / 1e9
     b_tops = b_ops / (bkern_ms / 1e3) / 1e12
-    batched = {"qps": round(bqps, 1), "recall_at_10": round(brec, 4), "nprobe": bnp,
-               "precision": "int8x2" if bprec == saq_amd.PREC_INT8X2 else "int8", "query_tile": bqt,
-               "coarse_quantiser": "exact (reference order)" if args.exact_probe else "fp32 MFMA",
-               "ms_per_step": round(bdt / args.steps * 1e3, 4), "layout_build_ms": round(t_prep * 1e3, 1),
+    prec_name = "int8x2" if bprec == saq_amd.PREC_INT8X2 else "int8"
+    batched = {"qps": round(bqps, 1), "recall_at_10": round(brec, 4),
+               "nprobe": bnp if npq is None else "U" + str(W["npq_range"]), "precision": prec_name,
+               "query_tile": bqt, "coarse_quantiser": "exact (reference order)" if args.exact_probe else "fp32 MFMA",
+               "recall_below_plain": below,
+               "ms_per_step": round(bdt / steps * 1e3, 4), "layout_build_ms": round(t_prep * 1e3, 1),
                "kernel_ms": round(bkern_ms, 4), "alg_bytes_per_launch": b_bytes,
                "hbm_frac": round(b_gbs / HBM_PEAK_GBS, 4), "int8_mfma_tops": round(b_tops, 2),
                "int8_mfma_frac": round(b_tops / INT8_PEAK_TOPS, 5)}
     log(f"batched: {bqps:.0f} QPS ({bqps / qps:.2f}x plain), tile kernel {bkern_ms:.3f} ms, {b_gbs:.0f} GB/s, "
         f"{b_tops:.1f} TOPS")
-    btraffic = read_traffic("bscan_traffic.json", args.n, bits, bnp, precision=batched["precision"], query_tile=bqt)
+    btraffic = read_traffic("bscan_traffic.json", n, bits, bnp, precision=prec_name, query_tile=bqt)
 
     cpu = None
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        probes = ix.probe(queries.cpu().numpy(), nprobe)
+        sample = np.arange(W["cpu_queries"] or nq)
+        qs_np = queries.cpu().numpy()[sample]
+        probes = ix.probe(qs_np, nprobe)
         try:
-            cq, cdt, passes, mism = cpu_baseline(ix, plan, queries.cpu().numpy(), var, rots, probes, args.topk,
-                                                 nprobe, 4.0, ids_gpu, threads)
+            cq, cdt, passes, mism = cpu_baseline(ix, plan, qs_np, var, rots, probes,
+                                                 npq_np[sample] if npq_np is not None else None, args.topk, 4.0,
+                                                 ids_gpu[sample], threads)
             cpu = {"value": round(cq, 1), "unit": "queries/s", "cores": threads, "kind": "port",
-                   "sample": f"{passes} passes over all {args.nq} queries at nprobe={nprobe} with the GPU's probe "
-                             f"lists (oracle: scalar C++, one query per thread), {cdt:.1f}s; queries whose IDs "
-                             f"differ from the GPU: {mism}"}
+                   "sample": f"{passes} passes over {len(sample)} of the {nq} queries with the GPU's probe lists "
+                             f"(oracle: scalar C++, one query per thread), {cdt:.1f}s; queries whose IDs differ "
+                             f"from the plain GPU scan: {mism}"}
         except Exception as e:  # the baseline is reported, never the measured path
             log(f"cpu baseline failed: {e}")
 
@@ -394,19 +477,19 @@ def main():
         "value": round(bqps, 1),
         "unit": "queries/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": steps,
         "warmup": args.warmup,
-        "ms_per_step": round(bdt / args.steps * 1e3, 4),
+        "ms_per_step": round(bdt / steps * 1e3, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "i8-mfma/i32/f32",
         "data": "synthetic",
-        "config": {"workload": f"batched SAQ IVF scan vs plain GPU scan: N={args.n} D={args.dim} "
-                               f"nlist={args.nlist} B={bits} nq={args.nq} k={args.topk}",
-                   "bits": bits, "plan": plan, "recall_at_10_by_bits_nprobe64": bits_tried,
+        "config": {"workload": f"{args.workload}: batched SAQ IVF scan vs plain GPU scan, N={n} D={D} "
+                               f"nlist={nlist} B={bits} nq={nq} k={args.topk}",
+                   "bits": bits, "plan": plan, "recall_at_10_by_bits": bits_tried,
                    "batched": batched, "plain": plain, "batched_vs_plain": round(bqps / qps, 3),
-                   "encode_mvec_per_s": round(args.n / t_enc / 1e6, 3)},
+                   "encode_mvec_per_s": round(n / t_enc / 1e6, 3)},
         "roofline": {"bound": "hbm", "achieved": round(b_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(b_gbs / HBM_PEAK_GBS, 4), "traffic": btraffic,
                      "kernel": "bscan_kernel", "alg_bytes_per_launch": b_bytes,

@@ -73,9 +73,13 @@ typedef struct {
 /* Search parameters (IVF::search arguments + SearcherConfig, ivf.hpp:91-94, config.h:75-78). */
 typedef struct {
     int32_t topk;
-    int32_t nprobe;
+    int32_t nprobe;    /* lists per query (the maximum, and the probe-list stride, when
+                          nprobe_per_query is set) */
     float searcher_vars_bound_m;
     int32_t dist_type; /* 0 = L2Sqr, 1 = IP (reproduces the reference's behaviour) */
+    const int32_t *nprobe_per_query; /* NULL, or nq entries in 1..nprobe, in the queries' memory
+                                        kind: query i scans the first entries of its list, as
+                                        IVF::search(query_i, topk, nprobe_per_query[i], ...) */
 } saqgpu_search_params;
 
 /* Per-query counters of QueryRuntimeMetrics (saqlib/quantization/caq/caq_estimator.hpp:21-25). */

@@ -42,7 +42,7 @@ class CaqCfg(C.Structure):
 
 class SearchParams(C.Structure):
     _fields_ = [("topk", C.c_int32), ("nprobe", C.c_int32), ("searcher_vars_bound_m", C.c_float),
-                ("dist_type", C.c_int32)]
+                ("dist_type", C.c_int32), ("nprobe_per_query", C.c_void_p)]
 
 
 class BatchedParams(C.Structure):
@@ -216,11 +216,14 @@ class Index:
         return out
 
     def search(self, queries, topk, nprobe, probes=None, m=4.0, dist_type=0, out_ids=None, out_dists=None,
-               metrics=None, stream=None, want_dists=True, want_metrics=True):
+               metrics=None, stream=None, want_dists=True, want_metrics=True, nprobe_per_query=None):
         """IVF::search for a batch (queries: nq x num_dim).  Returns (ids, dists, metrics)."""
         dev = not isinstance(queries, np.ndarray)
         nq = int(queries.shape[0])
-        p = SearchParams(topk, nprobe, m, dist_type)
+        npq = nprobe_per_query
+        if npq is not None and not dev:
+            npq = np.ascontiguousarray(npq, np.int32)
+        p = SearchParams(topk, nprobe, m, dist_type, _ptr(npq))
         if not dev:
             queries = np.ascontiguousarray(queries, np.float32)
             if probes is not None:

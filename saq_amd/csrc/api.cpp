@@ -23,7 +23,8 @@ hipError_t launch_query_prep(const IndexView &ix, const float *queries, int64_t 
                              const float *rot, const int64_t *rot_off, float m, float *qrot, float *bound,
                              int max_dim, hipStream_t st);
 hipError_t launch_scan(const IndexView &ix, const float *qrot, const float *bound, const uint32_t *probes,
-                       int nprobe, int topk, int64_t nq, int dist_type, uint32_t *out_ids, float *out_dists,
+                       int nprobe, const int32_t *npq, int topk, int64_t nq, int dist_type, uint32_t *out_ids,
+                       float *out_dists,
                        uint64_t *out_metrics, size_t smem, hipStream_t st, unsigned long long *prof, int cmax);
 size_t scan_smem_bytes(const IndexView &ix, int topk, int cmax);
 hipError_t launch_probe(const float *centroids, int K, int D, const float *queries, int64_t nq, int nprobe,
@@ -579,6 +580,10 @@ int saqgpu_search(saqgpu_index *ix, const float *queries, int64_t nq, const uint
     if (probes && !dev)
         for (int64_t i = 0; i < nq * p->nprobe; ++i)
             if (probes[i] >= (uint32_t)ix->num_cen) return fail(SAQGPU_EINVAL, "probe id out of range");
+    if (p->nprobe_per_query && !dev)
+        for (int64_t i = 0; i < nq; ++i)
+            if (p->nprobe_per_query[i] < 1 || p->nprobe_per_query[i] > p->nprobe)
+                return fail(SAQGPU_EINVAL, "nprobe_per_query entries must be in 1..nprobe");
     std::lock_guard<std::mutex> lk(ix->mu);
     HIPCHK(hipSetDevice(ix->device));
     hipStream_t st = (hipStream_t)stream;
@@ -599,11 +604,13 @@ int saqgpu_search(saqgpu_index *ix, const float *queries, int64_t nq, const uint
         rc |= ix->ws_ids.ensure(nq * p->topk);
         rc |= ix->ws_dists.ensure(nq * p->topk);
         rc |= ix->ws_met.ensure(nq * 3);
+        if (p->nprobe_per_query) rc |= ix->bq_npq.ensure(nq);
     }
     if (!probes || !dev) rc |= ix->ws_probes.ensure(nq * p->nprobe);
     if (rc) return fail(SAQGPU_ENOMEM, "workspace allocation failed");
     const float *dq = queries;
     const uint32_t *dp = probes;
+    const int32_t *dnpq = p->nprobe_per_query;
     uint32_t *dids = out_ids;
     float *dd = out_dists;
     uint64_t *dm = reinterpret_cast<uint64_t *>(metrics);
@@ -617,6 +624,10 @@ int saqgpu_search(saqgpu_index *ix, const float *queries, int64_t nq, const uint
         dids = ix->ws_ids.p;
         dd = out_dists ? ix->ws_dists.p : nullptr;
         dm = metrics ? ix->ws_met.p : nullptr;
+        if (dnpq) {
+            HIPCHK(hipMemcpyAsync(ix->bq_npq.p, p->nprobe_per_query, nq * 4, hipMemcpyHostToDevice, st));
+            dnpq = ix->bq_npq.p;
+        }
     }
     if (!probes) {
         if (ix->ws_cdist.ensure((size_t)nq * ix->num_cen)) return fail(SAQGPU_ENOMEM, "workspace allocation failed");
@@ -636,7 +647,7 @@ int saqgpu_search(saqgpu_index *ix, const float *queries, int64_t nq, const uint
     static const bool prof_on = getenv("SAQGPU_PROFILE") != nullptr;
     DevBuf<unsigned long long> prof;
     if (prof_on && prof.ensure(nq * 8)) return fail(SAQGPU_ENOMEM, "allocation failed");
-    HIPCHK(launch_scan(v, ix->ws_qrot.p, ix->ws_bound.p, dp, p->nprobe, p->topk, nq, p->dist_type, dids, dd, dm,
+    HIPCHK(launch_scan(v, ix->ws_qrot.p, ix->ws_bound.p, dp, p->nprobe, dnpq, p->topk, nq, p->dist_type, dids, dd, dm,
                        smem, st, prof_on ? prof.p : nullptr, cmax));
     if (prof_on) {
         std::vector<unsigned long long> h(nq * 8);

@@ -142,7 +142,8 @@ struct ScanArgs {
     const float *qrot;
     const float *bound;
     const uint32_t *probes;
-    int32_t nprobe;
+    int32_t nprobe;          // probe-list stride
+    const int32_t *npq;      // lists scanned per query (nullptr: nprobe)
     int32_t topk;
     int64_t nq;
     uint32_t *out_ids;
@@ -402,8 +403,9 @@ __global__ __launch_bounds__(kThreads) void scan_kernel(ScanArgs a) {
     const int ncb = NG * 16;
     const int v = tid, bl = v >> 5, ln = v & 31;
 
+    const int NPq = a.npq ? a.npq[q] : a.nprobe;
     auto next_valid = [&](int i) {
-        for (; i < a.nprobe; ++i) {
+        for (; i < NPq; ++i) {
             const uint32_t c = a.probes[q * a.nprobe + i];
             if (c < (uint32_t)ix.num_cen && ix.clu_n[c] > 0) break;  // bad ids are rejected on the host
         }
@@ -422,12 +424,12 @@ __global__ __launch_bounds__(kThreads) void scan_kernel(ScanArgs a) {
     };
 
     int pi = next_valid(0);
-    if (pi < a.nprobe) {
+    if (pi < NPq) {
         const uint32_t c0 = a.probes[q * a.nprobe + pi];
         fetch_crot(c0);
         fetch_step(c0, 0);
     }
-    while (pi < a.nprobe) {
+    while (pi < NPq) {
         const uint32_t cid = a.probes[q * a.nprobe + pi];
         const int n = ix.clu_n[cid];
         const int nb = (n + kBlock - 1) / kBlock;
@@ -621,7 +623,7 @@ __global__ __launch_bounds__(kThreads) void scan_kernel(ScanArgs a) {
             // the step's block records are consumed: fetch the next work unit's
             if (st + kStepBlocks < nb) {
                 fetch_step(cid, st + kStepBlocks);
-            } else if (pn < a.nprobe) {
+            } else if (pn < NPq) {
                 const uint32_t nc = a.probes[q * a.nprobe + pn];
                 fetch_crot(nc);
                 fetch_step(nc, 0);
@@ -900,9 +902,10 @@ hipError_t launch_query_prep(const IndexView &ix, const float *queries, int64_t 
 }
 
 hipError_t launch_scan(const IndexView &ix, const float *qrot, const float *bound, const uint32_t *probes,
-                       int nprobe, int topk, int64_t nq, int dist_type, uint32_t *out_ids, float *out_dists,
+                       int nprobe, const int32_t *npq, int topk, int64_t nq, int dist_type, uint32_t *out_ids,
+                       float *out_dists,
                        uint64_t *out_metrics, size_t smem, hipStream_t st, unsigned long long *prof, int cmax) {
-    ScanArgs a{ix, qrot, bound, probes, nprobe, topk, nq, out_ids, out_dists, out_metrics, prof, cmax};
+    ScanArgs a{ix, qrot, bound, probes, nprobe, npq, topk, nq, out_ids, out_dists, out_metrics, prof, cmax};
     if (ix.ngroups > 32 || ix.nseg > 32 || cmax > 64 || cmax < 1) return hipErrorInvalidValue;
     const void *fn = dist_type == 1 ? (const void *)scan_kernel<1> : (const void *)scan_kernel<0>;
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);

@@ -77,6 +77,25 @@ def test_topk_exceeds_probed_vectors():
     run_case(case, 1024, 2)
 
 
+def test_nprobe_per_query():
+    """Query i scans the first nprobe_per_query[i] lists of its probe list: the same as an
+    IVF::search call with that nprobe."""
+    import dataclasses
+    case = F.make_case(n=3000, d=128, K=16, nq=40, plan=[(64, 5), (64, 3)], seed=21)
+    probes = F.orc_probes(case, 8)
+    npq = np.random.default_rng(5).integers(1, 9, 40).astype(np.int32)
+    ix = F.gpu_index(case)
+    ids, d, met = ix.search(case.queries, 10, 8, probes=probes, nprobe_per_query=npq)
+    for v in np.unique(npq):
+        rows = np.nonzero(npq == v)[0]
+        sub = dataclasses.replace(case, queries=case.queries[rows])
+        ids_o, d_o, met_o = F.orc_search(sub, probes[rows, :v], 10)
+        assert np.array_equal(ids[rows], ids_o), f"nprobe {v}"
+        assert np.array_equal(d[rows].view(np.uint32), d_o.view(np.uint32))
+        assert np.array_equal(met[rows], met_o)
+    ix.close()
+
+
 def test_gpu_probe_matches_oracle():
     case = F.make_case(n=3000, d=200, K=40, nq=32, plan=[(128, 4), (128, 4)], seed=17)
     ix = F.gpu_index(case)
