@@ -137,9 +137,10 @@ int saqgpu_search(saqgpu_index *idx, const float *queries, int64_t nq, const uin
                   const saqgpu_search_params *params, uint32_t *out_ids, float *out_dists,
                   saqgpu_metrics *metrics, int32_t flags, void *stream);
 
-/* Replaces IVF::estimate (ivf.hpp:313-362) for one query: accurate, fast and variance
- * estimates of every vector of the probed clusters, in probe order; returns the count
- * through *n_out. */
+/* Replaces IVF::estimate (ivf.hpp:313-362) for one query (host pointers): accurate, fast and
+ * variance estimates of every vector of the probed clusters, in probe order; returns the count
+ * through *n_out.  probes may be NULL (the GPU coarse quantiser in the reference's order); the
+ * outputs must hold the summed sizes of the probed clusters; out_fast/out_vars may be NULL. */
 int saqgpu_estimate(saqgpu_index *idx, const float *query, const uint32_t *probes, int32_t nprobe,
                     float vars_bound_m, int32_t dist_type, uint32_t *out_ids, float *out_acc,
                     float *out_fast, float *out_vars, int64_t *n_out);

@@ -282,6 +282,23 @@ class Index:
         del keep
         return best.value
 
+    def estimate(self, query, nprobe, probes=None, m=4.0, dist_type=0):
+        """IVF::estimate for one query (host array of num_dim floats): every vector of the
+        nprobe probed lists in probe order.  Returns (ids, acc, fast, vars)."""
+        query = np.ascontiguousarray(query, np.float32).ravel()
+        if probes is None:
+            probes = self.probe(query[None, :], nprobe, dist_type=dist_type)[0]
+        probes = np.ascontiguousarray(probes, np.uint32)[:nprobe]
+        total = int(sum(int(self.cluster_sizes[c]) for c in probes))
+        ids = np.zeros(max(1, total), np.uint32)
+        acc, fast, var = (np.zeros(max(1, total), np.float32) for _ in range(3))
+        n = C.c_int64(0)
+        _check(lib().saqgpu_estimate(self.h, query.ctypes.data, probes.ctypes.data, nprobe, m, dist_type,
+                                     ids.ctypes.data, acc.ctypes.data, fast.ctypes.data, var.ctypes.data,
+                                     C.byref(n)))
+        k = n.value
+        return ids[:k], acc[:k], fast[:k], var[:k]
+
     def timing(self, enable):
         """Starts (enable=True) or stops event timing of the scan kernel; returns (ms, launches)
         recorded since the last call."""

@@ -30,6 +30,10 @@ size_t scan_smem_bytes(const IndexView &ix, int topk, int cmax);
 hipError_t launch_probe(const float *centroids, int K, int D, const float *queries, int64_t nq, int nprobe,
                         int dist_type, float *dist_ws, uint32_t *probes, hipStream_t st);
 hipError_t launch_sqnorm_rows(const float *x, int64_t rows, int D, float *out, hipStream_t st);
+size_t estimate_smem_bytes(const IndexView &ix);
+hipError_t launch_estimate(const IndexView &ix, const float *qrot, const float *bound, const uint32_t *probes,
+                           int nprobe, const int64_t *offs, int dist_type, uint32_t *out_ids, float *out_acc,
+                           float *out_fast, float *out_vars, hipStream_t st);
 hipError_t launch_probe_fast(const float *centroids, const float *cnorm, int K, int D, const float *queries,
                              int64_t nq, int nprobe, int dist_type, float *qnorm, float *dist_ws, uint32_t *probes,
                              hipStream_t st);
@@ -84,6 +88,14 @@ struct DevBuf {
         n = 0;
     }
 };
+
+template <class T>
+int upload(DevBuf<T> &b, const std::vector<T> &v) {
+    if (b.ensure(std::max<size_t>(1, v.size()))) return fail(SAQGPU_ENOMEM, "allocation failed");
+    if (!v.empty() && hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+        return fail(SAQGPU_EDEVICE, "upload failed");
+    return SAQGPU_OK;
+}
 }  // namespace
 
 struct saqgpu_index {
@@ -674,9 +686,64 @@ int saqgpu_search(saqgpu_index *ix, const float *queries, int64_t nq, const uint
     return SAQGPU_OK;
 }
 
-int saqgpu_estimate(saqgpu_index *, const float *, const uint32_t *, int32_t, float, int32_t, uint32_t *, float *,
-                    float *, float *, int64_t *) {
-    return fail(SAQGPU_EUNSUPPORTED, "estimate is not implemented yet");
+int saqgpu_estimate(saqgpu_index *ix, const float *query, const uint32_t *probes, int32_t nprobe, float vars_bound_m,
+                    int32_t dist_type, uint32_t *out_ids, float *out_acc, float *out_fast, float *out_vars,
+                    int64_t *n_out) {
+    if (!ix || !query || !out_ids || !out_acc) return fail(SAQGPU_EINVAL, "null argument");
+    if (nprobe < 1 || nprobe > ix->num_cen) return fail(SAQGPU_EINVAL, "nprobe must be in 1..K");
+    if (dist_type != 0 && dist_type != 1) return fail(SAQGPU_EINVAL, "dist_type must be 0 (L2) or 1 (IP)");
+    for (int c = 0; c < ix->num_cen; ++c)
+        if (!ix->clu_set[c] && ix->clu_n[c] > 0) return fail(SAQGPU_ESTATE, "index has clusters not yet set");
+    if (probes)
+        for (int i = 0; i < nprobe; ++i)
+            if (probes[i] >= (uint32_t)ix->num_cen) return fail(SAQGPU_EINVAL, "probe id out of range");
+    std::lock_guard<std::mutex> lk(ix->mu);
+    HIPCHK(hipSetDevice(ix->device));
+    const size_t smem = estimate_smem_bytes(ix->view());
+    int max_smem = 0;
+    HIPCHK(hipDeviceGetAttribute(&max_smem, hipDeviceAttributeMaxSharedMemoryPerBlock, ix->device));
+    if ((int64_t)smem > max_smem) return fail(SAQGPU_EUNSUPPORTED, "plan needs more LDS than a workgroup has");
+    int rc = 0;
+    rc |= ix->ws_q.ensure(ix->num_dim);
+    rc |= ix->ws_qrot.ensure(ix->D_pad);
+    rc |= ix->ws_bound.ensure(ix->nseg);
+    rc |= ix->ws_probes.ensure(nprobe);
+    rc |= ix->ws_cdist.ensure(ix->num_cen);
+    if (rc) return fail(SAQGPU_ENOMEM, "workspace allocation failed");
+    HIPCHK(hipMemcpy(ix->ws_q.p, query, ix->num_dim * 4, hipMemcpyHostToDevice));
+    std::vector<uint32_t> pr(nprobe);
+    if (probes) {
+        std::copy(probes, probes + nprobe, pr.begin());
+        HIPCHK(hipMemcpy(ix->ws_probes.p, probes, nprobe * 4, hipMemcpyHostToDevice));
+    } else {  // FlatInitializer::centroids_distances, as IVF::estimate does (ivf.hpp:321-322)
+        HIPCHK(launch_probe(ix->centroids.p, ix->num_cen, ix->num_dim, ix->ws_q.p, 1, nprobe, dist_type, ix->ws_cdist.p,
+                            ix->ws_probes.p, 0));
+        HIPCHK(hipMemcpy(pr.data(), ix->ws_probes.p, nprobe * 4, hipMemcpyDeviceToHost));
+    }
+    std::vector<int64_t> offs(nprobe);
+    int64_t tot = 0;
+    for (int i = 0; i < nprobe; ++i) {
+        offs[i] = tot;
+        tot += ix->clu_n[pr[i]];
+    }
+    DevBuf<int64_t> d_offs;
+    DevBuf<uint32_t> d_ids;
+    DevBuf<float> d_out;
+    if ((rc = upload(d_offs, offs))) return rc;
+    if (d_ids.ensure(std::max<int64_t>(1, tot)) || d_out.ensure(3 * std::max<int64_t>(1, tot)))
+        return fail(SAQGPU_ENOMEM, "allocation failed");
+    HIPCHK(launch_query_prep(ix->view(), ix->ws_q.p, 1, ix->variance.p, ix->has_rot ? ix->rot.p : nullptr,
+                             ix->d_rot_off.p, vars_bound_m, ix->ws_qrot.p, ix->ws_bound.p, ix->max_dim, 0));
+    HIPCHK(launch_estimate(ix->view(), ix->ws_qrot.p, ix->ws_bound.p, ix->ws_probes.p, nprobe, d_offs.p, dist_type,
+                           d_ids.p, d_out.p, d_out.p + tot, d_out.p + 2 * tot, 0));
+    if (tot) {
+        HIPCHK(hipMemcpy(out_ids, d_ids.p, tot * 4, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(out_acc, d_out.p, tot * 4, hipMemcpyDeviceToHost));
+        if (out_fast) HIPCHK(hipMemcpy(out_fast, d_out.p + tot, tot * 4, hipMemcpyDeviceToHost));
+        if (out_vars) HIPCHK(hipMemcpy(out_vars, d_out.p + 2 * tot, tot * 4, hipMemcpyDeviceToHost));
+    }
+    if (n_out) *n_out = tot;
+    return SAQGPU_OK;
 }
 
 }  // extern "C"
@@ -716,13 +783,6 @@ int make_slots(const saqgpu_index *ix, const uint32_t *cid, int64_t n, Slots &sl
     return SAQGPU_OK;
 }
 
-template <class T>
-int upload(DevBuf<T> &b, const std::vector<T> &v) {
-    if (b.ensure(std::max<size_t>(1, v.size()))) return fail(SAQGPU_ENOMEM, "allocation failed");
-    if (!v.empty() && hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
-        return fail(SAQGPU_EDEVICE, "upload failed");
-    return SAQGPU_OK;
-}
 }  // namespace
 
 extern "C" {

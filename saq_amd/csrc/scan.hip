@@ -27,6 +27,7 @@
 #include <cfloat>
 
 #include "common.hpp"
+#include "estimators.hpp"
 
 #pragma clang fp contract(off)
 
@@ -34,7 +35,6 @@ namespace saqgpu {
 
 constexpr int kThreads = 256;
 constexpr int kStepBlocks = 8;            // 256 vectors per step
-constexpr int kPos[16] = {3, 3, 2, 3, 1, 3, 2, 3, 0, 3, 2, 3, 1, 3, 2, 3};  // fastscan.hpp:22-39
 
 struct PrepArgs {
     IndexView ix;
@@ -153,11 +153,6 @@ struct ScanArgs {
     int32_t cmax;              // candidates per refine round (bounded by LDS for their records)
 };
 
-// Per-segment scalars kept in LDS for the current (query, cluster).
-enum {
-    SC_DELTA = 0, SC_SUMVL, SC_SUMQCB, SC_QL2, SC_SUMQ, SC_VFAC, SC_IPQC, SC_VL, SC_INV, SC_N = 12
-};
-
 // control block indices
 enum { C_PSIZE = 0, C_POS, C_MASK, C_CURSOR, C_N = 8 };
 
@@ -239,103 +234,6 @@ __device__ __forceinline__ void dma_to_lds(const uint8_t *src, uint8_t *dst, int
 
 // Waits for this wavefront's outstanding vector-memory operations (LDS DMA included).
 __device__ __forceinline__ void dma_wait() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
-
-// Code of dimension 16 i + gl in the reference's compacted layout K (1..8)
-// (code_helper.hpp:95-591); each layout keeps the dims = gl (mod 16) of lane gl together.
-template <int K>
-__device__ __forceinline__ int lane_code(const uint8_t *y, int i, int gl) {
-    if (K == 1) return (y[2 * i + (gl >> 3)] >> (gl & 7)) & 1;
-    if (K == 2) return (y[16 * (i >> 2) + gl] >> (2 * (i & 3))) & 3;
-    if (K == 3) {
-        const int c = i >> 2, p = i & 3;
-        return ((y[24 * c + gl] >> (2 * p)) & 3) | (((y[24 * c + 16 + (gl & 7)] >> (2 * p + (gl >> 3))) & 1) << 2);
-    }
-    if (K == 4) {
-        const int b = y[16 * (i >> 1) + gl];
-        return (i & 1) ? (b >> 4) : (b & 15);
-    }
-    if (K == 6 || K == 7) {
-        const int c = i >> 2, p = i & 3;
-        const int base = (K == 6 ? 48 : 56) * c;
-        int code;
-        if (p == 0) code = y[base + gl] & 0x3F;
-        else if (p == 1) code = y[base + 16 + gl] & 0x3F;
-        else if (p == 2) code = ((y[base + gl] >> 2) & 0x30) | (y[base + 32 + gl] & 0x0F);
-        else code = ((y[base + 16 + gl] >> 2) & 0x30) | ((y[base + 32 + gl] >> 4) & 0x0F);
-        if (K == 7) code |= ((y[base + 48 + (gl & 7)] >> (2 * p + (gl >> 3))) & 1) << 6;
-        return code;
-    }
-    return y[16 * i + gl];  // K == 8
-}
-
-// Lane accumulator of CodeHelper<K>::compute_ip (code_helper.hpp:105-591) for lane gl of a
-// 16-lane group: dims gl, gl+16, gl+32, ... in increasing order.  Codes and query values are
-// read eight dims ahead so the LDS reads overlap; the FMA chain keeps the reference's order
-// (a masked add for K = 1).
-template <int K>
-__device__ __forceinline__ float lane_acc_k(const float *r, const uint8_t *y, int D, int gl) {
-    float acc = 0.0f;
-    const int n = D / 16;  // a multiple of 4
-    int i0 = 0;
-    for (; i0 + 8 <= n; i0 += 8) {
-        int c[8];
-        float x[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            c[u] = lane_code<K>(y, i0 + u, gl);
-            x[u] = r[16 * (i0 + u) + gl];
-        }
-#pragma unroll
-        for (int u = 0; u < 8; ++u) acc = K == 1 ? acc + (c[u] ? x[u] : 0.0f) : __fmaf_rn(x[u], (float)c[u], acc);
-    }
-    for (; i0 < n; i0 += 4) {
-        int c[4];
-        float x[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            c[u] = lane_code<K>(y, i0 + u, gl);
-            x[u] = r[16 * (i0 + u) + gl];
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) acc = K == 1 ? acc + (c[u] ? x[u] : 0.0f) : __fmaf_rn(x[u], (float)c[u], acc);
-    }
-    return acc;
-}
-
-__device__ __forceinline__ float lane_acc(int k, const float *r, const uint8_t *y, int D, int gl) {
-    switch (k) {
-    case 1: return lane_acc_k<1>(r, y, D, gl);
-    case 2: return lane_acc_k<2>(r, y, D, gl);
-    case 3: return lane_acc_k<3>(r, y, D, gl);
-    case 4: return lane_acc_k<4>(r, y, D, gl);
-    case 6: return lane_acc_k<6>(r, y, D, gl);
-    case 7: return lane_acc_k<7>(r, y, D, gl);
-    case 8: return lane_acc_k<8>(r, y, D, gl);
-    default: return 0.0f;
-    }
-}
-
-// CodeHelper<k>::compute_ip for k = 1..12 by a 16-lane group; valid in every lane of it.
-__device__ __forceinline__ float ex_ip_group(int k, const float *r, const uint8_t *y, int D, int gl) {
-    if (k == 5) {  // 2 * IP4 + IP1 (code_helper.hpp:372-374)
-        const float a = tree16_xor(lane_acc(4, r, y, D, gl));
-        const float b = tree16_xor(lane_acc(1, r, y + D * 4 / 8, D, gl));
-        return 2 * a + b;
-    }
-    if (k > 8) {  // IP8 + 256 * IP_{k-8} (code_helper.hpp:85-88)
-        const float lo = tree16_xor(lane_acc(8, r, y, D, gl));
-        float hi;
-        if (k - 8 == 5) {
-            const float a = tree16_xor(lane_acc(4, r, y + D, D, gl));
-            const float b = tree16_xor(lane_acc(1, r, y + D + D * 4 / 8, D, gl));
-            hi = 2 * a + b;
-        } else {
-            hi = tree16_xor(lane_acc(k - 8, r, y + D, D, gl));
-        }
-        return lo + 256 * hi;
-    }
-    return tree16_xor(lane_acc(k, r, y, D, gl));
-}
 
 template <int DT>
 __device__ __forceinline__ float pool_distk(const Smem &m, int K) {
@@ -440,84 +338,7 @@ __global__ __launch_bounds__(kThreads) void scan_kernel(ScanArgs a) {
         mark(0);
 
         // ---- CaqCluEstimator::prepare for every segment (caq_estimator.hpp:94-106)
-        for (int d = tid; d < ix.D_pad; d += kThreads) m.r[d] = DT == 1 ? m.qr[d] : m.qr[d] - m.crbuf[d];
-        __syncthreads();
-        for (int s = wave; s < S; s += 4) {
-            const SegDesc sd = m.segs[s];
-            const float *rs = m.r + sd.doff;
-            const float *cs = m.crbuf + sd.doff;
-            const float sum_q = eigen_redux_wave(sd.dim, [rs](int i) { return rs[i]; });
-            const float ql2 = eigen_redux_wave(sd.dim, [rs](int i) { return rs[i] * rs[i]; });
-            float ipqc = 0.0f;
-            if (DT == 1) ipqc = eigen_redux_wave(sd.dim, [rs, cs](int i) { return rs[i] * cs[i]; });
-            if (lane == 0) {
-                float *sc = m.sc + s * SC_N;
-                const float ql2n = sqrtf(ql2);
-                sc[SC_SUMQ] = sum_q;
-                sc[SC_QL2] = ql2;
-                sc[SC_SUMQCB] = (float)(0.5 * (double)sum_q - (double)(0.58f * ql2n));
-                sc[SC_IPQC] = ipqc;
-                sc[SC_VFAC] = DT == 1 ? ipqc - bound[s] : ql2 - 2 * bound[s];
-            }
-        }
-        // LUT pass 1: per-codebook min / max of the fp32 table (fastscan.hpp:236-248)
-        for (int c = tid; c < ncb; c += kThreads) {
-            const int g = c >> 4;
-            const SegDesc &sd = m.segs[m.gseg[g]];
-            const float *rr = m.r + sd.doff + (g - sd.group_off) * 64 + (c & 15) * 4;
-            float l[16];
-            l[0] = 0;
-            float mn = 0.0f, mx = 0.0f;
-#pragma unroll
-            for (int j = 1; j < 16; ++j) {
-                l[j] = l[j - (j & -j)] + rr[kPos[j]];
-                mn = fminf(mn, l[j]);
-                mx = fmaxf(mx, l[j]);
-            }
-            m.cbmin[c] = mn;
-            m.cbmax[c] = mx;
-        }
-        __syncthreads();
-        mark(1);
-        for (int s = wave; s < S; s += 4) {
-            const SegDesc sd = m.segs[s];
-            if (sd.bits == 0) continue;
-            float mn = FLT_MAX, mx = -FLT_MAX;
-            for (int c = sd.group_off * 16 + lane; c < (sd.group_off + sd.ngroups) * 16; c += 64) {
-                mn = fminf(mn, m.cbmin[c]);
-                mx = fmaxf(mx, m.cbmax[c]);
-            }
-            for (int o = 32; o >= 1; o >>= 1) {
-                mn = fminf(mn, __shfl_xor(mn, o));
-                mx = fmaxf(mx, __shfl_xor(mx, o));
-            }
-            if (lane == 0) {
-                float *sc = m.sc + s * SC_N;
-                const float delta = (float)((double)(mx - mn) / ((1 << 16) - 0.01));
-                sc[SC_DELTA] = delta;
-                sc[SC_INV] = 1.0f / delta;  // the release build's hoisted reciprocal (lut.hpp:464)
-                sc[SC_VL] = mn;
-                sc[SC_SUMVL] = (float)(((double)mn + 0.5 * (double)delta) * (double)(float)(sd.dim / 4));
-            }
-        }
-        __syncthreads();
-        // LUT pass 2: quantise to u16 (lut.hpp:461-465)
-        for (int c = tid; c < ncb; c += kThreads) {
-            const int g = c >> 4;
-            const int s = m.gseg[g];
-            const SegDesc &sd = m.segs[s];
-            const float *rr = m.r + sd.doff + (g - sd.group_off) * 64 + (c & 15) * 4;
-            const float vl = m.sc[s * SC_N + SC_VL], inv = m.sc[s * SC_N + SC_INV];
-            float l[16];
-            l[0] = 0;
-#pragma unroll
-            for (int j = 1; j < 16; ++j) l[j] = l[j - (j & -j)] + rr[kPos[j]];
-            uint32_t *dst = reinterpret_cast<uint32_t *>(m.lut + c * 16);
-#pragma unroll
-            for (int j = 0; j < 16; j += 2)
-                dst[j / 2] = cvt_u16((l[j] - vl) * inv) | (cvt_u16((l[j + 1] - vl) * inv) << 16);
-        }
-        __syncthreads();
+        prepare_list<DT>(m.segs, S, m.gseg, NG, m.qr, m.crbuf, bound, m.r, m.sc, m.cbmin, m.cbmax, m.lut, ix.D_pad);
         mark(2);
         if (tid == 0) m.met[2] += (unsigned long long)nb * kBlock;  // total_comp_cnt
 
