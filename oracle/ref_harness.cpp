@@ -3,8 +3,9 @@
 // Exposes, through a C ABI, the parts of the reference (howarlii/SAQ) whose
 // sources compile on their own: the fast-scan kernels (quantization/fastscan/
 // fastscan.hpp, fastscan_highacc.hpp), the extra-bit code helpers
-// (utils/code_helper.hpp) and the reference's vendored Eigen 3.4 (reductions,
-// GEMV, GEMM).  Built by oracle/Makefile from the sources where they lie under
+// (utils/code_helper.hpp), the configuration structs that IVF::save writes raw
+// (quantization/config.h, with the fmt headers the image ships) and the
+// reference's vendored Eigen 3.4 (reductions, GEMV, GEMM).  Built by oracle/Makefile from the sources where they lie under
 // the reference tree, with the reference's release flags, into oracle/_ref/.
 // The headers that need glog/gflags (lut.hpp, caq_encoder.hpp, the searcher)
 // are not built: the image has neither library.  Used only to pin the oracle.
@@ -12,7 +13,10 @@
 #include <cstdlib>
 #include <cstring>
 
+#include <cstddef>
+
 #include "defines.hpp"
+#include "quantization/config.h"
 #include "quantization/fastscan/fastscan.hpp"
 #include "quantization/fastscan/fastscan_highacc.hpp"
 #include "utils/code_helper.hpp"
@@ -31,6 +35,27 @@ T *amalloc(size_t n) {
 }  // namespace
 
 extern "C" {
+
+// Byte layout of the configuration structs IVF::save writes raw (saq_data.hpp:30,
+// quantizer_data.hpp:29): sizes and field offsets.
+void ref_config_layout(int64_t *out) {
+    out[0] = sizeof(QuantizeConfig);
+    out[1] = offsetof(QuantizeConfig, avg_bits);
+    out[2] = offsetof(QuantizeConfig, seg_eqseg);
+    out[3] = offsetof(QuantizeConfig, enable_segmentation);
+    out[4] = offsetof(QuantizeConfig, use_compact_layout);
+    out[5] = offsetof(QuantizeConfig, single);
+    out[6] = sizeof(QuantSingleConfig);
+    out[7] = offsetof(QuantSingleConfig, quant_type);
+    out[8] = offsetof(QuantSingleConfig, random_rotation);
+    out[9] = offsetof(QuantSingleConfig, use_fastscan);
+    out[10] = offsetof(QuantSingleConfig, caq_adj_rd_lmt);
+    out[11] = offsetof(QuantSingleConfig, caq_adj_eps);
+    out[12] = offsetof(QuantSingleConfig, caq_ori_qB);
+    out[13] = sizeof(BaseQuantType);
+    const QuantizeConfig d{};
+    out[14] = d.single.use_fastscan ? 1 : 0;
+}
 
 void ref_pack_codes(int64_t D, const uint8_t *codes, int64_t num, uint8_t *blocks) {
     fastscan::pack_codes(D, codes, num, blocks);

@@ -1,7 +1,10 @@
 // ivf_run.cpp -- builds and searches an index through the saqlib_gpu::IVF host API, the way a
 // program written against saqlib::IVF (create_index.cpp / test_qps.cpp) would.
 //
-//   ivf_run <input.bin> <output.bin>
+//   ivf_run <input.bin> <output.bin> [--save <index file>] [--load <index file>]
+//
+// --save writes the constructed index in the reference's file format (IVF::save);
+// --load skips construction and searches an index read from such a file (IVF::load).
 //
 // input.bin (little endian): int64 n, d, K, nq, topk, nprobe, nseg, rd_lmt, then nseg pairs
 // (int64 dim, int64 bits), float32 eps, float32 avg_bits, then float32 base[n*d],
@@ -25,9 +28,20 @@ bool rd(FILE *f, T *p, size_t n) {
 }  // namespace
 
 int main(int argc, char **argv) {
-    if (argc != 3) {
-        fprintf(stderr, "usage: %s input.bin output.bin\n", argv[0]);
+    if (argc != 3 && argc != 5) {
+        fprintf(stderr, "usage: %s input.bin output.bin [--save|--load index_file]\n", argv[0]);
         return 2;
+    }
+    const char *save_path = nullptr, *load_path = nullptr;
+    if (argc == 5) {
+        if (std::strcmp(argv[3], "--save") == 0)
+            save_path = argv[4];
+        else if (std::strcmp(argv[3], "--load") == 0)
+            load_path = argv[4];
+        else {
+            fprintf(stderr, "unknown option %s\n", argv[3]);
+            return 2;
+        }
     }
     FILE *f = fopen(argv[1], "rb");
     if (!f) {
@@ -74,16 +88,25 @@ int main(int argc, char **argv) {
     int64_t fails = 0;
     try {
         saqlib_gpu::IVF ivf((size_t)n, (size_t)d, (size_t)K, cfg);
-        ivf.set_variance(var.data(), var.size());
-        const auto &auto_plan = ivf.make_plan();
-        if (auto_plan != plan) {
-            // the caller's plan wins (a loaded index carries its own); report the difference
-            fprintf(stderr, "note: DP plan differs from the given plan; using the given plan\n");
-            ivf.set_plan(plan);
+        if (load_path) {
+            ivf.load(load_path);
+            if (ivf.quant_plan() != plan) {
+                fprintf(stderr, "loaded plan differs from the input's\n");
+                return 4;
+            }
+        } else {
+            ivf.set_variance(var.data(), var.size());
+            const auto &auto_plan = ivf.make_plan();
+            if (auto_plan != plan) {
+                // the caller's plan wins (a loaded index carries its own); report the difference
+                fprintf(stderr, "note: DP plan differs from the given plan; using the given plan\n");
+                ivf.set_plan(plan);
+            }
+            if (has_rot) ivf.set_rotations(rots);
+            ivf.construct(base.data(), cent.data(), cids.data());
+            fails = ivf.check_failures();
+            if (save_path) ivf.save(save_path);
         }
-        if (has_rot) ivf.set_rotations(rots);
-        ivf.construct(base.data(), cent.data(), cids.data());
-        fails = ivf.check_failures();
         saqlib_gpu::SearcherConfig scfg;
         // one query through search(), the rest as one batch
         ivf.search(q.data(), (size_t)topk, (size_t)nprobe, scfg, ids.data(), &met[0]);

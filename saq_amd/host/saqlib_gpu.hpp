@@ -17,6 +17,8 @@
 #pragma once
 #include <cmath>
 #include <cstdint>
+#include <cstring>
+#include <fstream>
 #include <random>
 #include <stdexcept>
 #include <string>
@@ -195,6 +197,7 @@ class IVF {
             }
         }
         sizes_ = cluster_sizes;
+        centroids_.assign(centroids, centroids + num_cen_ * num_dim_);
         saqgpu_index_desc d{};
         d.num_data = (int64_t)num_data_;
         d.num_dim = (int32_t)num_dim_;
@@ -211,12 +214,203 @@ class IVF {
         check(saqgpu_index_create(&d, &h_), "create");
     }
 
+    // IVF::save (ivf.hpp:197-228): the reference's index file, readable by saqlib::IVF::load.
+    void save(const char *path) const {
+        require_index("save");
+        std::ofstream out(path, std::ios::binary);
+        if (!out) throw Error(SAQGPU_EINVAL, std::string("save: cannot open ") + path);
+        const uint64_t hdr[3] = {num_data_, num_dim_, num_cen_};
+        out.write(reinterpret_cast<const char *>(hdr), sizeof(hdr));
+        out.write(reinterpret_cast<const char *>(centroids_.data()), centroids_.size() * 4);  // FlatInitializer::save
+        // SaqData::save (saq_data.hpp:29-39)
+        uint8_t cfgb[kCfgBytes];
+        encode_cfg(cfgb);
+        out.write(reinterpret_cast<const char *>(cfgb), kCfgBytes);
+        put64(out, num_dim_);
+        put64(out, variance_.size());
+        out.write(reinterpret_cast<const char *>(variance_.data()), variance_.size() * 4);
+        put64(out, plan_.size());
+        for (size_t s = 0; s < plan_.size(); ++s) {  // BaseQuantizerData::save (quantizer_data.hpp:26-35)
+            put64(out, (uint64_t)plan_[s].first);
+            put64(out, (uint64_t)plan_[s].second);
+            uint8_t scfg[kSingleCfgBytes];
+            encode_single_cfg(scfg);
+            out.write(reinterpret_cast<const char *>(scfg), kSingleCfgBytes);
+            const char flag = rotations_.empty() ? 0 : 1;
+            out.write(&flag, 1);
+            if (flag) out.write(reinterpret_cast<const char *>(rotations_[s].data()), rotations_[s].size() * 4);
+        }
+        std::vector<uint64_t> sz(sizes_.begin(), sizes_.end());
+        out.write(reinterpret_cast<const char *>(sz.data()), sz.size() * 8);
+        for (size_t c = 0; c < num_cen_; ++c) {  // SaqCluData::save (cluster_data.hpp:327-336)
+            saqgpu_cluster_sizes cs;
+            check(saqgpu_cluster_sizes_of(h_, (int32_t)c, &cs), "save");
+            std::vector<float> sf(std::max<int64_t>(1, cs.short_factor_floats)), lf(std::max<int64_t>(1, cs.long_factor_floats));
+            std::vector<uint8_t> sc(std::max<int64_t>(1, cs.short_code_bytes)), lc(std::max<int64_t>(1, cs.long_code_bytes));
+            std::vector<uint32_t> ids(std::max<int64_t>(1, sizes_[c]));
+            std::vector<float> cen(cs.centroid_floats);
+            check(saqgpu_index_get_cluster(h_, (int32_t)c, sf.data(), sc.data(), lc.data(), lf.data(), ids.data(),
+                                           cen.data()),
+                  "save");
+            out.write(reinterpret_cast<const char *>(sf.data()), cs.short_factor_floats * 4);
+            out.write(reinterpret_cast<const char *>(sc.data()), cs.short_code_bytes);
+            out.write(reinterpret_cast<const char *>(lc.data()), file_long_bytes(c));
+            out.write(reinterpret_cast<const char *>(lf.data()), cs.long_factor_floats * 4);
+            out.write(reinterpret_cast<const char *>(ids.data()), sizes_[c] * 4);
+            out.write(reinterpret_cast<const char *>(cen.data()), cen.size() * 4);
+        }
+        if (!out) throw Error(SAQGPU_EDEVICE, std::string("save: write failed on ") + path);
+    }
+
+    // IVF::load (ivf.hpp:230-264): an index file written by saqlib::IVF::save (or save above),
+    // uploaded to the device.  Replaces the object's configuration, plan, rotations and data.
+    void load(const char *path) {
+        std::ifstream in(path, std::ios::binary);
+        if (!in) throw Error(SAQGPU_EINVAL, std::string("load: cannot open ") + path);
+        uint64_t hdr[3];
+        read_exact(in, hdr, sizeof(hdr));
+        num_data_ = hdr[0];
+        num_dim_ = hdr[1];
+        num_cen_ = hdr[2];
+        num_dim_padded_ = (num_dim_ + 63) / 64 * 64;
+        std::vector<float> cent(num_cen_ * num_dim_);
+        read_exact(in, cent.data(), cent.size() * 4);
+        uint8_t cfgb[kCfgBytes];
+        read_exact(in, cfgb, kCfgBytes);
+        decode_cfg(cfgb);
+        const uint64_t saq_dim = get64(in);
+        if (saq_dim != num_dim_) throw Error(SAQGPU_EINVAL, "load: SaqData dimension differs from the header's");
+        const uint64_t nvar = get64(in);
+        if (nvar > (1u << 24)) throw Error(SAQGPU_EINVAL, "load: bad variance length");
+        std::vector<float> var(nvar);
+        read_exact(in, var.data(), nvar * 4);
+        variance_.assign(num_dim_padded_, 0.0f);
+        std::copy(var.begin(), var.begin() + std::min<size_t>(nvar, num_dim_padded_), variance_.begin());
+        const uint64_t nseg = get64(in);
+        if (nseg == 0 || nseg > SAQGPU_MAX_SEG) throw Error(SAQGPU_EINVAL, "load: bad segment count");
+        plan_.clear();
+        rotations_.clear();
+        bool any_rot = false, all_rot = true;
+        std::vector<std::vector<float>> rots;
+        for (uint64_t s = 0; s < nseg; ++s) {
+            const uint64_t D = get64(in), b = get64(in);
+            if (D == 0 || D % 64 || D > 65536 || b > 16) throw Error(SAQGPU_EINVAL, "load: bad segment");
+            uint8_t scfg[kSingleCfgBytes];
+            read_exact(in, scfg, kSingleCfgBytes);
+            char flag = 0;
+            read_exact(in, &flag, 1);
+            plan_.emplace_back((int)D, (int)b);
+            std::vector<float> P;
+            if (flag) {
+                P.resize(D * D);
+                read_exact(in, P.data(), P.size() * 4);
+                any_rot = true;
+            } else {
+                all_rot = false;
+                P.assign(D * D, 0.0f);  // no rotator: the identity
+                for (uint64_t i = 0; i < D; ++i) P[i * D + i] = 1.0f;
+            }
+            rots.push_back(std::move(P));
+        }
+        if (any_rot) rotations_ = std::move(rots);  // identity blocks stand in for segments without one
+        cfg_.single.random_rotation = any_rot && all_rot;
+        std::vector<uint64_t> sz(num_cen_);
+        read_exact(in, sz.data(), sz.size() * 8);
+        std::vector<int64_t> sizes(sz.begin(), sz.end());
+        int64_t tot = 0;
+        for (auto x : sizes) tot += x;
+        if ((uint64_t)tot != num_data_) throw Error(SAQGPU_EINVAL, "load: cluster sizes do not add up to num_data");
+        create(cent.data(), sizes);
+        for (size_t c = 0; c < num_cen_; ++c) {
+            saqgpu_cluster_sizes cs;
+            check(saqgpu_cluster_sizes_of(h_, (int32_t)c, &cs), "load");
+            std::vector<float> sf(std::max<int64_t>(1, cs.short_factor_floats)), lf(std::max<int64_t>(1, cs.long_factor_floats));
+            std::vector<uint8_t> sc(std::max<int64_t>(1, cs.short_code_bytes)), lc(std::max<int64_t>(1, cs.long_code_bytes), 0);
+            std::vector<uint32_t> ids(std::max<int64_t>(1, sizes[c]));
+            std::vector<float> cen(cs.centroid_floats);
+            read_exact(in, sf.data(), cs.short_factor_floats * 4);
+            read_exact(in, sc.data(), cs.short_code_bytes);
+            read_exact(in, lc.data(), file_long_bytes(c));
+            read_exact(in, lf.data(), cs.long_factor_floats * 4);
+            read_exact(in, ids.data(), sizes[c] * 4);
+            read_exact(in, cen.data(), cen.size() * 4);
+            check(saqgpu_index_set_cluster(h_, (int32_t)c, sf.data(), sc.data(), lc.data(), lf.data(), ids.data(),
+                                           cen.data()),
+                  "load");
+        }
+    }
+
     void set_plan(std::vector<std::pair<int, int>> plan) { plan_ = std::move(plan); }
     const std::vector<float> &variance() const { return variance_; }
     const std::vector<std::vector<float>> &rotations() const { return rotations_; }
     const std::vector<int64_t> &cluster_sizes() const { return sizes_; }
 
   private:
+    // Byte layouts of QuantizeConfig / QuantSingleConfig as saqlib writes them raw
+    // (quantization/config.h:12-27, x86-64): QuantSingleConfig = {int quant_type @0,
+    // bool random_rotation @4, bool use_fastscan @5, int caq_adj_rd_lmt @8, float caq_adj_eps
+    // @12, int caq_ori_qB @16} (20 bytes); QuantizeConfig = {float avg_bits @0, int seg_eqseg
+    // @4, bool enable_segmentation @8, bool use_compact_layout @9, QuantSingleConfig @12}
+    // (32 bytes).
+    static constexpr int kCfgBytes = 32, kSingleCfgBytes = 20, kSingleOff = 12;
+
+    void encode_single_cfg(uint8_t *b) const {
+        std::memset(b, 0, kSingleCfgBytes);
+        const int32_t caq = 0, ori_qb = 0;
+        const int32_t rd = cfg_.single.caq_adj_rd_lmt;
+        const float eps = cfg_.single.caq_adj_eps;
+        std::memcpy(b + 0, &caq, 4);
+        b[4] = cfg_.single.random_rotation ? 1 : 0;
+        b[5] = 1;  // use_fastscan
+        std::memcpy(b + 8, &rd, 4);
+        std::memcpy(b + 12, &eps, 4);
+        std::memcpy(b + 16, &ori_qb, 4);
+    }
+    void encode_cfg(uint8_t *b) const {
+        std::memset(b, 0, kCfgBytes);
+        const float ab = cfg_.avg_bits;
+        const int32_t eq = cfg_.seg_eqseg;
+        std::memcpy(b + 0, &ab, 4);
+        std::memcpy(b + 4, &eq, 4);
+        b[8] = cfg_.enable_segmentation ? 1 : 0;
+        b[9] = cfg_.use_compact_layout ? 1 : 0;
+        encode_single_cfg(b + kSingleOff);
+    }
+    void decode_cfg(const uint8_t *b) {
+        std::memcpy(&cfg_.avg_bits, b + 0, 4);
+        std::memcpy(&cfg_.seg_eqseg, b + 4, 4);
+        cfg_.enable_segmentation = b[8] != 0;
+        cfg_.use_compact_layout = b[9] != 0;
+        cfg_.single.random_rotation = b[kSingleOff + 4] != 0;
+        std::memcpy(&cfg_.single.caq_adj_rd_lmt, b + kSingleOff + 8, 4);
+        std::memcpy(&cfg_.single.caq_adj_eps, b + kSingleOff + 12, 4);
+    }
+    // SaqCluData::save/load write longb_code_bytes_ * num_vec_ long-code bytes
+    // (cluster_data.hpp:226-229, 320, 330): the padded per-vector stride in the per-vector
+    // layout, the unpadded sum of the segments' code bytes in the compact one (the default for
+    // a single segment).
+    int64_t file_long_bytes(size_t c) const {
+        const int64_t n = sizes_[c];
+        const bool compact = cfg_.use_compact_layout || plan_.size() == 1;
+        int64_t per = 0;
+        for (auto &[D, b] : plan_) {
+            const int64_t lb = b > 1 ? (int64_t)D * (b - 1) / 8 : 0;
+            per += compact ? lb : (lb + 15) / 16 * 16;
+        }
+        return per * n;
+    }
+    static void put64(std::ofstream &o, uint64_t v) { o.write(reinterpret_cast<const char *>(&v), 8); }
+    static uint64_t get64(std::ifstream &i) {
+        uint64_t v = 0;
+        read_exact(i, &v, 8);
+        return v;
+    }
+    static void read_exact(std::ifstream &i, void *p, int64_t n) {
+        if (n <= 0) return;
+        i.read(reinterpret_cast<char *>(p), n);
+        if (i.gcount() != n) throw Error(SAQGPU_EINVAL, "load: file truncated");
+    }
+
     void require_index(const char *what) const {
         if (!h_) throw Error(SAQGPU_ESTATE, std::string(what) + ": index not constructed");
     }
@@ -272,6 +466,7 @@ class IVF {
     std::vector<std::pair<int, int>> plan_;
     std::vector<std::vector<float>> rotations_;
     std::vector<int64_t> sizes_;
+    std::vector<float> centroids_;
     int64_t check_failures_ = 0;
 };
 

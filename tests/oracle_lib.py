@@ -152,6 +152,7 @@ def ref():
         L.ref_lut_delta.restype = C.c_float
         L.ref_lut_delta.argtypes = [C.c_float, C.c_float]
         L.ref_code_floor.argtypes = [f32p, i64, C.c_double, C.c_double, C.c_int, f32p]
+        L.ref_config_layout.argtypes = [C.c_void_p]
         _ref = L
     return _ref
 

@@ -370,3 +370,23 @@ def test_cluster_sizes_layout():
     assert s.short_factor_floats == 2 * 3 * 64
     assert s.short_code_bytes == 2 * (4 * 128 + 4 * 64)
     assert s.long_stride == 64 + 16 and s.long_code_bytes == 80 * 33
+
+
+# ------------------------------------------------------------------ index file format
+def test_index_file_config_layout_matches_reference():
+    """The raw QuantizeConfig / QuantSingleConfig bytes IVF::save writes (saq_data.hpp:30,
+    quantizer_data.hpp:29): the offsets tests/index_file.py and saq_amd/host/saqlib_gpu.hpp
+    use against the compiled reference config.h."""
+    import index_file as IF
+    R = ref()
+    if R is None or not hasattr(R, "ref_config_layout"):
+        pytest.skip("reference harness not built")
+    out = np.zeros(16, np.int64)
+    R.ref_config_layout(out.ctypes.data)
+    assert out[0] == IF.CFG_BYTES and out[6] == IF.SINGLE_CFG_BYTES and out[5] == IF.SINGLE_OFF
+    assert [out[1], out[2], out[3], out[4]] == [IF.CFG_FIELDS[k][0] for k in
+                                                ("avg_bits", "seg_eqseg", "enable_segmentation", "use_compact_layout")]
+    assert [out[7], out[8], out[9], out[10], out[11], out[12]] == [
+        IF.SINGLE_FIELDS[k][0] for k in ("quant_type", "random_rotation", "use_fastscan", "caq_adj_rd_lmt",
+                                         "caq_adj_eps", "caq_ori_qB")]
+    assert out[13] == 4 and out[14] == 1  # int-sized enum; use_fastscan defaults to true
