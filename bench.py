@@ -15,6 +15,15 @@ zipf10m-d1024
     N = 10,000,000, D = 1024, a 2,048-component mixture with Zipf(1.1) component weights
     (std decay 1/sqrt(1 + i/32)), nlist = 16384, B = 4 (--bits), 10,000 queries whose
     nprobe is drawn per query from U[32, 256] (seed 3), k = 10.
+serve5m-d1536
+    N = 5,000,000, D = 1536, a 512-component mixture (std decay 1/sqrt(1 + i/64)),
+    nlist = 8192, B = 1.5, a pool of 4,096 queries; besides the batch measurements, the
+    serving loop at offered loads of 2k..50k queries/s.
+
+Every workload ends with the serving loop (Poisson arrivals of 1..64-query requests, k in
+{10, 100}, nprobe in [16, 128], dynamically batched onto the batched scan); its p50/p99
+request latency is in config.serving (the default workload runs one load, 30% of the
+batched throughput capped at 200k queries/s).
 
 A step = one search of the whole query batch (GPU coarse quantiser + query rotation +
 scan), inputs resident in HBM.  The plain scan is timed first; the batched scan is then
@@ -47,7 +56,67 @@ WORKLOADS = {
                        npq_range=None, steps=20, kmeans_iters=8, cpu_queries=0),
     "zipf10m-d1024": dict(n=10_000_000, dim=1024, comps=2048, tau=32, zipf=1.1, nlist=16384, nq=10000, bits=4.0,
                           npq_range=(32, 256), steps=3, kmeans_iters=6, cpu_queries=16),
+    "serve5m-d1536": dict(n=5_000_000, dim=1536, comps=512, tau=64, zipf=0.0, nlist=8192, nq=4096, bits=1.5,
+                          npq_range=(16, 128), steps=3, kmeans_iters=6, cpu_queries=16,
+                          serve_loads=(2000, 5000, 10000, 20000, 50000)),
 }
+
+
+def serve(torch, ix, queries, loads, seconds, seed, precision, exact_probe, log_fn):
+    """The serving loop: Poisson request arrivals at each offered load (queries/s), each request
+    1..64 queries with k in {10, 100} and one nprobe in [16, 128]; whatever has arrived when the
+    GPU frees up is served as one batched search (k = the batch's largest, nprobe per query,
+    at most 4096 queries).  Latency = completion - arrival of each request.  Returns one row per
+    load."""
+    rng = np.random.default_rng(seed)
+    dev = queries.device
+    nq_pool = queries.shape[0]
+    rows = []
+    for load in loads:
+        mean_q = 32.5
+        rate = load / mean_q  # requests/s
+        nreq = max(1, int(rate * seconds))
+        arr = np.cumsum(rng.exponential(1.0 / rate, nreq))
+        nqs = rng.integers(1, 65, nreq)
+        ks = np.where(rng.random(nreq) < 0.5, 10, 100)
+        nps = rng.integers(16, 129, nreq)
+        starts = rng.integers(0, nq_pool - 64, nreq)
+        lat = np.zeros(nreq)
+        batches = []
+        i = 0
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        while i < nreq:
+            now = time.perf_counter() - t0
+            if arr[i] > now:
+                time.sleep(min(arr[i] - now, 0.0005))
+                continue
+            j, tot = i, 0
+            while j < nreq and arr[j] <= now and tot + nqs[j] <= 4096:
+                tot += int(nqs[j])
+                j += 1
+            idx = np.concatenate([np.arange(starts[r], starts[r] + nqs[r]) for r in range(i, j)])
+            npq = np.concatenate([np.full(nqs[r], nps[r], np.int32) for r in range(i, j)])
+            kb = int(ks[i:j].max())
+            qb = queries[torch.from_numpy(idx).to(dev)]
+            npq_d = torch.from_numpy(npq).to(dev)
+            ids = torch.empty(tot, kb, dtype=torch.int32, device=dev)
+            ix.search_batched(qb, kb, int(npq.max()), nprobe_per_query=npq_d, precision=precision, out_ids=ids,
+                              want_dists=False, exact_probe=exact_probe)
+            torch.cuda.synchronize()
+            done = time.perf_counter() - t0
+            lat[i:j] = done - arr[i:j]
+            batches.append(tot)
+            i = j
+        wall = time.perf_counter() - t0
+        served = float(nqs.sum())
+        row = {"offered_qps": load, "achieved_qps": round(served / wall, 1), "requests": nreq,
+               "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+               "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
+               "mean_batch_queries": round(float(np.mean(batches)), 1)}
+        log_fn(f"serving {row}")
+        rows.append(row)
+    return rows
 
 
 def read_traffic(name, n, bits, nprobe, **extra):
@@ -234,6 +303,7 @@ def main():
     ap.add_argument("--nprobe", type=int, default=0, help="fixed nprobe (0 = smallest reaching the target)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=3.0, help="duration of each serving-loop load")
     ap.add_argument("--exact-probe", action="store_true",
                     help="batched scan with the reference-order coarse quantiser instead of the fp32-MFMA one")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -455,6 +525,10 @@ def main():
         f"{b_tops:.1f} TOPS")
     btraffic = read_traffic("bscan_traffic.json", n, bits, bnp, precision=prec_name, query_tile=bqt)
 
+    # ---- serving loop: latency under Poisson arrivals (dynamic batching over the batched scan)
+    loads = W.get("serve_loads") or (min(200000, int(0.3 * bqps)),)
+    serving = serve(torch, ix, queries, loads, args.serve_seconds, 5, bprec, args.exact_probe, log)
+
     cpu = None
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
@@ -489,6 +563,7 @@ def main():
                                f"nlist={nlist} B={bits} nq={nq} k={args.topk}",
                    "bits": bits, "plan": plan, "recall_at_10_by_bits": bits_tried,
                    "batched": batched, "plain": plain, "batched_vs_plain": round(bqps / qps, 3),
+                   "serving": serving, "p99_ms": serving[-1]["p99_ms"],
                    "encode_mvec_per_s": round(n / t_enc / 1e6, 3)},
         "roofline": {"bound": "hbm", "achieved": round(b_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(b_gbs / HBM_PEAK_GBS, 4), "traffic": btraffic,
