@@ -921,7 +921,9 @@ int orc_search(const orc_plan_t *plan, const float *variance, const float *const
     }
     for (int64_t i = 0; i < topk; ++i) {
         out_ids[i] = pool.ids[i];
-        if (out_dists) out_dists[i] = pool.dist[i];
+        // ResultPool::get (pool.hpp:36): (greater ? -1 : 1) * stored distance, so an unfilled
+        // IP slot reads -0.0
+        if (out_dists) out_dists[i] = (pool.greater ? -1.0f : 1.0f) * pool.dist[i];
     }
     for (auto &e : qs.seg) {
         mt.fast_bitsum += e.fast_bits;

@@ -17,6 +17,7 @@
 #include "../../include/saqgpu.h"
 #include "batched.hpp"
 #include "common.hpp"
+#include "slice_scan.hpp"
 
 namespace saqgpu {
 hipError_t launch_query_prep(const IndexView &ix, const float *queries, int64_t nq, const float *variance,
@@ -120,6 +121,12 @@ struct saqgpu_index {
     DevBuf<float> ws_q, ws_qrot, ws_bound, ws_dists, ws_cdist;
     DevBuf<uint32_t> ws_probes, ws_ids;
     DevBuf<uint64_t> ws_met;
+    // sliced scan (slice_scan.hip): pools between slices, block info and candidate records;
+    // the capacities grow to what a search needed and are kept for the next one
+    DevBuf<float> sl_pool_d;
+    DevBuf<uint32_t> sl_pool_id, sl_pair, sl_binfo, sl_cand, sl_ctr;
+    DevBuf<int32_t> sl_pool_size;
+    DevBuf<uint64_t> sl_met;
     // batched layout (batched.hip): built from the clusters on first use
     bool bq_ready = false;
     int bq_tile = 0;  // autotuned queries per work tile (0 = not tuned)
@@ -578,6 +585,101 @@ int saqgpu_probe(saqgpu_index *ix, const float *queries, int64_t nq, int32_t npr
     return SAQGPU_OK;
 }
 
+namespace {
+// The plain scan as slices of the probe sequence (slice_scan.hip): lists [0,1), [1,4),
+// [4,16), [16,64), ... of every query; per slice one list-scan launch over all (query, list)
+// pairs and one replay launch.  Block-info and candidate space that a slice runs out of is
+// flagged on the device; the search is then repeated with the space it asked for (the
+// capacities are kept for later searches), so results never depend on the capacities.
+int sliced_scan(saqgpu_index *ix, const IndexView &v, const uint32_t *dp, const saqgpu_search_params *p,
+                const int32_t *dnpq, int64_t nq, uint32_t *dids, float *dd, uint64_t *dm, hipStream_t st) {
+    const int K = p->topk, S = ix->nseg;
+    std::vector<int> bounds{0};
+    for (int b = 1; b < p->nprobe; b *= 4) bounds.push_back(b);
+    bounds.push_back(p->nprobe);
+    int Lmax = 1;
+    for (size_t i = 0; i + 1 < bounds.size(); ++i) Lmax = std::max(Lmax, bounds[i + 1] - bounds[i]);
+    // candidates refined per round: their extra-bit inner products sit in LDS
+    int cmax = 256;
+    while (cmax > 16 && list_scan_smem_bytes(v, cmax) > 40 * 1024) cmax /= 2;
+    const size_t smem = list_scan_smem_bytes(v, cmax);
+    int max_smem = 0;
+    HIPCHK(hipDeviceGetAttribute(&max_smem, hipDeviceAttributeMaxSharedMemoryPerBlock, ix->device));
+    if ((int64_t)smem > max_smem) return fail(SAQGPU_EUNSUPPORTED, "plan needs more LDS than a workgroup has");
+    int rc = 0;
+    rc |= ix->sl_pool_d.ensure(nq * K);
+    rc |= ix->sl_pool_id.ensure(nq * K);
+    rc |= ix->sl_pool_size.ensure(nq);
+    rc |= ix->sl_met.ensure(nq * 3);
+    rc |= ix->sl_pair.ensure(nq * Lmax);
+    rc |= ix->sl_ctr.ensure(8);
+    if (rc) return fail(SAQGPU_ENOMEM, "workspace allocation failed");
+    // first sizes: every list of the widest slice, and a quarter of its vectors as candidates
+    int64_t nonempty = 0;
+    for (int c = 0; c < ix->num_cen; ++c) nonempty += ix->clu_n[c] > 0;
+    const double mean_n = nonempty ? (double)ix->num_data / nonempty : 1.0;
+    const int64_t kCapMax = ((int64_t)1 << 31) - 1;
+    auto clamp_cap = [&](double x) { return (int64_t)std::min<double>((double)kCapMax, std::max(x, 4096.0)); };
+    // SAQGPU_SLICE_CAP=n (tests): start both at n dwords / records instead
+    const char *cap_env = getenv("SAQGPU_SLICE_CAP");
+    const int64_t cap0 = cap_env ? std::max<int64_t>(1, atoll(cap_env)) : 0;
+    if (ix->sl_binfo.n == 0)
+        if (ix->sl_binfo.alloc(cap0 ? cap0 : clamp_cap((double)nq * Lmax * (mean_n / kBlock + 1.5) * binfo_stride(S))))
+            return fail(SAQGPU_ENOMEM, "workspace allocation failed");
+    if (ix->sl_cand.n == 0)
+        if (ix->sl_cand.alloc((cap0 ? cap0 : clamp_cap((double)nq * Lmax * mean_n / 4 + 64.0 * nq)) * cand_stride(S)))
+            return fail(SAQGPU_ENOMEM, "workspace allocation failed");
+    for (int attempt = 0;; ++attempt) {
+        SliceArgs a{};
+        a.ix = v;
+        a.qrot = ix->ws_qrot.p;
+        a.bound = ix->ws_bound.p;
+        a.probes = dp;
+        a.nprobe = p->nprobe;
+        a.npq = dnpq;
+        a.topk = K;
+        a.nq = nq;
+        a.pool_d = ix->sl_pool_d.p;
+        a.pool_id = ix->sl_pool_id.p;
+        a.pool_size = ix->sl_pool_size.p;
+        a.met = ix->sl_met.p;
+        a.pair_blk = ix->sl_pair.p;
+        a.binfo = ix->sl_binfo.p;
+        a.cand = ix->sl_cand.p;
+        a.binfo_cap = (uint32_t)std::min<int64_t>(kCapMax, (int64_t)ix->sl_binfo.n);
+        a.cand_cap = (uint32_t)std::min<int64_t>(kCapMax, (int64_t)(ix->sl_cand.n / cand_stride(S)));
+        a.ctr = ix->sl_ctr.p;
+        a.cmax = cmax;
+        a.out_ids = dids;
+        a.out_dists = dd;
+        a.out_metrics = dm;
+        HIPCHK(launch_slice_reset(a.ctr, 1, st));
+        for (size_t i = 0; i + 1 < bounds.size(); ++i) {
+            a.p0 = bounds[i];
+            a.p1 = bounds[i + 1];
+            a.first = i == 0;
+            a.last = i + 2 == bounds.size();
+            if (i > 0) HIPCHK(launch_slice_reset(a.ctr, 0, st));
+            HIPCHK(launch_list_scan(a, p->dist_type, smem, st));
+            HIPCHK(launch_replay(a, p->dist_type, st));
+        }
+        HIPCHK(launch_slice_reset(a.ctr, 0, st));
+        uint32_t h[8];
+        HIPCHK(hipMemcpyAsync(h, a.ctr, sizeof(h), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        if (!h[2]) return SAQGPU_OK;
+        if (attempt >= 3) return fail(SAQGPU_ENOMEM, "scan workspace did not converge");
+        // grow to what the slices asked for (upper bounds: a short slice only raises later T)
+        const int64_t nb = clamp_cap(std::max<double>((double)h[3] * 1.25, (double)ix->sl_binfo.n));
+        const int64_t nc = clamp_cap(std::max<double>((double)h[4] * 1.25, (double)ix->sl_cand.n / cand_stride(S)));
+        if (nb > (int64_t)ix->sl_binfo.n && ix->sl_binfo.alloc(nb))
+            return fail(SAQGPU_ENOMEM, "workspace allocation failed");
+        if (nc * cand_stride(S) > (int64_t)ix->sl_cand.n && ix->sl_cand.alloc(nc * cand_stride(S)))
+            return fail(SAQGPU_ENOMEM, "workspace allocation failed");
+    }
+}
+}  // namespace
+
 int saqgpu_search(saqgpu_index *ix, const float *queries, int64_t nq, const uint32_t *probes,
                   const saqgpu_search_params *p, uint32_t *out_ids, float *out_dists, saqgpu_metrics *metrics,
                   int32_t flags, void *stream) {
@@ -600,14 +702,23 @@ int saqgpu_search(saqgpu_index *ix, const float *queries, int64_t nq, const uint
     HIPCHK(hipSetDevice(ix->device));
     hipStream_t st = (hipStream_t)stream;
     const IndexView v = ix->view();
-    // candidates per refine round: their long-code records are staged in LDS
-    // up to 64 per round while a workgroup's LDS stays within 80 KB (two workgroups per CU)
+    // SAQGPU_SCAN=fused: the single-kernel scan (one workgroup walks a query's whole probe
+    // sequence); kept for comparison, the sliced scan is the default
+    static const bool fused = [] {
+        const char *e = getenv("SAQGPU_SCAN");
+        return e && !strcmp(e, "fused");
+    }();
     int cmax = 64;
-    while (cmax > 8 && scan_smem_bytes(v, p->topk, cmax) > 80 * 1024) cmax /= 2;
-    size_t smem = scan_smem_bytes(v, p->topk, cmax);
-    int max_smem = 0;
-    HIPCHK(hipDeviceGetAttribute(&max_smem, hipDeviceAttributeMaxSharedMemoryPerBlock, ix->device));
-    if ((int64_t)smem > max_smem) return fail(SAQGPU_EUNSUPPORTED, "plan needs more LDS than a workgroup has");
+    size_t smem = 0;
+    if (fused) {
+        // candidates per refine round: their long-code records are staged in LDS
+        // up to 64 per round while a workgroup's LDS stays within 80 KB (two workgroups per CU)
+        while (cmax > 8 && scan_smem_bytes(v, p->topk, cmax) > 80 * 1024) cmax /= 2;
+        smem = scan_smem_bytes(v, p->topk, cmax);
+        int max_smem = 0;
+        HIPCHK(hipDeviceGetAttribute(&max_smem, hipDeviceAttributeMaxSharedMemoryPerBlock, ix->device));
+        if ((int64_t)smem > max_smem) return fail(SAQGPU_EUNSUPPORTED, "plan needs more LDS than a workgroup has");
+    }
     int rc = 0;
     rc |= ix->ws_qrot.ensure(nq * ix->D_pad);
     rc |= ix->ws_bound.ensure(nq * ix->nseg);
@@ -655,23 +766,28 @@ int saqgpu_search(saqgpu_index *ix, const float *queries, int64_t nq, const uint
         HIPCHK(hipEventCreate(&e1));
         HIPCHK(hipEventRecord(e0, st));
     }
-    // SAQGPU_PROFILE=1: per-phase shader clocks of the scan kernel, printed to stderr (measurement only)
-    static const bool prof_on = getenv("SAQGPU_PROFILE") != nullptr;
-    DevBuf<unsigned long long> prof;
-    if (prof_on && prof.ensure(nq * 8)) return fail(SAQGPU_ENOMEM, "allocation failed");
-    HIPCHK(launch_scan(v, ix->ws_qrot.p, ix->ws_bound.p, dp, p->nprobe, dnpq, p->topk, nq, p->dist_type, dids, dd, dm,
-                       smem, st, prof_on ? prof.p : nullptr, cmax));
-    if (prof_on) {
-        std::vector<unsigned long long> h(nq * 8);
-        HIPCHK(hipStreamSynchronize(st));
-        HIPCHK(hipMemcpy(h.data(), prof.p, nq * 8 * 8, hipMemcpyDeviceToHost));
-        double tot[8] = {0};
-        for (int64_t i = 0; i < nq; ++i)
-            for (int k = 0; k < 8; ++k) tot[k] += (double)h[i * 8 + k];
-        fprintf(stderr, "[saqgpu profile] per-query wall-clock ticks (100 MHz):");
-        for (int k = 0; k < 7; ++k) fprintf(stderr, " ph%d=%.0f", k, tot[k] / nq);
-        fprintf(stderr, "\n");
-        prof.release();
+    if (fused) {
+        // SAQGPU_PROFILE=1: per-phase shader clocks of the fused kernel, printed to stderr
+        static const bool prof_on = getenv("SAQGPU_PROFILE") != nullptr;
+        DevBuf<unsigned long long> prof;
+        if (prof_on && prof.ensure(nq * 8)) return fail(SAQGPU_ENOMEM, "allocation failed");
+        HIPCHK(launch_scan(v, ix->ws_qrot.p, ix->ws_bound.p, dp, p->nprobe, dnpq, p->topk, nq, p->dist_type, dids,
+                           dd, dm, smem, st, prof_on ? prof.p : nullptr, cmax));
+        if (prof_on) {
+            std::vector<unsigned long long> h(nq * 8);
+            HIPCHK(hipStreamSynchronize(st));
+            HIPCHK(hipMemcpy(h.data(), prof.p, nq * 8 * 8, hipMemcpyDeviceToHost));
+            double tot[8] = {0};
+            for (int64_t i = 0; i < nq; ++i)
+                for (int k = 0; k < 8; ++k) tot[k] += (double)h[i * 8 + k];
+            fprintf(stderr, "[saqgpu profile] per-query wall-clock ticks (100 MHz):");
+            for (int k = 0; k < 7; ++k) fprintf(stderr, " ph%d=%.0f", k, tot[k] / nq);
+            fprintf(stderr, "\n");
+            prof.release();
+        }
+    } else {
+        const int rc2 = sliced_scan(ix, v, dp, p, dnpq, nq, dids, dd, dm, st);
+        if (rc2 != SAQGPU_OK) return rc2;
     }
     if (ix->timing) {
         HIPCHK(hipEventRecord(e1, st));

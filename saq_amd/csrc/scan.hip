@@ -223,17 +223,6 @@ size_t scan_smem_bytes(const IndexView &ix, int topk, int cmax) {
     return carve_layout(nullptr, ix, topk, cmax, nullptr);
 }
 
-// LDS DMA of `bytes` (a multiple of 16) from global memory to LDS by the whole workgroup:
-// every wavefront moves 1 KB per global_load_lds_dwordx4 (lane l -> dst + 16 l).
-__device__ __forceinline__ void dma_to_lds(const uint8_t *src, uint8_t *dst, int bytes, int wave, int lane) {
-    for (int off = wave * 1024; off < bytes; off += 4 * 1024) {
-        const int i = off + lane * 16;
-        if (i < bytes) __builtin_amdgcn_global_load_lds(src + i, dst + off, 16, 0, 0);
-    }
-}
-
-// Waits for this wavefront's outstanding vector-memory operations (LDS DMA included).
-__device__ __forceinline__ void dma_wait() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 
 template <int DT>
 __device__ __forceinline__ float pool_distk(const Smem &m, int K) {

@@ -15,11 +15,11 @@ import saq_fixtures as F
 pytestmark = pytest.mark.gpu
 
 
-def run_case(case, topk, nprobe, m=4.0):
-    probes = F.orc_probes(case, nprobe)
-    ids_o, d_o, met_o = F.orc_search(case, probes, topk, m=m)
+def run_case(case, topk, nprobe, m=4.0, dist_type=0):
+    probes = F.orc_probes(case, nprobe, dist_type=dist_type)
+    ids_o, d_o, met_o = F.orc_search(case, probes, topk, m=m, dist_type=dist_type)
     ix = F.gpu_index(case)
-    ids_g, d_g, met_g = ix.search(case.queries, topk, nprobe, probes=probes, m=m)
+    ids_g, d_g, met_g = ix.search(case.queries, topk, nprobe, probes=probes, m=m, dist_type=dist_type)
     bad = np.nonzero((ids_o != ids_g).any(1))[0]
     assert len(bad) == 0, f"id mismatch in queries {bad[:5]}: oracle {ids_o[bad[0]][:10]} gpu {ids_g[bad[0]][:10]}"
     assert np.array_equal(d_o.view(np.uint32), d_g.view(np.uint32)), "distance bits differ"
@@ -75,6 +75,33 @@ def test_multiple_of_32_lists_and_empty_lists():
 def test_topk_exceeds_probed_vectors():
     case = F.make_case(n=600, d=128, K=8, nq=8, plan=[(128, 4)], seed=15)
     run_case(case, 1024, 2)
+
+
+@pytest.mark.parametrize("plan", [[(128, 4)], [(64, 5), (64, 3)], [(64, 9), (64, 0)]])
+@pytest.mark.parametrize("topk", [10, 100])
+def test_many_slices(plan, topk):
+    """nprobe 40 of 48 lists: the scan runs the slices [0,1) [1,4) [4,16) [16,40), each
+    refining the vectors whose fast estimate beats the k-th distance at its start."""
+    case = F.make_case(n=6000, d=128, K=48, nq=24, plan=plan, seed=23)
+    run_case(case, topk, 40)
+
+
+def test_scan_workspace_growth(monkeypatch):
+    """Block-info and candidate space far below what the slices need (16 records): the
+    search is repeated with the space the device asked for, and still equals the oracle."""
+    monkeypatch.setenv("SAQGPU_SLICE_CAP", "16")
+    for plan in ([(64, 4), (64, 2)], [(128, 4)]):
+        case = F.make_case(n=4000, d=128, K=24, nq=16, plan=plan, seed=25)
+        run_case(case, 10, 20)
+
+
+@pytest.mark.parametrize("plan", [[(128, 4)], [(64, 5), (64, 3)]])
+def test_inner_product_search(plan):
+    """IP: an unfilled pool's distk is -FLT_MAX (pool.hpp:32-34), so nothing is inserted and
+    every result is 0; the metrics still count the scanned blocks."""
+    case = F.make_case(n=2000, d=128, K=8, nq=12, plan=plan, seed=27)
+    ids = run_case(case, 10, 4, dist_type=1)
+    assert not ids.any()
 
 
 def test_nprobe_per_query():
