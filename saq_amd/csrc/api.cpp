@@ -124,7 +124,7 @@ struct saqgpu_index {
     // sliced scan (slice_scan.hip): pools between slices, block info and candidate records;
     // the capacities grow to what a search needed and are kept for the next one
     DevBuf<float> sl_pool_d;
-    DevBuf<uint32_t> sl_pool_id, sl_pair, sl_binfo, sl_cand, sl_ctr;
+    DevBuf<uint32_t> sl_pool_id, sl_pair, sl_binfo, sl_cand, sl_pre, sl_desc, sl_ctr;
     DevBuf<int32_t> sl_pool_size;
     DevBuf<uint64_t> sl_met;
     // batched layout (batched.hip): built from the clusters on first use
@@ -594,41 +594,53 @@ namespace {
 int sliced_scan(saqgpu_index *ix, const IndexView &v, const uint32_t *dp, const saqgpu_search_params *p,
                 const int32_t *dnpq, int64_t nq, uint32_t *dids, float *dd, uint64_t *dm, hipStream_t st) {
     const int K = p->topk, S = ix->nseg;
+    // slices of the work items (item 0: list 0's first chunk; 1: its rest; j >= 2: list j - 1)
+    const int items = p->nprobe + 1;
     std::vector<int> bounds{0};
-    for (int b = 1; b < p->nprobe; b *= 4) bounds.push_back(b);
-    bounds.push_back(p->nprobe);
+    for (int b = 1; b < items; b *= 4) bounds.push_back(b);
+    bounds.push_back(items);
     int Lmax = 1;
     for (size_t i = 0; i + 1 < bounds.size(); ++i) Lmax = std::max(Lmax, bounds[i + 1] - bounds[i]);
-    // candidates refined per round: their extra-bit inner products sit in LDS
-    int cmax = 256;
-    while (cmax > 16 && list_scan_smem_bytes(v, cmax) > 40 * 1024) cmax /= 2;
-    const size_t smem = list_scan_smem_bytes(v, cmax);
-    int max_smem = 0;
+    int max_smem = 0, ncu = 0;
     HIPCHK(hipDeviceGetAttribute(&max_smem, hipDeviceAttributeMaxSharedMemoryPerBlock, ix->device));
-    if ((int64_t)smem > max_smem) return fail(SAQGPU_EUNSUPPORTED, "plan needs more LDS than a workgroup has");
+    HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ix->device));
+    // refine wavefronts stage up to cmax long records each (4 wavefronts within 64 KB)
+    const size_t ls_smem = list_scan_smem_bytes(v);
+    int cmax = 32;
+    while (cmax > 4 && refine_smem_bytes(v, cmax) > 64 * 1024) cmax /= 2;
+    const size_t rf_smem = refine_smem_bytes(v, cmax);
+    if ((int64_t)std::max(ls_smem, rf_smem) > max_smem)
+        return fail(SAQGPU_EUNSUPPORTED, "plan needs more LDS than a workgroup has");
+    const int rf_grid = std::max(1, ncu) * 4;
     int rc = 0;
     rc |= ix->sl_pool_d.ensure(nq * K);
     rc |= ix->sl_pool_id.ensure(nq * K);
     rc |= ix->sl_pool_size.ensure(nq);
     rc |= ix->sl_met.ensure(nq * 3);
     rc |= ix->sl_pair.ensure(nq * Lmax);
-    rc |= ix->sl_ctr.ensure(8);
+    rc |= ix->sl_ctr.ensure(CTR_N);
     if (rc) return fail(SAQGPU_ENOMEM, "workspace allocation failed");
-    // first sizes: every list of the widest slice, and a quarter of its vectors as candidates
+    // first sizes: every list of the widest slice, a quarter of its vectors as candidates
     int64_t nonempty = 0;
     for (int c = 0; c < ix->num_cen; ++c) nonempty += ix->clu_n[c] > 0;
     const double mean_n = nonempty ? (double)ix->num_data / nonempty : 1.0;
     const int64_t kCapMax = ((int64_t)1 << 31) - 1;
     auto clamp_cap = [&](double x) { return (int64_t)std::min<double>((double)kCapMax, std::max(x, 4096.0)); };
-    // SAQGPU_SLICE_CAP=n (tests): start both at n dwords / records instead
+    // SAQGPU_SLICE_CAP=n (tests): start every capacity at n instead
     const char *cap_env = getenv("SAQGPU_SLICE_CAP");
     const int64_t cap0 = cap_env ? std::max<int64_t>(1, atoll(cap_env)) : 0;
-    if (ix->sl_binfo.n == 0)
-        if (ix->sl_binfo.alloc(cap0 ? cap0 : clamp_cap((double)nq * Lmax * (mean_n / kBlock + 1.5) * binfo_stride(S))))
+    const int bst = binfo_stride(S), rst = cand_stride(S), pst = pre_stride(S);
+    if (ix->sl_binfo.n == 0 &&
+        ix->sl_binfo.alloc(cap0 ? cap0 : clamp_cap((double)nq * Lmax * (mean_n / kBlock + 1.5) * bst)))
+        return fail(SAQGPU_ENOMEM, "workspace allocation failed");
+    if (ix->sl_cand.n == 0) {
+        const int64_t nc = cap0 ? cap0 : clamp_cap((double)nq * Lmax * mean_n / 4 + 64.0 * nq);
+        if (ix->sl_cand.alloc(nc * rst) || ix->sl_pre.alloc(nc * pst))
             return fail(SAQGPU_ENOMEM, "workspace allocation failed");
-    if (ix->sl_cand.n == 0)
-        if (ix->sl_cand.alloc((cap0 ? cap0 : clamp_cap((double)nq * Lmax * mean_n / 4 + 64.0 * nq)) * cand_stride(S)))
-            return fail(SAQGPU_ENOMEM, "workspace allocation failed");
+    }
+    if (ix->sl_desc.n == 0 &&
+        ix->sl_desc.alloc((cap0 ? cap0 : clamp_cap((double)nq * Lmax * (mean_n / 256 + 1))) * kDescStride))
+        return fail(SAQGPU_ENOMEM, "workspace allocation failed");
     for (int attempt = 0;; ++attempt) {
         SliceArgs a{};
         a.ix = v;
@@ -646,8 +658,11 @@ int sliced_scan(saqgpu_index *ix, const IndexView &v, const uint32_t *dp, const 
         a.pair_blk = ix->sl_pair.p;
         a.binfo = ix->sl_binfo.p;
         a.cand = ix->sl_cand.p;
+        a.pre = ix->sl_pre.p;
+        a.desc = ix->sl_desc.p;
         a.binfo_cap = (uint32_t)std::min<int64_t>(kCapMax, (int64_t)ix->sl_binfo.n);
-        a.cand_cap = (uint32_t)std::min<int64_t>(kCapMax, (int64_t)(ix->sl_cand.n / cand_stride(S)));
+        a.cand_cap = (uint32_t)std::min<int64_t>(kCapMax, (int64_t)(ix->sl_cand.n / rst));
+        a.desc_cap = (uint32_t)std::min<int64_t>(kCapMax, (int64_t)(ix->sl_desc.n / kDescStride));
         a.ctr = ix->sl_ctr.p;
         a.cmax = cmax;
         a.out_ids = dids;
@@ -660,21 +675,26 @@ int sliced_scan(saqgpu_index *ix, const IndexView &v, const uint32_t *dp, const 
             a.first = i == 0;
             a.last = i + 2 == bounds.size();
             if (i > 0) HIPCHK(launch_slice_reset(a.ctr, 0, st));
-            HIPCHK(launch_list_scan(a, p->dist_type, smem, st));
+            HIPCHK(launch_list_scan(a, p->dist_type, ls_smem, st));
+            HIPCHK(launch_refine(a, p->dist_type, rf_smem, rf_grid, st));
             HIPCHK(launch_replay(a, p->dist_type, st));
         }
         HIPCHK(launch_slice_reset(a.ctr, 0, st));
-        uint32_t h[8];
+        uint32_t h[CTR_N];
         HIPCHK(hipMemcpyAsync(h, a.ctr, sizeof(h), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
-        if (!h[2]) return SAQGPU_OK;
+        if (!h[CTR_OVERFLOW]) return SAQGPU_OK;
         if (attempt >= 3) return fail(SAQGPU_ENOMEM, "scan workspace did not converge");
-        // grow to what the slices asked for (upper bounds: a short slice only raises later T)
-        const int64_t nb = clamp_cap(std::max<double>((double)h[3] * 1.25, (double)ix->sl_binfo.n));
-        const int64_t nc = clamp_cap(std::max<double>((double)h[4] * 1.25, (double)ix->sl_cand.n / cand_stride(S)));
+        // grow to what the slices asked for (a slice that ran short only raises later T, so
+        // the counts are upper bounds once block info fits)
+        const int64_t nb = clamp_cap(std::max<double>((double)h[CTR_MAX_BINFO] * 1.25, (double)ix->sl_binfo.n));
+        const int64_t nc = clamp_cap(std::max<double>((double)h[CTR_MAX_CAND] * 1.25, (double)a.cand_cap));
+        const int64_t nd = clamp_cap(std::max<double>((double)h[CTR_MAX_DESC] * 1.25, (double)a.desc_cap));
         if (nb > (int64_t)ix->sl_binfo.n && ix->sl_binfo.alloc(nb))
             return fail(SAQGPU_ENOMEM, "workspace allocation failed");
-        if (nc * cand_stride(S) > (int64_t)ix->sl_cand.n && ix->sl_cand.alloc(nc * cand_stride(S)))
+        if (nc > (int64_t)a.cand_cap && (ix->sl_cand.alloc(nc * rst) || ix->sl_pre.alloc(nc * pst)))
+            return fail(SAQGPU_ENOMEM, "workspace allocation failed");
+        if (nd > (int64_t)a.desc_cap && ix->sl_desc.alloc(nd * kDescStride))
             return fail(SAQGPU_ENOMEM, "workspace allocation failed");
     }
 }

@@ -1,5 +1,5 @@
 // estimators.hpp -- the reference's per-list estimator state and extra-bit inner products,
-// shared by the plain scan (scan.hip) and IVF::estimate (estimate.hip).
+// shared by the plain scans (slice_scan.hip, scan.hip) and IVF::estimate (estimate.hip).
 #pragma once
 #include <cfloat>
 
@@ -112,10 +112,10 @@ __device__ __forceinline__ float ex_ip_group(int k, const float *r, const uint8_
 }
 
 // ---- one thread computes a whole CodeHelper<K>::compute_ip: its 16 accumulators are the 16
-// fp32 lanes of the reference's AVX-512 register (lane l takes dims l, l+16, ... in order), and
-// the closing sum is the GCC reduce tree (lo256+hi256, lo128+hi128, +swap64, lane0+lane1).
-// Codes stream from global memory 64 dims (8K bytes of the compacted layout) at a time; the
-// residual r (16-byte aligned, LDS) is read as float4.
+// fp32 lanes of the reference's AVX-512 register (lane l takes dims l, l+16, ... in order, by
+// FMA, a masked add for K = 1), and the closing sum is the GCC reduce tree (lo256+hi256,
+// lo128+hi128, +swap64, lane0+lane1) -- the same value as ex_ip_group.  The codes (16-byte
+// aligned; 8K bytes of the compacted layout per 64 dims) and r (16-byte aligned) are in LDS.
 
 __device__ __forceinline__ uint32_t dw_byte(const uint32_t *w, int j) { return (w[j >> 2] >> (8 * (j & 3))) & 0xFFu; }
 
@@ -146,51 +146,40 @@ __device__ __forceinline__ float ex_ip_thread_k(const float *r, const uint8_t *y
     float acc[16];
 #pragma unroll
     for (int l = 0; l < 16; ++l) acc[l] = 0.0f;
-    // the step's 8K bytes; the next step's are in flight while this one is summed
-    auto load_step = [&](int c, uint32_t *w) {
-        const uint8_t *src = y + 8 * K * c;
+    for (int c = 0; c < D / 64; ++c) {
+        uint32_t w[2 * K];  // the step's 8K bytes: 16-byte reads for even K, 8-byte for odd
         if (K % 2 == 0) {
+            const uint4 *src = reinterpret_cast<const uint4 *>(y + 8 * K * c);
 #pragma unroll
             for (int j = 0; j < K / 2; ++j) {
-                const uint4 t = *reinterpret_cast<const uint4 *>(src + 16 * j);
+                const uint4 t = src[j];
                 w[4 * j] = t.x;
                 w[4 * j + 1] = t.y;
                 w[4 * j + 2] = t.z;
                 w[4 * j + 3] = t.w;
             }
         } else {
+            const uint2 *src = reinterpret_cast<const uint2 *>(y + 8 * K * c);
 #pragma unroll
             for (int j = 0; j < K; ++j) {
-                const uint2 t = *reinterpret_cast<const uint2 *>(src + 8 * j);
+                const uint2 t = src[j];
                 w[2 * j] = t.x;
                 w[2 * j + 1] = t.y;
             }
         }
-    };
-    const int nst = D / 64;
-    uint32_t wn[2 * K];
-    load_step(0, wn);
-    for (int c = 0; c < nst; ++c) {
-        uint32_t w[2 * K];
-#pragma unroll
-        for (int j = 0; j < 2 * K; ++j) w[j] = wn[j];
-        if (c + 1 < nst) load_step(c + 1, wn);
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const float4 *rv = reinterpret_cast<const float4 *>(r + 64 * c + 16 * u);
-            float x[16];
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const float4 t = rv[j];
-                x[4 * j] = t.x;
-                x[4 * j + 1] = t.y;
-                x[4 * j + 2] = t.z;
-                x[4 * j + 3] = t.w;
-            }
+                const float x[4] = {t.x, t.y, t.z, t.w};
 #pragma unroll
-            for (int l = 0; l < 16; ++l) {
-                const int cd = step_code<K>(w, u, l);
-                acc[l] = K == 1 ? acc[l] + (cd ? x[l] : 0.0f) : __fmaf_rn(x[l], (float)cd, acc[l]);
+                for (int e = 0; e < 4; ++e) {
+                    const int l = 4 * j + e;
+                    const int cd = step_code<K>(w, u, l);
+                    acc[l] = K == 1 ? acc[l] + (cd ? x[e] : 0.0f) : __fmaf_rn(x[e], (float)cd, acc[l]);
+                }
             }
         }
     }
@@ -203,7 +192,7 @@ __device__ __forceinline__ float ex_ip_thread_k(const float *r, const uint8_t *y
     return acc[0] + acc[1];
 }
 
-static __device__ __noinline__ float ex_ip_thread_base(int k, const float *r, const uint8_t *y, int D) {
+__device__ __forceinline__ float ex_ip_thread_base(int k, const float *r, const uint8_t *y, int D) {
     switch (k) {
     case 1: return ex_ip_thread_k<1>(r, y, D);
     case 2: return ex_ip_thread_k<2>(r, y, D);
@@ -216,7 +205,7 @@ static __device__ __noinline__ float ex_ip_thread_base(int k, const float *r, co
     }
 }
 
-// CodeHelper<k>::compute_ip for k = 1..12 by one thread (same value as ex_ip_group).
+// CodeHelper<k>::compute_ip for k = 1..12 by one thread (the value ex_ip_group returns).
 __device__ __forceinline__ float ex_ip_thread(int k, const float *r, const uint8_t *y, int D) {
     if (k == 5) return 2 * ex_ip_thread_base(4, r, y, D) + ex_ip_thread_base(1, r, y + D * 4 / 8, D);
     if (k > 8) {
@@ -236,32 +225,47 @@ constexpr int kPrepThreads = 256;  // the workgroup size prepare_list is written
 // LUT's delta, vl and reciprocal) and the quantised u16 LUT of every code group.  segs, gseg
 // (group -> segment), qr, crbuf and the outputs live in LDS; bound is the query's per-segment
 // variance bound.  Every thread of a 256-thread workgroup calls it; it ends with a barrier.
+// One segment's scalars of CaqCluEstimator::prepare by one wavefront (all 64 lanes call it):
+// sum_q, |r|^2, the fast-estimate offset, <r, c> (IP) and the variance estimate's constant.
+// r and crbuf point at the whole padded vectors.
+template <int DT>
+__device__ __forceinline__ void segment_scalars(const SegDesc &sd, int s, const float *r, const float *crbuf,
+                                                const float *bound, float *sc) {
+    const int lane = threadIdx.x & 63;
+    const float *rs = r + sd.doff;
+    const float *cs = crbuf + sd.doff;
+    const float sum_q = eigen_redux_wave(sd.dim, [rs](int i) { return rs[i]; });
+    const float ql2 = eigen_redux_wave(sd.dim, [rs](int i) { return rs[i] * rs[i]; });
+    float ipqc = 0.0f;
+    if (DT == 1) ipqc = eigen_redux_wave(sd.dim, [rs, cs](int i) { return rs[i] * cs[i]; });
+    if (lane == 0) {
+        float *ss = sc + s * SC_N;
+        const float ql2n = sqrtf(ql2);
+        ss[SC_SUMQ] = sum_q;
+        ss[SC_QL2] = ql2;
+        ss[SC_SUMQCB] = (float)(0.5 * (double)sum_q - (double)(0.58f * ql2n));
+        ss[SC_IPQC] = ipqc;
+        ss[SC_VFAC] = DT == 1 ? ipqc - bound[s] : ql2 - 2 * bound[s];
+    }
+}
+
+// The first half of prepare_list: the residual and every segment's scalars except the LUT's.
+template <int DT>
+__device__ void prepare_scalars(const SegDesc *segs, int S, const float *qr, const float *crbuf, const float *bound,
+                                float *r, float *sc, int D_pad) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int d = tid; d < D_pad; d += kPrepThreads) r[d] = DT == 1 ? qr[d] : qr[d] - crbuf[d];
+    __syncthreads();
+    for (int s = wave; s < S; s += 4) segment_scalars<DT>(segs[s], s, r, crbuf, bound, sc);
+}
+
 template <int DT>
 __device__ void prepare_list(const SegDesc *segs, int S, const int8_t *gseg, int NG, const float *qr,
                              const float *crbuf, const float *bound, float *r, float *sc, float *cbmin,
                              float *cbmax, uint16_t *lut, int D_pad) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int ncb = NG * 16;
-    for (int d = tid; d < D_pad; d += kPrepThreads) r[d] = DT == 1 ? qr[d] : qr[d] - crbuf[d];
-    __syncthreads();
-    for (int s = wave; s < S; s += 4) {
-        const SegDesc sd = segs[s];
-        const float *rs = r + sd.doff;
-        const float *cs = crbuf + sd.doff;
-        const float sum_q = eigen_redux_wave(sd.dim, [rs](int i) { return rs[i]; });
-        const float ql2 = eigen_redux_wave(sd.dim, [rs](int i) { return rs[i] * rs[i]; });
-        float ipqc = 0.0f;
-        if (DT == 1) ipqc = eigen_redux_wave(sd.dim, [rs, cs](int i) { return rs[i] * cs[i]; });
-        if (lane == 0) {
-            float *ss = sc + s * SC_N;
-            const float ql2n = sqrtf(ql2);
-            ss[SC_SUMQ] = sum_q;
-            ss[SC_QL2] = ql2;
-            ss[SC_SUMQCB] = (float)(0.5 * (double)sum_q - (double)(0.58f * ql2n));
-            ss[SC_IPQC] = ipqc;
-            ss[SC_VFAC] = DT == 1 ? ipqc - bound[s] : ql2 - 2 * bound[s];
-        }
-    }
+    prepare_scalars<DT>(segs, S, qr, crbuf, bound, r, sc, D_pad);
     // LUT pass 1: per-codebook min / max of the fp32 table (fastscan.hpp:236-248)
     for (int c = tid; c < ncb; c += kPrepThreads) {
         const int g = c >> 4;

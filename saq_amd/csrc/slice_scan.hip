@@ -6,24 +6,24 @@
 // every decision depends on all earlier ones.  The k-th distance only ever decreases.  Split
 // the probe sequence into slices [p0, p1) (1, 3, 12, 48, ... lists): every vector the
 // reference refines inside a slice has a fast estimate below the k-th distance T reached at
-// the slice's start.  Each slice runs as two kernels:
+// the slice's start.  Each slice runs as three kernels:
 //
 //   list_scan_kernel   one workgroup per (query, probed list of the slice), all independent:
 //                      the list's u16 LUTs (CaqCluEstimator::prepare, caq_estimator.hpp:94-106),
 //                      the fast scan of every block (one thread per vector, exact integer LUT
 //                      sums, the reference's fp32 epilogue), the per-block pruning minima of
-//                      searchCluster, and -- for the superset {fast estimate < T} -- the
-//                      extra-bit inner products and accurate distances (compAccurateDist,
-//                      caq_estimator.hpp:190-217) with searchCluster's stage-3 running sums.
-//                      It writes one block-info record per block and one record per candidate.
+//                      searchCluster, and the superset {fast estimate < T} of the vectors the
+//                      reference may refine, compacted per 8-block chunk.
+//   refine_kernel      one workgroup per chunk with candidates: the extra-bit inner products
+//                      (CodeHelper<b-1>::compute_ip, 16 lanes per item in the reference's lane
+//                      order) from long records staged in LDS, the accurate distances
+//                      (compAccurateDist, caq_estimator.hpp:190-217) and searchCluster's
+//                      stage-3 running sums, written as one record per candidate.
 //   replay_kernel      one wavefront per query replays the slice in the reference's order: the
 //                      per-block masks of scanCluster (fixed at block start, the N2 last-block
 //                      skip), the stage 1/2/3 decisions of searchCluster and ResultPool inserts
 //                      (utils/pool.hpp:12-61), reading only block info and candidate records.
 //                      Its pool is the next slice's T.
-//
-// The streaming kernel carries the bytes (short codes, factors, the superset's long codes) and
-// has no sequential dependency; the replay reads a few bytes per block.
 #include <algorithm>
 #include <cfloat>
 
@@ -35,23 +35,21 @@
 
 namespace saqgpu {
 
-constexpr int kLT = 256;               // list-scan workgroup: one thread per vector of a chunk
-constexpr int kChunk = kLT / kBlock;   // 8 blocks per chunk
+constexpr int kLT = 256;               // workgroup of the list scan and the refine
+constexpr int kChunk = kLT / kBlock;   // 8 blocks (256 vectors, one per thread) per chunk
+
 // ---------------------------------------------------------------- list scan
 
 struct LSmem {
     SegDesc *segs;
-    int8_t *gseg, *bseq, *rseq;  // group -> segment; quantised segments; segments with extra bits
+    int8_t *gseg, *bseq;  // group -> segment; quantised segments in order
     float *r, *sc, *cbmin, *cbmax;
     uint16_t *lut;
-    float *xs, *ipq, *curr;  // [S][256], [S][256], [256]
-    int *candv;              // chunk-local thread of each candidate, in (block, lane) order
-    float *exv;              // [cmax][S] extra-bit inner products of a round of candidates
-    int *ctl;                // [0] block-info base, [1] candidate base, [2] segments with extra bits,
-                             // [4..8) wave counts
+    float *xs, *ipq;      // [S][256]
+    int *ctl;             // [0] block-info base, [1] candidate base, [4..8) wave counts
 };
 
-__device__ __host__ inline size_t ls_layout(unsigned char *base, const IndexView &ix, int cmax, LSmem *mp) {
+__device__ __host__ inline size_t ls_layout(unsigned char *base, const IndexView &ix, LSmem *mp) {
     const int S = ix.nseg;
     size_t off = 0;
     auto take = [&](size_t bytes) {
@@ -64,7 +62,6 @@ __device__ __host__ inline size_t ls_layout(unsigned char *base, const IndexView
     m.segs = reinterpret_cast<SegDesc *>(take((size_t)S * sizeof(SegDesc)));
     m.gseg = reinterpret_cast<int8_t *>(take(ix.ngroups));
     m.bseq = reinterpret_cast<int8_t *>(take(S));
-    m.rseq = reinterpret_cast<int8_t *>(take(S));
     m.r = reinterpret_cast<float *>(take(ix.D_pad * 4));
     m.sc = reinterpret_cast<float *>(take((size_t)S * SC_N * 4));
     m.cbmin = reinterpret_cast<float *>(take((size_t)ix.ngroups * 16 * 4));
@@ -72,15 +69,52 @@ __device__ __host__ inline size_t ls_layout(unsigned char *base, const IndexView
     m.lut = reinterpret_cast<uint16_t *>(take((size_t)ix.ngroups * 256 * 2));
     m.xs = reinterpret_cast<float *>(take((size_t)S * kLT * 4));
     m.ipq = reinterpret_cast<float *>(take((size_t)S * kLT * 4));
-    m.curr = reinterpret_cast<float *>(take(kLT * 4));
-    m.candv = reinterpret_cast<int *>(take(kLT * 4));
-    m.exv = reinterpret_cast<float *>(take((size_t)cmax * S * 4));
     m.ctl = reinterpret_cast<int *>(take(16 * 4));
     if (mp) *mp = m;
     return off + 16;
 }
 
-size_t list_scan_smem_bytes(const IndexView &ix, int cmax) { return ls_layout(nullptr, ix, cmax, nullptr); }
+size_t list_scan_smem_bytes(const IndexView &ix) { return ls_layout(nullptr, ix, nullptr); }
+
+// accurate distance of every segment (compAccurateDist, caq_estimator.hpp:190-217) and
+// searchCluster's stage-3 running sum (saq_searcher.hpp:149-156) of one candidate, written as
+// its record.  cur: its fast estimate; x(s), ipq(s), ex(s), resc(s): |o|, the fast-scan inner
+// product, the extra-bit inner product and the rescale factor of segment s.
+template <int DT, class X, class I, class E, class R>
+__device__ __forceinline__ void write_record(uint32_t *rec, uint32_t pid, int ln, float cur, const SegDesc *segs,
+                                             const float *scs, int S, X x_of, I ipq_of, E ex_of, R resc_of) {
+    float acc = cur;
+    rec[0] = pid;
+    rec[1] = (uint32_t)ln;
+    rec[2] = __float_as_uint(cur);
+    for (int s = 0; s < S; ++s) {
+        const int bits = segs[s].bits;
+        const float *sc = scs + s * SC_N;
+        const float x = x_of(s);
+        float av, cd;
+        if (bits == 0) {
+            av = DT == 1 ? sc[SC_IPQC] : x * x + sc[SC_QL2];
+            cd = DT == 1 ? sc[SC_VFAC] : max0(x * x + sc[SC_VFAC]);
+        } else {
+            const float ipq = ipq_of(s);
+            float f = (ipq - sc[SC_SUMQCB]) * segs[s].rescale_sqrtD;
+            f = f * x;
+            cd = DT == 1 ? f * 0.5f + sc[SC_IPQC] : max0(x * x + (sc[SC_QL2] - f));
+            const float sqd = segs[s].sqd;
+            const float ex = bits >= 2 ? ex_of(s) : 0.0f;
+            // Lut::getExtIP (lut.hpp:508-513)
+            const float t = (float)(((double)ipq + (double)ex * (double)sqd) +
+                                    (-1.0 + (double)(sqd / 2)) * (double)sc[SC_SUMQ]);
+            const float ipoq = resc_of(s) * t;
+            av = DT == 1 ? ipoq + sc[SC_IPQC] : x * x + sc[SC_QL2] - 2 * ipoq;
+        }
+        if (S == 1)
+            acc = av;
+        else
+            acc += av - cd;
+        rec[3 + s] = __float_as_uint(acc);
+    }
+}
 
 template <int DT>
 __global__ __launch_bounds__(kLT) void list_scan_kernel(SliceArgs a) {
@@ -90,37 +124,35 @@ __global__ __launch_bounds__(kLT) void list_scan_kernel(SliceArgs a) {
     const int L = a.p1 - a.p0;
     const int64_t pair = a.q0 * L + blockIdx.x;
     const int64_t q = pair / L;
-    const int p = a.p0 + (int)(pair - q * L);
-    const int S = ix.nseg, NG = ix.ngroups, K = a.topk, CM = a.cmax;
+    const int item = a.p0 + (int)(pair - q * L);
+    const int p = item_list(item);
+    const int S = ix.nseg, NG = ix.ngroups, K = a.topk;
     const int NPq = a.npq ? a.npq[q] : a.nprobe;
     const uint32_t cid = p < NPq ? a.probes[q * a.nprobe + p] : 0xFFFFFFFFu;
     const int n = cid < (uint32_t)ix.num_cen ? ix.clu_n[cid] : 0;  // bad ids: rejected on the host
-    if (n == 0) {
+    const int nb = (n + kBlock - 1) / kBlock;
+    const int b0 = item_first_block(item, nb), b1 = item_end_block(item, nb);
+    if (b1 <= b0) {
         if (tid == 0) a.pair_blk[pair] = kNoPair;
         return;  // uniform per workgroup
     }
-    const int nb = (n + kBlock - 1) / kBlock;
     const bool single = S == 1;
-    const int bst = binfo_stride(S), rst = cand_stride(S);
+    const int bst = binfo_stride(S), pst = pre_stride(S);
     LSmem m;
-    ls_layout(smem, ix, CM, &m);
+    ls_layout(smem, ix, &m);
     if (tid == 0) {
-        const uint32_t need = (uint32_t)nb * (uint32_t)bst;
-        const uint32_t base = atomicAdd(&a.ctr[0], need);
+        const uint32_t need = (uint32_t)(b1 - b0) * (uint32_t)bst;
+        const uint32_t base = atomicAdd(&a.ctr[CTR_BINFO], need);
         const bool ok = (uint64_t)base + need <= a.binfo_cap;
-        if (!ok) a.ctr[2] = 1;
+        if (!ok) a.ctr[CTR_OVERFLOW] = 1;
         m.ctl[0] = ok ? (int)base : -1;
         a.pair_blk[pair] = ok ? base : kNoPair;
     }
     for (int i = tid; i < S; i += kLT) m.segs[i] = ix.segs[i];
     uint32_t endmask = 0;
-    int nbq = 0, nrs = 0;
+    int nbq = 0;
     for (int s = 0; s < S; ++s) {
         const SegDesc sd = ix.segs[s];
-        if (sd.bits >= 2) {
-            if (tid == 0) m.rseq[nrs] = (int8_t)s;
-            ++nrs;
-        }
         if (sd.bits == 0) continue;
         endmask |= 1u << (sd.group_off + sd.ngroups - 1);
         for (int g = tid; g < sd.ngroups; g += kLT) m.gseg[sd.group_off + g] = (int8_t)s;
@@ -140,13 +172,13 @@ __global__ __launch_bounds__(kLT) void list_scan_kernel(SliceArgs a) {
     // CaqCluEstimator::prepare of every segment (caq_estimator.hpp:94-106)
     prepare_list<DT>(m.segs, S, m.gseg, NG, a.qrot + q * ix.D_pad, ix.crot + (int64_t)cid * ix.D_pad, a.bound + q * S,
                      m.r, m.sc, m.cbmin, m.cbmax, m.lut, ix.D_pad);
-    const int64_t vec0 = ix.clu_vec[cid];
     const uint8_t *blk0 = ix.blocks + ix.clu_blk[cid] * ix.block_stride;
-    uint32_t *binfo = a.binfo + binfo_base;
+    uint32_t *binfo = a.binfo + binfo_base - (int64_t)b0 * bst;  // indexed by the list's block
+    unsigned long long *cnt64 = reinterpret_cast<unsigned long long *>(a.ctr + CTR_CAND);
     const int v = tid, bl = v >> 5, ln = v & 31;
 
-    for (int st = 0; st < nb; st += kChunk) {
-        const int nbs = min(kChunk, nb - st);
+    for (int st = b0; st < b1; st += kChunk) {
+        const int nbs = min(kChunk, b1 - st);
         const bool active = bl < nbs;
         const int idx = (st + bl) * kBlock + ln;
         // ---- fast scan: exact integer LUT sums (accumulate_hacc, fastscan_highacc.hpp:66-139),
@@ -193,59 +225,56 @@ __global__ __launch_bounds__(kLT) void list_scan_kernel(SliceArgs a) {
         }
         // ---- per-vector estimates, block minima and the candidate test against T
         bool flag = false;
-        {
-            float cur = 0.0f;
-            if (single) {
-                const float *sc = m.sc;
-                const float x = active ? m.xs[v] : 0.0f;
-                if (m.segs[0].bits == 0) {
-                    cur = DT == 1 ? sc[SC_VFAC] : max0(x * x + sc[SC_VFAC]);
-                } else {
-                    const float ipq = active ? m.ipq[v] : 0.0f;
-                    float f = (ipq - sc[SC_SUMQCB]) * m.segs[0].rescale_sqrtD;
-                    f = f * x;
-                    cur = DT == 1 ? f * 0.5f + sc[SC_IPQC] : max0(x * x + (sc[SC_QL2] - f));
-                }
-                // scanCluster scores n % 32 vectors of the last block: none when n % 32 == 0 (N2)
-                const int cnt = (st + bl == nb - 1) ? n % kBlock : kBlock;
-                flag = active && ln < cnt && cur < T;
+        float cur = 0.0f;
+        if (single) {
+            const float *sc = m.sc;
+            const float x = active ? m.xs[v] : 0.0f;
+            if (m.segs[0].bits == 0) {
+                cur = DT == 1 ? sc[SC_VFAC] : max0(x * x + sc[SC_VFAC]);
             } else {
-                // stage 1: sum of the variance estimates (saq_searcher.hpp:95-110)
-                for (int s = 0; s < S; ++s) {
-                    const float x = active ? m.xs[s * kLT + v] : 0.0f;
-                    const float vd = DT == 1 ? m.sc[s * SC_N + SC_VFAC] : max0(x * x + m.sc[s * SC_N + SC_VFAC]);
-                    cur = cur + vd;
-                }
-                float m1 = cur;
-                for (int o = 16; o >= 1; o >>= 1) m1 = fminf(m1, __shfl_xor(m1, o, 32));
-                float M = m1;
-                // stage 2: each quantised segment's variance estimate replaced by its fast
-                // estimate, in segment order (saq_searcher.hpp:112-132)
-                for (int s = 0; s < S; ++s) {
-                    if (m.segs[s].bits == 0) continue;
-                    const float *sc = m.sc + s * SC_N;
-                    const float x = active ? m.xs[s * kLT + v] : 0.0f;
-                    const float vd = DT == 1 ? sc[SC_VFAC] : max0(x * x + sc[SC_VFAC]);
-                    const float ipq = active ? m.ipq[s * kLT + v] : 0.0f;
-                    float f = (ipq - sc[SC_SUMQCB]) * m.segs[s].rescale_sqrtD;
-                    f = f * x;
-                    const float fd = DT == 1 ? f * 0.5f + sc[SC_IPQC] : max0(x * x + (sc[SC_QL2] - f));
-                    cur = cur - vd;
-                    cur = cur + fd;
-                    float ms = cur;
-                    for (int o = 16; o >= 1; o >>= 1) ms = fminf(ms, __shfl_xor(ms, o, 32));
-                    M = fmaxf(M, ms);
-                    if (active && ln == 0) binfo[(st + bl) * bst + 4 + s] = __float_as_uint(ms);
-                }
-                if (active && ln == 0) {
-                    binfo[(st + bl) * bst + 2] = __float_as_uint(m1);
-                    binfo[(st + bl) * bst + 3] = __float_as_uint(M);
-                }
-                // a block is refined only if m1 <= distk and M <= distk, a vector only if
-                // cur < distk (saq_searcher.hpp:105-158); distk <= T throughout the slice
-                flag = active && idx < n && m1 <= T && M <= T && cur < T;
+                const float ipq = active ? m.ipq[v] : 0.0f;
+                float f = (ipq - sc[SC_SUMQCB]) * m.segs[0].rescale_sqrtD;
+                f = f * x;
+                cur = DT == 1 ? f * 0.5f + sc[SC_IPQC] : max0(x * x + (sc[SC_QL2] - f));
             }
-            if (active) m.curr[v] = cur;
+            // scanCluster scores n % 32 vectors of the last block: none when n % 32 == 0 (N2)
+            const int cnt = (st + bl == nb - 1) ? n % kBlock : kBlock;
+            flag = active && ln < cnt && cur < T;
+        } else {
+            // stage 1: sum of the variance estimates (saq_searcher.hpp:95-110)
+            for (int s = 0; s < S; ++s) {
+                const float x = active ? m.xs[s * kLT + v] : 0.0f;
+                const float vd = DT == 1 ? m.sc[s * SC_N + SC_VFAC] : max0(x * x + m.sc[s * SC_N + SC_VFAC]);
+                cur = cur + vd;
+            }
+            float m1 = cur;
+            for (int o = 16; o >= 1; o >>= 1) m1 = fminf(m1, __shfl_xor(m1, o, 32));
+            float M = m1;
+            // stage 2: each quantised segment's variance estimate replaced by its fast
+            // estimate, in segment order (saq_searcher.hpp:112-132)
+            for (int s = 0; s < S; ++s) {
+                if (m.segs[s].bits == 0) continue;
+                const float *sc = m.sc + s * SC_N;
+                const float x = active ? m.xs[s * kLT + v] : 0.0f;
+                const float vd = DT == 1 ? sc[SC_VFAC] : max0(x * x + sc[SC_VFAC]);
+                const float ipq = active ? m.ipq[s * kLT + v] : 0.0f;
+                float f = (ipq - sc[SC_SUMQCB]) * m.segs[s].rescale_sqrtD;
+                f = f * x;
+                const float fd = DT == 1 ? f * 0.5f + sc[SC_IPQC] : max0(x * x + (sc[SC_QL2] - f));
+                cur = cur - vd;
+                cur = cur + fd;
+                float ms = cur;
+                for (int o = 16; o >= 1; o >>= 1) ms = fminf(ms, __shfl_xor(ms, o, 32));
+                M = fmaxf(M, ms);
+                if (active && ln == 0) binfo[(st + bl) * bst + 4 + s] = __float_as_uint(ms);
+            }
+            if (active && ln == 0) {
+                binfo[(st + bl) * bst + 2] = __float_as_uint(m1);
+                binfo[(st + bl) * bst + 3] = __float_as_uint(M);
+            }
+            // a block is refined only if m1 <= distk and M <= distk, a vector only if
+            // cur < distk (saq_searcher.hpp:105-158); distk <= T throughout the slice
+            flag = active && idx < n && m1 <= T && M <= T && cur < T;
         }
         // ---- ordered compaction of the candidates (wave-major = block order, lane order)
         const unsigned long long bal = __ballot(flag);
@@ -257,12 +286,21 @@ __global__ __launch_bounds__(kLT) void list_scan_kernel(SliceArgs a) {
             if (w < wave) before += c;
             total += c;
         }
-        const int rank = before + __popcll(bal & ((1ull << lane) - 1));
-        if (flag) m.candv[rank] = v;
+        // the chunk's candidates: one descriptor for the refine kernel, `total` records
         if (tid == 0 && total) {
-            const uint32_t cb = atomicAdd(&a.ctr[1], (uint32_t)total);
-            const bool ok = (uint64_t)cb + total <= a.cand_cap;
-            if (!ok) a.ctr[2] = 1;
+            const unsigned long long old = atomicAdd(cnt64, (1ull << 32) | (unsigned long long)total);
+            const uint32_t cb = (uint32_t)old, d = (uint32_t)(old >> 32);
+            const bool ok = (uint64_t)cb + total <= a.cand_cap && d < a.desc_cap;
+            if (!ok) a.ctr[CTR_OVERFLOW] = 1;
+            // every descriptor slot below the counter is read by the refine kernel: one whose
+            // records did not fit carries no candidates
+            if (d < a.desc_cap) {
+                uint32_t *dd = a.desc + (int64_t)d * kDescStride;
+                dd[0] = (uint32_t)pair;
+                dd[1] = (uint32_t)st;
+                dd[2] = ok ? cb : 0u;
+                dd[3] = ok ? (uint32_t)total : 0u;
+            }
             m.ctl[1] = ok ? (int)cb : -1;
         }
         __syncthreads();
@@ -273,65 +311,160 @@ __global__ __launch_bounds__(kLT) void list_scan_kernel(SliceArgs a) {
             binfo[(st + bl) * bst + 0] = (uint32_t)(cbase + boff);
             binfo[(st + bl) * bst + 1] = cbase < 0 ? 0u : (uint32_t)__popc(half);
         }
-        // ---- refine the candidates, up to cmax at a time: one thread per (candidate, segment
-        // with extra bits) computes the extra-bit inner product from the long code in HBM
-        if (cbase >= 0) {
-            for (int c0 = 0; c0 < total; c0 += CM) {
-                const int nc = min(CM, total - c0);
-                for (int item = tid; item < nc * nrs; item += kLT) {
-                    const int c = item / nrs, s = m.rseq[item - c * nrs];
-                    const SegDesc &sd = m.segs[s];
-                    const int64_t gvec = vec0 + (int64_t)(st * kBlock + m.candv[c0 + c]);
-                    const uint8_t *y = ix.long_codes + gvec * ix.long_stride + sd.long_off;
-                    m.exv[c * S + s] = ex_ip_thread(sd.bits - 1, m.r + sd.doff, y, sd.dim);
-                }
-                __syncthreads();
-                // accurate distance of every segment (compAccurateDist, caq_estimator.hpp:190-217)
-                // and searchCluster's stage-3 running sum (saq_searcher.hpp:149-156)
-                if (tid < nc) {
-                    const int c = tid, vv = m.candv[c0 + c];
-                    const int64_t gvec = vec0 + (int64_t)(st * kBlock + vv);
-                    uint32_t *rec = a.cand + (int64_t)(cbase + c0 + c) * rst;
-                    float acc = m.curr[vv];
-                    rec[0] = ix.ids[gvec];
-                    rec[1] = (uint32_t)(vv & 31);
-                    rec[2] = __float_as_uint(acc);
-                    for (int s = 0; s < S; ++s) {
-                        const int bits = m.segs[s].bits;
-                        const float *sc = m.sc + s * SC_N;
-                        const float x = m.xs[s * kLT + vv];
-                        float av, cd;
-                        if (bits == 0) {
-                            av = DT == 1 ? sc[SC_IPQC] : x * x + sc[SC_QL2];
-                            cd = DT == 1 ? sc[SC_VFAC] : max0(x * x + sc[SC_VFAC]);
-                        } else {
-                            const float ipq = m.ipq[s * kLT + vv];
-                            float f = (ipq - sc[SC_SUMQCB]) * m.segs[s].rescale_sqrtD;
-                            f = f * x;
-                            cd = DT == 1 ? f * 0.5f + sc[SC_IPQC] : max0(x * x + (sc[SC_QL2] - f));
-                            const float sqd = m.segs[s].sqd;
-                            const float ex = bits >= 2 ? m.exv[c * S + s] : 0.0f;
-                            // Lut::getExtIP (lut.hpp:508-513)
-                            const float t = (float)(((double)ipq + (double)ex * (double)sqd) +
-                                                    (-1.0 + (double)(sqd / 2)) * (double)sc[SC_SUMQ]);
-                            const float ipoq = ix.exfac[(gvec * S + s) * 2] * t;
-                            av = DT == 1 ? ipoq + sc[SC_IPQC] : x * x + sc[SC_QL2] - 2 * ipoq;
-                        }
-                        if (single)
-                            acc = av;
-                        else
-                            acc += av - cd;
-                        rec[3 + s] = __float_as_uint(acc);
-                    }
-                }
-                __syncthreads();
-            }
+        if (flag && cbase >= 0) {
+            const int rank = before + __popcll(bal & ((1ull << lane) - 1));
+            uint32_t *pr = a.pre + (int64_t)(cbase + rank) * pst;
+            pr[0] = (uint32_t)(st * kBlock + v);
+            pr[1] = __float_as_uint(cur);
+            for (int s = 0; s < S; ++s) pr[2 + s] = __float_as_uint(m.segs[s].bits ? m.ipq[s * kLT + v] : 0.0f);
         }
-        __syncthreads();
+        __syncthreads();  // xs / ipq / ctl are rewritten by the next chunk
+    }
+}
+
+// ---------------------------------------------------------------- refine
+
+// Refine of the chunks the list scan handed over: one wavefront per chunk descriptor, each
+// with its own LDS (residual, scalars, staged long records), no workgroup barriers.
+struct RWmem {
+    float *r, *sc;
+    uint8_t *lbuf;        // cmax long records
+    float *exv;           // [cmax][S]
+    float *side;          // [cmax][2 S + 2]: fast estimate, PID, per segment |o| and rescale
+};
+
+constexpr int kRW = 4;  // wavefronts of a refine workgroup
+
+__device__ __host__ inline size_t rw_layout(unsigned char *base, const IndexView &ix, int cmax, RWmem *mp) {
+    const int S = ix.nseg;
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        off = (off + 15) & ~size_t(15);
+        unsigned char *p = base ? base + off : nullptr;
+        off += bytes;
+        return p;
+    };
+    RWmem m{};
+    m.lbuf = take((size_t)cmax * ix.long_stride);
+    m.r = reinterpret_cast<float *>(take(ix.D_pad * 4));
+    m.sc = reinterpret_cast<float *>(take((size_t)S * SC_N * 4));
+    m.exv = reinterpret_cast<float *>(take((size_t)cmax * S * 4));
+    m.side = reinterpret_cast<float *>(take((size_t)cmax * (2 * S + 2) * 4));
+    if (mp) *mp = m;
+    return (off + 15) & ~size_t(15);
+}
+
+size_t refine_smem_bytes(const IndexView &ix, int cmax) {
+    return kRW * rw_layout(nullptr, ix, cmax, nullptr) + 16 + (size_t)ix.nseg * sizeof(SegDesc) + 16;
+}
+
+// Persistent over the slice's chunk descriptors (wavefront w takes w, w + all wavefronts, ...).
+template <int DT>
+__global__ __launch_bounds__(kRW * 64) void refine_kernel(SliceArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const IndexView &ix = a.ix;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int S = ix.nseg, CM = a.cmax, L = a.p1 - a.p0;
+    const int rst = cand_stride(S), pst = pre_stride(S), sst = 2 * S + 2;
+    const size_t wbytes = rw_layout(nullptr, ix, CM, nullptr);
+    SegDesc *segs = reinterpret_cast<SegDesc *>(smem + kRW * wbytes + 16);
+    RWmem m;
+    rw_layout(smem + wave * wbytes, ix, CM, &m);
+    for (int i = tid; i < S; i += kRW * 64) segs[i] = ix.segs[i];
+    __syncthreads();
+    const unsigned long long c64 = *reinterpret_cast<const unsigned long long *>(a.ctr + CTR_CAND);
+    const uint32_t ndesc = min((uint32_t)(c64 >> 32), a.desc_cap);
+    const int chunks = (int)(ix.long_stride / 16);
+    const int grp = lane >> 4, gl = lane & 15;
+    for (uint32_t d = blockIdx.x * kRW + wave; d < ndesc; d += gridDim.x * kRW) {
+        const uint32_t *dd = a.desc + (int64_t)d * kDescStride;
+        const int64_t pair = dd[0];
+        const uint32_t cb = dd[2];
+        const int total = (int)dd[3];
+        if (total == 0) continue;  // uniform per wavefront
+        const int64_t q = pair / L;
+        const int p = item_list(a.p0 + (int)(pair - q * L));
+        const uint32_t cid = a.probes[q * a.nprobe + p];
+        const int64_t vec0 = ix.clu_vec[cid];
+        const uint8_t *blk0 = ix.blocks + ix.clu_blk[cid] * ix.block_stride;
+        // the residual and the per-segment scalars of this (query, list): the values the list
+        // scan derived (CaqCluEstimator::prepare, caq_estimator.hpp:94-106)
+        const float *qr = a.qrot + q * ix.D_pad, *cr = ix.crot + (int64_t)cid * ix.D_pad;
+        for (int i = lane; i < ix.D_pad; i += 64) m.r[i] = DT == 1 ? qr[i] : qr[i] - cr[i];
+        __builtin_amdgcn_wave_barrier();
+        for (int s = 0; s < S; ++s) segment_scalars<DT>(segs[s], s, m.r, cr, a.bound + q * S, m.sc);
+        for (int c0 = 0; c0 < total; c0 += CM) {
+            const int nc = min(CM, total - c0);
+            // long records by LDS DMA
+            const int nchunk = nc * chunks;
+            for (int off = 0; off < nchunk; off += 64) {
+                const int i = off + lane;
+                if (i < nchunk) {
+                    const int c = i / chunks, ch = i - c * chunks;
+                    const int64_t gvec = vec0 + (int64_t)a.pre[(int64_t)(cb + c0 + c) * pst];
+                    __builtin_amdgcn_global_load_lds(ix.long_codes + gvec * ix.long_stride + ch * 16,
+                                                     m.lbuf + (size_t)off * 16, 16, 0, 0);
+                }
+            }
+            // while they fly: each candidate's fast estimate, PID, |o| and rescale per segment
+            for (int i = lane; i < nc * (S + 1); i += 64) {
+                const int c = i / (S + 1), j = i - c * (S + 1);
+                const uint32_t *pr = a.pre + (int64_t)(cb + c0 + c) * pst;
+                const int vl = (int)pr[0];
+                const int64_t gvec = vec0 + vl;
+                float *sd = m.side + c * sst;
+                if (j == 0) {
+                    sd[0] = __uint_as_float(pr[1]);
+                    sd[1] = __uint_as_float(ix.ids[gvec]);
+                } else {
+                    const int s = j - 1;
+                    const float *fac = reinterpret_cast<const float *>(blk0 + (int64_t)(vl >> 5) * ix.block_stride);
+                    sd[2 + s] = fac[s * kBlock + (vl & 31)];
+                    sd[2 + S + s] = ix.exfac[(gvec * S + s) * 2];
+                }
+            }
+            dma_wait();
+            __builtin_amdgcn_wave_barrier();
+            // extra-bit inner products, segment by segment (one code path per wavefront step)
+            for (int s = 0; s < S; ++s) {
+                const SegDesc &sd = segs[s];
+                if (sd.bits < 2) continue;
+                for (int c = grp; c < nc; c += 4) {
+                    const float ex = ex_ip_group(sd.bits - 1, m.r + sd.doff, m.lbuf + c * ix.long_stride + sd.long_off,
+                                                 sd.dim, gl);
+                    if (gl == 0) m.exv[c * S + s] = ex;
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+            if (lane < nc) {
+                const int c = lane;
+                const uint32_t *pr = a.pre + (int64_t)(cb + c0 + c) * pst;
+                const float *sd = m.side + c * sst;
+                write_record<DT>(
+                    a.cand + (int64_t)(cb + c0 + c) * rst, __float_as_uint(sd[1]), (int)(pr[0] & 31), sd[0], segs,
+                    m.sc, S, [&](int s) { return sd[2 + s]; }, [&](int s) { return __uint_as_float(pr[2 + s]); },
+                    [&](int s) { return m.exv[c * S + s]; }, [&](int s) { return sd[2 + S + s]; });
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
     }
 }
 
 // ---------------------------------------------------------------- replay
+
+// Cross-lane moves without the LDS crossbar: readlane of a wave-uniform lane, and the DPP
+// whole-wave shift (lane i <- lane i - 1, lane 0 <- fill).
+__device__ __forceinline__ float rdlane(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ uint32_t rdlane(uint32_t v, int l) { return (uint32_t)__builtin_amdgcn_readlane((int)v, l); }
+__device__ __forceinline__ unsigned long long rdlane64(unsigned long long v, int l) {
+    const uint32_t lo = rdlane((uint32_t)v, l), hi = rdlane((uint32_t)(v >> 32), l);
+    return ((unsigned long long)hi << 32) | lo;
+}
+__device__ __forceinline__ uint32_t wave_shr1(uint32_t v, uint32_t fill) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)fill, (int)v, 0x138 /* wave_shr:1 */, 0xF, 0xF, false);
+}
 
 // The replay's ResultPool lives in registers: entry 64 r + lane is row r of lane `lane`
 // (KR rows cover topk <= 64 KR).
@@ -339,15 +472,17 @@ template <int KR>
 struct RegPool {
     float d[KR];
     uint32_t id[KR];
-    // entry i (the same value in every lane)
+    // entry i (the same value in every lane; i wave-uniform)
     __device__ __forceinline__ float dist_at(int i) const {
         float v = 0.0f;
 #pragma unroll
         for (int r = 0; r < KR; ++r)
-            if (r == (i >> 6)) v = __shfl(d[r], i & 63);
+            if (r == (i >> 6)) v = rdlane(d[r], i & 63);
         return v;
     }
 };
+
+constexpr int kMeta = 64;  // lists whose metadata the replay holds at once (one per lane)
 
 template <int DT, int KR>
 __global__ __launch_bounds__(64) void replay_kernel(SliceArgs a) {
@@ -359,9 +494,10 @@ __global__ __launch_bounds__(64) void replay_kernel(SliceArgs a) {
     const int L = a.p1 - a.p0;
     const bool single = S == 1;
     const int bst = binfo_stride(S), rst = cand_stride(S);
-    uint32_t *crec = reinterpret_cast<uint32_t *>(smem);  // the current chunk's candidate records (<= 256)
-    unsigned long long *tabs = reinterpret_cast<unsigned long long *>(crec + kLT * rst);  // kLT * rst dwords: 8-byte aligned
-    unsigned long long *fastcum = tabs, *cumbits = tabs + S;  // per segment prefix sums of bits
+    // LDS: the current chunk's candidate records (<= 256) and two block-info windows (64
+    // blocks each, double-buffered)
+    uint32_t *crec = reinterpret_cast<uint32_t *>(smem);
+    uint32_t *winfo = crec + kLT * rst;
 
     int size = a.first ? 0 : a.pool_size[q];
     RegPool<KR> pool;
@@ -371,16 +507,23 @@ __global__ __launch_bounds__(64) void replay_kernel(SliceArgs a) {
         pool.d[r] = (i < size) ? a.pool_d[q * K + i] : 0.0f;
         pool.id[r] = (i < size) ? a.pool_id[q * K + i] : 0u;
     }
-    if (lane == 0) {
+    // per segment (lane s): the prefix sums of scanned and refined bits, and which segments
+    // carry codes
+    unsigned long long fastcum = 0, cumbits = 0;
+    uint32_t qmask = 0;
+    {
         unsigned long long fc = 0, cb = 0;
         for (int s = 0; s < S; ++s) {
             const SegDesc sd = ix.segs[s];
             if (sd.bits > 0) {
                 fc += (unsigned long long)kBlock * sd.dim;
                 cb += (unsigned long long)sd.dim * (sd.bits - 1);
+                qmask |= 1u << s;
             }
-            fastcum[s] = fc;
-            cumbits[s] = cb;
+            if (s == lane) {
+                fastcum = fc;
+                cumbits = cb;
+            }
         }
     }
     // running metrics: the same value in every lane
@@ -390,7 +533,6 @@ __global__ __launch_bounds__(64) void replay_kernel(SliceArgs a) {
         acc_bits = a.met[q * 3 + 1];
         comp = a.met[q * 3 + 2];
     }
-    __syncthreads();
     const int sbits0 = ix.segs[0].bits, sdim0 = ix.segs[0].dim;
 
     float tail = size == K ? pool.dist_at(K - 1) : FLT_MAX;  // raw (negated for IP)
@@ -403,39 +545,79 @@ __global__ __launch_bounds__(64) void replay_kernel(SliceArgs a) {
         int lo = 0;
 #pragma unroll
         for (int r = 0; r < KR; ++r) lo += __popcll(__ballot(r * 64 + lane < size && pool.d[r] < d));
-        float carry_d = 0.0f;
-        uint32_t carry_i = 0;
+        uint32_t carry_d = 0, carry_i = 0;
 #pragma unroll
         for (int r = 0; r < KR; ++r) {
             const int i = r * 64 + lane;
-            const float up_d = __shfl_up(pool.d[r], 1);
-            const uint32_t up_i = __shfl_up(pool.id[r], 1);
-            const float last_d = __shfl(pool.d[r], 63);
-            const uint32_t last_i = __shfl(pool.id[r], 63);
+            const uint32_t db = __float_as_uint(pool.d[r]);
+            const uint32_t up_d = wave_shr1(db, carry_d);
+            const uint32_t up_i = wave_shr1(pool.id[r], carry_i);
+            carry_d = rdlane(db, 63);
+            carry_i = rdlane(pool.id[r], 63);
             if (i > lo) {
-                pool.d[r] = lane == 0 ? carry_d : up_d;
-                pool.id[r] = lane == 0 ? carry_i : up_i;
+                pool.d[r] = __uint_as_float(up_d);
+                pool.id[r] = up_i;
             } else if (i == lo) {
                 pool.d[r] = d;
                 pool.id[r] = u;
             }
-            carry_d = last_d;
-            carry_i = last_i;
         }
         size += (size < K);
         tail = size == K ? pool.dist_at(K - 1) : FLT_MAX;
         dk = DT == 1 ? -tail : tail;
     };
+
+    const int NPq = a.npq ? a.npq[q] : a.nprobe;
+    const int pend = min(a.p1, NPq + 1);  // items of the first NPq lists
+    // items [mbase, mbase + 64): lane j holds item mbase + j's block-info base, its blocks and
+    // its list's blocks (0 for item 1, which continues list 0)
+    int mbase = a.p0;
+    uint32_t m_base = kNoPair;
+    int m_nb = 0, m_lnb = 0;
+    auto load_meta = [&](int pb) {
+        mbase = pb;
+        const int it = pb + lane;
+        m_base = kNoPair;
+        m_nb = m_lnb = 0;
+        if (it < pend) {
+            m_base = a.pair_blk[q * L + (it - a.p0)];
+            if (m_base != kNoPair) {
+                const int nb = (ix.clu_n[a.probes[q * a.nprobe + item_list(it)]] + kBlock - 1) / kBlock;
+                m_nb = item_end_block(it, nb) - item_first_block(it, nb);
+                m_lnb = it == 1 ? 0 : nb;
+            }
+        }
+    };
+    // block info of the window (item p, blocks w0..w0+63) -> LDS, by LDS DMA (4 bytes a lane)
+    auto fetch_window = [&](int p, int w0, uint32_t *dst) {
+        const uint32_t base = rdlane(m_base, p - mbase);
+        const int nb = (int)rdlane((uint32_t)m_nb, p - mbase);
+        const int ndw = min(64, nb - w0) * bst;
+        const uint32_t *src = a.binfo + base + (int64_t)w0 * bst;
+        for (int i0 = 0; i0 < ndw; i0 += 64)
+            if (i0 + lane < ndw) __builtin_amdgcn_global_load_lds(src + i0 + lane, dst + i0, 4, 0, 0);
+    };
+    // first window at or after (p, w0) of a non-empty item of the slice (p == pend: none)
+    auto seek_window = [&](int &p, int &w0) {
+        for (;;) {
+            if (p >= pend) return;
+            if (p - mbase >= kMeta) load_meta(p);
+            const int nb = (int)rdlane((uint32_t)m_nb, p - mbase);
+            if (rdlane(m_base, p - mbase) != kNoPair && w0 < nb) return;
+            ++p;
+            w0 = 0;
+        }
+    };
     // The candidate records of a chunk (8 blocks, written together by one list-scan step) are
-    // contiguous: one LDS DMA brings them all.  Returns the LDS record index of the block's
-    // first candidate.  off/cnt/csum are the lanes' block info (csum: its chunk's total).
-    int loaded = -1;        // chunk (window-relative) whose records are in LDS
+    // contiguous: one LDS DMA brings them all.  Returns the LDS record index of block b's
+    // first candidate.  off / csum: the lanes' block info (csum: its chunk's total).
+    int loaded = -1;
     uint32_t loaded_base = 0;
     auto block_records = [&](int b, uint32_t off, uint32_t csum) -> int {
         const int ch = b >> 3;
-        const uint32_t cb = __shfl(off, ch * 8);
+        const uint32_t cb = rdlane(off, ch * 8);
         if (ch != loaded) {
-            const int ndw = (int)__shfl(csum, ch * 8) * rst;
+            const int ndw = (int)rdlane(csum, ch * 8) * rst;
             const uint32_t *src = a.cand + (int64_t)cb * rst;
             for (int i0 = 0; i0 < ndw; i0 += 64)
                 if (i0 + lane < ndw) __builtin_amdgcn_global_load_lds(src + i0 + lane, crec + i0, 4, 0, 0);
@@ -444,105 +626,119 @@ __global__ __launch_bounds__(64) void replay_kernel(SliceArgs a) {
             loaded = ch;
             loaded_base = cb;
         }
-        return (int)(__shfl(off, b) - loaded_base);
+        return (int)(rdlane(off, b) - loaded_base);
     };
 
-    const int NPq = a.npq ? a.npq[q] : a.nprobe;
-    const int pend = min(a.p1, NPq);
+    load_meta(a.p0);
+    int cp = a.p0, cw = 0;
+    seek_window(cp, cw);
+    int buf = 0;
+    if (cp < pend) fetch_window(cp, cw, winfo);
+    dma_wait();
+    __builtin_amdgcn_wave_barrier();
     unsigned long long fb_lane = 0;  // fast bits summed per lane, reduced at the end
-    for (int p = a.p0; p < pend; ++p) {
-        const uint32_t base = a.pair_blk[q * L + (p - a.p0)];
-        if (base == kNoPair) continue;  // empty list (or block-info overflow: searched again)
-        const uint32_t cid = a.probes[q * a.nprobe + p];
-        const int n = ix.clu_n[cid];
-        const int nb = (n + kBlock - 1) / kBlock;
-        comp += (unsigned long long)nb * kBlock;  // total_comp_cnt
-        const uint32_t *bi = a.binfo + base;
+    while (cp < pend) {
+        const int nb = (int)rdlane((uint32_t)m_nb, cp - mbase);
+        const uint32_t *wi = winfo + buf * 64 * bst;
+        if (cw == 0) {
+            // once per list: total_comp_cnt, and scanCluster's scanned bits
+            const int lnb = (int)rdlane((uint32_t)m_lnb, cp - mbase);
+            comp += (unsigned long long)lnb * kBlock;
+            if (single && sbits0 > 0) fast_bits += (unsigned long long)lnb * kBlock * sdim0;
+        }
+        // the next window's block info streams in while this one is replayed
+        int np = cp, nw = cw + 64;
+        seek_window(np, nw);
+        if (np < pend) fetch_window(np, nw, winfo + (buf ^ 1) * 64 * bst);
+        const int blk = cw + lane;
+        const bool valid = blk < nb;
+        uint32_t cnt = 0, off = 0;
+        if (valid) {
+            off = wi[lane * bst + 0];
+            cnt = wi[lane * bst + 1];
+        }
+        uint32_t csum = cnt;
+        for (int o = 1; o < 8; o <<= 1) csum += __shfl_xor(csum, o, 8);
+        loaded = -1;
         if (single) {
-            if (sbits0 > 0) fast_bits += (unsigned long long)nb * kBlock * sdim0;
-            for (int w0 = 0; w0 < nb; w0 += 64) {
-                const int blk = w0 + lane;
-                uint32_t cnt = 0, off = 0;
-                if (blk < nb) {
-                    off = bi[blk * bst + 0];
-                    cnt = bi[blk * bst + 1];
+            unsigned long long hb = __ballot(cnt > 0);
+            while (hb) {
+                const int b = __builtin_ctzll(hb);
+                hb &= hb - 1;
+                const int bc = (int)rdlane(cnt, b);
+                const uint32_t *br = crec + block_records(b, off, csum) * rst;
+                float cv = FLT_MAX, av = 0.0f;
+                uint32_t cid = 0;
+                if (lane < bc) {
+                    cid = br[lane * rst + 0];
+                    cv = __uint_as_float(br[lane * rst + 2]);
+                    av = __uint_as_float(br[lane * rst + 3]);
                 }
-                uint32_t csum = cnt;
-                for (int o = 1; o < 8; o <<= 1) csum += __shfl_xor(csum, o, 8);
-                loaded = -1;
-                unsigned long long hb = __ballot(cnt > 0);
-                while (hb) {
-                    const int b = __builtin_ctzll(hb);
-                    hb &= hb - 1;
-                    const int bc = __shfl((int)cnt, b);
-                    const uint32_t *br = crec + block_records(b, off, csum) * rst;
-                    // scanCluster: the block's mask is fixed at its start (saq_searcher.hpp:190-195)
-                    const float cv = lane < bc ? __uint_as_float(br[lane * rst + 2]) : FLT_MAX;
-                    unsigned long long msk = __ballot(lane < bc && cv < dk);
-                    if (sbits0 > 0) acc_bits += (unsigned long long)__popcll(msk) * sdim0 * (sbits0 - 1);
-                    while (msk) {
-                        const int j = __builtin_ctzll(msk);
-                        msk &= msk - 1;
-                        insert(br[j * rst + 0], __uint_as_float(br[j * rst + 3]));
-                    }
+                // scanCluster: the block's mask is fixed at its start (saq_searcher.hpp:190-195)
+                unsigned long long msk = __ballot(lane < bc && cv < dk);
+                if (sbits0 > 0) acc_bits += (unsigned long long)__popcll(msk) * sdim0 * (sbits0 - 1);
+                while (msk) {
+                    const int j = __builtin_ctzll(msk);
+                    msk &= msk - 1;
+                    insert(rdlane(cid, j), rdlane(av, j));
                 }
             }
         } else {
-            for (int w0 = 0; w0 < nb; w0 += 64) {
-                const int blk = w0 + lane;
-                const bool valid = blk < nb;
-                uint32_t cnt = 0, off = 0;
-                float m1 = FLT_MAX, M = FLT_MAX;
-                if (valid) {
-                    off = bi[blk * bst + 0];
-                    cnt = bi[blk * bst + 1];
-                    m1 = __uint_as_float(bi[blk * bst + 2]);
-                    M = __uint_as_float(bi[blk * bst + 3]);
-                }
-                uint32_t csum = cnt;
-                for (int o = 1; o < 8; o <<= 1) csum += __shfl_xor(csum, o, 8);
-                loaded = -1;
-                int cur = 0;
-                for (;;) {
-                    // blocks cur.. see the same distk until the next insert
-                    const bool pass1 = valid && lane >= cur && m1 <= dk;
-                    const unsigned long long cb = __ballot(pass1 && M <= dk && cnt > 0);
-                    const int stop = cb ? __builtin_ctzll(cb) : 64;
-                    if (pass1 && lane <= stop) {
-                        // stage 2 stops at the first quantised segment whose block minimum
-                        // exceeds distk (saq_searcher.hpp:112-132)
-                        int sb = S - 1;
-                        for (int s = 0; s < S; ++s) {
-                            if (ix.segs[s].bits > 0 && __uint_as_float(bi[blk * bst + 4 + s]) > dk) {
-                                sb = s;
-                                break;
-                            }
+            const float m1 = valid ? __uint_as_float(wi[lane * bst + 2]) : FLT_MAX;
+            const float M = valid ? __uint_as_float(wi[lane * bst + 3]) : FLT_MAX;
+            int cur = 0;
+            for (;;) {
+                // blocks cur.. see the same distk until the next insert
+                const bool pass1 = valid && lane >= cur && m1 <= dk;
+                const unsigned long long cbm = __ballot(pass1 && M <= dk && cnt > 0);
+                const int stop = cbm ? __builtin_ctzll(cbm) : 64;
+                // stage 2 stops at the first quantised segment whose block minimum exceeds
+                // distk (saq_searcher.hpp:112-132); the shuffle runs in every lane (a disabled
+                // source lane would read as 0)
+                const bool counted = pass1 && lane <= stop;
+                int sb = S - 1;
+                if (counted) {
+                    for (int s = 0; s < S; ++s) {
+                        if (((qmask >> s) & 1u) && __uint_as_float(wi[lane * bst + 4 + s]) > dk) {
+                            sb = s;
+                            break;
                         }
-                        fb_lane += fastcum[sb];
                     }
-                    if (!cb) break;
-                    const int bc = __shfl((int)cnt, stop);
-                    const uint32_t *br = crec + block_records(stop, off, csum) * rst;
-                    // stage 3 (saq_searcher.hpp:134-158): distk re-read after every insert
-                    const float cv = lane < bc ? __uint_as_float(br[lane * rst + 2]) : FLT_MAX;
-                    unsigned long long rem = bc >= 64 ? ~0ull : ((1ull << bc) - 1);
-                    for (;;) {
-                        const unsigned long long hit = __ballot(lane < bc && cv < dk) & rem;
-                        if (!hit) break;
-                        const int j = __builtin_ctzll(hit);
-                        rem &= ~((2ull << j) - 1);
-                        const float pv = lane < S ? __uint_as_float(br[j * rst + 3 + lane]) : 0.0f;
-                        const unsigned long long ge = __ballot(lane < S && pv >= dk);
-                        const int sb = ge ? __builtin_ctzll(ge) : S - 1;
-                        acc_bits += cumbits[sb];
-                        insert(br[j * rst + 0], __shfl(pv, sb));
-                    }
-                    cur = stop + 1;
                 }
+                const unsigned long long fc = __shfl(fastcum, sb);
+                if (counted) fb_lane += fc;
+                if (!cbm) break;
+                const int bc = (int)rdlane(cnt, stop);
+                const uint32_t *br = crec + block_records(stop, off, csum) * rst;
+                float cv = FLT_MAX;
+                uint32_t cid = 0;
+                if (lane < bc) {
+                    cid = br[lane * rst + 0];
+                    cv = __uint_as_float(br[lane * rst + 2]);
+                }
+                // stage 3 (saq_searcher.hpp:134-158): distk re-read after every insert
+                unsigned long long rem = bc >= 64 ? ~0ull : ((1ull << bc) - 1);
+                for (;;) {
+                    const unsigned long long hit = __ballot(lane < bc && cv < dk) & rem;
+                    if (!hit) break;
+                    const int j = __builtin_ctzll(hit);
+                    rem &= ~((2ull << j) - 1);
+                    const float pv = lane < S ? __uint_as_float(br[j * rst + 3 + lane]) : 0.0f;
+                    const unsigned long long ge = __ballot(lane < S && pv >= dk);
+                    const int sb = ge ? __builtin_ctzll(ge) : S - 1;
+                    acc_bits += rdlane64(cumbits, sb);
+                    insert(rdlane(cid, j), rdlane(pv, sb));
+                }
+                cur = stop + 1;
             }
         }
+        dma_wait();  // the next window's block info
+        __builtin_amdgcn_wave_barrier();
+        cp = np;
+        cw = nw;
+        buf ^= 1;
     }
-    // per-lane fast bits to lane 0's total
+    // per-lane fast bits to the total
     for (int o = 32; o >= 1; o >>= 1) fb_lane += __shfl_xor(fb_lane, o);
     fast_bits += fb_lane;
     if (a.last) {
@@ -580,21 +776,25 @@ __global__ __launch_bounds__(64) void replay_kernel(SliceArgs a) {
 }
 
 size_t replay_smem_bytes(const IndexView &ix) {
-    return (size_t)kLT * cand_stride(ix.nseg) * 4 + 8 + (size_t)ix.nseg * 16 + 16;
+    const int bst = binfo_stride(ix.nseg), rst = cand_stride(ix.nseg);
+    return (size_t)(kLT * rst + 2 * 64 * bst) * 4 + 16;
 }
 
-// counters of a slice: block-info and candidate space restart at zero, the most either
-// reached is kept in ctr[3], ctr[4] (the host sizes the buffers from them); ctr[2] is the
-// overflow flag of the whole search.
+// Counters of a slice: block-info and candidate space restart at zero; the most any slice
+// asked for is kept (the host sizes the buffers from it).  all: a new search (also clears
+// the overflow flag).
 __global__ void slice_reset_kernel(uint32_t *ctr, int all) {
     if (threadIdx.x == 0) {
+        unsigned long long *c64 = reinterpret_cast<unsigned long long *>(ctr + CTR_CAND);
         if (all) {
-            ctr[2] = ctr[3] = ctr[4] = 0;
+            ctr[CTR_OVERFLOW] = ctr[CTR_MAX_BINFO] = ctr[CTR_MAX_CAND] = ctr[CTR_MAX_DESC] = 0;
         } else {
-            ctr[3] = max(ctr[3], ctr[0]);
-            ctr[4] = max(ctr[4], ctr[1]);
+            ctr[CTR_MAX_BINFO] = max(ctr[CTR_MAX_BINFO], ctr[CTR_BINFO]);
+            ctr[CTR_MAX_CAND] = max(ctr[CTR_MAX_CAND], (uint32_t)*c64);
+            ctr[CTR_MAX_DESC] = max(ctr[CTR_MAX_DESC], (uint32_t)(*c64 >> 32));
         }
-        ctr[0] = ctr[1] = 0;
+        ctr[CTR_BINFO] = 0;
+        *c64 = 0;
     }
 }
 
@@ -624,6 +824,17 @@ hipError_t launch_list_scan(const SliceArgs &args, int dist_type, size_t smem, h
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
+}
+
+hipError_t launch_refine(const SliceArgs &a, int dist_type, size_t smem, int grid, hipStream_t st) {
+    const void *fn = dist_type == 1 ? (const void *)refine_kernel<1> : (const void *)refine_kernel<0>;
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+    if (e != hipSuccess) return e;
+    if (dist_type == 1)
+        hipLaunchKernelGGL(refine_kernel<1>, dim3(grid), dim3(kRW * 64), smem, st, a);
+    else
+        hipLaunchKernelGGL(refine_kernel<0>, dim3(grid), dim3(kRW * 64), smem, st, a);
+    return hipGetLastError();
 }
 
 template <int DT>

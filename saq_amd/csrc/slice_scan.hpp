@@ -14,6 +14,34 @@ __host__ __device__ inline int binfo_stride(int nseg) { return nseg == 1 ? 2 : 4
 // candidate record (dwords): [0] PID, [1] lane in its block, [2] fast estimate (the value
 // the masks compare), [3 + s] running accurate sum after segment s.
 __host__ __device__ inline int cand_stride(int nseg) { return nseg + 3; }
+// what the list scan hands the refine kernel per candidate (dwords): [0] vector in its list,
+// [1] fast estimate, [2 + s] segment s's fast-scan inner product (Lut::compFastIP)
+__host__ __device__ inline int pre_stride(int nseg) { return nseg + 2; }
+// chunk descriptor (dwords): [0] pair, [1] first block of the chunk, [2] first candidate, [3] candidates
+constexpr int kDescStride = 4;
+
+// counters (uint32 slots of SliceArgs::ctr)
+enum {
+    CTR_BINFO = 0,     // block-info dwords used by the current slice
+    CTR_OVERFLOW = 2,  // some slice ran out of space: the search is repeated with more
+    CTR_MAX_BINFO,     // most block-info dwords / candidate records / descriptors a slice asked for
+    CTR_MAX_CAND,
+    CTR_MAX_DESC,
+    CTR_CAND = 6,      // [6..8) one 64-bit counter: descriptors << 32 | candidate records
+    CTR_N = 8
+};
+
+// A query's probe sequence as work items: item 0 is the first chunk (8 blocks, 256 vectors) of
+// probed list 0, item 1 the rest of list 0, item j >= 2 all of probed list j - 1.  Slices are
+// ranges of items, so the first slice -- refined without a bound -- stays one chunk long.
+constexpr int kItemChunk = 8;  // blocks of item 0
+__host__ __device__ inline int item_list(int item) { return item == 0 ? 0 : item - 1; }
+__host__ __device__ inline int item_first_block(int item, int nb) {
+    return item == 1 ? (nb < kItemChunk ? nb : kItemChunk) : 0;
+}
+__host__ __device__ inline int item_end_block(int item, int nb) {
+    return item == 0 ? (nb < kItemChunk ? nb : kItemChunk) : nb;
+}
 
 struct SliceArgs {
     IndexView ix;
@@ -22,7 +50,7 @@ struct SliceArgs {
     const uint32_t *probes;
     int32_t nprobe;          // probe-list stride
     const int32_t *npq;      // lists scanned per query (nullptr: nprobe)
-    int32_t p0, p1;          // the slice
+    int32_t p0, p1;          // the slice: items [p0, p1)
     int32_t topk;
     int64_t nq;
     int64_t q0;              // first query of this launch (list-scan launches are split by query)
@@ -31,13 +59,16 @@ struct SliceArgs {
     uint32_t *pool_id;       // nq x topk
     int32_t *pool_size;      // nq
     uint64_t *met;           // nq x 3 running QueryRuntimeMetrics
-    uint32_t *pair_blk;      // nq x (p1 - p0): first block-info dword of each pair
+    uint32_t *pair_blk;      // nq x (p1 - p0): first block-info dword of each (query, item)
     uint32_t *binfo;
-    uint32_t *cand;
+    uint32_t *cand;          // candidate records
+    uint32_t *pre;           // list-scan output per candidate (pre_stride dwords)
+    uint32_t *desc;          // chunk descriptors
     uint32_t binfo_cap;      // dwords
     uint32_t cand_cap;       // records
-    uint32_t *ctr;           // [0] block-info dwords used, [1] candidate records used, [2] overflow
-    int32_t cmax;            // candidates whose long records are staged in LDS at once
+    uint32_t desc_cap;       // descriptors
+    uint32_t *ctr;           // CTR_N counters
+    int32_t cmax;            // candidates whose long records a refine wavefront stages in LDS at once
     uint32_t *out_ids;
     float *out_dists;
     uint64_t *out_metrics;
@@ -45,7 +76,9 @@ struct SliceArgs {
 
 hipError_t launch_slice_reset(uint32_t *ctr, int all, hipStream_t st);
 hipError_t launch_list_scan(const SliceArgs &args, int dist_type, size_t smem, hipStream_t st);
+hipError_t launch_refine(const SliceArgs &args, int dist_type, size_t smem, int grid, hipStream_t st);
 hipError_t launch_replay(const SliceArgs &a, int dist_type, hipStream_t st);
-size_t list_scan_smem_bytes(const IndexView &ix, int cmax);
+size_t list_scan_smem_bytes(const IndexView &ix);
+size_t refine_smem_bytes(const IndexView &ix, int cmax);
 
 }  // namespace saqgpu
