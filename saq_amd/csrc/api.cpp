@@ -625,7 +625,9 @@ int sliced_scan(saqgpu_index *ix, const IndexView &v, const uint32_t *dp, const 
     for (int c = 0; c < ix->num_cen; ++c) nonempty += ix->clu_n[c] > 0;
     const double mean_n = nonempty ? (double)ix->num_data / nonempty : 1.0;
     const int64_t kCapMax = ((int64_t)1 << 31) - 1;
-    auto clamp_cap = [&](double x) { return (int64_t)std::min<double>((double)kCapMax, std::max(x, 4096.0)); };
+    auto clamp_cap = [&](double x) {
+        return (int64_t)std::min<double>((double)kCapMax, std::max(x, 4096.0 * kSlots / 16));
+    };
     // SAQGPU_SLICE_CAP=n (tests): start every capacity at n instead
     const char *cap_env = getenv("SAQGPU_SLICE_CAP");
     const int64_t cap0 = cap_env ? std::max<int64_t>(1, atoll(cap_env)) : 0;
@@ -660,9 +662,10 @@ int sliced_scan(saqgpu_index *ix, const IndexView &v, const uint32_t *dp, const 
         a.cand = ix->sl_cand.p;
         a.pre = ix->sl_pre.p;
         a.desc = ix->sl_desc.p;
-        a.binfo_cap = (uint32_t)std::min<int64_t>(kCapMax, (int64_t)ix->sl_binfo.n);
-        a.cand_cap = (uint32_t)std::min<int64_t>(kCapMax, (int64_t)(ix->sl_cand.n / rst));
-        a.desc_cap = (uint32_t)std::min<int64_t>(kCapMax, (int64_t)(ix->sl_desc.n / kDescStride));
+        // per-slot shares of each buffer (the slots' reservations are independent)
+        a.binfo_cap = (uint32_t)std::min<int64_t>(kCapMax, (int64_t)ix->sl_binfo.n) / kSlots;
+        a.cand_cap = (uint32_t)std::min<int64_t>(kCapMax, (int64_t)(ix->sl_cand.n / rst)) / kSlots;
+        a.desc_cap = (uint32_t)std::min<int64_t>(kCapMax, (int64_t)(ix->sl_desc.n / kDescStride)) / kSlots;
         a.ctr = ix->sl_ctr.p;
         a.cmax = cmax;
         a.out_ids = dids;
@@ -685,16 +688,19 @@ int sliced_scan(saqgpu_index *ix, const IndexView &v, const uint32_t *dp, const 
         HIPCHK(hipStreamSynchronize(st));
         if (!h[CTR_OVERFLOW]) return SAQGPU_OK;
         if (attempt >= 3) return fail(SAQGPU_ENOMEM, "scan workspace did not converge");
-        // grow to what the slices asked for (a slice that ran short only raises later T, so
-        // the counts are upper bounds once block info fits)
-        const int64_t nb = clamp_cap(std::max<double>((double)h[CTR_MAX_BINFO] * 1.25, (double)ix->sl_binfo.n));
-        const int64_t nc = clamp_cap(std::max<double>((double)h[CTR_MAX_CAND] * 1.25, (double)a.cand_cap));
-        const int64_t nd = clamp_cap(std::max<double>((double)h[CTR_MAX_DESC] * 1.25, (double)a.desc_cap));
-        if (nb > (int64_t)ix->sl_binfo.n && ix->sl_binfo.alloc(nb))
+        // grow to what the slices asked for: every slot gets the largest slot's need (a slice
+        // that ran short only raises later T, so the counts are upper bounds once block info fits)
+        const int64_t nb = clamp_cap(std::max<double>((double)h[CTR_MAX_BINFO] * kSlots * 1.25,
+                                                      (double)a.binfo_cap * kSlots));
+        const int64_t nc = clamp_cap(std::max<double>((double)h[CTR_MAX_CAND] * kSlots * 1.25,
+                                                      (double)a.cand_cap * kSlots));
+        const int64_t nd = clamp_cap(std::max<double>((double)h[CTR_MAX_DESC] * kSlots * 1.25,
+                                                      (double)a.desc_cap * kSlots));
+        if (nb > (int64_t)a.binfo_cap * kSlots && ix->sl_binfo.alloc(nb))
             return fail(SAQGPU_ENOMEM, "workspace allocation failed");
-        if (nc > (int64_t)a.cand_cap && (ix->sl_cand.alloc(nc * rst) || ix->sl_pre.alloc(nc * pst)))
+        if (nc > (int64_t)a.cand_cap * kSlots && (ix->sl_cand.alloc(nc * rst) || ix->sl_pre.alloc(nc * pst)))
             return fail(SAQGPU_ENOMEM, "workspace allocation failed");
-        if (nd > (int64_t)a.desc_cap && ix->sl_desc.alloc(nd * kDescStride))
+        if (nd > (int64_t)a.desc_cap * kSlots && ix->sl_desc.alloc(nd * kDescStride))
             return fail(SAQGPU_ENOMEM, "workspace allocation failed");
     }
 }

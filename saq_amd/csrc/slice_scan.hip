@@ -38,6 +38,20 @@ namespace saqgpu {
 constexpr int kLT = 256;               // workgroup of the list scan and the refine
 constexpr int kChunk = kLT / kBlock;   // 8 blocks (256 vectors, one per thread) per chunk
 
+// Cross-lane moves without the LDS crossbar: readlane of a wave-uniform lane, and the DPP
+// whole-wave shift (lane i <- lane i - 1, lane 0 <- fill).
+__device__ __forceinline__ float rdlane(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ uint32_t rdlane(uint32_t v, int l) { return (uint32_t)__builtin_amdgcn_readlane((int)v, l); }
+__device__ __forceinline__ unsigned long long rdlane64(unsigned long long v, int l) {
+    const uint32_t lo = rdlane((uint32_t)v, l), hi = rdlane((uint32_t)(v >> 32), l);
+    return ((unsigned long long)hi << 32) | lo;
+}
+__device__ __forceinline__ uint32_t wave_shr1(uint32_t v, uint32_t fill) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)fill, (int)v, 0x138 /* wave_shr:1 */, 0xF, 0xF, false);
+}
+
 // ---------------------------------------------------------------- list scan
 
 struct LSmem {
@@ -140,11 +154,13 @@ __global__ __launch_bounds__(kLT) void list_scan_kernel(SliceArgs a) {
     const int bst = binfo_stride(S), pst = pre_stride(S);
     LSmem m;
     ls_layout(smem, ix, &m);
+    const int slot = blockIdx.x % kSlots;
     if (tid == 0) {
         const uint32_t need = (uint32_t)(b1 - b0) * (uint32_t)bst;
-        const uint32_t base = atomicAdd(&a.ctr[CTR_BINFO], need);
-        const bool ok = (uint64_t)base + need <= a.binfo_cap;
+        const uint32_t loc = atomicAdd(&a.ctr[CTR_BINFO + slot], need);
+        const bool ok = (uint64_t)loc + need <= a.binfo_cap;
         if (!ok) a.ctr[CTR_OVERFLOW] = 1;
+        const uint32_t base = (uint32_t)slot * a.binfo_cap + loc;
         m.ctl[0] = ok ? (int)base : -1;
         a.pair_blk[pair] = ok ? base : kNoPair;
     }
@@ -174,7 +190,7 @@ __global__ __launch_bounds__(kLT) void list_scan_kernel(SliceArgs a) {
                      m.r, m.sc, m.cbmin, m.cbmax, m.lut, ix.D_pad);
     const uint8_t *blk0 = ix.blocks + ix.clu_blk[cid] * ix.block_stride;
     uint32_t *binfo = a.binfo + binfo_base - (int64_t)b0 * bst;  // indexed by the list's block
-    unsigned long long *cnt64 = reinterpret_cast<unsigned long long *>(a.ctr + CTR_CAND);
+    unsigned long long *cnt64 = reinterpret_cast<unsigned long long *>(a.ctr + CTR_CAND) + slot;
     const int v = tid, bl = v >> 5, ln = v & 31;
 
     for (int st = b0; st < b1; st += kChunk) {
@@ -289,13 +305,14 @@ __global__ __launch_bounds__(kLT) void list_scan_kernel(SliceArgs a) {
         // the chunk's candidates: one descriptor for the refine kernel, `total` records
         if (tid == 0 && total) {
             const unsigned long long old = atomicAdd(cnt64, (1ull << 32) | (unsigned long long)total);
-            const uint32_t cb = (uint32_t)old, d = (uint32_t)(old >> 32);
-            const bool ok = (uint64_t)cb + total <= a.cand_cap && d < a.desc_cap;
+            const uint32_t cl = (uint32_t)old, dl = (uint32_t)(old >> 32);
+            const bool ok = (uint64_t)cl + total <= a.cand_cap && dl < a.desc_cap;
             if (!ok) a.ctr[CTR_OVERFLOW] = 1;
-            // every descriptor slot below the counter is read by the refine kernel: one whose
+            const uint32_t cb = (uint32_t)slot * a.cand_cap + cl;
+            // every descriptor below the slot's counter is read by the refine kernel: one whose
             // records did not fit carries no candidates
-            if (d < a.desc_cap) {
-                uint32_t *dd = a.desc + (int64_t)d * kDescStride;
+            if (dl < a.desc_cap) {
+                uint32_t *dd = a.desc + ((int64_t)slot * a.desc_cap + dl) * kDescStride;
                 dd[0] = (uint32_t)pair;
                 dd[1] = (uint32_t)st;
                 dd[2] = ok ? cb : 0u;
@@ -372,12 +389,23 @@ __global__ __launch_bounds__(kRW * 64) void refine_kernel(SliceArgs a) {
     rw_layout(smem + wave * wbytes, ix, CM, &m);
     for (int i = tid; i < S; i += kRW * 64) segs[i] = ix.segs[i];
     __syncthreads();
-    const unsigned long long c64 = *reinterpret_cast<const unsigned long long *>(a.ctr + CTR_CAND);
-    const uint32_t ndesc = min((uint32_t)(c64 >> 32), a.desc_cap);
+    // descriptors of every slot, numbered slot by slot: lane s holds slot s's count and the
+    // inclusive prefix
+    const uint32_t cnt_s = lane < kSlots
+        ? min((uint32_t)(reinterpret_cast<const unsigned long long *>(a.ctr + CTR_CAND)[lane] >> 32), a.desc_cap)
+        : 0u;
+    uint32_t incl = cnt_s;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t t = __shfl_up(incl, o);
+        if (lane >= o) incl += t;
+    }
+    const uint32_t ndesc = rdlane(incl, 63);
     const int chunks = (int)(ix.long_stride / 16);
     const int grp = lane >> 4, gl = lane & 15;
-    for (uint32_t d = blockIdx.x * kRW + wave; d < ndesc; d += gridDim.x * kRW) {
-        const uint32_t *dd = a.desc + (int64_t)d * kDescStride;
+    for (uint32_t g = blockIdx.x * kRW + wave; g < ndesc; g += gridDim.x * kRW) {
+        const int slot = __popcll(__ballot(incl <= g));
+        const uint32_t local = g - (rdlane(incl, slot) - rdlane(cnt_s, slot));
+        const uint32_t *dd = a.desc + ((int64_t)slot * a.desc_cap + local) * kDescStride;
         const int64_t pair = dd[0];
         const uint32_t cb = dd[2];
         const int total = (int)dd[3];
@@ -451,20 +479,6 @@ __global__ __launch_bounds__(kRW * 64) void refine_kernel(SliceArgs a) {
 }
 
 // ---------------------------------------------------------------- replay
-
-// Cross-lane moves without the LDS crossbar: readlane of a wave-uniform lane, and the DPP
-// whole-wave shift (lane i <- lane i - 1, lane 0 <- fill).
-__device__ __forceinline__ float rdlane(float v, int l) {
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
-}
-__device__ __forceinline__ uint32_t rdlane(uint32_t v, int l) { return (uint32_t)__builtin_amdgcn_readlane((int)v, l); }
-__device__ __forceinline__ unsigned long long rdlane64(unsigned long long v, int l) {
-    const uint32_t lo = rdlane((uint32_t)v, l), hi = rdlane((uint32_t)(v >> 32), l);
-    return ((unsigned long long)hi << 32) | lo;
-}
-__device__ __forceinline__ uint32_t wave_shr1(uint32_t v, uint32_t fill) {
-    return (uint32_t)__builtin_amdgcn_update_dpp((int)fill, (int)v, 0x138 /* wave_shr:1 */, 0xF, 0xF, false);
-}
 
 // The replay's ResultPool lives in registers: entry 64 r + lane is row r of lane `lane`
 // (KR rows cover topk <= 64 KR).
@@ -780,21 +794,36 @@ size_t replay_smem_bytes(const IndexView &ix) {
     return (size_t)(kLT * rst + 2 * 64 * bst) * 4 + 16;
 }
 
-// Counters of a slice: block-info and candidate space restart at zero; the most any slice
-// asked for is kept (the host sizes the buffers from it).  all: a new search (also clears
-// the overflow flag).
+// Counters of a slice: block-info and candidate space restart at zero; the most any slot of
+// any slice asked for is kept (the host sizes the buffers from it).  all: a new search (also
+// clears the overflow flag).  One wavefront, lane s = slot s.
 __global__ void slice_reset_kernel(uint32_t *ctr, int all) {
-    if (threadIdx.x == 0) {
-        unsigned long long *c64 = reinterpret_cast<unsigned long long *>(ctr + CTR_CAND);
+    const int s = threadIdx.x;
+    unsigned long long *c64 = reinterpret_cast<unsigned long long *>(ctr + CTR_CAND);
+    uint32_t b = 0, c = 0, d = 0;
+    if (s < kSlots) {
+        b = ctr[CTR_BINFO + s];
+        c = (uint32_t)c64[s];
+        d = (uint32_t)(c64[s] >> 32);
+    }
+    for (int o = 32; o >= 1; o >>= 1) {
+        b = max(b, (uint32_t)__shfl_xor(b, o));
+        c = max(c, (uint32_t)__shfl_xor(c, o));
+        d = max(d, (uint32_t)__shfl_xor(d, o));
+    }
+    __syncthreads();
+    if (s == 0) {
         if (all) {
             ctr[CTR_OVERFLOW] = ctr[CTR_MAX_BINFO] = ctr[CTR_MAX_CAND] = ctr[CTR_MAX_DESC] = 0;
         } else {
-            ctr[CTR_MAX_BINFO] = max(ctr[CTR_MAX_BINFO], ctr[CTR_BINFO]);
-            ctr[CTR_MAX_CAND] = max(ctr[CTR_MAX_CAND], (uint32_t)*c64);
-            ctr[CTR_MAX_DESC] = max(ctr[CTR_MAX_DESC], (uint32_t)(*c64 >> 32));
+            ctr[CTR_MAX_BINFO] = max(ctr[CTR_MAX_BINFO], b);
+            ctr[CTR_MAX_CAND] = max(ctr[CTR_MAX_CAND], c);
+            ctr[CTR_MAX_DESC] = max(ctr[CTR_MAX_DESC], d);
         }
-        ctr[CTR_BINFO] = 0;
-        *c64 = 0;
+    }
+    if (s < kSlots) {
+        ctr[CTR_BINFO + s] = 0;
+        c64[s] = 0;
     }
 }
 

@@ -20,15 +20,18 @@ __host__ __device__ inline int pre_stride(int nseg) { return nseg + 2; }
 // chunk descriptor (dwords): [0] pair, [1] first block of the chunk, [2] first candidate, [3] candidates
 constexpr int kDescStride = 4;
 
-// counters (uint32 slots of SliceArgs::ctr)
+// Block info, candidate records and descriptors are reserved from kSlots independent counters,
+// each owning an equal share of every buffer: atomics on one address serialise.
+constexpr int kSlots = 64;
+// counters (uint32 entries of SliceArgs::ctr)
 enum {
-    CTR_BINFO = 0,     // block-info dwords used by the current slice
-    CTR_OVERFLOW = 2,  // some slice ran out of space: the search is repeated with more
-    CTR_MAX_BINFO,     // most block-info dwords / candidate records / descriptors a slice asked for
-    CTR_MAX_CAND,
+    CTR_OVERFLOW = 0,  // some slice ran out of space: the search is repeated with more
+    CTR_MAX_BINFO,     // the most block-info dwords / candidate records / descriptors one slot
+    CTR_MAX_CAND,      //   of one slice asked for
     CTR_MAX_DESC,
-    CTR_CAND = 6,      // [6..8) one 64-bit counter: descriptors << 32 | candidate records
-    CTR_N = 8
+    CTR_BINFO = 4,                    // [kSlots] block-info dwords used per slot
+    CTR_CAND = CTR_BINFO + kSlots,    // [kSlots] 64-bit: descriptors << 32 | candidate records
+    CTR_N = CTR_CAND + 2 * kSlots
 };
 
 // A query's probe sequence as work items: item 0 is the first chunk (8 blocks, 256 vectors) of
@@ -64,9 +67,9 @@ struct SliceArgs {
     uint32_t *cand;          // candidate records
     uint32_t *pre;           // list-scan output per candidate (pre_stride dwords)
     uint32_t *desc;          // chunk descriptors
-    uint32_t binfo_cap;      // dwords
-    uint32_t cand_cap;       // records
-    uint32_t desc_cap;       // descriptors
+    uint32_t binfo_cap;      // dwords per slot
+    uint32_t cand_cap;       // records per slot
+    uint32_t desc_cap;       // descriptors per slot
     uint32_t *ctr;           // CTR_N counters
     int32_t cmax;            // candidates whose long records a refine wavefront stages in LDS at once
     uint32_t *out_ids;
