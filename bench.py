@@ -304,6 +304,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=3.0, help="duration of each serving-loop load")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving loop (kernel profiling runs)")
     ap.add_argument("--exact-probe", action="store_true",
                     help="batched scan with the reference-order coarse quantiser instead of the fp32-MFMA one")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -448,6 +449,7 @@ def main():
         W["npq_range"]), "ms_per_step": round(dt / steps * 1e3, 4),
              "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                          "kernel": "sliced scan: list_scan_kernel + refine_kernel + replay_kernel per slice",
                           "alg_bytes_per_launch": int(alg_bytes), "kernel_ms": round(kern_ms, 4)}}
     log(f"plain: {qps:.0f} QPS, scan kernel {kern_ms:.3f} ms, {achieved:.0f} GB/s")
 
@@ -527,7 +529,8 @@ def main():
 
     # ---- serving loop: latency under Poisson arrivals (dynamic batching over the batched scan)
     loads = W.get("serve_loads") or (min(200000, int(0.3 * bqps)),)
-    serving = serve(torch, ix, queries, loads, args.serve_seconds, 5, bprec, args.exact_probe, log)
+    serving = [] if args.no_serve else serve(torch, ix, queries, loads, args.serve_seconds, 5, bprec,
+                                             args.exact_probe, log)
 
     cpu = None
     if not args.no_cpu:
@@ -563,7 +566,7 @@ def main():
                                f"nlist={nlist} B={bits} nq={nq} k={args.topk}",
                    "bits": bits, "plan": plan, "recall_at_10_by_bits": bits_tried,
                    "batched": batched, "plain": plain, "batched_vs_plain": round(bqps / qps, 3),
-                   "serving": serving, "p99_ms": serving[-1]["p99_ms"],
+                   "serving": serving, "p99_ms": serving[-1]["p99_ms"] if serving else None,
                    "encode_mvec_per_s": round(n / t_enc / 1e6, 3)},
         "roofline": {"bound": "hbm", "achieved": round(b_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(b_gbs / HBM_PEAK_GBS, 4), "traffic": btraffic,

@@ -591,8 +591,10 @@ namespace {
 // pairs and one replay launch.  Block-info and candidate space that a slice runs out of is
 // flagged on the device; the search is then repeated with the space it asked for (the
 // capacities are kept for later searches), so results never depend on the capacities.
+// e0 / e1 (optional): events recorded around the kernels of the attempt that succeeds.
 int sliced_scan(saqgpu_index *ix, const IndexView &v, const uint32_t *dp, const saqgpu_search_params *p,
-                const int32_t *dnpq, int64_t nq, uint32_t *dids, float *dd, uint64_t *dm, hipStream_t st) {
+                const int32_t *dnpq, int64_t nq, uint32_t *dids, float *dd, uint64_t *dm, hipStream_t st,
+                hipEvent_t e0, hipEvent_t e1) {
     const int K = p->topk, S = ix->nseg;
     // slices of the work items (item 0: list 0's first chunk; 1: its rest; j >= 2: list j - 1)
     const int items = p->nprobe + 1;
@@ -671,6 +673,7 @@ int sliced_scan(saqgpu_index *ix, const IndexView &v, const uint32_t *dp, const 
         a.out_ids = dids;
         a.out_dists = dd;
         a.out_metrics = dm;
+        if (e0) HIPCHK(hipEventRecord(e0, st));
         HIPCHK(launch_slice_reset(a.ctr, 1, st));
         for (size_t i = 0; i + 1 < bounds.size(); ++i) {
             a.p0 = bounds[i];
@@ -683,6 +686,7 @@ int sliced_scan(saqgpu_index *ix, const IndexView &v, const uint32_t *dp, const 
             HIPCHK(launch_replay(a, p->dist_type, st));
         }
         HIPCHK(launch_slice_reset(a.ctr, 0, st));
+        if (e1) HIPCHK(hipEventRecord(e1, st));
         uint32_t h[CTR_N];
         HIPCHK(hipMemcpyAsync(h, a.ctr, sizeof(h), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
@@ -790,7 +794,7 @@ int saqgpu_search(saqgpu_index *ix, const float *queries, int64_t nq, const uint
     if (ix->timing) {
         HIPCHK(hipEventCreate(&e0));
         HIPCHK(hipEventCreate(&e1));
-        HIPCHK(hipEventRecord(e0, st));
+        if (fused) HIPCHK(hipEventRecord(e0, st));
     }
     if (fused) {
         // SAQGPU_PROFILE=1: per-phase shader clocks of the fused kernel, printed to stderr
@@ -811,14 +815,12 @@ int saqgpu_search(saqgpu_index *ix, const float *queries, int64_t nq, const uint
             fprintf(stderr, "\n");
             prof.release();
         }
+        if (ix->timing) HIPCHK(hipEventRecord(e1, st));
     } else {
-        const int rc2 = sliced_scan(ix, v, dp, p, dnpq, nq, dids, dd, dm, st);
+        const int rc2 = sliced_scan(ix, v, dp, p, dnpq, nq, dids, dd, dm, st, e0, e1);
         if (rc2 != SAQGPU_OK) return rc2;
     }
-    if (ix->timing) {
-        HIPCHK(hipEventRecord(e1, st));
-        ix->events.emplace_back(e0, e1);
-    }
+    if (ix->timing) ix->events.emplace_back(e0, e1);
     if (!dev) {
         HIPCHK(hipMemcpyAsync(out_ids, dids, nq * p->topk * 4, hipMemcpyDeviceToHost, st));
         if (out_dists) HIPCHK(hipMemcpyAsync(out_dists, dd, nq * p->topk * 4, hipMemcpyDeviceToHost, st));
