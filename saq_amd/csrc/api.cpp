@@ -643,7 +643,7 @@ int sliced_scan(saqgpu_index *ix, const IndexView &v, const uint32_t *dp, const 
             return fail(SAQGPU_ENOMEM, "workspace allocation failed");
     }
     if (ix->sl_desc.n == 0 &&
-        ix->sl_desc.alloc((cap0 ? cap0 : clamp_cap((double)nq * Lmax * (mean_n / 256 + 1))) * kDescStride))
+        ix->sl_desc.alloc((cap0 ? cap0 : clamp_cap((double)nq * Lmax * (mean_n / 256 + 1))) * desc_stride(S)))
         return fail(SAQGPU_ENOMEM, "workspace allocation failed");
     for (int attempt = 0;; ++attempt) {
         SliceArgs a{};
@@ -667,7 +667,7 @@ int sliced_scan(saqgpu_index *ix, const IndexView &v, const uint32_t *dp, const 
         // per-slot shares of each buffer (the slots' reservations are independent)
         a.binfo_cap = (uint32_t)std::min<int64_t>(kCapMax, (int64_t)ix->sl_binfo.n) / kSlots;
         a.cand_cap = (uint32_t)std::min<int64_t>(kCapMax, (int64_t)(ix->sl_cand.n / rst)) / kSlots;
-        a.desc_cap = (uint32_t)std::min<int64_t>(kCapMax, (int64_t)(ix->sl_desc.n / kDescStride)) / kSlots;
+        a.desc_cap = (uint32_t)std::min<int64_t>(kCapMax, (int64_t)(ix->sl_desc.n / desc_stride(S))) / kSlots;
         a.ctr = ix->sl_ctr.p;
         a.cmax = cmax;
         a.out_ids = dids;
@@ -704,7 +704,7 @@ int sliced_scan(saqgpu_index *ix, const IndexView &v, const uint32_t *dp, const 
             return fail(SAQGPU_ENOMEM, "workspace allocation failed");
         if (nc > (int64_t)a.cand_cap * kSlots && (ix->sl_cand.alloc(nc * rst) || ix->sl_pre.alloc(nc * pst)))
             return fail(SAQGPU_ENOMEM, "workspace allocation failed");
-        if (nd > (int64_t)a.desc_cap * kSlots && ix->sl_desc.alloc(nd * kDescStride))
+        if (nd > (int64_t)a.desc_cap * kSlots && ix->sl_desc.alloc(nd * desc_stride(S)))
             return fail(SAQGPU_ENOMEM, "workspace allocation failed");
     }
 }

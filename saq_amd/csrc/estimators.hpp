@@ -253,8 +253,21 @@ __device__ __forceinline__ void segment_scalars(const SegDesc &sd, int s, const 
 template <int DT>
 __device__ void prepare_scalars(const SegDesc *segs, int S, const float *qr, const float *crbuf, const float *bound,
                                 float *r, float *sc, int D_pad) {
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    for (int d = tid; d < D_pad; d += kPrepThreads) r[d] = DT == 1 ? qr[d] : qr[d] - crbuf[d];
+    const int tid = threadIdx.x, wave = tid >> 6;
+    // the residual; a thread's loads (up to 8 each of q and c) are all in flight at once
+    float qv[8], cv[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        const int d = tid + u * kPrepThreads;
+        qv[u] = d < D_pad ? qr[d] : 0.0f;
+        cv[u] = (DT != 1 && d < D_pad) ? crbuf[d] : 0.0f;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        const int d = tid + u * kPrepThreads;
+        if (d < D_pad) r[d] = DT == 1 ? qv[u] : qv[u] - cv[u];
+    }
+    for (int d = tid + 8 * kPrepThreads; d < D_pad; d += kPrepThreads) r[d] = DT == 1 ? qr[d] : qr[d] - crbuf[d];
     __syncthreads();
     for (int s = wave; s < S; s += 4) segment_scalars<DT>(segs[s], s, r, crbuf, bound, sc);
 }

@@ -151,7 +151,7 @@ __global__ __launch_bounds__(kLT) void list_scan_kernel(SliceArgs a) {
         return;  // uniform per workgroup
     }
     const bool single = S == 1;
-    const int bst = binfo_stride(S), pst = pre_stride(S);
+    const int bst = binfo_stride(S), pst = pre_stride(S), dstr = desc_stride(S);
     LSmem m;
     ls_layout(smem, ix, &m);
     const int slot = blockIdx.x % kSlots;
@@ -202,7 +202,15 @@ __global__ __launch_bounds__(kLT) void list_scan_kernel(SliceArgs a) {
         if (active) {
             const uint8_t *rec = blk0 + (int64_t)(st + bl) * ix.block_stride;
             const float *fac = reinterpret_cast<const float *>(rec);
-            for (int s = 0; s < S; ++s) m.xs[s * kLT + v] = fac[s * kBlock + ln];
+            {  // |o| of every segment: the loads first, then the stores
+                float xv[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) xv[u] = u < S ? fac[u * kBlock + ln] : 0.0f;
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    if (u < S) m.xs[u * kLT + v] = xv[u];
+                for (int s = 8; s < S; ++s) m.xs[s * kLT + v] = fac[s * kBlock + ln];
+            }
             const uint8_t *codes = rec + ix.fac_floats * 4 + ln * 8;
             uint32_t acc = 0;
             int bk = 0;
@@ -312,16 +320,24 @@ __global__ __launch_bounds__(kLT) void list_scan_kernel(SliceArgs a) {
             // every descriptor below the slot's counter is read by the refine kernel: one whose
             // records did not fit carries no candidates
             if (dl < a.desc_cap) {
-                uint32_t *dd = a.desc + ((int64_t)slot * a.desc_cap + dl) * kDescStride;
-                dd[0] = (uint32_t)pair;
-                dd[1] = (uint32_t)st;
+                uint32_t *dd = a.desc + ((int64_t)slot * a.desc_cap + dl) * dstr;
+                dd[0] = (uint32_t)q;
+                dd[1] = cid;
                 dd[2] = ok ? cb : 0u;
                 dd[3] = ok ? (uint32_t)total : 0u;
             }
             m.ctl[1] = ok ? (int)cb : -1;
+            m.ctl[2] = dl < a.desc_cap ? (int)dl : -1;
         }
         __syncthreads();
         const int cbase = total ? m.ctl[1] : 0;
+        if (total && m.ctl[2] >= 0 && tid < kDescSc * S) {
+            // the descriptor's copy of the list's scalars (the refine kernel's estimator state)
+            constexpr int kSc[kDescSc] = {SC_SUMQCB, SC_QL2, SC_SUMQ, SC_VFAC, SC_IPQC};
+            const int s = tid / kDescSc, j = tid - s * kDescSc;
+            a.desc[((int64_t)slot * a.desc_cap + m.ctl[2]) * dstr + kDescHead + tid] =
+                __float_as_uint(m.sc[s * SC_N + kSc[j]]);
+        }
         if (active && ln == 0) {
             const uint32_t half = lane < 32 ? (uint32_t)bal : (uint32_t)(bal >> 32);
             const int boff = before + (lane < 32 ? 0 : __popc((uint32_t)bal));
@@ -342,12 +358,15 @@ __global__ __launch_bounds__(kLT) void list_scan_kernel(SliceArgs a) {
 // ---------------------------------------------------------------- refine
 
 // Refine of the chunks the list scan handed over: one wavefront per chunk descriptor, each
-// with its own LDS (residual, scalars, staged long records), no workgroup barriers.
+// with its own LDS, no workgroup barriers.  Every global read is an LDS DMA issued in a batch
+// (the query and centroid rows, the round's pre-records, then the long records and the
+// per-candidate PIDs, |o| and rescale factors), so a round waits for memory twice.
 struct RWmem {
-    float *r, *sc;
+    float *r, *cbuf, *sc;
     uint8_t *lbuf;        // cmax long records
     float *exv;           // [cmax][S]
-    float *side;          // [cmax][2 S + 2]: fast estimate, PID, per segment |o| and rescale
+    uint32_t *pre;        // [cmax][pre_stride]
+    float *side;          // [cmax][2 S + 1]: PID, per segment |o|, per segment rescale
 };
 
 constexpr int kRW = 4;  // wavefronts of a refine workgroup
@@ -364,9 +383,11 @@ __device__ __host__ inline size_t rw_layout(unsigned char *base, const IndexView
     RWmem m{};
     m.lbuf = take((size_t)cmax * ix.long_stride);
     m.r = reinterpret_cast<float *>(take(ix.D_pad * 4));
+    m.cbuf = reinterpret_cast<float *>(take(ix.D_pad * 4));
     m.sc = reinterpret_cast<float *>(take((size_t)S * SC_N * 4));
     m.exv = reinterpret_cast<float *>(take((size_t)cmax * S * 4));
-    m.side = reinterpret_cast<float *>(take((size_t)cmax * (2 * S + 2) * 4));
+    m.pre = reinterpret_cast<uint32_t *>(take((size_t)cmax * pre_stride(S) * 4));
+    m.side = reinterpret_cast<float *>(take((size_t)cmax * (2 * S + 1) * 4 + 256));
     if (mp) *mp = m;
     return (off + 15) & ~size_t(15);
 }
@@ -375,14 +396,20 @@ size_t refine_smem_bytes(const IndexView &ix, int cmax) {
     return kRW * rw_layout(nullptr, ix, cmax, nullptr) + 16 + (size_t)ix.nseg * sizeof(SegDesc) + 16;
 }
 
+// 4-byte LDS DMA of n consecutive dwords by one wavefront (lane l -> dst + l of each 64)
+__device__ __forceinline__ void dma_dwords(const uint32_t *src, uint32_t *dst, int n, int lane) {
+    for (int i0 = 0; i0 < n; i0 += 64)
+        if (i0 + lane < n) __builtin_amdgcn_global_load_lds(src + i0 + lane, dst + i0, 4, 0, 0);
+}
+
 // Persistent over the slice's chunk descriptors (wavefront w takes w, w + all wavefronts, ...).
 template <int DT>
 __global__ __launch_bounds__(kRW * 64) void refine_kernel(SliceArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     const IndexView &ix = a.ix;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int S = ix.nseg, CM = a.cmax, L = a.p1 - a.p0;
-    const int rst = cand_stride(S), pst = pre_stride(S), sst = 2 * S + 2;
+    const int S = ix.nseg, CM = a.cmax;
+    const int rst = cand_stride(S), pst = pre_stride(S), sst = 2 * S + 1, dstr = desc_stride(S);
     const size_t wbytes = rw_layout(nullptr, ix, CM, nullptr);
     SegDesc *segs = reinterpret_cast<SegDesc *>(smem + kRW * wbytes + 16);
     RWmem m;
@@ -405,50 +432,64 @@ __global__ __launch_bounds__(kRW * 64) void refine_kernel(SliceArgs a) {
     for (uint32_t g = blockIdx.x * kRW + wave; g < ndesc; g += gridDim.x * kRW) {
         const int slot = __popcll(__ballot(incl <= g));
         const uint32_t local = g - (rdlane(incl, slot) - rdlane(cnt_s, slot));
-        const uint32_t *dd = a.desc + ((int64_t)slot * a.desc_cap + local) * kDescStride;
-        const int64_t pair = dd[0];
+        const uint32_t *dd = a.desc + ((int64_t)slot * a.desc_cap + local) * dstr;
+        const int64_t q = dd[0];
+        const uint32_t cid = dd[1];
         const uint32_t cb = dd[2];
         const int total = (int)dd[3];
         if (total == 0) continue;  // uniform per wavefront
-        const int64_t q = pair / L;
-        const int p = item_list(a.p0 + (int)(pair - q * L));
-        const uint32_t cid = a.probes[q * a.nprobe + p];
         const int64_t vec0 = ix.clu_vec[cid];
-        const uint8_t *blk0 = ix.blocks + ix.clu_blk[cid] * ix.block_stride;
+        const int64_t blk0 = ix.clu_blk[cid];
         // the residual and the per-segment scalars of this (query, list): the values the list
         // scan derived (CaqCluEstimator::prepare, caq_estimator.hpp:94-106)
-        const float *qr = a.qrot + q * ix.D_pad, *cr = ix.crot + (int64_t)cid * ix.D_pad;
-        for (int i = lane; i < ix.D_pad; i += 64) m.r[i] = DT == 1 ? qr[i] : qr[i] - cr[i];
+        dma_dwords(reinterpret_cast<const uint32_t *>(a.qrot + q * ix.D_pad), reinterpret_cast<uint32_t *>(m.r),
+                   ix.D_pad, lane);
+        dma_dwords(reinterpret_cast<const uint32_t *>(ix.crot + (int64_t)cid * ix.D_pad),
+                   reinterpret_cast<uint32_t *>(m.cbuf), ix.D_pad, lane);
+        dma_dwords(a.pre + (int64_t)cb * pst, m.pre, min(CM, total) * pst, lane);
+        if (lane < kDescSc * S) {  // the list scan's scalars of this (query, list)
+            constexpr int kSc[kDescSc] = {SC_SUMQCB, SC_QL2, SC_SUMQ, SC_VFAC, SC_IPQC};
+            const int s = lane / kDescSc, j = lane - s * kDescSc;
+            m.sc[s * SC_N + kSc[j]] = __uint_as_float(dd[kDescHead + lane]);
+        }
+        dma_wait();
         __builtin_amdgcn_wave_barrier();
-        for (int s = 0; s < S; ++s) segment_scalars<DT>(segs[s], s, m.r, cr, a.bound + q * S, m.sc);
+        if (DT != 1)
+            for (int i = lane; i < ix.D_pad; i += 64) m.r[i] = m.r[i] - m.cbuf[i];
+        __builtin_amdgcn_wave_barrier();
         for (int c0 = 0; c0 < total; c0 += CM) {
             const int nc = min(CM, total - c0);
-            // long records by LDS DMA
+            if (c0 > 0) {  // the round's pre-records (the first round's came with the rows)
+                dma_dwords(a.pre + (int64_t)(cb + c0) * pst, m.pre, nc * pst, lane);
+                dma_wait();
+                __builtin_amdgcn_wave_barrier();
+            }
+            // long records, and per candidate its PID, |o| and rescale of every segment
             const int nchunk = nc * chunks;
             for (int off = 0; off < nchunk; off += 64) {
                 const int i = off + lane;
                 if (i < nchunk) {
                     const int c = i / chunks, ch = i - c * chunks;
-                    const int64_t gvec = vec0 + (int64_t)a.pre[(int64_t)(cb + c0 + c) * pst];
+                    const int64_t gvec = vec0 + (int64_t)m.pre[c * pst];
                     __builtin_amdgcn_global_load_lds(ix.long_codes + gvec * ix.long_stride + ch * 16,
                                                      m.lbuf + (size_t)off * 16, 16, 0, 0);
                 }
             }
-            // while they fly: each candidate's fast estimate, PID, |o| and rescale per segment
-            for (int i = lane; i < nc * (S + 1); i += 64) {
-                const int c = i / (S + 1), j = i - c * (S + 1);
-                const uint32_t *pr = a.pre + (int64_t)(cb + c0 + c) * pst;
-                const int vl = (int)pr[0];
-                const int64_t gvec = vec0 + vl;
-                float *sd = m.side + c * sst;
-                if (j == 0) {
-                    sd[0] = __uint_as_float(pr[1]);
-                    sd[1] = __uint_as_float(ix.ids[gvec]);
-                } else {
-                    const int s = j - 1;
-                    const float *fac = reinterpret_cast<const float *>(blk0 + (int64_t)(vl >> 5) * ix.block_stride);
-                    sd[2 + s] = fac[s * kBlock + (vl & 31)];
-                    sd[2 + S + s] = ix.exfac[(gvec * S + s) * 2];
+            const int nside = nc * sst;
+            for (int i0 = 0; i0 < nside; i0 += 64) {
+                const int i = i0 + lane;
+                if (i < nside) {
+                    const int c = i / sst, j = i - c * sst;
+                    const int vl = (int)m.pre[c * pst];
+                    const int64_t gvec = vec0 + vl;
+                    const void *src;
+                    if (j == 0)
+                        src = ix.ids + gvec;
+                    else if (j <= S)
+                        src = ix.blocks + (blk0 + (vl >> 5)) * ix.block_stride + ((j - 1) * kBlock + (vl & 31)) * 4;
+                    else
+                        src = ix.exfac + (gvec * S + (j - 1 - S)) * 2;
+                    __builtin_amdgcn_global_load_lds(src, m.side + i0, 4, 0, 0);
                 }
             }
             dma_wait();
@@ -466,12 +507,13 @@ __global__ __launch_bounds__(kRW * 64) void refine_kernel(SliceArgs a) {
             __builtin_amdgcn_wave_barrier();
             if (lane < nc) {
                 const int c = lane;
-                const uint32_t *pr = a.pre + (int64_t)(cb + c0 + c) * pst;
+                const uint32_t *pr = m.pre + c * pst;
                 const float *sd = m.side + c * sst;
                 write_record<DT>(
-                    a.cand + (int64_t)(cb + c0 + c) * rst, __float_as_uint(sd[1]), (int)(pr[0] & 31), sd[0], segs,
-                    m.sc, S, [&](int s) { return sd[2 + s]; }, [&](int s) { return __uint_as_float(pr[2 + s]); },
-                    [&](int s) { return m.exv[c * S + s]; }, [&](int s) { return sd[2 + S + s]; });
+                    a.cand + (int64_t)(cb + c0 + c) * rst, __float_as_uint(sd[0]), (int)(pr[0] & 31),
+                    __uint_as_float(pr[1]), segs, m.sc, S, [&](int s) { return sd[1 + s]; },
+                    [&](int s) { return __uint_as_float(pr[2 + s]); }, [&](int s) { return m.exv[c * S + s]; },
+                    [&](int s) { return sd[1 + S + s]; });
             }
             __builtin_amdgcn_wave_barrier();
         }

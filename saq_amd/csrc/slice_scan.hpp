@@ -17,8 +17,10 @@ __host__ __device__ inline int cand_stride(int nseg) { return nseg + 3; }
 // what the list scan hands the refine kernel per candidate (dwords): [0] vector in its list,
 // [1] fast estimate, [2 + s] segment s's fast-scan inner product (Lut::compFastIP)
 __host__ __device__ inline int pre_stride(int nseg) { return nseg + 2; }
-// chunk descriptor (dwords): [0] pair, [1] first block of the chunk, [2] first candidate, [3] candidates
-constexpr int kDescStride = 4;
+// chunk descriptor (dwords): [0] query, [1] list id, [2] first candidate, [3] candidates, then
+// per segment the list scan's scalars SC_SUMQCB, SC_QL2, SC_SUMQ, SC_VFAC, SC_IPQC
+constexpr int kDescHead = 4, kDescSc = 5;
+__host__ __device__ inline int desc_stride(int nseg) { return kDescHead + kDescSc * nseg; }
 
 // Block info, candidate records and descriptors are reserved from kSlots independent counters,
 // each owning an equal share of every buffer: atomics on one address serialise.
