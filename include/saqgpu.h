@@ -132,7 +132,8 @@ int saqgpu_probe(saqgpu_index *idx, const float *queries, int64_t nq, int32_t np
 
 /* Replaces IVF::search (ivf.hpp:275-311) for a batch of queries (nq x num_dim).  probes may be
  * NULL (the GPU coarse quantiser picks them) or nq x nprobe cluster ids.  out_ids is
- * nq x topk (unfilled = 0, like ResultPool::copy_results); out_dists and metrics may be NULL. */
+ * nq x topk (unfilled = 0, like ResultPool::copy_results); out_dists and metrics may be NULL.
+ * Returns when the results are written (also with SAQGPU_DEVICE_PTRS). */
 int saqgpu_search(saqgpu_index *idx, const float *queries, int64_t nq, const uint32_t *probes,
                   const saqgpu_search_params *params, uint32_t *out_ids, float *out_dists,
                   saqgpu_metrics *metrics, int32_t flags, void *stream);
