@@ -310,33 +310,46 @@ __global__ __launch_bounds__(kLT) void list_scan_kernel(SliceArgs a) {
             if (w < wave) before += c;
             total += c;
         }
-        // the chunk's candidates: one descriptor for the refine kernel, `total` records
+        // the chunk's candidates: `total` contiguous records and one descriptor per cmax of them
+        // (a refine wavefront takes one descriptor in one round)
+        const int ndesc = (total + a.cmax - 1) / a.cmax;
         if (tid == 0 && total) {
-            const unsigned long long old = atomicAdd(cnt64, (1ull << 32) | (unsigned long long)total);
+            const unsigned long long old =
+                atomicAdd(cnt64, ((unsigned long long)ndesc << 32) | (unsigned long long)total);
             const uint32_t cl = (uint32_t)old, dl = (uint32_t)(old >> 32);
-            const bool ok = (uint64_t)cl + total <= a.cand_cap && dl < a.desc_cap;
+            const bool ok = (uint64_t)cl + total <= a.cand_cap && (uint64_t)dl + ndesc <= a.desc_cap;
             if (!ok) a.ctr[CTR_OVERFLOW] = 1;
-            const uint32_t cb = (uint32_t)slot * a.cand_cap + cl;
-            // every descriptor below the slot's counter is read by the refine kernel: one whose
-            // records did not fit carries no candidates
-            if (dl < a.desc_cap) {
-                uint32_t *dd = a.desc + ((int64_t)slot * a.desc_cap + dl) * dstr;
-                dd[0] = (uint32_t)q;
-                dd[1] = cid;
-                dd[2] = ok ? cb : 0u;
-                dd[3] = ok ? (uint32_t)total : 0u;
-            }
-            m.ctl[1] = ok ? (int)cb : -1;
-            m.ctl[2] = dl < a.desc_cap ? (int)dl : -1;
+            m.ctl[1] = ok ? (int)(slot * a.cand_cap + cl) : -1;
+            m.ctl[2] = (int)dl;
         }
         __syncthreads();
         const int cbase = total ? m.ctl[1] : 0;
-        if (total && m.ctl[2] >= 0 && tid < kDescSc * S) {
-            // the descriptor's copy of the list's scalars (the refine kernel's estimator state)
-            constexpr int kSc[kDescSc] = {SC_SUMQCB, SC_QL2, SC_SUMQ, SC_VFAC, SC_IPQC};
-            const int s = tid / kDescSc, j = tid - s * kDescSc;
-            a.desc[((int64_t)slot * a.desc_cap + m.ctl[2]) * dstr + kDescHead + tid] =
-                __float_as_uint(m.sc[s * SC_N + kSc[j]]);
+        if (total) {
+            // every descriptor below the slot's counter is read by the refine kernel: those whose
+            // records did not fit carry no candidates.  Head words, then the list's scalars (the
+            // refine kernel's estimator state).
+            const uint32_t dl = (uint32_t)m.ctl[2];
+            const int per = kDescHead + kDescSc * S;
+            for (int i = tid; i < ndesc * per; i += kLT) {
+                const int k = i / per, w = i - k * per;
+                if (dl + k >= a.desc_cap) continue;
+                uint32_t *dd = a.desc + ((int64_t)slot * a.desc_cap + dl + k) * dstr;
+                uint32_t val;
+                if (w == 0) {
+                    val = (uint32_t)q;
+                } else if (w == 1) {
+                    val = cid;
+                } else if (w == 2) {
+                    val = cbase >= 0 ? (uint32_t)(cbase + k * a.cmax) : 0u;
+                } else if (w == 3) {
+                    val = cbase >= 0 ? (uint32_t)min(a.cmax, total - k * a.cmax) : 0u;
+                } else {
+                    constexpr int kSc[kDescSc] = {SC_SUMQCB, SC_QL2, SC_SUMQ, SC_VFAC, SC_IPQC};
+                    const int t = w - kDescHead, s = t / kDescSc, j = t - s * kDescSc;
+                    val = __float_as_uint(m.sc[s * SC_N + kSc[j]]);
+                }
+                dd[w] = val;
+            }
         }
         if (active && ln == 0) {
             const uint32_t half = lane < 32 ? (uint32_t)bal : (uint32_t)(bal >> 32);
@@ -381,9 +394,10 @@ __device__ __host__ inline size_t rw_layout(unsigned char *base, const IndexView
         return p;
     };
     RWmem m{};
-    m.lbuf = take((size_t)cmax * ix.long_stride);
+    // the long records; before them the buffer holds the centroid row
+    m.lbuf = take(std::max<size_t>((size_t)cmax * ix.long_stride, (size_t)ix.D_pad * 4));
+    m.cbuf = reinterpret_cast<float *>(m.lbuf);
     m.r = reinterpret_cast<float *>(take(ix.D_pad * 4));
-    m.cbuf = reinterpret_cast<float *>(take(ix.D_pad * 4));
     m.sc = reinterpret_cast<float *>(take((size_t)S * SC_N * 4));
     m.exv = reinterpret_cast<float *>(take((size_t)cmax * S * 4));
     m.pre = reinterpret_cast<uint32_t *>(take((size_t)cmax * pre_stride(S) * 4));
