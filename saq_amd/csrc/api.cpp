@@ -127,6 +127,13 @@ struct saqgpu_index {
     DevBuf<uint32_t> sl_pool_id, sl_pair, sl_binfo, sl_cand, sl_pre, sl_desc, sl_ctr;
     DevBuf<int32_t> sl_pool_size;
     DevBuf<uint64_t> sl_met;
+    // query groups of the sliced scan: each runs its slices on its own stream
+    struct SliceGroup {
+        hipStream_t stream;
+        hipEvent_t done;
+    };
+    std::vector<SliceGroup> sl_groups;
+    hipEvent_t sl_fork = nullptr;
     // batched layout (batched.hip): built from the clusters on first use
     bool bq_ready = false;
     int bq_tile = 0;  // autotuned queries per work tile (0 = not tuned)
@@ -372,6 +379,21 @@ void saqgpu_index_destroy(saqgpu_index *ix) {
     ix->bq_npq.release();
     ix->bq_cand_d.release();
     ix->bq_cand_id.release();
+    ix->sl_pool_d.release();
+    ix->sl_pool_id.release();
+    ix->sl_pool_size.release();
+    ix->sl_met.release();
+    ix->sl_pair.release();
+    ix->sl_binfo.release();
+    ix->sl_cand.release();
+    ix->sl_pre.release();
+    ix->sl_desc.release();
+    ix->sl_ctr.release();
+    for (auto &g : ix->sl_groups) {
+        (void)hipStreamDestroy(g.stream);
+        (void)hipEventDestroy(g.done);
+    }
+    if (ix->sl_fork) (void)hipEventDestroy(ix->sl_fork);
     delete ix;
 }
 
@@ -592,9 +614,16 @@ namespace {
 // flagged on the device; the search is then repeated with the space it asked for (the
 // capacities are kept for later searches), so results never depend on the capacities.
 // e0 / e1 (optional): events recorded around the kernels of the attempt that succeeds.
+//
+// The queries are split into up to kGroups contiguous groups (256 queries or more each), each
+// with its own share of the workspace; group 0 runs on st, the others on streams of their own
+// forked from and joined back into st (four streams in all, the hardware queues a process
+// gets by default).  A group's replay, one latency-bound wavefront per query, then overlaps
+// the other groups' list scans and refines.  SAQGPU_GROUPS=n caps the group count.
 int sliced_scan(saqgpu_index *ix, const IndexView &v, const uint32_t *dp, const saqgpu_search_params *p,
                 const int32_t *dnpq, int64_t nq, uint32_t *dids, float *dd, uint64_t *dm, hipStream_t st,
                 hipEvent_t e0, hipEvent_t e1) {
+    constexpr int kGroups = 4;
     const int K = p->topk, S = ix->nseg;
     // slices of the work items (item 0: list 0's first chunk; 1: its rest; j >= 2: list j - 1)
     const int items = p->nprobe + 1;
@@ -613,27 +642,38 @@ int sliced_scan(saqgpu_index *ix, const IndexView &v, const uint32_t *dp, const 
     const size_t rf_smem = refine_smem_bytes(v, cmax);
     if ((int64_t)std::max(ls_smem, rf_smem) > max_smem)
         return fail(SAQGPU_EUNSUPPORTED, "plan needs more LDS than a workgroup has");
-    const int rf_grid = std::max(1, ncu) * 4;
+    int G = (int)std::max<int64_t>(1, std::min<int64_t>(kGroups, nq / 256));
+    if (const char *ge = getenv("SAQGPU_GROUPS")) G = std::max(1, std::min(G, atoi(ge)));
+    // the refine kernel is persistent (workgroups loop over the descriptors); with four groups
+    // in flight each one's refine takes half the chip's slots
+    const int rf_grid = std::max(1, ncu) * (G >= 4 ? 2 : 4);
+    // group 0 runs on st itself, groups 1.. on streams of their own
+    while ((int)ix->sl_groups.size() < G - 1) {
+        saqgpu_index::SliceGroup g{};
+        HIPCHK(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&g.done, hipEventDisableTiming));
+        ix->sl_groups.push_back(g);
+    }
+    if (!ix->sl_fork) HIPCHK(hipEventCreateWithFlags(&ix->sl_fork, hipEventDisableTiming));
     int rc = 0;
     rc |= ix->sl_pool_d.ensure(nq * K);
     rc |= ix->sl_pool_id.ensure(nq * K);
     rc |= ix->sl_pool_size.ensure(nq);
     rc |= ix->sl_met.ensure(nq * 3);
     rc |= ix->sl_pair.ensure(nq * Lmax);
-    rc |= ix->sl_ctr.ensure(CTR_N);
+    rc |= ix->sl_ctr.ensure((size_t)kGroups * CTR_N);
     if (rc) return fail(SAQGPU_ENOMEM, "workspace allocation failed");
     // first sizes: every list of the widest slice, a quarter of its vectors as candidates
     int64_t nonempty = 0;
     for (int c = 0; c < ix->num_cen; ++c) nonempty += ix->clu_n[c] > 0;
     const double mean_n = nonempty ? (double)ix->num_data / nonempty : 1.0;
     const int64_t kCapMax = ((int64_t)1 << 31) - 1;
-    auto clamp_cap = [&](double x) {
-        return (int64_t)std::min<double>((double)kCapMax, std::max(x, 4096.0 * kSlots / 16));
-    };
+    const int64_t unit = (int64_t)kSlots * kGroups;  // buffers split into kGroups x kSlots shares
+    auto clamp_cap = [&](double x) { return (int64_t)std::min<double>((double)kCapMax, std::max(x, 64.0 * unit)); };
     // SAQGPU_SLICE_CAP=n (tests): start every capacity at n instead
     const char *cap_env = getenv("SAQGPU_SLICE_CAP");
     const int64_t cap0 = cap_env ? std::max<int64_t>(1, atoll(cap_env)) : 0;
-    const int bst = binfo_stride(S), rst = cand_stride(S), pst = pre_stride(S);
+    const int bst = binfo_stride(S), rst = cand_stride(S), pst = pre_stride(S), dst = desc_stride(S);
     if (ix->sl_binfo.n == 0 &&
         ix->sl_binfo.alloc(cap0 ? cap0 : clamp_cap((double)nq * Lmax * (mean_n / kBlock + 1.5) * bst)))
         return fail(SAQGPU_ENOMEM, "workspace allocation failed");
@@ -643,69 +683,93 @@ int sliced_scan(saqgpu_index *ix, const IndexView &v, const uint32_t *dp, const 
             return fail(SAQGPU_ENOMEM, "workspace allocation failed");
     }
     if (ix->sl_desc.n == 0 &&
-        ix->sl_desc.alloc((cap0 ? cap0 : clamp_cap((double)nq * Lmax * (mean_n / 256 + 1))) * desc_stride(S)))
+        ix->sl_desc.alloc((cap0 ? cap0 : clamp_cap((double)nq * Lmax * (mean_n / 64 + 1))) * dst))
         return fail(SAQGPU_ENOMEM, "workspace allocation failed");
     for (int attempt = 0;; ++attempt) {
-        SliceArgs a{};
-        a.ix = v;
-        a.qrot = ix->ws_qrot.p;
-        a.bound = ix->ws_bound.p;
-        a.probes = dp;
-        a.nprobe = p->nprobe;
-        a.npq = dnpq;
-        a.topk = K;
-        a.nq = nq;
-        a.pool_d = ix->sl_pool_d.p;
-        a.pool_id = ix->sl_pool_id.p;
-        a.pool_size = ix->sl_pool_size.p;
-        a.met = ix->sl_met.p;
-        a.pair_blk = ix->sl_pair.p;
-        a.binfo = ix->sl_binfo.p;
-        a.cand = ix->sl_cand.p;
-        a.pre = ix->sl_pre.p;
-        a.desc = ix->sl_desc.p;
-        // per-slot shares of each buffer (the slots' reservations are independent)
-        a.binfo_cap = (uint32_t)std::min<int64_t>(kCapMax, (int64_t)ix->sl_binfo.n) / kSlots;
-        a.cand_cap = (uint32_t)std::min<int64_t>(kCapMax, (int64_t)(ix->sl_cand.n / rst)) / kSlots;
-        a.desc_cap = (uint32_t)std::min<int64_t>(kCapMax, (int64_t)(ix->sl_desc.n / desc_stride(S))) / kSlots;
-        a.ctr = ix->sl_ctr.p;
-        a.cmax = cmax;
-        a.out_ids = dids;
-        a.out_dists = dd;
-        a.out_metrics = dm;
-        if (e0) HIPCHK(hipEventRecord(e0, st));
-        HIPCHK(launch_slice_reset(a.ctr, 1, st));
-        for (size_t i = 0; i + 1 < bounds.size(); ++i) {
-            a.p0 = bounds[i];
-            a.p1 = bounds[i + 1];
-            a.first = i == 0;
-            a.last = i + 2 == bounds.size();
-            if (i > 0) HIPCHK(launch_slice_reset(a.ctr, 0, st));
-            HIPCHK(launch_list_scan(a, p->dist_type, ls_smem, st));
-            HIPCHK(launch_refine(a, p->dist_type, rf_smem, rf_grid, st));
-            HIPCHK(launch_replay(a, p->dist_type, st));
+        // per group and slot shares of each buffer (every slot's reservations are independent)
+        const int64_t bcap = std::min<int64_t>(kCapMax, (int64_t)ix->sl_binfo.n) / unit;
+        const int64_t ccap = std::min<int64_t>(kCapMax, (int64_t)(ix->sl_cand.n / rst)) / unit;
+        const int64_t dcap = std::min<int64_t>(kCapMax, (int64_t)(ix->sl_desc.n / dst)) / unit;
+        std::vector<SliceArgs> ga(G);
+        for (int g = 0; g < G; ++g) {
+            const int64_t q0 = nq * g / G, q1 = nq * (g + 1) / G;
+            SliceArgs &a = ga[g];
+            a = SliceArgs{};
+            a.ix = v;
+            a.qrot = ix->ws_qrot.p + q0 * ix->D_pad;
+            a.bound = ix->ws_bound.p + q0 * S;
+            a.probes = dp + q0 * p->nprobe;
+            a.nprobe = p->nprobe;
+            a.npq = dnpq ? dnpq + q0 : nullptr;
+            a.topk = K;
+            a.nq = q1 - q0;
+            a.pool_d = ix->sl_pool_d.p + q0 * K;
+            a.pool_id = ix->sl_pool_id.p + q0 * K;
+            a.pool_size = ix->sl_pool_size.p + q0;
+            a.met = ix->sl_met.p + q0 * 3;
+            a.pair_blk = ix->sl_pair.p + q0 * Lmax;
+            a.binfo = ix->sl_binfo.p + g * bcap * kSlots;
+            a.cand = ix->sl_cand.p + g * ccap * kSlots * rst;
+            a.pre = ix->sl_pre.p + g * ccap * kSlots * pst;
+            a.desc = ix->sl_desc.p + g * dcap * kSlots * dst;
+            a.binfo_cap = (uint32_t)bcap;
+            a.cand_cap = (uint32_t)ccap;
+            a.desc_cap = (uint32_t)dcap;
+            a.ctr = ix->sl_ctr.p + g * CTR_N;
+            a.cmax = cmax;
+            a.out_ids = dids + q0 * K;
+            a.out_dists = dd ? dd + q0 * K : nullptr;
+            a.out_metrics = dm ? dm + q0 * 3 : nullptr;
         }
-        HIPCHK(launch_slice_reset(a.ctr, 0, st));
+        if (e0) HIPCHK(hipEventRecord(e0, st));
+        auto gstream = [&](int g) { return g == 0 ? st : ix->sl_groups[g - 1].stream; };
+        if (G > 1) HIPCHK(hipEventRecord(ix->sl_fork, st));
+        for (int g = 0; g < G; ++g) {
+            if (g > 0) HIPCHK(hipStreamWaitEvent(gstream(g), ix->sl_fork, 0));
+            HIPCHK(launch_slice_reset(ga[g].ctr, 1, gstream(g)));
+        }
+        for (size_t i = 0; i + 1 < bounds.size(); ++i) {
+            for (int g = 0; g < G; ++g) {
+                SliceArgs &a = ga[g];
+                const hipStream_t gs = gstream(g);
+                a.p0 = bounds[i];
+                a.p1 = bounds[i + 1];
+                a.first = i == 0;
+                a.last = i + 2 == bounds.size();
+                if (i > 0) HIPCHK(launch_slice_reset(a.ctr, 0, gs));
+                HIPCHK(launch_list_scan(a, p->dist_type, ls_smem, gs));
+                HIPCHK(launch_refine(a, p->dist_type, rf_smem, rf_grid, gs));
+                HIPCHK(launch_replay(a, p->dist_type, gs));
+            }
+        }
+        for (int g = 0; g < G; ++g) {
+            HIPCHK(launch_slice_reset(ga[g].ctr, 0, gstream(g)));
+            if (g == 0) continue;
+            HIPCHK(hipEventRecord(ix->sl_groups[g - 1].done, gstream(g)));
+            HIPCHK(hipStreamWaitEvent(st, ix->sl_groups[g - 1].done, 0));
+        }
         if (e1) HIPCHK(hipEventRecord(e1, st));
-        uint32_t h[CTR_N];
-        HIPCHK(hipMemcpyAsync(h, a.ctr, sizeof(h), hipMemcpyDeviceToHost, st));
+        std::vector<uint32_t> h((size_t)G * CTR_N);
+        HIPCHK(hipMemcpyAsync(h.data(), ix->sl_ctr.p, h.size() * 4, hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
-        if (!h[CTR_OVERFLOW]) return SAQGPU_OK;
+        uint32_t over = 0, mb = 0, mc = 0, md = 0;
+        for (int g = 0; g < G; ++g) {
+            over |= h[g * CTR_N + CTR_OVERFLOW];
+            mb = std::max(mb, h[g * CTR_N + CTR_MAX_BINFO]);
+            mc = std::max(mc, h[g * CTR_N + CTR_MAX_CAND]);
+            md = std::max(md, h[g * CTR_N + CTR_MAX_DESC]);
+        }
+        if (!over) return SAQGPU_OK;
         if (attempt >= 3) return fail(SAQGPU_ENOMEM, "scan workspace did not converge");
-        // grow to what the slices asked for: every slot gets the largest slot's need (a slice
+        // grow to what the slices asked for: every share gets the largest share's need (a slice
         // that ran short only raises later T, so the counts are upper bounds once block info fits)
-        const int64_t nb = clamp_cap(std::max<double>((double)h[CTR_MAX_BINFO] * kSlots * 1.25,
-                                                      (double)a.binfo_cap * kSlots));
-        const int64_t nc = clamp_cap(std::max<double>((double)h[CTR_MAX_CAND] * kSlots * 1.25,
-                                                      (double)a.cand_cap * kSlots));
-        const int64_t nd = clamp_cap(std::max<double>((double)h[CTR_MAX_DESC] * kSlots * 1.25,
-                                                      (double)a.desc_cap * kSlots));
-        if (nb > (int64_t)a.binfo_cap * kSlots && ix->sl_binfo.alloc(nb))
+        const int64_t nb = clamp_cap(std::max<double>((double)mb * unit * 1.25, (double)bcap * unit));
+        const int64_t nc = clamp_cap(std::max<double>((double)mc * unit * 1.25, (double)ccap * unit));
+        const int64_t nd = clamp_cap(std::max<double>((double)md * unit * 1.25, (double)dcap * unit));
+        if (nb > bcap * unit && ix->sl_binfo.alloc(nb)) return fail(SAQGPU_ENOMEM, "workspace allocation failed");
+        if (nc > ccap * unit && (ix->sl_cand.alloc(nc * rst) || ix->sl_pre.alloc(nc * pst)))
             return fail(SAQGPU_ENOMEM, "workspace allocation failed");
-        if (nc > (int64_t)a.cand_cap * kSlots && (ix->sl_cand.alloc(nc * rst) || ix->sl_pre.alloc(nc * pst)))
-            return fail(SAQGPU_ENOMEM, "workspace allocation failed");
-        if (nd > (int64_t)a.desc_cap * kSlots && ix->sl_desc.alloc(nd * desc_stride(S)))
-            return fail(SAQGPU_ENOMEM, "workspace allocation failed");
+        if (nd > dcap * unit && ix->sl_desc.alloc(nd * dst)) return fail(SAQGPU_ENOMEM, "workspace allocation failed");
     }
 }
 }  // namespace

@@ -86,6 +86,14 @@ def test_many_slices(plan, topk):
     run_case(case, topk, 40)
 
 
+@pytest.mark.parametrize("plan", [[(128, 4)], [(64, 5), (64, 3)]])
+def test_query_groups(plan):
+    """1100 queries: the scan splits them into 4 groups, each on its own stream with its own
+    share of the workspace; every group equals the oracle."""
+    case = F.make_case(n=12000, d=128, K=40, nq=1100, plan=plan, seed=29)
+    run_case(case, 10, 20)
+
+
 def test_scan_workspace_growth(monkeypatch):
     """Block-info and candidate space far below what the slices need (16 records): the
     search is repeated with the space the device asked for, and still equals the oracle."""
