@@ -5,6 +5,7 @@
 // common.hpp, and dispatches the gfx950 kernels.  No CPU fallback exists: every search
 // and encode runs on the GPU or fails with a status.
 #include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
 
 #include <algorithm>
 #include <cstdio>
@@ -35,9 +36,8 @@ size_t estimate_smem_bytes(const IndexView &ix);
 hipError_t launch_estimate(const IndexView &ix, const float *qrot, const float *bound, const uint32_t *probes,
                            int nprobe, const int64_t *offs, int dist_type, uint32_t *out_ids, float *out_acc,
                            float *out_fast, float *out_vars, hipStream_t st);
-hipError_t launch_probe_fast(const float *centroids, const float *cnorm, int K, int D, const float *queries,
-                             int64_t nq, int nprobe, int dist_type, float *qnorm, float *dist_ws, uint32_t *probes,
-                             hipStream_t st);
+hipError_t launch_probe_select_ip(const float *ip, const float *qnorm, const float *cnorm, int K, int64_t nq,
+                                  int nprobe, int dist_type, uint32_t *probes, hipStream_t st);
 hipError_t launch_encode_segment(const IndexView &ix, const float *data, const uint32_t *src, const int64_t *gvec,
                                  const int64_t *gblk, const int32_t *lane, const uint32_t *clu, const float *cent,
                                  const float *rot, int seg, int seg_dim, int64_t nslots, int64_t ld, int rd_lmt,
@@ -134,6 +134,8 @@ struct saqgpu_index {
     };
     std::vector<SliceGroup> sl_groups;
     hipEvent_t sl_fork = nullptr;
+    // rocBLAS handle of the coarse quantiser's GEMM (created on first use)
+    rocblas_handle blas = nullptr;
     // batched layout (batched.hip): built from the clusters on first use
     bool bq_ready = false;
     int bq_tile = 0;  // autotuned queries per work tile (0 = not tuned)
@@ -394,6 +396,7 @@ void saqgpu_index_destroy(saqgpu_index *ix) {
         (void)hipEventDestroy(g.done);
     }
     if (ix->sl_fork) (void)hipEventDestroy(ix->sl_fork);
+    if (ix->blas) (void)rocblas_destroy_handle(ix->blas);
     delete ix;
 }
 
@@ -1210,6 +1213,28 @@ int default_query_tile(const saqgpu_index *ix, int64_t nq, int nprobe) {
     return per_list >= 48 ? 64 : (per_list >= 24 ? 32 : 16);
 }
 
+// The batched scan's coarse quantiser: q.c for every (query, centroid) from rocBLAS's fp32 GEMM
+// into ws_cdist, then the exact top-nprobe selection over (|q|^2 + |c|^2) - 2 q.c (or q.c for
+// IP).  Not FlatInitializer's reduction order: a probe list can differ from the reference's
+// where two centroid distances lie within fp32 rounding of each other.
+int probe_gemm(saqgpu_index *ix, const float *dq, int64_t nq, int nprobe, int dist_type, uint32_t *probes,
+               hipStream_t st) {
+    const int K = ix->num_cen, D = ix->num_dim;
+    if (!ix->blas && rocblas_create_handle(&ix->blas) != rocblas_status_success) {
+        ix->blas = nullptr;
+        return fail(SAQGPU_EDEVICE, "rocblas_create_handle failed");
+    }
+    if (rocblas_set_stream(ix->blas, st) != rocblas_status_success) return fail(SAQGPU_EDEVICE, "rocblas_set_stream failed");
+    HIPCHK(launch_sqnorm_rows(dq, nq, D, ix->bq_qnorm.p, st));
+    // column-major view: ip^T (K x nq, ld K) = centroids (D x K, ld D)^T * queries (D x nq, ld D)
+    const float one = 1.0f, zero = 0.0f;
+    if (rocblas_sgemm(ix->blas, rocblas_operation_transpose, rocblas_operation_none, K, (int)nq, D, &one,
+                      ix->centroids.p, D, dq, D, &zero, ix->ws_cdist.p, K) != rocblas_status_success)
+        return fail(SAQGPU_EDEVICE, "rocblas_sgemm failed");
+    HIPCHK(launch_probe_select_ip(ix->ws_cdist.p, ix->bq_qnorm.p, ix->bq_cnorm.p, K, nq, nprobe, dist_type, probes, st));
+    return SAQGPU_OK;
+}
+
 int search_batched_locked(saqgpu_index *ix, const float *queries, int64_t nq, const uint32_t *probes,
                           const saqgpu_batched_params *p, int query_tile, uint32_t *out_ids, float *out_dists,
                           int32_t flags, hipStream_t st, bool time_it) {
@@ -1267,9 +1292,8 @@ int search_batched_locked(saqgpu_index *ix, const float *queries, int64_t nq, co
             return fail(SAQGPU_ENOMEM, "workspace allocation failed");
         if (p->exact_probe)
             HIPCHK(launch_probe(ix->centroids.p, K, ix->num_dim, dq, nq, NP, 0, ix->ws_cdist.p, ix->ws_probes.p, st));
-        else
-            HIPCHK(launch_probe_fast(ix->centroids.p, ix->bq_cnorm.p, K, ix->num_dim, dq, nq, NP, 0, ix->bq_qnorm.p,
-                                     ix->ws_cdist.p, ix->ws_probes.p, st));
+        else if ((rc = probe_gemm(ix, dq, nq, NP, 0, ix->ws_probes.p, st)))
+            return rc;
         dp = ix->ws_probes.p;
     }
     HIPCHK(launch_query_prep(ix->view(), dq, nq, ix->variance.p, ix->has_rot ? ix->rot.p : nullptr, ix->d_rot_off.p,

@@ -8,6 +8,9 @@
 //                        64-bit radix select on (distance key, centroid id), then sorts them
 //                        ascending (descending similarity for IP).  Equal distances are ordered
 //                        by id; std::partial_sort leaves the order of exact ties unspecified.
+//                        Its kIP form reads q.c products of a library GEMM (the batched scan's
+//                        approximate coarse quantiser) and forms |q|^2 + |c|^2 - 2 q.c itself.
+//   sqnorm_rows_kernel   |x|^2 per row for that form.
 #include <cfloat>
 
 #include "common.hpp"
@@ -196,15 +199,7 @@ __global__ __launch_bounds__(256) void probe_tile_kernel(const float *__restrict
         }
 }
 
-// ---- fp32-MFMA coarse quantiser (batched mode)
-// dist[q][c] = |q|^2 + |c|^2 - 2 q.c (or q.c for IP) with q.c on v_mfma_f32_16x16x4_f32.  This is
-// not FlatInitializer's reduction order, so a probe list can differ from the reference's where
-// two centroid distances lie within fp32 rounding of each other; the batched scan, which is
-// approximate by design, may use it.  A workgroup covers 64 queries x 64 centroids, each wave
-// 32 x 32, with 32-dim chunks of both staged through LDS.
-typedef float v4f __attribute__((ext_vector_type(4)));
-constexpr int kGT = 64, kGK = 32;
-
+// |x_r|^2 per row, one wave per row (the coarse quantiser's norms).
 __global__ __launch_bounds__(256) void sqnorm_rows_kernel(const float *__restrict__ x, int64_t rows, int D,
                                                           float *__restrict__ out) {
     const int lane = threadIdx.x & 63;
@@ -217,60 +212,6 @@ __global__ __launch_bounds__(256) void sqnorm_rows_kernel(const float *__restric
     if (lane == 0) out[r] = s;
 }
 
-template <int DT>
-__global__ __launch_bounds__(256) void probe_gemm_kernel(const float *__restrict__ cent, const float *__restrict__ cnorm,
-                                                         int K, int D, const float *__restrict__ queries,
-                                                         const float *__restrict__ qnorm, int64_t nq,
-                                                         float *__restrict__ out) {
-    __shared__ float As[kGT][kGK + 1];
-    __shared__ float Bs[kGT][kGK + 1];
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const int64_t q0 = (int64_t)blockIdx.y * kGT;
-    const int c0 = blockIdx.x * kGT;
-    const int qb = (w >> 1) * 32, cbb = (w & 1) * 32;
-    v4f acc[2][2];
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b) acc[a][b] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
-    for (int k0 = 0; k0 < D; k0 += kGK) {
-        __syncthreads();
-        for (int e = tid; e < kGT * kGK; e += 256) {
-            const int r = e / kGK, k = e % kGK;
-            const int64_t q = q0 + r;
-            const int c = c0 + r;
-            As[r][k] = (q < nq && k0 + k < D) ? queries[q * D + k0 + k] : 0.0f;
-            Bs[r][k] = (c < K && k0 + k < D) ? cent[(int64_t)c * D + k0 + k] : 0.0f;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int kk = 0; kk < kGK; kk += 4) {
-            float av[2], bv[2];
-#pragma unroll
-            for (int a = 0; a < 2; ++a) av[a] = As[qb + a * 16 + lane % 16][kk + lane / 16];
-#pragma unroll
-            for (int b = 0; b < 2; ++b) bv[b] = Bs[cbb + b * 16 + lane % 16][kk + lane / 16];
-#pragma unroll
-            for (int a = 0; a < 2; ++a)
-#pragma unroll
-                for (int b = 0; b < 2; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[a], bv[b], acc[a][b], 0, 0, 0);
-        }
-    }
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int64_t q = q0 + qb + a * 16 + 4 * (lane / 16) + r;
-                const int c = c0 + cbb + b * 16 + lane % 16;
-                if (q < nq && c < K) {
-                    const float ip = acc[a][b][r];
-                    out[q * K + c] = DT == 1 ? ip : (qnorm[q] + cnorm[c]) - 2.0f * ip;
-                }
-            }
-}
-
 constexpr int kSelMax = 4096;
 
 // 64-bit key: ascending order = probe order.
@@ -281,56 +222,96 @@ __device__ __forceinline__ unsigned long long sel_key(float d, int id) {
     return ((unsigned long long)k << 32) | (uint32_t)id;
 }
 
-template <int DT>
+// Inclusive prefix sum of v over the workgroup's 256 threads (4 waves); wsum: 4 ints of LDS.
+__device__ __forceinline__ int block_scan256(int v, int *wsum) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int u = __shfl_up(v, o);
+        if (lane >= o) v += u;
+    }
+    if (lane == 63) wsum[w] = v;
+    __syncthreads();
+    for (int i = 0; i < w; ++i) v += wsum[i];
+    __syncthreads();
+    return v;
+}
+
+// One workgroup per query.  Radix select of the nprobe-th smallest 64-bit key (distance key,
+// centroid id), 8 bits per pass from the top; a pass whose chosen byte bucket is taken whole
+// ends the search (every key below the bucket's upper end is selected), so distinct distances
+// need at most 4 passes and the id bytes are only looked at for exact distance ties.  The
+// selected keys are then ordered: by rank counting for up to 256 of them, else by a bitonic
+// sort in LDS.
+//
+// kIP: dist holds inner products q.c and the distance is formed on the fly, (|q|^2 + |c|^2) -
+// 2 q.c for L2 (qnorm, cnorm) or q.c itself for IP.
+template <int DT, bool kIP = false>
 __global__ __launch_bounds__(256) void probe_select_kernel(const float *__restrict__ dist, int K, int nprobe,
-                                                           uint32_t *__restrict__ probes) {
+                                                           uint32_t *__restrict__ probes,
+                                                           const float *__restrict__ qnorm = nullptr,
+                                                           const float *__restrict__ cnorm = nullptr) {
     __shared__ unsigned int hist[256];
+    __shared__ int wsum[4];
     __shared__ unsigned long long prefix_s;
-    __shared__ int rank_s;
-    __shared__ int cnt_s;
+    __shared__ int rank_s, done_s, cnt_s;
     __shared__ unsigned long long sel[kSelMax];
     const int64_t q = blockIdx.x;
     const float *dq = dist + q * K;
     const int tid = threadIdx.x;
-    if (tid == 0) {
-        prefix_s = 0;
-        rank_s = nprobe;  // find the nprobe-th smallest key (1-based)
-        cnt_s = 0;
-    }
-    // radix select of the nprobe-th smallest 64-bit key, 8 bits per pass from the top
+    const float qn = kIP && DT == 0 ? qnorm[q] : 0.0f;
+    auto key = [&](int i) {
+        const float v = dq[i];
+        return sel_key<DT>(kIP && DT == 0 ? (qn + cnorm[i]) - 2.0f * v : v, i);
+    };
+    unsigned long long pre = 0, thr = ~0ull;
+    int r = nprobe;
     for (int pass = 0; pass < 8; ++pass) {
         const int shift = 56 - 8 * pass;
-        for (int i = tid; i < 256; i += 256) hist[i] = 0;
+        hist[tid] = 0;
         __syncthreads();
-        const unsigned long long pre = prefix_s;
         const unsigned long long hmask = pass == 0 ? 0ull : (~0ull << (shift + 8));
         for (int i = tid; i < K; i += 256) {
-            const unsigned long long k = sel_key<DT>(dq[i], i);
+            const unsigned long long k = key(i);
             if ((k & hmask) == pre) atomicAdd(&hist[(k >> shift) & 0xFF], 1u);
         }
         __syncthreads();
-        if (tid == 0) {
-            int r = rank_s;
-            int b = 0;
-            for (; b < 256; ++b) {
-                if ((int)hist[b] >= r) break;
-                r -= hist[b];
-            }
-            prefix_s = pre | ((unsigned long long)b << shift);
-            rank_s = r;
+        const int h = (int)hist[tid];
+        const int incl = block_scan256(h, wsum);
+        const int excl = incl - h;
+        if (excl < r && r <= incl) {  // exactly one thread: the bucket holding the r-th key
+            prefix_s = pre | ((unsigned long long)tid << shift);
+            rank_s = r - excl;
+            done_s = (r - excl) == h;
         }
         __syncthreads();
+        pre = prefix_s;
+        r = rank_s;
+        if (done_s || pass == 7) {
+            thr = pass == 7 ? pre : (pre | ((1ull << shift) - 1));  // exactly nprobe keys <= thr
+            break;
+        }
     }
-    const unsigned long long thr = prefix_s;  // exactly nprobe keys are <= thr
+    if (tid == 0) cnt_s = 0;
+    __syncthreads();
     for (int i = tid; i < K; i += 256) {
-        const unsigned long long k = sel_key<DT>(dq[i], i);
+        const unsigned long long k = key(i);
         if (k <= thr) {
             const int p = atomicAdd(&cnt_s, 1);
             if (p < kSelMax) sel[p] = k;
         }
     }
     __syncthreads();
-    int n = min(cnt_s, kSelMax);
+    const int n = min(cnt_s, kSelMax);
+    if (n <= 256) {
+        if (tid < n) {
+            const unsigned long long k = sel[tid];
+            int rank = 0;
+            for (int j = 0; j < n; ++j) rank += sel[j] < k;
+            if (rank < nprobe) probes[q * nprobe + rank] = (uint32_t)(k & 0xFFFFFFFFu);
+        }
+        return;
+    }
     int P = 1;
     while (P < n) P <<= 1;
     for (int i = n + tid; i < P; i += 256) sel[i] = ~0ull;
@@ -360,22 +341,18 @@ hipError_t launch_sqnorm_rows(const float *x, int64_t rows, int D, float *out, h
     return hipGetLastError();
 }
 
-// The batched scan's coarse quantiser: fp32-MFMA distances (cnorm = |c|^2 per centroid, qnorm
-// workspace of nq floats), then the same exact top-nprobe selection.
-hipError_t launch_probe_fast(const float *centroids, const float *cnorm, int K, int D, const float *queries,
-                             int64_t nq, int nprobe, int dist_type, float *qnorm, float *dist_ws, uint32_t *probes,
-                             hipStream_t st) {
-    if (nprobe > kSelMax || (nq + kGT - 1) / kGT > 65535) return hipErrorInvalidValue;
-    hipError_t e = launch_sqnorm_rows(queries, nq, D, qnorm, st);
-    if (e != hipSuccess) return e;
-    dim3 g((unsigned)((K + kGT - 1) / kGT), (unsigned)((nq + kGT - 1) / kGT));
-    if (dist_type == 1) {
-        hipLaunchKernelGGL(probe_gemm_kernel<1>, g, dim3(256), 0, st, centroids, cnorm, K, D, queries, qnorm, nq, dist_ws);
-        hipLaunchKernelGGL(probe_select_kernel<1>, dim3((unsigned)nq), dim3(256), 0, st, dist_ws, K, nprobe, probes);
-    } else {
-        hipLaunchKernelGGL(probe_gemm_kernel<0>, g, dim3(256), 0, st, centroids, cnorm, K, D, queries, qnorm, nq, dist_ws);
-        hipLaunchKernelGGL(probe_select_kernel<0>, dim3((unsigned)nq), dim3(256), 0, st, dist_ws, K, nprobe, probes);
-    }
+// Top-nprobe selection over inner products ip (nq x K, from a library GEMM): distances are
+// (qnorm + cnorm) - 2 ip for L2, ip for IP.
+hipError_t launch_probe_select_ip(const float *ip, const float *qnorm, const float *cnorm, int K, int64_t nq,
+                                  int nprobe, int dist_type, uint32_t *probes, hipStream_t st) {
+    if (nprobe > kSelMax) return hipErrorInvalidValue;
+    if (nq == 0) return hipSuccess;
+    if (dist_type == 1)
+        hipLaunchKernelGGL((probe_select_kernel<1, true>), dim3((unsigned)nq), dim3(256), 0, st, ip, K, nprobe, probes,
+                           qnorm, cnorm);
+    else
+        hipLaunchKernelGGL((probe_select_kernel<0, true>), dim3((unsigned)nq), dim3(256), 0, st, ip, K, nprobe, probes,
+                           qnorm, cnorm);
     return hipGetLastError();
 }
 
@@ -402,13 +379,15 @@ hipError_t launch_probe(const float *centroids, int K, int D, const float *queri
             probe_tile<1>(vec, gt, st, centroids, K, D, queries, nq, dist_ws);
         else
             hipLaunchKernelGGL(probe_dist_kernel<1>, g1, dim3(256), sm, st, centroids, K, D, queries, dist_ws);
-        hipLaunchKernelGGL(probe_select_kernel<1>, dim3((unsigned)nq), dim3(256), 0, st, dist_ws, K, nprobe, probes);
+        hipLaunchKernelGGL((probe_select_kernel<1, false>), dim3((unsigned)nq), dim3(256), 0, st, dist_ws, K, nprobe, probes,
+                           nullptr, nullptr);
     } else {
         if (tiled)
             probe_tile<0>(vec, gt, st, centroids, K, D, queries, nq, dist_ws);
         else
             hipLaunchKernelGGL(probe_dist_kernel<0>, g1, dim3(256), sm, st, centroids, K, D, queries, dist_ws);
-        hipLaunchKernelGGL(probe_select_kernel<0>, dim3((unsigned)nq), dim3(256), 0, st, dist_ws, K, nprobe, probes);
+        hipLaunchKernelGGL((probe_select_kernel<0, false>), dim3((unsigned)nq), dim3(256), 0, st, dist_ws, K, nprobe, probes,
+                           nullptr, nullptr);
     }
     return hipGetLastError();
 }

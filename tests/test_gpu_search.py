@@ -172,6 +172,32 @@ def test_gpu_probe_matches_oracle_across_dims(d, dist_type):
     ix.close()
 
 
+@pytest.mark.parametrize("dist_type", [0, 1])
+def test_gpu_probe_exact_ties(dist_type):
+    """Exact distance ties are ordered by centroid id: duplicated centroids tie in pairs, and a
+    zero query ties every centroid under IP, so the radix select has to reach the id bytes."""
+    from oracle_lib import oracle
+    O = oracle()
+    case = F.make_case(n=1500, d=96, K=300, nq=12, plan=[(128, 2)], seed=23)
+    case.centroids[150:] = case.centroids[:150]
+    case.queries[0] = 0.0
+    case.queries[5] = case.centroids[7]
+    ix = F.gpu_index(case)
+    ranks = []
+    for q in case.queries:
+        pr = np.zeros(case.K, np.uint32)
+        pd = np.zeros(case.K, np.float32)
+        O.orc_probe(case.centroids, case.K, case.num_dim, np.ascontiguousarray(q), case.K, dist_type, pr,
+                    pd.ctypes.data)
+        key = -pd if dist_type == 1 else pd
+        ranks.append(pr[np.lexsort((pr, key))])
+    for nprobe in (1, 2, 149, 150, 151, 256, 257, 300):
+        got = ix.probe(case.queries, nprobe, dist_type=dist_type)
+        want = np.stack([r[:nprobe] for r in ranks])
+        assert np.array_equal(want, got), f"nprobe={nprobe}"
+    ix.close()
+
+
 def test_cluster_layout_round_trip():
     for plan, compact in (([(256, 4)], False), ([(64, 1), (64, 5), (128, 3)], False), ([(128, 5), (128, 0)], True)):
         case = F.make_case(n=1500, d=256, K=6, nq=4, plan=plan, seed=19, compact=compact)
