@@ -1250,16 +1250,18 @@ int search_batched_locked(saqgpu_index *ix, const float *queries, int64_t nq, co
     if ((int64_t)bscan_smem_bytes(bv, topk, qtm, qs) > max_smem)
         return fail(SAQGPU_EUNSUPPORTED, "query tile needs more LDS than a workgroup has; use a smaller query_tile");
     const int64_t npairs = nq * NP;
-    const int64_t max_tiles = npairs + K;
+    const int64_t max_tiles = bq_max_tiles(npairs, K), max_cands = bq_max_cands(npairs, query_tile);
+    if (max_tiles >= INT32_MAX || max_cands * topk >= ((int64_t)1 << 40))
+        return fail(SAQGPU_EINVAL, "batch too large: split the queries");
     rc = 0;
     rc |= ix->ws_qrot.ensure(nq * ix->D_pad);
     rc |= ix->ws_bound.ensure(nq * ix->nseg);
-    rc |= ix->bq_ws.ensure(3 * (size_t)K + 2 * npairs + 3 * max_tiles + 3);
+    rc |= ix->bq_ws.ensure(5 * (size_t)K + 2 * npairs + 5 * max_tiles + 4);
     rc |= ix->bq_qq.ensure((size_t)qs * npairs * ix->bq_kq);
     rc |= ix->bq_pc.ensure((size_t)npairs * (3 * ix->bq_nqs + 1));
     rc |= ix->bq_pcorr.ensure((size_t)npairs * 2 * std::max(1, ix->bq_nqs));
-    rc |= ix->bq_cand_d.ensure(npairs * topk);
-    rc |= ix->bq_cand_id.ensure(npairs * topk);
+    rc |= ix->bq_cand_d.ensure(max_cands * topk);
+    rc |= ix->bq_cand_id.ensure(max_cands * topk);
     if (!dev) {
         rc |= ix->ws_q.ensure(nq * ix->num_dim);
         rc |= ix->ws_ids.ensure(nq * topk);
@@ -1303,21 +1305,26 @@ int search_batched_locked(saqgpu_index *ix, const float *queries, int64_t nq, co
     w.counts = wsp;
     w.cstart = wsp + K;
     w.tstart = wsp + 2 * K;
-    w.slot = wsp + 3 * K;
+    w.candstart = wsp + 3 * K;
+    w.nsplit = wsp + 4 * K;
+    w.slot = wsp + 5 * K;
     w.pairs = w.slot + npairs;
     w.tile_clu = w.pairs + npairs;
     w.tile_first = w.tile_clu + max_tiles;
     w.tile_cnt = w.tile_first + max_tiles;
-    w.ntiles = w.tile_cnt + max_tiles;
+    w.tile_v0 = w.tile_cnt + max_tiles;
+    w.tile_cand = w.tile_v0 + max_tiles;
+    w.ntiles = w.tile_cand + max_tiles;
     w.npairs = w.ntiles + 1;
     w.work = w.npairs + 1;
+    w.sv = w.work + 1;
     BqPairBufs pb;
     pb.qq1 = ix->bq_qq.p;
     pb.qq2 = qs == 2 ? ix->bq_qq.p + npairs * ix->bq_kq : nullptr;
     pb.pc = ix->bq_pc.p;
     pb.pq2 = ix->bq_pc.p + npairs * 3 * ix->bq_nqs;
     pb.pcorr = ix->bq_pcorr.p;
-    HIPCHK(launch_bq_group(dp, dnpq, nq, NP, K, query_tile, ix->d_bq_order.p, w, st));
+    HIPCHK(launch_bq_group(dp, dnpq, nq, NP, K, query_tile, ix->d_bq_order.p, ix->d_clu_n.p, w, st));
     HIPCHK(launch_bq_pairprep(bv, ix->ws_qrot.p, dp, w, pb, NP, npairs, qs, st));
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (time_it) {
@@ -1331,7 +1338,7 @@ int search_batched_locked(saqgpu_index *ix, const float *queries, int64_t nq, co
         HIPCHK(hipEventRecord(e1, st));
         ix->events.emplace_back(e0, e1);
     }
-    HIPCHK(launch_bq_merge(ix->bq_cand_d.p, ix->bq_cand_id.p, dnpq, nq, NP, topk, dids, dd, st));
+    HIPCHK(launch_bq_merge(ix->bq_cand_d.p, ix->bq_cand_id.p, dp, w, dnpq, nq, NP, topk, dids, dd, st));
     if (!dev) {
         HIPCHK(hipMemcpyAsync(out_ids, dids, nq * topk * 4, hipMemcpyDeviceToHost, st));
         if (out_dists) HIPCHK(hipMemcpyAsync(out_dists, dd, nq * topk * 4, hipMemcpyDeviceToHost, st));

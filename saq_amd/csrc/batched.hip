@@ -180,43 +180,74 @@ __global__ void bq_count_kernel(const uint32_t *probes, const int32_t *npq, int6
     slot[i] = atomicAdd(&counts[probes[i]], 1);
 }
 
-// One workgroup: exclusive prefix sums of the pair counts and of the tile counts per list,
-// taken over the lists in the given order (largest lists first, so the longest tiles start
-// first and the persistent kernel's tail stays short).
-__global__ __launch_bounds__(1024) void bq_scan_kernel(const int32_t *counts, const int32_t *order, int K, int QT,
-                                                       int32_t *cstart, int32_t *tstart, int32_t *ntiles,
-                                                       int32_t *npairs) {
-    __shared__ int32_t part_c[1024], part_t[1024];
-    const int t = threadIdx.x;
+// One workgroup.  The split size first: W = sum over lists of (query tiles x list vectors) is
+// the launch's work in vector-tiles, and a list is cut into ranges of sv = max(kMinSplit,
+// W / kSplitWaves) vectors (a multiple of 64), so no work tile holds more than about half of a
+// resident wavefront's share and one long list cannot set the kernel's tail.  Then exclusive
+// prefix sums, over the lists in the given order (largest first, so the longest tiles start
+// first), of the pair counts, of the work tiles (query tiles x ranges) and of the candidate
+// lists (pairs x ranges).  The bounds sum_c tiles_c <= npairs / QT + K + kSplitWaves and
+// sum_c cands_c <= npairs + kSplitWaves * QT follow from sv >= W / kSplitWaves.
+__global__ __launch_bounds__(1024) void bq_scan_kernel(const int32_t *counts, const int32_t *order,
+                                                       const int32_t *clu_n, int K, int QT, int32_t *cstart,
+                                                       int32_t *tstart, int32_t *candstart, int32_t *nsplit,
+                                                       int32_t *ntiles, int32_t *npairs, int32_t *sv_out) {
+    __shared__ int32_t part_c[1024], part_t[1024], part_k[1024];
+    __shared__ long long wsum[16];
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     const int per = (K + 1023) / 1024;
     const int b = t * per, e = min(K, b + per);
-    int32_t sc = 0, st = 0;
+    long long wk = 0;
     for (int i = b; i < e; ++i) {
         const int c = order[i];
+        wk += (long long)((counts[c] + QT - 1) / QT) * clu_n[c];
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) wk += __shfl_xor(wk, o);
+    if (lane == 0) wsum[wv] = wk;
+    __syncthreads();
+    long long W = 0;
+    for (int i = 0; i < 16; ++i) W += wsum[i];
+    const long long want = (W + kSplitWaves - 1) / kSplitWaves;
+    const int sv = (int)max((long long)kMinSplit, (want + 63) / 64 * 64);
+    auto splits = [&](int c) { return max(1, (clu_n[c] + sv - 1) / sv); };
+    int32_t sc = 0, st = 0, sk = 0;
+    for (int i = b; i < e; ++i) {
+        const int c = order[i];
+        const int ns = splits(c);
         sc += counts[c];
-        st += (counts[c] + QT - 1) / QT;
+        st += (counts[c] + QT - 1) / QT * ns;
+        sk += counts[c] * ns;
     }
     part_c[t] = sc;
     part_t[t] = st;
+    part_k[t] = sk;
     __syncthreads();
     for (int off = 1; off < 1024; off <<= 1) {
-        const int32_t a = t >= off ? part_c[t - off] : 0, bt = t >= off ? part_t[t - off] : 0;
+        const int32_t a = t >= off ? part_c[t - off] : 0, bt = t >= off ? part_t[t - off] : 0,
+                      bk = t >= off ? part_k[t - off] : 0;
         __syncthreads();
         part_c[t] += a;
         part_t[t] += bt;
+        part_k[t] += bk;
         __syncthreads();
     }
-    int32_t rc = part_c[t] - sc, rt = part_t[t] - st;
+    int32_t rc = part_c[t] - sc, rt = part_t[t] - st, rk = part_k[t] - sk;
     for (int i = b; i < e; ++i) {
         const int c = order[i];
+        const int ns = splits(c);
         cstart[c] = rc;
         tstart[c] = rt;
+        candstart[c] = rk;
+        nsplit[c] = ns;
         rc += counts[c];
-        rt += (counts[c] + QT - 1) / QT;
+        rt += (counts[c] + QT - 1) / QT * ns;
+        rk += counts[c] * ns;
     }
     if (t == 1023) {
         *ntiles = part_t[1023];
         *npairs = part_c[1023];
+        *sv_out = sv;
     }
 }
 
@@ -227,16 +258,24 @@ __global__ void bq_scatter_kernel(const uint32_t *probes, const int32_t *slot, i
     pairs[cstart[probes[i]] + slot[i]] = (int32_t)i;
 }
 
-__global__ void bq_tiles_kernel(const int32_t *counts, const int32_t *cstart, const int32_t *tstart, int K, int QT,
-                                int32_t *tile_clu, int32_t *tile_first, int32_t *tile_cnt) {
+// Work tiles of list c: every query tile of QT of its pairs times every vector range of sv
+// vectors.  Candidate list of (pair slot j, range r) of list c: candstart[c] + r * counts[c] + j.
+__global__ void bq_tiles_kernel(const int32_t *counts, const int32_t *cstart, const int32_t *tstart,
+                                const int32_t *candstart, const int32_t *nsplit, const int32_t *sv_p, int K, int QT,
+                                int32_t *tile_clu, int32_t *tile_first, int32_t *tile_cnt, int32_t *tile_v0,
+                                int32_t *tile_cand) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= K) return;
-    const int n = counts[c];
-    for (int t = 0, i = 0; i < n; ++t, i += QT) {
-        tile_clu[tstart[c] + t] = c;
-        tile_first[tstart[c] + t] = cstart[c] + i;
-        tile_cnt[tstart[c] + t] = min(QT, n - i);
-    }
+    const int n = counts[c], ns = nsplit[c], sv = *sv_p;
+    int t = tstart[c];
+    for (int i = 0; i < n; i += QT)
+        for (int r = 0; r < ns; ++r, ++t) {
+            tile_clu[t] = c;
+            tile_first[t] = cstart[c] + i;
+            tile_cnt[t] = min(QT, n - i);
+            tile_v0[t] = r * sv;
+            tile_cand[t] = candstart[c] + r * n + i;
+        }
 }
 
 // ------------------------------------------------------------------ wave-held top-k
@@ -445,9 +484,8 @@ struct BScanArgs {
     const float *pc;
     const int32_t *pcorr;
     const float *pq2;
-    const int32_t *pairs;
-    const int32_t *tile_clu, *tile_first, *tile_cnt;
-    const int32_t *ntiles;
+    const int32_t *tile_clu, *tile_first, *tile_cnt, *tile_v0, *tile_cand;
+    const int32_t *ntiles, *sv;
     int32_t *work;  // tile counter, zero at launch
     int topk;
     float *cand_d;  // (nq * NP) x topk
@@ -515,7 +553,67 @@ __device__ __forceinline__ v4i mfma(v4i a, v4i b, v4i c) {
     return __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, c, 0, 0, 0);
 }
 
-constexpr int kKB = 2;  // 64-dim chunks whose operands are loaded together
+// One quantised segment's contraction for a chunk of NVW mblocks: every 64-dim chunk of the
+// segment, the code operands of kKB chunks (4 for nibble codes, whose loads are half as wide)
+// loaded for all NVW mblocks together, then their MFMAs.  KIND: 0 = 4-bit codes stored as
+// nibbles, 1 = one int8 plane, 2 = a shifted low plane and a high plane (9+ bits).  The kind is
+// a template parameter so that no branch separates the loads of a step.
+template <int KIND, int QTM, int QS, int NVW>
+__device__ __forceinline__ void seg_mma(const BSeg &bs, const uint8_t *const (&rec)[NVW],
+                                        const int8_t *const (&arow1)[QTM], const int8_t *const (&arow2)[QTM],
+                                        v4i (&acc1)[QTM][NVW], v4i (&acc2)[QTM][NVW], v4i (&ach1)[QTM][NVW],
+                                        v4i (&ach2)[QTM][NVW], int lane) {
+    constexpr int kKB = KIND == 0 ? 4 : 2;
+    const int nk = bs.dim / 64;
+    for (int kc0 = 0; kc0 < nk; kc0 += kKB) {
+        v4i A1[kKB][QTM], A2[kKB][QTM], B[kKB][NVW], BH[kKB][NVW];
+        uint2 BN[kKB][NVW];
+#pragma unroll
+        for (int u = 0; u < kKB; ++u) {
+            // a chunk past nk repeats the last one with a zero A operand: every load and MFMA of
+            // the step is unconditional, so the loads stay together
+            const int kc = min(kc0 + u, nk - 1);
+            const bool live = kc0 + u < nk;
+#pragma unroll
+            for (int t = 0; t < QTM; ++t) {
+                const v4i a1 = *reinterpret_cast<const v4i *>(arow1[t] + bs.kq_off + kc * 64);
+                A1[u][t] = live ? a1 : v4i{0, 0, 0, 0};
+                if (QS == 2) {
+                    const v4i a2 = *reinterpret_cast<const v4i *>(arow2[t] + bs.kq_off + kc * 64);
+                    A2[u][t] = live ? a2 : v4i{0, 0, 0, 0};
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < NVW; ++j) {
+                const uint8_t *p = rec[j] + bs.code_off + (int64_t)kc * bs.chunk_bytes;
+                if (KIND == 0) {
+                    BN[u][j] = *reinterpret_cast<const uint2 *>(p + lane * 8);
+                } else {
+                    B[u][j] = *reinterpret_cast<const v4i *>(p + lane * 16);
+                    if (KIND == 2) BH[u][j] = *reinterpret_cast<const v4i *>(p + 1024 + lane * 16);
+                }
+            }
+        }
+        // keep the scheduler from interleaving the step's loads with its MFMAs: all of them are
+        // in flight before the first MFMA waits
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < kKB; ++u) {
+#pragma unroll
+            for (int t = 0; t < QTM; ++t)
+#pragma unroll
+                for (int j = 0; j < NVW; ++j) {
+                    const v4i b = KIND == 0 ? unpack_nibbles(BN[u][j]) : B[u][j];
+                    acc1[t][j] = mfma(A1[u][t], b, acc1[t][j]);
+                    if (QS == 2) acc2[t][j] = mfma(A2[u][t], b, acc2[t][j]);
+                    if (KIND == 2) {
+                        ach1[t][j] = mfma(A1[u][t], BH[u][j], ach1[t][j]);
+                        if (QS == 2) ach2[t][j] = mfma(A2[u][t], BH[u][j], ach2[t][j]);
+                    }
+                }
+        }
+    }
+}
 
 template <int QTM, int QS, int KR>
 __global__ __launch_bounds__(kBThreads) void bscan_kernel(BScanArgs a) {
@@ -528,7 +626,7 @@ __global__ __launch_bounds__(kBThreads) void bscan_kernel(BScanArgs a) {
     const size_t per_wave = wave_smem<QTM, QS>(bv.nqs, a.topk, nullptr, nullptr);
     WSmem m;
     wave_smem<QTM, QS>(bv.nqs, a.topk, &m, smem_raw + w * per_wave);
-    const int ntiles = *a.ntiles;
+    const int ntiles = *a.ntiles, sv = *a.sv;
     for (;;) {
         int tile = 0;
         if (lane == 0) tile = atomicAdd(a.work, 1);
@@ -561,15 +659,17 @@ __global__ __launch_bounds__(kBThreads) void bscan_kernel(BScanArgs a) {
             arow1[t] = a.qq1 + g * bv.kq + 16 * (lane / 16);
             arow2[t] = QS == 2 ? a.qq2 + g * bv.kq + 16 * (lane / 16) : nullptr;
         }
+        // the tile's vector range [v0, v0 + sv) of the list, in mblocks (v0 and sv are multiples of 64)
         const int n = bv.clu_n[c];
-        const int nmb = (n + kMB - 1) / kMB;
+        const int v0 = a.tile_v0[tile];
+        const int mb_beg = v0 / kMB, mb_end = min((n + kMB - 1) / kMB, (v0 + sv) / kMB);
         const int64_t mb0 = bv.clu_mb[c];
-        for (int cb = 0; cb < nmb; cb += NVW) {
+        for (int cb = mb_beg; cb < mb_end; cb += NVW) {
             const uint8_t *rec[NVW];
             float dist[QTM][NVW][4];
 #pragma unroll
             for (int j = 0; j < NVW; ++j) {
-                rec[j] = bv.mb + (mb0 + min(cb + j, nmb - 1)) * bv.mb_stride;  // clamped duplicates are discarded
+                rec[j] = bv.mb + (mb0 + min(cb + j, mb_end - 1)) * bv.mb_stride;  // clamped duplicates are discarded
                 const float o2 = reinterpret_cast<const float *>(rec[j] + bv.fac_off)[lane % kMB];
 #pragma unroll
                 for (int t = 0; t < QTM; ++t)
@@ -584,45 +684,12 @@ __global__ __launch_bounds__(kBThreads) void bscan_kernel(BScanArgs a) {
                 for (int t = 0; t < QTM; ++t)
 #pragma unroll
                     for (int j = 0; j < NVW; ++j) acc1[t][j] = acc2[t][j] = ach1[t][j] = ach2[t][j] = v4i{0, 0, 0, 0};
-                const int nk = bs.dim / 64;
-                const int kind = bs.bits <= 4 ? 0 : (bs.bits <= 8 ? 1 : 2);
-                for (int kc0 = 0; kc0 < nk; kc0 += kKB) {
-                    v4i A1[kKB][QTM], A2[kKB][QTM], B[kKB][NVW], BH[kKB][NVW];
-#pragma unroll
-                    for (int u = 0; u < kKB; ++u) {
-                        const int kc = min(kc0 + u, nk - 1);  // a repeated chunk past nk is not accumulated
-#pragma unroll
-                        for (int t = 0; t < QTM; ++t) {
-                            A1[u][t] = *reinterpret_cast<const v4i *>(arow1[t] + bs.kq_off + kc * 64);
-                            if (QS == 2) A2[u][t] = *reinterpret_cast<const v4i *>(arow2[t] + bs.kq_off + kc * 64);
-                        }
-#pragma unroll
-                        for (int j = 0; j < NVW; ++j) {
-                            const uint8_t *p = rec[j] + bs.code_off + (int64_t)kc * bs.chunk_bytes;
-                            if (kind == 0) {
-                                B[u][j] = unpack_nibbles(*reinterpret_cast<const uint2 *>(p + lane * 8));
-                            } else {
-                                B[u][j] = *reinterpret_cast<const v4i *>(p + lane * 16);
-                                if (kind == 2) BH[u][j] = *reinterpret_cast<const v4i *>(p + 1024 + lane * 16);
-                            }
-                        }
-                    }
-#pragma unroll
-                    for (int u = 0; u < kKB; ++u) {
-                        if (kc0 + u >= nk) break;
-#pragma unroll
-                        for (int t = 0; t < QTM; ++t)
-#pragma unroll
-                            for (int j = 0; j < NVW; ++j) {
-                                acc1[t][j] = mfma(A1[u][t], B[u][j], acc1[t][j]);
-                                if (QS == 2) acc2[t][j] = mfma(A2[u][t], B[u][j], acc2[t][j]);
-                                if (kind == 2) {
-                                    ach1[t][j] = mfma(A1[u][t], BH[u][j], ach1[t][j]);
-                                    if (QS == 2) ach2[t][j] = mfma(A2[u][t], BH[u][j], ach2[t][j]);
-                                }
-                            }
-                    }
-                }
+                if (bs.bits <= 4)
+                    seg_mma<0, QTM, QS, NVW>(bs, rec, arow1, arow2, acc1, acc2, ach1, ach2, lane);
+                else if (bs.bits <= 8)
+                    seg_mma<1, QTM, QS, NVW>(bs, rec, arow1, arow2, acc1, acc2, ach1, ach2, lane);
+                else
+                    seg_mma<2, QTM, QS, NVW>(bs, rec, arow1, arow2, acc1, acc2, ach1, ach2, lane);
                 // epilogue of the segment
 #pragma unroll
                 for (int j = 0; j < NVW; ++j) {
@@ -646,11 +713,29 @@ __global__ __launch_bounds__(kBThreads) void bscan_kernel(BScanArgs a) {
                         }
                 }
             }
+            // ---- rows (queries) with a distance of this chunk at or below their k-th distance: the
+            // k-th distance only falls, so no other row can take a candidate of this chunk
+            unsigned long long rows = 0;
+#pragma unroll
+            for (int t = 0; t < QTM; ++t)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const float kdr = m.kd[t * 16 + 4 * (lane / 16) + r];
+                    bool hit = false;
+#pragma unroll
+                    for (int j = 0; j < NVW; ++j) hit |= cb + j < mb_end && dist[t][j][r] <= kdr;
+                    const unsigned long long b = __ballot(hit);
+#pragma unroll
+                    for (int g = 0; g < 4; ++g)
+                        if ((b >> (16 * g)) & 0xFFFFull) rows |= 1ull << (t * 16 + 4 * g + r);
+                }
+            if (cnt < 64) rows &= (1ull << cnt) - 1;
+            if (!rows) continue;
             // ---- the chunk's distances to LDS, then the per-query lists
 #pragma unroll
             for (int j = 0; j < NVW; ++j) {
                 const int lv = j * kMB + lane % kMB;
-                const bool valid = cb + j < nmb;
+                const bool valid = cb + j < mb_end;
 #pragma unroll
                 for (int t = 0; t < QTM; ++t)
 #pragma unroll
@@ -662,7 +747,9 @@ __global__ __launch_bounds__(kBThreads) void bscan_kernel(BScanArgs a) {
             }
             wave_sync();
             const float x_id_pad = INFINITY;
-            for (int i = 0; i < cnt; ++i) {
+            while (rows) {
+                const int i = __builtin_ctzll(rows);
+                rows &= rows - 1;
                 const float x = lane < CVW ? m.dt[i * CVW + lane] : x_id_pad;
                 const uint32_t xi = lane < CVW ? m.dids[lane] : 0xFFFFFFFFu;
                 if (!__ballot(dless(x, xi, m.kd[i], m.ki[i]))) continue;
@@ -682,7 +769,7 @@ __global__ __launch_bounds__(kBThreads) void bscan_kernel(BScanArgs a) {
         }
         // ---- the tile's per-query lists to global memory
         for (int i = 0; i < cnt; ++i) {
-            const int64_t pr = a.pairs[first + i];
+            const int64_t pr = (int64_t)a.tile_cand[tile] + i;
             const int nn = m.tkn[i];
             for (int e = lane; e < a.topk; e += 64) {
                 a.cand_d[pr * a.topk + e] = e < nn ? m.tkd[i * a.topk + e] : INFINITY;
@@ -694,11 +781,17 @@ __global__ __launch_bounds__(kBThreads) void bscan_kernel(BScanArgs a) {
 }
 
 // ------------------------------------------------------------------ merge
-// One wave per query: the k smallest (distance, id) pairs over its lists' candidate lists.
+// One wave per query: the k smallest (distance, id) pairs over the candidate lists of its
+// pairs (one list per vector range of the probed list).  The lanes take 64 probes at a time
+// and load the heads of their r-th lists together; only lists whose head beats the current
+// k-th pair are then walked (each list is sorted, so a list whose head does not beat it holds
+// nothing that does).
 template <int KR>
 __global__ __launch_bounds__(256) void bq_merge_kernel(const float *cand_d, const uint32_t *cand_id,
-                                                      const int32_t *npq, int64_t nq, int NP, int topk,
-                                                      uint32_t *out_ids, float *out_d) {
+                                                      const uint32_t *probes, const int32_t *slot,
+                                                      const int32_t *counts, const int32_t *candstart,
+                                                      const int32_t *nsplit, const int32_t *npq, int64_t nq,
+                                                      int NP, int topk, uint32_t *out_ids, float *out_d) {
     const int lane = threadIdx.x & 63;
     const int64_t q = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (q >= nq) return;
@@ -712,16 +805,40 @@ __global__ __launch_bounds__(256) void bq_merge_kernel(const float *cand_d, cons
     }
     tk.refresh();
     const int lim = npq ? npq[q] : NP;
-    for (int p = 0; p < lim; ++p) {
-        const float *cd = cand_d + (q * NP + p) * topk;
-        const uint32_t *ci = cand_id + (q * NP + p) * topk;
-        for (int e0 = 0; e0 < topk; e0 += 64) {
-            const int e = e0 + lane;
-            const float x = e < topk ? cd[e] : INFINITY;
-            const uint32_t xi = e < topk ? ci[e] : 0xFFFFFFFFu;
-            // each list is sorted: once its head is not below the k-th pair, nothing is
-            if (!dless(__shfl(x, 0), __shfl(xi, 0), tk.kd, tk.kid)) break;
-            tk.offer(x, xi);
+    for (int p0 = 0; p0 < lim; p0 += 64) {
+        const int p = p0 + lane;
+        int ns = 0, cn = 0;
+        int64_t base = 0;
+        if (p < lim) {
+            const int64_t i = q * NP + p;
+            const int c = (int)probes[i];
+            ns = nsplit[c];
+            cn = counts[c];
+            base = (int64_t)candstart[c] + slot[i];
+        }
+        int rmax = ns;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) rmax = max(rmax, __shfl_xor(rmax, o));
+        for (int r = 0; r < rmax; ++r) {
+            const int64_t li = base + (int64_t)r * cn;  // this lane's r-th list
+            const bool has = r < ns;
+            const float hd = has ? cand_d[li * topk] : INFINITY;
+            const uint32_t hi = has ? cand_id[li * topk] : 0xFFFFFFFFu;
+            unsigned long long m = __ballot(has && dless(hd, hi, tk.kd, tk.kid));
+            while (m) {
+                const int src = __builtin_ctzll(m);
+                m &= m - 1;
+                const int64_t L = __shfl(li, src);
+                const float *cd = cand_d + L * topk;
+                const uint32_t *ci = cand_id + L * topk;
+                for (int e0 = 0; e0 < topk; e0 += 64) {
+                    const int e = e0 + lane;
+                    const float x = e < topk ? cd[e] : INFINITY;
+                    const uint32_t xi = e < topk ? ci[e] : 0xFFFFFFFFu;
+                    if (!dless(__shfl(x, 0), __shfl(xi, 0), tk.kd, tk.kid)) break;
+                    tk.offer(x, xi);
+                }
+            }
         }
     }
 #pragma unroll
@@ -748,17 +865,18 @@ hipError_t launch_bq_build(const IndexView &ix, const BatchedView &bv, const int
 }
 
 hipError_t launch_bq_group(const uint32_t *probes, const int32_t *npq, int64_t nq, int NP, int K, int QT,
-                           const int32_t *order, const BqWork &w, hipStream_t st) {
+                           const int32_t *order, const int32_t *clu_n, const BqWork &w, hipStream_t st) {
     hipLaunchKernelGGL(bq_fill_kernel, dim3(64), dim3(256), 0, st, w.counts, (int64_t)K, 0);
     hipLaunchKernelGGL(bq_fill_kernel, dim3(1), dim3(64), 0, st, w.work, (int64_t)1, 0);
     const int64_t n = nq * NP;
     const unsigned g = (unsigned)((n + 255) / 256);
     hipLaunchKernelGGL(bq_count_kernel, dim3(g), dim3(256), 0, st, probes, npq, nq, NP, w.counts, w.slot);
-    hipLaunchKernelGGL(bq_scan_kernel, dim3(1), dim3(1024), 0, st, w.counts, order, K, QT, w.cstart, w.tstart,
-                       w.ntiles, w.npairs);
+    hipLaunchKernelGGL(bq_scan_kernel, dim3(1), dim3(1024), 0, st, w.counts, order, clu_n, K, QT, w.cstart, w.tstart,
+                       w.candstart, w.nsplit, w.ntiles, w.npairs, w.sv);
     hipLaunchKernelGGL(bq_scatter_kernel, dim3(g), dim3(256), 0, st, probes, w.slot, n, w.cstart, w.pairs);
-    hipLaunchKernelGGL(bq_tiles_kernel, dim3((K + 255) / 256), dim3(256), 0, st, w.counts, w.cstart, w.tstart, K, QT,
-                       w.tile_clu, w.tile_first, w.tile_cnt);
+    hipLaunchKernelGGL(bq_tiles_kernel, dim3((K + 255) / 256), dim3(256), 0, st, w.counts, w.cstart, w.tstart,
+                       w.candstart, w.nsplit, w.sv, K, QT, w.tile_clu, w.tile_first, w.tile_cnt, w.tile_v0,
+                       w.tile_cand);
     return hipGetLastError();
 }
 
@@ -826,11 +944,13 @@ hipError_t launch_bscan_tiles(const BatchedView &bv, const BqWork &w, const BqPa
     a.pc = pb.pc;
     a.pcorr = pb.pcorr;
     a.pq2 = pb.pq2;
-    a.pairs = w.pairs;
     a.tile_clu = w.tile_clu;
     a.tile_first = w.tile_first;
     a.tile_cnt = w.tile_cnt;
+    a.tile_v0 = w.tile_v0;
+    a.tile_cand = w.tile_cand;
     a.ntiles = w.ntiles;
+    a.sv = w.sv;
     a.work = w.work;
     a.topk = topk;
     a.cand_d = cand_d;
@@ -840,18 +960,20 @@ hipError_t launch_bscan_tiles(const BatchedView &bv, const BqWork &w, const BqPa
     return qs == 2 ? launch_bscan_q<4, 2>(a, grid, st) : launch_bscan_q<4, 1>(a, grid, st);
 }
 
-hipError_t launch_bq_merge(const float *cand_d, const uint32_t *cand_id, const int32_t *npq, int64_t nq, int NP,
-                           int topk, uint32_t *out_ids, float *out_d, hipStream_t st) {
+hipError_t launch_bq_merge(const float *cand_d, const uint32_t *cand_id, const uint32_t *probes, const BqWork &w,
+                           const int32_t *npq, int64_t nq, int NP, int topk, uint32_t *out_ids, float *out_d,
+                           hipStream_t st) {
     const unsigned g = (unsigned)((nq + 3) / 4);
+    auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(g), dim3(256), 0, st, cand_d, cand_id, probes, w.slot, w.counts, w.candstart,
+                           w.nsplit, npq, nq, NP, topk, out_ids, out_d);
+    };
     if (topk <= 64)
-        hipLaunchKernelGGL(bq_merge_kernel<1>, dim3(g), dim3(256), 0, st, cand_d, cand_id, npq, nq, NP, topk, out_ids,
-                           out_d);
+        go(bq_merge_kernel<1>);
     else if (topk <= 128)
-        hipLaunchKernelGGL(bq_merge_kernel<2>, dim3(g), dim3(256), 0, st, cand_d, cand_id, npq, nq, NP, topk, out_ids,
-                           out_d);
+        go(bq_merge_kernel<2>);
     else
-        hipLaunchKernelGGL(bq_merge_kernel<4>, dim3(g), dim3(256), 0, st, cand_d, cand_id, npq, nq, NP, topk, out_ids,
-                           out_d);
+        go(bq_merge_kernel<4>);
     return hipGetLastError();
 }
 

@@ -7,6 +7,10 @@ namespace saqgpu {
 constexpr int kMB = 16;     // vectors per MFMA column block (one mblock record)
 constexpr int kBWaves = 4;  // waves per tile workgroup
 constexpr int kBThreads = 64 * kBWaves;
+// Vector ranges of a list's work tiles: at least kMinSplit vectors, at most about the launch's
+// work / kSplitWaves (the resident wavefronts of a 256-CU chip at 2 waves per SIMD).
+constexpr int kMinSplit = 512;
+constexpr int kSplitWaves = 2048;
 
 struct BSeg {
     int32_t dim, bits, doff;
@@ -29,10 +33,16 @@ struct BatchedView {
     const BSeg *segs;
 };
 
-// Grouping workspace: (query, list) pairs grouped by list, and the work tiles over them.
+// Grouping workspace: (query, list) pairs grouped by list, and the work tiles over them (a
+// work tile: up to QT pairs of one list and one vector range of it, whose per-pair top-k goes
+// to candidate lists tile_cand .. + cnt - 1).
 struct BqWork {
-    int32_t *counts, *slot, *cstart, *tstart, *ntiles, *npairs, *pairs, *tile_clu, *tile_first, *tile_cnt, *work;
+    int32_t *counts, *slot, *cstart, *tstart, *candstart, *nsplit, *ntiles, *npairs, *sv, *pairs;
+    int32_t *tile_clu, *tile_first, *tile_cnt, *tile_v0, *tile_cand, *work;
 };
+// Upper bounds of the work tiles and candidate lists of a launch.
+inline int64_t bq_max_tiles(int64_t npairs, int K) { return npairs + K + kSplitWaves; }
+inline int64_t bq_max_cands(int64_t npairs, int QT) { return npairs + (int64_t)kSplitWaves * QT; }
 
 // Per grouped pair: the quantised residual rows (A operand) and the pair's constants.
 struct BqPairBufs {
@@ -48,9 +58,10 @@ hipError_t launch_bq_build(const IndexView &ix, const BatchedView &bv, const int
                            hipStream_t st);
 // order: the lists by decreasing size (the order work tiles are handed out in).
 hipError_t launch_bq_group(const uint32_t *probes, const int32_t *npq, int64_t nq, int NP, int K, int QT,
-                           const int32_t *order, const BqWork &w, hipStream_t st);
-hipError_t launch_bq_merge(const float *cand_d, const uint32_t *cand_id, const int32_t *npq, int64_t nq, int NP,
-                           int topk, uint32_t *out_ids, float *out_d, hipStream_t st);
+                           const int32_t *order, const int32_t *clu_n, const BqWork &w, hipStream_t st);
+hipError_t launch_bq_merge(const float *cand_d, const uint32_t *cand_id, const uint32_t *probes, const BqWork &w,
+                           const int32_t *npq, int64_t nq, int NP, int topk, uint32_t *out_ids, float *out_d,
+                           hipStream_t st);
 
 // Quantised residual rows and constants of every grouped pair (max_pairs bounds the grid).
 hipError_t launch_bq_pairprep(const BatchedView &bv, const float *qrot, const uint32_t *probes, const BqWork &w,

@@ -84,6 +84,16 @@ def test_query_tiles_give_identical_results():
         assert np.array_equal(d.view(np.uint32), out[0][1].view(np.uint32))
 
 
+@pytest.mark.parametrize("topk,tiles", [(10, (16, 64)), (100, (16, 32))])
+def test_long_lists_split_into_vector_ranges(topk, tiles):
+    """Lists of ~4000 vectors: each (list, query tile) becomes 16 work tiles of 256-vector
+    ranges with candidate lists of their own, which the merge reads back per pair."""
+    case = F.make_case(n=8000, d=128, K=2, nq=150, plan=[(64, 6), (64, 3)], seed=63)
+    out = [run(case, topk, 2, PREC_INT8, query_tile=t) for t in tiles]
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1].view(np.uint32), out[1][1].view(np.uint32))
+
+
 def test_uneven_nprobe_per_query():
     case = F.make_case(n=2000, d=128, K=16, nq=40, plan=[(64, 4), (64, 4)], seed=59)
     npq = np.random.default_rng(3).integers(1, 9, 40).astype(np.int32)
@@ -104,7 +114,7 @@ def test_empty_lists_and_gpu_probe():
 
 
 def test_fast_probe_selects_the_same_lists_without_near_ties():
-    """The fp32-MFMA coarse quantiser differs from the exact one only where two centroid
+    """The GEMM-based coarse quantiser differs from the exact one only where two centroid
     distances tie within fp32 rounding; on this data no query has such a tie at its nprobe-th
     list, so every result equals the exact-probe result."""
     case = F.make_case(n=4000, d=200, K=24, nq=64, plan=[(128, 5), (128, 3)], seed=67)
