@@ -382,6 +382,37 @@ struct WaveTopK {
     }
 };
 
+// ------------------------------------------------------------------ buffered selection (k <= 64)
+// A wave-wide sorted list of 64 (distance, id) pairs (lane l holds the l-th smallest; unused
+// places hold (INF, 0xFFFFFFFF)) absorbs up to 64 candidates at once: the candidates are
+// sorted by a bitonic network across the lanes, reversed and taken pairwise-smaller against
+// the list (the 64 smallest of both, as a bitonic sequence), then a bitonic merge sorts them.
+__device__ __forceinline__ void cswap_lane(float &d, uint32_t &id, int j, bool keep_small) {
+    const float pd = __shfl_xor(d, j);
+    const uint32_t pi = __shfl_xor(id, j);
+    const bool other_less = dless(pd, pi, d, id);
+    if (other_less == keep_small) {
+        d = pd;
+        id = pi;
+    }
+}
+
+__device__ __forceinline__ void merge_into64(float &d, uint32_t &id, float cd, uint32_t ci) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int k = 2; k <= 64; k <<= 1)
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) cswap_lane(cd, ci, j, ((lane & j) == 0) == ((lane & k) == 0 || k == 64));
+    const float rd = __shfl(cd, 63 - lane);
+    const uint32_t ri = __shfl(ci, 63 - lane);
+    if (dless(rd, ri, d, id)) {
+        d = rd;
+        id = ri;
+    }
+#pragma unroll
+    for (int j = 32; j > 0; j >>= 1) cswap_lane(d, id, j, (lane & j) == 0);
+}
+
 // ------------------------------------------------------------------ pair preparation
 // One wave per grouped (query, list) pair: the residual r = q_rot - c_rot of every segment,
 // quantised to int8 (plus the second int8 term for int8x2) as the pair's row of the MFMA A
@@ -503,11 +534,18 @@ struct WSmem {
     int32_t *tkn;    // [QT]
     float *tkd;      // [QT][topk]
     uint32_t *tki;   // [QT][topk]
+    float *fac;      // [2][NVW][16 * (2 + nqs)]: factor blocks of the current and the next chunk
+    float *cbd;      // [QT][kCap]: candidates of a row not yet merged into its list (k <= 64)
+    uint32_t *cbi;   // [QT][kCap]
+    int32_t *cbn;    // [QT]
 };
 
 // mblocks per wave chunk: 64 vectors' accumulators per query tile for int8, half that for
 // int8x2 (whose second term doubles the accumulators)
 __host__ __device__ constexpr int nvw_of(int qtm, int qs) { return qs == 2 ? (qtm >= 2 ? 1 : 2) : 4 / qtm; }
+
+// candidate buffer places per row: 64 for 16-query tiles, fewer for larger tiles (LDS)
+__host__ __device__ constexpr int cand_cap(int qtm) { return 64 / qtm; }
 
 template <int QTM, int QS>
 __host__ __device__ inline size_t wave_smem(int nqs, int topk, WSmem *m, uint8_t *base) {
@@ -529,6 +567,11 @@ __host__ __device__ inline size_t wave_smem(int nqs, int topk, WSmem *m, uint8_t
     s.tkn = (int32_t *)take(QT * 4);
     s.tkd = (float *)take((size_t)QT * topk * 4);
     s.tki = (uint32_t *)take((size_t)QT * topk * 4);
+    s.fac = (float *)take((size_t)2 * nvw_of(QTM, QS) * 64 * (2 + nqs));
+    const int cap = topk <= 64 ? cand_cap(QTM) : 0;
+    s.cbd = (float *)take((size_t)QT * cap * 4);
+    s.cbi = (uint32_t *)take((size_t)QT * cap * 4);
+    s.cbn = (int32_t *)take((size_t)QT * 4);
     if (m) *m = s;
     return off;
 }
@@ -538,6 +581,35 @@ __host__ __device__ inline size_t wave_smem(int nqs, int topk, WSmem *m, uint8_t
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     __builtin_amdgcn_wave_barrier();
+}
+
+// The row's sorted list (k <= 64 places, padded) back to LDS with its k-th pair.
+__device__ __forceinline__ void store_row(const WSmem &m, int i, int topk, float d, uint32_t id) {
+    const int lane = threadIdx.x & 63;
+    if (lane < topk) {
+        m.tkd[i * topk + lane] = d;
+        m.tki[i * topk + lane] = id;
+    }
+    const float kd = __shfl(d, topk - 1);
+    const uint32_t ki = __shfl(id, topk - 1);
+    if (lane == 0) {
+        m.kd[i] = kd;
+        m.ki[i] = ki;
+    }
+    wave_sync();
+}
+
+// Merges row i's nb buffered candidates into its list and empties the buffer.
+__device__ __forceinline__ void flush_row(const WSmem &m, int i, int topk, int nb, int cap) {
+    const int lane = threadIdx.x & 63;
+    float d = lane < topk ? m.tkd[i * topk + lane] : INFINITY;
+    uint32_t id = lane < topk ? m.tki[i * topk + lane] : 0xFFFFFFFFu;
+    const float cd = lane < nb ? m.cbd[i * cap + lane] : INFINITY;
+    const uint32_t ci = lane < nb ? m.cbi[i * cap + lane] : 0xFFFFFFFFu;
+    merge_into64(d, id, cd, ci);
+    store_row(m, i, topk, d, id);
+    if (lane == 0) m.cbn[i] = 0;
+    wave_sync();
 }
 
 __device__ __forceinline__ v4i unpack_nibbles(uint2 x) {
@@ -616,7 +688,7 @@ __device__ __forceinline__ void seg_mma(const BSeg &bs, const uint8_t *const (&r
 }
 
 template <int QTM, int QS, int KR>
-__global__ __launch_bounds__(kBThreads) void bscan_kernel(BScanArgs a) {
+__global__ __launch_bounds__(kBThreads, (QTM == 4 && QS == 2) ? 1 : 2) void bscan_kernel(BScanArgs a) {
     constexpr int QT = 16 * QTM;
     constexpr int NVW = nvw_of(QTM, QS);  // mblocks per chunk
     constexpr int CVW = 16 * NVW;         // vectors per chunk
@@ -649,7 +721,13 @@ __global__ __launch_bounds__(kBThreads) void bscan_kernel(BScanArgs a) {
             m.kd[row] = INFINITY;
             m.ki[row] = 0xFFFFFFFFu;
             m.tkn[row] = 0;
+            m.cbn[row] = 0;
         }
+        if (KR == 1)
+            for (int e = lane; e < QT * a.topk; e += 64) {
+                m.tkd[e] = INFINITY;
+                m.tki[e] = 0xFFFFFFFFu;
+            }
         wave_sync();
         // A operand rows of this lane: query row t*16 + lane%16 of the tile
         const int8_t *arow1[QTM], *arow2[QTM];
@@ -664,18 +742,37 @@ __global__ __launch_bounds__(kBThreads) void bscan_kernel(BScanArgs a) {
         const int v0 = a.tile_v0[tile];
         const int mb_beg = v0 / kMB, mb_end = min((n + kMB - 1) / kMB, (v0 + sv) / kMB);
         const int64_t mb0 = bv.clu_mb[c];
-        for (int cb = mb_beg; cb < mb_end; cb += NVW) {
+        // factor blocks (|o|^2, the rescale of every quantised segment, the ids) of a chunk's
+        // mblocks move to LDS by DMA one chunk ahead of their use
+        const int fac_words = kMB * (2 + bv.nqs);
+        auto fetch_fac = [&](int cbn, int slot) {
+#pragma unroll
+            for (int j = 0; j < NVW; ++j) {
+                const uint8_t *src = bv.mb + (mb0 + min(cbn + j, mb_end - 1)) * bv.mb_stride + bv.fac_off;
+                float *dst = m.fac + (slot * NVW + j) * fac_words;
+                for (int off = 0; off < fac_words * 4; off += 1024)
+                    if (off + lane * 16 < fac_words * 4)
+                        __builtin_amdgcn_global_load_lds(src + off + lane * 16, reinterpret_cast<uint8_t *>(dst) + off,
+                                                         16, 0, 0);
+            }
+        };
+        int slot = 0;
+        if (mb_beg < mb_end) fetch_fac(mb_beg, 0);
+        for (int cb = mb_beg; cb < mb_end; cb += NVW, slot ^= 1) {
             const uint8_t *rec[NVW];
             float dist[QTM][NVW][4];
+            dma_wait();  // this chunk's factor blocks (the previous chunk's reads of the other slot are done)
+            const float *fac = m.fac + slot * NVW * fac_words;
 #pragma unroll
             for (int j = 0; j < NVW; ++j) {
                 rec[j] = bv.mb + (mb0 + min(cb + j, mb_end - 1)) * bv.mb_stride;  // clamped duplicates are discarded
-                const float o2 = reinterpret_cast<const float *>(rec[j] + bv.fac_off)[lane % kMB];
+                const float o2 = fac[j * fac_words + lane % kMB];
 #pragma unroll
                 for (int t = 0; t < QTM; ++t)
 #pragma unroll
                     for (int r = 0; r < 4; ++r) dist[t][j][r] = o2 + m.q2[t * 16 + 4 * (lane / 16) + r];
             }
+            if (cb + NVW < mb_end) fetch_fac(cb + NVW, slot ^ 1);
             for (int s = 0; s < bv.nseg; ++s) {
                 const BSeg bs = bv.segs[s];
                 if (bs.qs < 0) continue;
@@ -693,8 +790,7 @@ __global__ __launch_bounds__(kBThreads) void bscan_kernel(BScanArgs a) {
                 // epilogue of the segment
 #pragma unroll
                 for (int j = 0; j < NVW; ++j) {
-                    const float resc =
-                        reinterpret_cast<const float *>(rec[j] + bv.fac_off)[kMB * (1 + bs.qs) + lane % kMB];
+                    const float resc = fac[j * fac_words + kMB * (1 + bs.qs) + lane % kMB];
 #pragma unroll
                     for (int t = 0; t < QTM; ++t)
 #pragma unroll
@@ -742,12 +838,45 @@ __global__ __launch_bounds__(kBThreads) void bscan_kernel(BScanArgs a) {
                     for (int r = 0; r < 4; ++r)
                         m.dt[(t * 16 + 4 * (lane / 16) + r) * CVW + lv] = valid ? dist[t][j][r] : INFINITY;
                 if (lane < kMB)
-                    m.dids[lv] = valid ? reinterpret_cast<const uint32_t *>(rec[j] + bv.fac_off)[kMB * (1 + bv.nqs) + lane]
-                                       : 0xFFFFFFFFu;
+                    m.dids[lv] = valid ? __float_as_uint(fac[j * fac_words + kMB * (1 + bv.nqs) + lane]) : 0xFFFFFFFFu;
             }
             wave_sync();
             const float x_id_pad = INFINITY;
-            while (rows) {
+            while (KR == 1 && rows) {
+                // buffered: candidates that beat the row's k-th pair are appended to its buffer;
+                // a full buffer is merged into the list (which lowers the k-th pair)
+                const int i = __builtin_ctzll(rows);
+                rows &= rows - 1;
+                const float x = lane < CVW ? m.dt[i * CVW + lane] : x_id_pad;
+                const uint32_t xi = lane < CVW ? m.dids[lane] : 0xFFFFFFFFu;
+                bool pass = dless(x, xi, m.kd[i], m.ki[i]);
+                unsigned long long b = __ballot(pass);
+                if (!b) continue;
+                constexpr int kCap = cand_cap(QTM);
+                int nb = m.cbn[i];
+                if (nb + __popcll(b) > kCap) {
+                    if (nb) flush_row(m, i, a.topk, nb, kCap);
+                    nb = 0;
+                    pass = dless(x, xi, m.kd[i], m.ki[i]);
+                    b = __ballot(pass);
+                    if (__popcll(b) > kCap) {  // more than a buffer holds: merge the chunk's row directly
+                        float d = lane < a.topk ? m.tkd[i * a.topk + lane] : INFINITY;
+                        uint32_t id = lane < a.topk ? m.tki[i * a.topk + lane] : 0xFFFFFFFFu;
+                        merge_into64(d, id, pass ? x : INFINITY, pass ? xi : 0xFFFFFFFFu);
+                        store_row(m, i, a.topk, d, id);
+                        b = 0;
+                    }
+                }
+                if (pass && b) {
+                    const int pos = nb + __builtin_amdgcn_mbcnt_hi((uint32_t)(b >> 32),
+                                                                    __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0));
+                    m.cbd[i * kCap + pos] = x;
+                    m.cbi[i * kCap + pos] = xi;
+                }
+                if (lane == 0) m.cbn[i] = nb + __popcll(b);
+                wave_sync();
+            }
+            while (KR > 1 && rows) {
                 const int i = __builtin_ctzll(rows);
                 rows &= rows - 1;
                 const float x = lane < CVW ? m.dt[i * CVW + lane] : x_id_pad;
@@ -768,8 +897,20 @@ __global__ __launch_bounds__(kBThreads) void bscan_kernel(BScanArgs a) {
             wave_sync();  // the next chunk rewrites dt and dids
         }
         // ---- the tile's per-query lists to global memory
+        if (KR == 1)
+            for (int i = 0; i < cnt; ++i) {
+                const int nb = m.cbn[i];
+                if (nb) flush_row(m, i, a.topk, nb, cand_cap(QTM));
+            }
         for (int i = 0; i < cnt; ++i) {
             const int64_t pr = (int64_t)a.tile_cand[tile] + i;
+            if (KR == 1) {  // padded lists: unused places already hold (INF, 0xFFFFFFFF)
+                for (int e = lane; e < a.topk; e += 64) {
+                    a.cand_d[pr * a.topk + e] = m.tkd[i * a.topk + e];
+                    a.cand_id[pr * a.topk + e] = m.tki[i * a.topk + e];
+                }
+                continue;
+            }
             const int nn = m.tkn[i];
             for (int e = lane; e < a.topk; e += 64) {
                 a.cand_d[pr * a.topk + e] = e < nn ? m.tkd[i * a.topk + e] : INFINITY;

@@ -5,7 +5,7 @@
 namespace saqgpu {
 
 constexpr int kMB = 16;     // vectors per MFMA column block (one mblock record)
-constexpr int kBWaves = 4;  // waves per tile workgroup
+constexpr int kBWaves = 1;  // waves per tile workgroup (one: LDS is then allocated per wavefront)
 constexpr int kBThreads = 64 * kBWaves;
 // Vector ranges of a list's work tiles: at least kMinSplit vectors, at most about the launch's
 // work / kSplitWaves (the resident wavefronts of a 256-CU chip at 2 waves per SIMD).
