@@ -584,14 +584,14 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 // The row's sorted list (k <= 64 places, padded) back to LDS with its k-th pair.
-__device__ __forceinline__ void store_row(const WSmem &m, int i, int topk, float d, uint32_t id) {
+__device__ __forceinline__ void store_row(const WSmem m, int i, int topk, float d, uint32_t id) {
     const int lane = threadIdx.x & 63;
     if (lane < topk) {
         m.tkd[i * topk + lane] = d;
         m.tki[i * topk + lane] = id;
     }
-    const float kd = __shfl(d, topk - 1);
-    const uint32_t ki = __shfl(id, topk - 1);
+    const float kd = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(d), topk - 1));
+    const uint32_t ki = (uint32_t)__builtin_amdgcn_readlane((int)id, topk - 1);
     if (lane == 0) {
         m.kd[i] = kd;
         m.ki[i] = ki;
@@ -599,14 +599,40 @@ __device__ __forceinline__ void store_row(const WSmem &m, int i, int topk, float
     wave_sync();
 }
 
-// Merges row i's nb buffered candidates into its list and empties the buffer.
-__device__ __forceinline__ void flush_row(const WSmem &m, int i, int topk, int nb, int cap) {
+// Merges row i's nb buffered candidates into its list and empties the buffer: up to
+// kSeqInsert of them one by one (a position ballot and a one-lane shift each), more by the
+// bitonic merge.
+constexpr int kSeqInsert = 8;
+
+__device__ __forceinline__ void flush_row(const WSmem m, int i, int topk, int nb, int cap) {
     const int lane = threadIdx.x & 63;
     float d = lane < topk ? m.tkd[i * topk + lane] : INFINITY;
     uint32_t id = lane < topk ? m.tki[i * topk + lane] : 0xFFFFFFFFu;
-    const float cd = lane < nb ? m.cbd[i * cap + lane] : INFINITY;
-    const uint32_t ci = lane < nb ? m.cbi[i * cap + lane] : 0xFFFFFFFFu;
-    merge_into64(d, id, cd, ci);
+    if (nb <= kSeqInsert) {
+        float kd = m.kd[i];
+        uint32_t ki = m.ki[i];
+        for (int e = 0; e < nb; ++e) {
+            const float x = m.cbd[i * cap + e];
+            const uint32_t xi = m.cbi[i * cap + e];
+            if (!dless(x, xi, kd, ki)) continue;
+            const int pos = __popcll(__ballot(dless(d, id, x, xi)));
+            const float ud = __shfl_up(d, 1);
+            const uint32_t ui = __shfl_up(id, 1);
+            if (lane > pos) {
+                d = ud;
+                id = ui;
+            } else if (lane == pos) {
+                d = x;
+                id = xi;
+            }
+            kd = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(d), topk - 1));
+            ki = (uint32_t)__builtin_amdgcn_readlane((int)id, topk - 1);
+        }
+    } else {
+        const float cd = lane < nb ? m.cbd[i * cap + lane] : INFINITY;
+        const uint32_t ci = lane < nb ? m.cbi[i * cap + lane] : 0xFFFFFFFFu;
+        merge_into64(d, id, cd, ci);
+    }
     store_row(m, i, topk, d, id);
     if (lane == 0) m.cbn[i] = 0;
     wave_sync();
@@ -699,10 +725,14 @@ __global__ __launch_bounds__(kBThreads, (QTM == 4 && QS == 2) ? 1 : 2) void bsca
     WSmem m;
     wave_smem<QTM, QS>(bv.nqs, a.topk, &m, smem_raw + w * per_wave);
     const int ntiles = *a.ntiles, sv = *a.sv;
-    for (;;) {
-        int tile = 0;
-        if (lane == 0) tile = atomicAdd(a.work, 1);
-        tile = __shfl(tile, 0);
+    // the first tile of every wavefront is its own index; later ones come from the counter
+    const int nwaves = gridDim.x * kBWaves;
+    for (int round = 0;; ++round) {
+        int tile = blockIdx.x * kBWaves + w;
+        if (round > 0) {
+            if (lane == 0) tile = nwaves + atomicAdd(a.work, 1);
+            tile = __shfl(tile, 0);
+        }
         if (tile >= ntiles) break;
         const int c = a.tile_clu[tile], first = a.tile_first[tile], cnt = a.tile_cnt[tile];
         // ---- the tile's pair constants to LDS; rows past cnt repeat the last pair
