@@ -169,8 +169,8 @@ typedef struct {
     int32_t precision;                /* SAQGPU_PREC_INT8 or SAQGPU_PREC_INT8X2 */
     int32_t query_tile;               /* queries per work tile: 16, 32 or 64; 0 = the autotuned choice */
     int32_t exact_probe;              /* with probes == NULL: 1 = the reference's coarse quantiser order
-                                         (as saqgpu_probe), 0 = fp32-MFMA distances (probe lists may
-                                         differ where two centroid distances tie within fp32 rounding) */
+                                         (as saqgpu_probe), 0 = an fp32 GEMM's |c|^2 - 2 q.c (probe lists
+                                         may differ where two centroid distances tie within fp32 rounding) */
 } saqgpu_batched_params;
 
 /* Builds the batched code layout (full codes in MFMA operand order plus per-vector factors)
@@ -178,7 +178,8 @@ typedef struct {
 int saqgpu_prepare_batched(saqgpu_index *idx);
 
 /* Batched search: probes as in saqgpu_search (NULL = GPU coarse quantiser, nprobe lists per
- * query); out_ids nq x topk (unfilled = 0), out_dists optional. */
+ * query); out_ids nq x topk (unfilled = 0), out_dists optional.  Segments of more than 2048
+ * dimensions and indexes of more than 16384 lists are not supported (SAQGPU_EUNSUPPORTED). */
 int saqgpu_search_batched(saqgpu_index *idx, const float *queries, int64_t nq, const uint32_t *probes,
                           const saqgpu_batched_params *params, uint32_t *out_ids, float *out_dists,
                           int32_t flags, void *stream);

@@ -36,8 +36,8 @@ size_t estimate_smem_bytes(const IndexView &ix);
 hipError_t launch_estimate(const IndexView &ix, const float *qrot, const float *bound, const uint32_t *probes,
                            int nprobe, const int64_t *offs, int dist_type, uint32_t *out_ids, float *out_acc,
                            float *out_fast, float *out_vars, hipStream_t st);
-hipError_t launch_probe_select_ip(const float *ip, const float *qnorm, const float *cnorm, int K, int64_t nq,
-                                  int nprobe, int dist_type, uint32_t *probes, hipStream_t st);
+hipError_t launch_probe_select_ip(const float *ip, const float *cnorm, int K, int64_t nq, int nprobe, int dist_type,
+                                  uint32_t *probes, hipStream_t st);
 hipError_t launch_encode_segment(const IndexView &ix, const float *data, const uint32_t *src, const int64_t *gvec,
                                  const int64_t *gblk, const int32_t *lane, const uint32_t *clu, const float *cent,
                                  const float *rot, int seg, int seg_dim, int64_t nslots, int64_t ld, int rd_lmt,
@@ -149,7 +149,7 @@ struct saqgpu_index {
     DevBuf<BSeg> d_bsegs;
     DevBuf<int32_t> bq_ws, bq_npq, bq_pcorr;
     DevBuf<int8_t> bq_qq;
-    DevBuf<float> bq_pc, bq_cnorm, bq_qnorm;
+    DevBuf<float> bq_pc, bq_cnorm;
     DevBuf<float> bq_cand_d;
     DevBuf<uint32_t> bq_cand_id;
     // measurement
@@ -377,7 +377,6 @@ void saqgpu_index_destroy(saqgpu_index *ix) {
     ix->bq_qq.release();
     ix->bq_pc.release();
     ix->bq_cnorm.release();
-    ix->bq_qnorm.release();
     ix->bq_npq.release();
     ix->bq_cand_d.release();
     ix->bq_cand_id.release();
@@ -1133,6 +1132,11 @@ int prepare_batched_locked(saqgpu_index *ix) {
     if (ix->bq_ready) return SAQGPU_OK;
     for (int c = 0; c < ix->num_cen; ++c)
         if (!ix->clu_set[c] && ix->clu_n[c] > 0) return fail(SAQGPU_ESTATE, "index has clusters not yet set");
+    for (int s = 0; s < ix->nseg; ++s)
+        if (ix->segs[s].dim > 2048)
+            return fail(SAQGPU_EUNSUPPORTED, "the batched scan takes segments of up to 2048 dimensions");
+    if (ix->num_cen > 1024 * kScanRounds)
+        return fail(SAQGPU_EUNSUPPORTED, "the batched scan takes up to 16384 lists");
     ix->bsegs.assign(ix->nseg, BSeg{});
     int kq = 0, nqs = 0;
     int64_t code = 0;
@@ -1214,8 +1218,7 @@ int default_query_tile(const saqgpu_index *ix, int64_t nq, int nprobe) {
 }
 
 // The batched scan's coarse quantiser: q.c for every (query, centroid) from rocBLAS's fp32 GEMM
-// into ws_cdist, then the exact top-nprobe selection over (|q|^2 + |c|^2) - 2 q.c (or q.c for
-// IP).  Not FlatInitializer's reduction order: a probe list can differ from the reference's
+// into ws_cdist, then the exact top-nprobe selection over |c|^2 - 2 q.c (or q.c for IP).  Not FlatInitializer's reduction order: a probe list can differ from the reference's
 // where two centroid distances lie within fp32 rounding of each other.
 int probe_gemm(saqgpu_index *ix, const float *dq, int64_t nq, int nprobe, int dist_type, uint32_t *probes,
                hipStream_t st) {
@@ -1225,13 +1228,12 @@ int probe_gemm(saqgpu_index *ix, const float *dq, int64_t nq, int nprobe, int di
         return fail(SAQGPU_EDEVICE, "rocblas_create_handle failed");
     }
     if (rocblas_set_stream(ix->blas, st) != rocblas_status_success) return fail(SAQGPU_EDEVICE, "rocblas_set_stream failed");
-    HIPCHK(launch_sqnorm_rows(dq, nq, D, ix->bq_qnorm.p, st));
     // column-major view: ip^T (K x nq, ld K) = centroids (D x K, ld D)^T * queries (D x nq, ld D)
     const float one = 1.0f, zero = 0.0f;
     if (rocblas_sgemm(ix->blas, rocblas_operation_transpose, rocblas_operation_none, K, (int)nq, D, &one,
                       ix->centroids.p, D, dq, D, &zero, ix->ws_cdist.p, K) != rocblas_status_success)
         return fail(SAQGPU_EDEVICE, "rocblas_sgemm failed");
-    HIPCHK(launch_probe_select_ip(ix->ws_cdist.p, ix->bq_qnorm.p, ix->bq_cnorm.p, K, nq, nprobe, dist_type, probes, st));
+    HIPCHK(launch_probe_select_ip(ix->ws_cdist.p, ix->bq_cnorm.p, K, nq, nprobe, dist_type, probes, st));
     return SAQGPU_OK;
 }
 
@@ -1290,7 +1292,7 @@ int search_batched_locked(saqgpu_index *ix, const float *queries, int64_t nq, co
         dd = out_dists ? ix->ws_dists.p : nullptr;
     }
     if (!probes) {
-        if (ix->ws_cdist.ensure((size_t)nq * K) || ix->bq_qnorm.ensure(nq))
+        if (ix->ws_cdist.ensure((size_t)nq * K))
             return fail(SAQGPU_ENOMEM, "workspace allocation failed");
         if (p->exact_probe)
             HIPCHK(launch_probe(ix->centroids.p, K, ix->num_dim, dq, nq, NP, 0, ix->ws_cdist.p, ix->ws_probes.p, st));

@@ -182,8 +182,8 @@ __global__ void bq_count_kernel(const uint32_t *probes, const int32_t *npq, int6
 
 // One workgroup.  The split size first: W = sum over lists of (query tiles x list vectors) is
 // the launch's work in vector-tiles, and a list is cut into ranges of sv = max(kMinSplit,
-// W / kSplitWaves) vectors (a multiple of 64), so no work tile holds more than about half of a
-// resident wavefront's share and one long list cannot set the kernel's tail.  Then exclusive
+// W / kSplitWaves) vectors (a multiple of 64), so no work tile holds more than about one resident
+// wavefront's share and one long list cannot set the kernel's tail.  Then exclusive
 // prefix sums, over the lists in the given order (largest first, so the longest tiles start
 // first), of the pair counts, of the work tiles (query tiles x ranges) and of the candidate
 // lists (pairs x ranges).  The bounds sum_c tiles_c <= npairs / QT + K + kSplitWaves and
@@ -192,61 +192,81 @@ __global__ __launch_bounds__(1024) void bq_scan_kernel(const int32_t *counts, co
                                                        const int32_t *clu_n, int K, int QT, int32_t *cstart,
                                                        int32_t *tstart, int32_t *candstart, int32_t *nsplit,
                                                        int32_t *ntiles, int32_t *npairs, int32_t *sv_out) {
-    __shared__ int32_t part_c[1024], part_t[1024], part_k[1024];
-    __shared__ long long wsum[16];
+    // thread t holds lists order[t * per .. t * per + per - 1] (per <= kScanRounds)
+    __shared__ int wsum[3][16];
+    __shared__ long long wk_s[16];
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     const int per = (K + 1023) / 1024;
-    const int b = t * per, e = min(K, b + per);
-    long long wk = 0;
-    for (int i = b; i < e; ++i) {
-        const int c = order[i];
-        wk += (long long)((counts[c] + QT - 1) / QT) * clu_n[c];
+    int oc[kScanRounds], cn[kScanRounds], nv[kScanRounds];
+#pragma unroll
+    for (int j = 0; j < kScanRounds; ++j) oc[j] = j < per && t * per + j < K ? order[t * per + j] : -1;
+#pragma unroll
+    for (int j = 0; j < kScanRounds; ++j) {
+        cn[j] = oc[j] >= 0 ? counts[oc[j]] : 0;
+        nv[j] = oc[j] >= 0 ? clu_n[oc[j]] : 0;
     }
+    long long wk = 0;
+#pragma unroll
+    for (int j = 0; j < kScanRounds; ++j) wk += (long long)((cn[j] + QT - 1) / QT) * nv[j];
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) wk += __shfl_xor(wk, o);
-    if (lane == 0) wsum[wv] = wk;
+    if (lane == 0) wk_s[wv] = wk;
     __syncthreads();
     long long W = 0;
-    for (int i = 0; i < 16; ++i) W += wsum[i];
+    for (int i = 0; i < 16; ++i) W += wk_s[i];
     const long long want = (W + kSplitWaves - 1) / kSplitWaves;
     const int sv = (int)max((long long)kMinSplit, (want + 63) / 64 * 64);
-    auto splits = [&](int c) { return max(1, (clu_n[c] + sv - 1) / sv); };
-    int32_t sc = 0, st = 0, sk = 0;
-    for (int i = b; i < e; ++i) {
-        const int c = order[i];
-        const int ns = splits(c);
-        sc += counts[c];
-        st += (counts[c] + QT - 1) / QT * ns;
-        sk += counts[c] * ns;
+    int ns[kScanRounds];
+    int sc = 0, st = 0, sk = 0;
+#pragma unroll
+    for (int j = 0; j < kScanRounds; ++j) {
+        ns[j] = max(1, (nv[j] + sv - 1) / sv);
+        sc += cn[j];
+        st += (cn[j] + QT - 1) / QT * ns[j];
+        sk += cn[j] * ns[j];
     }
-    part_c[t] = sc;
-    part_t[t] = st;
-    part_k[t] = sk;
+    // exclusive prefix of the three per-thread sums over the workgroup
+    int ic = sc, it = st, ik = sk;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int uc = __shfl_up(ic, o), ut = __shfl_up(it, o), uk = __shfl_up(ik, o);
+        if (lane >= o) {
+            ic += uc;
+            it += ut;
+            ik += uk;
+        }
+    }
+    if (lane == 63) {
+        wsum[0][wv] = ic;
+        wsum[1][wv] = it;
+        wsum[2][wv] = ik;
+    }
     __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        const int32_t a = t >= off ? part_c[t - off] : 0, bt = t >= off ? part_t[t - off] : 0,
-                      bk = t >= off ? part_k[t - off] : 0;
-        __syncthreads();
-        part_c[t] += a;
-        part_t[t] += bt;
-        part_k[t] += bk;
-        __syncthreads();
+    int bc = 0, bt = 0, bk = 0, tc = 0, tt = 0, tk = 0;
+    for (int i = 0; i < 16; ++i) {
+        const bool before = i < wv;
+        bc += before ? wsum[0][i] : 0;
+        bt += before ? wsum[1][i] : 0;
+        bk += before ? wsum[2][i] : 0;
+        tc += wsum[0][i];
+        tt += wsum[1][i];
+        tk += wsum[2][i];
     }
-    int32_t rc = part_c[t] - sc, rt = part_t[t] - st, rk = part_k[t] - sk;
-    for (int i = b; i < e; ++i) {
-        const int c = order[i];
-        const int ns = splits(c);
-        cstart[c] = rc;
-        tstart[c] = rt;
-        candstart[c] = rk;
-        nsplit[c] = ns;
-        rc += counts[c];
-        rt += (counts[c] + QT - 1) / QT * ns;
-        rk += counts[c] * ns;
+    int rc = bc + ic - sc, rt = bt + it - st, rk = bk + ik - sk;
+#pragma unroll
+    for (int j = 0; j < kScanRounds; ++j) {
+        if (oc[j] < 0) continue;
+        cstart[oc[j]] = rc;
+        tstart[oc[j]] = rt;
+        candstart[oc[j]] = rk;
+        nsplit[oc[j]] = ns[j];
+        rc += cn[j];
+        rt += (cn[j] + QT - 1) / QT * ns[j];
+        rk += cn[j] * ns[j];
     }
-    if (t == 1023) {
-        *ntiles = part_t[1023];
-        *npairs = part_c[1023];
+    if (t == 0) {
+        *ntiles = tt;
+        *npairs = tc;
         *sv_out = sv;
     }
 }
@@ -447,6 +467,83 @@ __device__ __forceinline__ int wave_isum(int v) {
     return v;
 }
 
+// Quantises one segment's residual r (held as float4 per lane: element 4 lane + 256 k) and
+// writes the pair's row bytes and constants; NK = float4 registers per lane (dim <= 256 NK).
+template <int NK>
+__device__ __forceinline__ void pairprep_segment(const PairPrepArgs &a, const BSeg &bs, int64_t g, const float *qr,
+                                                 const float *cr, float &q2) {
+    const int lane = threadIdx.x & 63;
+    const BatchedView &bv = a.bv;
+    float4 r[NK];
+    float amax = 0.0f, sr = 0.0f, sq = 0.0f;
+#pragma unroll
+    for (int k = 0; k < NK; ++k) {
+        const int d = 4 * lane + 256 * k;
+        if (d < bs.dim) {
+            const float4 x = *reinterpret_cast<const float4 *>(qr + bs.doff + d);
+            const float4 y = *reinterpret_cast<const float4 *>(cr + bs.doff + d);
+            r[k] = make_float4(x.x - y.x, x.y - y.y, x.z - y.z, x.w - y.w);
+        } else {
+            r[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+        amax = fmaxf(amax, fmaxf(fmaxf(fabsf(r[k].x), fabsf(r[k].y)), fmaxf(fabsf(r[k].z), fabsf(r[k].w))));
+        sr += (r[k].x + r[k].y) + (r[k].z + r[k].w);
+        sq += (r[k].x * r[k].x + r[k].y * r[k].y) + (r[k].z * r[k].z + r[k].w * r[k].w);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        sq += __shfl_xor(sq, o);
+        sr += __shfl_xor(sr, o);
+        amax = fmaxf(amax, __shfl_xor(amax, o));
+    }
+    q2 += sq;
+    if (bs.qs < 0) return;
+    const float s1 = amax / 127.0f, inv1 = amax > 0.0f ? 127.0f / amax : 0.0f;
+    const float s2 = s1 / 254.0f, inv2 = amax > 0.0f ? 254.0f * 127.0f / amax : 0.0f;
+    int sum1 = 0, sum2 = 0;
+    int8_t *row1 = a.qq1 + g * bv.kq + bs.kq_off;
+    int8_t *row2 = a.qs == 2 ? a.qq2 + g * bv.kq + bs.kq_off : nullptr;
+    auto q8 = [](float x) { return (int)fminf(127.0f, fmaxf(-127.0f, rintf(x))); };
+#pragma unroll
+    for (int k = 0; k < NK; ++k) {
+        const int d = 4 * lane + 256 * k;
+        if (d >= bs.dim) continue;
+        const float e[4] = {r[k].x, r[k].y, r[k].z, r[k].w};
+        int v1[4], v2[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            v1[u] = q8(e[u] * inv1);
+            sum1 += v1[u];
+            v2[u] = q8((e[u] - (float)v1[u] * s1) * inv2);
+        }
+        *reinterpret_cast<uint32_t *>(row1 + d) =
+            (uint32_t)(uint8_t)v1[0] | ((uint32_t)(uint8_t)v1[1] << 8) | ((uint32_t)(uint8_t)v1[2] << 16) |
+            ((uint32_t)(uint8_t)v1[3] << 24);
+        if (row2) {
+            sum2 += (v2[0] + v2[1]) + (v2[2] + v2[3]);
+            *reinterpret_cast<uint32_t *>(row2 + d) =
+                (uint32_t)(uint8_t)v2[0] | ((uint32_t)(uint8_t)v2[1] << 8) | ((uint32_t)(uint8_t)v2[2] << 16) |
+                ((uint32_t)(uint8_t)v2[3] << 24);
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        sum1 += __shfl_xor(sum1, o);
+        sum2 += __shfl_xor(sum2, o);
+    }
+    if (lane == 0) {
+        float *pc = a.pc + (g * bv.nqs + bs.qs) * 3;
+        pc[0] = bs.sqd * s1;
+        pc[1] = bs.sqd * s2;
+        pc[2] = (-1.0f + bs.sqd / 2) * sr;
+        int32_t *cc = a.pcorr + (g * bv.nqs + bs.qs) * 2;
+        cc[0] = bs.bits >= 8 ? 128 * sum1 : 0;
+        cc[1] = bs.bits >= 8 ? 128 * sum2 : 0;
+    }
+}
+
+// One wave per grouped pair; segments up to 2048 dimensions are held in registers (a segment
+// dimension and offset are multiples of 64, rows are 16-byte aligned).
 __global__ __launch_bounds__(256) void bq_pairprep_kernel(PairPrepArgs a) {
     const int lane = threadIdx.x & 63;
     const int64_t g = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -460,45 +557,14 @@ __global__ __launch_bounds__(256) void bq_pairprep_kernel(PairPrepArgs a) {
     float q2 = 0.0f;
     for (int s = 0; s < bv.nseg; ++s) {
         const BSeg bs = bv.segs[s];
-        float amax = 0.0f, sr = 0.0f, sq = 0.0f;
-        for (int d = lane; d < bs.dim; d += 64) {
-            const float r = qr[bs.doff + d] - cr[bs.doff + d];
-            amax = fmaxf(amax, fabsf(r));
-            sr += r;
-            sq += r * r;
-        }
-        q2 += wave_sum(sq);
-        if (bs.qs < 0) continue;
-        amax = wave_max(amax);
-        sr = wave_sum(sr);
-        const float s1 = amax / 127.0f, inv1 = amax > 0.0f ? 127.0f / amax : 0.0f;
-        const float s2 = s1 / 254.0f, inv2 = amax > 0.0f ? 254.0f * 127.0f / amax : 0.0f;
-        int sum1 = 0, sum2 = 0;
-        int8_t *row1 = a.qq1 + g * bv.kq + bs.kq_off;
-        int8_t *row2 = a.qs == 2 ? a.qq2 + g * bv.kq + bs.kq_off : nullptr;
-        for (int d = lane; d < bs.dim; d += 64) {
-            const float r = qr[bs.doff + d] - cr[bs.doff + d];
-            const int v1 = (int)fminf(127.0f, fmaxf(-127.0f, rintf(r * inv1)));
-            row1[d] = (int8_t)v1;
-            sum1 += v1;
-            if (row2) {
-                const float rem = r - (float)v1 * s1;
-                const int v2 = (int)fminf(127.0f, fmaxf(-127.0f, rintf(rem * inv2)));
-                row2[d] = (int8_t)v2;
-                sum2 += v2;
-            }
-        }
-        sum1 = wave_isum(sum1);
-        sum2 = wave_isum(sum2);
-        if (lane == 0) {
-            float *pc = a.pc + (g * bv.nqs + bs.qs) * 3;
-            pc[0] = bs.sqd * s1;
-            pc[1] = bs.sqd * s2;
-            pc[2] = (-1.0f + bs.sqd / 2) * sr;
-            int32_t *cc = a.pcorr + (g * bv.nqs + bs.qs) * 2;
-            cc[0] = bs.bits >= 8 ? 128 * sum1 : 0;
-            cc[1] = bs.bits >= 8 ? 128 * sum2 : 0;
-        }
+        if (bs.dim <= 256)
+            pairprep_segment<1>(a, bs, g, qr, cr, q2);
+        else if (bs.dim <= 512)
+            pairprep_segment<2>(a, bs, g, qr, cr, q2);
+        else if (bs.dim <= 1024)
+            pairprep_segment<4>(a, bs, g, qr, cr, q2);
+        else
+            pairprep_segment<8>(a, bs, g, qr, cr, q2);
     }
     if (lane == 0) a.pq2[g] = q2;
 }
@@ -959,7 +1025,7 @@ __global__ __launch_bounds__(kBThreads, (QTM == 4 && QS == 2) ? 1 : 2) void bsca
 // nothing that does).
 template <int KR>
 __global__ __launch_bounds__(256) void bq_merge_kernel(const float *cand_d, const uint32_t *cand_id,
-                                                      const uint32_t *probes, const int32_t *slot,
+                                                      const uint32_t *probes, const int32_t *slot_of,
                                                       const int32_t *counts, const int32_t *candstart,
                                                       const int32_t *nsplit, const int32_t *npq, int64_t nq,
                                                       int NP, int topk, uint32_t *out_ids, float *out_d) {
@@ -976,6 +1042,57 @@ __global__ __launch_bounds__(256) void bq_merge_kernel(const float *cand_d, cons
     }
     tk.refresh();
     const int lim = npq ? npq[q] : NP;
+    if (KR == 1) {
+        // k <= 64: the lists whose head beats the k-th pair, merged g = 64 / k at a time (lane l
+        // takes element l % k of the (l / k)-th of them) by one bitonic merge each
+        float d = INFINITY;
+        uint32_t id = 0xFFFFFFFFu;
+        float kd = INFINITY;
+        uint32_t ki = 0xFFFFFFFFu;
+        const int g = 64 / topk, slot = lane / topk, e = lane % topk;
+        for (int p0 = 0; p0 < lim; p0 += 64) {
+            const int p = p0 + lane;
+            int ns = 0, cn = 0;
+            int64_t base = 0;
+            if (p < lim) {
+                const int64_t i = q * NP + p;
+                const int c = (int)probes[i];
+                ns = nsplit[c];
+                cn = counts[c];
+                base = (int64_t)candstart[c] + slot_of[i];
+            }
+            int rmax = ns;
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) rmax = max(rmax, __shfl_xor(rmax, o));
+            for (int r = 0; r < rmax; ++r) {
+                const int64_t li = base + (int64_t)r * cn;  // this lane's r-th list
+                const bool has = r < ns;
+                const float hd = has ? cand_d[li * topk] : INFINITY;
+                const uint32_t hi = has ? cand_id[li * topk] : 0xFFFFFFFFu;
+                unsigned long long m = __ballot(has && dless(hd, hi, kd, ki));
+                while (m) {
+                    // the slot-th set lane of m holds this lane's list
+                    unsigned long long mm = m;
+                    for (int j = 0; j < slot && mm; ++j) mm &= mm - 1;
+                    const int src = mm && slot < g ? __builtin_ctzll(mm) : -1;
+                    for (int j = 0; j < g && m; ++j) m &= m - 1;
+                    const int64_t L = __shfl(li, src < 0 ? 0 : src);
+                    const float x = src >= 0 ? cand_d[L * topk + e] : INFINITY;
+                    const uint32_t xi = src >= 0 ? cand_id[L * topk + e] : 0xFFFFFFFFu;
+                    merge_into64(d, id, x, xi);
+                    kd = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(d), topk - 1));
+                    ki = (uint32_t)__builtin_amdgcn_readlane((int)id, topk - 1);
+                    m &= __ballot(has && dless(hd, hi, kd, ki));
+                }
+            }
+        }
+        if (lane < topk) {
+            const bool ok = id != 0xFFFFFFFFu || d != INFINITY;
+            out_ids[q * topk + lane] = ok ? id : 0u;
+            if (out_d) out_d[q * topk + lane] = ok ? d : 0.0f;
+        }
+        return;
+    }
     for (int p0 = 0; p0 < lim; p0 += 64) {
         const int p = p0 + lane;
         int ns = 0, cn = 0;
@@ -985,7 +1102,7 @@ __global__ __launch_bounds__(256) void bq_merge_kernel(const float *cand_d, cons
             const int c = (int)probes[i];
             ns = nsplit[c];
             cn = counts[c];
-            base = (int64_t)candstart[c] + slot[i];
+            base = (int64_t)candstart[c] + slot_of[i];
         }
         int rmax = ns;
 #pragma unroll

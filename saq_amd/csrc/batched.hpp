@@ -11,6 +11,9 @@ constexpr int kBThreads = 64 * kBWaves;
 // work / kSplitWaves (the resident wavefronts of a 256-CU chip at 2 waves per SIMD).
 constexpr int kMinSplit = 512;
 constexpr int kSplitWaves = 2048;
+// Lists the grouping scan holds in registers: 1024 threads x kScanRounds each (the nlist limit of the
+// batched scan).
+constexpr int kScanRounds = 16;
 
 struct BSeg {
     int32_t dim, bits, doff;

@@ -9,8 +9,8 @@
 //                        ascending (descending similarity for IP).  Equal distances are ordered
 //                        by id; std::partial_sort leaves the order of exact ties unspecified.
 //                        Its kIP form reads q.c products of a library GEMM (the batched scan's
-//                        approximate coarse quantiser) and forms |q|^2 + |c|^2 - 2 q.c itself.
-//   sqnorm_rows_kernel   |x|^2 per row for that form.
+//                        approximate coarse quantiser) and ranks by |c|^2 - 2 q.c.
+//   sqnorm_rows_kernel   |x|^2 per row (the centroid norms of that form).
 #include <cfloat>
 
 #include "common.hpp"
@@ -244,12 +244,11 @@ __device__ __forceinline__ int block_scan256(int v, int *wsum) {
 // selected keys are then ordered: by rank counting for up to 256 of them, else by a bitonic
 // sort in LDS.
 //
-// kIP: dist holds inner products q.c and the distance is formed on the fly, (|q|^2 + |c|^2) -
-// 2 q.c for L2 (qnorm, cnorm) or q.c itself for IP.
+// kIP: dist holds inner products q.c and the key is formed on the fly: |c|^2 - 2 q.c for L2
+// (|q|^2 is the same for every centroid of a query and is left out), q.c itself for IP.
 template <int DT, bool kIP = false>
 __global__ __launch_bounds__(256) void probe_select_kernel(const float *__restrict__ dist, int K, int nprobe,
                                                            uint32_t *__restrict__ probes,
-                                                           const float *__restrict__ qnorm = nullptr,
                                                            const float *__restrict__ cnorm = nullptr) {
     __shared__ unsigned int hist[256];
     __shared__ int wsum[4];
@@ -259,10 +258,9 @@ __global__ __launch_bounds__(256) void probe_select_kernel(const float *__restri
     const int64_t q = blockIdx.x;
     const float *dq = dist + q * K;
     const int tid = threadIdx.x;
-    const float qn = kIP && DT == 0 ? qnorm[q] : 0.0f;
     auto key = [&](int i) {
         const float v = dq[i];
-        return sel_key<DT>(kIP && DT == 0 ? (qn + cnorm[i]) - 2.0f * v : v, i);
+        return sel_key<DT>(kIP && DT == 0 ? cnorm[i] - 2.0f * v : v, i);
     };
     unsigned long long pre = 0, thr = ~0ull;
     int r = nprobe;
@@ -341,18 +339,18 @@ hipError_t launch_sqnorm_rows(const float *x, int64_t rows, int D, float *out, h
     return hipGetLastError();
 }
 
-// Top-nprobe selection over inner products ip (nq x K, from a library GEMM): distances are
-// (qnorm + cnorm) - 2 ip for L2, ip for IP.
-hipError_t launch_probe_select_ip(const float *ip, const float *qnorm, const float *cnorm, int K, int64_t nq,
-                                  int nprobe, int dist_type, uint32_t *probes, hipStream_t st) {
+// Top-nprobe selection over inner products ip (nq x K, from a library GEMM): keys are
+// cnorm - 2 ip for L2 (cnorm = |c|^2), ip for IP.
+hipError_t launch_probe_select_ip(const float *ip, const float *cnorm, int K, int64_t nq, int nprobe, int dist_type,
+                                  uint32_t *probes, hipStream_t st) {
     if (nprobe > kSelMax) return hipErrorInvalidValue;
     if (nq == 0) return hipSuccess;
     if (dist_type == 1)
         hipLaunchKernelGGL((probe_select_kernel<1, true>), dim3((unsigned)nq), dim3(256), 0, st, ip, K, nprobe, probes,
-                           qnorm, cnorm);
+                           cnorm);
     else
         hipLaunchKernelGGL((probe_select_kernel<0, true>), dim3((unsigned)nq), dim3(256), 0, st, ip, K, nprobe, probes,
-                           qnorm, cnorm);
+                           cnorm);
     return hipGetLastError();
 }
 
@@ -380,14 +378,14 @@ hipError_t launch_probe(const float *centroids, int K, int D, const float *queri
         else
             hipLaunchKernelGGL(probe_dist_kernel<1>, g1, dim3(256), sm, st, centroids, K, D, queries, dist_ws);
         hipLaunchKernelGGL((probe_select_kernel<1, false>), dim3((unsigned)nq), dim3(256), 0, st, dist_ws, K, nprobe, probes,
-                           nullptr, nullptr);
+                           nullptr);
     } else {
         if (tiled)
             probe_tile<0>(vec, gt, st, centroids, K, D, queries, nq, dist_ws);
         else
             hipLaunchKernelGGL(probe_dist_kernel<0>, g1, dim3(256), sm, st, centroids, K, D, queries, dist_ws);
         hipLaunchKernelGGL((probe_select_kernel<0, false>), dim3((unsigned)nq), dim3(256), 0, st, dist_ws, K, nprobe, probes,
-                           nullptr, nullptr);
+                           nullptr);
     }
     return hipGetLastError();
 }
