@@ -407,9 +407,20 @@ struct WaveTopK {
 // places hold (INF, 0xFFFFFFFF)) absorbs up to 64 candidates at once: the candidates are
 // sorted by a bitonic network across the lanes, reversed and taken pairwise-smaller against
 // the list (the 64 smallest of both, as a bitonic sequence), then a bitonic merge sorts them.
-__device__ __forceinline__ void cswap_lane(float &d, uint32_t &id, int j, bool keep_small) {
-    const float pd = __shfl_xor(d, j);
-    const uint32_t pi = __shfl_xor(id, j);
+// The value of lane (lane ^ J): DPP quad permutations for J = 1, 2 (a VALU operand modifier),
+// ds_swizzle for J = 4, 8, 16 (no address operand), ds_bpermute for J = 32.
+template <int J>
+__device__ __forceinline__ int xor_lane(int v) {
+    if constexpr (J == 1) return __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
+    else if constexpr (J == 2) return __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
+    else if constexpr (J < 32) return __builtin_amdgcn_ds_swizzle(v, 0x1F | (J << 10));  // bit mode: lane ^ J
+    else return __shfl_xor(v, 32);
+}
+
+template <int J>
+__device__ __forceinline__ void cswap_lane(float &d, uint32_t &id, bool keep_small) {
+    const float pd = __int_as_float(xor_lane<J>(__float_as_int(d)));
+    const uint32_t pi = (uint32_t)xor_lane<J>((int)id);
     const bool other_less = dless(pd, pi, d, id);
     if (other_less == keep_small) {
         d = pd;
@@ -417,20 +428,36 @@ __device__ __forceinline__ void cswap_lane(float &d, uint32_t &id, int j, bool k
     }
 }
 
+// one stage (k) of the bitonic sort: steps j = k/2 .. 1
+template <int K, int J>
+__device__ __forceinline__ void bitonic_steps(float &d, uint32_t &id, int lane) {
+    if constexpr (J >= 1) {
+        cswap_lane<J>(d, id, ((lane & J) == 0) == ((lane & K) == 0 || K == 64));
+        bitonic_steps<K, J / 2>(d, id, lane);
+    }
+}
+
 __device__ __forceinline__ void merge_into64(float &d, uint32_t &id, float cd, uint32_t ci) {
     const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int k = 2; k <= 64; k <<= 1)
-#pragma unroll
-        for (int j = k >> 1; j > 0; j >>= 1) cswap_lane(cd, ci, j, ((lane & j) == 0) == ((lane & k) == 0 || k == 64));
+    bitonic_steps<2, 1>(cd, ci, lane);
+    bitonic_steps<4, 2>(cd, ci, lane);
+    bitonic_steps<8, 4>(cd, ci, lane);
+    bitonic_steps<16, 8>(cd, ci, lane);
+    bitonic_steps<32, 16>(cd, ci, lane);
+    bitonic_steps<64, 32>(cd, ci, lane);
     const float rd = __shfl(cd, 63 - lane);
     const uint32_t ri = __shfl(ci, 63 - lane);
     if (dless(rd, ri, d, id)) {
         d = rd;
         id = ri;
     }
-#pragma unroll
-    for (int j = 32; j > 0; j >>= 1) cswap_lane(d, id, j, (lane & j) == 0);
+    // the 64 smallest of both, a bitonic sequence: sort ascending
+    cswap_lane<32>(d, id, (lane & 32) == 0);
+    cswap_lane<16>(d, id, (lane & 16) == 0);
+    cswap_lane<8>(d, id, (lane & 8) == 0);
+    cswap_lane<4>(d, id, (lane & 4) == 0);
+    cswap_lane<2>(d, id, (lane & 2) == 0);
+    cswap_lane<1>(d, id, (lane & 1) == 0);
 }
 
 // ------------------------------------------------------------------ pair preparation
