@@ -452,6 +452,18 @@ def main():
                           "kernel": "sliced scan: list_scan_kernel + refine_kernel + replay_kernel per slice",
                           "alg_bytes_per_launch": int(alg_bytes), "kernel_ms": round(kern_ms, 4)}}
     log(f"plain: {qps:.0f} QPS, scan kernel {kern_ms:.3f} ms, {achieved:.0f} GB/s")
+    # the two forms of the exact coarse quantiser (GEMM filter, Eigen order for every pair) on the
+    # full batch: their lists must be equal
+    p_np = W["npq_range"][1] if W["npq_range"] else nprobe
+    prev = os.environ.pop("SAQGPU_PROBE", None)
+    pr_gemm = ix.probe(queries, p_np).cpu().numpy()
+    os.environ["SAQGPU_PROBE"] = "tile"
+    pr_tile = ix.probe(queries, p_np).cpu().numpy()
+    del os.environ["SAQGPU_PROBE"]
+    if prev is not None:
+        os.environ["SAQGPU_PROBE"] = prev
+    plain["probe_lists_differing_queries"] = int((pr_gemm != pr_tile).any(1).sum())
+    log(f"exact coarse quantiser, GEMM filter vs every pair: {plain['probe_lists_differing_queries']} queries differ")
 
     # ---- batched int8-MFMA scan at a recall no lower than the plain scan's
     torch.cuda.synchronize()

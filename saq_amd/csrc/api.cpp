@@ -38,6 +38,9 @@ hipError_t launch_estimate(const IndexView &ix, const float *qrot, const float *
                            float *out_fast, float *out_vars, hipStream_t st);
 hipError_t launch_probe_select_ip(const float *ip, const float *cnorm, int K, int64_t nq, int nprobe, int dist_type,
                                   uint32_t *probes, hipStream_t st);
+hipError_t launch_probe_exact(const float *ip, const float *cnorm, const float *centroids, int K, int D,
+                              const float *queries, int64_t nq, int nprobe, int dist_type, uint32_t *probes,
+                              hipStream_t st);
 hipError_t launch_encode_segment(const IndexView &ix, const float *data, const uint32_t *src, const int64_t *gvec,
                                  const int64_t *gblk, const int32_t *lane, const uint32_t *clu, const float *cent,
                                  const float *rot, int seg, int seg_dim, int64_t nslots, int64_t ld, int rd_lmt,
@@ -149,7 +152,9 @@ struct saqgpu_index {
     DevBuf<BSeg> d_bsegs;
     DevBuf<int32_t> bq_ws, bq_npq, bq_pcorr;
     DevBuf<int8_t> bq_qq;
-    DevBuf<float> bq_pc, bq_cnorm;
+    DevBuf<float> bq_pc;
+    DevBuf<float> cnorm;  // |c|^2 per centroid (GEMM-based coarse quantisers), computed once
+    bool cnorm_ready = false;
     DevBuf<float> bq_cand_d;
     DevBuf<uint32_t> bq_cand_id;
     // measurement
@@ -219,6 +224,55 @@ struct saqgpu_index {
         return L;
     }
 };
+
+namespace {
+// |c|^2 of every centroid, once per index (the centroids do not change).
+int ensure_cnorm(saqgpu_index *ix, hipStream_t st) {
+    if (ix->cnorm_ready) return SAQGPU_OK;
+    if (ix->cnorm.ensure(ix->num_cen)) return fail(SAQGPU_ENOMEM, "allocation failed");
+    HIPCHK(launch_sqnorm_rows(ix->centroids.p, ix->num_cen, ix->num_dim, ix->cnorm.p, st));
+    ix->cnorm_ready = true;
+    return SAQGPU_OK;
+}
+
+// q.c for every (query, centroid) into ws_cdist (nq x K) by rocBLAS's fp32 GEMM: in column-major
+// terms ip^T (K x nq, ld K) = centroids (D x K, ld D)^T * queries (D x nq, ld D).
+int ip_gemm(saqgpu_index *ix, const float *dq, int64_t nq, hipStream_t st) {
+    const int K = ix->num_cen, D = ix->num_dim;
+    if (!ix->blas && rocblas_create_handle(&ix->blas) != rocblas_status_success) {
+        ix->blas = nullptr;
+        return fail(SAQGPU_EDEVICE, "rocblas_create_handle failed");
+    }
+    if (rocblas_set_stream(ix->blas, st) != rocblas_status_success) return fail(SAQGPU_EDEVICE, "rocblas_set_stream failed");
+    const float one = 1.0f, zero = 0.0f;
+    if (rocblas_sgemm(ix->blas, rocblas_operation_transpose, rocblas_operation_none, K, (int)nq, D, &one,
+                      ix->centroids.p, D, dq, D, &zero, ix->ws_cdist.p, K) != rocblas_status_success)
+        return fail(SAQGPU_EDEVICE, "rocblas_sgemm failed");
+    return SAQGPU_OK;
+}
+
+// FlatInitializer::centroids_distances + partial_sort (initializer.hpp:52-76): the GEMM filter
+// (probe_exact_kernel: candidates within a proven fp32 error bound, their distances in Eigen's
+// order, an exact selection) when nprobe <= 1024 and a query fits in LDS, else the distances of
+// every pair in Eigen's order (probe_tile_kernel).  Both give the same lists.  ws_cdist holds
+// nq x num_cen floats.  SAQGPU_PROBE=tile selects the second form.
+int exact_probe(saqgpu_index *ix, const float *dq, int64_t nq, int nprobe, int dist_type, uint32_t *probes,
+                hipStream_t st) {
+    const char *e = getenv("SAQGPU_PROBE");
+    const bool tile = e && !strcmp(e, "tile");
+    if (!tile && nprobe <= 1024 && (size_t)ix->num_dim * 4 <= 48 * 1024) {
+        int rc;
+        if ((rc = ensure_cnorm(ix, st)) || (rc = ip_gemm(ix, dq, nq, st))) return rc;
+        HIPCHK(launch_probe_exact(ix->ws_cdist.p, ix->cnorm.p, ix->centroids.p, ix->num_cen, ix->num_dim, dq, nq,
+                                  nprobe, dist_type, probes, st));
+        return SAQGPU_OK;
+    }
+    HIPCHK(launch_probe(ix->centroids.p, ix->num_cen, ix->num_dim, dq, nq, nprobe, dist_type, ix->ws_cdist.p, probes,
+                        st));
+    return SAQGPU_OK;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -376,7 +430,7 @@ void saqgpu_index_destroy(saqgpu_index *ix) {
     ix->bq_pcorr.release();
     ix->bq_qq.release();
     ix->bq_pc.release();
-    ix->bq_cnorm.release();
+    ix->cnorm.release();
     ix->bq_npq.release();
     ix->bq_cand_d.release();
     ix->bq_cand_id.release();
@@ -601,7 +655,7 @@ int saqgpu_probe(saqgpu_index *ix, const float *queries, int64_t nq, int32_t npr
         dp = ix->ws_probes.p;
     }
     if (ix->ws_cdist.ensure((size_t)nq * ix->num_cen)) return fail(SAQGPU_ENOMEM, "workspace allocation failed");
-    HIPCHK(launch_probe(ix->centroids.p, ix->num_cen, ix->num_dim, dq, nq, nprobe, dist_type, ix->ws_cdist.p, dp, st));
+    if (int rc = exact_probe(ix, dq, nq, nprobe, dist_type, dp, st)) return rc;
     if (!dev) {
         HIPCHK(hipMemcpyAsync(probes, dp, nq * nprobe * 4, hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
@@ -850,8 +904,8 @@ int saqgpu_search(saqgpu_index *ix, const float *queries, int64_t nq, const uint
     }
     if (!probes) {
         if (ix->ws_cdist.ensure((size_t)nq * ix->num_cen)) return fail(SAQGPU_ENOMEM, "workspace allocation failed");
-        HIPCHK(launch_probe(ix->centroids.p, ix->num_cen, ix->num_dim, dq, nq, p->nprobe, p->dist_type,
-                            ix->ws_cdist.p, ix->ws_probes.p, st));
+        int rc;
+        if ((rc = exact_probe(ix, dq, nq, p->nprobe, p->dist_type, ix->ws_probes.p, st))) return rc;
         dp = ix->ws_probes.p;
     }
     HIPCHK(launch_query_prep(v, dq, nq, ix->variance.p, ix->has_rot ? ix->rot.p : nullptr, ix->d_rot_off.p,
@@ -926,8 +980,7 @@ int saqgpu_estimate(saqgpu_index *ix, const float *query, const uint32_t *probes
         std::copy(probes, probes + nprobe, pr.begin());
         HIPCHK(hipMemcpy(ix->ws_probes.p, probes, nprobe * 4, hipMemcpyHostToDevice));
     } else {  // FlatInitializer::centroids_distances, as IVF::estimate does (ivf.hpp:321-322)
-        HIPCHK(launch_probe(ix->centroids.p, ix->num_cen, ix->num_dim, ix->ws_q.p, 1, nprobe, dist_type, ix->ws_cdist.p,
-                            ix->ws_probes.p, 0));
+        if ((rc = exact_probe(ix, ix->ws_q.p, 1, nprobe, dist_type, ix->ws_probes.p, 0))) return rc;
         HIPCHK(hipMemcpy(pr.data(), ix->ws_probes.p, nprobe * 4, hipMemcpyDeviceToHost));
     }
     std::vector<int64_t> offs(nprobe);
@@ -1187,8 +1240,7 @@ int prepare_batched_locked(saqgpu_index *ix) {
     if ((rc = upload(d_mb_clu, mb_clu)) || (rc = upload(d_cs, chunk_seg)) || (rc = upload(d_ck, chunk_kc))) return rc;
     HIPCHK(launch_bq_build(ix->view(), batched_view(ix), d_mb_clu.p, tot, d_cs.p, d_ck.p, (int)chunk_seg.size(),
                            ix->bq_mb.p, 0));
-    if (ix->bq_cnorm.ensure(ix->num_cen)) return fail(SAQGPU_ENOMEM, "allocation failed");
-    HIPCHK(launch_sqnorm_rows(ix->centroids.p, ix->num_cen, ix->num_dim, ix->bq_cnorm.p, 0));
+    if ((rc = ensure_cnorm(ix, 0))) return rc;
     HIPCHK(hipDeviceSynchronize());
     d_mb_clu.release();
     d_cs.release();
@@ -1222,18 +1274,9 @@ int default_query_tile(const saqgpu_index *ix, int64_t nq, int nprobe) {
 // where two centroid distances lie within fp32 rounding of each other.
 int probe_gemm(saqgpu_index *ix, const float *dq, int64_t nq, int nprobe, int dist_type, uint32_t *probes,
                hipStream_t st) {
-    const int K = ix->num_cen, D = ix->num_dim;
-    if (!ix->blas && rocblas_create_handle(&ix->blas) != rocblas_status_success) {
-        ix->blas = nullptr;
-        return fail(SAQGPU_EDEVICE, "rocblas_create_handle failed");
-    }
-    if (rocblas_set_stream(ix->blas, st) != rocblas_status_success) return fail(SAQGPU_EDEVICE, "rocblas_set_stream failed");
-    // column-major view: ip^T (K x nq, ld K) = centroids (D x K, ld D)^T * queries (D x nq, ld D)
-    const float one = 1.0f, zero = 0.0f;
-    if (rocblas_sgemm(ix->blas, rocblas_operation_transpose, rocblas_operation_none, K, (int)nq, D, &one,
-                      ix->centroids.p, D, dq, D, &zero, ix->ws_cdist.p, K) != rocblas_status_success)
-        return fail(SAQGPU_EDEVICE, "rocblas_sgemm failed");
-    HIPCHK(launch_probe_select_ip(ix->ws_cdist.p, ix->bq_cnorm.p, K, nq, nprobe, dist_type, probes, st));
+    int rc;
+    if ((rc = ensure_cnorm(ix, st)) || (rc = ip_gemm(ix, dq, nq, st))) return rc;
+    HIPCHK(launch_probe_select_ip(ix->ws_cdist.p, ix->cnorm.p, ix->num_cen, nq, nprobe, dist_type, probes, st));
     return SAQGPU_OK;
 }
 
@@ -1295,7 +1338,9 @@ int search_batched_locked(saqgpu_index *ix, const float *queries, int64_t nq, co
         if (ix->ws_cdist.ensure((size_t)nq * K))
             return fail(SAQGPU_ENOMEM, "workspace allocation failed");
         if (p->exact_probe)
-            HIPCHK(launch_probe(ix->centroids.p, K, ix->num_dim, dq, nq, NP, 0, ix->ws_cdist.p, ix->ws_probes.p, st));
+        {
+            if ((rc = exact_probe(ix, dq, nq, NP, 0, ix->ws_probes.p, st))) return rc;
+        }
         else if ((rc = probe_gemm(ix, dq, nq, NP, 0, ix->ws_probes.p, st)))
             return rc;
         dp = ix->ws_probes.p;

@@ -354,6 +354,230 @@ hipError_t launch_probe_select_ip(const float *ip, const float *cnorm, int K, in
     return hipGetLastError();
 }
 
+// ---- exact coarse quantiser through a GEMM filter
+// One workgroup per query.  key_c = |c|^2 - 2 q.c (L2) or -q.c (IP) from the fp32 GEMM products;
+// the reference's distance d_c (Eigen order, fp32) differs from key_c + |q|^2 (L2) or -key_c (IP)
+// by at most E_c = 2 (D + 2) u (|q| + |c|)^2 (u = 2^-24: the forward error bounds of the two
+// fp32 sums, the squared differences and the final additions), used here with a factor 4 of
+// margin.  If T is the nprobe-th smallest key and E the largest E_c, every centroid of the
+// reference's top nprobe has key <= T + 2E: its distance is at most the nprobe-th smallest
+// distance, which is at most the largest distance among the nprobe smallest keys.  Those
+// candidates get the reference's distance (the same Eigen-order sum as probe_dist_kernel) and an
+// exact (distance, id) selection; more than kExactCand candidates fall back to all K.
+constexpr int kExactCand = 1024;
+
+__device__ __forceinline__ float block_max256(float v, float *wtmp) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    if (lane == 0) wtmp[w] = v;
+    __syncthreads();
+    v = fmaxf(fmaxf(wtmp[0], wtmp[1]), fmaxf(wtmp[2], wtmp[3]));
+    __syncthreads();
+    return v;
+}
+
+template <int DT>
+__global__ __launch_bounds__(256) void probe_exact_kernel(const float *__restrict__ ip, const float *__restrict__ cnorm,
+                                                          const float *__restrict__ cent, int K, int D,
+                                                          const float *__restrict__ queries, int nprobe,
+                                                          uint32_t *__restrict__ probes) {
+    extern __shared__ float qsh[];  // D floats
+    __shared__ unsigned int hist[256];
+    __shared__ int wsum[4];
+    __shared__ float wtmp[4];
+    __shared__ unsigned long long prefix_s;
+    __shared__ int rank_s, done_s, cnt_s;
+    __shared__ unsigned long long sel[kExactCand];
+    const int64_t q = blockIdx.x;
+    const int tid = threadIdx.x;
+    // |q|^2 summed in any order; its rounding, like that of |c|, is far inside the margin (the
+    // norms are inflated by 1e-4 relative)
+    float qn = 0.0f;
+    for (int i = tid; i < D; i += 256) {
+        const float x = queries[q * D + i];
+        qsh[i] = x;
+        qn += x * x;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) qn += __shfl_xor(qn, o);
+    if ((tid & 63) == 0) wtmp[tid >> 6] = qn;
+    __syncthreads();
+    qn = (wtmp[0] + wtmp[1]) + (wtmp[2] + wtmp[3]);
+    __syncthreads();
+    const float qa = sqrtf(qn) * 1.0001f;
+    const float scale = 8.0f * (float)(D + 2) * 5.9604645e-8f;  // 4 x 2 (D + 2) u
+    const float *iq = ip + q * K;
+    auto keyf = [&](int c) { return DT == 1 ? -iq[c] : cnorm[c] - 2.0f * iq[c]; };
+    float emax = 0.0f;
+    for (int c = tid; c < K; c += 256) {
+        const float cc = sqrtf(fmaxf(cnorm[c], 0.0f)) * 1.0001f;
+        emax = fmaxf(emax, scale * (qa + cc) * (qa + cc));
+    }
+    emax = block_max256(emax, wtmp);
+    // radix select of the nprobe-th smallest (key, id) (as probe_select_kernel)
+    unsigned long long pre = 0, thr = ~0ull;
+    int r = nprobe;
+    for (int pass = 0; pass < 8; ++pass) {
+        const int shift = 56 - 8 * pass;
+        hist[tid] = 0;
+        __syncthreads();
+        const unsigned long long hmask = pass == 0 ? 0ull : (~0ull << (shift + 8));
+        for (int c = tid; c < K; c += 256) {
+            const unsigned long long k = ((unsigned long long)float_key(keyf(c)) << 32) | (uint32_t)c;
+            if ((k & hmask) == pre) atomicAdd(&hist[(k >> shift) & 0xFF], 1u);
+        }
+        __syncthreads();
+        const int h = (int)hist[tid];
+        const int incl = block_scan256(h, wsum);
+        const int excl = incl - h;
+        if (excl < r && r <= incl) {
+            prefix_s = pre | ((unsigned long long)tid << shift);
+            rank_s = r - excl;
+            done_s = (r - excl) == h;
+        }
+        __syncthreads();
+        pre = prefix_s;
+        r = rank_s;
+        if (done_s || pass == 7) {
+            thr = pass == 7 ? pre : (pre | ((1ull << shift) - 1));
+            break;
+        }
+    }
+    // T = the nprobe-th smallest key: the largest key <= thr
+    float tkey = -INFINITY;
+    for (int c = tid; c < K; c += 256) {
+        const float kf = keyf(c);
+        const unsigned long long k = ((unsigned long long)float_key(kf) << 32) | (uint32_t)c;
+        if (k <= thr) tkey = fmaxf(tkey, kf);
+    }
+    tkey = block_max256(tkey, wtmp);
+    const float lim = tkey + 2.0f * emax;
+    if (tid == 0) cnt_s = 0;
+    __syncthreads();
+    for (int c = tid; c < K; c += 256)
+        if (keyf(c) <= lim) {
+            const int p = atomicAdd(&cnt_s, 1);
+            if (p < kExactCand) sel[p] = (uint32_t)c;
+        }
+    __syncthreads();
+    const bool all = cnt_s > kExactCand;  // too many candidates: every centroid
+    const int n = all ? K : cnt_s;
+    auto exact = [&](int c) {
+        const float *row = cent + (int64_t)c * D;
+        float d;
+        if (DT == 1)
+            d = eigen_redux_thread(D, [&](int i) { return qsh[i] * row[i]; });
+        else
+            d = eigen_redux_thread(D, [&](int i) {
+                const float t = row[i] - qsh[i];
+                return t * t;
+            });
+        return sel_key<DT>(d, c);
+    };
+    if (!all) {
+        for (int j = tid; j < n; j += 256) sel[j] = exact((int)sel[j]);
+        __syncthreads();
+        int P = 1;
+        while (P < n) P <<= 1;
+        for (int i = n + tid; i < P; i += 256) sel[i] = ~0ull;
+        __syncthreads();
+        for (int k = 2; k <= P; k <<= 1)
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                for (int i = tid; i < P; i += 256) {
+                    const int ixj = i ^ j;
+                    if (ixj > i) {
+                        const unsigned long long x = sel[i], y = sel[ixj];
+                        if ((x > y) == ((i & k) == 0)) {
+                            sel[i] = y;
+                            sel[ixj] = x;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        for (int i = tid; i < nprobe; i += 256) probes[q * nprobe + i] = (uint32_t)(sel[i] & 0xFFFFFFFFu);
+        return;
+    }
+    // fallback: the nprobe smallest exact keys over all K by a second radix select
+    pre = 0;
+    thr = ~0ull;
+    r = nprobe;
+    for (int pass = 0; pass < 8; ++pass) {
+        const int shift = 56 - 8 * pass;
+        hist[tid] = 0;
+        __syncthreads();
+        const unsigned long long hmask = pass == 0 ? 0ull : (~0ull << (shift + 8));
+        for (int c = tid; c < K; c += 256) {
+            const unsigned long long k = exact(c);
+            if ((k & hmask) == pre) atomicAdd(&hist[(k >> shift) & 0xFF], 1u);
+        }
+        __syncthreads();
+        const int h = (int)hist[tid];
+        const int incl = block_scan256(h, wsum);
+        const int excl = incl - h;
+        if (excl < r && r <= incl) {
+            prefix_s = pre | ((unsigned long long)tid << shift);
+            rank_s = r - excl;
+            done_s = (r - excl) == h;
+        }
+        __syncthreads();
+        pre = prefix_s;
+        r = rank_s;
+        if (done_s || pass == 7) {
+            thr = pass == 7 ? pre : (pre | ((1ull << shift) - 1));
+            break;
+        }
+    }
+    if (tid == 0) cnt_s = 0;
+    __syncthreads();
+    for (int c = tid; c < K; c += 256) {
+        const unsigned long long k = exact(c);
+        if (k <= thr) {
+            const int p = atomicAdd(&cnt_s, 1);
+            if (p < kExactCand) sel[p] = k;
+        }
+    }
+    __syncthreads();
+    const int m = min(cnt_s, kExactCand);  // == nprobe <= kExactCand
+    int P = 1;
+    while (P < m) P <<= 1;
+    for (int i = m + tid; i < P; i += 256) sel[i] = ~0ull;
+    __syncthreads();
+    for (int k = 2; k <= P; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = tid; i < P; i += 256) {
+                const int ixj = i ^ j;
+                if (ixj > i) {
+                    const unsigned long long x = sel[i], y = sel[ixj];
+                    if ((x > y) == ((i & k) == 0)) {
+                        sel[i] = y;
+                        sel[ixj] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    for (int i = tid; i < nprobe; i += 256) probes[q * nprobe + i] = (uint32_t)(sel[i] & 0xFFFFFFFFu);
+}
+
+// The exact coarse quantiser over GEMM products ip (nq x K) and centroid norms cnorm: probe
+// lists equal to probe_tile + probe_select's.  nprobe <= kExactCand.
+hipError_t launch_probe_exact(const float *ip, const float *cnorm, const float *centroids, int K, int D,
+                              const float *queries, int64_t nq, int nprobe, int dist_type, uint32_t *probes,
+                              hipStream_t st) {
+    if (nprobe > kExactCand || (size_t)D * 4 > 48 * 1024) return hipErrorInvalidValue;
+    if (nq == 0) return hipSuccess;
+    const size_t sm = (size_t)D * 4;
+    if (dist_type == 1)
+        hipLaunchKernelGGL(probe_exact_kernel<1>, dim3((unsigned)nq), dim3(256), sm, st, ip, cnorm, centroids, K, D,
+                           queries, nprobe, probes);
+    else
+        hipLaunchKernelGGL(probe_exact_kernel<0>, dim3((unsigned)nq), dim3(256), sm, st, ip, cnorm, centroids, K, D,
+                           queries, nprobe, probes);
+    return hipGetLastError();
+}
+
 template <int DT>
 static void probe_tile(bool vec, dim3 g, hipStream_t st, const float *centroids, int K, int D, const float *queries,
                        int64_t nq, float *dist_ws) {

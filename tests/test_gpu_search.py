@@ -145,11 +145,16 @@ def test_gpu_probe_matches_oracle():
     ix.close()
 
 
+@pytest.mark.parametrize("path", ["gemm", "tile"])
 @pytest.mark.parametrize("d", [32, 40, 48, 64, 100, 960])
 @pytest.mark.parametrize("dist_type", [0, 1])
-def test_gpu_probe_matches_oracle_across_dims(d, dist_type):
-    """The register-tiled probe (D >= 32) keeps Eigen's reduction order for every D: the full
-    32-sum streams, the trailing 16-element packet (D mod 32 >= 16) and the scalar tail."""
+def test_gpu_probe_matches_oracle_across_dims(d, dist_type, path, monkeypatch):
+    """Both exact coarse quantisers give the oracle's lists for every D: the GEMM filter
+    (candidates within the fp32 error bound, recomputed in Eigen's order) and the register-tiled
+    probe (Eigen's order for every pair: the full 32-sum streams, the trailing 16-element packet
+    (D mod 32 >= 16) and the scalar tail)."""
+    if path == "tile":
+        monkeypatch.setenv("SAQGPU_PROBE", "tile")
     from oracle_lib import oracle
     O = oracle()
     dp = (d + 63) // 64 * 64
@@ -172,10 +177,13 @@ def test_gpu_probe_matches_oracle_across_dims(d, dist_type):
     ix.close()
 
 
+@pytest.mark.parametrize("path", ["gemm", "tile"])
 @pytest.mark.parametrize("dist_type", [0, 1])
-def test_gpu_probe_exact_ties(dist_type):
+def test_gpu_probe_exact_ties(dist_type, path, monkeypatch):
     """Exact distance ties are ordered by centroid id: duplicated centroids tie in pairs, and a
     zero query ties every centroid under IP, so the radix select has to reach the id bytes."""
+    if path == "tile":
+        monkeypatch.setenv("SAQGPU_PROBE", "tile")
     from oracle_lib import oracle
     O = oracle()
     case = F.make_case(n=1500, d=96, K=300, nq=12, plan=[(128, 2)], seed=23)
