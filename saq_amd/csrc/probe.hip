@@ -59,6 +59,38 @@ __device__ __forceinline__ float eigen_redux_thread(int n, F elem) {
     return r;
 }
 
+// eigen_redux_thread's result computed by the 32 lanes of a half-wave (l = lane % 32), all
+// lanes returning it: lane l carries Eigen's running sum l (p0[l] for l < 16, p1[l - 16] above)
+// over the elements l, l + 32, ... below floor(n/32)*32, so a row is read in coalesced runs; the
+// combine p0 + p1, the trailing 16-element packet, the halving tree and the scalar tail follow
+// in Eigen's order.
+template <class F>
+__device__ __forceinline__ float eigen_redux_half(int n, F elem, int l) {
+    const int base = threadIdx.x & 32;
+    if (n < 16) {
+        float r = elem(0);
+        for (int i = 1; i < n; ++i) r = r + elem(i);
+        return r;
+    }
+    const int aligned = n / 16 * 16, aligned2 = n / 32 * 32;
+    float p0;
+    if (aligned > 16) {
+        float acc = elem(l);
+        for (int idx = 32; idx < aligned2; idx += 32) acc = acc + elem(idx + l);
+        const float p1 = __shfl(acc, base + ((l + 16) & 31));
+        p0 = acc + p1;
+        if (aligned > aligned2 && l < 16) p0 = p0 + elem(aligned2 + l);
+    } else {
+        p0 = l < 16 ? elem(l) : 0.0f;
+    }
+    const float b = p0 + __shfl(p0, base + ((l + 8) & 31));   // b[i] = p0[i] + p0[i + 8], i < 8
+    const float c = b + __shfl(b, base + ((l + 4) & 31));     // c[i] = b[i] + b[i + 4], i < 4
+    const float c0 = __shfl(c, base), c1 = __shfl(c, base + 1), c2 = __shfl(c, base + 2), c3 = __shfl(c, base + 3);
+    float r = (c0 + c2) + (c1 + c3);
+    for (int i = aligned; i < n; ++i) r = r + elem(i);
+    return r;
+}
+
 // grid (ceil(K/256), nq)
 template <int DT>
 __global__ __launch_bounds__(256) void probe_dist_kernel(const float *__restrict__ cent, int K, int D,
@@ -476,8 +508,24 @@ __global__ __launch_bounds__(256) void probe_exact_kernel(const float *__restric
         return sel_key<DT>(d, c);
     };
     if (!all) {
-        for (int j = tid; j < n; j += 256) sel[j] = exact((int)sel[j]);
-        __syncthreads();
+        // the candidates' distances, two per wavefront (one per half-wave) at a time
+        const int half = tid >> 5, l = tid & 31;
+        for (int j0 = 0; j0 < n; j0 += 8) {
+            const int j = j0 + half;
+            const int c = j < n ? (int)sel[j] : 0;
+            const float *row = cent + (int64_t)c * D;
+            float d;
+            if (DT == 1)
+                d = eigen_redux_half(D, [&](int i) { return qsh[i] * row[i]; }, l);
+            else
+                d = eigen_redux_half(D, [&](int i) {
+                    const float t = row[i] - qsh[i];
+                    return t * t;
+                }, l);
+            __syncthreads();  // every half-wave has read its candidate id
+            if (j < n && l == 0) sel[j] = sel_key<DT>(d, c);
+            __syncthreads();
+        }
         int P = 1;
         while (P < n) P <<= 1;
         for (int i = n + tid; i < P; i += 256) sel[i] = ~0ull;

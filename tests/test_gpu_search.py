@@ -206,6 +206,21 @@ def test_gpu_probe_exact_ties(dist_type, path, monkeypatch):
     ix.close()
 
 
+@pytest.mark.parametrize("dist_type", [0, 1])
+def test_gpu_probe_candidate_fallback(dist_type):
+    """1500 identical centroids: every key ties, so the GEMM filter's candidate set exceeds its
+    1024 places and the kernel recomputes all K in Eigen's order; ties are ordered by id."""
+    case = F.make_case(n=3000, d=64, K=1500, nq=6, plan=[(64, 2)], seed=29, encode=False)
+    case.centroids[:] = case.centroids[0]
+    from saq_amd import Index
+    ix = Index(case.num_dim, case.plan, case.centroids, np.zeros(case.K, np.int64), case.variance)
+    for nprobe in (1, 20, 1024):
+        got = ix.probe(case.queries, nprobe, dist_type=dist_type)
+        want = np.tile(np.arange(nprobe, dtype=np.uint32), (case.queries.shape[0], 1))
+        assert np.array_equal(want, got), f"nprobe={nprobe}"
+    ix.close()
+
+
 def test_cluster_layout_round_trip():
     for plan, compact in (([(256, 4)], False), ([(64, 1), (64, 5), (128, 3)], False), ([(128, 5), (128, 0)], True)):
         case = F.make_case(n=1500, d=256, K=6, nq=4, plan=plan, seed=19, compact=compact)
