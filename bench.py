@@ -456,6 +456,7 @@ def main():
     # full batch: their lists must be equal
     p_np = W["npq_range"][1] if W["npq_range"] else nprobe
     prev = os.environ.pop("SAQGPU_PROBE", None)
+    os.environ["SAQGPU_PROBE"] = "gemm"
     pr_gemm = ix.probe(queries, p_np).cpu().numpy()
     os.environ["SAQGPU_PROBE"] = "tile"
     pr_tile = ix.probe(queries, p_np).cpu().numpy()

@@ -253,14 +253,16 @@ int ip_gemm(saqgpu_index *ix, const float *dq, int64_t nq, hipStream_t st) {
 
 // FlatInitializer::centroids_distances + partial_sort (initializer.hpp:52-76): the GEMM filter
 // (probe_exact_kernel: candidates within a proven fp32 error bound, their distances in Eigen's
-// order, an exact selection) when nprobe <= 1024 and a query fits in LDS, else the distances of
-// every pair in Eigen's order (probe_tile_kernel).  Both give the same lists.  ws_cdist holds
-// nq x num_cen floats.  SAQGPU_PROBE=tile selects the second form.
+// order, an exact selection) when nprobe <= 64 and a query fits in LDS, else the distances of
+// every pair in Eigen's order (probe_tile_kernel).  Both give the same lists.  With a larger
+// nprobe the window around the nprobe-th key holds too many centroids for the filter to pay
+// (at 16384 lists and nprobe 256, often more than its 1024 places).  ws_cdist holds nq x num_cen
+// floats.  SAQGPU_PROBE=tile / =gemm forces a form (gemm: any nprobe <= 1024).
 int exact_probe(saqgpu_index *ix, const float *dq, int64_t nq, int nprobe, int dist_type, uint32_t *probes,
                 hipStream_t st) {
     const char *e = getenv("SAQGPU_PROBE");
-    const bool tile = e && !strcmp(e, "tile");
-    if (!tile && nprobe <= 1024 && (size_t)ix->num_dim * 4 <= 48 * 1024) {
+    const bool tile = e && !strcmp(e, "tile"), gemm = e && !strcmp(e, "gemm");
+    if (!tile && nprobe <= (gemm ? 1024 : 64) && (size_t)ix->num_dim * 4 <= 48 * 1024) {
         int rc;
         if ((rc = ensure_cnorm(ix, st)) || (rc = ip_gemm(ix, dq, nq, st))) return rc;
         HIPCHK(launch_probe_exact(ix->ws_cdist.p, ix->cnorm.p, ix->centroids.p, ix->num_cen, ix->num_dim, dq, nq,

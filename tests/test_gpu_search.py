@@ -153,8 +153,7 @@ def test_gpu_probe_matches_oracle_across_dims(d, dist_type, path, monkeypatch):
     (candidates within the fp32 error bound, recomputed in Eigen's order) and the register-tiled
     probe (Eigen's order for every pair: the full 32-sum streams, the trailing 16-element packet
     (D mod 32 >= 16) and the scalar tail)."""
-    if path == "tile":
-        monkeypatch.setenv("SAQGPU_PROBE", "tile")
+    monkeypatch.setenv("SAQGPU_PROBE", path)
     from oracle_lib import oracle
     O = oracle()
     dp = (d + 63) // 64 * 64
@@ -182,8 +181,7 @@ def test_gpu_probe_matches_oracle_across_dims(d, dist_type, path, monkeypatch):
 def test_gpu_probe_exact_ties(dist_type, path, monkeypatch):
     """Exact distance ties are ordered by centroid id: duplicated centroids tie in pairs, and a
     zero query ties every centroid under IP, so the radix select has to reach the id bytes."""
-    if path == "tile":
-        monkeypatch.setenv("SAQGPU_PROBE", "tile")
+    monkeypatch.setenv("SAQGPU_PROBE", path)
     from oracle_lib import oracle
     O = oracle()
     case = F.make_case(n=1500, d=96, K=300, nq=12, plan=[(128, 2)], seed=23)
@@ -207,9 +205,10 @@ def test_gpu_probe_exact_ties(dist_type, path, monkeypatch):
 
 
 @pytest.mark.parametrize("dist_type", [0, 1])
-def test_gpu_probe_candidate_fallback(dist_type):
+def test_gpu_probe_candidate_fallback(dist_type, monkeypatch):
     """1500 identical centroids: every key ties, so the GEMM filter's candidate set exceeds its
     1024 places and the kernel recomputes all K in Eigen's order; ties are ordered by id."""
+    monkeypatch.setenv("SAQGPU_PROBE", "gemm")
     case = F.make_case(n=3000, d=64, K=1500, nq=6, plan=[(64, 2)], seed=29, encode=False)
     case.centroids[:] = case.centroids[0]
     from saq_amd import Index
