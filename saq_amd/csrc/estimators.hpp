@@ -111,112 +111,6 @@ __device__ __forceinline__ float ex_ip_group(int k, const float *r, const uint8_
     return tree16_xor(lane_acc(k, r, y, D, gl));
 }
 
-// ---- one thread computes a whole CodeHelper<K>::compute_ip: its 16 accumulators are the 16
-// fp32 lanes of the reference's AVX-512 register (lane l takes dims l, l+16, ... in order, by
-// FMA, a masked add for K = 1), and the closing sum is the GCC reduce tree (lo256+hi256,
-// lo128+hi128, +swap64, lane0+lane1) -- the same value as ex_ip_group.  The codes (16-byte
-// aligned; 8K bytes of the compacted layout per 64 dims) and r (16-byte aligned) are in LDS.
-
-__device__ __forceinline__ uint32_t dw_byte(const uint32_t *w, int j) { return (w[j >> 2] >> (8 * (j & 3))) & 0xFFu; }
-
-// code of dim 16 u + l inside a 64-dim step of the compacted layout K (code_helper.hpp:95-591)
-template <int K>
-__device__ __forceinline__ int step_code(const uint32_t *w, int u, int l) {
-    if (K == 1) return (dw_byte(w, 2 * u + (l >> 3)) >> (l & 7)) & 1;
-    if (K == 2) return (dw_byte(w, l) >> (2 * u)) & 3;
-    if (K == 3) return ((dw_byte(w, l) >> (2 * u)) & 3) | (((dw_byte(w, 16 + (l & 7)) >> (2 * u + (l >> 3))) & 1) << 2);
-    if (K == 4) {
-        const uint32_t b = dw_byte(w, 16 * (u >> 1) + l);
-        return (u & 1) ? (b >> 4) : (b & 15);
-    }
-    if (K == 6 || K == 7) {
-        int code;
-        if (u == 0) code = dw_byte(w, l) & 0x3F;
-        else if (u == 1) code = dw_byte(w, 16 + l) & 0x3F;
-        else if (u == 2) code = ((dw_byte(w, l) >> 2) & 0x30) | (dw_byte(w, 32 + l) & 0x0F);
-        else code = ((dw_byte(w, 16 + l) >> 2) & 0x30) | ((dw_byte(w, 32 + l) >> 4) & 0x0F);
-        if (K == 7) code |= ((dw_byte(w, 48 + (l & 7)) >> (2 * u + (l >> 3))) & 1) << 6;
-        return code;
-    }
-    return dw_byte(w, 16 * u + l);  // K == 8
-}
-
-template <int K>
-__device__ __forceinline__ float ex_ip_thread_k(const float *r, const uint8_t *y, int D) {
-    float acc[16];
-#pragma unroll
-    for (int l = 0; l < 16; ++l) acc[l] = 0.0f;
-    for (int c = 0; c < D / 64; ++c) {
-        uint32_t w[2 * K];  // the step's 8K bytes: 16-byte reads for even K, 8-byte for odd
-        if (K % 2 == 0) {
-            const uint4 *src = reinterpret_cast<const uint4 *>(y + 8 * K * c);
-#pragma unroll
-            for (int j = 0; j < K / 2; ++j) {
-                const uint4 t = src[j];
-                w[4 * j] = t.x;
-                w[4 * j + 1] = t.y;
-                w[4 * j + 2] = t.z;
-                w[4 * j + 3] = t.w;
-            }
-        } else {
-            const uint2 *src = reinterpret_cast<const uint2 *>(y + 8 * K * c);
-#pragma unroll
-            for (int j = 0; j < K; ++j) {
-                const uint2 t = src[j];
-                w[2 * j] = t.x;
-                w[2 * j + 1] = t.y;
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const float4 *rv = reinterpret_cast<const float4 *>(r + 64 * c + 16 * u);
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const float4 t = rv[j];
-                const float x[4] = {t.x, t.y, t.z, t.w};
-#pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    const int l = 4 * j + e;
-                    const int cd = step_code<K>(w, u, l);
-                    acc[l] = K == 1 ? acc[l] + (cd ? x[e] : 0.0f) : __fmaf_rn(x[e], (float)cd, acc[l]);
-                }
-            }
-        }
-    }
-#pragma unroll
-    for (int l = 0; l < 8; ++l) acc[l] = acc[l] + acc[l + 8];
-#pragma unroll
-    for (int l = 0; l < 4; ++l) acc[l] = acc[l] + acc[l + 4];
-#pragma unroll
-    for (int l = 0; l < 2; ++l) acc[l] = acc[l] + acc[l + 2];
-    return acc[0] + acc[1];
-}
-
-__device__ __forceinline__ float ex_ip_thread_base(int k, const float *r, const uint8_t *y, int D) {
-    switch (k) {
-    case 1: return ex_ip_thread_k<1>(r, y, D);
-    case 2: return ex_ip_thread_k<2>(r, y, D);
-    case 3: return ex_ip_thread_k<3>(r, y, D);
-    case 4: return ex_ip_thread_k<4>(r, y, D);
-    case 6: return ex_ip_thread_k<6>(r, y, D);
-    case 7: return ex_ip_thread_k<7>(r, y, D);
-    case 8: return ex_ip_thread_k<8>(r, y, D);
-    default: return 0.0f;
-    }
-}
-
-// CodeHelper<k>::compute_ip for k = 1..12 by one thread (the value ex_ip_group returns).
-__device__ __forceinline__ float ex_ip_thread(int k, const float *r, const uint8_t *y, int D) {
-    if (k == 5) return 2 * ex_ip_thread_base(4, r, y, D) + ex_ip_thread_base(1, r, y + D * 4 / 8, D);
-    if (k > 8) {
-        const float lo = ex_ip_thread_base(8, r, y, D);
-        const float hi = k - 8 == 5 ? 2 * ex_ip_thread_base(4, r, y + D, D) + ex_ip_thread_base(1, r, y + D + D * 4 / 8, D)
-                                    : ex_ip_thread_base(k - 8, r, y + D, D);
-        return lo + 256 * hi;
-    }
-    return ex_ip_thread_base(k, r, y, D);
-}
-
 constexpr int kPrepThreads = 256;  // the workgroup size prepare_list is written for
 
 // CaqCluEstimator::prepare of every segment for one (query, list) (caq_estimator.hpp:94-106,
