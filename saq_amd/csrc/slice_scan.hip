@@ -214,27 +214,36 @@ __global__ __launch_bounds__(kLT) void list_scan_kernel(SliceArgs a) {
             const uint8_t *codes = rec + ix.fac_floats * 4 + ln * 8;
             uint32_t acc = 0;
             int bk = 0;
-            for (int g0 = 0; g0 < NG; g0 += 8) {
-                uint2 w[8];
+            // the code words of 8 groups at a time, the next 8 loading while these are summed
+            // (indices past NG repeat the last group and are not summed)
+            uint2 w[8], wn[8];
 #pragma unroll
-                for (int u = 0; u < 8; ++u)
-                    if (g0 + u < NG) w[u] = *reinterpret_cast<const uint2 *>(codes + (g0 + u) * 256);
+            for (int u = 0; u < 8; ++u) w[u] = *reinterpret_cast<const uint2 *>(codes + min(u, NG - 1) * 256);
+            for (int g0 = 0; g0 < NG; g0 += 8) {
+#pragma unroll
+                for (int u = 0; u < 8; ++u)  // unconditional, so the waits below count only w's loads
+                    wn[u] = *reinterpret_cast<const uint2 *>(codes + min(g0 + 8 + u, NG - 1) * 256);
 #pragma unroll
                 for (int u = 0; u < 8; ++u) {
                     const int g = g0 + u;
                     if (g >= NG) break;
-                    const uint16_t *t = m.lut + g * 256;
+                    // byte b of xh / xl is twice the high / low nibble of code byte b: one byte
+                    // extract and add per LUT address (the integer sum is order-free)
+                    const uint8_t *t = reinterpret_cast<const uint8_t *>(m.lut + g * 256);
+                    uint32_t xh = (w[u].x >> 3) & 0x1E1E1E1Eu, xl = (w[u].x << 1) & 0x1E1E1E1Eu;
+                    uint32_t yh = (w[u].y >> 3) & 0x1E1E1E1Eu, yl = (w[u].y << 1) & 0x1E1E1E1Eu;
+                    // keep the four masked words as they are (no instruction): otherwise the mask is
+                    // folded back into a shift and an and per nibble
+                    asm("" : "+v"(xh), "+v"(xl), "+v"(yh), "+v"(yl));
 #pragma unroll
                     for (int b = 0; b < 4; ++b) {
-                        const uint32_t byte = (w[u].x >> (8 * b)) & 0xFF;
-                        acc += t[(2 * b) * 16 + (byte >> 4)];
-                        acc += t[(2 * b + 1) * 16 + (byte & 15)];
+                        acc += *reinterpret_cast<const uint16_t *>(t + (2 * b) * 32 + __builtin_amdgcn_ubfe(xh, 8 * b, 8));
+                        acc += *reinterpret_cast<const uint16_t *>(t + (2 * b + 1) * 32 + __builtin_amdgcn_ubfe(xl, 8 * b, 8));
                     }
 #pragma unroll
                     for (int b = 0; b < 4; ++b) {
-                        const uint32_t byte = (w[u].y >> (8 * b)) & 0xFF;
-                        acc += t[(2 * b + 8) * 16 + (byte >> 4)];
-                        acc += t[(2 * b + 9) * 16 + (byte & 15)];
+                        acc += *reinterpret_cast<const uint16_t *>(t + (2 * b + 8) * 32 + __builtin_amdgcn_ubfe(yh, 8 * b, 8));
+                        acc += *reinterpret_cast<const uint16_t *>(t + (2 * b + 9) * 32 + __builtin_amdgcn_ubfe(yl, 8 * b, 8));
                     }
                     if ((endmask >> g) & 1u) {
                         const int s = m.bseq[bk++];
@@ -245,6 +254,8 @@ __global__ __launch_bounds__(kLT) void list_scan_kernel(SliceArgs a) {
                         acc = 0;
                     }
                 }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) w[u] = wn[u];
             }
         }
         // ---- per-vector estimates, block minima and the candidate test against T
