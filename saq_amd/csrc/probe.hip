@@ -10,7 +10,10 @@
 //                        by id; std::partial_sort leaves the order of exact ties unspecified.
 //                        Its kIP form reads q.c products of a library GEMM (the batched scan's
 //                        approximate coarse quantiser) and ranks by |c|^2 - 2 q.c.
-//   sqnorm_rows_kernel   |x|^2 per row (the centroid norms of that form).
+//   probe_exact_kernel   the exact lists from a library GEMM's q.c: the centroids within a proven
+//                        fp32 error bound of the nprobe-th key, recomputed in Eigen's order
+//                        (by half-waves), then an exact (distance, id) selection.
+//   sqnorm_rows_kernel   |x|^2 per row (the centroid norms of the GEMM-based forms).
 #include <cfloat>
 
 #include "common.hpp"
