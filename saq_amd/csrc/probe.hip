@@ -412,7 +412,9 @@ __device__ __forceinline__ float block_max256(float v, float *wtmp) {
     return v;
 }
 
-template <int DT>
+// KPT > 0: K <= 256 KPT, and every thread keeps the keys of its centroids (tid + 256 j) in
+// registers for the passes below; KPT = 0 forms them again in every pass.
+template <int DT, int KPT>
 __global__ __launch_bounds__(256) void probe_exact_kernel(const float *__restrict__ ip, const float *__restrict__ cnorm,
                                                           const float *__restrict__ cent, int K, int D,
                                                           const float *__restrict__ queries, int nprobe,
@@ -444,7 +446,15 @@ __global__ __launch_bounds__(256) void probe_exact_kernel(const float *__restric
     const float scale = 8.0f * (float)(D + 2) * 5.9604645e-8f;  // 4 x 2 (D + 2) u
     const float *iq = ip + q * K;
     auto keyf = [&](int c) { return DT == 1 ? -iq[c] : cnorm[c] - 2.0f * iq[c]; };
+    float kv[KPT > 0 ? KPT : 1];
     float emax = 0.0f;
+    if constexpr (KPT > 0) {
+#pragma unroll
+        for (int j = 0; j < KPT; ++j) {
+            const int c = tid + 256 * j;
+            kv[j] = c < K ? keyf(c) : 0.0f;
+        }
+    }
     for (int c = tid; c < K; c += 256) {
         const float cc = sqrtf(fmaxf(cnorm[c], 0.0f)) * 1.0001f;
         emax = fmaxf(emax, scale * (qa + cc) * (qa + cc));
@@ -458,9 +468,18 @@ __global__ __launch_bounds__(256) void probe_exact_kernel(const float *__restric
         hist[tid] = 0;
         __syncthreads();
         const unsigned long long hmask = pass == 0 ? 0ull : (~0ull << (shift + 8));
-        for (int c = tid; c < K; c += 256) {
-            const unsigned long long k = ((unsigned long long)float_key(keyf(c)) << 32) | (uint32_t)c;
-            if ((k & hmask) == pre) atomicAdd(&hist[(k >> shift) & 0xFF], 1u);
+        if constexpr (KPT > 0) {
+#pragma unroll
+            for (int j = 0; j < KPT; ++j) {
+                const int c = tid + 256 * j;
+                const unsigned long long k = ((unsigned long long)float_key(kv[j]) << 32) | (uint32_t)c;
+                if (c < K && (k & hmask) == pre) atomicAdd(&hist[(k >> shift) & 0xFF], 1u);
+            }
+        } else {
+            for (int c = tid; c < K; c += 256) {
+                const unsigned long long k = ((unsigned long long)float_key(keyf(c)) << 32) | (uint32_t)c;
+                if ((k & hmask) == pre) atomicAdd(&hist[(k >> shift) & 0xFF], 1u);
+            }
         }
         __syncthreads();
         const int h = (int)hist[tid];
@@ -481,20 +500,40 @@ __global__ __launch_bounds__(256) void probe_exact_kernel(const float *__restric
     }
     // T = the nprobe-th smallest key: the largest key <= thr
     float tkey = -INFINITY;
-    for (int c = tid; c < K; c += 256) {
-        const float kf = keyf(c);
-        const unsigned long long k = ((unsigned long long)float_key(kf) << 32) | (uint32_t)c;
-        if (k <= thr) tkey = fmaxf(tkey, kf);
+    if constexpr (KPT > 0) {
+#pragma unroll
+        for (int j = 0; j < KPT; ++j) {
+            const int c = tid + 256 * j;
+            const unsigned long long k = ((unsigned long long)float_key(kv[j]) << 32) | (uint32_t)c;
+            if (c < K && k <= thr) tkey = fmaxf(tkey, kv[j]);
+        }
+    } else {
+        for (int c = tid; c < K; c += 256) {
+            const float kf = keyf(c);
+            const unsigned long long k = ((unsigned long long)float_key(kf) << 32) | (uint32_t)c;
+            if (k <= thr) tkey = fmaxf(tkey, kf);
+        }
     }
     tkey = block_max256(tkey, wtmp);
     const float lim = tkey + 2.0f * emax;
     if (tid == 0) cnt_s = 0;
     __syncthreads();
-    for (int c = tid; c < K; c += 256)
-        if (keyf(c) <= lim) {
-            const int p = atomicAdd(&cnt_s, 1);
-            if (p < kExactCand) sel[p] = (uint32_t)c;
+    if constexpr (KPT > 0) {
+#pragma unroll
+        for (int j = 0; j < KPT; ++j) {
+            const int c = tid + 256 * j;
+            if (c < K && kv[j] <= lim) {
+                const int p = atomicAdd(&cnt_s, 1);
+                if (p < kExactCand) sel[p] = (uint32_t)c;
+            }
         }
+    } else {
+        for (int c = tid; c < K; c += 256)
+            if (keyf(c) <= lim) {
+                const int p = atomicAdd(&cnt_s, 1);
+                if (p < kExactCand) sel[p] = (uint32_t)c;
+            }
+    }
     __syncthreads();
     const bool all = cnt_s > kExactCand;  // too many candidates: every centroid
     const int n = all ? K : cnt_s;
@@ -620,12 +659,14 @@ hipError_t launch_probe_exact(const float *ip, const float *cnorm, const float *
     if (nprobe > kExactCand || (size_t)D * 4 > 48 * 1024) return hipErrorInvalidValue;
     if (nq == 0) return hipSuccess;
     const size_t sm = (size_t)D * 4;
+    auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3((unsigned)nq), dim3(256), sm, st, ip, cnorm, centroids, K, D, queries, nprobe,
+                           probes);
+    };
     if (dist_type == 1)
-        hipLaunchKernelGGL(probe_exact_kernel<1>, dim3((unsigned)nq), dim3(256), sm, st, ip, cnorm, centroids, K, D,
-                           queries, nprobe, probes);
+        K <= 4096 ? go(probe_exact_kernel<1, 16>) : go(probe_exact_kernel<1, 0>);
     else
-        hipLaunchKernelGGL(probe_exact_kernel<0>, dim3((unsigned)nq), dim3(256), sm, st, ip, cnorm, centroids, K, D,
-                           queries, nprobe, probes);
+        K <= 4096 ? go(probe_exact_kernel<0, 16>) : go(probe_exact_kernel<0, 0>);
     return hipGetLastError();
 }
 

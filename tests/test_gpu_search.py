@@ -220,6 +220,33 @@ def test_gpu_probe_candidate_fallback(dist_type, monkeypatch):
     ix.close()
 
 
+@pytest.mark.parametrize("dist_type", [0, 1])
+def test_gpu_probe_more_than_4096_lists(dist_type, monkeypatch):
+    """K = 4500: the GEMM filter reads its keys again in every pass (they no longer fit 16 per
+    thread); lists equal the oracle's."""
+    monkeypatch.setenv("SAQGPU_PROBE", "gemm")
+    from oracle_lib import oracle
+    from saq_amd import Index
+    O = oracle()
+    rng = np.random.default_rng(31)
+    K, d, nq = 4500, 96, 8
+    cent = rng.standard_normal((K, d)).astype(np.float32)
+    queries = rng.standard_normal((nq, d)).astype(np.float32)
+    var = np.ones(128, np.float32)
+    ix = Index(d, [(128, 2)], cent, np.zeros(K, np.int64), var)
+    for nprobe in (1, 40):
+        got = ix.probe(queries, nprobe, dist_type=dist_type)
+        want = np.zeros((nq, nprobe), np.uint32)
+        for i, q in enumerate(queries):
+            pr = np.zeros(K, np.uint32)
+            pd = np.zeros(K, np.float32)
+            O.orc_probe(cent, K, d, np.ascontiguousarray(q), K, dist_type, pr, pd.ctypes.data)
+            key = -pd if dist_type == 1 else pd
+            want[i] = pr[np.lexsort((pr, key))][:nprobe]
+        assert np.array_equal(want, got), f"nprobe={nprobe}"
+    ix.close()
+
+
 def test_cluster_layout_round_trip():
     for plan, compact in (([(256, 4)], False), ([(64, 1), (64, 5), (128, 3)], False), ([(128, 5), (128, 0)], True)):
         case = F.make_case(n=1500, d=256, K=6, nq=4, plan=plan, seed=19, compact=compact)
