@@ -281,7 +281,9 @@ __device__ __forceinline__ int block_scan256(int v, int *wsum) {
 //
 // kIP: dist holds inner products q.c and the key is formed on the fly: |c|^2 - 2 q.c for L2
 // (|q|^2 is the same for every centroid of a query and is left out), q.c itself for IP.
-template <int DT, bool kIP = false>
+// KPT > 0: K <= 256 KPT, the keys of a thread's centroids (tid + 256 j) stay in registers
+// across the passes; KPT = 0 forms them again in every pass.
+template <int DT, bool kIP = false, int KPT = 0>
 __global__ __launch_bounds__(256) void probe_select_kernel(const float *__restrict__ dist, int K, int nprobe,
                                                            uint32_t *__restrict__ probes,
                                                            const float *__restrict__ cnorm = nullptr) {
@@ -297,6 +299,11 @@ __global__ __launch_bounds__(256) void probe_select_kernel(const float *__restri
         const float v = dq[i];
         return sel_key<DT>(kIP && DT == 0 ? cnorm[i] - 2.0f * v : v, i);
     };
+    unsigned long long kv[KPT > 0 ? KPT : 1];
+    if constexpr (KPT > 0) {
+#pragma unroll
+        for (int j = 0; j < KPT; ++j) kv[j] = tid + 256 * j < K ? key(tid + 256 * j) : ~0ull;
+    }
     unsigned long long pre = 0, thr = ~0ull;
     int r = nprobe;
     for (int pass = 0; pass < 8; ++pass) {
@@ -304,9 +311,15 @@ __global__ __launch_bounds__(256) void probe_select_kernel(const float *__restri
         hist[tid] = 0;
         __syncthreads();
         const unsigned long long hmask = pass == 0 ? 0ull : (~0ull << (shift + 8));
-        for (int i = tid; i < K; i += 256) {
-            const unsigned long long k = key(i);
-            if ((k & hmask) == pre) atomicAdd(&hist[(k >> shift) & 0xFF], 1u);
+        if constexpr (KPT > 0) {
+#pragma unroll
+            for (int j = 0; j < KPT; ++j)
+                if (tid + 256 * j < K && (kv[j] & hmask) == pre) atomicAdd(&hist[(kv[j] >> shift) & 0xFF], 1u);
+        } else {
+            for (int i = tid; i < K; i += 256) {
+                const unsigned long long k = key(i);
+                if ((k & hmask) == pre) atomicAdd(&hist[(k >> shift) & 0xFF], 1u);
+            }
         }
         __syncthreads();
         const int h = (int)hist[tid];
@@ -327,11 +340,20 @@ __global__ __launch_bounds__(256) void probe_select_kernel(const float *__restri
     }
     if (tid == 0) cnt_s = 0;
     __syncthreads();
-    for (int i = tid; i < K; i += 256) {
-        const unsigned long long k = key(i);
-        if (k <= thr) {
-            const int p = atomicAdd(&cnt_s, 1);
-            if (p < kSelMax) sel[p] = k;
+    if constexpr (KPT > 0) {
+#pragma unroll
+        for (int j = 0; j < KPT; ++j)
+            if (tid + 256 * j < K && kv[j] <= thr) {
+                const int p = atomicAdd(&cnt_s, 1);
+                if (p < kSelMax) sel[p] = kv[j];
+            }
+    } else {
+        for (int i = tid; i < K; i += 256) {
+            const unsigned long long k = key(i);
+            if (k <= thr) {
+                const int p = atomicAdd(&cnt_s, 1);
+                if (p < kSelMax) sel[p] = k;
+            }
         }
     }
     __syncthreads();
@@ -380,12 +402,13 @@ hipError_t launch_probe_select_ip(const float *ip, const float *cnorm, int K, in
                                   uint32_t *probes, hipStream_t st) {
     if (nprobe > kSelMax) return hipErrorInvalidValue;
     if (nq == 0) return hipSuccess;
+    auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3((unsigned)nq), dim3(256), 0, st, ip, K, nprobe, probes, cnorm);
+    };
     if (dist_type == 1)
-        hipLaunchKernelGGL((probe_select_kernel<1, true>), dim3((unsigned)nq), dim3(256), 0, st, ip, K, nprobe, probes,
-                           cnorm);
+        K <= 4096 ? go(probe_select_kernel<1, true, 16>) : go(probe_select_kernel<1, true, 0>);
     else
-        hipLaunchKernelGGL((probe_select_kernel<0, true>), dim3((unsigned)nq), dim3(256), 0, st, ip, K, nprobe, probes,
-                           cnorm);
+        K <= 4096 ? go(probe_select_kernel<0, true, 16>) : go(probe_select_kernel<0, true, 0>);
     return hipGetLastError();
 }
 
