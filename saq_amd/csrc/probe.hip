@@ -272,6 +272,59 @@ __device__ __forceinline__ int block_scan256(int v, int *wsum) {
     return v;
 }
 
+// LDS of block_radix_kth.
+struct RadixLds {
+    unsigned int hist[256];
+    int wsum[4];
+    unsigned long long prefix;
+    int rank, done;
+};
+
+// The r-th smallest of the workgroup's distinct 64-bit keys (1 <= r <= their number), as the
+// value thr with exactly r keys <= thr: a radix select by 8-bit digits from the top.  A pass
+// whose bucket is taken whole ends the search, so distinct float keys take at most 4 passes
+// and only exact ties reach the id bytes.  A thread's
+// keys: kv[0 .. KPT) (centroids tid + 256 j below K) or, for KPT = 0, key(c) for c = tid,
+// tid + 256, ... below K.
+template <int KPT, class KeyFn>
+__device__ __forceinline__ unsigned long long block_radix_kth(const unsigned long long (&kv)[KPT > 0 ? KPT : 1],
+                                                              KeyFn key, int K, int r, RadixLds &sh) {
+    const int tid = threadIdx.x;
+    unsigned long long pre = 0;
+    for (int pass = 0; pass < 8; ++pass) {
+        const int shift = 56 - 8 * pass;
+        sh.hist[tid] = 0;
+        __syncthreads();
+        const unsigned long long hmask = pass == 0 ? 0ull : (~0ull << (shift + 8));
+        if constexpr (KPT > 0) {
+#pragma unroll
+            for (int j = 0; j < KPT; ++j)
+                if (tid + 256 * j < K && (kv[j] & hmask) == pre) atomicAdd(&sh.hist[(kv[j] >> shift) & 0xFF], 1u);
+        } else {
+            for (int c = tid; c < K; c += 256) {
+                const unsigned long long k = key(c);
+                if ((k & hmask) == pre) atomicAdd(&sh.hist[(k >> shift) & 0xFF], 1u);
+            }
+        }
+        __syncthreads();
+        const int h = (int)sh.hist[tid];
+        const int incl = block_scan256(h, sh.wsum);
+        const int excl = incl - h;
+        if (excl < r && r <= incl) {  // exactly one thread: the bucket holding the r-th key
+            sh.prefix = pre | ((unsigned long long)tid << shift);
+            sh.rank = r - excl;
+            sh.done = (r - excl) == h;
+        }
+        __syncthreads();
+        pre = sh.prefix;
+        r = sh.rank;
+        const bool done = sh.done;
+        __syncthreads();
+        if (done || pass == 7) return pass == 7 ? pre : (pre | ((1ull << shift) - 1));
+    }
+    return pre;  // not reached
+}
+
 // One workgroup per query.  Radix select of the nprobe-th smallest 64-bit key (distance key,
 // centroid id), 8 bits per pass from the top; a pass whose chosen byte bucket is taken whole
 // ends the search (every key below the bucket's upper end is selected), so distinct distances
@@ -287,57 +340,23 @@ template <int DT, bool kIP = false, int KPT = 0>
 __global__ __launch_bounds__(256) void probe_select_kernel(const float *__restrict__ dist, int K, int nprobe,
                                                            uint32_t *__restrict__ probes,
                                                            const float *__restrict__ cnorm = nullptr) {
-    __shared__ unsigned int hist[256];
-    __shared__ int wsum[4];
-    __shared__ unsigned long long prefix_s;
-    __shared__ int rank_s, done_s, cnt_s;
+    __shared__ RadixLds rsh;
+    __shared__ int cnt_s;
     __shared__ unsigned long long sel[kSelMax];
     const int64_t q = blockIdx.x;
     const float *dq = dist + q * K;
     const int tid = threadIdx.x;
-    auto key = [&](int i) {
+    auto key = [=](int i) {
         const float v = dq[i];
         return sel_key<DT>(kIP && DT == 0 ? cnorm[i] - 2.0f * v : v, i);
     };
     unsigned long long kv[KPT > 0 ? KPT : 1];
+    kv[0] = 0;
     if constexpr (KPT > 0) {
 #pragma unroll
         for (int j = 0; j < KPT; ++j) kv[j] = tid + 256 * j < K ? key(tid + 256 * j) : ~0ull;
     }
-    unsigned long long pre = 0, thr = ~0ull;
-    int r = nprobe;
-    for (int pass = 0; pass < 8; ++pass) {
-        const int shift = 56 - 8 * pass;
-        hist[tid] = 0;
-        __syncthreads();
-        const unsigned long long hmask = pass == 0 ? 0ull : (~0ull << (shift + 8));
-        if constexpr (KPT > 0) {
-#pragma unroll
-            for (int j = 0; j < KPT; ++j)
-                if (tid + 256 * j < K && (kv[j] & hmask) == pre) atomicAdd(&hist[(kv[j] >> shift) & 0xFF], 1u);
-        } else {
-            for (int i = tid; i < K; i += 256) {
-                const unsigned long long k = key(i);
-                if ((k & hmask) == pre) atomicAdd(&hist[(k >> shift) & 0xFF], 1u);
-            }
-        }
-        __syncthreads();
-        const int h = (int)hist[tid];
-        const int incl = block_scan256(h, wsum);
-        const int excl = incl - h;
-        if (excl < r && r <= incl) {  // exactly one thread: the bucket holding the r-th key
-            prefix_s = pre | ((unsigned long long)tid << shift);
-            rank_s = r - excl;
-            done_s = (r - excl) == h;
-        }
-        __syncthreads();
-        pre = prefix_s;
-        r = rank_s;
-        if (done_s || pass == 7) {
-            thr = pass == 7 ? pre : (pre | ((1ull << shift) - 1));  // exactly nprobe keys <= thr
-            break;
-        }
-    }
+    const unsigned long long thr = block_radix_kth<KPT>(kv, key, K, nprobe, rsh);  // exactly nprobe keys <= thr
     if (tid == 0) cnt_s = 0;
     __syncthreads();
     if constexpr (KPT > 0) {
@@ -443,11 +462,9 @@ __global__ __launch_bounds__(256) void probe_exact_kernel(const float *__restric
                                                           const float *__restrict__ queries, int nprobe,
                                                           uint32_t *__restrict__ probes) {
     extern __shared__ float qsh[];  // D floats
-    __shared__ unsigned int hist[256];
-    __shared__ int wsum[4];
+    __shared__ RadixLds rsh;
     __shared__ float wtmp[4];
-    __shared__ unsigned long long prefix_s;
-    __shared__ int rank_s, done_s, cnt_s;
+    __shared__ int cnt_s;
     __shared__ unsigned long long sel[kExactCand];
     const int64_t q = blockIdx.x;
     const int tid = threadIdx.x;
@@ -483,44 +500,20 @@ __global__ __launch_bounds__(256) void probe_exact_kernel(const float *__restric
         emax = fmaxf(emax, scale * (qa + cc) * (qa + cc));
     }
     emax = block_max256(emax, wtmp);
-    // radix select of the nprobe-th smallest (key, id) (as probe_select_kernel)
-    unsigned long long pre = 0, thr = ~0ull;
-    int r = nprobe;
-    for (int pass = 0; pass < 8; ++pass) {
-        const int shift = 56 - 8 * pass;
-        hist[tid] = 0;
-        __syncthreads();
-        const unsigned long long hmask = pass == 0 ? 0ull : (~0ull << (shift + 8));
-        if constexpr (KPT > 0) {
+    // the nprobe-th smallest (key, id)
+    const float *cn = cnorm;
+    auto key64 = [=](int c) {
+        const float kf = DT == 1 ? -iq[c] : cn[c] - 2.0f * iq[c];
+        return ((unsigned long long)float_key(kf) << 32) | (uint32_t)c;
+    };
+    unsigned long long kk[KPT > 0 ? KPT : 1];
+    kk[0] = 0;
+    if constexpr (KPT > 0) {
 #pragma unroll
-            for (int j = 0; j < KPT; ++j) {
-                const int c = tid + 256 * j;
-                const unsigned long long k = ((unsigned long long)float_key(kv[j]) << 32) | (uint32_t)c;
-                if (c < K && (k & hmask) == pre) atomicAdd(&hist[(k >> shift) & 0xFF], 1u);
-            }
-        } else {
-            for (int c = tid; c < K; c += 256) {
-                const unsigned long long k = ((unsigned long long)float_key(keyf(c)) << 32) | (uint32_t)c;
-                if ((k & hmask) == pre) atomicAdd(&hist[(k >> shift) & 0xFF], 1u);
-            }
-        }
-        __syncthreads();
-        const int h = (int)hist[tid];
-        const int incl = block_scan256(h, wsum);
-        const int excl = incl - h;
-        if (excl < r && r <= incl) {
-            prefix_s = pre | ((unsigned long long)tid << shift);
-            rank_s = r - excl;
-            done_s = (r - excl) == h;
-        }
-        __syncthreads();
-        pre = prefix_s;
-        r = rank_s;
-        if (done_s || pass == 7) {
-            thr = pass == 7 ? pre : (pre | ((1ull << shift) - 1));
-            break;
-        }
+        for (int j = 0; j < KPT; ++j)
+            kk[j] = ((unsigned long long)float_key(kv[j]) << 32) | (uint32_t)(tid + 256 * j);
     }
+    const unsigned long long thr = block_radix_kth<KPT>(kk, key64, K, nprobe, rsh);
     // T = the nprobe-th smallest key: the largest key <= thr
     float tkey = -INFINITY;
     if constexpr (KPT > 0) {
@@ -613,40 +606,13 @@ __global__ __launch_bounds__(256) void probe_exact_kernel(const float *__restric
         return;
     }
     // fallback: the nprobe smallest exact keys over all K by a second radix select
-    pre = 0;
-    thr = ~0ull;
-    r = nprobe;
-    for (int pass = 0; pass < 8; ++pass) {
-        const int shift = 56 - 8 * pass;
-        hist[tid] = 0;
-        __syncthreads();
-        const unsigned long long hmask = pass == 0 ? 0ull : (~0ull << (shift + 8));
-        for (int c = tid; c < K; c += 256) {
-            const unsigned long long k = exact(c);
-            if ((k & hmask) == pre) atomicAdd(&hist[(k >> shift) & 0xFF], 1u);
-        }
-        __syncthreads();
-        const int h = (int)hist[tid];
-        const int incl = block_scan256(h, wsum);
-        const int excl = incl - h;
-        if (excl < r && r <= incl) {
-            prefix_s = pre | ((unsigned long long)tid << shift);
-            rank_s = r - excl;
-            done_s = (r - excl) == h;
-        }
-        __syncthreads();
-        pre = prefix_s;
-        r = rank_s;
-        if (done_s || pass == 7) {
-            thr = pass == 7 ? pre : (pre | ((1ull << shift) - 1));
-            break;
-        }
-    }
+    const unsigned long long none[1] = {0};
+    const unsigned long long ethr = block_radix_kth<0>(none, exact, K, nprobe, rsh);
     if (tid == 0) cnt_s = 0;
     __syncthreads();
     for (int c = tid; c < K; c += 256) {
         const unsigned long long k = exact(c);
-        if (k <= thr) {
+        if (k <= ethr) {
             const int p = atomicAdd(&cnt_s, 1);
             if (p < kExactCand) sel[p] = k;
         }
