@@ -1303,7 +1303,7 @@ int search_batched_locked(saqgpu_index *ix, const float *queries, int64_t nq, co
     rc = 0;
     rc |= ix->ws_qrot.ensure(nq * ix->D_pad);
     rc |= ix->ws_bound.ensure(nq * ix->nseg);
-    rc |= ix->bq_ws.ensure(5 * (size_t)K + 2 * npairs + 5 * max_tiles + 4);
+    rc |= ix->bq_ws.ensure(4 * (size_t)K + 2 * npairs + 5 * max_tiles + 4);
     rc |= ix->bq_qq.ensure((size_t)qs * npairs * ix->bq_kq);
     rc |= ix->bq_pc.ensure((size_t)npairs * (3 * ix->bq_nqs + 1));
     rc |= ix->bq_pcorr.ensure((size_t)npairs * 2 * std::max(1, ix->bq_nqs));
@@ -1353,10 +1353,9 @@ int search_batched_locked(saqgpu_index *ix, const float *queries, int64_t nq, co
     BqWork w;
     w.counts = wsp;
     w.cstart = wsp + K;
-    w.tstart = wsp + 2 * K;
-    w.candstart = wsp + 3 * K;
-    w.nsplit = wsp + 4 * K;
-    w.slot = wsp + 5 * K;
+    w.candstart = wsp + 2 * K;
+    w.nsplit = wsp + 3 * K;
+    w.slot = wsp + 4 * K;
     w.pairs = w.slot + npairs;
     w.tile_clu = w.pairs + npairs;
     w.tile_first = w.tile_clu + max_tiles;

@@ -160,9 +160,10 @@ __global__ __launch_bounds__(64) void bq_build_factors_kernel(IndexView ix, Batc
 }
 
 // ------------------------------------------------------------------ grouping
-__global__ void bq_fill_kernel(int32_t *p, int64_t n, int32_t v) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        p[i] = v;
+// Zeroes the per-list pair counts and the tile counter of a launch.
+__global__ void bq_zero_kernel(int32_t *counts, int K, int32_t *work) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < K; i += gridDim.x * blockDim.x) counts[i] = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *work = 0;
 }
 
 // (q, p) -> slot within its list's pair group
@@ -186,12 +187,15 @@ __global__ void bq_count_kernel(const uint32_t *probes, const int32_t *npq, int6
 // wavefront's share and one long list cannot set the kernel's tail.  Then exclusive
 // prefix sums, over the lists in the given order (largest first, so the longest tiles start
 // first), of the pair counts, of the work tiles (query tiles x ranges) and of the candidate
-// lists (pairs x ranges).  The bounds sum_c tiles_c <= npairs / QT + K + kSplitWaves and
-// sum_c cands_c <= npairs + kSplitWaves * QT follow from sv >= W / kSplitWaves.
+// lists (pairs x ranges), and every list's work tiles.  The bounds sum_c tiles_c <= npairs / QT
+// + K + kSplitWaves and sum_c cands_c <= npairs + kSplitWaves * QT follow from
+// sv >= W / kSplitWaves.
 __global__ __launch_bounds__(1024) void bq_scan_kernel(const int32_t *counts, const int32_t *order,
                                                        const int32_t *clu_n, int K, int QT, int32_t *cstart,
-                                                       int32_t *tstart, int32_t *candstart, int32_t *nsplit,
-                                                       int32_t *ntiles, int32_t *npairs, int32_t *sv_out) {
+                                                       int32_t *candstart, int32_t *nsplit, int32_t *ntiles,
+                                                       int32_t *npairs, int32_t *sv_out, int32_t *tile_clu,
+                                                       int32_t *tile_first, int32_t *tile_cnt, int32_t *tile_v0,
+                                                       int32_t *tile_cand) {
     // thread t holds lists order[t * per .. t * per + per - 1] (per <= kScanRounds)
     __shared__ int wsum[3][16];
     __shared__ long long wk_s[16];
@@ -256,13 +260,22 @@ __global__ __launch_bounds__(1024) void bq_scan_kernel(const int32_t *counts, co
 #pragma unroll
     for (int j = 0; j < kScanRounds; ++j) {
         if (oc[j] < 0) continue;
-        cstart[oc[j]] = rc;
-        tstart[oc[j]] = rt;
-        candstart[oc[j]] = rk;
-        nsplit[oc[j]] = ns[j];
-        rc += cn[j];
-        rt += (cn[j] + QT - 1) / QT * ns[j];
-        rk += cn[j] * ns[j];
+        const int c = oc[j], n = cn[j];
+        cstart[c] = rc;
+        candstart[c] = rk;
+        nsplit[c] = ns[j];
+        // the list's work tiles: every query tile of QT of its pairs times every vector range;
+        // the candidate list of (pair slot i, range r) is candstart[c] + r * n + i
+        for (int i = 0; i < n; i += QT)
+            for (int r = 0; r < ns[j]; ++r, ++rt) {
+                tile_clu[rt] = c;
+                tile_first[rt] = rc + i;
+                tile_cnt[rt] = min(QT, n - i);
+                tile_v0[rt] = r * sv;
+                tile_cand[rt] = rk + r * n + i;
+            }
+        rc += n;
+        rk += n * ns[j];
     }
     if (t == 0) {
         *ntiles = tt;
@@ -276,26 +289,6 @@ __global__ void bq_scatter_kernel(const uint32_t *probes, const int32_t *slot, i
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n || slot[i] < 0) return;
     pairs[cstart[probes[i]] + slot[i]] = (int32_t)i;
-}
-
-// Work tiles of list c: every query tile of QT of its pairs times every vector range of sv
-// vectors.  Candidate list of (pair slot j, range r) of list c: candstart[c] + r * counts[c] + j.
-__global__ void bq_tiles_kernel(const int32_t *counts, const int32_t *cstart, const int32_t *tstart,
-                                const int32_t *candstart, const int32_t *nsplit, const int32_t *sv_p, int K, int QT,
-                                int32_t *tile_clu, int32_t *tile_first, int32_t *tile_cnt, int32_t *tile_v0,
-                                int32_t *tile_cand) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= K) return;
-    const int n = counts[c], ns = nsplit[c], sv = *sv_p;
-    int t = tstart[c];
-    for (int i = 0; i < n; i += QT)
-        for (int r = 0; r < ns; ++r, ++t) {
-            tile_clu[t] = c;
-            tile_first[t] = cstart[c] + i;
-            tile_cnt[t] = min(QT, n - i);
-            tile_v0[t] = r * sv;
-            tile_cand[t] = candstart[c] + r * n + i;
-        }
 }
 
 // ------------------------------------------------------------------ wave-held top-k
@@ -478,20 +471,10 @@ struct PairPrepArgs {
     float *pq2;         // [pairs]: |r|^2 over all segments
 };
 
-__device__ __forceinline__ float wave_max(float v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
-    return v;
-}
-__device__ __forceinline__ float wave_sum(float v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    return v;
-}
-__device__ __forceinline__ int wave_isum(int v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    return v;
+// One butterfly step (partner lane ^ J) of wave reductions, through xor_lane's DPP / swizzle.
+template <int J>
+__device__ __forceinline__ float xor_f(float v) {
+    return __int_as_float(xor_lane<J>(__float_as_int(v)));
 }
 
 // Quantises one segment's residual r (held as float4 per lane: element 4 lane + 256 k) and
@@ -517,12 +500,17 @@ __device__ __forceinline__ void pairprep_segment(const PairPrepArgs &a, const BS
         sr += (r[k].x + r[k].y) + (r[k].z + r[k].w);
         sq += (r[k].x * r[k].x + r[k].y * r[k].y) + (r[k].z * r[k].z + r[k].w * r[k].w);
     }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        sq += __shfl_xor(sq, o);
-        sr += __shfl_xor(sr, o);
-        amax = fmaxf(amax, __shfl_xor(amax, o));
-    }
+    auto red3 = [&](auto step) {
+        sq += step(sq);
+        sr += step(sr);
+        amax = fmaxf(amax, step(amax));
+    };
+    red3([](float v) { return xor_f<32>(v); });
+    red3([](float v) { return xor_f<16>(v); });
+    red3([](float v) { return xor_f<8>(v); });
+    red3([](float v) { return xor_f<4>(v); });
+    red3([](float v) { return xor_f<2>(v); });
+    red3([](float v) { return xor_f<1>(v); });
     q2 += sq;
     if (bs.qs < 0) return;
     const float s1 = amax / 127.0f, inv1 = amax > 0.0f ? 127.0f / amax : 0.0f;
@@ -553,11 +541,12 @@ __device__ __forceinline__ void pairprep_segment(const PairPrepArgs &a, const BS
                 ((uint32_t)(uint8_t)v2[3] << 24);
         }
     }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        sum1 += __shfl_xor(sum1, o);
-        sum2 += __shfl_xor(sum2, o);
-    }
+    sum1 += xor_lane<32>(sum1), sum2 += xor_lane<32>(sum2);
+    sum1 += xor_lane<16>(sum1), sum2 += xor_lane<16>(sum2);
+    sum1 += xor_lane<8>(sum1), sum2 += xor_lane<8>(sum2);
+    sum1 += xor_lane<4>(sum1), sum2 += xor_lane<4>(sum2);
+    sum1 += xor_lane<2>(sum1), sum2 += xor_lane<2>(sum2);
+    sum1 += xor_lane<1>(sum1), sum2 += xor_lane<1>(sum2);
     if (lane == 0) {
         float *pc = a.pc + (g * bv.nqs + bs.qs) * 3;
         pc[0] = bs.sqd * s1;
@@ -1181,17 +1170,14 @@ hipError_t launch_bq_build(const IndexView &ix, const BatchedView &bv, const int
 
 hipError_t launch_bq_group(const uint32_t *probes, const int32_t *npq, int64_t nq, int NP, int K, int QT,
                            const int32_t *order, const int32_t *clu_n, const BqWork &w, hipStream_t st) {
-    hipLaunchKernelGGL(bq_fill_kernel, dim3(64), dim3(256), 0, st, w.counts, (int64_t)K, 0);
-    hipLaunchKernelGGL(bq_fill_kernel, dim3(1), dim3(64), 0, st, w.work, (int64_t)1, 0);
+    hipLaunchKernelGGL(bq_zero_kernel, dim3(16), dim3(256), 0, st, w.counts, K, w.work);
     const int64_t n = nq * NP;
     const unsigned g = (unsigned)((n + 255) / 256);
     hipLaunchKernelGGL(bq_count_kernel, dim3(g), dim3(256), 0, st, probes, npq, nq, NP, w.counts, w.slot);
-    hipLaunchKernelGGL(bq_scan_kernel, dim3(1), dim3(1024), 0, st, w.counts, order, clu_n, K, QT, w.cstart, w.tstart,
-                       w.candstart, w.nsplit, w.ntiles, w.npairs, w.sv);
+    hipLaunchKernelGGL(bq_scan_kernel, dim3(1), dim3(1024), 0, st, w.counts, order, clu_n, K, QT, w.cstart,
+                       w.candstart, w.nsplit, w.ntiles, w.npairs, w.sv, w.tile_clu, w.tile_first, w.tile_cnt,
+                       w.tile_v0, w.tile_cand);
     hipLaunchKernelGGL(bq_scatter_kernel, dim3(g), dim3(256), 0, st, probes, w.slot, n, w.cstart, w.pairs);
-    hipLaunchKernelGGL(bq_tiles_kernel, dim3((K + 255) / 256), dim3(256), 0, st, w.counts, w.cstart, w.tstart,
-                       w.candstart, w.nsplit, w.sv, K, QT, w.tile_clu, w.tile_first, w.tile_cnt, w.tile_v0,
-                       w.tile_cand);
     return hipGetLastError();
 }
 

@@ -40,7 +40,7 @@ struct BatchedView {
 // work tile: up to QT pairs of one list and one vector range of it, whose per-pair top-k goes
 // to candidate lists tile_cand .. + cnt - 1).
 struct BqWork {
-    int32_t *counts, *slot, *cstart, *tstart, *candstart, *nsplit, *ntiles, *npairs, *sv, *pairs;
+    int32_t *counts, *slot, *cstart, *candstart, *nsplit, *ntiles, *npairs, *sv, *pairs;
     int32_t *tile_clu, *tile_first, *tile_cnt, *tile_v0, *tile_cand, *work;
 };
 // Upper bounds of the work tiles and candidate lists of a launch.
