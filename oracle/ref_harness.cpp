@@ -20,6 +20,7 @@
 #include "quantization/fastscan/fastscan.hpp"
 #include "quantization/fastscan/fastscan_highacc.hpp"
 #include "utils/code_helper.hpp"
+#include "utils/space.hpp"
 
 using namespace saqlib;
 
@@ -147,5 +148,17 @@ void ref_code_floor(const float *o, int64_t D, double v_mi, double delta, int co
     Eigen::VectorXf cf = ((ov.array() - v_mi) / delta).floor().array().min(code_max);
     std::memcpy(out, cf.data(), D * sizeof(float));
 }
+
+// The single-vector estimator's bit-plane kernels (utils/space.hpp) and its query step
+// (CaqEstimatorSingleImpl::prepare, caq/caq_estimator.hpp:300-301, with kNumBits = 8).
+void ref_new_transpose_bin(const uint16_t *q, int64_t D, uint64_t *tq) { utils::new_transpose_bin(q, tq, D, 8); }
+
+float ref_warmup_ip_x0_q(const uint64_t *x, const uint64_t *tq, float delta, float vl, int64_t D) {
+    return utils::warmup_ip_x0_q(x, tq, delta, vl, D, 8);
+}
+
+float ref_mask_ip_x0_q(const float *q, const uint64_t *x, int64_t D) { return utils::mask_ip_x0_q(q, x, D); }
+
+float ref_single_delta(float vl, float vr) { return (vr - vl) / ((1 << 8) - 0.01); }
 
 }  // extern "C"

@@ -999,4 +999,235 @@ int orc_estimate(const orc_plan_t *plan, const float *variance, const float *con
     return (int)w;
 }
 
+// ---------------------------------------------------------------- single-vector path
+
+void orc_single_layout(const orc_plan_t *plan, orc_single_layout_t *out) {
+    // SaqSingleDataWrapper ctor + set_memory_base (quantization/single_data.hpp:180-285): the four
+    // sections and each segment's place in them; bytes = calculate_memory_size (:287-320).
+    std::memset(out, 0, sizeof(*out));
+    const int S = plan->num_seg;
+    int64_t code = 0, lng = 0;
+    for (int s = 0; s < S; ++s) {
+        const int64_t D = rd_up(plan->seg_dim[s], 64), b = plan->seg_bits[s];
+        out->seg_code_off[s] = b > 0 ? code : -1;
+        out->seg_long_off[s] = b > 1 ? lng : -1;
+        if (b > 0) code += D / 8;
+        if (b > 1) lng += D * (b - 1) / 8;
+    }
+    out->fac_off = 0;
+    out->code_off = rd_up((int64_t)S * 2 * 4, 64);
+    out->long_off = out->code_off + rd_up(code, 64);
+    out->exfac_off = out->long_off + lng;
+    out->bytes = out->exfac_off + (int64_t)S * 8;
+    for (int s = 0; s < S; ++s) {
+        if (out->seg_code_off[s] >= 0) out->seg_code_off[s] += out->code_off;
+        if (out->seg_long_off[s] >= 0) out->seg_long_off[s] += out->long_off;
+    }
+}
+
+void orc_single_transpose_bin(const uint16_t *q, int64_t D, uint64_t *tq) {
+    // utils::new_transpose_bin (utils/space.hpp:266-293) with b_query = 8: the 8 bit planes of
+    // each 64 dims, plane p at tq[p], dim k of the block at bit 63 - k.  The reference shifts
+    // 32-bit lanes by 16 - 8, which equals a 16-bit shift because every value is below 256.
+    for (int64_t b = 0; b < D / 64; ++b)
+        for (int p = 0; p < 8; ++p) {
+            uint64_t w = 0;
+            for (int k = 0; k < 64; ++k) w |= (uint64_t)((q[64 * b + k] >> p) & 1u) << (63 - k);
+            tq[8 * b + p] = w;
+        }
+}
+
+float orc_single_warmup_ip(const uint64_t *x, const uint64_t *tq, float delta, float vl, int64_t D) {
+    // utils::warmup_ip_x0_q (utils/space.hpp:126-226): integer sums, so any order gives the same
+    // ip and popcount; then delta * ip + vl * popcount in float.
+    uint64_t ip = 0, ppc = 0;
+    for (int64_t b = 0; b < D / 64; ++b) {
+        ppc += (uint64_t)__builtin_popcountll(x[b]);
+        for (int p = 0; p < 8; ++p) ip += (uint64_t)__builtin_popcountll(x[b] & tq[8 * b + p]) << p;
+    }
+    return (delta * (float)ip) + (vl * (float)ppc);
+}
+
+float orc_single_mask_ip(const float *q, const uint64_t *x, int64_t D) {
+    // utils::mask_ip_x0_q (utils/space.hpp:228-265): bits reversed so bit k is dim k; 16 lanes,
+    // lane l takes dims 64 b + 16 m + l (m = 0..3) masked (+0.0 when clear), block by block; then
+    // _mm512_reduce_add_ps.  The source adds the four masked loads of a block to the sum one after
+    // another; the release build (-Ofast, CMakeLists.txt:92) reassociates that chain into
+    // sum + ((m0 + m1) + (m2 + m3)), which is the order the reference executes and the one kept
+    // here (checked bit for bit against the compiled reference kernel).
+    float acc[16];
+    for (int l = 0; l < 16; ++l) acc[l] = 0.0f;
+    for (int64_t b = 0; b < D / 64; ++b) {
+        const uint64_t w = x[b];
+        for (int l = 0; l < 16; ++l) {
+            float mv[4];
+            for (int m = 0; m < 4; ++m) {
+                const int k = 16 * m + l;
+                mv[m] = ((w >> (63 - k)) & 1u) ? q[64 * b + k] : 0.0f;
+            }
+            acc[l] = acc[l] + ((mv[0] + mv[1]) + (mv[2] + mv[3]));
+        }
+    }
+    return orc_tree16(acc);
+}
+
+int64_t orc_single_quantize(const orc_plan_t *plan, const float *const *P, const float *data, int64_t n,
+                            const orc_caq_cfg_t *cfg, uint8_t *records, int64_t stride) {
+    // SAQuantizerSingle::quantize (quantization/saq_quantizer.hpp:78-94) -> QuantizerSingle::quantize
+    // (quantizer.hpp:105-124): per segment the row slice (zero-padded past num_dim) times P_s (a
+    // row-vector GEMV), CAQEncoder::encode_and_fac without a centroid, then
+    // store_quantization_result (:134-160): o_l2norm; for bits > 0 ip_cent_oa = 0, the MSB codes
+    // MSB-first with the bytes of every 8 reversed (:166-188), the ExFactor and, for bits > 1, the
+    // CodeHelper<bits-1> compacted low bits (:194-207).  A 1-bit segment packs the quantised
+    // codes (encode_and_fac hands none over below 2 bits, caq_encoder.hpp:238-244).
+    orc_single_layout_t L;
+    orc_single_layout(plan, &L);
+    int64_t nfail = 0;
+    std::vector<float> x, o;
+    std::vector<uint16_t> code, low;
+    for (int64_t i = 0; i < n; ++i) {
+        uint8_t *rec = records + i * stride;
+        std::memset(rec, 0, (size_t)L.bytes);
+        float *fac = reinterpret_cast<float *>(rec + L.fac_off);
+        float *exf = reinterpret_cast<float *>(rec + L.exfac_off);
+        int64_t off = 0;
+        for (int s = 0; s < plan->num_seg; ++s) {
+            const int64_t D = plan->seg_dim[s];
+            const int bits = plan->seg_bits[s];
+            x.assign(D, 0.0f);
+            const int64_t copy = std::min<int64_t>(D, plan->num_dim - off);
+            for (int64_t j = 0; j < copy; ++j) x[j] = data[i * plan->num_dim + off + j];
+            o.resize(D);
+            if (P && P[s])
+                orc_gemv_rowvec(x.data(), P[s], D, o.data());
+            else
+                o = x;
+            code.assign(D, 0);
+            orc_code_info_t info;
+            if (orc_caq_encode(o.data(), D, bits, cfg, nullptr, code.data(), &info)) ++nfail;
+            fac[2 * s] = info.o_l2norm;
+            off += D;
+            if (bits == 0) continue;
+            fac[2 * s + 1] = 0.0f;
+            const uint16_t sb = (uint16_t)(1u << (bits - 1));
+            uint8_t *sc = rec + L.seg_code_off[s];
+            for (int64_t j = 0; j < D / 8; ++j) {
+                uint8_t byte = 0;
+                for (int t = 0; t < 8; ++t)
+                    if (code[8 * j + t] & sb) byte |= (uint8_t)(0x80u >> t);
+                sc[j + 7 - 2 * (j % 8)] = byte;
+            }
+            exf[2 * s] = info.fac_rescale;
+            exf[2 * s + 1] = info.fac_error;
+            if (bits > 1) {
+                low.resize(D);
+                for (int64_t j = 0; j < D; ++j) low[j] = code[j] & (uint16_t)(sb - 1);
+                orc_compact_code(bits - 1, low.data(), D, rec + L.seg_long_off[s]);
+            }
+        }
+    }
+    return nfail;
+}
+
+void orc_single_estimate(const orc_plan_t *plan, const float *variance, const float *const *P,
+                         const uint8_t *records, int64_t nrec, int64_t stride, const float *query, float m,
+                         int dist_type, float *out_vars, float *out_fast, float *out_acc) {
+    // SaqSingleEstimator<Any> (quantization/saq_estimator.hpp:248-294) over CaqSingleEstimator
+    // (caq/caq_estimator.hpp:220-369, 450-497): the per-segment query of SaqEstimatorBase (:28-48),
+    // prepare (:292-308), varsEstDist (:310-315), compFastDist (:317-340), compAccurateDist
+    // (:342-368); the segment estimates are summed in segment order from 0.
+    orc_single_layout_t L;
+    orc_single_layout(plan, &L);
+    struct Seg {
+        int64_t D;
+        int bits;
+        float bound, q_l2sqr, q_l2norm, sum_q, delta, vlh, fconst;
+        std::vector<float> qr;
+        std::vector<uint64_t> tq;
+    };
+    const bool ip = dist_type == 1;
+    std::vector<Seg> sg(plan->num_seg);
+    int64_t off = 0;
+    for (int s = 0; s < plan->num_seg; ++s) {
+        Seg &e = sg[s];
+        e.D = plan->seg_dim[s];
+        e.bits = plan->seg_bits[s];
+        std::vector<float> cq(e.D, 0.0f);
+        const int64_t copy = std::min<int64_t>(e.D, plan->num_dim - off);
+        for (int64_t i = 0; i < copy; ++i) cq[i] = query[off + i];
+        e.bound = std::sqrt(orc_eigen_var_q2(variance + off, cq.data(), e.D)) * m;
+        e.qr.resize(e.D);
+        if (P && P[s])
+            orc_gemv_rowvec(cq.data(), P[s], e.D, e.qr.data());
+        else
+            e.qr = cq;
+        off += e.D;
+        e.q_l2sqr = orc_eigen_sqnorm(e.qr.data(), e.D);
+        e.q_l2norm = std::sqrt(e.q_l2sqr);
+        e.sum_q = orc_eigen_sum(e.qr.data(), e.D);
+        float vl = e.qr[0], vr = e.qr[0];
+        for (int64_t i = 1; i < e.D; ++i) {
+            vl = std::min(vl, e.qr[i]);
+            vr = std::max(vr, e.qr[i]);
+        }
+        e.delta = (float)((double)(vr - vl) / ((1 << 8) - 0.01));
+        std::vector<uint16_t> qsq(e.D);
+        for (int64_t i = 0; i < e.D; ++i) {
+            const float t = (e.qr[i] - vl) / e.delta;  // NaN when delta is 0: the x86 cast gives 0
+            qsq[i] = (t == t) ? (uint16_t)(int32_t)t : 0;
+        }
+        e.tq.resize(e.D / 64 * 8);
+        orc_single_transpose_bin(qsq.data(), e.D, e.tq.data());
+        e.vlh = (float)((double)vl + 0.5 * (double)e.delta);
+        const float one_over_sqrtD = (float)(1.0 / (double)std::sqrt((float)e.D));
+        e.fconst = (4.0f / 0.8f) * one_over_sqrtD;  // 4 / est_error * one_over_sqrtD_
+    }
+    for (int64_t r = 0; r < nrec; ++r) {
+        const uint8_t *rec = records + r * stride;
+        const float *fac = reinterpret_cast<const float *>(rec + L.fac_off);
+        const float *exf = reinterpret_cast<const float *>(rec + L.exfac_off);
+        float rv = 0.0f, rf = 0.0f, ra = 0.0f;
+        for (int s = 0; s < plan->num_seg; ++s) {
+            const Seg &e = sg[s];
+            const float o = fac[2 * s];
+            float v;
+            if (ip) {
+                v = 0.0f - e.bound;  // ip_q_c_ is never set for single records
+            } else {
+                const float t = o * o + e.q_l2sqr - 2 * e.bound;
+                v = (0.0f < t) ? t : 0.0f;  // std::max(0.0f, t)
+            }
+            float f = v, a;
+            const float o2 = o * o;
+            if (e.bits == 0) {
+                a = ip ? 0.0f : o2 + e.q_l2sqr;
+            } else {
+                const uint64_t *x = reinterpret_cast<const uint64_t *>(rec + L.seg_code_off[s]);
+                const float tmp = orc_single_warmup_ip(x, e.tq.data(), e.delta, e.vlh, e.D);
+                const double cst = 0.5 * (double)e.sum_q - (double)(0.58f * e.q_l2norm);
+                const float q1 = (float)((((double)tmp - cst) * (double)e.fconst) * (double)o);
+                if (ip) {
+                    f = (float)((double)q1 * 0.5);
+                } else {
+                    const float t = e.q_l2sqr + o2 - q1;
+                    f = (t < 0.0f) ? 0.0f : t;  // std::max(t, 0.0f)
+                }
+                const float ipo1 = orc_single_mask_ip(e.qr.data(), x, e.D);
+                const double ex = e.bits > 1 ? (double)orc_compute_ip(e.bits - 1, e.qr.data(), rec + L.seg_long_off[s], e.D)
+                                             : 0.0;
+                const double cd = 2.0 / (1 << e.bits);
+                const float tt = (float)((double)ipo1 + ex * cd + (-1.0 + cd / 2) * (double)e.sum_q);
+                const float ipoq = exf[2 * s] * tt;
+                a = ip ? ipoq + 0.0f : o2 + e.q_l2sqr - 2 * ipoq;
+            }
+            rv += v;
+            rf += f;
+            ra += a;
+        }
+        if (out_vars) out_vars[r] = rv;
+        if (out_fast) out_fast[r] = rf;
+        if (out_acc) out_acc[r] = ra;
+    }
+}
+
 }  // extern "C"

@@ -136,6 +136,33 @@ int orc_estimate(const orc_plan_t *plan, const float *variance, const float *con
                  const float *query, float m, int dist_type, uint32_t *out_ids, float *out_acc,
                  float *out_fast, float *out_vars, orc_metrics_t *metrics);
 
+/* ---- single-vector quantiser / estimator (SaqSingleDataWrapper records) ---- */
+/* Record layout of SaqSingleDataWrapper (quantization/single_data.hpp:180-246): 2 short
+ * factors per segment (o_l2norm, ip_cent_oa), padded to 64 bytes; the short codes of the
+ * segments with bits > 0, padded to 64 bytes; the long codes; one ExFactor per segment.
+ * bytes = SaqSingleDataWrapper::calculate_memory_size (:263-296). */
+typedef struct {
+    int64_t fac_off, code_off, long_off, exfac_off, bytes;
+    int64_t seg_code_off[ORC_MAX_SEG]; /* -1 for a 0-bit segment */
+    int64_t seg_long_off[ORC_MAX_SEG]; /* -1 below 2 bits */
+} orc_single_layout_t;
+
+void orc_single_layout(const orc_plan_t *plan, orc_single_layout_t *out);
+/* utils::new_transpose_bin with b_query = 8: tq[8 * blk + p] holds bit p of q[64 blk + k] at
+ * bit 63 - k. */
+void orc_single_transpose_bin(const uint16_t *q, int64_t D, uint64_t *tq);
+float orc_single_warmup_ip(const uint64_t *x, const uint64_t *tq, float delta, float vl, int64_t D);
+float orc_single_mask_ip(const float *q, const uint64_t *x, int64_t D);
+/* SAQuantizerSingle::quantize of n rows (num_dim floats each) into records of `stride` bytes
+ * (zeroed first). Returns the number of vectors whose adjustment hit CHECK_GE(ip, 0). */
+int64_t orc_single_quantize(const orc_plan_t *plan, const float *const *P, const float *data, int64_t n,
+                            const orc_caq_cfg_t *cfg, uint8_t *records, int64_t stride);
+/* SaqSingleEstimator<Any> for one query over nrec records: varsEstDist, compFastDist and
+ * compAccurateDist of each record. */
+void orc_single_estimate(const orc_plan_t *plan, const float *variance, const float *const *P,
+                         const uint8_t *records, int64_t nrec, int64_t stride, const float *query,
+                         float m, int dist_type, float *out_vars, float *out_fast, float *out_acc);
+
 #ifdef __cplusplus
 }
 #endif

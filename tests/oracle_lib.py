@@ -19,6 +19,7 @@ u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
 u16p = np.ctypeslib.ndpointer(np.uint16, flags="C_CONTIGUOUS")
 u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
 i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
 i64 = C.c_int64
 
 
@@ -53,6 +54,12 @@ class CodeInfo(C.Structure):
     _fields_ = [("o_l2norm", C.c_float), ("ip_cent_oa", C.c_float), ("fac_rescale", C.c_float),
                 ("fac_error", C.c_float), ("ip_o_oa", C.c_double), ("oa_l2sqr", C.c_double),
                 ("delta", C.c_double), ("v_mi", C.c_double), ("rounds", C.c_int32), ("status", C.c_int32)]
+
+
+class SingleLayout(C.Structure):
+    _fields_ = [("fac_off", C.c_int64), ("code_off", C.c_int64), ("long_off", C.c_int64),
+                ("exfac_off", C.c_int64), ("bytes", C.c_int64), ("seg_code_off", C.c_int64 * MAX_SEG),
+                ("seg_long_off", C.c_int64 * MAX_SEG)]
 
 
 class Metrics(C.Structure):
@@ -116,6 +123,16 @@ def oracle():
         L.orc_estimate.restype = C.c_int
         L.orc_estimate.argtypes = [C.POINTER(Plan), f32p, C.c_void_p, C.c_void_p, u32p, i64, f32p,
                                    C.c_float, C.c_int, u32p, f32p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_single_layout.argtypes = [C.POINTER(Plan), C.POINTER(SingleLayout)]
+        L.orc_single_transpose_bin.argtypes = [u16p, i64, u64p]
+        L.orc_single_warmup_ip.restype = C.c_float
+        L.orc_single_warmup_ip.argtypes = [u64p, u64p, C.c_float, C.c_float, i64]
+        L.orc_single_mask_ip.restype = C.c_float
+        L.orc_single_mask_ip.argtypes = [f32p, u64p, i64]
+        L.orc_single_quantize.restype = C.c_int64
+        L.orc_single_quantize.argtypes = [C.POINTER(Plan), C.c_void_p, f32p, i64, C.POINTER(CaqCfg), u8p, i64]
+        L.orc_single_estimate.argtypes = [C.POINTER(Plan), f32p, C.c_void_p, u8p, i64, i64, f32p, C.c_float,
+                                          C.c_int, f32p, f32p, f32p]
         _orc = L
     return _orc
 
@@ -153,6 +170,13 @@ def ref():
         L.ref_lut_delta.argtypes = [C.c_float, C.c_float]
         L.ref_code_floor.argtypes = [f32p, i64, C.c_double, C.c_double, C.c_int, f32p]
         L.ref_config_layout.argtypes = [C.c_void_p]
+        L.ref_new_transpose_bin.argtypes = [u16p, i64, u64p]
+        L.ref_warmup_ip_x0_q.restype = C.c_float
+        L.ref_warmup_ip_x0_q.argtypes = [u64p, u64p, C.c_float, C.c_float, i64]
+        L.ref_mask_ip_x0_q.restype = C.c_float
+        L.ref_mask_ip_x0_q.argtypes = [f32p, u64p, i64]
+        L.ref_single_delta.restype = C.c_float
+        L.ref_single_delta.argtypes = [C.c_float, C.c_float]
         _ref = L
     return _ref
 

@@ -9,7 +9,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from oracle_lib import CaqCfg, Cluster, Metrics, Sizes, make_plan, oracle
+from oracle_lib import CaqCfg, Cluster, Metrics, SingleLayout, Sizes, make_plan, oracle
 from saq_amd import data as sd
 
 
@@ -166,6 +166,36 @@ def orc_search(case, probes, topk, m=4.0, dist_type=0, nthreads=8):
                        dists.ctypes.data, C.cast(mets, C.c_void_p))
     met = np.array([[x.fast_bitsum, x.acc_bitsum, x.total_comp_cnt] for x in mets], np.uint64)
     return ids, dists, met
+
+
+def single_layout(case):
+    lay = SingleLayout()
+    oracle().orc_single_layout(C.byref(case.orc_plan()), C.byref(lay))
+    return lay
+
+
+def orc_single_quantize(case, data, stride=None):
+    """SAQuantizerSingle::quantize of every row of data into records of `stride` bytes."""
+    lay = single_layout(case)
+    stride = stride or (lay.bytes + 63) // 64 * 64
+    data = np.ascontiguousarray(data, np.float32)
+    n = data.shape[0]
+    recs = np.zeros(max(1, n) * stride, np.uint8)
+    cfg = CaqCfg(case.rd_lmt, case.eps, 1 if case.rotations[0] is not None else 0)
+    nfail = oracle().orc_single_quantize(C.byref(case.orc_plan()), C.cast(orc_rot_ptrs(case), C.c_void_p),
+                                         data if n else np.zeros(1, np.float32), n, C.byref(cfg), recs, stride)
+    return recs[:n * stride], stride, nfail
+
+
+def orc_single_estimate(case, recs, stride, query, m=4.0, dist_type=0):
+    nrec = len(recs) // stride
+    out = np.zeros((3, max(1, nrec)), np.float32)
+    vpad = np.zeros(case.D_pad, np.float32)
+    vpad[:case.num_dim] = case.variance
+    oracle().orc_single_estimate(C.byref(case.orc_plan()), vpad, C.cast(orc_rot_ptrs(case), C.c_void_p),
+                                 recs if nrec else np.zeros(1, np.uint8), nrec, stride,
+                                 np.ascontiguousarray(query, np.float32), m, dist_type, out[0], out[1], out[2])
+    return out[0, :nrec], out[1, :nrec], out[2, :nrec]
 
 
 def gpu_index(case, device=0):
