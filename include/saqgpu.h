@@ -146,6 +146,35 @@ int saqgpu_estimate(saqgpu_index *idx, const float *query, const uint32_t *probe
                     float vars_bound_m, int32_t dist_type, uint32_t *out_ids, float *out_acc,
                     float *out_fast, float *out_vars, int64_t *n_out);
 
+/* ---- single-vector records (SaqSingleDataWrapper, quantization/single_data.hpp:156-320) -----
+ * One record per vector, independent of the lists: per segment {o_l2norm, ip_cent_oa} padded to
+ * 64 bytes, the short codes of segments with bits > 0 (MSB-first, bytes of every 8 reversed)
+ * padded to 64 bytes, the CodeHelper<b-1> long codes, one ExFactor per segment.  The index
+ * supplies the SaqData (plan, rotations, variance); its lists need not be set.  Records of a
+ * batch sit record_stride bytes apart (>= the record size, a multiple of 8). */
+
+/* SaqSingleDataWrapper::calculate_memory_size (single_data.hpp:287-320) for the index's plan.
+ * offsets (may be NULL) receives 3 + 2 * nseg int64: short-code section, long-code section,
+ * ExFactor section, then per segment its short-code and long-code offset (-1 where none). */
+int saqgpu_single_layout(const saqgpu_index *idx, int64_t *record_bytes, int64_t *offsets);
+
+/* Replaces SAQuantizerSingle::quantize (quantization/saq_quantizer.hpp:78-94 ->
+ * QuantizerSingle::quantize, quantizer.hpp:105-207) for n rows of data (n x num_dim): each row
+ * becomes one record.  Host pointers unless flags has SAQGPU_DEVICE_PTRS.  Returns the number of
+ * vectors whose adjustment loop hit CHECK_GE(ip, 0) in *n_check_failed (may be NULL). */
+int saqgpu_single_quantize(saqgpu_index *idx, const float *data, int64_t n, const saqgpu_caq_cfg *cfg,
+                           uint8_t *records, int64_t record_stride, int32_t flags, int64_t *n_check_failed,
+                           void *stream);
+
+/* Replaces SaqSingleEstimator<Any> (quantization/saq_estimator.hpp:248-294 over
+ * CaqSingleEstimator, caq/caq_estimator.hpp:220-369, 450-497) for nq queries (nq x num_dim) and
+ * nrec records: out_vars / out_fast / out_acc[q * nrec + r] = varsEstDist / compFastDist /
+ * compAccurateDist of record r for query q.  out_vars and out_fast may be NULL.  Host pointers
+ * unless flags has SAQGPU_DEVICE_PTRS; returns when the results are written. */
+int saqgpu_single_estimate(saqgpu_index *idx, const float *queries, int64_t nq, const uint8_t *records,
+                           int64_t nrec, int64_t record_stride, float vars_bound_m, int32_t dist_type,
+                           float *out_vars, float *out_fast, float *out_acc, int32_t flags, void *stream);
+
 /* SaqDataMaker::analyze_plan (saqlib/quantization/saq_data.hpp:119-227): segment plan from the
  * per-dimension variance (num_dim floats).  dims/bits receive up to SAQGPU_MAX_SEG entries. */
 int saqgpu_make_plan(const float *variance, int32_t num_dim, float avg_bits, int32_t enable_segmentation,

@@ -77,6 +77,12 @@ _SIGS = {
                                         C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
     "saqgpu_autotune_batched": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.POINTER(BatchedParams),
                                           C.c_int32, C.c_void_p, C.POINTER(C.c_int32)]),
+    "saqgpu_single_layout": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.c_void_p]),
+    "saqgpu_single_quantize": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(CaqCfg), C.c_void_p,
+                                         C.c_int64, C.c_int32, C.c_void_p, C.c_void_p]),
+    "saqgpu_single_estimate": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_int64,
+                                         C.c_float, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32,
+                                         C.c_void_p]),
     "saqgpu_index_device_bytes": (C.c_int64, [C.c_void_p]),
     "saqgpu_timing": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
     "saqgpu_make_plan": (C.c_int, [C.c_void_p, C.c_int32, C.c_float, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
@@ -298,6 +304,48 @@ class Index:
                                      C.byref(n)))
         k = n.value
         return ids[:k], acc[:k], fast[:k], var[:k]
+
+    def single_layout(self):
+        """SaqSingleDataWrapper record size for this plan and its offsets: (bytes, code section,
+        long-code section, ExFactor section, [(short-code, long-code offset) per segment])."""
+        nb = C.c_int64(0)
+        off = np.zeros(3 + 2 * len(self.plan), np.int64)
+        _check(lib().saqgpu_single_layout(self.h, C.byref(nb), off.ctypes.data))
+        return (int(nb.value), int(off[0]), int(off[1]), int(off[2]),
+                [(int(off[3 + 2 * s]), int(off[4 + 2 * s])) for s in range(len(self.plan))])
+
+    def single_quantize(self, data, rd_lmt=6, eps=1e-8, stride=None, out=None, stream=None):
+        """SAQuantizerSingle::quantize of every row of data into SaqSingleDataWrapper records
+        `stride` bytes apart (default: the record size rounded up to 64).  Host ndarray in, uint8
+        ndarray out; or device tensors with `out` given.  Returns (records, stride, n_check_failed)."""
+        nbytes = self.single_layout()[0]
+        stride = stride or (nbytes + 63) // 64 * 64
+        dev = not isinstance(data, np.ndarray)
+        n = int(data.shape[0])
+        if not dev:
+            data = np.ascontiguousarray(data, np.float32)
+            out = np.zeros(max(1, n) * stride, np.uint8) if out is None else out
+        cfg = CaqCfg(rd_lmt, eps)
+        nfail = C.c_int64(0)
+        _check(lib().saqgpu_single_quantize(self.h, _ptr(data), n, C.byref(cfg), _ptr(out), stride,
+                                            DEVICE_PTRS if dev else 0, C.byref(nfail), stream))
+        return out, stride, int(nfail.value)
+
+    def single_estimate(self, queries, records, nrec, stride, m=4.0, dist_type=0, out=None, stream=None,
+                        want_all=True):
+        """SaqSingleEstimator for every (query, record): returns (vars, fast, acc), each nq x nrec
+        (vars/fast None when want_all is False).  Host ndarrays, or device tensors with `out` =
+        (vars or None, fast or None, acc) given."""
+        dev = not isinstance(queries, np.ndarray)
+        nq = int(queries.shape[0])
+        if not dev:
+            queries = np.ascontiguousarray(queries, np.float32)
+            shp = (nq, nrec)
+            out = tuple(np.zeros(shp, np.float32) if (want_all or i == 2) else None for i in range(3))
+        ov, of, oa = out
+        _check(lib().saqgpu_single_estimate(self.h, _ptr(queries), nq, _ptr(records), nrec, stride, m, dist_type,
+                                            _ptr(ov), _ptr(of), _ptr(oa), DEVICE_PTRS if dev else 0, stream))
+        return ov, of, oa
 
     def timing(self, enable):
         """Starts (enable=True) or stops event timing of the scan kernel; returns (ms, launches)

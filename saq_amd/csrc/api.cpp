@@ -48,6 +48,16 @@ hipError_t launch_encode_segment(const IndexView &ix, const float *data, const u
 hipError_t launch_caq_only(const IndexView &ix, const int64_t *gvec, const int64_t *gblk, const int32_t *lane,
                            const uint32_t *clu, const float *cent, int seg, int64_t nslots, int64_t ld, int rd_lmt,
                            float eps, float *O, uint16_t *code, int *fail, hipStream_t st);
+hipError_t launch_single_quantize_segment(const IndexView &ix, const float *data, const float *rot, int seg,
+                                          int seg_dim, int64_t nslots, int64_t ld, int rd_lmt, float eps, float *O,
+                                          uint16_t *code, int *fail, uint8_t *srec, int64_t sstride,
+                                          int32_t code_off, int32_t long_off, int32_t exf_off, hipStream_t st);
+struct SingleQ;
+size_t single_query_state_bytes();
+hipError_t launch_single_estimate(const IndexView &ix, const float *qrot, const float *bound, int64_t nq,
+                                  SingleQ *qs, uint64_t *planes, const uint8_t *recs, int64_t stride, int64_t nrec,
+                                  const int32_t *soff, int32_t exf_off, int32_t dist_type, float *out_vars,
+                                  float *out_fast, float *out_acc, hipStream_t st);
 }  // namespace saqgpu
 
 using namespace saqgpu;
@@ -1487,6 +1497,172 @@ int saqgpu_autotune_batched(saqgpu_index *ix, const float *queries, int64_t nq, 
     }
     ix->bq_tile = best_qt;
     if (best_query_tile) *best_query_tile = best_qt;
+    return SAQGPU_OK;
+}
+
+}  // extern "C"
+
+namespace {
+// SaqSingleDataWrapper's sections (single_data.hpp:180-285): factors, short codes (64-byte
+// padded), long codes, ExFactors; per segment its short- and long-code offsets (-1: none).
+struct SingleLayout {
+    int64_t code_off = 0, long_off = 0, exf_off = 0, bytes = 0;
+    std::vector<int64_t> seg_code, seg_long;
+};
+
+SingleLayout single_layout(const saqgpu_index *ix) {
+    SingleLayout L;
+    const auto r64 = [](int64_t x) { return (x + 63) / 64 * 64; };
+    int64_t code = 0, lng = 0;
+    for (int s = 0; s < ix->nseg; ++s) {
+        const int64_t D = ix->seg_dim[s], b = ix->seg_bits[s];
+        L.seg_code.push_back(b > 0 ? code : -1);
+        L.seg_long.push_back(b > 1 ? lng : -1);
+        if (b > 0) code += D / 8;
+        if (b > 1) lng += D * (b - 1) / 8;
+    }
+    L.code_off = r64((int64_t)ix->nseg * 8);
+    L.long_off = L.code_off + r64(code);
+    L.exf_off = L.long_off + lng;
+    L.bytes = L.exf_off + (int64_t)ix->nseg * 8;
+    for (int s = 0; s < ix->nseg; ++s) {
+        if (L.seg_code[s] >= 0) L.seg_code[s] += L.code_off;
+        if (L.seg_long[s] >= 0) L.seg_long[s] += L.long_off;
+    }
+    return L;
+}
+
+int check_stride(const SingleLayout &L, int64_t stride) {
+    if (stride < L.bytes || stride % 8) return fail(SAQGPU_EINVAL, "record_stride must be >= the record size and a multiple of 8");
+    return SAQGPU_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int saqgpu_single_layout(const saqgpu_index *ix, int64_t *record_bytes, int64_t *offsets) {
+    if (!ix || !record_bytes) return fail(SAQGPU_EINVAL, "null argument");
+    const SingleLayout L = single_layout(ix);
+    *record_bytes = L.bytes;
+    if (offsets) {
+        offsets[0] = L.code_off;
+        offsets[1] = L.long_off;
+        offsets[2] = L.exf_off;
+        for (int s = 0; s < ix->nseg; ++s) {
+            offsets[3 + 2 * s] = L.seg_code[s];
+            offsets[4 + 2 * s] = L.seg_long[s];
+        }
+    }
+    return SAQGPU_OK;
+}
+
+int saqgpu_single_quantize(saqgpu_index *ix, const float *data, int64_t n, const saqgpu_caq_cfg *cfg,
+                           uint8_t *records, int64_t record_stride, int32_t flags, int64_t *n_check_failed,
+                           void *stream) {
+    if (!ix || !cfg || n < 0 || (n > 0 && (!data || !records))) return fail(SAQGPU_EINVAL, "bad argument");
+    if (cfg->caq_adj_rd_lmt < 0) return fail(SAQGPU_EINVAL, "caq_adj_rd_lmt must be >= 0");
+    const SingleLayout L = single_layout(ix);
+    if (int rc = check_stride(L, record_stride)) return rc;
+    if (n_check_failed) *n_check_failed = 0;
+    if (n == 0) return SAQGPU_OK;
+    std::lock_guard<std::mutex> lk(ix->mu);
+    HIPCHK(hipSetDevice(ix->device));
+    hipStream_t st = (hipStream_t)stream;
+    const bool dev = flags & SAQGPU_DEVICE_PTRS;
+    DevBuf<float> d_data, d_O;
+    DevBuf<uint8_t> d_rec;
+    DevBuf<uint16_t> d_code;
+    DevBuf<int> d_fail;
+    const float *ddata = data;
+    uint8_t *drec = records;
+    if (!dev) {
+        if (d_data.ensure(n * ix->num_dim) || d_rec.ensure(n * record_stride)) return fail(SAQGPU_ENOMEM, "allocation failed");
+        HIPCHK(hipMemcpyAsync(d_data.p, data, n * ix->num_dim * 4, hipMemcpyHostToDevice, st));
+        ddata = d_data.p;
+        drec = d_rec.p;
+    }
+    // the reference writes only the fields a record uses; unused bytes read as zero here
+    HIPCHK(hipMemsetAsync(drec, 0, n * record_stride, st));
+    const int64_t chunk = std::min<int64_t>(n, 1 << 18);
+    if (d_O.ensure(chunk * ix->max_dim) || d_code.ensure(chunk * ix->max_dim) || d_fail.ensure(1))
+        return fail(SAQGPU_ENOMEM, "allocation failed");
+    HIPCHK(hipMemsetAsync(d_fail.p, 0, sizeof(int), st));
+    const IndexView v = ix->view();
+    for (int64_t off = 0; off < n; off += chunk) {
+        const int64_t cnt = std::min(chunk, n - off);
+        for (int s = 0; s < ix->nseg; ++s)
+            HIPCHK(launch_single_quantize_segment(
+                v, ddata + off * ix->num_dim, ix->has_rot ? ix->rot.p + ix->rot_off[s] : nullptr, s, ix->segs[s].dim,
+                cnt, chunk, cfg->caq_adj_rd_lmt, cfg->caq_adj_eps, d_O.p, d_code.p, d_fail.p,
+                drec + off * record_stride, record_stride, (int32_t)L.seg_code[s], (int32_t)L.seg_long[s],
+                (int32_t)L.exf_off, st));
+    }
+    int nf = 0;
+    HIPCHK(hipMemcpyAsync(&nf, d_fail.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    if (!dev) HIPCHK(hipMemcpyAsync(records, drec, n * record_stride, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (n_check_failed) *n_check_failed = nf;
+    return SAQGPU_OK;
+}
+
+int saqgpu_single_estimate(saqgpu_index *ix, const float *queries, int64_t nq, const uint8_t *records,
+                           int64_t nrec, int64_t record_stride, float vars_bound_m, int32_t dist_type,
+                           float *out_vars, float *out_fast, float *out_acc, int32_t flags, void *stream) {
+    if (!ix || nq < 0 || nrec < 0 || (nq > 0 && !queries) || (nrec > 0 && !records) || (nq * nrec > 0 && !out_acc))
+        return fail(SAQGPU_EINVAL, "bad argument");
+    if (dist_type != 0 && dist_type != 1) return fail(SAQGPU_EINVAL, "dist_type must be 0 (L2) or 1 (IP)");
+    const SingleLayout L = single_layout(ix);
+    if (int rc = check_stride(L, record_stride)) return rc;
+    if (nq == 0 || nrec == 0) return SAQGPU_OK;
+    if (nq * ((nrec + 15) / 16) > 0x7fffffffLL) return fail(SAQGPU_EUNSUPPORTED, "nq x nrec too large for one launch");
+    std::lock_guard<std::mutex> lk(ix->mu);
+    HIPCHK(hipSetDevice(ix->device));
+    hipStream_t st = (hipStream_t)stream;
+    const bool dev = flags & SAQGPU_DEVICE_PTRS;
+    const size_t smem = (size_t)ix->D_pad * 4 + (size_t)ix->D_pad / 64 * 64;
+    int max_smem = 0;
+    HIPCHK(hipDeviceGetAttribute(&max_smem, hipDeviceAttributeMaxSharedMemoryPerBlock, ix->device));
+    if ((int64_t)smem > max_smem) return fail(SAQGPU_EUNSUPPORTED, "dimension needs more LDS than a workgroup has");
+    std::vector<int32_t> soff(2 * ix->nseg);
+    for (int s = 0; s < ix->nseg; ++s) {
+        soff[2 * s] = (int32_t)L.seg_code[s];
+        soff[2 * s + 1] = (int32_t)L.seg_long[s];
+    }
+    DevBuf<int32_t> d_soff;
+    DevBuf<float> d_q, d_qrot, d_bound, d_out;
+    DevBuf<uint8_t> d_rec, d_qs;
+    DevBuf<uint64_t> d_pl;
+    int rc = 0;
+    if ((rc = upload(d_soff, soff))) return rc;
+    if (d_qrot.ensure(nq * ix->D_pad) || d_bound.ensure(nq * ix->nseg) ||
+        d_qs.ensure(nq * ix->nseg * single_query_state_bytes()) || d_pl.ensure(nq * (ix->D_pad / 64 * 8)))
+        return fail(SAQGPU_ENOMEM, "workspace allocation failed");
+    const float *dq = queries;
+    const uint8_t *drec = records;
+    float *ov = out_vars, *of = out_fast, *oa = out_acc;
+    if (!dev) {
+        if (d_q.ensure(nq * ix->num_dim) || d_rec.ensure(nrec * record_stride) || d_out.ensure(3 * nq * nrec))
+            return fail(SAQGPU_ENOMEM, "allocation failed");
+        HIPCHK(hipMemcpyAsync(d_q.p, queries, nq * ix->num_dim * 4, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(d_rec.p, records, nrec * record_stride, hipMemcpyHostToDevice, st));
+        dq = d_q.p;
+        drec = d_rec.p;
+        oa = d_out.p;
+        of = out_fast ? d_out.p + nq * nrec : nullptr;
+        ov = out_vars ? d_out.p + 2 * nq * nrec : nullptr;
+    }
+    // SaqEstimatorBase (saq_estimator.hpp:28-48): the rotated segment queries and their bounds
+    HIPCHK(launch_query_prep(ix->view(), dq, nq, ix->variance.p, ix->has_rot ? ix->rot.p : nullptr,
+                             ix->d_rot_off.p, vars_bound_m, d_qrot.p, d_bound.p, ix->max_dim, st));
+    HIPCHK(launch_single_estimate(ix->view(), d_qrot.p, d_bound.p, nq, reinterpret_cast<SingleQ *>(d_qs.p), d_pl.p,
+                                  drec, record_stride, nrec, d_soff.p, (int32_t)L.exf_off, dist_type, ov, of, oa,
+                                  st));
+    if (!dev) {
+        HIPCHK(hipMemcpyAsync(out_acc, oa, nq * nrec * 4, hipMemcpyDeviceToHost, st));
+        if (out_fast) HIPCHK(hipMemcpyAsync(out_fast, of, nq * nrec * 4, hipMemcpyDeviceToHost, st));
+        if (out_vars) HIPCHK(hipMemcpyAsync(out_vars, ov, nq * nrec * 4, hipMemcpyDeviceToHost, st));
+    }
+    HIPCHK(hipStreamSynchronize(st));
     return SAQGPU_OK;
 }
 

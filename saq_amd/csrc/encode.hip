@@ -38,6 +38,11 @@ struct EncArgs {
     float *O;              // [D_s][ld]
     uint16_t *code;        // [D_s][ld]
     int *fail;
+    // SaqSingleDataWrapper records instead of the index's lists (src, clu, gvec, gblk, lane and
+    // cent unused; slot t is row t and record t; no centroid)
+    uint8_t *srec;
+    int64_t sstride;
+    int32_t s_code_off, s_long_off, s_exf_off;
 };
 
 constexpr int kRotRows = 8;
@@ -62,7 +67,7 @@ __global__ __launch_bounds__(256) void rotate_kernel(EncArgs a) {
                 const int d = sd.doff + i0 + threadIdx.x;
                 float v = 0.0f;
                 if (r < a.nslots && i0 + (int)threadIdx.x < D && d < ix.num_dim)
-                    v = a.data[(int64_t)a.src[r] * ix.num_dim + d];
+                    v = a.data[(a.src ? (int64_t)a.src[r] : r) * ix.num_dim + d];
                 xs[t][threadIdx.x] = v;
             }
             __syncthreads();
@@ -90,8 +95,8 @@ __global__ __launch_bounds__(256) void rotate_kernel(EncArgs a) {
         if (r >= a.nslots) break;
         const int d = sd.doff + j;
         float x = acc[t];
-        if (!a.rot) x = d < ix.num_dim ? a.data[(int64_t)a.src[r] * ix.num_dim + d] : 0.0f;
-        a.O[(int64_t)j * a.ld + r] = x - ix.crot[(int64_t)a.clu[r] * ix.D_pad + d];
+        if (!a.rot) x = d < ix.num_dim ? a.data[(a.src ? (int64_t)a.src[r] : r) * ix.num_dim + d] : 0.0f;
+        a.O[(int64_t)j * a.ld + r] = a.srec ? x : x - ix.crot[(int64_t)a.clu[r] * ix.D_pad + d];
     }
 }
 
@@ -320,6 +325,34 @@ __global__ __launch_bounds__(256) void caq_kernel(EncArgs a) {
         v_mi *= sr;
         v_mx *= sr;
     }
+    if (a.srec) {  // QuantizerSingle::store_quantization_result (quantizer.hpp:134-207)
+        const uint32_t short_bit = bits ? 1u << (bits - 1) : 0u;
+        uint8_t *rec = a.srec + t * a.sstride;
+        float *fac = reinterpret_cast<float *>(rec) + 2 * a.seg;
+        fac[0] = (float)o_l2norm;
+        if (bits == 0) return;
+        fac[1] = 0.0f;  // ip_cent_oa: no centroid
+        float *ef = reinterpret_cast<float *>(rec + a.s_exf_off) + 2 * a.seg;
+        ef[0] = (float)fac_rescale;
+        ef[1] = (float)fac_error;
+        // MSB-first bytes, the bytes of every 8 reversed (:166-188): byte j of a group of 8 lands
+        // in byte 7 - j of a little-endian word
+        uint64_t *sc = reinterpret_cast<uint64_t *>(rec + a.s_code_off);
+        for (int g = 0; g < D / 64; ++g) {
+            uint64_t w = 0;
+            for (int b = 0; b < 8; ++b) {
+                uint32_t byte = 0;
+                for (int tt = 0; tt < 8; ++tt)
+                    byte |= ((code[(int64_t)(64 * g + 8 * b + tt) * ld] & short_bit) ? 0x80u : 0u) >> tt;
+                w |= (uint64_t)byte << (8 * (7 - b));
+            }
+            sc[g] = w;
+        }
+        if (bits > 1)
+            compact_code(bits - 1, D, rec + a.s_long_off,
+                         [&](int d) { return (uint32_t)code[(int64_t)d * ld] & (short_bit - 1u); });
+        return;
+    }
     const int64_t gv = a.gvec[t], gb = a.gblk[t];
     const int ln = a.lane[t];
     uint8_t *rec = const_cast<uint8_t *>(ix.blocks) + gb * ix.block_stride;
@@ -361,6 +394,18 @@ hipError_t launch_encode_segment(const IndexView &ix, const float *data, const u
                                  const float *rot, int seg, int seg_dim, int64_t nslots, int64_t ld, int rd_lmt,
                                  float eps, float *O, uint16_t *code, int *fail, hipStream_t st) {
     EncArgs a{ix, data, src, gvec, gblk, lane, clu, cent, rot, seg, nslots, ld, rd_lmt, eps, O, code, fail};
+    dim3 g1((unsigned)((nslots + kRotRows - 1) / kRotRows), (unsigned)((seg_dim + 255) / 256));
+    hipLaunchKernelGGL(rotate_kernel, g1, dim3(256), 0, st, a);
+    hipLaunchKernelGGL(caq_kernel, dim3((unsigned)((nslots + 255) / 256)), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_single_quantize_segment(const IndexView &ix, const float *data, const float *rot, int seg,
+                                          int seg_dim, int64_t nslots, int64_t ld, int rd_lmt, float eps, float *O,
+                                          uint16_t *code, int *fail, uint8_t *srec, int64_t sstride,
+                                          int32_t code_off, int32_t long_off, int32_t exf_off, hipStream_t st) {
+    EncArgs a{ix, data, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, rot, seg, nslots, ld, rd_lmt, eps,
+              O, code, fail, srec, sstride, code_off, long_off, exf_off};
     dim3 g1((unsigned)((nslots + kRotRows - 1) / kRotRows), (unsigned)((seg_dim + 255) / 256));
     hipLaunchKernelGGL(rotate_kernel, g1, dim3(256), 0, st, a);
     hipLaunchKernelGGL(caq_kernel, dim3((unsigned)((nslots + 255) / 256)), dim3(256), 0, st, a);

@@ -1,0 +1,215 @@
+// single.hip -- the single-vector estimator on gfx950: SaqSingleEstimator over records in the
+// SaqSingleDataWrapper layout (quantization/saq_estimator.hpp:248-294,
+// caq/caq_estimator.hpp:220-369, 450-497, quantization/single_data.hpp:156-320).
+//
+//   single_prep_kernel  one wavefront per (query, segment): CaqEstimatorSingleImpl::prepare on
+//                       the rotated segment query (:292-308): |q|^2 and sum_q in Eigen's redux
+//                       order, min/max, the 8-bit quantised query and its 8 bit planes per 64
+//                       dims (utils::new_transpose_bin, utils/space.hpp:266-293) taken by
+//                       ballots, bit 63 - k = dim k.
+//   single_est_kernel   a 16-lane group per (query, record), 16 records per workgroup: the
+//                       record's short code against the planes by popcount (warmup_ip_x0_q,
+//                       space.hpp:126-226), the masked sum of the query over the code's set bits
+//                       (mask_ip_x0_q, :228-265) and the extra-bit inner product
+//                       (CodeHelper<b-1>::compute_ip) on the 16-lane grouping the reference's
+//                       AVX-512 kernels use, then varsEstDist / compFastDist / compAccurateDist
+//                       summed over segments in order.
+#include <cfloat>
+
+#include "common.hpp"
+#include "estimators.hpp"
+
+#pragma clang fp contract(off)
+
+namespace saqgpu {
+
+// Per (query, segment) state of CaqEstimatorSingleImpl.
+struct SingleQ {
+    float q_l2sqr, q_l2norm, sum_q, delta, vlh, pad_[3];
+};
+
+struct SinglePrepArgs {
+    IndexView ix;
+    const float *qrot;  // nq x D_pad
+    int64_t nq;
+    SingleQ *qs;        // nq x nseg
+    uint64_t *planes;   // nq x (D_pad / 64 * 8)
+};
+
+// grid (nq, nseg), 64 threads
+__global__ __launch_bounds__(64) void single_prep_kernel(SinglePrepArgs a) {
+    const IndexView &ix = a.ix;
+    const int64_t q = blockIdx.x;
+    const int s = blockIdx.y;
+    const int lane = threadIdx.x;
+    const SegDesc sd = ix.segs[s];
+    const int D = sd.dim;
+    const float *qr = a.qrot + q * ix.D_pad + sd.doff;
+    const float l2 = eigen_redux_wave(D, [&](int i) { return qr[i] * qr[i]; });
+    const float sum = eigen_redux_wave(D, [&](int i) { return qr[i]; });
+    float vl = qr[lane], vr = qr[lane];
+    for (int i = lane + 64; i < D; i += 64) {
+        vl = fminf(vl, qr[i]);
+        vr = fmaxf(vr, qr[i]);
+    }
+    for (int o = 32; o >= 1; o >>= 1) {
+        vl = fminf(vl, __shfl_xor(vl, o));
+        vr = fmaxf(vr, __shfl_xor(vr, o));
+    }
+    // (q_vr_ - q_vl_) / ((1 << 8) - 0.01): a float difference divided in double, kept as float
+    const float delta = (float)((double)(vr - vl) / ((1 << 8) - 0.01));
+    uint64_t *pl = a.planes + q * (ix.D_pad / 64 * 8) + sd.doff / 64 * 8;
+    for (int b = 0; b < D / 64; ++b) {
+        // query_sq_ = ((q - vl) / delta).cast<uint16_t>(), as x86 truncates (NaN when delta is 0 -> 0)
+        const uint32_t v = cvt_u16((qr[64 * b + lane] - vl) / delta);
+#pragma unroll
+        for (int p = 0; p < 8; ++p) {
+            const uint64_t m = __builtin_bitreverse64(__ballot((v >> p) & 1u));
+            if (lane == p) pl[8 * b + p] = m;
+        }
+    }
+    if (lane == 0) {
+        SingleQ st;
+        st.q_l2sqr = l2;
+        st.q_l2norm = sqrtf(l2);
+        st.sum_q = sum;
+        st.delta = delta;
+        st.vlh = (float)((double)vl + 0.5 * (double)delta);  // q_vl_ + 0.5 * delta_ as a float argument
+        st.pad_[0] = st.pad_[1] = st.pad_[2] = 0.0f;
+        a.qs[q * ix.nseg + s] = st;
+    }
+}
+
+struct SingleEstArgs {
+    IndexView ix;
+    const float *qrot;       // nq x D_pad
+    const float *bound;      // nq x nseg
+    const SingleQ *qs;       // nq x nseg
+    const uint64_t *planes;  // nq x (D_pad / 64 * 8)
+    const uint8_t *recs;
+    int64_t stride, nrec, nq;
+    const int32_t *soff;     // per segment: short-code offset, long-code offset (-1: none)
+    int32_t exf_off;
+    int32_t dist_type;
+    float *out_vars, *out_fast, *out_acc;  // nq x nrec each (vars, fast may be null)
+};
+
+constexpr int kSRec = 16;  // records per workgroup (one per 16-lane group)
+
+// grid nq * ceil(nrec / 16), 256 threads; workgroup w takes query w % nq of record tile
+// w / nq, so the workgroups of one tile run back to back and its records come from HBM once and
+// from L2 for the other queries.
+__global__ __launch_bounds__(256) void single_est_kernel(SingleEstArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const IndexView &ix = a.ix;
+    const int64_t q = (int64_t)blockIdx.x % a.nq, tile = (int64_t)blockIdx.x / a.nq;
+    const int tid = threadIdx.x, grp = tid >> 4, gl = tid & 15;
+    float *qr = reinterpret_cast<float *>(smem);
+    uint64_t *pl = reinterpret_cast<uint64_t *>(smem + (size_t)ix.D_pad * 4);
+    const int npl = ix.D_pad / 64 * 8;
+    for (int i = tid; i < ix.D_pad; i += 256) qr[i] = a.qrot[q * ix.D_pad + i];
+    for (int i = tid; i < npl; i += 256) pl[i] = a.planes[q * npl + i];
+    __syncthreads();
+    const int64_t r = tile * kSRec + grp;
+    if (r >= a.nrec) return;
+    const uint8_t *rec = a.recs + r * a.stride;
+    const float *fac = reinterpret_cast<const float *>(rec);
+    const float *exf = reinterpret_cast<const float *>(rec + a.exf_off);
+    const bool ip = a.dist_type == 1;
+    float rv = 0.0f, rf = 0.0f, ra = 0.0f;
+    for (int s = 0; s < ix.nseg; ++s) {
+        const SegDesc sd = ix.segs[s];
+        const SingleQ st = a.qs[q * ix.nseg + s];
+        const float bound = a.bound[q * ix.nseg + s];
+        const float o = fac[2 * s];
+        float v;
+        if (ip) {
+            v = 0.0f - bound;  // ip_q_c_ - bound; ip_q_c_ stays 0 for single records
+        } else {
+            const float t = o * o + st.q_l2sqr - 2 * bound;
+            v = (0.0f < t) ? t : 0.0f;  // std::max(0.0f, t)
+        }
+        float f = v, acc;
+        const float o2 = o * o;
+        if (sd.bits == 0) {
+            acc = ip ? 0.0f : o2 + st.q_l2sqr;
+        } else {
+            const uint64_t *x = reinterpret_cast<const uint64_t *>(rec + a.soff[2 * s]);
+            const uint64_t *ps = pl + sd.doff / 64 * 8;
+            const float *qs = qr + sd.doff;
+            const int nb = sd.dim / 64;
+            // warmup_ip_x0_q: exact integer sums, any order
+            uint32_t ipi = 0, ppc = 0;
+            for (int b = gl; b < nb; b += 16) {
+                const uint64_t w = x[b];
+                ppc += (uint32_t)__popcll(w);
+#pragma unroll
+                for (int p = 0; p < 8; ++p) ipi += (uint32_t)__popcll(w & ps[8 * b + p]) << p;
+            }
+#pragma unroll
+            for (int o2x = 8; o2x >= 1; o2x >>= 1) {
+                ipi += __shfl_xor(ipi, o2x, 16);
+                ppc += __shfl_xor(ppc, o2x, 16);
+            }
+            const float tmp = (st.delta * (float)ipi) + (st.vlh * (float)ppc);
+            // (tmp - (0.5 * sum_q - 0.58f * |q|)) * (4 / 0.8f * 1/sqrt(D)) * |o|, in double
+            const double cst = 0.5 * (double)st.sum_q - (double)(0.58f * st.q_l2norm);
+            const float q1 = (float)((((double)tmp - cst) * (double)sd.rescale_sqrtD) * (double)o);
+            if (ip) {
+                f = (float)((double)q1 * 0.5);
+            } else {
+                const float t = st.q_l2sqr + o2 - q1;
+                f = (t < 0.0f) ? 0.0f : t;  // std::max(t, 0.0f)
+            }
+            // mask_ip_x0_q: lane gl takes dims 64 b + 16 m + gl; the release build adds a
+            // block's four masked values as (m0 + m1) + (m2 + m3)
+            float mk = 0.0f;
+            for (int b = 0; b < nb; ++b) {
+                const uint64_t w = x[b];
+                float mv[4];
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    const int k = 16 * m + gl;
+                    mv[m] = ((w >> (63 - k)) & 1u) ? qs[64 * b + k] : 0.0f;
+                }
+                mk = mk + ((mv[0] + mv[1]) + (mv[2] + mv[3]));
+            }
+            const float ipo1 = tree16_xor(mk);
+            const double ex = sd.bits > 1 ? (double)ex_ip_group(sd.bits - 1, qs, rec + a.soff[2 * s + 1], sd.dim, gl)
+                                          : 0.0;
+            const double cd = 2.0 / (1 << sd.bits);
+            const float tt = (float)((double)ipo1 + ex * cd + (-1.0 + cd / 2) * (double)st.sum_q);
+            const float ipoq = exf[2 * s] * tt;
+            acc = ip ? ipoq + 0.0f : o2 + st.q_l2sqr - 2 * ipoq;
+        }
+        rv += v;
+        rf += f;
+        ra += acc;
+    }
+    if (gl == 0) {
+        const int64_t i = q * a.nrec + r;
+        if (a.out_vars) a.out_vars[i] = rv;
+        if (a.out_fast) a.out_fast[i] = rf;
+        a.out_acc[i] = ra;
+    }
+}
+
+hipError_t launch_single_estimate(const IndexView &ix, const float *qrot, const float *bound, int64_t nq,
+                                  SingleQ *qs, uint64_t *planes, const uint8_t *recs, int64_t stride, int64_t nrec,
+                                  const int32_t *soff, int32_t exf_off, int32_t dist_type, float *out_vars,
+                                  float *out_fast, float *out_acc, hipStream_t st) {
+    if (nq == 0) return hipSuccess;
+    SinglePrepArgs p{ix, qrot, nq, qs, planes};
+    hipLaunchKernelGGL(single_prep_kernel, dim3((unsigned)nq, (unsigned)ix.nseg), dim3(64), 0, st, p);
+    if (nrec == 0) return hipGetLastError();
+    SingleEstArgs a{ix, qrot, bound, qs, planes, recs, stride, nrec, nq, soff, exf_off, dist_type,
+                    out_vars, out_fast, out_acc};
+    const size_t smem = (size_t)ix.D_pad * 4 + (size_t)ix.D_pad / 64 * 8 * 8;
+    hipLaunchKernelGGL(single_est_kernel, dim3((unsigned)(nq * ((nrec + kSRec - 1) / kSRec))), dim3(256), smem, st,
+                       a);
+    return hipGetLastError();
+}
+
+size_t single_query_state_bytes() { return sizeof(SingleQ); }
+
+}  // namespace saqgpu

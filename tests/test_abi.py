@@ -42,6 +42,10 @@ def test_bad_arguments_fail_without_a_device():
     bits[0] = 14
     dims[0] = 128
     assert L.saqgpu_index_create(C.byref(desc), C.byref(h)) == 1  # more than 13 bits
+    nb = C.c_int64(0)
+    assert L.saqgpu_single_layout(None, C.byref(nb), None) == 1
+    assert L.saqgpu_single_quantize(None, None, 0, None, None, 0, 0, None, None) == 1
+    assert L.saqgpu_single_estimate(None, None, 0, None, 0, 0, 4.0, 0, None, None, None, 0, None) == 1
 
 
 @pytest.mark.parametrize("d,avg", [(128, 4.0), (256, 1.5), (960, 4.0), (960, 1.5), (1024, 2.0), (1536, 1.5)])
