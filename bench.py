@@ -32,7 +32,8 @@ recall@10 is at least the plain scan's (with per-query nprobe: the same nprobe, 
 cheaper precision that reaches it), with the query tile its autotuner picks.  `value` is
 the batched scan's throughput; the plain scan's is in config.plain.  The JSON line also
 carries each scan kernel's roofline (algorithmic bytes / HIP-event kernel time vs 8 TB/s;
-int8 MFMA ops vs 5 POP/s for the batched kernel), the GPU encoder's throughput, and a CPU
+int8 MFMA ops vs 5 POP/s for the batched kernel), the GPU encoder's throughput, the
+single-vector records' quantiser and estimator rates (config.single_vector), and a CPU
 baseline: the oracle (CPU restatement of the reference) timed on this host's cores on a
 bounded sample of the same queries, whose IDs are also compared with the plain GPU scan's.
 """
@@ -241,6 +242,38 @@ def recall(found, gt, k):
     return hit / (len(gt) * k)
 
 
+def single_records(torch, ix, base, queries, dev, nrec, nqs, reps=3):
+    """The single-vector path (SAQuantizerSingle / SaqSingleEstimator) on nrec base rows and nqs
+    queries, device buffers throughout: quantiser throughput, and the estimator's pair rate and
+    its record bandwidth (each record is read from HBM once; the tile's other queries hit L2)."""
+    nbytes = ix.single_layout()[0]
+    stride = (nbytes + 63) // 64 * 64
+    data = base[:nrec].contiguous()
+    recs = torch.zeros(nrec * stride, dtype=torch.uint8, device=dev)
+    q = queries[:nqs].contiguous()
+    outs = tuple(torch.empty(nqs, nrec, dtype=torch.float32, device=dev) for _ in range(3))
+    ix.single_quantize(data, stride=stride, out=recs)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ix.single_quantize(data, stride=stride, out=recs)
+    torch.cuda.synchronize()
+    t_q = time.perf_counter() - t0
+    ix.single_estimate(q, recs, nrec, stride, out=outs)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ix.single_estimate(q, recs, nrec, stride, out=outs)
+    torch.cuda.synchronize()
+    t_e = (time.perf_counter() - t0) / reps
+    res = {"records": nrec, "record_bytes": nbytes, "queries": nqs,
+           "quantize_mvec_per_s": round(nrec / t_q / 1e6, 3), "estimate_ms": round(t_e * 1e3, 3),
+           "estimate_pairs_per_s": round(nqs * nrec / t_e, 1),
+           "estimate_record_gbs": round(nrec * nbytes / t_e / 1e9, 1)}
+    log(f"single-vector records: {res}")
+    del recs, outs
+    return res
+
+
 def cpu_baseline(ix, plan, queries_np, variance, rots, probes, npq, topk, m, gpu_ids, threads, min_seconds=10.0):
     """The oracle (CPU restatement of the reference, scalar C++) on this host's cores.  Only the
     lists the sample probes are downloaded from the GPU index."""
@@ -305,6 +338,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=3.0, help="duration of each serving-loop load")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving loop (kernel profiling runs)")
+    ap.add_argument("--no-single", action="store_true", help="skip the single-vector record measurement")
     ap.add_argument("--exact-probe", action="store_true",
                     help="batched scan with the reference-order coarse quantiser instead of the fp32-MFMA one")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -398,6 +432,7 @@ def main():
     torch.cuda.synchronize()
     t_enc = time.perf_counter() - t0
     log(f"encode {n / t_enc / 1e6:.2f} Mvec/s ({t_enc * 1e3:.0f} ms), check failures {nfail}")
+    single = None if args.no_single else single_records(torch, ix, base, queries, dev, min(n, 1 << 20), 16)
 
     # ---- plain nprobe
     if npq is not None:
@@ -580,7 +615,7 @@ def main():
                    "bits": bits, "plan": plan, "recall_at_10_by_bits": bits_tried,
                    "batched": batched, "plain": plain, "batched_vs_plain": round(bqps / qps, 3),
                    "serving": serving, "p99_ms": serving[-1]["p99_ms"] if serving else None,
-                   "encode_mvec_per_s": round(n / t_enc / 1e6, 3)},
+                   "encode_mvec_per_s": round(n / t_enc / 1e6, 3), "single_vector": single},
         "roofline": {"bound": "hbm", "achieved": round(b_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(b_gbs / HBM_PEAK_GBS, 4), "traffic": btraffic,
                      "kernel": "bscan_kernel", "alg_bytes_per_launch": b_bytes,

@@ -80,6 +80,8 @@ __global__ __launch_bounds__(64) void single_prep_kernel(SinglePrepArgs a) {
     }
 }
 
+constexpr size_t kSingleStageBytes = 48 * 1024;  // LDS of the staged form (records + one query)
+
 struct SingleEstArgs {
     IndexView ix;
     const float *qrot;       // nq x D_pad
@@ -95,24 +97,14 @@ struct SingleEstArgs {
 };
 
 constexpr int kSRec = 16;  // records per workgroup (one per 16-lane group)
+constexpr int kSQ = 16;    // queries per workgroup of the staged form
 
-// grid nq * ceil(nrec / 16), 256 threads; workgroup w takes query w % nq of record tile
-// w / nq, so the workgroups of one tile run back to back and its records come from HBM once and
-// from L2 for the other queries.
-__global__ __launch_bounds__(256) void single_est_kernel(SingleEstArgs a) {
-    extern __shared__ __align__(16) unsigned char smem[];
+// varsEstDist, compFastDist and compAccurateDist of record rec (any address space) for query q,
+// by the 16-lane group of lane gl, summed over segments; qr and pl are the query's rotated
+// vector and bit planes.  Valid in every lane of the group.
+__device__ __forceinline__ float3 single_pair(const SingleEstArgs &a, const float *qr, const uint64_t *pl,
+                                              const uint8_t *rec, int64_t q, int gl) {
     const IndexView &ix = a.ix;
-    const int64_t q = (int64_t)blockIdx.x % a.nq, tile = (int64_t)blockIdx.x / a.nq;
-    const int tid = threadIdx.x, grp = tid >> 4, gl = tid & 15;
-    float *qr = reinterpret_cast<float *>(smem);
-    uint64_t *pl = reinterpret_cast<uint64_t *>(smem + (size_t)ix.D_pad * 4);
-    const int npl = ix.D_pad / 64 * 8;
-    for (int i = tid; i < ix.D_pad; i += 256) qr[i] = a.qrot[q * ix.D_pad + i];
-    for (int i = tid; i < npl; i += 256) pl[i] = a.planes[q * npl + i];
-    __syncthreads();
-    const int64_t r = tile * kSRec + grp;
-    if (r >= a.nrec) return;
-    const uint8_t *rec = a.recs + r * a.stride;
     const float *fac = reinterpret_cast<const float *>(rec);
     const float *exf = reinterpret_cast<const float *>(rec + a.exf_off);
     const bool ip = a.dist_type == 1;
@@ -186,11 +178,70 @@ __global__ __launch_bounds__(256) void single_est_kernel(SingleEstArgs a) {
         rf += f;
         ra += acc;
     }
+    return make_float3(rv, rf, ra);
+}
+
+// grid nq * ceil(nrec / 16), 256 threads, records read in place (records too large to stage):
+// workgroup w takes query w % nq of record tile w / nq, so the workgroups of one tile run back
+// to back and its records come from HBM once and from L2 for the other queries.
+__global__ __launch_bounds__(256) void single_est_kernel(SingleEstArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const IndexView &ix = a.ix;
+    const int64_t q = (int64_t)blockIdx.x % a.nq, tile = (int64_t)blockIdx.x / a.nq;
+    const int tid = threadIdx.x, grp = tid >> 4, gl = tid & 15;
+    float *qr = reinterpret_cast<float *>(smem);
+    uint64_t *pl = reinterpret_cast<uint64_t *>(smem + (size_t)ix.D_pad * 4);
+    const int npl = ix.D_pad / 64 * 8;
+    for (int i = tid; i < ix.D_pad; i += 256) qr[i] = a.qrot[q * ix.D_pad + i];
+    for (int i = tid; i < npl; i += 256) pl[i] = a.planes[q * npl + i];
+    __syncthreads();
+    const int64_t r = tile * kSRec + grp;
+    if (r >= a.nrec) return;
+    const float3 e = single_pair(a, qr, pl, a.recs + r * a.stride, q, gl);
     if (gl == 0) {
         const int64_t i = q * a.nrec + r;
-        if (a.out_vars) a.out_vars[i] = rv;
-        if (a.out_fast) a.out_fast[i] = rf;
-        a.out_acc[i] = ra;
+        if (a.out_vars) a.out_vars[i] = e.x;
+        if (a.out_fast) a.out_fast[i] = e.y;
+        a.out_acc[i] = e.z;
+    }
+}
+
+// grid ceil(nrec / 16) * ceil(nq / 16), 256 threads: the workgroup copies its 16 records to LDS
+// once and takes up to 16 queries over them, each query's rotated vector and planes staged in
+// turn; the record tile varies slowest so a tile's workgroups run back to back.
+__global__ __launch_bounds__(256) void single_est_tile_kernel(SingleEstArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const IndexView &ix = a.ix;
+    const int64_t nqb = (a.nq + kSQ - 1) / kSQ;
+    const int64_t qb = (int64_t)blockIdx.x % nqb, tile = (int64_t)blockIdx.x / nqb;
+    const int tid = threadIdx.x, grp = tid >> 4, gl = tid & 15;
+    const int64_t r0 = tile * kSRec;
+    const int nr = (int)min<int64_t>(kSRec, a.nrec - r0);
+    uint8_t *lrec = smem;
+    float *qr = reinterpret_cast<float *>(smem + (size_t)kSRec * a.stride);
+    uint64_t *pl = reinterpret_cast<uint64_t *>(smem + (size_t)kSRec * a.stride + (size_t)ix.D_pad * 4);
+    const int npl = ix.D_pad / 64 * 8;
+    {  // the tile's records, 8 bytes a lane (stride is a multiple of 8)
+        const uint2 *src = reinterpret_cast<const uint2 *>(a.recs + r0 * a.stride);
+        uint2 *dst = reinterpret_cast<uint2 *>(lrec);
+        const int nw = (int)(nr * a.stride / 8);
+        for (int i = tid; i < nw; i += 256) dst[i] = src[i];
+    }
+    const int64_t qend = min<int64_t>(a.nq, (qb + 1) * kSQ);
+    for (int64_t q = qb * kSQ; q < qend; ++q) {
+        __syncthreads();
+        for (int i = tid; i < ix.D_pad; i += 256) qr[i] = a.qrot[q * ix.D_pad + i];
+        for (int i = tid; i < npl; i += 256) pl[i] = a.planes[q * npl + i];
+        __syncthreads();
+        if (grp < nr) {
+            const float3 e = single_pair(a, qr, pl, lrec + (size_t)grp * a.stride, q, gl);
+            if (gl == 0) {
+                const int64_t i = q * a.nrec + r0 + grp;
+                if (a.out_vars) a.out_vars[i] = e.x;
+                if (a.out_fast) a.out_fast[i] = e.y;
+                a.out_acc[i] = e.z;
+            }
+        }
     }
 }
 
@@ -204,9 +255,14 @@ hipError_t launch_single_estimate(const IndexView &ix, const float *qrot, const 
     if (nrec == 0) return hipGetLastError();
     SingleEstArgs a{ix, qrot, bound, qs, planes, recs, stride, nrec, nq, soff, exf_off, dist_type,
                     out_vars, out_fast, out_acc};
-    const size_t smem = (size_t)ix.D_pad * 4 + (size_t)ix.D_pad / 64 * 8 * 8;
-    hipLaunchKernelGGL(single_est_kernel, dim3((unsigned)(nq * ((nrec + kSRec - 1) / kSRec))), dim3(256), smem, st,
-                       a);
+    const size_t qbytes = (size_t)ix.D_pad * 4 + (size_t)ix.D_pad / 64 * 8 * 8;
+    const int64_t ntiles = (nrec + kSRec - 1) / kSRec;
+    if ((size_t)kSRec * stride + qbytes <= kSingleStageBytes) {
+        hipLaunchKernelGGL(single_est_tile_kernel, dim3((unsigned)(ntiles * ((nq + kSQ - 1) / kSQ))), dim3(256),
+                           (size_t)kSRec * stride + qbytes, st, a);
+    } else {
+        hipLaunchKernelGGL(single_est_kernel, dim3((unsigned)(nq * ntiles)), dim3(256), qbytes, st, a);
+    }
     return hipGetLastError();
 }
 

@@ -9,10 +9,11 @@ import saq_fixtures as F
 pytestmark = pytest.mark.gpu
 
 PLANS = [
-    (200, None, True),                           # the DP plan, ragged num_dim, rotated
-    (256, [(64, 0), (64, 1), (128, 5)], True),   # 0-bit, 1-bit and a 5-bit (IP4 + IP1) segment
-    (128, [(128, 9)], False),                    # 9 bits (IP8 + 256 IP1), no rotation
-    (256, [(64, 3), (192, 7)], True),
+    (200, None, True, 300),                           # the DP plan, ragged num_dim, rotated
+    (256, [(64, 0), (64, 1), (128, 5)], True, 300),   # 0-bit, 1-bit and a 5-bit (IP4 + IP1) segment
+    (128, [(128, 9)], False, 300),                    # 9 bits (IP8 + 256 IP1), no rotation
+    (256, [(64, 3), (192, 7)], True, 300),
+    (1536, [(1536, 13)], True, 100),                  # records too large to stage: read in place
 ]
 
 
@@ -20,9 +21,9 @@ def _case(d, plan, rotate, n=300, seed=7):
     return F.make_case(n=n, d=d, K=4, nq=5, plan=plan, avg_bits=4.0, seed=seed, rotate=rotate, encode=False)
 
 
-@pytest.mark.parametrize("d,plan,rotate", PLANS)
-def test_single_records_match_oracle(d, plan, rotate):
-    case = _case(d, plan, rotate)
+@pytest.mark.parametrize("d,plan,rotate,n", PLANS)
+def test_single_records_match_oracle(d, plan, rotate, n):
+    case = _case(d, plan, rotate, n=n)
     ix = F.gpu_index(case)
     try:
         nbytes, code_off, long_off, exf_off, segs = ix.single_layout()
