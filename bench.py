@@ -242,10 +242,12 @@ def recall(found, gt, k):
     return hit / (len(gt) * k)
 
 
-def single_records(torch, ix, base, queries, dev, nrec, nqs, reps=3):
+def single_records(torch, ix, base, queries, dev, nrec, nqs, reps=3, cpu=None):
     """The single-vector path (SAQuantizerSingle / SaqSingleEstimator) on nrec base rows and nqs
     queries, device buffers throughout: quantiser throughput, and the estimator's pair rate and
-    its record bandwidth (each record is read from HBM once; the tile's other queries hit L2)."""
+    its record bandwidth (each record is read from HBM once; the tile's other queries hit L2).
+    cpu = (plan, rotations, variance): also time the oracle's estimator (one thread) on a bounded
+    sample of the same records and queries and compare its estimates with the GPU's."""
     nbytes = ix.single_layout()[0]
     stride = (nbytes + 63) // 64 * 64
     data = base[:nrec].contiguous()
@@ -269,6 +271,33 @@ def single_records(torch, ix, base, queries, dev, nrec, nqs, reps=3):
            "quantize_mvec_per_s": round(nrec / t_q / 1e6, 3), "estimate_ms": round(t_e * 1e3, 3),
            "estimate_pairs_per_s": round(nqs * nrec / t_e, 1),
            "estimate_record_gbs": round(nrec * nbytes / t_e / 1e9, 1)}
+    if cpu is not None:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import ctypes as C
+        from oracle_lib import make_plan, oracle
+        plan, rots, variance = cpu
+        O = oracle()
+        ns = min(nrec, 20000)
+        host = recs[:ns * stride].cpu().numpy()
+        qs = q.cpu().numpy()
+        p = make_plan([x for x, _ in plan], [b for _, b in plan], qs.shape[1])
+        rp = (C.c_void_p * len(plan))(*[r.ctypes.data for r in rots])
+        vpad = np.zeros(sum(x for x, _ in plan), np.float32)
+        vpad[:len(variance)] = variance
+        got = [o[:, :ns].cpu().numpy() for o in outs]
+        wv, wf, wa = (np.zeros(ns, np.float32) for _ in range(3))
+        pairs, dt, mism = 0, 0.0, 0
+        while dt < 3.0 and pairs < nqs * ns:
+            qi = (pairs // ns) % nqs
+            t0 = time.perf_counter()
+            O.orc_single_estimate(C.byref(p), vpad, rp, host, ns, stride, np.ascontiguousarray(qs[qi]), 4.0, 0,
+                                  wv, wf, wa)
+            dt += time.perf_counter() - t0
+            pairs += ns
+            mism += int(np.sum((wa != got[2][qi]) | (wf != got[1][qi]) | (wv != got[0][qi])))
+        res["cpu_baseline"] = {"pairs_per_s": round(pairs / dt, 1), "cores": 1, "kind": "port",
+                               "sample": f"{pairs} (query, record) pairs: {ns} records x {pairs // ns} queries, "
+                                         f"{dt:.1f}s", "estimates_differing_from_gpu": mism}
     log(f"single-vector records: {res}")
     del recs, outs
     return res
@@ -432,7 +461,8 @@ def main():
     torch.cuda.synchronize()
     t_enc = time.perf_counter() - t0
     log(f"encode {n / t_enc / 1e6:.2f} Mvec/s ({t_enc * 1e3:.0f} ms), check failures {nfail}")
-    single = None if args.no_single else single_records(torch, ix, base, queries, dev, min(n, 1 << 20), 16)
+    single = None if args.no_single else single_records(torch, ix, base, queries, dev, min(n, 1 << 20), 16,
+                                                         cpu=None if args.no_cpu else (plan, rots, var))
 
     # ---- plain nprobe
     if npq is not None:

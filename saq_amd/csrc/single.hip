@@ -80,7 +80,7 @@ __global__ __launch_bounds__(64) void single_prep_kernel(SinglePrepArgs a) {
     }
 }
 
-constexpr size_t kSingleStageBytes = 48 * 1024;  // LDS of the staged form (records + one query)
+constexpr size_t kSingleStageBytes = 48 * 1024;  // LDS of the staged form (records + 4 queries)
 
 struct SingleEstArgs {
     IndexView ix;
@@ -206,9 +206,195 @@ __global__ __launch_bounds__(256) void single_est_kernel(SingleEstArgs a) {
     }
 }
 
+constexpr int kQN = 4;  // queries evaluated together per record pass of the staged form
+
+template <int N>
+struct FQ {
+    float v[N];
+};
+
+// lane_acc_k (the 16-lane accumulator of CodeHelper<K>::compute_ip) for N queries at once: each
+// dim's code is decoded once and fed to every query's own FMA chain, in the same order.
+template <int K, int N>
+__device__ __forceinline__ FQ<N> lane_acc_n(const float *r, int rstride, const uint8_t *y, int D, int gl) {
+    FQ<N> acc;
+#pragma unroll
+    for (int t = 0; t < N; ++t) acc.v[t] = 0.0f;
+    const int n = D / 16;  // a multiple of 4
+    for (int i0 = 0; i0 < n; i0 += 4) {
+        int c[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) c[u] = lane_code<K>(y, i0 + u, gl);
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int t = 0; t < N; ++t) {
+                const float x = r[t * rstride + 16 * (i0 + u) + gl];
+                acc.v[t] = K == 1 ? acc.v[t] + (c[u] ? x : 0.0f) : __fmaf_rn(x, (float)c[u], acc.v[t]);
+            }
+    }
+    return acc;
+}
+
+template <int N>
+__device__ __forceinline__ FQ<N> lane_acc_nk(int k, const float *r, int rstride, const uint8_t *y, int D, int gl) {
+    switch (k) {
+    case 1: return lane_acc_n<1, N>(r, rstride, y, D, gl);
+    case 2: return lane_acc_n<2, N>(r, rstride, y, D, gl);
+    case 3: return lane_acc_n<3, N>(r, rstride, y, D, gl);
+    case 4: return lane_acc_n<4, N>(r, rstride, y, D, gl);
+    case 6: return lane_acc_n<6, N>(r, rstride, y, D, gl);
+    case 7: return lane_acc_n<7, N>(r, rstride, y, D, gl);
+    default: return lane_acc_n<8, N>(r, rstride, y, D, gl);
+    }
+}
+
+// ex_ip_group for N queries (query t's segment vector at r + t * rstride): k = 5 is
+// 2 * IP4 + IP1, k > 8 is IP8 + 256 * IP_{k-8}, each part reduced by the 16-lane tree.
+template <int N>
+__device__ __forceinline__ FQ<N> ex_ip_n(int k, const float *r, int rstride, const uint8_t *y, int D, int gl) {
+    FQ<N> out;
+    const int klo = k > 8 ? 8 : k;
+    if (klo == 5) {
+        const FQ<N> a4 = lane_acc_nk<N>(4, r, rstride, y, D, gl);
+        const FQ<N> a1 = lane_acc_nk<N>(1, r, rstride, y + D * 4 / 8, D, gl);
+#pragma unroll
+        for (int t = 0; t < N; ++t) out.v[t] = 2 * tree16_xor(a4.v[t]) + tree16_xor(a1.v[t]);
+    } else {
+        const FQ<N> a = lane_acc_nk<N>(klo, r, rstride, y, D, gl);
+#pragma unroll
+        for (int t = 0; t < N; ++t) out.v[t] = tree16_xor(a.v[t]);
+    }
+    if (k > 8) {
+        const int kh = k - 8;
+        FQ<N> hi;
+        if (kh == 5) {
+            const FQ<N> a4 = lane_acc_nk<N>(4, r, rstride, y + D, D, gl);
+            const FQ<N> a1 = lane_acc_nk<N>(1, r, rstride, y + D + D * 4 / 8, D, gl);
+#pragma unroll
+            for (int t = 0; t < N; ++t) hi.v[t] = 2 * tree16_xor(a4.v[t]) + tree16_xor(a1.v[t]);
+        } else {
+            const FQ<N> a = lane_acc_nk<N>(kh, r, rstride, y + D, D, gl);
+#pragma unroll
+            for (int t = 0; t < N; ++t) hi.v[t] = tree16_xor(a.v[t]);
+        }
+#pragma unroll
+        for (int t = 0; t < N; ++t) out.v[t] = out.v[t] + 256 * hi.v[t];
+    }
+    return out;
+}
+
+// single_pair for kQN queries q0 .. q0 + kQN - 1 (rows t of qr / pl, vectors D_pad and planes
+// npl apart): the record's words and codes are read once for all of them.
+struct Est3 {
+    float v[kQN], f[kQN], a[kQN];
+};
+
+__device__ __forceinline__ Est3 single_pair_n(const SingleEstArgs &a, const float *qr, const uint64_t *pl, int npl,
+                                             const uint8_t *rec, int64_t q0, int gl) {
+    const IndexView &ix = a.ix;
+    const float *fac = reinterpret_cast<const float *>(rec);
+    const float *exf = reinterpret_cast<const float *>(rec + a.exf_off);
+    const bool ip = a.dist_type == 1;
+    Est3 r;
+#pragma unroll
+    for (int t = 0; t < kQN; ++t) r.v[t] = r.f[t] = r.a[t] = 0.0f;
+    for (int s = 0; s < ix.nseg; ++s) {
+        const SegDesc sd = ix.segs[s];
+        const float o = fac[2 * s];
+        const float o2 = o * o;
+        SingleQ st[kQN];
+        float v[kQN];
+#pragma unroll
+        for (int t = 0; t < kQN; ++t) {
+            const int64_t q = min<int64_t>(q0 + t, a.nq - 1);
+            st[t] = a.qs[q * ix.nseg + s];
+            const float bound = a.bound[q * ix.nseg + s];
+            if (ip) {
+                v[t] = 0.0f - bound;
+            } else {
+                const float x = o * o + st[t].q_l2sqr - 2 * bound;
+                v[t] = (0.0f < x) ? x : 0.0f;
+            }
+        }
+        if (sd.bits == 0) {
+#pragma unroll
+            for (int t = 0; t < kQN; ++t) {
+                r.v[t] += v[t];
+                r.f[t] += v[t];
+                r.a[t] += ip ? 0.0f : o2 + st[t].q_l2sqr;
+            }
+            continue;
+        }
+        const uint64_t *x = reinterpret_cast<const uint64_t *>(rec + a.soff[2 * s]);
+        const int nb = sd.dim / 64;
+        const int pof = sd.doff / 64 * 8;
+        uint32_t ipi[kQN], ppc = 0;
+#pragma unroll
+        for (int t = 0; t < kQN; ++t) ipi[t] = 0;
+        for (int b = gl; b < nb; b += 16) {
+            const uint64_t w = x[b];
+            ppc += (uint32_t)__popcll(w);
+#pragma unroll
+            for (int t = 0; t < kQN; ++t)
+#pragma unroll
+                for (int p = 0; p < 8; ++p) ipi[t] += (uint32_t)__popcll(w & pl[t * npl + pof + 8 * b + p]) << p;
+        }
+#pragma unroll
+        for (int o2x = 8; o2x >= 1; o2x >>= 1) {
+            ppc += __shfl_xor(ppc, o2x, 16);
+#pragma unroll
+            for (int t = 0; t < kQN; ++t) ipi[t] += __shfl_xor(ipi[t], o2x, 16);
+        }
+        float mk[kQN];
+#pragma unroll
+        for (int t = 0; t < kQN; ++t) mk[t] = 0.0f;
+        const float *qs = qr + sd.doff;
+        for (int b = 0; b < nb; ++b) {
+            const uint64_t w = x[b];
+            bool on[4];
+#pragma unroll
+            for (int m = 0; m < 4; ++m) on[m] = (w >> (63 - (16 * m + gl))) & 1u;
+#pragma unroll
+            for (int t = 0; t < kQN; ++t) {
+                float mv[4];
+#pragma unroll
+                for (int m = 0; m < 4; ++m) mv[m] = on[m] ? qs[t * ix.D_pad + 64 * b + 16 * m + gl] : 0.0f;
+                mk[t] = mk[t] + ((mv[0] + mv[1]) + (mv[2] + mv[3]));
+            }
+        }
+        FQ<kQN> ex;
+        if (sd.bits > 1)
+            ex = ex_ip_n<kQN>(sd.bits - 1, qs, ix.D_pad, rec + a.soff[2 * s + 1], sd.dim, gl);
+        const double cd = 2.0 / (1 << sd.bits);
+#pragma unroll
+        for (int t = 0; t < kQN; ++t) {
+            const float tmp = (st[t].delta * (float)ipi[t]) + (st[t].vlh * (float)ppc);
+            const double cst = 0.5 * (double)st[t].sum_q - (double)(0.58f * st[t].q_l2norm);
+            const float q1 = (float)((((double)tmp - cst) * (double)sd.rescale_sqrtD) * (double)o);
+            float f;
+            if (ip) {
+                f = (float)((double)q1 * 0.5);
+            } else {
+                const float y = st[t].q_l2sqr + o2 - q1;
+                f = (y < 0.0f) ? 0.0f : y;
+            }
+            const float ipo1 = tree16_xor(mk[t]);
+            const double e = sd.bits > 1 ? (double)ex.v[t] : 0.0;
+            const float tt = (float)((double)ipo1 + e * cd + (-1.0 + cd / 2) * (double)st[t].sum_q);
+            const float ipoq = exf[2 * s] * tt;
+            r.v[t] += v[t];
+            r.f[t] += f;
+            r.a[t] += ip ? ipoq + 0.0f : o2 + st[t].q_l2sqr - 2 * ipoq;
+        }
+    }
+    return r;
+}
+
 // grid ceil(nrec / 16) * ceil(nq / 16), 256 threads: the workgroup copies its 16 records to LDS
-// once and takes up to 16 queries over them, each query's rotated vector and planes staged in
-// turn; the record tile varies slowest so a tile's workgroups run back to back.
+// once and takes up to 16 queries over them, 4 at a time (their rotated vectors and planes
+// staged together, every record's codes decoded once for the 4); the record tile varies slowest
+// so a tile's workgroups run back to back.
 __global__ __launch_bounds__(256) void single_est_tile_kernel(SingleEstArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     const IndexView &ix = a.ix;
@@ -217,10 +403,10 @@ __global__ __launch_bounds__(256) void single_est_tile_kernel(SingleEstArgs a) {
     const int tid = threadIdx.x, grp = tid >> 4, gl = tid & 15;
     const int64_t r0 = tile * kSRec;
     const int nr = (int)min<int64_t>(kSRec, a.nrec - r0);
+    const int npl = ix.D_pad / 64 * 8;
     uint8_t *lrec = smem;
     float *qr = reinterpret_cast<float *>(smem + (size_t)kSRec * a.stride);
-    uint64_t *pl = reinterpret_cast<uint64_t *>(smem + (size_t)kSRec * a.stride + (size_t)ix.D_pad * 4);
-    const int npl = ix.D_pad / 64 * 8;
+    uint64_t *pl = reinterpret_cast<uint64_t *>(smem + (size_t)kSRec * a.stride + (size_t)kQN * ix.D_pad * 4);
     {  // the tile's records, 8 bytes a lane (stride is a multiple of 8)
         const uint2 *src = reinterpret_cast<const uint2 *>(a.recs + r0 * a.stride);
         uint2 *dst = reinterpret_cast<uint2 *>(lrec);
@@ -228,19 +414,25 @@ __global__ __launch_bounds__(256) void single_est_tile_kernel(SingleEstArgs a) {
         for (int i = tid; i < nw; i += 256) dst[i] = src[i];
     }
     const int64_t qend = min<int64_t>(a.nq, (qb + 1) * kSQ);
-    for (int64_t q = qb * kSQ; q < qend; ++q) {
+    for (int64_t q0 = qb * kSQ; q0 < qend; q0 += kQN) {
         __syncthreads();
-        for (int i = tid; i < ix.D_pad; i += 256) qr[i] = a.qrot[q * ix.D_pad + i];
-        for (int i = tid; i < npl; i += 256) pl[i] = a.planes[q * npl + i];
+        for (int t = 0; t < kQN; ++t) {  // queries past nq repeat the last one (not written)
+            const int64_t q = min<int64_t>(q0 + t, a.nq - 1);
+            for (int i = tid; i < ix.D_pad; i += 256) qr[t * ix.D_pad + i] = a.qrot[q * ix.D_pad + i];
+            for (int i = tid; i < npl; i += 256) pl[t * npl + i] = a.planes[q * npl + i];
+        }
         __syncthreads();
         if (grp < nr) {
-            const float3 e = single_pair(a, qr, pl, lrec + (size_t)grp * a.stride, q, gl);
-            if (gl == 0) {
-                const int64_t i = q * a.nrec + r0 + grp;
-                if (a.out_vars) a.out_vars[i] = e.x;
-                if (a.out_fast) a.out_fast[i] = e.y;
-                a.out_acc[i] = e.z;
-            }
+            const Est3 e = single_pair_n(a, qr, pl, npl, lrec + (size_t)grp * a.stride, q0, gl);
+            if (gl == 0)
+#pragma unroll
+                for (int t = 0; t < kQN; ++t) {
+                    if (q0 + t >= qend) break;
+                    const int64_t i = (q0 + t) * a.nrec + r0 + grp;
+                    if (a.out_vars) a.out_vars[i] = e.v[t];
+                    if (a.out_fast) a.out_fast[i] = e.f[t];
+                    a.out_acc[i] = e.a[t];
+                }
         }
     }
 }
@@ -257,9 +449,9 @@ hipError_t launch_single_estimate(const IndexView &ix, const float *qrot, const 
                     out_vars, out_fast, out_acc};
     const size_t qbytes = (size_t)ix.D_pad * 4 + (size_t)ix.D_pad / 64 * 8 * 8;
     const int64_t ntiles = (nrec + kSRec - 1) / kSRec;
-    if ((size_t)kSRec * stride + qbytes <= kSingleStageBytes) {
+    if ((size_t)kSRec * stride + kQN * qbytes <= kSingleStageBytes) {
         hipLaunchKernelGGL(single_est_tile_kernel, dim3((unsigned)(ntiles * ((nq + kSQ - 1) / kSQ))), dim3(256),
-                           (size_t)kSRec * stride + qbytes, st, a);
+                           (size_t)kSRec * stride + kQN * qbytes, st, a);
     } else {
         hipLaunchKernelGGL(single_est_kernel, dim3((unsigned)(nq * ntiles)), dim3(256), qbytes, st, a);
     }
