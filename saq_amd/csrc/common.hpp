@@ -81,6 +81,8 @@ __device__ __forceinline__ float eigen_redux_wave(int n, F elem) {
     float v = 0.0f;
     if (n >= 32) {
         v = elem(t);
+        // unrolled so that elements read from global memory are loaded eight at a time
+#pragma unroll 8
         for (int idx = 32; idx < aligned2; idx += 32) v = v + elem(idx + t);
     } else if (t < 16) {
         v = elem(t);

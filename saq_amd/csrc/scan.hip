@@ -2,8 +2,9 @@
 //
 // Reproduces IVF::search (saqlib/index/ivf.hpp:275-311) for a batch of queries:
 //   query_prep_kernel  per (query, segment): the rotated query q_s * P_s in the order of
-//                      Eigen's column-major GEMV and the variance bound
-//                      m * sqrt(sum var * q^2) (saq_estimator.hpp:28-48, caq_estimator.hpp:56-82)
+//                      Eigen's column-major GEMV (saq_estimator.hpp:28-48, caq_estimator.hpp:56-82)
+//   query_bound_kernel per (query, segment), one wavefront: the variance bound
+//                      m * sqrt(sum var * q^2) (caq_estimator.hpp:80-82)
 //   scan_kernel        one workgroup per query walks its probe list in order.  Per probed
 //                      cluster it builds every segment's u16 fast-scan LUT in LDS
 //                      (Lut::prepare, lut.hpp:60-87), then streams the cluster in steps of
@@ -48,11 +49,11 @@ struct PrepArgs {
     float *bound;  // nq x nseg
 };
 
-constexpr int kPrepQ = 16;
+constexpr int kPrepQ = 8;
 
-// grid (ceil(nq/16), nseg, ceil(D_s/256)): 256 outputs x 16 queries per workgroup.
+// grid (ceil(nq/8), nseg, ceil(D_s/256)): 256 outputs x 8 queries per workgroup.
 __global__ __launch_bounds__(256) void query_prep_kernel(PrepArgs a) {
-    __shared__ float qs[kPrepQ][256];
+    __shared__ __align__(16) float qs[kPrepQ][256];
     const IndexView &ix = a.ix;
     const int s = blockIdx.y;
     const SegDesc sd = ix.segs[s];
@@ -77,17 +78,40 @@ __global__ __launch_bounds__(256) void query_prep_kernel(PrepArgs a) {
         }
         __syncthreads();
         if (j < D && bc == 16 && P) {
-            // blocks of 16 input dims (256-dim chunks hold whole blocks): 16 independent row
-            // loads, then the block's FMA chain per query, then res += c
+            // blocks of 16 input dims (256-dim chunks hold whole blocks): the block's 16 rows of P
+            // are loaded two blocks ahead (the last block's rows again past the chunk's end, so
+            // the loads are unconditional), then the block's FMA chain per query, then res += c
             const int iend = min(256, D - i0);
+            float pa[16], pb[16];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) pa[u] = P[(int64_t)(i0 + u) * D + j];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) pb[u] = P[(int64_t)(i0 + min(16, iend - 16) + u) * D + j];
             for (int ib = 0; ib < iend; ib += 16) {
                 float pr[16];
+                const int inext = min(ib + 32, iend - 16);
 #pragma unroll
-                for (int u = 0; u < 16; ++u) pr[u] = P[(int64_t)(i0 + ib + u) * D + j];
+                for (int u = 0; u < 16; ++u) {
+                    pr[u] = pa[u];
+                    pa[u] = pb[u];
+                    pb[u] = P[(int64_t)(i0 + inext + u) * D + j];
+                }
+                // each query's 16 inputs of the block in four 16-byte LDS reads, then its chain
 #pragma unroll
-                for (int u = 0; u < 16; ++u)
+                for (int t = 0; t < kPrepQ; ++t) {
+                    const float4 *xv = reinterpret_cast<const float4 *>(&qs[t][ib]);
+                    float x[16];
 #pragma unroll
-                    for (int t = 0; t < kPrepQ; ++t) c[t] = __fmaf_rn(pr[u], qs[t][ib + u], c[t]);
+                    for (int v = 0; v < 4; ++v) {
+                        const float4 f = xv[v];
+                        x[4 * v] = f.x;
+                        x[4 * v + 1] = f.y;
+                        x[4 * v + 2] = f.z;
+                        x[4 * v + 3] = f.w;
+                    }
+#pragma unroll
+                    for (int u = 0; u < 16; ++u) c[t] = __fmaf_rn(pr[u], x[u], c[t]);
+                }
 #pragma unroll
                 for (int t = 0; t < kPrepQ; ++t) {
                     acc[t] = acc[t] + c[t];  // pmadd(c, alpha = 1, res)
@@ -120,21 +144,22 @@ __global__ __launch_bounds__(256) void query_prep_kernel(PrepArgs a) {
             a.qrot[q * ix.D_pad + d] = P ? acc[t] : (d < ix.num_dim ? a.queries[q * ix.num_dim + d] : 0.0f);
         }
     }
-    if (blockIdx.z == 0) {
-        // bound = sqrt((var.segment * q^2).sum()) * m, Eigen redux order
-        const int wave = threadIdx.x >> 6;
-        for (int t = wave; t < kPrepQ; t += 4) {
-            const int64_t q = q0 + t;
-            if (q >= a.nq) continue;
-            const float *qq = a.queries + q * ix.num_dim;
-            const float v2 = eigen_redux_wave(D, [&](int i) {
-                const int d = sd.doff + i;
-                const float x = d < ix.num_dim ? qq[d] : 0.0f;
-                return a.variance[d] * (x * x);
-            });
-            if ((threadIdx.x & 63) == 0) a.bound[q * ix.nseg + s] = sqrtf(v2) * a.m;
-        }
-    }
+}
+
+// grid (nq, nseg), one wavefront each: bound = sqrt((var.segment * q^2).sum()) * m of one query's
+// segment (saq_estimator.hpp:42-44), Eigen redux order
+__global__ __launch_bounds__(64) void query_bound_kernel(PrepArgs a) {
+    const IndexView &ix = a.ix;
+    const int64_t q = blockIdx.x;
+    const int s = blockIdx.y;
+    const SegDesc sd = ix.segs[s];
+    const float *qq = a.queries + q * ix.num_dim;
+    const float v2 = eigen_redux_wave(sd.dim, [&](int i) {
+        const int d = sd.doff + i;
+        const float x = d < ix.num_dim ? qq[d] : 0.0f;
+        return a.variance[d] * (x * x);
+    });
+    if (threadIdx.x == 0) a.bound[q * ix.nseg + s] = sqrtf(v2) * a.m;
 }
 
 struct ScanArgs {
@@ -706,8 +731,10 @@ hipError_t launch_query_prep(const IndexView &ix, const float *queries, int64_t 
                              const float *rot, const int64_t *rot_off, float m, float *qrot, float *bound,
                              int max_dim, hipStream_t st) {
     PrepArgs a{ix, queries, nq, variance, rot, rot_off, m, qrot, bound};
+    if (nq == 0) return hipSuccess;
     dim3 grid((unsigned)((nq + kPrepQ - 1) / kPrepQ), (unsigned)ix.nseg, (unsigned)((max_dim + 255) / 256));
     hipLaunchKernelGGL(query_prep_kernel, grid, dim3(256), 0, st, a);
+    hipLaunchKernelGGL(query_bound_kernel, dim3((unsigned)nq, (unsigned)ix.nseg), dim3(64), 0, st, a);
     return hipGetLastError();
 }
 
