@@ -558,8 +558,97 @@ __device__ __forceinline__ void pairprep_segment(const PairPrepArgs &a, const BS
     }
 }
 
-// One wave per grouped pair; segments up to 2048 dimensions are held in registers (a segment
-// dimension and offset are multiples of 64, rows are 16-byte aligned).
+// The pair's whole residual in NKT float4 registers per lane (element 4 lane + 256 k of the
+// padded vector, D_pad <= 256 NKT), loaded in one round; then every segment from the lanes'
+// elements inside it (segment offsets and dimensions are multiples of 64, so no float4 straddles
+// two segments).  Same quantisation and constants as pairprep_segment.
+template <int NKT>
+__device__ __forceinline__ void pairprep_whole(const PairPrepArgs &a, int64_t g, const float *qr, const float *cr) {
+    const int lane = threadIdx.x & 63;
+    const BatchedView &bv = a.bv;
+    float4 r[NKT];
+#pragma unroll
+    for (int k = 0; k < NKT; ++k) {
+        const int d = 4 * lane + 256 * k;
+        if (d < bv.D_pad) {
+            const float4 x = *reinterpret_cast<const float4 *>(qr + d);
+            const float4 y = *reinterpret_cast<const float4 *>(cr + d);
+            r[k] = make_float4(x.x - y.x, x.y - y.y, x.z - y.z, x.w - y.w);
+        } else {
+            r[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+    }
+    float q2 = 0.0f;
+    for (int s = 0; s < bv.nseg; ++s) {
+        const BSeg bs = bv.segs[s];
+        float amax = 0.0f, sr = 0.0f, sq = 0.0f;
+#pragma unroll
+        for (int k = 0; k < NKT; ++k) {
+            const int d = 4 * lane + 256 * k;
+            if (d >= bs.doff && d < bs.doff + bs.dim) {
+                amax = fmaxf(amax, fmaxf(fmaxf(fabsf(r[k].x), fabsf(r[k].y)), fmaxf(fabsf(r[k].z), fabsf(r[k].w))));
+                sr += (r[k].x + r[k].y) + (r[k].z + r[k].w);
+                sq += (r[k].x * r[k].x + r[k].y * r[k].y) + (r[k].z * r[k].z + r[k].w * r[k].w);
+            }
+        }
+        sq += xor_f<32>(sq), sr += xor_f<32>(sr), amax = fmaxf(amax, xor_f<32>(amax));
+        sq += xor_f<16>(sq), sr += xor_f<16>(sr), amax = fmaxf(amax, xor_f<16>(amax));
+        sq += xor_f<8>(sq), sr += xor_f<8>(sr), amax = fmaxf(amax, xor_f<8>(amax));
+        sq += xor_f<4>(sq), sr += xor_f<4>(sr), amax = fmaxf(amax, xor_f<4>(amax));
+        sq += xor_f<2>(sq), sr += xor_f<2>(sr), amax = fmaxf(amax, xor_f<2>(amax));
+        sq += xor_f<1>(sq), sr += xor_f<1>(sr), amax = fmaxf(amax, xor_f<1>(amax));
+        q2 += sq;
+        if (bs.qs < 0) continue;
+        const float s1 = amax / 127.0f, inv1 = amax > 0.0f ? 127.0f / amax : 0.0f;
+        const float s2 = s1 / 254.0f, inv2 = amax > 0.0f ? 254.0f * 127.0f / amax : 0.0f;
+        int sum1 = 0, sum2 = 0;
+        int8_t *row1 = a.qq1 + g * bv.kq + bs.kq_off;
+        int8_t *row2 = a.qs == 2 ? a.qq2 + g * bv.kq + bs.kq_off : nullptr;
+#pragma unroll
+        for (int k = 0; k < NKT; ++k) {
+            const int d = 4 * lane + 256 * k;
+            if (d < bs.doff || d >= bs.doff + bs.dim) continue;
+            const float e[4] = {r[k].x, r[k].y, r[k].z, r[k].w};
+            int v1[4], v2[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                v1[u] = (int)fminf(127.0f, fmaxf(-127.0f, rintf(e[u] * inv1)));
+                sum1 += v1[u];
+                v2[u] = (int)fminf(127.0f, fmaxf(-127.0f, rintf((e[u] - (float)v1[u] * s1) * inv2)));
+            }
+            *reinterpret_cast<uint32_t *>(row1 + d - bs.doff) =
+                (uint32_t)(uint8_t)v1[0] | ((uint32_t)(uint8_t)v1[1] << 8) | ((uint32_t)(uint8_t)v1[2] << 16) |
+                ((uint32_t)(uint8_t)v1[3] << 24);
+            if (row2) {
+                sum2 += (v2[0] + v2[1]) + (v2[2] + v2[3]);
+                *reinterpret_cast<uint32_t *>(row2 + d - bs.doff) =
+                    (uint32_t)(uint8_t)v2[0] | ((uint32_t)(uint8_t)v2[1] << 8) | ((uint32_t)(uint8_t)v2[2] << 16) |
+                    ((uint32_t)(uint8_t)v2[3] << 24);
+            }
+        }
+        sum1 += xor_lane<32>(sum1), sum2 += xor_lane<32>(sum2);
+        sum1 += xor_lane<16>(sum1), sum2 += xor_lane<16>(sum2);
+        sum1 += xor_lane<8>(sum1), sum2 += xor_lane<8>(sum2);
+        sum1 += xor_lane<4>(sum1), sum2 += xor_lane<4>(sum2);
+        sum1 += xor_lane<2>(sum1), sum2 += xor_lane<2>(sum2);
+        sum1 += xor_lane<1>(sum1), sum2 += xor_lane<1>(sum2);
+        if (lane == 0) {
+            float *pc = a.pc + (g * bv.nqs + bs.qs) * 3;
+            pc[0] = bs.sqd * s1;
+            pc[1] = bs.sqd * s2;
+            pc[2] = (-1.0f + bs.sqd / 2) * sr;
+            int32_t *cc = a.pcorr + (g * bv.nqs + bs.qs) * 2;
+            cc[0] = bs.bits >= 8 ? 128 * sum1 : 0;
+            cc[1] = bs.bits >= 8 ? 128 * sum2 : 0;
+        }
+    }
+    if (lane == 0) a.pq2[g] = q2;
+}
+
+// One wave per grouped pair.  NKT > 0: vectors of up to 256 NKT padded dimensions, loaded whole
+// in one round; NKT = 0: segment by segment, segments up to 2048 dimensions held in registers (a
+// segment dimension and offset are multiples of 64, rows are 16-byte aligned).
+template <int NKT>
 __global__ __launch_bounds__(256) void bq_pairprep_kernel(PairPrepArgs a) {
     const int lane = threadIdx.x & 63;
     const int64_t g = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -570,6 +659,10 @@ __global__ __launch_bounds__(256) void bq_pairprep_kernel(PairPrepArgs a) {
     const int c = (int)a.probes[pair];
     const float *qr = a.qrot + q * bv.D_pad;
     const float *cr = bv.crot + (int64_t)c * bv.D_pad;
+    if constexpr (NKT > 0) {
+        pairprep_whole<NKT>(a, g, qr, cr);
+        return;
+    }
     float q2 = 0.0f;
     for (int s = 0; s < bv.nseg; ++s) {
         const BSeg bs = bv.segs[s];
@@ -1232,7 +1325,13 @@ hipError_t launch_bq_pairprep(const BatchedView &bv, const float *qrot, const ui
     a.pc = pb.pc;
     a.pcorr = pb.pcorr;
     a.pq2 = pb.pq2;
-    hipLaunchKernelGGL(bq_pairprep_kernel, dim3((unsigned)((max_pairs + 3) / 4)), dim3(256), 0, st, a);
+    const dim3 grid((unsigned)((max_pairs + 3) / 4));
+    if (bv.D_pad <= 512)
+        hipLaunchKernelGGL(bq_pairprep_kernel<2>, grid, dim3(256), 0, st, a);
+    else if (bv.D_pad <= 1024)
+        hipLaunchKernelGGL(bq_pairprep_kernel<4>, grid, dim3(256), 0, st, a);
+    else
+        hipLaunchKernelGGL(bq_pairprep_kernel<0>, grid, dim3(256), 0, st, a);
     return hipGetLastError();
 }
 
