@@ -151,7 +151,8 @@ int saqgpu_estimate(saqgpu_index *idx, const float *query, const uint32_t *probe
  * 64 bytes, the short codes of segments with bits > 0 (MSB-first, bytes of every 8 reversed)
  * padded to 64 bytes, the CodeHelper<b-1> long codes, one ExFactor per segment.  The index
  * supplies the SaqData (plan, rotations, variance); its lists need not be set.  Records of a
- * batch sit record_stride bytes apart (>= the record size, a multiple of 8). */
+ * batch sit record_stride bytes apart (>= the record size, a multiple of 8) from an 8-byte
+ * aligned base. */
 
 /* SaqSingleDataWrapper::calculate_memory_size (single_data.hpp:287-320) for the index's plan.
  * offsets (may be NULL) receives 3 + 2 * nseg int64: short-code section, long-code section,

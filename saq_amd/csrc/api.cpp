@@ -1569,6 +1569,7 @@ int saqgpu_single_quantize(saqgpu_index *ix, const float *data, int64_t n, const
     HIPCHK(hipSetDevice(ix->device));
     hipStream_t st = (hipStream_t)stream;
     const bool dev = flags & SAQGPU_DEVICE_PTRS;
+    if (dev && reinterpret_cast<uintptr_t>(records) % 8) return fail(SAQGPU_EINVAL, "records must be 8-byte aligned");
     DevBuf<float> d_data, d_O;
     DevBuf<uint8_t> d_rec;
     DevBuf<uint16_t> d_code;
@@ -1619,6 +1620,7 @@ int saqgpu_single_estimate(saqgpu_index *ix, const float *queries, int64_t nq, c
     HIPCHK(hipSetDevice(ix->device));
     hipStream_t st = (hipStream_t)stream;
     const bool dev = flags & SAQGPU_DEVICE_PTRS;
+    if (dev && reinterpret_cast<uintptr_t>(records) % 8) return fail(SAQGPU_EINVAL, "records must be 8-byte aligned");
     const size_t smem = (size_t)ix->D_pad * 4 + (size_t)ix->D_pad / 64 * 64;
     int max_smem = 0;
     HIPCHK(hipDeviceGetAttribute(&max_smem, hipDeviceAttributeMaxSharedMemoryPerBlock, ix->device));

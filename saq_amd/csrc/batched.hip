@@ -209,9 +209,13 @@ __global__ __launch_bounds__(1024) void bq_scan_kernel(const int32_t *counts, co
         cn[j] = oc[j] >= 0 ? counts[oc[j]] : 0;
         nv[j] = oc[j] >= 0 ? clu_n[oc[j]] : 0;
     }
+    // per is uniform: the slots past it (all zero) are skipped, with their integer divisions
     long long wk = 0;
 #pragma unroll
-    for (int j = 0; j < kScanRounds; ++j) wk += (long long)((cn[j] + QT - 1) / QT) * nv[j];
+    for (int j = 0; j < kScanRounds; ++j) {
+        if (j >= per) break;
+        wk += (long long)((cn[j] + QT - 1) / QT) * nv[j];
+    }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) wk += __shfl_xor(wk, o);
     if (lane == 0) wk_s[wv] = wk;
@@ -224,6 +228,8 @@ __global__ __launch_bounds__(1024) void bq_scan_kernel(const int32_t *counts, co
     int sc = 0, st = 0, sk = 0;
 #pragma unroll
     for (int j = 0; j < kScanRounds; ++j) {
+        ns[j] = 1;
+        if (j >= per) continue;
         ns[j] = max(1, (nv[j] + sv - 1) / sv);
         sc += cn[j];
         st += (cn[j] + QT - 1) / QT * ns[j];
@@ -259,6 +265,7 @@ __global__ __launch_bounds__(1024) void bq_scan_kernel(const int32_t *counts, co
     int rc = bc + ic - sc, rt = bt + it - st, rk = bk + ik - sk;
 #pragma unroll
     for (int j = 0; j < kScanRounds; ++j) {
+        if (j >= per) break;
         if (oc[j] < 0) continue;
         const int c = oc[j], n = cn[j];
         cstart[c] = rc;
