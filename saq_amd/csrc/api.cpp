@@ -8,6 +8,7 @@
 #include <rocblas/rocblas.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -677,7 +678,8 @@ int saqgpu_probe(saqgpu_index *ix, const float *queries, int64_t nq, int32_t npr
 
 namespace {
 // The plain scan as slices of the probe sequence (slice_scan.hip): lists [0,1), [1,4),
-// [4,16), [16,64), ... of every query; per slice one list-scan launch over all (query, list)
+// [4,16), [16,64), ... of every query (below 64 items the same count of slices, spaced evenly on a
+// log scale); per slice one list-scan launch over all (query, list)
 // pairs and one replay launch.  Block-info and candidate space that a slice runs out of is
 // flagged on the device; the search is then repeated with the space it asked for (the
 // capacities are kept for later searches), so results never depend on the capacities.
@@ -697,6 +699,22 @@ int sliced_scan(saqgpu_index *ix, const IndexView &v, const uint32_t *dp, const 
     const int items = p->nprobe + 1;
     std::vector<int> bounds{0};
     for (int b = 1; b < items; b *= 4) bounds.push_back(b);
+    const size_t nb = bounds.size();
+    if (nb > 2 && (items <= 64 || items - bounds[nb - 1] < (bounds[nb - 1] - bounds[nb - 2]) / 2)) {
+        // few slices, or a short last one: the same number of slices, their ends spaced evenly on a
+        // log scale from 1 to items (17 items: [0,1) [1,3) [3,7) [7,17) rather than [0,1) [1,4)
+        // [4,16) [16,17)), so no slice is a short tail and the candidate superset of the largest
+        // one shrinks
+        const int ns = (int)bounds.size();  // slices
+        const double r = std::pow((double)items, 1.0 / (ns - 1));
+        bounds.assign(1, 0);
+        double x = 1.0;
+        for (int i = 1; i < ns; ++i, x *= r) {
+            const int b = std::max(bounds.back() + 1, (int)std::lround(x));
+            if (b >= items) break;
+            bounds.push_back(b);
+        }
+    }
     bounds.push_back(items);
     int Lmax = 1;
     for (size_t i = 0; i + 1 < bounds.size(); ++i) Lmax = std::max(Lmax, bounds[i + 1] - bounds[i]);

@@ -80,8 +80,8 @@ def test_topk_exceeds_probed_vectors():
 @pytest.mark.parametrize("plan", [[(128, 4)], [(64, 5), (64, 3)], [(64, 9), (64, 0)]])
 @pytest.mark.parametrize("topk", [10, 100])
 def test_many_slices(plan, topk):
-    """nprobe 40 of 48 lists: the scan runs the slices [0,1) [1,4) [4,16) [16,40), each
-    refining the vectors whose fast estimate beats the k-th distance at its start."""
+    """nprobe 40 of 48 lists: the scan runs the slices [0,1) [1,3) [3,12) [12,41) of its 41 work
+    items, each refining the vectors whose fast estimate beats the k-th distance at its start."""
     case = F.make_case(n=6000, d=128, K=48, nq=24, plan=plan, seed=23)
     run_case(case, topk, 40)
 
