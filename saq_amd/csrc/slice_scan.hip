@@ -90,10 +90,32 @@ __device__ __host__ inline size_t ls_layout(unsigned char *base, const IndexView
 
 size_t list_scan_smem_bytes(const IndexView &ix) { return ls_layout(nullptr, ix, nullptr); }
 
-// accurate distance of every segment (compAccurateDist, caq_estimator.hpp:190-217) and
-// searchCluster's stage-3 running sum (saq_searcher.hpp:149-156) of one candidate, written as
-// its record.  cur: its fast estimate; x(s), ipq(s), ex(s), resc(s): |o|, the fast-scan inner
-// product, the extra-bit inner product and the rescale factor of segment s.
+// One segment of searchCluster's stage-3 running sum (saq_searcher.hpp:149-156): the accurate
+// distance av of segment s (compAccurateDist, caq_estimator.hpp:190-217) replaces its fast
+// estimate cd.  x, ipq, ex, resc: |o|, the fast-scan inner product, the extra-bit inner product
+// (0 below 2 bits) and the rescale factor; sc: the segment's scalars.
+template <int DT>
+__device__ __forceinline__ float stage3_step(const SegDesc &sd, const float *sc, int S, float x, float ipq, float ex,
+                                             float resc, float acc) {
+    float av, cd;
+    if (sd.bits == 0) {
+        av = DT == 1 ? sc[SC_IPQC] : x * x + sc[SC_QL2];
+        cd = DT == 1 ? sc[SC_VFAC] : max0(x * x + sc[SC_VFAC]);
+    } else {
+        float f = (ipq - sc[SC_SUMQCB]) * sd.rescale_sqrtD;
+        f = f * x;
+        cd = DT == 1 ? f * 0.5f + sc[SC_IPQC] : max0(x * x + (sc[SC_QL2] - f));
+        const float sqd = sd.sqd;
+        // Lut::getExtIP (lut.hpp:508-513)
+        const float t = (float)(((double)ipq + (double)ex * (double)sqd) + (-1.0 + (double)(sqd / 2)) * (double)sc[SC_SUMQ]);
+        const float ipoq = resc * t;
+        av = DT == 1 ? ipoq + sc[SC_IPQC] : x * x + sc[SC_QL2] - 2 * ipoq;
+    }
+    return S == 1 ? av : acc + (av - cd);
+}
+
+// The candidate's record: its accurate distance's running sum after every segment.  cur: its
+// fast estimate; x(s), ipq(s), ex(s), resc(s) as for stage3_step.
 template <int DT, class X, class I, class E, class R>
 __device__ __forceinline__ void write_record(uint32_t *rec, uint32_t pid, int ln, float cur, const SegDesc *segs,
                                              const float *scs, int S, X x_of, I ipq_of, E ex_of, R resc_of) {
@@ -103,29 +125,8 @@ __device__ __forceinline__ void write_record(uint32_t *rec, uint32_t pid, int ln
     rec[2] = __float_as_uint(cur);
     for (int s = 0; s < S; ++s) {
         const int bits = segs[s].bits;
-        const float *sc = scs + s * SC_N;
-        const float x = x_of(s);
-        float av, cd;
-        if (bits == 0) {
-            av = DT == 1 ? sc[SC_IPQC] : x * x + sc[SC_QL2];
-            cd = DT == 1 ? sc[SC_VFAC] : max0(x * x + sc[SC_VFAC]);
-        } else {
-            const float ipq = ipq_of(s);
-            float f = (ipq - sc[SC_SUMQCB]) * segs[s].rescale_sqrtD;
-            f = f * x;
-            cd = DT == 1 ? f * 0.5f + sc[SC_IPQC] : max0(x * x + (sc[SC_QL2] - f));
-            const float sqd = segs[s].sqd;
-            const float ex = bits >= 2 ? ex_of(s) : 0.0f;
-            // Lut::getExtIP (lut.hpp:508-513)
-            const float t = (float)(((double)ipq + (double)ex * (double)sqd) +
-                                    (-1.0 + (double)(sqd / 2)) * (double)sc[SC_SUMQ]);
-            const float ipoq = resc_of(s) * t;
-            av = DT == 1 ? ipoq + sc[SC_IPQC] : x * x + sc[SC_QL2] - 2 * ipoq;
-        }
-        if (S == 1)
-            acc = av;
-        else
-            acc += av - cd;
+        acc = stage3_step<DT>(segs[s], scs + s * SC_N, S, x_of(s), bits > 0 ? ipq_of(s) : 0.0f,
+                              bits >= 2 ? ex_of(s) : 0.0f, bits > 0 ? resc_of(s) : 0.0f, acc);
         rec[3 + s] = __float_as_uint(acc);
     }
 }
@@ -391,6 +392,7 @@ struct RWmem {
     float *exv;           // [cmax][S]
     uint32_t *pre;        // [cmax][pre_stride]
     float *side;          // [cmax][2 S + 1]: PID, per segment |o|, per segment rescale
+    int *ord;             // [cmax] the candidates still refined, in order
 };
 
 constexpr int kRW = 4;  // wavefronts of a refine workgroup
@@ -413,6 +415,7 @@ __device__ __host__ inline size_t rw_layout(unsigned char *base, const IndexView
     m.exv = reinterpret_cast<float *>(take((size_t)cmax * S * 4));
     m.pre = reinterpret_cast<uint32_t *>(take((size_t)cmax * pre_stride(S) * 4));
     m.side = reinterpret_cast<float *>(take((size_t)cmax * (2 * S + 1) * 4 + 256));
+    m.ord = reinterpret_cast<int *>(take((size_t)cmax * 4));
     if (mp) *mp = m;
     return (off + 15) & ~size_t(15);
 }
@@ -465,6 +468,15 @@ __global__ __launch_bounds__(kRW * 64) void refine_kernel(SliceArgs a) {
         if (total == 0) continue;  // uniform per wavefront
         const int64_t vec0 = ix.clu_vec[cid];
         const int64_t blk0 = ix.clu_blk[cid];
+        // the k-th distance at the start of the slice, as the list scan took it: the replay's
+        // distk never exceeds it, so once a candidate's running sum reaches it stage 3 stops at
+        // or before that segment and the later segments are never read
+        float T;
+        {
+            const int size = a.first ? 0 : a.pool_size[q];
+            const float raw = size == a.topk ? a.pool_d[q * a.topk + a.topk - 1] : FLT_MAX;
+            T = DT == 1 ? -raw : raw;
+        }
         // the residual and the per-segment scalars of this (query, list): the values the list
         // scan derived (CaqCluEstimator::prepare, caq_estimator.hpp:94-106)
         dma_dwords(reinterpret_cast<const uint32_t *>(a.qrot + q * ix.D_pad), reinterpret_cast<uint32_t *>(m.r),
@@ -517,16 +529,35 @@ __global__ __launch_bounds__(kRW * 64) void refine_kernel(SliceArgs a) {
                     __builtin_amdgcn_global_load_lds(src, m.side + i0, 4, 0, 0);
                 }
             }
+            for (int i = lane; i < nc * S; i += 64) m.exv[i] = 0.0f;  // segments never refined read 0
             dma_wait();
             __builtin_amdgcn_wave_barrier();
-            // extra-bit inner products, segment by segment (one code path per wavefront step)
+            // extra-bit inner products, segment by segment (one code path per wavefront step), of
+            // the candidates whose running sum (lane c) has not reached T
+            unsigned long long live = __ballot(lane < nc);
+            float run = lane < nc ? __uint_as_float(m.pre[lane * pst + 1]) : 0.0f;
             for (int s = 0; s < S; ++s) {
                 const SegDesc &sd = segs[s];
-                if (sd.bits < 2) continue;
-                for (int c = grp; c < nc; c += 4) {
-                    const float ex = ex_ip_group(sd.bits - 1, m.r + sd.doff, m.lbuf + c * ix.long_stride + sd.long_off,
-                                                 sd.dim, gl);
-                    if (gl == 0) m.exv[c * S + s] = ex;
+                if (sd.bits >= 2) {
+                    if ((live >> lane) & 1ull) m.ord[__popcll(live & ((1ull << lane) - 1))] = lane;
+                    __builtin_amdgcn_wave_barrier();
+                    const int nl = __popcll(live);
+                    for (int k = grp; k < nl; k += 4) {
+                        const int c = m.ord[k];
+                        const float ex = ex_ip_group(sd.bits - 1, m.r + sd.doff,
+                                                     m.lbuf + c * ix.long_stride + sd.long_off, sd.dim, gl);
+                        if (gl == 0) m.exv[c * S + s] = ex;
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                }
+                if (S > 1 && s + 1 < S) {
+                    if ((live >> lane) & 1ull) {
+                        const float *sd2 = m.side + lane * sst;
+                        run = stage3_step<DT>(sd, m.sc + s * SC_N, S, sd2[1 + s],
+                                              sd.bits > 0 ? __uint_as_float(m.pre[lane * pst + 2 + s]) : 0.0f,
+                                              m.exv[lane * S + s], sd.bits > 0 ? sd2[1 + S + s] : 0.0f, run);
+                    }
+                    live &= ~__ballot(((live >> lane) & 1ull) && run >= T);
                 }
             }
             __builtin_amdgcn_wave_barrier();
