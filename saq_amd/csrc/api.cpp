@@ -685,7 +685,7 @@ namespace {
 // capacities are kept for later searches), so results never depend on the capacities.
 // e0 / e1 (optional): events recorded around the kernels of the attempt that succeeds.
 //
-// The queries are split into up to kGroups contiguous groups (256 queries or more each), each
+// The queries are split into up to kGroups contiguous groups (200 queries or more each), each
 // with its own share of the workspace; group 0 runs on st, the others on streams of their own
 // forked from and joined back into st (four streams in all, the hardware queues a process
 // gets by default).  A group's replay, one latency-bound wavefront per query, then overlaps
@@ -728,7 +728,7 @@ int sliced_scan(saqgpu_index *ix, const IndexView &v, const uint32_t *dp, const 
     const size_t rf_smem = refine_smem_bytes(v, cmax);
     if ((int64_t)std::max(ls_smem, rf_smem) > max_smem)
         return fail(SAQGPU_EUNSUPPORTED, "plan needs more LDS than a workgroup has");
-    int G = (int)std::max<int64_t>(1, std::min<int64_t>(kGroups, nq / 256));
+    int G = (int)std::max<int64_t>(1, std::min<int64_t>(kGroups, nq / 200));
     if (const char *ge = getenv("SAQGPU_GROUPS")) G = std::max(1, std::min(G, atoi(ge)));
     // the refine kernel is persistent (workgroups loop over the descriptors); with four groups
     // in flight each one's refine takes half the chip's slots
