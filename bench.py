@@ -463,6 +463,16 @@ def main():
     log(f"encode {n / t_enc / 1e6:.2f} Mvec/s ({t_enc * 1e3:.0f} ms), check failures {nfail}")
     single = None if args.no_single else single_records(torch, ix, base, queries, dev, min(n, 1 << 20), 16,
                                                          cpu=None if args.no_cpu else (plan, rots, var))
+    # IVF::estimate (every vector of the probed lists, accurate / fast / variance estimates): one
+    # query per call, host buffers, as the reference's test_relative_error drives it
+    qn = queries[:8].cpu().numpy()
+    ix.estimate(qn[0], 64)
+    t0 = time.perf_counter()
+    nv = sum(len(ix.estimate(q, 64)[0]) for q in qn)
+    t_est = (time.perf_counter() - t0) / len(qn)
+    est = {"nprobe": 64, "vectors_per_query": nv // len(qn), "ms_per_query": round(t_est * 1e3, 3),
+           "vectors_per_s": round(nv / len(qn) / t_est, 1)}
+    log(f"estimate: {est}")
 
     # ---- plain nprobe
     if npq is not None:
@@ -645,7 +655,8 @@ def main():
                    "bits": bits, "plan": plan, "recall_at_10_by_bits": bits_tried,
                    "batched": batched, "plain": plain, "batched_vs_plain": round(bqps / qps, 3),
                    "serving": serving, "p99_ms": serving[-1]["p99_ms"] if serving else None,
-                   "encode_mvec_per_s": round(n / t_enc / 1e6, 3), "single_vector": single},
+                   "encode_mvec_per_s": round(n / t_enc / 1e6, 3), "single_vector": single,
+                   "estimate": est},
         "roofline": {"bound": "hbm", "achieved": round(b_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(b_gbs / HBM_PEAK_GBS, 4), "traffic": btraffic,
                      "kernel": "bscan_kernel", "alg_bytes_per_launch": b_bytes,
