@@ -133,6 +133,9 @@ struct saqgpu_index {
     // search workspace
     std::mutex mu;
     DevBuf<float> ws_q, ws_qrot, ws_bound, ws_dists, ws_cdist;
+    DevBuf<int64_t> ws_eoffs;  // IVF::estimate: per probe its first output slot
+    DevBuf<uint32_t> ws_eids;  //   and its outputs
+    DevBuf<float> ws_eout;
     DevBuf<uint32_t> ws_probes, ws_ids;
     DevBuf<uint64_t> ws_met;
     // sliced scan (slice_scan.hip): pools between slices, block info and candidate records;
@@ -435,6 +438,9 @@ void saqgpu_index_destroy(saqgpu_index *ix) {
     ix->ws_probes.release();
     ix->ws_ids.release();
     ix->ws_met.release();
+    ix->ws_eoffs.release();
+    ix->ws_eids.release();
+    ix->ws_eout.release();
     ix->bq_mb.release();
     ix->d_clu_mb.release();
     ix->d_bq_order.release();
@@ -1019,9 +1025,10 @@ int saqgpu_estimate(saqgpu_index *ix, const float *query, const uint32_t *probes
         offs[i] = tot;
         tot += ix->clu_n[pr[i]];
     }
-    DevBuf<int64_t> d_offs;
-    DevBuf<uint32_t> d_ids;
-    DevBuf<float> d_out;
+    // the index's estimate workspace, grown to the largest call and kept
+    DevBuf<int64_t> &d_offs = ix->ws_eoffs;
+    DevBuf<uint32_t> &d_ids = ix->ws_eids;
+    DevBuf<float> &d_out = ix->ws_eout;
     if ((rc = upload(d_offs, offs))) return rc;
     if (d_ids.ensure(std::max<int64_t>(1, tot)) || d_out.ensure(3 * std::max<int64_t>(1, tot)))
         return fail(SAQGPU_ENOMEM, "allocation failed");
