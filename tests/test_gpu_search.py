@@ -89,9 +89,9 @@ def test_many_slices(plan, topk):
 @pytest.mark.parametrize("topk", [1, 10])
 def test_refine_early_stop(topk):
     """A plan whose first segment carries most of the distance (64 dims at 9 bits, then 64 x 1
-    and 64 x 2) over 24 of 32 lists: most candidates' running sums reach the slice's starting
-    k-th distance after segment 0, so the refine skips their later segments; IDs, distances
-    and metrics still equal the oracle's."""
+    and 64 x 2) over 24 of 32 lists: a candidate's running sum can reach the slice's starting
+    k-th distance after segment 0, and the refine then skips its later segments; IDs, distances
+    and metrics must still equal the oracle's."""
     case = F.make_case(n=6000, d=192, K=32, nq=24, plan=[(64, 9), (64, 1), (64, 2)], seed=31, tau=8)
     run_case(case, topk, 24)
 
