@@ -31,6 +31,9 @@ typedef enum {
 } saqgpu_status;
 
 #define SAQGPU_MAX_SEG 64
+/* Segments with bits > 0 hold at most this many 64-dim code groups in all (4096 quantised
+ * dimensions); saqgpu_index_create returns SAQGPU_EUNSUPPORTED beyond it. */
+#define SAQGPU_MAX_GROUPS 64
 #define SAQGPU_MAX_TOPK 1024
 #define SAQGPU_DEVICE_PTRS 1
 

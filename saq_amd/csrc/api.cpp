@@ -25,11 +25,6 @@ namespace saqgpu {
 hipError_t launch_query_prep(const IndexView &ix, const float *queries, int64_t nq, const float *variance,
                              const float *rot, const int64_t *rot_off, float m, float *qrot, float *bound,
                              int max_dim, hipStream_t st);
-hipError_t launch_scan(const IndexView &ix, const float *qrot, const float *bound, const uint32_t *probes,
-                       int nprobe, const int32_t *npq, int topk, int64_t nq, int dist_type, uint32_t *out_ids,
-                       float *out_dists,
-                       uint64_t *out_metrics, size_t smem, hipStream_t st, unsigned long long *prof, int cmax);
-size_t scan_smem_bytes(const IndexView &ix, int topk, int cmax);
 hipError_t launch_probe(const float *centroids, int K, int D, const float *queries, int64_t nq, int nprobe,
                         int dist_type, float *dist_ws, uint32_t *probes, hipStream_t st);
 hipError_t launch_sqnorm_rows(const float *x, int64_t rows, int D, float *out, hipStream_t st);
@@ -130,8 +125,9 @@ struct saqgpu_index {
     DevBuf<int64_t> d_clu_blk, d_clu_vec, d_rot_off;
     DevBuf<int32_t> d_clu_n;
     DevBuf<SegDesc> d_segs;
-    // search workspace
+    // search workspace; calls on the index serialise on mu and, on the device, on ws_free
     std::mutex mu;
+    hipEvent_t ws_free = nullptr;
     DevBuf<float> ws_q, ws_qrot, ws_bound, ws_dists, ws_cdist;
     DevBuf<int64_t> ws_eoffs;  // IVF::estimate: per probe its first output slot
     DevBuf<uint32_t> ws_eids;  //   and its outputs
@@ -240,6 +236,29 @@ struct saqgpu_index {
 };
 
 namespace {
+// One call on an index.  The host mutex serialises the index's host-side state.  On the device,
+// the previous call recorded ws_free on its stream once its work was queued; this call's stream
+// waits for it before touching the workspace.  Calls with SAQGPU_DEVICE_PTRS return before their
+// kernels end, and the next call may come on another stream or thread.
+struct IndexCall {
+    saqgpu_index *ix;
+    hipStream_t st;
+    std::lock_guard<std::mutex> lk;
+    bool begun = false;
+    IndexCall(saqgpu_index *i, hipStream_t s) : ix(i), st(s), lk(i->mu) {}
+    int begin() {
+        HIPCHK(hipSetDevice(ix->device));
+        HIPCHK(hipStreamWaitEvent(st, ix->ws_free, 0));
+        begun = true;
+        return SAQGPU_OK;
+    }
+    ~IndexCall() {
+        if (begun) (void)hipEventRecord(ix->ws_free, st);
+    }
+};
+}  // namespace
+
+namespace {
 // |c|^2 of every centroid, once per index (the centroids do not change).
 int ensure_cnorm(saqgpu_index *ix, hipStream_t st) {
     if (ix->cnorm_ready) return SAQGPU_OK;
@@ -307,11 +326,19 @@ int saqgpu_index_create(const saqgpu_index_desc *d, saqgpu_index **out) {
         dsum += d->seg_dim[s];
     }
     if (dsum != (int)rd_up(d->num_dim, 64)) return fail(SAQGPU_EINVAL, "segment dims must sum to the padded dimension");
+    int qdims = 0;
+    for (int s = 0; s < d->num_seg; ++s) qdims += d->seg_bits[s] ? d->seg_dim[s] : 0;
+    if (qdims > 64 * SAQGPU_MAX_GROUPS)
+        return fail(SAQGPU_EUNSUPPORTED, "at most 4096 quantised dimensions (64 code groups of 64) are supported");
     auto *ix = new saqgpu_index();
     ix->device = d->device;
     if (hipSetDevice(d->device) != hipSuccess) {
         delete ix;
         return fail(SAQGPU_EDEVICE, "hipSetDevice failed");
+    }
+    if (hipEventCreateWithFlags(&ix->ws_free, hipEventDisableTiming) != hipSuccess) {
+        delete ix;
+        return fail(SAQGPU_EDEVICE, "hipEventCreate failed");
     }
     ix->num_data = d->num_data;
     ix->num_dim = d->num_dim;
@@ -469,6 +496,7 @@ void saqgpu_index_destroy(saqgpu_index *ix) {
     }
     if (ix->sl_fork) (void)hipEventDestroy(ix->sl_fork);
     if (ix->blas) (void)rocblas_destroy_handle(ix->blas);
+    if (ix->ws_free) (void)hipEventDestroy(ix->ws_free);
     delete ix;
 }
 
@@ -519,7 +547,8 @@ int saqgpu_index_set_cluster(saqgpu_index *ix, int32_t cid, const float *short_f
                              const uint8_t *long_code, const float *long_factors, const uint32_t *ids,
                              const float *centroid_rot) {
     if (!ix || cid < 0 || cid >= ix->num_cen) return fail(SAQGPU_EINVAL, "bad cluster id");
-    HIPCHK(hipSetDevice(ix->device));
+    IndexCall call(const_cast<saqgpu_index *>(ix), nullptr);
+    if (int rc_ = call.begin()) return rc_;
     const int64_t n = ix->clu_n[cid];
     const auto L = ix->ref_layout(n);
     const int S = ix->nseg;
@@ -588,7 +617,8 @@ int saqgpu_index_set_cluster(saqgpu_index *ix, int32_t cid, const float *short_f
 int saqgpu_index_get_cluster(const saqgpu_index *ix, int32_t cid, float *short_factors, uint8_t *short_code,
                              uint8_t *long_code, float *long_factors, uint32_t *ids, float *centroid_rot) {
     if (!ix || cid < 0 || cid >= ix->num_cen) return fail(SAQGPU_EINVAL, "bad cluster id");
-    HIPCHK(hipSetDevice(ix->device));
+    IndexCall call(const_cast<saqgpu_index *>(ix), nullptr);
+    if (int rc_ = call.begin()) return rc_;
     const int64_t n = ix->clu_n[cid];
     const auto L = ix->ref_layout(n);
     const int S = ix->nseg;
@@ -660,8 +690,8 @@ int saqgpu_probe(saqgpu_index *ix, const float *queries, int64_t nq, int32_t npr
     if (nprobe < 1 || nprobe > ix->num_cen) return fail(SAQGPU_EINVAL, "nprobe must be in 1..K");
     if (dist_type != 0 && dist_type != 1) return fail(SAQGPU_EINVAL, "dist_type must be 0 (L2) or 1 (IP)");
     if (nq == 0) return SAQGPU_OK;
-    std::lock_guard<std::mutex> lk(ix->mu);
-    HIPCHK(hipSetDevice(ix->device));
+    IndexCall call(ix, (hipStream_t)stream);
+    if (int rc_ = call.begin()) return rc_;
     hipStream_t st = (hipStream_t)stream;
     const bool dev = flags & SAQGPU_DEVICE_PTRS;
     const float *dq = queries;
@@ -884,27 +914,10 @@ int saqgpu_search(saqgpu_index *ix, const float *queries, int64_t nq, const uint
         for (int64_t i = 0; i < nq; ++i)
             if (p->nprobe_per_query[i] < 1 || p->nprobe_per_query[i] > p->nprobe)
                 return fail(SAQGPU_EINVAL, "nprobe_per_query entries must be in 1..nprobe");
-    std::lock_guard<std::mutex> lk(ix->mu);
-    HIPCHK(hipSetDevice(ix->device));
+    IndexCall call(ix, (hipStream_t)stream);
+    if (int rc_ = call.begin()) return rc_;
     hipStream_t st = (hipStream_t)stream;
     const IndexView v = ix->view();
-    // SAQGPU_SCAN=fused: the single-kernel scan (one workgroup walks a query's whole probe
-    // sequence); kept for comparison, the sliced scan is the default
-    static const bool fused = [] {
-        const char *e = getenv("SAQGPU_SCAN");
-        return e && !strcmp(e, "fused");
-    }();
-    int cmax = 64;
-    size_t smem = 0;
-    if (fused) {
-        // candidates per refine round: their long-code records are staged in LDS
-        // up to 64 per round while a workgroup's LDS stays within 80 KB (two workgroups per CU)
-        while (cmax > 8 && scan_smem_bytes(v, p->topk, cmax) > 80 * 1024) cmax /= 2;
-        smem = scan_smem_bytes(v, p->topk, cmax);
-        int max_smem = 0;
-        HIPCHK(hipDeviceGetAttribute(&max_smem, hipDeviceAttributeMaxSharedMemoryPerBlock, ix->device));
-        if ((int64_t)smem > max_smem) return fail(SAQGPU_EUNSUPPORTED, "plan needs more LDS than a workgroup has");
-    }
     int rc = 0;
     rc |= ix->ws_qrot.ensure(nq * ix->D_pad);
     rc |= ix->ws_bound.ensure(nq * ix->nseg);
@@ -950,32 +963,9 @@ int saqgpu_search(saqgpu_index *ix, const float *queries, int64_t nq, const uint
     if (ix->timing) {
         HIPCHK(hipEventCreate(&e0));
         HIPCHK(hipEventCreate(&e1));
-        if (fused) HIPCHK(hipEventRecord(e0, st));
     }
-    if (fused) {
-        // SAQGPU_PROFILE=1: per-phase shader clocks of the fused kernel, printed to stderr
-        static const bool prof_on = getenv("SAQGPU_PROFILE") != nullptr;
-        DevBuf<unsigned long long> prof;
-        if (prof_on && prof.ensure(nq * 8)) return fail(SAQGPU_ENOMEM, "allocation failed");
-        HIPCHK(launch_scan(v, ix->ws_qrot.p, ix->ws_bound.p, dp, p->nprobe, dnpq, p->topk, nq, p->dist_type, dids,
-                           dd, dm, smem, st, prof_on ? prof.p : nullptr, cmax));
-        if (prof_on) {
-            std::vector<unsigned long long> h(nq * 8);
-            HIPCHK(hipStreamSynchronize(st));
-            HIPCHK(hipMemcpy(h.data(), prof.p, nq * 8 * 8, hipMemcpyDeviceToHost));
-            double tot[8] = {0};
-            for (int64_t i = 0; i < nq; ++i)
-                for (int k = 0; k < 8; ++k) tot[k] += (double)h[i * 8 + k];
-            fprintf(stderr, "[saqgpu profile] per-query wall-clock ticks (100 MHz):");
-            for (int k = 0; k < 7; ++k) fprintf(stderr, " ph%d=%.0f", k, tot[k] / nq);
-            fprintf(stderr, "\n");
-            prof.release();
-        }
-        if (ix->timing) HIPCHK(hipEventRecord(e1, st));
-    } else {
-        const int rc2 = sliced_scan(ix, v, dp, p, dnpq, nq, dids, dd, dm, st, e0, e1);
-        if (rc2 != SAQGPU_OK) return rc2;
-    }
+    const int rc2 = sliced_scan(ix, v, dp, p, dnpq, nq, dids, dd, dm, st, e0, e1);
+    if (rc2 != SAQGPU_OK) return rc2;
     if (ix->timing) ix->events.emplace_back(e0, e1);
     if (!dev) {
         HIPCHK(hipMemcpyAsync(out_ids, dids, nq * p->topk * 4, hipMemcpyDeviceToHost, st));
@@ -997,8 +987,8 @@ int saqgpu_estimate(saqgpu_index *ix, const float *query, const uint32_t *probes
     if (probes)
         for (int i = 0; i < nprobe; ++i)
             if (probes[i] >= (uint32_t)ix->num_cen) return fail(SAQGPU_EINVAL, "probe id out of range");
-    std::lock_guard<std::mutex> lk(ix->mu);
-    HIPCHK(hipSetDevice(ix->device));
+    IndexCall call(ix, nullptr);
+    if (int rc_ = call.begin()) return rc_;
     const size_t smem = estimate_smem_bytes(ix->view());
     int max_smem = 0;
     HIPCHK(hipDeviceGetAttribute(&max_smem, hipDeviceAttributeMaxSharedMemoryPerBlock, ix->device));
@@ -1092,8 +1082,8 @@ int saqgpu_encode(saqgpu_index *ix, const float *data, const uint32_t *cluster_i
     if (!ix || !data || !cluster_ids || !cfg) return fail(SAQGPU_EINVAL, "null argument");
     if (n != ix->num_data) return fail(SAQGPU_EINVAL, "n must equal the index's num_data");
     if (cfg->caq_adj_rd_lmt < 0) return fail(SAQGPU_EINVAL, "caq_adj_rd_lmt must be >= 0");
-    std::lock_guard<std::mutex> lk(ix->mu);
-    HIPCHK(hipSetDevice(ix->device));
+    IndexCall call(ix, (hipStream_t)stream);
+    if (int rc_ = call.begin()) return rc_;
     hipStream_t st = (hipStream_t)stream;
     const bool dev = flags & SAQGPU_DEVICE_PTRS;
     std::vector<uint32_t> hcid;
@@ -1153,8 +1143,8 @@ int saqgpu_encode_residuals(saqgpu_index *ix, int32_t cid, int32_t seg, const fl
                             const saqgpu_caq_cfg *cfg) {
     if (!ix || !o || !cfg || cid < 0 || cid >= ix->num_cen || seg < 0 || seg >= ix->nseg)
         return fail(SAQGPU_EINVAL, "bad argument");
-    std::lock_guard<std::mutex> lk(ix->mu);
-    HIPCHK(hipSetDevice(ix->device));
+    IndexCall call(ix, nullptr);
+    if (int rc_ = call.begin()) return rc_;
     const int64_t n = ix->clu_n[cid];
     if (n == 0) return SAQGPU_OK;
     const int D = ix->segs[seg].dim;
@@ -1436,8 +1426,8 @@ extern "C" {
 
 int saqgpu_prepare_batched(saqgpu_index *ix) {
     if (!ix) return fail(SAQGPU_EINVAL, "null index");
-    std::lock_guard<std::mutex> lk(ix->mu);
-    HIPCHK(hipSetDevice(ix->device));
+    IndexCall call(ix, nullptr);
+    if (int rc_ = call.begin()) return rc_;
     return prepare_batched_locked(ix);
 }
 
@@ -1451,8 +1441,8 @@ int saqgpu_search_batched(saqgpu_index *ix, const float *queries, int64_t nq, co
     if (probes && !(flags & SAQGPU_DEVICE_PTRS))
         for (int64_t i = 0; i < nq * p->nprobe; ++i)
             if (probes[i] >= (uint32_t)ix->num_cen) return fail(SAQGPU_EINVAL, "probe id out of range");
-    std::lock_guard<std::mutex> lk(ix->mu);
-    HIPCHK(hipSetDevice(ix->device));
+    IndexCall call(ix, (hipStream_t)stream);
+    if (int rc_ = call.begin()) return rc_;
     const int qt = p->query_tile ? p->query_tile : (ix->bq_tile ? ix->bq_tile : default_query_tile(ix, nq, p->nprobe));
     return search_batched_locked(ix, queries, nq, probes, p, qt, out_ids, out_dists, flags, (hipStream_t)stream,
                                  ix->timing);
@@ -1463,8 +1453,8 @@ int saqgpu_autotune_batched(saqgpu_index *ix, const float *queries, int64_t nq, 
     if (!ix || !queries || !p || nq < 1) return fail(SAQGPU_EINVAL, "null argument");
     int rc = check_batched_params(ix, p, nq, flags);
     if (rc) return rc;
-    std::lock_guard<std::mutex> lk(ix->mu);
-    HIPCHK(hipSetDevice(ix->device));
+    IndexCall call(ix, (hipStream_t)stream);
+    if (int rc_ = call.begin()) return rc_;
     hipStream_t st = (hipStream_t)stream;
     const bool dev = flags & SAQGPU_DEVICE_PTRS;
     DevBuf<uint32_t> ids;
@@ -1590,8 +1580,8 @@ int saqgpu_single_quantize(saqgpu_index *ix, const float *data, int64_t n, const
     if (int rc = check_stride(L, record_stride)) return rc;
     if (n_check_failed) *n_check_failed = 0;
     if (n == 0) return SAQGPU_OK;
-    std::lock_guard<std::mutex> lk(ix->mu);
-    HIPCHK(hipSetDevice(ix->device));
+    IndexCall call(ix, (hipStream_t)stream);
+    if (int rc_ = call.begin()) return rc_;
     hipStream_t st = (hipStream_t)stream;
     const bool dev = flags & SAQGPU_DEVICE_PTRS;
     if (dev && reinterpret_cast<uintptr_t>(records) % 8) return fail(SAQGPU_EINVAL, "records must be 8-byte aligned");
@@ -1641,8 +1631,8 @@ int saqgpu_single_estimate(saqgpu_index *ix, const float *queries, int64_t nq, c
     if (int rc = check_stride(L, record_stride)) return rc;
     if (nq == 0 || nrec == 0) return SAQGPU_OK;
     if (nq * ((nrec + 15) / 16) > 0x7fffffffLL) return fail(SAQGPU_EUNSUPPORTED, "nq x nrec too large for one launch");
-    std::lock_guard<std::mutex> lk(ix->mu);
-    HIPCHK(hipSetDevice(ix->device));
+    IndexCall call(ix, (hipStream_t)stream);
+    if (int rc_ = call.begin()) return rc_;
     hipStream_t st = (hipStream_t)stream;
     const bool dev = flags & SAQGPU_DEVICE_PTRS;
     if (dev && reinterpret_cast<uintptr_t>(records) % 8) return fail(SAQGPU_EINVAL, "records must be 8-byte aligned");

@@ -1,6 +1,6 @@
 // batched.hip -- the batched, cluster-grouped SAQ scan on int8 MFMA (gfx950).
 //
-// The plain scan (scan.hip) walks every query's probe list on its own and reproduces the
+// The plain scan (slice_scan.hip) follows every query's probe sequence and reproduces the
 // reference bit for bit.  This path serves many queries at once instead: the (query, list)
 // pairs of a batch are grouped by list, and each work tile -- one list and up to 16*QTM of
 // the queries that probe it -- turns the estimate of every vector of the list into a dense

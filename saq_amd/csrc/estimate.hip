@@ -74,12 +74,12 @@ __global__ __launch_bounds__(256) void estimate_kernel(EstArgs a) {
     const int n = ix.clu_n[c];
     if (n == 0) return;  // uniform per workgroup
     for (int i = tid; i < S; i += 256) m.segs[i] = ix.segs[i];
-    uint32_t endmask = 0;
+    unsigned long long endmask = 0;
     int nbq = 0;
     for (int s = 0; s < S; ++s) {
         const SegDesc sd = ix.segs[s];
         if (sd.bits == 0) continue;
-        endmask |= 1u << (sd.group_off + sd.ngroups - 1);
+        endmask |= 1ull << (sd.group_off + sd.ngroups - 1);
         for (int g = tid; g < sd.ngroups; g += 256) m.gseg[sd.group_off + g] = (int8_t)s;
         if (tid == 0) m.bseq[nbq] = (int8_t)s;
         ++nbq;
@@ -120,7 +120,7 @@ __global__ __launch_bounds__(256) void estimate_kernel(EstArgs a) {
                     acc += t[(2 * b + 8) * 16 + (byte >> 4)];
                     acc += t[(2 * b + 9) * 16 + (byte & 15)];
                 }
-                if ((endmask >> g) & 1u) {
+                if ((endmask >> g) & 1ull) {
                     const int s = m.bseq[bk++];
                     const float *sc = m.sc + s * SC_N;
                     float t2 = (float)(int32_t)acc * sc[SC_DELTA];  // lut.hpp:495-497

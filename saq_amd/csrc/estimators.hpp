@@ -1,5 +1,5 @@
 // estimators.hpp -- the reference's per-list estimator state and extra-bit inner products,
-// shared by the plain scans (slice_scan.hip, scan.hip) and IVF::estimate (estimate.hip).
+// shared by the plain scan (slice_scan.hip) and IVF::estimate (estimate.hip).
 #pragma once
 #include <cfloat>
 

@@ -166,12 +166,12 @@ __global__ __launch_bounds__(kLT) void list_scan_kernel(SliceArgs a) {
         a.pair_blk[pair] = ok ? base : kNoPair;
     }
     for (int i = tid; i < S; i += kLT) m.segs[i] = ix.segs[i];
-    uint32_t endmask = 0;
+    unsigned long long endmask = 0;
     int nbq = 0;
     for (int s = 0; s < S; ++s) {
         const SegDesc sd = ix.segs[s];
         if (sd.bits == 0) continue;
-        endmask |= 1u << (sd.group_off + sd.ngroups - 1);
+        endmask |= 1ull << (sd.group_off + sd.ngroups - 1);
         for (int g = tid; g < sd.ngroups; g += kLT) m.gseg[sd.group_off + g] = (int8_t)s;
         if (tid == 0) m.bseq[nbq] = (int8_t)s;
         ++nbq;
@@ -219,7 +219,8 @@ __global__ __launch_bounds__(kLT) void list_scan_kernel(SliceArgs a) {
             // (indices past NG repeat the last group and are not summed)
             uint2 w[8], wn[8];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) w[u] = *reinterpret_cast<const uint2 *>(codes + min(u, NG - 1) * 256);
+            for (int u = 0; u < 8; ++u)  // NG == 0 (every segment at 0 bits): nothing to load
+                w[u] = NG > 0 ? *reinterpret_cast<const uint2 *>(codes + min(u, NG - 1) * 256) : uint2{0u, 0u};
             for (int g0 = 0; g0 < NG; g0 += 8) {
 #pragma unroll
                 for (int u = 0; u < 8; ++u)  // unconditional, so the waits below count only w's loads
@@ -246,7 +247,7 @@ __global__ __launch_bounds__(kLT) void list_scan_kernel(SliceArgs a) {
                         acc += *reinterpret_cast<const uint16_t *>(t + (2 * b + 8) * 32 + __builtin_amdgcn_ubfe(yh, 8 * b, 8));
                         acc += *reinterpret_cast<const uint16_t *>(t + (2 * b + 9) * 32 + __builtin_amdgcn_ubfe(yl, 8 * b, 8));
                     }
-                    if ((endmask >> g) & 1u) {
+                    if ((endmask >> g) & 1ull) {
                         const int s = m.bseq[bk++];
                         const float *sc = m.sc + s * SC_N;
                         float t2 = (float)(int32_t)acc * sc[SC_DELTA];  // lut.hpp:495-497
@@ -622,7 +623,7 @@ __global__ __launch_bounds__(64) void replay_kernel(SliceArgs a) {
     // per segment (lane s): the prefix sums of scanned and refined bits, and which segments
     // carry codes
     unsigned long long fastcum = 0, cumbits = 0;
-    uint32_t qmask = 0;
+    unsigned long long qmask = 0;
     {
         unsigned long long fc = 0, cb = 0;
         for (int s = 0; s < S; ++s) {
@@ -630,7 +631,7 @@ __global__ __launch_bounds__(64) void replay_kernel(SliceArgs a) {
             if (sd.bits > 0) {
                 fc += (unsigned long long)kBlock * sd.dim;
                 cb += (unsigned long long)sd.dim * (sd.bits - 1);
-                qmask |= 1u << s;
+                qmask |= 1ull << s;
             }
             if (s == lane) {
                 fastcum = fc;
@@ -811,7 +812,7 @@ __global__ __launch_bounds__(64) void replay_kernel(SliceArgs a) {
                 int sb = S - 1;
                 if (counted) {
                     for (int s = 0; s < S; ++s) {
-                        if (((qmask >> s) & 1u) && __uint_as_float(wi[lane * bst + 4 + s]) > dk) {
+                        if (((qmask >> s) & 1ull) && __uint_as_float(wi[lane * bst + 4 + s]) > dk) {
                             sb = s;
                             break;
                         }

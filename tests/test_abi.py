@@ -28,6 +28,21 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(L, n), n
 
 
+def test_create_refuses_more_than_64_code_groups():
+    """The fast scan flushes segment sums at 64-bit group masks: more than 4096 quantised dims
+    are refused (SAQGPU_EUNSUPPORTED) before any device work."""
+    L = saq_amd.lib()
+    h = C.c_void_p()
+    sizes = np.array([10], np.int64)
+    for dims, bits in (([4160], [1]), ([2048, 2048, 64], [2, 3, 1]), ([8192], [4])):
+        d = np.array(dims, np.int32)
+        b = np.array(bits, np.int32)
+        desc = saq_amd._Desc(10, int(d.sum()), 1, len(dims), 0, d.ctypes.data, b.ctypes.data, None, None, None,
+                             sizes.ctypes.data, 0)
+        assert L.saqgpu_index_create(C.byref(desc), C.byref(h)) == 5
+        assert b"4096" in L.saqgpu_last_error()
+
+
 def test_bad_arguments_fail_without_a_device():
     L = saq_amd.lib()
     h = C.c_void_p()
