@@ -133,6 +133,18 @@ int saqgpu_encode_residuals(saqgpu_index *idx, int32_t cid, int32_t seg, const f
 int saqgpu_probe(saqgpu_index *idx, const float *queries, int64_t nq, int32_t nprobe,
                  int32_t dist_type, uint32_t *probes, int32_t flags, void *stream);
 
+/* No counterpart in the reference: saqgpu_probe for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  It
+ * allocates nothing: queries (nq x num_dim) and probes (nq x nprobe) are device pointers on the
+ * index's device, the work is queued on stream, and the (query, centroid) distances go to one
+ * fixed device array per GPU of SAQGPU_NOALLOC_SCRATCH_FLOATS floats, held until the process
+ * ends.  nq * num_cen beyond it is refused (SAQGPU_EINVAL): split the queries.  The lists equal
+ * saqgpu_probe's.  Every call on a GPU shares that GPU's array, so calls that overlap on one GPU
+ * (other indexes, other threads or streams) race; calls on one index are ordered as usual. */
+#define SAQGPU_NOALLOC_SCRATCH_FLOATS (1 << 24)
+int saqgpu_probe_noalloc(saqgpu_index *idx, const float *queries, int64_t nq, int32_t nprobe, int32_t dist_type,
+                         uint32_t *probes, void *stream);
+
 /* Replaces IVF::search (ivf.hpp:275-311) for a batch of queries (nq x num_dim).  probes may be
  * NULL (the GPU coarse quantiser picks them) or nq x nprobe cluster ids.  out_ids is
  * nq x topk (unfilled = 0, like ResultPool::copy_results); out_dists and metrics may be NULL.

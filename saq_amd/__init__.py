@@ -52,6 +52,7 @@ class BatchedParams(C.Structure):
 
 PREC_INT8 = 1
 PREC_INT8X2 = 2
+SAQGPU_NOALLOC_SCRATCH_FLOATS = 1 << 24  # include/saqgpu.h
 
 _lib = None
 
@@ -68,6 +69,8 @@ _SIGS = {
                                           C.POINTER(CaqCfg)]),
     "saqgpu_probe": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_void_p, C.c_int32,
                                C.c_void_p]),
+    "saqgpu_probe_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_void_p,
+                                       C.c_void_p]),
     "saqgpu_search": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.POINTER(SearchParams),
                                 C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
     "saqgpu_estimate": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_float, C.c_int32,
@@ -219,6 +222,13 @@ class Index:
                 out = np.zeros((nq, nprobe), np.uint32)
         _check(lib().saqgpu_probe(self.h, _ptr(queries), nq, nprobe, dist_type, _ptr(out),
                                   DEVICE_PTRS if dev else 0, stream))
+        return out
+
+    def probe_noalloc(self, queries, nprobe, out, dist_type=0, stream=None):
+        """saqgpu_probe_noalloc: device tensors only (queries nq x num_dim, out nq x nprobe int32);
+        queued on stream, no device allocation."""
+        _check(lib().saqgpu_probe_noalloc(self.h, _ptr(queries), int(queries.shape[0]), nprobe, dist_type,
+                                          _ptr(out), stream))
         return out
 
     def search(self, queries, topk, nprobe, probes=None, m=4.0, dist_type=0, out_ids=None, out_dists=None,

@@ -58,6 +58,11 @@ hipError_t launch_single_estimate(const IndexView &ix, const float *qrot, const 
 
 using namespace saqgpu;
 
+// The fixed device scratch of saqgpu_probe_noalloc: one copy per GPU, held until the process ends.
+// Host code never uses the array's name as an address: hipGetSymbolAddress gives each device's
+// copy (scratch_setup, from saqgpu_index_create), and only those pointers reach kernels.
+static __device__ __align__(256) float scratch[SAQGPU_NOALLOC_SCRATCH_FLOATS];
+
 namespace {
 thread_local std::string g_err;
 
@@ -105,6 +110,27 @@ int upload(DevBuf<T> &b, const std::vector<T> &v) {
     if (!v.empty() && hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
         return fail(SAQGPU_EDEVICE, "upload failed");
     return SAQGPU_OK;
+}
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float *g_scratch[kMaxDevices] = {};
+
+// Resolves the device address of this device's scratch array (the device must be current).
+int scratch_setup(int device) {
+    if (device < 0 || device >= kMaxDevices) return fail(SAQGPU_EINVAL, "device ordinal out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch[device]) return SAQGPU_OK;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(SAQGPU_EDEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch[device] = static_cast<float *>(p);
+    return SAQGPU_OK;
+}
+
+float *scratch_of(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return device >= 0 && device < kMaxDevices ? g_scratch[device] : nullptr;
 }
 }  // namespace
 
@@ -335,6 +361,10 @@ int saqgpu_index_create(const saqgpu_index_desc *d, saqgpu_index **out) {
     if (hipSetDevice(d->device) != hipSuccess) {
         delete ix;
         return fail(SAQGPU_EDEVICE, "hipSetDevice failed");
+    }
+    if (int rc = scratch_setup(d->device)) {
+        delete ix;
+        return rc;
     }
     if (hipEventCreateWithFlags(&ix->ws_free, hipEventDisableTiming) != hipSuccess) {
         delete ix;
@@ -709,6 +739,27 @@ int saqgpu_probe(saqgpu_index *ix, const float *queries, int64_t nq, int32_t npr
         HIPCHK(hipMemcpyAsync(probes, dp, nq * nprobe * 4, hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
     }
+    return SAQGPU_OK;
+}
+
+int saqgpu_probe_noalloc(saqgpu_index *ix, const float *queries, int64_t nq, int32_t nprobe, int32_t dist_type,
+                         uint32_t *probes, void *stream) {
+    if (!ix || !queries || !probes || nq < 0) return fail(SAQGPU_EINVAL, "null argument");
+    if (nprobe < 1 || nprobe > ix->num_cen) return fail(SAQGPU_EINVAL, "nprobe must be in 1..K");
+    if (nprobe > 4096) return fail(SAQGPU_EUNSUPPORTED, "nprobe above 4096 is not supported");
+    if (dist_type != 0 && dist_type != 1) return fail(SAQGPU_EINVAL, "dist_type must be 0 (L2) or 1 (IP)");
+    if (nq > SAQGPU_NOALLOC_SCRATCH_FLOATS / ix->num_cen)
+        return fail(SAQGPU_EINVAL, "nq x num_cen exceeds the fixed scratch of SAQGPU_NOALLOC_SCRATCH_FLOATS "
+                                   "floats: split the queries");
+    if (nq == 0) return SAQGPU_OK;
+    hipStream_t st = (hipStream_t)stream;
+    IndexCall call(ix, st);
+    if (int rc = call.begin()) return rc;
+    float *ws = scratch_of(ix->device);
+    if (!ws) return fail(SAQGPU_ESTATE, "device scratch not set up");
+    // every (query, centroid) distance is written before the selection reads it: nothing in the
+    // scratch needs clearing
+    HIPCHK(launch_probe(ix->centroids.p, ix->num_cen, ix->num_dim, queries, nq, nprobe, dist_type, ws, probes, st));
     return SAQGPU_OK;
 }
 
