@@ -11,6 +11,15 @@
 // Eigen pmadd.  Reductions follow the vectorised order the reference executes:
 // Eigen's redux (third/Eigen/src/Core/Redux.h:227-272) with AVX512 predux, and
 // GCC's _mm512_reduce_add_ps, which share one 16->8->4->2->1 tree.
+//
+// How each part is pinned (tests/test_oracle_*.py): the reductions, GEMV, LUT packing and
+// quantisation, fast-scan accumulate, code packers and compute_ip bit for bit against the
+// reference's own headers compiled in oracle/_ref; the cluster and single-vector estimators by
+// float64 closed forms; scanCluster / searchCluster by independent block models; the CAQ
+// adjustment by an exact-rational model; the plan DP by brute force.  Parity unpinned: the
+// encoder's float64 sums under the reference's -Ofast (this file keeps source order, and no
+// -Ofast encoder can be built here: its headers include glog) and the rotation GEMM order of
+// IVF::construct (orc_rotate_rows follows the GEMV order).
 #include "saq_oracle.h"
 
 #include <algorithm>
@@ -69,7 +78,7 @@ struct Pool {
     std::vector<uint32_t> ids;
     std::vector<float> dist;
     Pool(int64_t k, bool g) : greater(g), cap(k), ids(k + 1, 0), dist(k + 1, 0.0f) {}
-    int64_t bsearch(float d) const {  // pool.hpp:267-276
+    int64_t bsearch(float d) const {  // pool.hpp:51-60
         int64_t lo = 0, len = size;
         while (len > 1) {
             int64_t half = len >> 1;
@@ -78,7 +87,7 @@ struct Pool {
         }
         return (lo < size && dist[lo] < d) ? lo + 1 : lo;
     }
-    void insert(uint32_t u, float d) {  // pool.hpp:233-246
+    void insert(uint32_t u, float d) {  // pool.hpp:17-30
         if (greater) d = -d;
         if (size == cap && d > dist[size - 1]) return;
         int64_t lo = bsearch(d);
@@ -88,7 +97,7 @@ struct Pool {
         dist[lo] = d;
         size += (size < cap);
     }
-    float distk_raw() const { return size == cap ? dist[size - 1] : FLT_MAX; }  // pool.hpp:262-265
+    float distk_raw() const { return size == cap ? dist[size - 1] : FLT_MAX; }  // pool.hpp:46-49
     float distk() const { return greater ? -distk_raw() : distk_raw(); }
 };
 
@@ -186,9 +195,9 @@ struct SegEst {
         lut16.resize(D / 4 * 16);
         orc_lut_prepare(D, r.data(), lut16.data(), &sc);
         q_l2sqr = sc.q_l2sqr;
-        // lut.hpp:491-492: set1_ps(0.5 * sum_q_ - const_bound * q_l2norm_), set1_ps(4 / 0.8f * one_over_sqrtD_)
+        // lut.hpp:104-105: set1_ps(0.5 * sum_q_ - const_bound * q_l2norm_), set1_ps(4 / 0.8f * one_over_sqrtD_)
         sumq_cb = (float)(0.5 * (double)sc.sum_q - (double)(0.58f * sc.q_l2norm));
-        const float one_over_sqrtD = (float)(1.0 / (double)std::sqrt((float)D));  // lut.hpp:434
+        const float one_over_sqrtD = (float)(1.0 / (double)std::sqrt((float)D));  // lut.hpp:47
         rescale_sqrtD = (4 / 0.8f) * one_over_sqrtD;
     }
     void vars_est(const float *x, float *out) const {  // caq_estimator.hpp:116-136
@@ -206,7 +215,7 @@ struct SegEst {
         }
         uint32_t S[kBlock];
         orc_accumulate(codes, lut16.data(), D, S);
-        for (int j = 0; j < kBlock; ++j) {  // lut.hpp:494-505
+        for (int j = 0; j < kBlock; ++j) {  // lut.hpp:107-118
             float t = (float)(int32_t)S[j] * sc.delta;
             t = t + sc.sum_vl_lut;
             ipxb[j] = t;
@@ -223,7 +232,7 @@ struct SegEst {
         const float o_l2sqr = x * x;
         if (bits == 0) return ip ? ip_q_c : o_l2sqr + q_l2sqr;
         const float sqd = (float)(2.0 / (1 << bits));
-        const double ex = (double)orc_compute_ip(bits - 1, r.data(), lc, D);  // lut.hpp:508-513
+        const double ex = (double)orc_compute_ip(bits - 1, r.data(), lc, D);  // lut.hpp:121-126
         const float t = (float)(((double)ipxb[j] + ex * (double)sqd) + (-1.0 + (double)(sqd / 2)) * (double)sc.sum_q);
         const float ipoq = (*rescale_ptr) * t;
         acc_bits += D * (bits - 1);
@@ -438,7 +447,7 @@ void orc_lut_prepare(int64_t D, const float *r, uint16_t *lut16, orc_lut_scalars
         vr = std::max(vr, lf[i]);
     }
     sc->delta = (float)((double)(vr - vl) / ((1 << 16) - 0.01));
-    // lut.hpp:464 divides by the loop-invariant delta_; the reference's release build (-Ofast,
+    // lut.hpp:74-77 divides by the loop-invariant delta_; the reference's release build (-Ofast,
     // i.e. -freciprocal-math) hoists 1/delta_ and multiplies.  Restated as built, which the
     // reference kernels compiled with those flags confirm entry for entry.
     const float inv = 1.0f / sc->delta;
@@ -743,7 +752,7 @@ void orc_cluster_sizes(const orc_plan_t *plan, int64_t n, orc_cluster_sizes_t *o
 int orc_quantize_segment(const orc_plan_t *plan, int seg, const float *X, int64_t n, const float *centroid,
                          const float *P, const orc_caq_cfg_t *cfg, orc_cluster_out_t *clu, float *O_out) {
     // QuantizerCluster::quantize (quantizer.hpp:40-73) + ClusterPacker (cluster_packer.hpp:59-139).
-    // X: n x D_s rows already sliced / zero-padded for this segment (saq_quantizer.hpp:88-108).
+    // X: n x D_s rows already sliced / zero-padded for this segment (saq_quantizer.hpp:40-58).
     // Output arrays must be zero-initialised by the caller (padded lanes stay zero).
     const Layout L = make_layout(plan, n);
     const int64_t D = plan->seg_dim[seg];
@@ -921,7 +930,7 @@ int orc_search(const orc_plan_t *plan, const float *variance, const float *const
     }
     for (int64_t i = 0; i < topk; ++i) {
         out_ids[i] = pool.ids[i];
-        // ResultPool::get (pool.hpp:36): (greater ? -1 : 1) * stored distance, so an unfilled
+        // ResultPool::get (pool.hpp:38): (greater ? -1 : 1) * stored distance, so an unfilled
         // IP slot reads -0.0
         if (out_dists) out_dists[i] = (pool.greater ? -1.0f : 1.0f) * pool.dist[i];
     }
@@ -997,6 +1006,41 @@ int orc_estimate(const orc_plan_t *plan, const float *variance, const float *con
     }
     if (metrics) *metrics = mt;
     return (int)w;
+}
+
+int64_t orc_estimate_lanes(const orc_plan_t *plan, const float *variance, const float *const *P,
+                           const orc_cluster_t *clusters, const uint32_t *probe, int64_t nprobe, const float *query,
+                           float m, float *out_vars, float *out_fast, float *out_acc) {
+    // The per-segment estimates searchCluster combines (saq_searcher.hpp:95-158), for every lane of
+    // every block of the probed lists, padded lanes of a last block included: out_*[(blk * S + s) * 32
+    // + lane] with blk counted over the probed lists in order.  varsEstDist and compFastDist of every
+    // segment (a 0-bit segment's fast estimate is its variance estimate, caq_estimator.hpp:149-152);
+    // compAccurateDist of every real vector (0 for padded lanes).  L2 only.  Returns the block count.
+    QueryState qs;
+    init_query(plan, variance, P, query, m, 0, qs);
+    const int S = plan->num_seg;
+    int64_t w = 0;
+    for (int64_t pi = 0; pi < nprobe; ++pi) {
+        const orc_cluster_t &c = clusters[probe[pi]];
+        const Layout L = make_layout(plan, c.num_vec);
+        for (int s = 0; s < S; ++s) qs.seg[s].prepare(c.centroid_rot + L.cen_off[s]);
+        for (int64_t blk = 0; blk < L.num_blocks; ++blk, ++w) {
+            for (int s = 0; s < S; ++s) {
+                float *v = out_vars + (w * S + s) * kBlock, *f = out_fast + (w * S + s) * kBlock;
+                float *a = out_acc + (w * S + s) * kBlock;
+                const float *x = o_l2norm_row(L, c, s, blk);
+                qs.seg[s].vars_est(x, v);
+                qs.seg[s].fast(x, block_code(L, c, s, blk), f);
+                for (int j = 0; j < kBlock; ++j) {
+                    const int64_t idx = blk * kBlock + j;
+                    a[j] = idx < c.num_vec ? qs.seg[s].accurate(x[j], long_code(L, c, s, idx), j,
+                                                                 &c.long_factors[(idx * S + s) * 2])
+                                           : 0.0f;
+                }
+            }
+        }
+    }
+    return w;
 }
 
 // ---------------------------------------------------------------- single-vector path

@@ -136,6 +136,10 @@ int orc_estimate(const orc_plan_t *plan, const float *variance, const float *con
                  const float *query, float m, int dist_type, uint32_t *out_ids, float *out_acc,
                  float *out_fast, float *out_vars, orc_metrics_t *metrics);
 
+int64_t orc_estimate_lanes(const orc_plan_t *plan, const float *variance, const float *const *P,
+                           const orc_cluster_t *clusters, const uint32_t *probe, int64_t nprobe, const float *query,
+                           float m, float *out_vars, float *out_fast, float *out_acc);
+
 /* ---- single-vector quantiser / estimator (SaqSingleDataWrapper records) ---- */
 /* Record layout of SaqSingleDataWrapper (quantization/single_data.hpp:180-246): 2 short
  * factors per segment (o_l2norm, ip_cent_oa), padded to 64 bytes; the short codes of the

@@ -33,7 +33,7 @@ struct SegDesc {
     int32_t ngroups;     // D_s / 64 when bits > 0, else 0
     int32_t long_off;    // byte offset of the segment's slot in a vector's long record
     int32_t long_bytes;  // D_s * (b_s - 1) / 8
-    float rescale_sqrtD; // (4 / 0.8f) * (float)(1.0 / sqrt((float)D_s)), lut.hpp:434,492
+    float rescale_sqrtD; // (4 / 0.8f) * (float)(1.0 / sqrt((float)D_s)), lut.hpp:47,492
     float sqd;           // (float)(2.0 / (1 << b_s)), caq_estimator.hpp:59
     int32_t pad_;
 };

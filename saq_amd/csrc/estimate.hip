@@ -123,7 +123,7 @@ __global__ __launch_bounds__(256) void estimate_kernel(EstArgs a) {
                 if ((endmask >> g) & 1ull) {
                     const int s = m.bseq[bk++];
                     const float *sc = m.sc + s * SC_N;
-                    float t2 = (float)(int32_t)acc * sc[SC_DELTA];  // lut.hpp:495-497
+                    float t2 = (float)(int32_t)acc * sc[SC_DELTA];  // lut.hpp:108-109
                     t2 = t2 + sc[SC_SUMVL];
                     m.ipq[s * 256 + v] = t2;
                     acc = 0;

@@ -207,13 +207,13 @@ __device__ void prepare_list(const SegDesc *segs, int S, const int8_t *gseg, int
             float *ss = sc + s * SC_N;
             const float delta = (float)((double)(mx - mn) / ((1 << 16) - 0.01));
             ss[SC_DELTA] = delta;
-            ss[SC_INV] = 1.0f / delta;  // the release build's hoisted reciprocal (lut.hpp:464)
+            ss[SC_INV] = 1.0f / delta;  // the release build's hoisted reciprocal (lut.hpp:74-77)
             ss[SC_VL] = mn;
             ss[SC_SUMVL] = (float)(((double)mn + 0.5 * (double)delta) * (double)(float)(sd.dim / 4));
         }
     }
     __syncthreads();
-    // LUT pass 2: quantise to u16 (lut.hpp:461-465)
+    // LUT pass 2: quantise to u16 (lut.hpp:74-77)
     for (int c = tid; c < ncb; c += kPrepThreads) {
         const int g = c >> 4;
         const int s = gseg[g];

@@ -106,7 +106,7 @@ __device__ __forceinline__ float stage3_step(const SegDesc &sd, const float *sc,
         f = f * x;
         cd = DT == 1 ? f * 0.5f + sc[SC_IPQC] : max0(x * x + (sc[SC_QL2] - f));
         const float sqd = sd.sqd;
-        // Lut::getExtIP (lut.hpp:508-513)
+        // Lut::getExtIP (lut.hpp:121-126)
         const float t = (float)(((double)ipq + (double)ex * (double)sqd) + (-1.0 + (double)(sqd / 2)) * (double)sc[SC_SUMQ]);
         const float ipoq = resc * t;
         av = DT == 1 ? ipoq + sc[SC_IPQC] : x * x + sc[SC_QL2] - 2 * ipoq;
@@ -250,7 +250,7 @@ __global__ __launch_bounds__(kLT) void list_scan_kernel(SliceArgs a) {
                     if ((endmask >> g) & 1ull) {
                         const int s = m.bseq[bk++];
                         const float *sc = m.sc + s * SC_N;
-                        float t2 = (float)(int32_t)acc * sc[SC_DELTA];  // lut.hpp:495-497
+                        float t2 = (float)(int32_t)acc * sc[SC_DELTA];  // lut.hpp:108-109
                         t2 = t2 + sc[SC_SUMVL];
                         m.ipq[s * kLT + v] = t2;
                         acc = 0;
@@ -650,7 +650,7 @@ __global__ __launch_bounds__(64) void replay_kernel(SliceArgs a) {
 
     float tail = size == K ? pool.dist_at(K - 1) : FLT_MAX;  // raw (negated for IP)
     float dk = DT == 1 ? -tail : tail;
-    // ResultPool::insert (pool.hpp:233-246): reject when full and d > tail; the new entry goes
+    // ResultPool::insert (pool.hpp:17-30): reject when full and d > tail; the new entry goes
     // before every entry >= d (ties: new before old), the rest move up one, the (k+1)-th drops
     auto insert = [&](uint32_t u, float d) {
         if (DT == 1) d = -d;

@@ -123,6 +123,9 @@ def oracle():
         L.orc_estimate.restype = C.c_int
         L.orc_estimate.argtypes = [C.POINTER(Plan), f32p, C.c_void_p, C.c_void_p, u32p, i64, f32p,
                                    C.c_float, C.c_int, u32p, f32p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_estimate_lanes.restype = i64
+        L.orc_estimate_lanes.argtypes = [C.POINTER(Plan), f32p, C.c_void_p, C.c_void_p, u32p, i64, f32p,
+                                         C.c_float, f32p, f32p, f32p]
         L.orc_single_layout.argtypes = [C.POINTER(Plan), C.POINTER(SingleLayout)]
         L.orc_single_transpose_bin.argtypes = [u16p, i64, u64p]
         L.orc_single_warmup_ip.restype = C.c_float

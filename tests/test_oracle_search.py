@@ -14,7 +14,13 @@ checks that a plausible mistake in the oracle's restatement would fail:
 * with k at least the number of probed vectors nothing can be pruned, so every probed
   vector comes back with its accurate distance (multi-segment plans included);
 * pruning only ever loses candidates: the i-th returned distance is never below the i-th
-  smallest accurate estimate.
+  smallest accurate estimate;
+* several segments (searchCluster, saq_searcher.hpp:68-173): an independent float32 model of the
+  three stages, fed the oracle's per-segment variance, fast and accurate estimates of every lane
+  (padded lanes of a last block included), must give the oracle search's IDs, distances and
+  QueryRuntimeMetrics bit for bit, while its cases skip blocks at stage 1, stop stage 2 early and
+  insert partial stage-3 sums; the model with a planted mistake (distk fixed per block, or the
+  complete instead of the partial sum inserted) no longer matches.
 """
 import ctypes as C
 
@@ -45,7 +51,7 @@ def estimate(case, probe, q, m=4.0):
 
 
 def pool_insert(pool, k, u, d):
-    """utils/pool.hpp:36-44: reject when full and d > tail; new entries go before equal ones."""
+    """utils/pool.hpp:17-30: reject when full and d > tail; new entries go before equal ones."""
     if len(pool) == k and d > pool[-1][1]:
         return
     pos = 0
@@ -160,3 +166,131 @@ def test_pruning_only_loses_candidates(plan):
         filled = np.count_nonzero(dists[i] != 0) or k
         assert np.all(dists[i][:filled] >= best[:filled] - 1e-5 * np.abs(best[:filled]))
         assert set(ids[i].tolist()) <= set(eid.tolist())
+
+
+# ---- searchCluster (saq_searcher.hpp:68-173), several segments
+def lanes(case, probe, q, m=4.0):
+    """The oracle's per-segment estimates of every lane of every probed block:
+    (vars, fast, acc), each blocks x S x 32 float32."""
+    O = oracle()
+    S = len(case.plan)
+    nb = int(sum((case.clusters[c]["num_vec"] + 31) // 32 for c in probe))
+    out = [np.zeros(max(1, nb) * S * 32, np.float32) for _ in range(3)]
+    vpad = np.zeros(case.D_pad, np.float32)
+    vpad[:case.num_dim] = case.variance
+    w = O.orc_estimate_lanes(C.byref(case.orc_plan()), vpad, F.orc_rot_ptrs(case), F.orc_clusters(case),
+                             np.ascontiguousarray(probe, np.uint32), len(probe), np.ascontiguousarray(q), m, *out)
+    assert w == nb
+    return [o[:nb * S * 32].reshape(nb, S, 32) for o in out]
+
+
+def search_cluster_model(case, probe, q, k, stats, mistake=None):
+    """saq_searcher.hpp:87-161 in float32, lane by lane, from the per-segment estimates alone:
+    stage 1 sums the variance estimates in segment order and skips the block when the minimum
+    over all 32 lanes (padded ones included) exceeds distk; stage 2 swaps, segment by segment
+    (bits > 0 only), the variance estimate for the fast one and stops when the minimum exceeds
+    distk; stage 3 (only when the last minimum is <= distk) takes lanes whose running sum is below
+    distk (re-read after every insert) up to the list's end, adds acc_s - cd_s per segment, stops
+    adding once the sum reaches distk, and inserts that (possibly partial) sum.
+    mistake (to show the comparison has power): "block_distk" keeps the block's starting distk
+    through stage 3, "full_sum" inserts the complete sum instead of the partial one."""
+    vars_, fast, acc = lanes(case, probe, q)
+    S = len(case.plan)
+    bits = [b for _, b in case.plan]
+    f32 = np.float32
+    pool = []
+    fast_bits = acc_bits = comp = 0
+    blk_g = 0
+    for c in probe:
+        n = int(case.clusters[c]["num_vec"])
+        ids = case.clusters[c]["ids"]
+        nb = (n + 31) // 32
+        distk = f32(pool[-1][1]) if len(pool) == k else f32(FLT_MAX)
+        for b in range(nb):
+            V, Fa, A = vars_[blk_g], fast[blk_g], acc[blk_g]
+            blk_g += 1
+            curr = np.zeros(32, np.float32)
+            for s in range(S):
+                curr = curr + V[s]
+            mi = curr.min()
+            if mi > distk:
+                stats["stage1_skip"] += 1
+                continue
+            cd = V.copy()
+            for s in range(S):
+                if bits[s] == 0:
+                    continue
+                curr = curr - cd[s]
+                cd[s] = Fa[s]
+                fast_bits += 32 * case.plan[s][0]
+                curr = curr + cd[s]
+                mi = curr.min()
+                if mi > distk:
+                    stats["stage2_break"] += 1
+                    break
+            if mi > distk:
+                continue
+            for j in range(32):
+                if not curr[j] < distk:
+                    continue
+                idx = 32 * b + j
+                if idx >= n:
+                    break
+                a = f32(curr[j])
+                for s in range(S):
+                    a = f32(a + f32(A[s, j] - cd[s, j]))
+                    if bits[s] > 0:
+                        acc_bits += case.plan[s][0] * (bits[s] - 1)
+                    if a >= distk and mistake != "full_sum":
+                        if s < S - 1:
+                            stats["partial"] += 1
+                        break
+                pool_insert(pool, k, int(ids[idx]), float(a))
+                if mistake != "block_distk":
+                    distk = f32(pool[-1][1]) if len(pool) == k else f32(FLT_MAX)
+        comp += nb * 32
+    return pool, (fast_bits, acc_bits, comp)
+
+
+@pytest.mark.parametrize("plan", [[(64, 4), (64, 2)], [(128, 7), (64, 3), (64, 0)], [(64, 1), (64, 5), (128, 2)],
+                                  [(64, 0), (64, 6), (64, 0), (64, 3)]])
+@pytest.mark.parametrize("topk", [1, 5, 20])
+def test_multi_segment_search_matches_block_model(plan, topk):
+    d = sum(x for x, _ in plan)
+    sizes = [96, 37, 160, 64, 33, 10, 250]
+    case = F.make_case(n=sum(sizes), d=d, K=len(sizes), nq=12, plan=plan, seed=31 + topk, tau=16,
+                       cluster_ids=skewed_ids(sum(sizes), len(sizes), 7, sizes))
+    probes = F.orc_probes(case, len(sizes))
+    ids, dists, met = F.orc_search(case, probes, topk)
+    stats = {"stage1_skip": 0, "stage2_break": 0, "partial": 0}
+    for i in range(case.queries.shape[0]):
+        pool, mets = search_cluster_model(case, probes[i], case.queries[i], topk, stats)
+        want_ids = [u for u, _ in pool] + [0] * (topk - len(pool))
+        assert list(ids[i]) == want_ids, f"query {i}"
+        got = dists[i][:len(pool)]
+        assert np.array_equal(got.view(np.uint32), np.array([d for _, d in pool], np.float32).view(np.uint32))
+        assert tuple(int(x) for x in met[i]) == mets, f"query {i} metrics"
+    assert stats["partial"] > 0, "no partial stage-3 sum was inserted"
+
+
+@pytest.mark.parametrize("mistake", [None, "block_distk", "full_sum"])
+def test_multi_segment_model_branches_and_power(mistake):
+    """One configuration that skips blocks at stage 1, stops stage 2 early and inserts partial
+    stage-3 sums; the model with a planted mistake no longer reproduces the oracle."""
+    plan = [(128, 7), (64, 3), (64, 0)]
+    sizes = [96, 37, 160, 64, 33, 10, 250]
+    case = F.make_case(n=sum(sizes), d=256, K=len(sizes), nq=12, plan=plan, seed=32, tau=16,
+                       cluster_ids=skewed_ids(sum(sizes), len(sizes), 7, sizes))
+    probes = F.orc_probes(case, len(sizes))
+    ids, dists, met = F.orc_search(case, probes, 5)
+    stats = {"stage1_skip": 0, "stage2_break": 0, "partial": 0}
+    same = True
+    for i in range(case.queries.shape[0]):
+        pool, mets = search_cluster_model(case, probes[i], case.queries[i], 5, stats, mistake)
+        same &= [u for u, _ in pool] == list(ids[i]) and \
+            np.array_equal(np.array([d for _, d in pool], np.float32), dists[i]) and \
+            tuple(int(x) for x in met[i]) == mets
+    if mistake is None:
+        assert same and all(v > 0 for v in stats.values()), stats
+    else:
+        assert not same
