@@ -138,7 +138,7 @@ int saqgpu_probe(saqgpu_index *idx, const float *queries, int64_t nq, int32_t np
  * allocates nothing: queries (nq x num_dim) and probes (nq x nprobe) are device pointers on the
  * index's device, the work is queued on stream, and the (query, centroid) distances go to one
  * fixed device array per GPU of SAQGPU_NOALLOC_SCRATCH_FLOATS floats, held until the process
- * ends.  nq * num_cen beyond it is refused (SAQGPU_EINVAL): split the queries.  The lists equal
+ * ends.  nq * (num_cen + 1) beyond it is refused (SAQGPU_EINVAL): split the queries.  The lists equal
  * saqgpu_probe's.  Every call on a GPU shares that GPU's array, so calls that overlap on one GPU
  * (other indexes, other threads or streams) race; calls on one index are ordered as usual. */
 #define SAQGPU_NOALLOC_SCRATCH_FLOATS (1 << 24)

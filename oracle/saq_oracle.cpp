@@ -1008,6 +1008,39 @@ int orc_estimate(const orc_plan_t *plan, const float *variance, const float *con
     return (int)w;
 }
 
+void orc_decode_cluster(const orc_plan_t *plan, const orc_cluster_t *c, int seg, int32_t *codes, float *o_l2norm,
+                        float *rescale) {
+    // The full b-bit codes of one segment of a list, read back from its SaqCluData arrays: the
+    // MSB from the fast-scan blocks (inverse of pack_codes, fastscan.hpp:75-112, bits MSB-first per
+    // byte as cluster_packer.hpp:124-139 stores them), the low b-1 bits from the CodeHelper<b-1>
+    // long code; plus |o| (short factors) and the ExFactor rescale of every vector.
+    const Layout L = make_layout(plan, c->num_vec);
+    const int64_t D = plan->seg_dim[seg];
+    const int bits = plan->seg_bits[seg];
+    for (int64_t v = 0; v < c->num_vec; ++v) {
+        const int64_t blk = v / kBlock, j = v % kBlock;
+        o_l2norm[v] = o_l2norm_row(L, *c, seg, blk)[j];
+        rescale[v] = c->long_factors[(v * L.nseg + seg) * 2];
+        if (bits == 0) {
+            for (int64_t d = 0; d < D; ++d) codes[v * D + d] = 0;
+            continue;
+        }
+        const uint8_t *b = block_code(L, *c, seg, blk);
+        int jpos = 0;  // the byte lane of vector j within a 16-byte half (kPerm0)
+        while (kPerm0[jpos] != (j & 15)) ++jpos;
+        const uint8_t *lc = long_code(L, *c, seg, v);
+        for (int64_t i = 0; i < D / 8; ++i) {
+            const uint8_t hi = b[32 * i + jpos], lo = b[32 * i + 16 + jpos];
+            const uint8_t byte = j < 16 ? (uint8_t)(((hi & 15) << 4) | (lo & 15)) : (uint8_t)((hi & 0xF0) | (lo >> 4));
+            for (int t = 0; t < 8; ++t) {
+                const int64_t d = 8 * i + t;
+                const int msb = (byte >> (7 - t)) & 1;
+                codes[v * D + d] = (msb << (bits - 1)) | (bits > 1 ? orc_code_at(bits - 1, lc, D, d) : 0);
+            }
+        }
+    }
+}
+
 int64_t orc_estimate_lanes(const orc_plan_t *plan, const float *variance, const float *const *P,
                            const orc_cluster_t *clusters, const uint32_t *probe, int64_t nprobe, const float *query,
                            float m, float *out_vars, float *out_fast, float *out_acc) {

@@ -136,6 +136,8 @@ int orc_estimate(const orc_plan_t *plan, const float *variance, const float *con
                  const float *query, float m, int dist_type, uint32_t *out_ids, float *out_acc,
                  float *out_fast, float *out_vars, orc_metrics_t *metrics);
 
+void orc_decode_cluster(const orc_plan_t *plan, const orc_cluster_t *c, int seg, int32_t *codes, float *o_l2norm,
+                        float *rescale);
 int64_t orc_estimate_lanes(const orc_plan_t *plan, const float *variance, const float *const *P,
                            const orc_cluster_t *clusters, const uint32_t *probe, int64_t nprobe, const float *query,
                            float m, float *out_vars, float *out_fast, float *out_acc);

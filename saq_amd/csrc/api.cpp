@@ -26,7 +26,7 @@ hipError_t launch_query_prep(const IndexView &ix, const float *queries, int64_t 
                              const float *rot, const int64_t *rot_off, float m, float *qrot, float *bound,
                              int max_dim, hipStream_t st);
 hipError_t launch_probe(const float *centroids, int K, int D, const float *queries, int64_t nq, int nprobe,
-                        int dist_type, float *dist_ws, uint32_t *probes, hipStream_t st);
+                        int dist_type, float *dist_ws, int32_t *ties, uint32_t *probes, hipStream_t st);
 hipError_t launch_sqnorm_rows(const float *x, int64_t rows, int D, float *out, hipStream_t st);
 size_t estimate_smem_bytes(const IndexView &ix);
 hipError_t launch_estimate(const IndexView &ix, const float *qrot, const float *bound, const uint32_t *probes,
@@ -34,9 +34,9 @@ hipError_t launch_estimate(const IndexView &ix, const float *qrot, const float *
                            float *out_fast, float *out_vars, hipStream_t st);
 hipError_t launch_probe_select_ip(const float *ip, const float *cnorm, int K, int64_t nq, int nprobe, int dist_type,
                                   uint32_t *probes, hipStream_t st);
-hipError_t launch_probe_exact(const float *ip, const float *cnorm, const float *centroids, int K, int D,
+hipError_t launch_probe_exact(float *ip, const float *cnorm, const float *centroids, int K, int D,
                               const float *queries, int64_t nq, int nprobe, int dist_type, uint32_t *probes,
-                              hipStream_t st);
+                              int32_t *ties, hipStream_t st);
 hipError_t launch_encode_segment(const IndexView &ix, const float *data, const uint32_t *src, const int64_t *gvec,
                                  const int64_t *gblk, const int32_t *lane, const uint32_t *clu, const float *cent,
                                  const float *rot, int seg, int seg_dim, int64_t nslots, int64_t ld, int rd_lmt,
@@ -155,6 +155,7 @@ struct saqgpu_index {
     std::mutex mu;
     hipEvent_t ws_free = nullptr;
     DevBuf<float> ws_q, ws_qrot, ws_bound, ws_dists, ws_cdist;
+    DevBuf<int32_t> ws_ties;   // per query: its probe list holds exact distance ties
     DevBuf<int64_t> ws_eoffs;  // IVF::estimate: per probe its first output slot
     DevBuf<uint32_t> ws_eids;  //   and its outputs
     DevBuf<float> ws_eout;
@@ -319,17 +320,18 @@ int ip_gemm(saqgpu_index *ix, const float *dq, int64_t nq, hipStream_t st) {
 // floats.  SAQGPU_PROBE=tile / =gemm forces a form (gemm: any nprobe <= 1024).
 int exact_probe(saqgpu_index *ix, const float *dq, int64_t nq, int nprobe, int dist_type, uint32_t *probes,
                 hipStream_t st) {
+    if (ix->ws_ties.ensure(std::max<int64_t>(1, nq))) return fail(SAQGPU_ENOMEM, "workspace allocation failed");
     const char *e = getenv("SAQGPU_PROBE");
     const bool tile = e && !strcmp(e, "tile"), gemm = e && !strcmp(e, "gemm");
     if (!tile && nprobe <= (gemm ? 1024 : 64) && (size_t)ix->num_dim * 4 <= 48 * 1024) {
         int rc;
         if ((rc = ensure_cnorm(ix, st)) || (rc = ip_gemm(ix, dq, nq, st))) return rc;
         HIPCHK(launch_probe_exact(ix->ws_cdist.p, ix->cnorm.p, ix->centroids.p, ix->num_cen, ix->num_dim, dq, nq,
-                                  nprobe, dist_type, probes, st));
+                                  nprobe, dist_type, probes, ix->ws_ties.p, st));
         return SAQGPU_OK;
     }
-    HIPCHK(launch_probe(ix->centroids.p, ix->num_cen, ix->num_dim, dq, nq, nprobe, dist_type, ix->ws_cdist.p, probes,
-                        st));
+    HIPCHK(launch_probe(ix->centroids.p, ix->num_cen, ix->num_dim, dq, nq, nprobe, dist_type, ix->ws_cdist.p,
+                        ix->ws_ties.p, probes, st));
     return SAQGPU_OK;
 }
 
@@ -495,6 +497,7 @@ void saqgpu_index_destroy(saqgpu_index *ix) {
     ix->ws_probes.release();
     ix->ws_ids.release();
     ix->ws_met.release();
+    ix->ws_ties.release();
     ix->ws_eoffs.release();
     ix->ws_eids.release();
     ix->ws_eout.release();
@@ -748,8 +751,8 @@ int saqgpu_probe_noalloc(saqgpu_index *ix, const float *queries, int64_t nq, int
     if (nprobe < 1 || nprobe > ix->num_cen) return fail(SAQGPU_EINVAL, "nprobe must be in 1..K");
     if (nprobe > 4096) return fail(SAQGPU_EUNSUPPORTED, "nprobe above 4096 is not supported");
     if (dist_type != 0 && dist_type != 1) return fail(SAQGPU_EINVAL, "dist_type must be 0 (L2) or 1 (IP)");
-    if (nq > SAQGPU_NOALLOC_SCRATCH_FLOATS / ix->num_cen)
-        return fail(SAQGPU_EINVAL, "nq x num_cen exceeds the fixed scratch of SAQGPU_NOALLOC_SCRATCH_FLOATS "
+    if (nq > SAQGPU_NOALLOC_SCRATCH_FLOATS / (ix->num_cen + 1))
+        return fail(SAQGPU_EINVAL, "nq x (num_cen + 1) exceeds the fixed scratch of SAQGPU_NOALLOC_SCRATCH_FLOATS "
                                    "floats: split the queries");
     if (nq == 0) return SAQGPU_OK;
     hipStream_t st = (hipStream_t)stream;
@@ -757,9 +760,10 @@ int saqgpu_probe_noalloc(saqgpu_index *ix, const float *queries, int64_t nq, int
     if (int rc = call.begin()) return rc;
     float *ws = scratch_of(ix->device);
     if (!ws) return fail(SAQGPU_ESTATE, "device scratch not set up");
-    // every (query, centroid) distance is written before the selection reads it: nothing in the
-    // scratch needs clearing
-    HIPCHK(launch_probe(ix->centroids.p, ix->num_cen, ix->num_dim, queries, nq, nprobe, dist_type, ws, probes, st));
+    // the scratch holds nq x num_cen distances, then nq tie flags; every one is written before it is
+    // read, so nothing in it needs clearing
+    HIPCHK(launch_probe(ix->centroids.p, ix->num_cen, ix->num_dim, queries, nq, nprobe, dist_type, ws,
+                        reinterpret_cast<int32_t *>(ws + nq * ix->num_cen), probes, st));
     return SAQGPU_OK;
 }
 

@@ -6,13 +6,20 @@
 //                        16-lane packet accumulators, trailing packet, predux tree, scalar tail.
 //   probe_select_kernel  one workgroup per query selects the nprobe best centroids by an exact
 //                        64-bit radix select on (distance key, centroid id), then sorts them
-//                        ascending (descending similarity for IP).  Equal distances are ordered
-//                        by id; std::partial_sort leaves the order of exact ties unspecified.
-//                        Its kIP form reads q.c products of a library GEMM (the batched scan's
-//                        approximate coarse quantiser) and ranks by |c|^2 - 2 q.c.
+//                        ascending (descending similarity for IP).  That is the reference's
+//                        order whenever the distances involved are distinct; a query with two
+//                        equal distances among its first nprobe + 1 is flagged for
+//                        probe_heap_kernel.  Its kIP form reads q.c products of a library GEMM
+//                        (the batched scan's approximate coarse quantiser) and ranks by
+//                        |c|^2 - 2 q.c.
 //   probe_exact_kernel   the exact lists from a library GEMM's q.c: the centroids within a proven
 //                        fp32 error bound of the nprobe-th key, recomputed in Eigen's order
 //                        (by half-waves), then an exact (distance, id) selection.
+//   probe_heap_kernel    the flagged queries only: std::partial_sort's own order for exact ties,
+//                        by running the heap selection libstdc++ (GCC 11, the reference's
+//                        toolchain) implements -- make_heap over the first nprobe centroids,
+//                        a pop_heap for every later one that compares below the top, then
+//                        sort_heap -- over every centroid's distance in id order.
 //   sqnorm_rows_kernel   |x|^2 per row (the centroid norms of the GEMM-based forms).
 #include <cfloat>
 
@@ -278,6 +285,7 @@ struct RadixLds {
     int wsum[4];
     unsigned long long prefix;
     int rank, done;
+    int last_pass;  // the pass that ended the select: 4 and above only when float keys tie across the r-th
 };
 
 // The r-th smallest of the workgroup's distinct 64-bit keys (1 <= r <= their number), as the
@@ -314,6 +322,7 @@ __device__ __forceinline__ unsigned long long block_radix_kth(const unsigned lon
             sh.prefix = pre | ((unsigned long long)tid << shift);
             sh.rank = r - excl;
             sh.done = (r - excl) == h;
+            sh.last_pass = pass;
         }
         __syncthreads();
         pre = sh.prefix;
@@ -336,10 +345,15 @@ __device__ __forceinline__ unsigned long long block_radix_kth(const unsigned lon
 // (|q|^2 is the same for every centroid of a query and is left out), q.c itself for IP.
 // KPT > 0: K <= 256 KPT, the keys of a thread's centroids (tid + 256 j) stay in registers
 // across the passes; KPT = 0 forms them again in every pass.
+// ties (exact distances only, may be null): ties[q] = 1 when two of the query's nprobe + 1 smallest
+// distances are equal, where std::partial_sort's order is its own (probe_heap_kernel).  A tie across
+// the nprobe-th place is one the select resolves in the id bytes (a pass of 4 or more); one within
+// the selection shows as equal float keys side by side once they are ordered.
 template <int DT, bool kIP = false, int KPT = 0>
 __global__ __launch_bounds__(256) void probe_select_kernel(const float *__restrict__ dist, int K, int nprobe,
                                                            uint32_t *__restrict__ probes,
-                                                           const float *__restrict__ cnorm = nullptr) {
+                                                           const float *__restrict__ cnorm = nullptr,
+                                                           int32_t *__restrict__ ties = nullptr) {
     __shared__ RadixLds rsh;
     __shared__ int cnt_s;
     __shared__ unsigned long long sel[kSelMax];
@@ -377,12 +391,22 @@ __global__ __launch_bounds__(256) void probe_select_kernel(const float *__restri
     }
     __syncthreads();
     const int n = min(cnt_s, kSelMax);
+    const bool boundary_tie = rsh.last_pass >= 4;
     if (n <= 256) {
+        bool tie = false;
         if (tid < n) {
             const unsigned long long k = sel[tid];
-            int rank = 0;
-            for (int j = 0; j < n; ++j) rank += sel[j] < k;
+            int rank = 0, same = 0;
+            for (int j = 0; j < n; ++j) {
+                rank += sel[j] < k;
+                same += (sel[j] >> 32) == (k >> 32);
+            }
+            tie = same > 1;
             if (rank < nprobe) probes[q * nprobe + rank] = (uint32_t)(k & 0xFFFFFFFFu);
+        }
+        if (ties) {
+            tie = __syncthreads_or(tie);
+            if (tid == 0) ties[q] = (tie || boundary_tie) ? 1 : 0;
         }
         return;
     }
@@ -407,6 +431,12 @@ __global__ __launch_bounds__(256) void probe_select_kernel(const float *__restri
         }
     }
     for (int i = tid; i < nprobe && i < n; i += 256) probes[q * nprobe + i] = (uint32_t)(sel[i] & 0xFFFFFFFFu);
+    if (ties) {
+        bool tie = false;
+        for (int i = tid; i + 1 < n; i += 256) tie |= (sel[i] >> 32) == (sel[i + 1] >> 32);
+        tie = __syncthreads_or(tie);
+        if (tid == 0) ties[q] = (tie || boundary_tie) ? 1 : 0;
+    }
 }
 
 hipError_t launch_sqnorm_rows(const float *x, int64_t rows, int D, float *out, hipStream_t st) {
@@ -422,7 +452,7 @@ hipError_t launch_probe_select_ip(const float *ip, const float *cnorm, int K, in
     if (nprobe > kSelMax) return hipErrorInvalidValue;
     if (nq == 0) return hipSuccess;
     auto go = [&](auto kern) {
-        hipLaunchKernelGGL(kern, dim3((unsigned)nq), dim3(256), 0, st, ip, K, nprobe, probes, cnorm);
+        hipLaunchKernelGGL(kern, dim3((unsigned)nq), dim3(256), 0, st, ip, K, nprobe, probes, cnorm, (int32_t *)nullptr);
     };
     if (dist_type == 1)
         K <= 4096 ? go(probe_select_kernel<1, true, 16>) : go(probe_select_kernel<1, true, 0>);
@@ -456,11 +486,13 @@ __device__ __forceinline__ float block_max256(float v, float *wtmp) {
 
 // KPT > 0: K <= 256 KPT, and every thread keeps the keys of its centroids (tid + 256 j) in
 // registers for the passes below; KPT = 0 forms them again in every pass.
+// ties[q] as probe_select_kernel's (the candidates hold every centroid whose distance can reach the
+// nprobe-th, ties at the boundary included); the all-K fallback always flags its query.
 template <int DT, int KPT>
 __global__ __launch_bounds__(256) void probe_exact_kernel(const float *__restrict__ ip, const float *__restrict__ cnorm,
                                                           const float *__restrict__ cent, int K, int D,
                                                           const float *__restrict__ queries, int nprobe,
-                                                          uint32_t *__restrict__ probes) {
+                                                          uint32_t *__restrict__ probes, int32_t *__restrict__ ties) {
     extern __shared__ float qsh[];  // D floats
     __shared__ RadixLds rsh;
     __shared__ float wtmp[4];
@@ -603,6 +635,10 @@ __global__ __launch_bounds__(256) void probe_exact_kernel(const float *__restric
                 __syncthreads();
             }
         for (int i = tid; i < nprobe; i += 256) probes[q * nprobe + i] = (uint32_t)(sel[i] & 0xFFFFFFFFu);
+        bool tie = false;
+        for (int i = tid; i < nprobe && i + 1 < n; i += 256) tie |= (sel[i] >> 32) == (sel[i + 1] >> 32);
+        tie = __syncthreads_or(tie);
+        if (tid == 0) ties[q] = tie ? 1 : 0;
         return;
     }
     // fallback: the nprobe smallest exact keys over all K by a second radix select
@@ -638,24 +674,145 @@ __global__ __launch_bounds__(256) void probe_exact_kernel(const float *__restric
             __syncthreads();
         }
     for (int i = tid; i < nprobe; i += 256) probes[q * nprobe + i] = (uint32_t)(sel[i] & 0xFFFFFFFFu);
+    if (tid == 0) ties[q] = 1;  // every key tied somewhere near the boundary: let the heap order them
+}
+
+// ---- std::partial_sort's order for exact ties
+// The comparator of FlatInitializer::centroids_distances (initializer.hpp:61-70): Candidate's
+// operator< (distance <) for L2 and std::greater<>, i.e. Candidate's operator> = !(a < b)
+// (defines.hpp:52-54), for IP.
+template <int DT>
+__device__ __forceinline__ bool cand_less(float a, float b) {
+    return DT == 1 ? !(a < b) : a < b;
+}
+
+// libstdc++'s __adjust_heap (with its __push_heap) on the heap hd/hi of len elements: the hole at
+// `hole` sinks to a leaf along the larger children, then value (vd, vi) rises from there.
+template <int DT>
+__device__ void heap_adjust(float *hd, uint32_t *hi, int hole, int len, float vd, uint32_t vi) {
+    const int top = hole;
+    int child = hole;
+    while (child < (len - 1) / 2) {
+        child = 2 * (child + 1);
+        if (cand_less<DT>(hd[child], hd[child - 1])) child--;
+        hd[hole] = hd[child];
+        hi[hole] = hi[child];
+        hole = child;
+    }
+    if ((len & 1) == 0 && child == (len - 2) / 2) {
+        child = 2 * (child + 1);
+        hd[hole] = hd[child - 1];
+        hi[hole] = hi[child - 1];
+        hole = child - 1;
+    }
+    int parent = (hole - 1) / 2;
+    while (hole > top && cand_less<DT>(hd[parent], vd)) {
+        hd[hole] = hd[parent];
+        hi[hole] = hi[parent];
+        hole = parent;
+        parent = (hole - 1) / 2;
+    }
+    hd[hole] = vd;
+    hi[hole] = vi;
+}
+
+constexpr int kHeapChunk = 2048;  // distances staged in LDS per step of the selection
+
+// One workgroup per query; queries without a tie flag return at once.  recompute: dist holds GEMM
+// products, replaced first by the reference's distances (Eigen order, by half-waves) of every
+// centroid; otherwise dist already holds them.  Then one thread runs std::partial_sort (GCC 11
+// libstdc++: __heap_select = __make_heap of [0, nprobe) + __pop_heap of every later element that
+// compares below the top, then __sort_heap) over the candidates (id c, distance dist[c]) in id
+// order, the distances staged through LDS a chunk at a time by the whole workgroup.
+template <int DT>
+__global__ __launch_bounds__(256) void probe_heap_kernel(float *__restrict__ dist, int recompute,
+                                                         const float *__restrict__ cent, int K, int D,
+                                                         const float *__restrict__ queries, int nprobe,
+                                                         const int32_t *__restrict__ ties,
+                                                         uint32_t *__restrict__ probes) {
+    __shared__ float hd[kSelMax];
+    __shared__ uint32_t hi[kSelMax];
+    __shared__ float chunk[kHeapChunk];
+    const int64_t q = blockIdx.x;
+    if (!ties[q]) return;  // uniform per workgroup
+    const int tid = threadIdx.x;
+    float *row = dist + q * K;
+    const float *qv = queries + q * D;
+    if (recompute) {
+        const int half = tid >> 5, l = tid & 31;
+        for (int c0 = 0; c0 < K; c0 += 8) {
+            const int c = min(c0 + half, K - 1);
+            const float *cr = cent + (int64_t)c * D;
+            float d;
+            if (DT == 1)
+                d = eigen_redux_half(D, [&](int i) { return qv[i] * cr[i]; }, l);
+            else
+                d = eigen_redux_half(D, [&](int i) {
+                    const float t = cr[i] - qv[i];
+                    return t * t;
+                }, l);
+            if (c0 + half < K && l == 0) row[c] = d;
+        }
+        __syncthreads();  // the row's writes are visible to the workgroup
+    }
+    for (int c0 = 0; c0 < K; c0 += kHeapChunk) {
+        const int cn = min(kHeapChunk, K - c0);
+        for (int i = tid; i < cn; i += 256) chunk[i] = row[c0 + i];
+        __syncthreads();
+        if (tid == 0) {
+            for (int i = 0; i < cn; ++i) {
+                const int c = c0 + i;
+                const float d = chunk[i];
+                if (c < nprobe) {  // the first nprobe candidates in place
+                    hd[c] = d;
+                    hi[c] = (uint32_t)c;
+                    if (c == nprobe - 1 && nprobe >= 2)  // __make_heap
+                        for (int parent = (nprobe - 2) / 2;; --parent) {
+                            heap_adjust<DT>(hd, hi, parent, nprobe, hd[parent], hi[parent]);
+                            if (parent == 0) break;
+                        }
+                } else if (cand_less<DT>(d, hd[0])) {  // __pop_heap(first, middle, i)
+                    heap_adjust<DT>(hd, hi, 0, nprobe, d, (uint32_t)c);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (tid == 0)
+        for (int last = nprobe - 1; last >= 1; --last) {  // __sort_heap
+            const float vd = hd[last];
+            const uint32_t vi = hi[last];
+            hd[last] = hd[0];
+            hi[last] = hi[0];
+            heap_adjust<DT>(hd, hi, 0, last, vd, vi);
+        }
+    __syncthreads();
+    for (int i = tid; i < nprobe; i += 256) probes[q * nprobe + i] = hi[i];
 }
 
 // The exact coarse quantiser over GEMM products ip (nq x K) and centroid norms cnorm: probe
 // lists equal to probe_tile + probe_select's.  nprobe <= kExactCand.
-hipError_t launch_probe_exact(const float *ip, const float *cnorm, const float *centroids, int K, int D,
+// ip is overwritten with exact distances in the rows of queries with ties; ties: nq ints.
+hipError_t launch_probe_exact(float *ip, const float *cnorm, const float *centroids, int K, int D,
                               const float *queries, int64_t nq, int nprobe, int dist_type, uint32_t *probes,
-                              hipStream_t st) {
+                              int32_t *ties, hipStream_t st) {
     if (nprobe > kExactCand || (size_t)D * 4 > 48 * 1024) return hipErrorInvalidValue;
     if (nq == 0) return hipSuccess;
     const size_t sm = (size_t)D * 4;
     auto go = [&](auto kern) {
         hipLaunchKernelGGL(kern, dim3((unsigned)nq), dim3(256), sm, st, ip, cnorm, centroids, K, D, queries, nprobe,
-                           probes);
+                           probes, ties);
     };
     if (dist_type == 1)
         K <= 4096 ? go(probe_exact_kernel<1, 16>) : go(probe_exact_kernel<1, 0>);
     else
         K <= 4096 ? go(probe_exact_kernel<0, 16>) : go(probe_exact_kernel<0, 0>);
+    if (dist_type == 1)
+        hipLaunchKernelGGL(probe_heap_kernel<1>, dim3((unsigned)nq), dim3(256), 0, st, ip, 1, centroids, K, D, queries,
+                           nprobe, ties, probes);
+    else
+        hipLaunchKernelGGL(probe_heap_kernel<0>, dim3((unsigned)nq), dim3(256), 0, st, ip, 1, centroids, K, D, queries,
+                           nprobe, ties, probes);
     return hipGetLastError();
 }
 
@@ -668,8 +825,9 @@ static void probe_tile(bool vec, dim3 g, hipStream_t st, const float *centroids,
         hipLaunchKernelGGL((probe_tile_kernel<DT, false>), g, dim3(256), 0, st, centroids, K, D, queries, nq, dist_ws);
 }
 
+// dist_ws: nq x K floats; ties: nq ints.
 hipError_t launch_probe(const float *centroids, int K, int D, const float *queries, int64_t nq, int nprobe,
-                        int dist_type, float *dist_ws, uint32_t *probes, hipStream_t st) {
+                        int dist_type, float *dist_ws, int32_t *ties, uint32_t *probes, hipStream_t st) {
     if (nprobe > kSelMax) return hipErrorInvalidValue;
     const bool tiled = D >= 32 && (nq + kPT - 1) / kPT <= 65535;
     const bool vec = D % 4 == 0 && ((uintptr_t)centroids % 16) == 0 && ((uintptr_t)queries % 16) == 0;
@@ -683,14 +841,18 @@ hipError_t launch_probe(const float *centroids, int K, int D, const float *queri
         else
             hipLaunchKernelGGL(probe_dist_kernel<1>, g1, dim3(256), sm, st, centroids, K, D, queries, dist_ws);
         hipLaunchKernelGGL((probe_select_kernel<1, false>), dim3((unsigned)nq), dim3(256), 0, st, dist_ws, K, nprobe, probes,
-                           nullptr);
+                           nullptr, ties);
+        hipLaunchKernelGGL(probe_heap_kernel<1>, dim3((unsigned)nq), dim3(256), 0, st, dist_ws, 0, centroids, K, D,
+                           queries, nprobe, ties, probes);
     } else {
         if (tiled)
             probe_tile<0>(vec, gt, st, centroids, K, D, queries, nq, dist_ws);
         else
             hipLaunchKernelGGL(probe_dist_kernel<0>, g1, dim3(256), sm, st, centroids, K, D, queries, dist_ws);
         hipLaunchKernelGGL((probe_select_kernel<0, false>), dim3((unsigned)nq), dim3(256), 0, st, dist_ws, K, nprobe, probes,
-                           nullptr);
+                           nullptr, ties);
+        hipLaunchKernelGGL(probe_heap_kernel<0>, dim3((unsigned)nq), dim3(256), 0, st, dist_ws, 0, centroids, K, D,
+                           queries, nprobe, ties, probes);
     }
     return hipGetLastError();
 }

@@ -123,6 +123,7 @@ def oracle():
         L.orc_estimate.restype = C.c_int
         L.orc_estimate.argtypes = [C.POINTER(Plan), f32p, C.c_void_p, C.c_void_p, u32p, i64, f32p,
                                    C.c_float, C.c_int, u32p, f32p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_decode_cluster.argtypes = [C.POINTER(Plan), C.c_void_p, C.c_int, i32p, f32p, f32p]
         L.orc_estimate_lanes.restype = i64
         L.orc_estimate_lanes.argtypes = [C.POINTER(Plan), f32p, C.c_void_p, C.c_void_p, u32p, i64, f32p,
                                          C.c_float, f32p, f32p, f32p]

@@ -230,3 +230,34 @@ def test_inner_product_estimates_closed_forms(kw):
             assert np.all(np.abs(var - want_v) <= 1e-5 * (np.abs(want_v) + 1) * S)
             assert np.all(np.abs(fast - want_f) <= tol_f + 1e-5)
             assert np.all(np.abs(acc - want_a) <= tol_a + 1e-5)
+
+
+def decode(case, c, s):
+    """orc_decode_cluster: full codes, |o| and rescale of list c's segment s."""
+    O = oracle()
+    n = case.clusters[c]["num_vec"]
+    D = case.plan[s][0]
+    codes = np.zeros(max(1, n) * D, np.int32)
+    on = np.zeros(max(1, n), np.float32)
+    rs = np.zeros(max(1, n), np.float32)
+    clus = F.orc_clusters(case)
+    O.orc_decode_cluster(C.byref(case.orc_plan()), C.cast(C.pointer(clus[c]), C.c_void_p), s, codes, on, rs)
+    return codes[:n * D].reshape(n, D), on[:n], rs[:n]
+
+
+@pytest.mark.parametrize("kw", CASES)
+def test_decode_cluster_round_trip(kw):
+    """The bulk decoder the full-size GPU tests use reads back exactly the codes the encoder
+    produced (whose packing is checked against the list's bytes in segment_codes)."""
+    kw = dict(kw)
+    d = kw.pop("d")
+    case = F.make_case(n=300, d=d, K=3, nq=1, seed=47, **kw)
+    S = len(case.plan)
+    for c in range(case.K):
+        n = case.clusters[c]["num_vec"]
+        for s in range(S):
+            codes, on, rs = decode(case, c, s)
+            _, want = segment_codes(case, c, s)
+            assert np.array_equal(codes, want.astype(np.int32))
+            assert np.array_equal(on, o_l2norm(case, c, s, n))
+            assert np.array_equal(rs, case.clusters[c]["long_factors"].reshape(n, S, 2)[:, s, 0])

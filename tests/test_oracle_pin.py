@@ -390,3 +390,65 @@ def test_index_file_config_layout_matches_reference():
         IF.SINGLE_FIELDS[k][0] for k in ("quant_type", "random_rotation", "use_fastscan", "caq_adj_rd_lmt",
                                          "caq_adj_eps", "caq_ori_qB")]
     assert out[13] == 4 and out[14] == 1  # int-sized enum; use_fastscan defaults to true
+
+
+# ---- std::partial_sort's tie order (FlatInitializer::centroids_distances, initializer.hpp:52-76)
+def _adjust(h, hole, n, v, less):
+    top, child = hole, hole
+    while child < (n - 1) // 2:
+        child = 2 * (child + 1)
+        if less(h[child], h[child - 1]):
+            child -= 1
+        h[hole] = h[child]
+        hole = child
+    if n % 2 == 0 and child == (n - 2) // 2:
+        child = 2 * (child + 1)
+        h[hole] = h[child - 1]
+        hole = child - 1
+    parent = (hole - 1) // 2
+    while hole > top and less(h[parent], v):
+        h[hole] = h[parent]
+        hole = parent
+        parent = (hole - 1) // 2
+    h[hole] = v
+
+
+def stl_partial_sort(items, m, less):
+    """GCC 11 libstdc++'s std::partial_sort (bits/stl_algo.h __partial_sort = __heap_select +
+    __sort_heap; bits/stl_heap.h __make_heap / __pop_heap / __adjust_heap / __push_heap), the
+    library the reference builds with: the first m of items in its order."""
+    h = list(items[:m])
+    if m >= 2:
+        for parent in range((m - 2) // 2, -1, -1):
+            _adjust(h, parent, m, h[parent], less)
+    for x in items[m:]:
+        if less(x, h[0]):
+            _adjust(h, 0, m, x, less)
+    for last in range(m - 1, 0, -1):
+        v = h[last]
+        h[last] = h[0]
+        _adjust(h, 0, last, v, less)
+    return h
+
+
+@pytest.mark.parametrize("dist_type", [0, 1])
+def test_probe_order_is_libstdcxx_partial_sort(dist_type):
+    """The oracle's coarse quantiser orders exact distance ties as std::partial_sort does (the
+    reference's toolchain): Candidate's operator< for L2, std::greater<> through operator> =
+    !(a < b) for IP (defines.hpp:52-54).  Duplicated centroids make every query tie."""
+    O = oracle()
+    rng = np.random.default_rng(5)
+    K, d = 300, 16
+    cent = rng.integers(-2, 3, (K, d)).astype(np.float32)  # small integer grid: many exact ties
+    cent[150:] = cent[:150]
+    for qi in range(6):
+        q = rng.integers(-2, 3, d).astype(np.float32)
+        dist = np.array([O.orc_eigen_dot(q, np.ascontiguousarray(c), d) if dist_type else
+                         O.orc_eigen_diff_sqnorm(np.ascontiguousarray(c), q, d) for c in cent], np.float32)
+        items = [(i, float(dist[i])) for i in range(K)]
+        less = (lambda a, b: not (a[1] < b[1])) if dist_type else (lambda a, b: a[1] < b[1])
+        for m in (1, 2, 7, 150, 151, 299, 300):
+            want = [i for i, _ in stl_partial_sort(items, m, less)]
+            got = np.zeros(m, np.uint32)
+            O.orc_probe(cent, K, d, q, m, dist_type, got, None)
+            assert list(got) == want, f"query {qi} nprobe {m}"
