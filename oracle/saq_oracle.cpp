@@ -27,8 +27,13 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <atomic>
+#include <chrono>
 #include <thread>
 #include <vector>
+#if defined(ORC_AVX512)
+#include <immintrin.h>
+#endif
 
 namespace {
 
@@ -178,6 +183,9 @@ struct SegEst {
     std::vector<float> qrot;  // query_data_ (rotated query segment)
     std::vector<float> r;     // Lut::query_ (residual, or the query for IP)
     std::vector<uint16_t> lut16;
+#if defined(ORC_AVX512)
+    std::vector<uint8_t> lutpk;  // per 8-dim column: lo bytes of its 2 codebooks' LUTs, then hi bytes
+#endif
     orc_lut_scalars_t sc{};
     float bound = 0, ip_q_c = 0, q_l2sqr = 0;
     float sumq_cb = 0, rescale_sqrtD = 0;
@@ -194,6 +202,16 @@ struct SegEst {
         }
         lut16.resize(D / 4 * 16);
         orc_lut_prepare(D, r.data(), lut16.data(), &sc);
+#if defined(ORC_AVX512)
+        lutpk.resize(D / 8 * 64);
+        for (int64_t i = 0; i < D / 8; ++i)
+            for (int h = 0; h < 2; ++h)
+                for (int e = 0; e < 16; ++e) {
+                    const uint16_t v = lut16[(2 * i + h) * 16 + e];
+                    lutpk[i * 64 + 16 * h + e] = (uint8_t)(v & 0xFF);
+                    lutpk[i * 64 + 32 + 16 * h + e] = (uint8_t)(v >> 8);
+                }
+#endif
         q_l2sqr = sc.q_l2sqr;
         // lut.hpp:104-105: set1_ps(0.5 * sum_q_ - const_bound * q_l2norm_), set1_ps(4 / 0.8f * one_over_sqrtD_)
         sumq_cb = (float)(0.5 * (double)sc.sum_q - (double)(0.58f * sc.q_l2norm));
@@ -214,7 +232,14 @@ struct SegEst {
             return;
         }
         uint32_t S[kBlock];
+#if defined(ORC_AVX512)
+        accumulate_avx512(codes, lutpk.data(), D, S);
+        fast_epilogue_avx512(S, x, out);
+        fast_bits += kBlock * D;
+        return;
+#else
         orc_accumulate(codes, lut16.data(), D, S);
+#endif
         for (int j = 0; j < kBlock; ++j) {  // lut.hpp:107-118
             float t = (float)(int32_t)S[j] * sc.delta;
             t = t + sc.sum_vl_lut;
@@ -228,6 +253,73 @@ struct SegEst {
         }
         fast_bits += kBlock * D;
     }
+#if defined(ORC_AVX512)
+    // The exact integer sums of accumulate_hacc with byte shuffles: per 8-dim column one 32-byte load
+    // (two codebooks x 32 vectors), its low and high nibbles index the LUTs' low and high bytes
+    // (vpshufb per 16-byte lane = per codebook), the bytes summed in u16 lanes and folded into u32
+    // every 128 columns (128 x 255 < 2^16).  Lane j of a half stands for vector kPerm0[j] (low
+    // nibbles) or kPerm0[j] + 16 (high nibbles).
+    static void accumulate_avx512(const uint8_t *codes, const uint8_t *lutpk, int64_t D, uint32_t *S) {
+        const __m256i nib = _mm256_set1_epi8(0x0F);
+        __m512i t32[4] = {_mm512_setzero_si512(), _mm512_setzero_si512(), _mm512_setzero_si512(),
+                          _mm512_setzero_si512()};  // (lo, hi) x (low-nibble vectors, high-nibble vectors)
+        const int64_t cols = D / 8;
+        for (int64_t c0 = 0; c0 < cols; c0 += 128) {
+            __m512i a_lo = _mm512_setzero_si512(), a_hi = _mm512_setzero_si512();
+            __m512i b_lo = _mm512_setzero_si512(), b_hi = _mm512_setzero_si512();
+            const int64_t c1 = std::min(cols, c0 + 128);
+            for (int64_t i = c0; i < c1; ++i) {
+                const __m256i cw = _mm256_loadu_si256(reinterpret_cast<const __m256i *>(codes + 32 * i));
+                const __m256i tlo = _mm256_loadu_si256(reinterpret_cast<const __m256i *>(lutpk + 64 * i));
+                const __m256i thi = _mm256_loadu_si256(reinterpret_cast<const __m256i *>(lutpk + 64 * i + 32));
+                const __m256i il = _mm256_and_si256(cw, nib), ih = _mm256_and_si256(_mm256_srli_epi16(cw, 4), nib);
+                a_lo = _mm512_add_epi16(a_lo, _mm512_cvtepu8_epi16(_mm256_shuffle_epi8(tlo, il)));
+                a_hi = _mm512_add_epi16(a_hi, _mm512_cvtepu8_epi16(_mm256_shuffle_epi8(thi, il)));
+                b_lo = _mm512_add_epi16(b_lo, _mm512_cvtepu8_epi16(_mm256_shuffle_epi8(tlo, ih)));
+                b_hi = _mm512_add_epi16(b_hi, _mm512_cvtepu8_epi16(_mm256_shuffle_epi8(thi, ih)));
+            }
+            const __m512i *acc[4] = {&a_lo, &a_hi, &b_lo, &b_hi};
+            for (int k = 0; k < 4; ++k) {
+                const __m512i v = *acc[k];
+                // u16 lanes 0..15: codebook 2i, 16..31: codebook 2i+1; both belong to the same vectors
+                const __m512i w = _mm512_add_epi32(_mm512_cvtepu16_epi32(_mm512_castsi512_si256(v)),
+                                                   _mm512_cvtepu16_epi32(_mm512_extracti64x4_epi64(v, 1)));
+                t32[k] = _mm512_add_epi32(t32[k], w);
+            }
+        }
+        alignas(64) uint32_t lo_a[16], hi_a[16], lo_b[16], hi_b[16];
+        _mm512_store_si512(reinterpret_cast<__m512i *>(lo_a), t32[0]);
+        _mm512_store_si512(reinterpret_cast<__m512i *>(hi_a), t32[1]);
+        _mm512_store_si512(reinterpret_cast<__m512i *>(lo_b), t32[2]);
+        _mm512_store_si512(reinterpret_cast<__m512i *>(hi_b), t32[3]);
+        for (int j = 0; j < 16; ++j) {
+            S[kPerm0[j]] = lo_a[j] + (hi_a[j] << 8);
+            S[kPerm0[j] + 16] = lo_b[j] + (hi_b[j] << 8);
+        }
+    }
+    // lut.hpp:107-118 + caq_estimator.hpp:161-176 on 16 lanes, the same IEEE operations in the same
+    // order as the scalar form (no contraction).
+    void fast_epilogue_avx512(const uint32_t *S, const float *x, float *out) {
+        const __m512 delta = _mm512_set1_ps(sc.delta), vl = _mm512_set1_ps(sc.sum_vl_lut);
+        const __m512 sq = _mm512_set1_ps(sumq_cb), rs = _mm512_set1_ps(rescale_sqrtD);
+        const __m512 q2 = _mm512_set1_ps(q_l2sqr), qc = _mm512_set1_ps(ip_q_c), half = _mm512_set1_ps(0.5f);
+        for (int h = 0; h < 2; ++h) {
+            const __m512i s = _mm512_loadu_si512(reinterpret_cast<const void *>(S + 16 * h));
+            __m512 t = _mm512_mul_ps(_mm512_cvtepi32_ps(s), delta);
+            t = _mm512_add_ps(t, vl);
+            _mm512_storeu_ps(ipxb + 16 * h, t);
+            const __m512 xv = _mm512_loadu_ps(x + 16 * h);
+            __m512 f = _mm512_mul_ps(_mm512_sub_ps(t, sq), rs);
+            f = _mm512_mul_ps(f, xv);
+            __m512 o;
+            if (ip)
+                o = _mm512_add_ps(_mm512_mul_ps(f, half), qc);
+            else
+                o = _mm512_max_ps(_mm512_setzero_ps(), _mm512_add_ps(_mm512_mul_ps(xv, xv), _mm512_sub_ps(q2, f)));
+            _mm512_storeu_ps(out + 16 * h, o);
+        }
+    }
+#endif
     float accurate(float x, const uint8_t *lc, int64_t j, const float *rescale_ptr) {  // :190-217
         const float o_l2sqr = x * x;
         if (bits == 0) return ip ? ip_q_c : o_l2sqr + q_l2sqr;
@@ -955,6 +1047,30 @@ int orc_search_batch(const orc_plan_t *plan, const float *variance, const float 
                 orc_search(plan, variance, P, clusters, probes + i * nprobe, nprobe, queries + i * plan->num_dim, topk,
                            m, dist_type, out_ids + i * topk, out_dists ? out_dists + i * topk : nullptr,
                            metrics ? metrics + i : nullptr);
+        });
+    for (auto &x : th) x.join();
+    return 0;
+}
+
+int orc_search_batch_timed(const orc_plan_t *plan, const float *variance, const float *const *P,
+                           const orc_cluster_t *clusters, const uint32_t *probes, int64_t nprobe,
+                           const int32_t *npq, const float *queries, int64_t nq, int64_t topk, float m,
+                           int nthreads, uint32_t *out_ids, float *out_dists, double *lat_ms) {
+    // test_qps's timing loop (src/test_qps.cpp:77-84, 115-123): threads take the next query from a
+    // shared counter and time each IVF::search call; lat_ms[i] = query i's wall time.  npq (may be
+    // null): per-query nprobe, the first npq[i] entries of its probe row.
+    if (nthreads < 1) nthreads = 1;
+    std::atomic<int64_t> next{0};
+    std::vector<std::thread> th;
+    for (int t = 0; t < nthreads; ++t)
+        th.emplace_back([&]() {
+            for (int64_t i; (i = next.fetch_add(1)) < nq;) {
+                const auto t0 = std::chrono::steady_clock::now();
+                orc_search(plan, variance, P, clusters, probes + i * nprobe, npq ? npq[i] : nprobe,
+                           queries + i * plan->num_dim, topk, m, 0, out_ids + i * topk,
+                           out_dists ? out_dists + i * topk : nullptr, nullptr);
+                lat_ms[i] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            }
         });
     for (auto &x : th) x.join();
     return 0;

@@ -131,6 +131,10 @@ int orc_search_batch(const orc_plan_t *plan, const float *variance, const float 
                      const orc_cluster_t *clusters, const uint32_t *probes, int64_t nprobe,
                      const float *queries, int64_t nq, int64_t topk, float m, int dist_type,
                      int nthreads, uint32_t *out_ids, float *out_dists, orc_metrics_t *metrics);
+int orc_search_batch_timed(const orc_plan_t *plan, const float *variance, const float *const *P,
+                           const orc_cluster_t *clusters, const uint32_t *probes, int64_t nprobe,
+                           const int32_t *npq, const float *queries, int64_t nq, int64_t topk, float m,
+                           int nthreads, uint32_t *out_ids, float *out_dists, double *lat_ms);
 int orc_estimate(const orc_plan_t *plan, const float *variance, const float *const *P,
                  const orc_cluster_t *clusters, const uint32_t *probe, int64_t nprobe,
                  const float *query, float m, int dist_type, uint32_t *out_ids, float *out_acc,

@@ -70,75 +70,104 @@ _orc = None
 _ref = None
 
 
+CPUBASE_SO = os.path.join(ROOT, "oracle", "libcpubase.so")
+_cpub = None
+
+
+def host_has_avx512():
+    try:
+        flags = open("/proc/cpuinfo").read()
+    except OSError:
+        return False
+    return all(f in flags for f in ("avx512f", "avx512bw", "avx512vl", "avx512dq"))
+
+
+def cpubase():
+    """The oracle with its fast scan on AVX-512 (the benchmark's CPU baseline), or None where the
+    host has no AVX-512 or the library is not built."""
+    global _cpub
+    if _cpub is None:
+        if not host_has_avx512() or not os.path.exists(CPUBASE_SO):
+            return None
+        _cpub = _bind(C.CDLL(CPUBASE_SO))
+    return _cpub
+
+
 def oracle():
     global _orc
     if _orc is None:
         if not os.path.exists(ORACLE_SO):
             raise RuntimeError("oracle not built: run make -C oracle")
-        L = C.CDLL(ORACLE_SO)
-        L.orc_eigen_sum.restype = C.c_float
-        L.orc_eigen_sum.argtypes = [f32p, i64]
-        L.orc_eigen_sqnorm.restype = C.c_float
-        L.orc_eigen_sqnorm.argtypes = [f32p, i64]
-        L.orc_eigen_dot.restype = C.c_float
-        L.orc_eigen_dot.argtypes = [f32p, f32p, i64]
-        L.orc_eigen_diff_sqnorm.restype = C.c_float
-        L.orc_eigen_diff_sqnorm.argtypes = [f32p, f32p, i64]
-        L.orc_eigen_var_q2.restype = C.c_float
-        L.orc_eigen_var_q2.argtypes = [f32p, f32p, i64]
-        L.orc_gemv_rowvec.argtypes = [f32p, f32p, i64, f32p]
-        L.orc_tree16.restype = C.c_float
-        L.orc_tree16.argtypes = [f32p]
-        L.orc_pack_lut.argtypes = [i64, f32p, f32p]
-        L.orc_pack_codes.argtypes = [i64, u8p, i64, u8p]
-        L.orc_lut_prepare.argtypes = [i64, f32p, u16p, C.POINTER(LutScalars)]
-        L.orc_accumulate.argtypes = [u8p, u16p, i64, u32p]
-        L.orc_cvt_u16.restype = C.c_uint16
-        L.orc_cvt_u16.argtypes = [C.c_float]
-        L.orc_compute_ip.restype = C.c_float
-        L.orc_compute_ip.argtypes = [C.c_int, f32p, u8p, i64]
-        L.orc_compact_code.argtypes = [C.c_int, u16p, i64, u8p]
-        L.orc_code_at.restype = C.c_int
-        L.orc_code_at.argtypes = [C.c_int, u8p, i64, i64]
-        L.orc_caq_encode.restype = C.c_int
-        L.orc_caq_encode.argtypes = [f32p, i64, C.c_int, C.POINTER(CaqCfg), C.c_void_p, u16p,
-                                     C.POINTER(CodeInfo)]
-        L.orc_rotate_rows.argtypes = [f32p, i64, i64, C.c_void_p, f32p, f32p]
-        L.orc_quantize_segment.restype = C.c_int
-        L.orc_quantize_segment.argtypes = [C.POINTER(Plan), C.c_int, f32p, i64, f32p, C.c_void_p,
-                                           C.POINTER(CaqCfg), C.c_void_p, C.c_void_p]
-        L.orc_cluster_sizes.argtypes = [C.POINTER(Plan), i64, C.POINTER(Sizes)]
-        L.orc_plan_dp.restype = C.c_int
-        L.orc_plan_dp.argtypes = [f32p, i64, C.c_float, i32p, i32p]
-        L.orc_plan_eqseg.restype = C.c_int
-        L.orc_plan_eqseg.argtypes = [i64, C.c_float, C.c_int, i32p, i32p]
-        L.orc_probe.argtypes = [f32p, i64, i64, f32p, i64, C.c_int, u32p, C.c_void_p]
-        L.orc_search.restype = C.c_int
-        L.orc_search.argtypes = [C.POINTER(Plan), f32p, C.c_void_p, C.c_void_p, u32p, i64, f32p, i64,
-                                 C.c_float, C.c_int, u32p, C.c_void_p, C.c_void_p]
-        L.orc_search_batch.restype = C.c_int
-        L.orc_search_batch.argtypes = [C.POINTER(Plan), f32p, C.c_void_p, C.c_void_p, u32p, i64, f32p,
-                                       i64, i64, C.c_float, C.c_int, C.c_int, u32p, C.c_void_p,
-                                       C.c_void_p]
-        L.orc_estimate.restype = C.c_int
-        L.orc_estimate.argtypes = [C.POINTER(Plan), f32p, C.c_void_p, C.c_void_p, u32p, i64, f32p,
-                                   C.c_float, C.c_int, u32p, f32p, C.c_void_p, C.c_void_p, C.c_void_p]
-        L.orc_decode_cluster.argtypes = [C.POINTER(Plan), C.c_void_p, C.c_int, i32p, f32p, f32p]
-        L.orc_estimate_lanes.restype = i64
-        L.orc_estimate_lanes.argtypes = [C.POINTER(Plan), f32p, C.c_void_p, C.c_void_p, u32p, i64, f32p,
-                                         C.c_float, f32p, f32p, f32p]
-        L.orc_single_layout.argtypes = [C.POINTER(Plan), C.POINTER(SingleLayout)]
-        L.orc_single_transpose_bin.argtypes = [u16p, i64, u64p]
-        L.orc_single_warmup_ip.restype = C.c_float
-        L.orc_single_warmup_ip.argtypes = [u64p, u64p, C.c_float, C.c_float, i64]
-        L.orc_single_mask_ip.restype = C.c_float
-        L.orc_single_mask_ip.argtypes = [f32p, u64p, i64]
-        L.orc_single_quantize.restype = C.c_int64
-        L.orc_single_quantize.argtypes = [C.POINTER(Plan), C.c_void_p, f32p, i64, C.POINTER(CaqCfg), u8p, i64]
-        L.orc_single_estimate.argtypes = [C.POINTER(Plan), f32p, C.c_void_p, u8p, i64, i64, f32p, C.c_float,
-                                          C.c_int, f32p, f32p, f32p]
-        _orc = L
+        _orc = _bind(C.CDLL(ORACLE_SO))
     return _orc
+
+
+def _bind(L):
+    L.orc_eigen_sum.restype = C.c_float
+    L.orc_eigen_sum.argtypes = [f32p, i64]
+    L.orc_eigen_sqnorm.restype = C.c_float
+    L.orc_eigen_sqnorm.argtypes = [f32p, i64]
+    L.orc_eigen_dot.restype = C.c_float
+    L.orc_eigen_dot.argtypes = [f32p, f32p, i64]
+    L.orc_eigen_diff_sqnorm.restype = C.c_float
+    L.orc_eigen_diff_sqnorm.argtypes = [f32p, f32p, i64]
+    L.orc_eigen_var_q2.restype = C.c_float
+    L.orc_eigen_var_q2.argtypes = [f32p, f32p, i64]
+    L.orc_gemv_rowvec.argtypes = [f32p, f32p, i64, f32p]
+    L.orc_tree16.restype = C.c_float
+    L.orc_tree16.argtypes = [f32p]
+    L.orc_pack_lut.argtypes = [i64, f32p, f32p]
+    L.orc_pack_codes.argtypes = [i64, u8p, i64, u8p]
+    L.orc_lut_prepare.argtypes = [i64, f32p, u16p, C.POINTER(LutScalars)]
+    L.orc_accumulate.argtypes = [u8p, u16p, i64, u32p]
+    L.orc_cvt_u16.restype = C.c_uint16
+    L.orc_cvt_u16.argtypes = [C.c_float]
+    L.orc_compute_ip.restype = C.c_float
+    L.orc_compute_ip.argtypes = [C.c_int, f32p, u8p, i64]
+    L.orc_compact_code.argtypes = [C.c_int, u16p, i64, u8p]
+    L.orc_code_at.restype = C.c_int
+    L.orc_code_at.argtypes = [C.c_int, u8p, i64, i64]
+    L.orc_caq_encode.restype = C.c_int
+    L.orc_caq_encode.argtypes = [f32p, i64, C.c_int, C.POINTER(CaqCfg), C.c_void_p, u16p,
+                                 C.POINTER(CodeInfo)]
+    L.orc_rotate_rows.argtypes = [f32p, i64, i64, C.c_void_p, f32p, f32p]
+    L.orc_quantize_segment.restype = C.c_int
+    L.orc_quantize_segment.argtypes = [C.POINTER(Plan), C.c_int, f32p, i64, f32p, C.c_void_p,
+                                       C.POINTER(CaqCfg), C.c_void_p, C.c_void_p]
+    L.orc_cluster_sizes.argtypes = [C.POINTER(Plan), i64, C.POINTER(Sizes)]
+    L.orc_plan_dp.restype = C.c_int
+    L.orc_plan_dp.argtypes = [f32p, i64, C.c_float, i32p, i32p]
+    L.orc_plan_eqseg.restype = C.c_int
+    L.orc_plan_eqseg.argtypes = [i64, C.c_float, C.c_int, i32p, i32p]
+    L.orc_probe.argtypes = [f32p, i64, i64, f32p, i64, C.c_int, u32p, C.c_void_p]
+    L.orc_search.restype = C.c_int
+    L.orc_search.argtypes = [C.POINTER(Plan), f32p, C.c_void_p, C.c_void_p, u32p, i64, f32p, i64,
+                             C.c_float, C.c_int, u32p, C.c_void_p, C.c_void_p]
+    L.orc_search_batch.restype = C.c_int
+    L.orc_search_batch.argtypes = [C.POINTER(Plan), f32p, C.c_void_p, C.c_void_p, u32p, i64, f32p,
+                                   i64, i64, C.c_float, C.c_int, C.c_int, u32p, C.c_void_p,
+                                   C.c_void_p]
+    L.orc_estimate.restype = C.c_int
+    L.orc_estimate.argtypes = [C.POINTER(Plan), f32p, C.c_void_p, C.c_void_p, u32p, i64, f32p,
+                               C.c_float, C.c_int, u32p, f32p, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.orc_decode_cluster.argtypes = [C.POINTER(Plan), C.c_void_p, C.c_int, i32p, f32p, f32p]
+    L.orc_estimate_lanes.restype = i64
+    L.orc_estimate_lanes.argtypes = [C.POINTER(Plan), f32p, C.c_void_p, C.c_void_p, u32p, i64, f32p,
+                                     C.c_float, f32p, f32p, f32p]
+    L.orc_single_layout.argtypes = [C.POINTER(Plan), C.POINTER(SingleLayout)]
+    L.orc_single_transpose_bin.argtypes = [u16p, i64, u64p]
+    L.orc_single_warmup_ip.restype = C.c_float
+    L.orc_single_warmup_ip.argtypes = [u64p, u64p, C.c_float, C.c_float, i64]
+    L.orc_single_mask_ip.restype = C.c_float
+    L.orc_single_mask_ip.argtypes = [f32p, u64p, i64]
+    L.orc_single_quantize.restype = C.c_int64
+    L.orc_single_quantize.argtypes = [C.POINTER(Plan), C.c_void_p, f32p, i64, C.POINTER(CaqCfg), u8p, i64]
+    L.orc_single_estimate.argtypes = [C.POINTER(Plan), f32p, C.c_void_p, u8p, i64, i64, f32p, C.c_float,
+                                      C.c_int, f32p, f32p, f32p]
+    L.orc_search_batch_timed.argtypes = [C.POINTER(Plan), f32p, C.c_void_p, C.c_void_p, u32p, i64, C.c_void_p,
+                                         f32p, i64, i64, C.c_float, C.c_int, u32p, C.c_void_p,
+                                         np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")]
+    return L
 
 
 def ref():
