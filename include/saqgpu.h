@@ -36,6 +36,12 @@ typedef enum {
 #define SAQGPU_MAX_GROUPS 64
 #define SAQGPU_MAX_TOPK 1024
 #define SAQGPU_DEVICE_PTRS 1
+/* saqgpu_search_batched only, with SAQGPU_DEVICE_PTRS and a non-null stream: the call's device work
+ * (coarse quantiser, rotation, grouping, pair prep, tile kernel, merge) is captured into a HIP graph,
+ * which later calls with the same shape, parameters and buffers replay as one graph launch (the
+ * buffers' contents may change between calls).  The first call of a new shape runs eagerly and
+ * captures; the index keeps one graph. */
+#define SAQGPU_USE_GRAPH 2
 
 typedef struct saqgpu_index saqgpu_index;
 
@@ -212,7 +218,9 @@ typedef struct {
     int32_t nprobe;                   /* lists per query (the maximum when nprobe_per_query is set) */
     const int32_t *nprobe_per_query;  /* NULL, or nq entries in 1..nprobe, in the queries' memory kind */
     int32_t precision;                /* SAQGPU_PREC_INT8 or SAQGPU_PREC_INT8X2 */
-    int32_t query_tile;               /* queries per work tile: 16, 32 or 64; 0 = the autotuned choice */
+    int32_t query_tile;               /* queries per work tile: 16, 32, 64 or 128 (a workgroup of 1, 2, 4
+                                         or 4 waves reading each code chunk once for them); 0 = the
+                                         autotuned choice */
     int32_t exact_probe;              /* with probes == NULL: 1 = the reference's coarse quantiser order
                                          (as saqgpu_probe), 0 = an fp32 GEMM's |c|^2 - 2 q.c (probe lists
                                          may differ where two centroid distances tie within fp32 rounding) */
@@ -229,7 +237,7 @@ int saqgpu_search_batched(saqgpu_index *idx, const float *queries, int64_t nq, c
                           const saqgpu_batched_params *params, uint32_t *out_ids, float *out_dists,
                           int32_t flags, void *stream);
 
-/* The project's autotuner for the batched scan: times every query tile (16, 32, 64) on this
+/* The project's autotuner for the batched scan: times every query tile (16, 32, 64, 128) on this
  * batch and keeps the fastest as the index's choice for query_tile = 0; returns it. */
 int saqgpu_autotune_batched(saqgpu_index *idx, const float *queries, int64_t nq, const uint32_t *probes,
                             const saqgpu_batched_params *params, int32_t flags, void *stream,

@@ -16,6 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libsaqgpu.so")
 MAX_SEG = 64
 DEVICE_PTRS = 1
+USE_GRAPH = 2  # include/saqgpu.h SAQGPU_USE_GRAPH
 
 
 class SaqGpuError(RuntimeError):
@@ -266,8 +267,11 @@ class Index:
         return p, npq
 
     def search_batched(self, queries, topk, nprobe, probes=None, nprobe_per_query=None, precision=PREC_INT8,
-                       query_tile=0, out_ids=None, out_dists=None, stream=None, want_dists=True, exact_probe=False):
-        """Batched int8-MFMA search (saqgpu_search_batched).  Returns (ids, dists)."""
+                       query_tile=0, out_ids=None, out_dists=None, stream=None, want_dists=True, exact_probe=False,
+                       graph=False):
+        """Batched int8-MFMA search (saqgpu_search_batched).  Returns (ids, dists).  graph: replay a
+        captured HIP graph while the shape and device buffers stay the same (device tensors and a
+        stream only)."""
         dev = not isinstance(queries, np.ndarray)
         nq = int(queries.shape[0])
         p, keep = self._bparams(topk, nprobe, nprobe_per_query, precision, query_tile, dev, exact_probe)
@@ -279,7 +283,8 @@ class Index:
             if want_dists and out_dists is None:
                 out_dists = np.zeros((nq, topk), np.float32)
         _check(lib().saqgpu_search_batched(self.h, _ptr(queries), nq, _ptr(probes), C.byref(p), _ptr(out_ids),
-                                           _ptr(out_dists), DEVICE_PTRS if dev else 0, stream))
+                                           _ptr(out_dists), (DEVICE_PTRS if dev else 0) | (USE_GRAPH if graph else 0),
+                                           stream))
         del keep
         return out_ids, out_dists
 

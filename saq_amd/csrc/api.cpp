@@ -134,6 +134,14 @@ float *scratch_of(int device) {
 }
 }  // namespace
 
+// What a captured batched step depends on: its shape, parameters and every buffer it touches.
+struct BatchedGraphKey {
+    int64_t nq;
+    int nprobe, topk, precision, query_tile, exact_probe;
+    const void *queries, *probes, *npq, *out_ids, *out_dists, *stream, *ws, *qq, *cand, *cdist, *wprobes;
+    bool operator==(const BatchedGraphKey &o) const { return std::memcmp(this, &o, sizeof(*this)) == 0; }
+};
+
 struct saqgpu_index {
     int device = 0;
     int64_t num_data = 0;
@@ -187,13 +195,16 @@ struct saqgpu_index {
     DevBuf<int64_t> d_clu_mb;
     DevBuf<int32_t> d_bq_order;  // lists by decreasing size
     DevBuf<BSeg> d_bsegs;
-    DevBuf<int32_t> bq_ws, bq_npq, bq_pcorr;
+    DevBuf<int32_t> bq_ws, bq_npq;
     DevBuf<int8_t> bq_qq;
     DevBuf<float> bq_pc;
     DevBuf<float> cnorm;  // |c|^2 per centroid (GEMM-based coarse quantisers), computed once
     bool cnorm_ready = false;
     DevBuf<float> bq_cand_d;
     DevBuf<uint32_t> bq_cand_id;
+    // the captured batched step (SAQGPU_USE_GRAPH) and what it was captured with
+    hipGraphExec_t bq_graph = nullptr;
+    BatchedGraphKey bq_graph_key{};
     // measurement
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
@@ -506,7 +517,6 @@ void saqgpu_index_destroy(saqgpu_index *ix) {
     ix->d_bq_order.release();
     ix->d_bsegs.release();
     ix->bq_ws.release();
-    ix->bq_pcorr.release();
     ix->bq_qq.release();
     ix->bq_pc.release();
     ix->cnorm.release();
@@ -529,6 +539,7 @@ void saqgpu_index_destroy(saqgpu_index *ix) {
     }
     if (ix->sl_fork) (void)hipEventDestroy(ix->sl_fork);
     if (ix->blas) (void)rocblas_destroy_handle(ix->blas);
+    if (ix->bq_graph) (void)hipGraphExecDestroy(ix->bq_graph);
     if (ix->ws_free) (void)hipEventDestroy(ix->ws_free);
     delete ix;
 }
@@ -1336,8 +1347,9 @@ int check_batched_params(const saqgpu_index *ix, const saqgpu_batched_params *p,
     if (p->nprobe < 1 || p->nprobe > ix->num_cen) return fail(SAQGPU_EINVAL, "nprobe must be in 1..K");
     if (p->precision != SAQGPU_PREC_INT8 && p->precision != SAQGPU_PREC_INT8X2)
         return fail(SAQGPU_EINVAL, "precision must be SAQGPU_PREC_INT8 or SAQGPU_PREC_INT8X2");
-    if (p->query_tile != 0 && p->query_tile != 16 && p->query_tile != 32 && p->query_tile != 64)
-        return fail(SAQGPU_EINVAL, "query_tile must be 0, 16, 32 or 64");
+    if (p->query_tile != 0 && p->query_tile != 16 && p->query_tile != 32 && p->query_tile != 64 &&
+        p->query_tile != 128)
+        return fail(SAQGPU_EINVAL, "query_tile must be 0, 16, 32, 64 or 128");
     if (p->nprobe_per_query && !(flags & SAQGPU_DEVICE_PTRS))
         for (int64_t i = 0; i < nq; ++i)
             if (p->nprobe_per_query[i] < 1 || p->nprobe_per_query[i] > p->nprobe)
@@ -1348,7 +1360,7 @@ int check_batched_params(const saqgpu_index *ix, const saqgpu_batched_params *p,
 // Queries per work tile when the autotuner has not run: the expected queries per probed list.
 int default_query_tile(const saqgpu_index *ix, int64_t nq, int nprobe) {
     const double per_list = (double)nq * nprobe / std::max(1, ix->num_cen);
-    return per_list >= 48 ? 64 : (per_list >= 24 ? 32 : 16);
+    return per_list >= 96 ? 128 : (per_list >= 48 ? 64 : (per_list >= 24 ? 32 : 16));
 }
 
 // The batched scan's coarse quantiser: q.c for every (query, centroid) from rocBLAS's fp32 GEMM
@@ -1368,25 +1380,27 @@ int search_batched_locked(saqgpu_index *ix, const float *queries, int64_t nq, co
     int rc = prepare_batched_locked(ix);
     if (rc) return rc;
     const bool dev = flags & SAQGPU_DEVICE_PTRS;
+    const bool graph = (flags & SAQGPU_USE_GRAPH) != 0;
+    if (graph && !dev) return fail(SAQGPU_EINVAL, "SAQGPU_USE_GRAPH needs SAQGPU_DEVICE_PTRS");
+    if (graph && !st) return fail(SAQGPU_EINVAL, "SAQGPU_USE_GRAPH needs a stream other than the null stream");
     const int NP = p->nprobe, K = ix->num_cen, topk = p->topk;
-    const int qtm = query_tile / 16;
     const int qs = p->precision == SAQGPU_PREC_INT8X2 ? 2 : 1;
     const BatchedView bv = batched_view(ix);
     int max_smem = 0;
     HIPCHK(hipDeviceGetAttribute(&max_smem, hipDeviceAttributeMaxSharedMemoryPerBlock, ix->device));
-    if ((int64_t)bscan_smem_bytes(bv, topk, qtm, qs) > max_smem)
+    if ((int64_t)bscan_smem_bytes(bv, topk, query_tile, qs) > max_smem)
         return fail(SAQGPU_EUNSUPPORTED, "query tile needs more LDS than a workgroup has; use a smaller query_tile");
     const int64_t npairs = nq * NP;
     const int64_t max_tiles = bq_max_tiles(npairs, K), max_cands = bq_max_cands(npairs, query_tile);
     if (max_tiles >= INT32_MAX || max_cands * topk >= ((int64_t)1 << 40))
         return fail(SAQGPU_EINVAL, "batch too large: split the queries");
+    // every allocation first: none may happen while a graph is being captured
     rc = 0;
     rc |= ix->ws_qrot.ensure(nq * ix->D_pad);
     rc |= ix->ws_bound.ensure(nq * ix->nseg);
-    rc |= ix->bq_ws.ensure(4 * (size_t)K + 2 * npairs + 5 * max_tiles + 4);
+    rc |= ix->bq_ws.ensure(4 * (size_t)K + 2 * npairs + 5 * max_tiles + 4 + nq);
     rc |= ix->bq_qq.ensure((size_t)qs * npairs * ix->bq_kq);
     rc |= ix->bq_pc.ensure((size_t)npairs * (3 * ix->bq_nqs + 1));
-    rc |= ix->bq_pcorr.ensure((size_t)npairs * 2 * std::max(1, ix->bq_nqs));
     rc |= ix->bq_cand_d.ensure(max_cands * topk);
     rc |= ix->bq_cand_id.ensure(max_cands * topk);
     if (!dev) {
@@ -1396,7 +1410,20 @@ int search_batched_locked(saqgpu_index *ix, const float *queries, int64_t nq, co
         if (p->nprobe_per_query) rc |= ix->bq_npq.ensure(nq);
     }
     if (!probes || !dev) rc |= ix->ws_probes.ensure(npairs);
+    if (!probes) {
+        rc |= ix->ws_cdist.ensure((size_t)nq * K);
+        if (p->exact_probe) rc |= ix->ws_ties.ensure(std::max<int64_t>(1, nq));
+    }
     if (rc) return fail(SAQGPU_ENOMEM, "workspace allocation failed");
+    // the replay of a captured graph: the same shape, parameters and buffers as the capture
+    BatchedGraphKey key{nq, NP, topk, p->precision, query_tile, p->exact_probe, queries, probes,
+                        p->nprobe_per_query, out_ids, out_dists, st, ix->bq_ws.p, ix->bq_qq.p, ix->bq_cand_d.p,
+                        ix->ws_cdist.p, ix->ws_probes.p};
+    if (graph && ix->bq_graph && key == ix->bq_graph_key) {
+        HIPCHK(hipGraphLaunch(ix->bq_graph, st));
+        return SAQGPU_OK;
+    }
+    if (!probes && (rc = ensure_cnorm(ix, st))) return rc;
     const float *dq = queries;
     const uint32_t *dp = probes;
     const int32_t *dnpq = p->nprobe_per_query;
@@ -1416,19 +1443,6 @@ int search_batched_locked(saqgpu_index *ix, const float *queries, int64_t nq, co
         dids = ix->ws_ids.p;
         dd = out_dists ? ix->ws_dists.p : nullptr;
     }
-    if (!probes) {
-        if (ix->ws_cdist.ensure((size_t)nq * K))
-            return fail(SAQGPU_ENOMEM, "workspace allocation failed");
-        if (p->exact_probe)
-        {
-            if ((rc = exact_probe(ix, dq, nq, NP, 0, ix->ws_probes.p, st))) return rc;
-        }
-        else if ((rc = probe_gemm(ix, dq, nq, NP, 0, ix->ws_probes.p, st)))
-            return rc;
-        dp = ix->ws_probes.p;
-    }
-    HIPCHK(launch_query_prep(ix->view(), dq, nq, ix->variance.p, ix->has_rot ? ix->rot.p : nullptr, ix->d_rot_off.p,
-                             4.0f, ix->ws_qrot.p, ix->ws_bound.p, ix->max_dim, st));
     int32_t *wsp = ix->bq_ws.p;
     BqWork w;
     w.counts = wsp;
@@ -1446,27 +1460,68 @@ int search_batched_locked(saqgpu_index *ix, const float *queries, int64_t nq, co
     w.npairs = w.ntiles + 1;
     w.work = w.npairs + 1;
     w.sv = w.work + 1;
+    w.qbound = reinterpret_cast<uint32_t *>(w.sv + 1);
     BqPairBufs pb;
     pb.qq1 = ix->bq_qq.p;
     pb.qq2 = qs == 2 ? ix->bq_qq.p + npairs * ix->bq_kq : nullptr;
     pb.pc = ix->bq_pc.p;
     pb.pq2 = ix->bq_pc.p + npairs * 3 * ix->bq_nqs;
-    pb.pcorr = ix->bq_pcorr.p;
-    HIPCHK(launch_bq_group(dp, dnpq, nq, NP, K, query_tile, ix->d_bq_order.p, ix->d_clu_n.p, w, st));
-    HIPCHK(launch_bq_pairprep(bv, ix->ws_qrot.p, dp, w, pb, NP, npairs, qs, st));
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (time_it) {
-        HIPCHK(hipEventCreate(&e0));
-        HIPCHK(hipEventCreate(&e1));
-        HIPCHK(hipEventRecord(e0, st));
-    }
     const int grid = (int)std::min<int64_t>(max_tiles, 1 << 20);
-    HIPCHK(launch_bscan_tiles(bv, w, pb, topk, ix->bq_cand_d.p, ix->bq_cand_id.p, qtm, qs, grid, st));
-    if (time_it) {
-        HIPCHK(hipEventRecord(e1, st));
-        ix->events.emplace_back(e0, e1);
+    // the step's device work (also what a graph captures): coarse quantiser, query rotation,
+    // grouping, pair prep, tile kernel, merge
+    auto enqueue = [&](bool timed) -> int {
+        int rc2;
+        if (!probes) {
+            if (p->exact_probe) {
+                if ((rc2 = exact_probe(ix, dq, nq, NP, 0, ix->ws_probes.p, st))) return rc2;
+            } else if ((rc2 = probe_gemm(ix, dq, nq, NP, 0, ix->ws_probes.p, st))) {
+                return rc2;
+            }
+            dp = ix->ws_probes.p;
+        }
+        HIPCHK(launch_query_prep(ix->view(), dq, nq, ix->variance.p, ix->has_rot ? ix->rot.p : nullptr,
+                                 ix->d_rot_off.p, 4.0f, ix->ws_qrot.p, ix->ws_bound.p, ix->max_dim, st));
+        HIPCHK(launch_bq_group(dp, dnpq, nq, NP, K, query_tile, ix->d_bq_order.p, ix->d_clu_n.p, w, st));
+        HIPCHK(launch_bq_pairprep(bv, ix->ws_qrot.p, dp, w, pb, NP, npairs, qs, st));
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (timed) {
+            HIPCHK(hipEventCreate(&e0));
+            HIPCHK(hipEventCreate(&e1));
+            HIPCHK(hipEventRecord(e0, st));
+        }
+        HIPCHK(launch_bscan_tiles(bv, w, pb, NP, topk, ix->bq_cand_d.p, ix->bq_cand_id.p, query_tile, qs, grid,
+                                  st));
+        if (timed) {
+            HIPCHK(hipEventRecord(e1, st));
+            ix->events.emplace_back(e0, e1);
+        }
+        HIPCHK(launch_bq_merge(ix->bq_cand_d.p, ix->bq_cand_id.p, dp, w, dnpq, nq, NP, topk, dids, dd, st));
+        return SAQGPU_OK;
+    };
+    if ((rc = enqueue(time_it && !graph))) return rc;
+    if (graph) {
+        // this call ran eagerly (which also warms the libraries up); capture the same work for the
+        // calls that follow with the same key
+        if (ix->bq_graph) {
+            (void)hipGraphExecDestroy(ix->bq_graph);
+            ix->bq_graph = nullptr;
+        }
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        const int crc = enqueue(false);
+        const hipError_t ce = hipStreamEndCapture(st, &g);
+        if (crc) {
+            if (g) (void)hipGraphDestroy(g);
+            return crc;
+        }
+        if (ce != hipSuccess) return fail(SAQGPU_EDEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+        hipGraphExec_t ex = nullptr;
+        const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ie != hipSuccess) return fail(SAQGPU_EDEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+        ix->bq_graph = ex;
+        ix->bq_graph_key = key;
     }
-    HIPCHK(launch_bq_merge(ix->bq_cand_d.p, ix->bq_cand_id.p, dp, w, dnpq, nq, NP, topk, dids, dd, st));
     if (!dev) {
         HIPCHK(hipMemcpyAsync(out_ids, dids, nq * topk * 4, hipMemcpyDeviceToHost, st));
         if (out_dists) HIPCHK(hipMemcpyAsync(out_dists, dd, nq * topk * 4, hipMemcpyDeviceToHost, st));
@@ -1543,8 +1598,8 @@ int saqgpu_autotune_batched(saqgpu_index *ix, const float *queries, int64_t nq, 
     const int qs = p->precision == SAQGPU_PREC_INT8X2 ? 2 : 1;
     float best = 1e30f;
     int best_qt = 16;
-    for (int qt : {16, 32, 64}) {
-        if ((int64_t)bscan_smem_bytes(bv, p->topk, qt / 16, qs) > max_smem) continue;
+    for (int qt : {16, 32, 64, 128}) {
+        if ((int64_t)bscan_smem_bytes(bv, p->topk, qt, qs) > max_smem) continue;
         hipEvent_t a, b;
         HIPCHK(hipEventCreate(&a));
         HIPCHK(hipEventCreate(&b));

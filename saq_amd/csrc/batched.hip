@@ -128,7 +128,7 @@ __global__ __launch_bounds__(64) void bq_build_codes_kernel(IndexView ix, Batche
     }
 }
 
-// grid (total mblocks), 16 threads: |o|^2, the rescale of every quantised segment and the id.
+// grid (total mblocks), 16 threads: |o|^2, -2 x the rescale of every quantised segment and the id.
 __global__ __launch_bounds__(64) void bq_build_factors_kernel(IndexView ix, BatchedView bv, const int32_t *mb_clu,
                                                               uint8_t *out) {
     const int64_t mb = blockIdx.x;
@@ -153,7 +153,7 @@ __global__ __launch_bounds__(64) void bq_build_factors_kernel(IndexView ix, Batc
         const float x = norms[s * kBlock + vi % kBlock];
         o2 = o2 + x * x;
         const int q = bv.segs[s].qs;
-        if (q >= 0) f[kMB * (1 + q) + l] = ix.exfac[(gv * bv.nseg + s) * 2];
+        if (q >= 0) f[kMB * (1 + q) + l] = -2.0f * ix.exfac[(gv * bv.nseg + s) * 2];  // dist -= 2 rescale ip
     }
     f[l] = o2;
     idp[l] = ix.ids[gv];
@@ -161,8 +161,11 @@ __global__ __launch_bounds__(64) void bq_build_factors_kernel(IndexView ix, Batc
 
 // ------------------------------------------------------------------ grouping
 // Zeroes the per-list pair counts and the tile counter of a launch.
-__global__ void bq_zero_kernel(int32_t *counts, int K, int32_t *work) {
+// Also sets every query's distance bound to +inf (a key above every finite distance's).
+__global__ void bq_zero_kernel(int32_t *counts, int K, int32_t *work, uint32_t *qbound, int64_t nq) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < K; i += gridDim.x * blockDim.x) counts[i] = 0;
+    for (int64_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += gridDim.x * blockDim.x)
+        qbound[i] = 0xFFFFFFFFu;
     if (blockIdx.x == 0 && threadIdx.x == 0) *work = 0;
 }
 
@@ -183,15 +186,16 @@ __global__ void bq_count_kernel(const uint32_t *probes, const int32_t *npq, int6
 
 // One workgroup.  The split size first: W = sum over lists of (query tiles x list vectors) is
 // the launch's work in vector-tiles, and a list is cut into ranges of sv = max(kMinSplit,
-// W / kSplitWaves) vectors (a multiple of 64), so no work tile holds more than about one resident
-// wavefront's share and one long list cannot set the kernel's tail.  Then exclusive
+// W / workers) vectors (a multiple of 64; workers = the tile kernel's resident workgroups, at most
+// kSplitWaves), so no work tile holds more than about one resident workgroup's share and one long
+// list cannot set the kernel's tail.  Then exclusive
 // prefix sums, over the lists in the given order (largest first, so the longest tiles start
 // first), of the pair counts, of the work tiles (query tiles x ranges) and of the candidate
 // lists (pairs x ranges), and every list's work tiles.  The bounds sum_c tiles_c <= npairs / QT
 // + K + kSplitWaves and sum_c cands_c <= npairs + kSplitWaves * QT follow from
 // sv >= W / kSplitWaves.
 __global__ __launch_bounds__(1024) void bq_scan_kernel(const int32_t *counts, const int32_t *order,
-                                                       const int32_t *clu_n, int K, int QT, int32_t *cstart,
+                                                       const int32_t *clu_n, int K, int QT, int workers, int32_t *cstart,
                                                        int32_t *candstart, int32_t *nsplit, int32_t *ntiles,
                                                        int32_t *npairs, int32_t *sv_out, int32_t *tile_clu,
                                                        int32_t *tile_first, int32_t *tile_cnt, int32_t *tile_v0,
@@ -222,7 +226,7 @@ __global__ __launch_bounds__(1024) void bq_scan_kernel(const int32_t *counts, co
     __syncthreads();
     long long W = 0;
     for (int i = 0; i < 16; ++i) W += wk_s[i];
-    const long long want = (W + kSplitWaves - 1) / kSplitWaves;
+    const long long want = (W + workers - 1) / workers;
     const int sv = (int)max((long long)kMinSplit, (want + 63) / 64 * 64);
     int ns[kScanRounds];
     int sc = 0, st = 0, sk = 0;
@@ -473,10 +477,29 @@ struct PairPrepArgs {
     const int32_t *npairs;  // active pairs (device scalar)
     int NP, qs;
     int8_t *qq1, *qq2;  // [pairs][kq]
-    float *pc;          // [pairs][nqs][3]: sqd * s1, sqd * s2, (-1 + sqd / 2) * sum r
-    int32_t *pcorr;     // [pairs][nqs][2]: shift corrections of the two terms
+    float *pc;          // [pairs][nqs][3]: a1 = sqd * s1, a2 = sqd * s2, and the constant term
+                        // (-1 + sqd / 2) * sum r + a1 * corr1 + a2 * corr2 (corr: the code shift's
+                        // correction 128 * sum q of each term, b >= 8)
     float *pq2;         // [pairs]: |r|^2 over all segments
 };
+
+// The pair's constants of one quantised segment: the term scales a1 = sqd * s1, a2 = sqd * s2 and
+// the constant term of its inner product, (-1 + sqd / 2) * sum r plus the code shift's correction
+// a_t * 128 * sum q_t of each term (codes of 8 and more bits are stored shifted by -128), so that
+// the tile kernel forms ip_s = a1 * acc1 + a2 * acc2 + c with fused multiply-adds.
+__device__ __forceinline__ void write_pair_consts(const PairPrepArgs &a, const BSeg &bs, int64_t g, float s1, float s2,
+                                                  float sr, int sum1, int sum2) {
+    const float a1 = bs.sqd * s1, a2 = bs.sqd * s2;
+    float c = (-1.0f + bs.sqd / 2) * sr;
+    if (bs.bits >= 8) {
+        c = fmaf(a1, (float)(128 * sum1), c);
+        if (a.qs == 2) c = fmaf(a2, (float)(128 * sum2), c);
+    }
+    float *pc = a.pc + (g * a.bv.nqs + bs.qs) * 3;
+    pc[0] = a1;
+    pc[1] = a2;
+    pc[2] = c;
+}
 
 // One butterfly step (partner lane ^ J) of wave reductions, through xor_lane's DPP / swizzle.
 template <int J>
@@ -554,15 +577,7 @@ __device__ __forceinline__ void pairprep_segment(const PairPrepArgs &a, const BS
     sum1 += xor_lane<4>(sum1), sum2 += xor_lane<4>(sum2);
     sum1 += xor_lane<2>(sum1), sum2 += xor_lane<2>(sum2);
     sum1 += xor_lane<1>(sum1), sum2 += xor_lane<1>(sum2);
-    if (lane == 0) {
-        float *pc = a.pc + (g * bv.nqs + bs.qs) * 3;
-        pc[0] = bs.sqd * s1;
-        pc[1] = bs.sqd * s2;
-        pc[2] = (-1.0f + bs.sqd / 2) * sr;
-        int32_t *cc = a.pcorr + (g * bv.nqs + bs.qs) * 2;
-        cc[0] = bs.bits >= 8 ? 128 * sum1 : 0;
-        cc[1] = bs.bits >= 8 ? 128 * sum2 : 0;
-    }
+    if (lane == 0) write_pair_consts(a, bs, g, s1, s2, sr, sum1, sum2);
 }
 
 // The pair's whole residual in NKT float4 registers per lane (element 4 lane + 256 k of the
@@ -639,15 +654,7 @@ __device__ __forceinline__ void pairprep_whole(const PairPrepArgs &a, int64_t g,
         sum1 += xor_lane<4>(sum1), sum2 += xor_lane<4>(sum2);
         sum1 += xor_lane<2>(sum1), sum2 += xor_lane<2>(sum2);
         sum1 += xor_lane<1>(sum1), sum2 += xor_lane<1>(sum2);
-        if (lane == 0) {
-            float *pc = a.pc + (g * bv.nqs + bs.qs) * 3;
-            pc[0] = bs.sqd * s1;
-            pc[1] = bs.sqd * s2;
-            pc[2] = (-1.0f + bs.sqd / 2) * sr;
-            int32_t *cc = a.pcorr + (g * bv.nqs + bs.qs) * 2;
-            cc[0] = bs.bits >= 8 ? 128 * sum1 : 0;
-            cc[1] = bs.bits >= 8 ? 128 * sum2 : 0;
-        }
+        if (lane == 0) write_pair_consts(a, bs, g, s1, s2, sr, sum1, sum2);
     }
     if (lane == 0) a.pq2[g] = q2;
 }
@@ -686,33 +693,50 @@ __global__ __launch_bounds__(256) void bq_pairprep_kernel(PairPrepArgs a) {
 }
 
 // ------------------------------------------------------------------ tile kernel
-// Persistent; every wave takes its own work tiles (one list and up to 16*QTM of its queries)
-// from a shared counter.  Per chunk of 16*NVW vectors and per quantised segment, the B
-// operands of two 64-dim chunks for all NVW mblocks are loaded together, then the MFMAs run;
-// after the segments the chunk's distances go through the wave's LDS to the per-query top-k,
-// which a query only touches when one of the chunk's candidates beats its k-th pair.
+// Persistent.  A workgroup of W waves takes one work tile at a time (one list, one vector range of
+// it and up to W * 16 * QTM of its queries) from a shared counter; wave w owns the tile's query
+// rows 16 QTM w .. 16 QTM (w + 1) - 1 and walks the same vector range as its sibling waves, so
+// the W waves read every code chunk of the range at about the same time from the same CU: one
+// HBM read serves the workgroup's W * 16 * QTM queries, the other waves' reads hitting the caches.
+// Per chunk of 16*NVW vectors and per quantised segment, the B operands of 2-4 64-dim chunks for
+// all NVW mblocks are loaded together, then the MFMAs run; each segment's int32 sums fold into the
+// fp32 distances with three fused multiply-adds per (query, vector); after the segments the
+// chunk's distances go through the wave's LDS to the per-query top-k, which a query only touches
+// when one of the chunk's candidates beats its k-th pair.
 struct BScanArgs {
     BatchedView bv;
     const int8_t *qq1, *qq2;
     const float *pc;
-    const int32_t *pcorr;
     const float *pq2;
     const int32_t *tile_clu, *tile_first, *tile_cnt, *tile_v0, *tile_cand;
     const int32_t *ntiles, *sv;
     int32_t *work;  // tile counter, zero at launch
+    const int32_t *pairs;  // grouped pair -> q * NP + p
+    int NP;
+    uint32_t *qbound;      // per query: an upper bound of its k-th distance (float_key), shared by its tiles
     int topk;
     float *cand_d;  // (nq * NP) x topk
     uint32_t *cand_id;
 };
 
+// A query's distance bound: the k-th distance of any one of its lists' full top-k lists is at
+// least its final k-th distance, so a candidate above it can be dropped by every other tile of the
+// query.  Kept as float_key (order-preserving) for atomicMin.
+__device__ __forceinline__ float key_float(uint32_t k) {
+    if (k >= 0xFF800000u) return INFINITY;
+    return __uint_as_float((k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k);
+}
+
 struct WSmem {
     float *dt;       // [QT][CVW]
     uint32_t *dids;  // [CVW]
-    float *qc;       // [QT][nqs][3]
-    int32_t *qcorr;  // [QT][nqs][2]
+    float *ka;       // [nqs][QT] a1 of each row
+    float *kb;       // [nqs][QT] a2
+    float *kc;       // [nqs][QT] constant term
     float *q2;       // [QT]
-    float *kd;       // [QT] k-th distance
+    float *kd;       // [QT] k-th distance (the query's bound until the row's list is full)
     uint32_t *ki;    // [QT] k-th id
+    float *rb;       // [QT] the query's bound at the tile's start
     int32_t *tkn;    // [QT]
     float *tkd;      // [QT][topk]
     uint32_t *tki;   // [QT][topk]
@@ -722,13 +746,15 @@ struct WSmem {
     int32_t *cbn;    // [QT]
 };
 
-// mblocks per wave chunk: 64 vectors' accumulators per query tile for int8, half that for
-// int8x2 (whose second term doubles the accumulators)
-__host__ __device__ constexpr int nvw_of(int qtm, int qs) { return qs == 2 ? (qtm >= 2 ? 1 : 2) : 4 / qtm; }
+// mblocks per wave chunk: 32 vectors per chunk for int8 (fewer registers: 3 waves per SIMD, 20%
+// faster on the 10M x 1024 workload than 64 at 2 waves), half a chunk's accumulators for int8x2
+// (whose second term doubles them)
+__host__ __device__ constexpr int nvw_of(int qtm, int qs) { return qs == 2 ? (qtm >= 2 ? 1 : 2) : (qtm <= 2 ? 2 : 1); }
 
 // candidate buffer places per row: 64 for 16-query tiles, fewer for larger tiles (LDS)
 __host__ __device__ constexpr int cand_cap(int qtm) { return 64 / qtm; }
 
+// LDS of one wave (QTM query tiles of 16 rows).
 template <int QTM, int QS>
 __host__ __device__ inline size_t wave_smem(int nqs, int topk, WSmem *m, uint8_t *base) {
     constexpr int QT = 16 * QTM, CVW = 16 * nvw_of(QTM, QS);
@@ -741,11 +767,13 @@ __host__ __device__ inline size_t wave_smem(int nqs, int topk, WSmem *m, uint8_t
     WSmem s;
     s.dt = (float *)take((size_t)QT * CVW * 4);
     s.dids = (uint32_t *)take(CVW * 4);
-    s.qc = (float *)take((size_t)QT * nqs * 3 * 4);
-    s.qcorr = (int32_t *)take((size_t)QT * nqs * 2 * 4);
+    s.ka = (float *)take((size_t)QT * nqs * 4);
+    s.kb = (float *)take((size_t)QT * nqs * 4);
+    s.kc = (float *)take((size_t)QT * nqs * 4);
     s.q2 = (float *)take(QT * 4);
     s.kd = (float *)take(QT * 4);
     s.ki = (uint32_t *)take(QT * 4);
+    s.rb = (float *)take(QT * 4);
     s.tkn = (int32_t *)take(QT * 4);
     s.tkd = (float *)take((size_t)QT * topk * 4);
     s.tki = (uint32_t *)take((size_t)QT * topk * 4);
@@ -758,6 +786,12 @@ __host__ __device__ inline size_t wave_smem(int nqs, int topk, WSmem *m, uint8_t
     return off;
 }
 
+// LDS of a workgroup of W waves: W waves' areas and the tile index it broadcasts.
+template <int W, int QTM, int QS>
+__host__ __device__ inline size_t wg_smem(int nqs, int topk) {
+    return W * wave_smem<QTM, QS>(nqs, topk, nullptr, nullptr) + 16;
+}
+
 // LDS traffic between the lanes of one wave: keep the compiler from moving accesses across
 // this point (a wave's LDS operations complete in order).
 __device__ __forceinline__ void wave_sync() {
@@ -765,15 +799,20 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_wave_barrier();
 }
 
-// The row's sorted list (k <= 64 places, padded) back to LDS with its k-th pair.
+// The row's sorted list (k <= 64 places, padded) back to LDS with its k-th pair; while the list
+// is not full (its k-th place still +inf) the threshold stays the query's bound.
 __device__ __forceinline__ void store_row(const WSmem m, int i, int topk, float d, uint32_t id) {
     const int lane = threadIdx.x & 63;
     if (lane < topk) {
         m.tkd[i * topk + lane] = d;
         m.tki[i * topk + lane] = id;
     }
-    const float kd = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(d), topk - 1));
-    const uint32_t ki = (uint32_t)__builtin_amdgcn_readlane((int)id, topk - 1);
+    float kd = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(d), topk - 1));
+    uint32_t ki = (uint32_t)__builtin_amdgcn_readlane((int)id, topk - 1);
+    if (kd == INFINITY) {
+        kd = m.rb[i];
+        ki = 0xFFFFFFFFu;
+    }
     if (lane == 0) {
         m.kd[i] = kd;
         m.ki[i] = ki;
@@ -895,9 +934,9 @@ __device__ __forceinline__ void seg_mma(const BSeg &bs, const uint8_t *const (&r
     }
 }
 
-template <int QTM, int QS, int KR>
-__global__ __launch_bounds__(kBThreads, (QTM == 4 && QS == 2) ? 1 : 2) void bscan_kernel(BScanArgs a) {
-    constexpr int QT = 16 * QTM;
+template <int W, int QTM, int QS, int KR>
+__global__ __launch_bounds__(64 * W, (W * QTM >= 8 || (QTM == 4 && QS == 2)) ? 1 : 2) void bscan_kernel(BScanArgs a) {
+    constexpr int QT = 16 * QTM;          // query rows of a wave
     constexpr int NVW = nvw_of(QTM, QS);  // mblocks per chunk
     constexpr int CVW = 16 * NVW;         // vectors per chunk
     extern __shared__ __align__(16) uint8_t smem_raw[];
@@ -906,31 +945,42 @@ __global__ __launch_bounds__(kBThreads, (QTM == 4 && QS == 2) ? 1 : 2) void bsca
     const size_t per_wave = wave_smem<QTM, QS>(bv.nqs, a.topk, nullptr, nullptr);
     WSmem m;
     wave_smem<QTM, QS>(bv.nqs, a.topk, &m, smem_raw + w * per_wave);
+    int32_t *tile_s = reinterpret_cast<int32_t *>(smem_raw + W * per_wave);
     const int ntiles = *a.ntiles, sv = *a.sv;
-    // the first tile of every wavefront is its own index; later ones come from the counter
-    const int nwaves = gridDim.x * kBWaves;
     for (int round = 0;; ++round) {
-        int tile = blockIdx.x * kBWaves + w;
-        if (round > 0) {
-            if (lane == 0) tile = nwaves + atomicAdd(a.work, 1);
-            tile = __shfl(tile, 0);
+        // the first tile of every workgroup is its own index; later ones come from the counter
+        int tile;
+        if (W == 1) {
+            tile = blockIdx.x;
+            if (round > 0) {
+                if (lane == 0) tile = gridDim.x + atomicAdd(a.work, 1);
+                tile = __shfl(tile, 0);
+            }
+        } else {
+            if (threadIdx.x == 0) *tile_s = round == 0 ? (int)blockIdx.x : (int)gridDim.x + atomicAdd(a.work, 1);
+            __syncthreads();
+            tile = *tile_s;
+            __syncthreads();  // every wave has read it before the next round writes it
         }
         if (tile >= ntiles) break;
-        const int c = a.tile_clu[tile], first = a.tile_first[tile], cnt = a.tile_cnt[tile];
-        // ---- the tile's pair constants to LDS; rows past cnt repeat the last pair
+        const int c = a.tile_clu[tile];
+        const int first = a.tile_first[tile] + w * QT, cnt = min(QT, a.tile_cnt[tile] - w * QT);
+        if (cnt <= 0) continue;  // the tile's queries end before this wave's rows
+        // ---- the wave's pair constants to LDS; rows past cnt repeat the last pair
         for (int e = lane; e < QT * bv.nqs; e += 64) {
-            const int row = e / bv.nqs, s = e % bv.nqs;
-            const int64_t g = first + min(row, cnt - 1);
-            const float *pc = a.pc + (g * bv.nqs + s) * 3;
-            m.qc[e * 3 + 0] = pc[0];
-            m.qc[e * 3 + 1] = pc[1];
-            m.qc[e * 3 + 2] = pc[2];
-            m.qcorr[e * 2 + 0] = a.pcorr[(g * bv.nqs + s) * 2];
-            m.qcorr[e * 2 + 1] = a.pcorr[(g * bv.nqs + s) * 2 + 1];
+            const int row = e % QT, s = e / QT;
+            const float *pc = a.pc + ((int64_t)(first + min(row, cnt - 1)) * bv.nqs + s) * 3;
+            m.ka[e] = pc[0];
+            m.kb[e] = pc[1];
+            m.kc[e] = pc[2];
         }
         for (int row = lane; row < QT; row += 64) {
-            m.q2[row] = a.pq2[first + min(row, cnt - 1)];
-            m.kd[row] = INFINITY;
+            const int g = first + min(row, cnt - 1);
+            m.q2[row] = a.pq2[g];
+            // top-k lists (k <= 64) start from the query's bound; longer ones from +inf
+            const float b = KR == 1 ? key_float(a.qbound[a.pairs[g] / a.NP]) : INFINITY;
+            m.kd[row] = b;
+            m.rb[row] = b;
             m.ki[row] = 0xFFFFFFFFu;
             m.tkn[row] = 0;
             m.cbn[row] = 0;
@@ -941,7 +991,7 @@ __global__ __launch_bounds__(kBThreads, (QTM == 4 && QS == 2) ? 1 : 2) void bsca
                 m.tki[e] = 0xFFFFFFFFu;
             }
         wave_sync();
-        // A operand rows of this lane: query row t*16 + lane%16 of the tile
+        // A operand rows of this lane: query row t*16 + lane%16 of the wave's rows
         const int8_t *arow1[QTM], *arow2[QTM];
 #pragma unroll
         for (int t = 0; t < QTM; ++t) {
@@ -954,7 +1004,7 @@ __global__ __launch_bounds__(kBThreads, (QTM == 4 && QS == 2) ? 1 : 2) void bsca
         const int v0 = a.tile_v0[tile];
         const int mb_beg = v0 / kMB, mb_end = min((n + kMB - 1) / kMB, (v0 + sv) / kMB);
         const int64_t mb0 = bv.clu_mb[c];
-        // factor blocks (|o|^2, the rescale of every quantised segment, the ids) of a chunk's
+        // factor blocks (|o|^2, -2 x the rescale of every quantised segment, the ids) of a chunk's
         // mblocks move to LDS by DMA one chunk ahead of their use
         const int fac_words = kMB * (2 + bv.nqs);
         auto fetch_fac = [&](int cbn, int slot) {
@@ -976,14 +1026,20 @@ __global__ __launch_bounds__(kBThreads, (QTM == 4 && QS == 2) ? 1 : 2) void bsca
             dma_wait();  // this chunk's factor blocks (the previous chunk's reads of the other slot are done)
             const float *fac = m.fac + slot * NVW * fac_words;
 #pragma unroll
-            for (int j = 0; j < NVW; ++j) {
-                rec[j] = bv.mb + (mb0 + min(cb + j, mb_end - 1)) * bv.mb_stride;  // clamped duplicates are discarded
-                const float o2 = fac[j * fac_words + lane % kMB];
+            for (int t = 0; t < QTM; ++t) {
+                const float4 q2 = *reinterpret_cast<const float4 *>(m.q2 + t * 16 + 4 * (lane / 16));
 #pragma unroll
-                for (int t = 0; t < QTM; ++t)
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) dist[t][j][r] = o2 + m.q2[t * 16 + 4 * (lane / 16) + r];
+                for (int j = 0; j < NVW; ++j) {
+                    const float o2 = fac[j * fac_words + lane % kMB];
+                    dist[t][j][0] = o2 + q2.x;
+                    dist[t][j][1] = o2 + q2.y;
+                    dist[t][j][2] = o2 + q2.z;
+                    dist[t][j][3] = o2 + q2.w;
+                }
             }
+#pragma unroll
+            for (int j = 0; j < NVW; ++j)
+                rec[j] = bv.mb + (mb0 + min(cb + j, mb_end - 1)) * bv.mb_stride;  // clamped duplicates are discarded
             if (cb + NVW < mb_end) fetch_fac(cb + NVW, slot ^ 1);
             for (int s = 0; s < bv.nseg; ++s) {
                 const BSeg bs = bv.segs[s];
@@ -993,32 +1049,38 @@ __global__ __launch_bounds__(kBThreads, (QTM == 4 && QS == 2) ? 1 : 2) void bsca
                 for (int t = 0; t < QTM; ++t)
 #pragma unroll
                     for (int j = 0; j < NVW; ++j) acc1[t][j] = acc2[t][j] = ach1[t][j] = ach2[t][j] = v4i{0, 0, 0, 0};
+                const bool two = bs.bits > 8;
                 if (bs.bits <= 4)
                     seg_mma<0, QTM, QS, NVW>(bs, rec, arow1, arow2, acc1, acc2, ach1, ach2, lane);
-                else if (bs.bits <= 8)
+                else if (!two)
                     seg_mma<1, QTM, QS, NVW>(bs, rec, arow1, arow2, acc1, acc2, ach1, ach2, lane);
                 else
                     seg_mma<2, QTM, QS, NVW>(bs, rec, arow1, arow2, acc1, acc2, ach1, ach2, lane);
-                // epilogue of the segment
+                // epilogue of the segment: ip = a1 acc1 (+ 256 a1 acch1) (+ a2 acc2 ...) + c, then
+                // dist += (-2 rescale) ip, all fused multiply-adds
 #pragma unroll
-                for (int j = 0; j < NVW; ++j) {
-                    const float resc = fac[j * fac_words + kMB * (1 + bs.qs) + lane % kMB];
+                for (int t = 0; t < QTM; ++t) {
+                    const int rb = bs.qs * QT + t * 16 + 4 * (lane / 16);
+                    const float4 ka = *reinterpret_cast<const float4 *>(m.ka + rb);
+                    const float4 kc = *reinterpret_cast<const float4 *>(m.kc + rb);
+                    float4 kb = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    if (QS == 2) kb = *reinterpret_cast<const float4 *>(m.kb + rb);
+                    const float av[4] = {ka.x, ka.y, ka.z, ka.w}, bv2[4] = {kb.x, kb.y, kb.z, kb.w};
+                    const float cv[4] = {kc.x, kc.y, kc.z, kc.w};
 #pragma unroll
-                    for (int t = 0; t < QTM; ++t)
+                    for (int j = 0; j < NVW; ++j) {
+                        const float m2r = fac[j * fac_words + kMB * (1 + bs.qs) + lane % kMB];
 #pragma unroll
                         for (int r = 0; r < 4; ++r) {
-                            const int qi = t * 16 + 4 * (lane / 16) + r;
-                            const float *qc = m.qc + (qi * bv.nqs + bs.qs) * 3;
-                            const int32_t *cc = m.qcorr + (qi * bv.nqs + bs.qs) * 2;
-                            const float i1 = (float)(acc1[t][j][r] + cc[0]) + 256.0f * (float)ach1[t][j][r];
-                            float ip = qc[0] * i1;
+                            float ip = fmaf(av[r], (float)acc1[t][j][r], cv[r]);
+                            if (two) ip = fmaf(256.0f * av[r], (float)ach1[t][j][r], ip);
                             if (QS == 2) {
-                                const float i2 = (float)(acc2[t][j][r] + cc[1]) + 256.0f * (float)ach2[t][j][r];
-                                ip = ip + qc[1] * i2;
+                                ip = fmaf(bv2[r], (float)acc2[t][j][r], ip);
+                                if (two) ip = fmaf(256.0f * bv2[r], (float)ach2[t][j][r], ip);
                             }
-                            ip = resc * (ip + qc[2]);
-                            dist[t][j][r] = dist[t][j][r] - 2.0f * ip;
+                            dist[t][j][r] = fmaf(m2r, ip, dist[t][j][r]);
                         }
+                    }
                 }
             }
             // ---- rows (queries) with a distance of this chunk at or below their k-th distance: the
@@ -1108,19 +1170,22 @@ __global__ __launch_bounds__(kBThreads, (QTM == 4 && QS == 2) ? 1 : 2) void bsca
             }
             wave_sync();  // the next chunk rewrites dt and dids
         }
-        // ---- the tile's per-query lists to global memory
+        // ---- the wave's per-query lists to global memory
         if (KR == 1)
             for (int i = 0; i < cnt; ++i) {
                 const int nb = m.cbn[i];
                 if (nb) flush_row(m, i, a.topk, nb, cand_cap(QTM));
             }
         for (int i = 0; i < cnt; ++i) {
-            const int64_t pr = (int64_t)a.tile_cand[tile] + i;
+            const int64_t pr = (int64_t)a.tile_cand[tile] + w * QT + i;
             if (KR == 1) {  // padded lists: unused places already hold (INF, 0xFFFFFFFF)
                 for (int e = lane; e < a.topk; e += 64) {
                     a.cand_d[pr * a.topk + e] = m.tkd[i * a.topk + e];
                     a.cand_id[pr * a.topk + e] = m.tki[i * a.topk + e];
                 }
+                // a full list's k-th distance bounds the query's for the tiles still to come
+                const float kth = m.tkd[i * a.topk + a.topk - 1];
+                if (lane == 0 && kth < m.rb[i]) atomicMin(&a.qbound[a.pairs[first + i] / a.NP], float_key(kth));
                 continue;
             }
             const int nn = m.tkn[i];
@@ -1256,6 +1321,16 @@ __global__ __launch_bounds__(256) void bq_merge_kernel(const float *cand_d, cons
 }
 
 // ------------------------------------------------------------------ host-side launchers
+// The (waves per workgroup, query tiles of 16 per wave) of a query tile of qt queries.
+static void tile_shape(int qt, int &W, int &qtm) {
+    switch (qt) {
+    case 16: W = 1, qtm = 1; break;
+    case 32: W = 2, qtm = 1; break;
+    case 64: W = 4, qtm = 1; break;
+    default: W = 4, qtm = 2; break;  // 128
+    }
+}
+
 hipError_t launch_bq_build(const IndexView &ix, const BatchedView &bv, const int32_t *mb_clu, int64_t total_mb,
                            const int32_t *chunk_seg, const int32_t *chunk_kc, int nchunks, uint8_t *out,
                            hipStream_t st) {
@@ -1270,51 +1345,54 @@ hipError_t launch_bq_build(const IndexView &ix, const BatchedView &bv, const int
 
 hipError_t launch_bq_group(const uint32_t *probes, const int32_t *npq, int64_t nq, int NP, int K, int QT,
                            const int32_t *order, const int32_t *clu_n, const BqWork &w, hipStream_t st) {
-    hipLaunchKernelGGL(bq_zero_kernel, dim3(16), dim3(256), 0, st, w.counts, K, w.work);
+    int W, qtm;
+    tile_shape(QT, W, qtm);
+    const int workers = kSplitWaves / W;  // resident workgroups of the tile kernel
+    hipLaunchKernelGGL(bq_zero_kernel, dim3(16), dim3(256), 0, st, w.counts, K, w.work, w.qbound, nq);
     const int64_t n = nq * NP;
     const unsigned g = (unsigned)((n + 255) / 256);
     hipLaunchKernelGGL(bq_count_kernel, dim3(g), dim3(256), 0, st, probes, npq, nq, NP, w.counts, w.slot);
-    hipLaunchKernelGGL(bq_scan_kernel, dim3(1), dim3(1024), 0, st, w.counts, order, clu_n, K, QT, w.cstart,
+    hipLaunchKernelGGL(bq_scan_kernel, dim3(1), dim3(1024), 0, st, w.counts, order, clu_n, K, QT, workers, w.cstart,
                        w.candstart, w.nsplit, w.ntiles, w.npairs, w.sv, w.tile_clu, w.tile_first, w.tile_cnt,
                        w.tile_v0, w.tile_cand);
     hipLaunchKernelGGL(bq_scatter_kernel, dim3(g), dim3(256), 0, st, probes, w.slot, n, w.cstart, w.pairs);
     return hipGetLastError();
 }
 
-template <int QTM, int QS, int KR>
+template <int W, int QTM, int QS, int KR>
 static hipError_t launch_bscan_t(const BScanArgs &a, int grid, hipStream_t st) {
-    const size_t smem = kBWaves * wave_smem<QTM, QS>(a.bv.nqs, a.topk, nullptr, nullptr);
-    auto kern = bscan_kernel<QTM, QS, KR>;
+    const size_t smem = wg_smem<W, QTM, QS>(a.bv.nqs, a.topk);
+    auto kern = bscan_kernel<W, QTM, QS, KR>;
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
     if (e != hipSuccess) return e;
     int per_cu = 0;
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kBThreads, smem);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * W, smem);
     if (e != hipSuccess) return e;
     int dev = 0, ncu = 0;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
     const int g = std::max(1, std::min(grid, std::max(1, per_cu) * ncu));
-    hipLaunchKernelGGL(kern, dim3(g), dim3(kBThreads), smem, st, a);
+    hipLaunchKernelGGL(kern, dim3(g), dim3(64 * W), smem, st, a);
     return hipGetLastError();
 }
 
-size_t bscan_smem_bytes(const BatchedView &bv, int topk, int qtm, int qs) {
+size_t bscan_smem_bytes(const BatchedView &bv, int topk, int qt, int qs) {
+    int W, qtm;
+    tile_shape(qt, W, qtm);
     if (qs == 2) {
-        if (qtm == 1) return kBWaves * wave_smem<1, 2>(bv.nqs, topk, nullptr, nullptr);
-        if (qtm == 2) return kBWaves * wave_smem<2, 2>(bv.nqs, topk, nullptr, nullptr);
-        return kBWaves * wave_smem<4, 2>(bv.nqs, topk, nullptr, nullptr);
+        if (qtm == 1) return (size_t)W * wave_smem<1, 2>(bv.nqs, topk, nullptr, nullptr) + 16;
+        return (size_t)W * wave_smem<2, 2>(bv.nqs, topk, nullptr, nullptr) + 16;
     }
-    if (qtm == 1) return kBWaves * wave_smem<1, 1>(bv.nqs, topk, nullptr, nullptr);
-    if (qtm == 2) return kBWaves * wave_smem<2, 1>(bv.nqs, topk, nullptr, nullptr);
-    return kBWaves * wave_smem<4, 1>(bv.nqs, topk, nullptr, nullptr);
+    if (qtm == 1) return (size_t)W * wave_smem<1, 1>(bv.nqs, topk, nullptr, nullptr) + 16;
+    return (size_t)W * wave_smem<2, 1>(bv.nqs, topk, nullptr, nullptr) + 16;
 }
 
-template <int QTM, int QS>
+template <int W, int QTM, int QS>
 static hipError_t launch_bscan_q(const BScanArgs &a, int grid, hipStream_t st) {
-    if (a.topk <= 64) return launch_bscan_t<QTM, QS, 1>(a, grid, st);
-    if (a.topk <= 128) return launch_bscan_t<QTM, QS, 2>(a, grid, st);
-    return launch_bscan_t<QTM, QS, 4>(a, grid, st);
+    if (a.topk <= 64) return launch_bscan_t<W, QTM, QS, 1>(a, grid, st);
+    if (a.topk <= 128) return launch_bscan_t<W, QTM, QS, 2>(a, grid, st);
+    return launch_bscan_t<W, QTM, QS, 4>(a, grid, st);
 }
 
 hipError_t launch_bq_pairprep(const BatchedView &bv, const float *qrot, const uint32_t *probes, const BqWork &w,
@@ -1330,7 +1408,6 @@ hipError_t launch_bq_pairprep(const BatchedView &bv, const float *qrot, const ui
     a.qq1 = pb.qq1;
     a.qq2 = pb.qq2;
     a.pc = pb.pc;
-    a.pcorr = pb.pcorr;
     a.pq2 = pb.pq2;
     const dim3 grid((unsigned)((max_pairs + 3) / 4));
     if (bv.D_pad <= 512)
@@ -1342,14 +1419,13 @@ hipError_t launch_bq_pairprep(const BatchedView &bv, const float *qrot, const ui
     return hipGetLastError();
 }
 
-hipError_t launch_bscan_tiles(const BatchedView &bv, const BqWork &w, const BqPairBufs &pb, int topk,
-                              float *cand_d, uint32_t *cand_id, int qtm, int qs, int grid, hipStream_t st) {
+hipError_t launch_bscan_tiles(const BatchedView &bv, const BqWork &w, const BqPairBufs &pb, int NP, int topk,
+                              float *cand_d, uint32_t *cand_id, int qt, int qs, int grid, hipStream_t st) {
     BScanArgs a;
     a.bv = bv;
     a.qq1 = pb.qq1;
     a.qq2 = pb.qq2;
     a.pc = pb.pc;
-    a.pcorr = pb.pcorr;
     a.pq2 = pb.pq2;
     a.tile_clu = w.tile_clu;
     a.tile_first = w.tile_first;
@@ -1359,12 +1435,18 @@ hipError_t launch_bscan_tiles(const BatchedView &bv, const BqWork &w, const BqPa
     a.ntiles = w.ntiles;
     a.sv = w.sv;
     a.work = w.work;
+    a.pairs = w.pairs;
+    a.NP = NP;
+    a.qbound = w.qbound;
     a.topk = topk;
     a.cand_d = cand_d;
     a.cand_id = cand_id;
-    if (qtm == 1) return qs == 2 ? launch_bscan_q<1, 2>(a, grid, st) : launch_bscan_q<1, 1>(a, grid, st);
-    if (qtm == 2) return qs == 2 ? launch_bscan_q<2, 2>(a, grid, st) : launch_bscan_q<2, 1>(a, grid, st);
-    return qs == 2 ? launch_bscan_q<4, 2>(a, grid, st) : launch_bscan_q<4, 1>(a, grid, st);
+    switch (qt) {
+    case 16: return qs == 2 ? launch_bscan_q<1, 1, 2>(a, grid, st) : launch_bscan_q<1, 1, 1>(a, grid, st);
+    case 32: return qs == 2 ? launch_bscan_q<2, 1, 2>(a, grid, st) : launch_bscan_q<2, 1, 1>(a, grid, st);
+    case 64: return qs == 2 ? launch_bscan_q<4, 1, 2>(a, grid, st) : launch_bscan_q<4, 1, 1>(a, grid, st);
+    default: return qs == 2 ? launch_bscan_q<4, 2, 2>(a, grid, st) : launch_bscan_q<4, 2, 1>(a, grid, st);
+    }
 }
 
 hipError_t launch_bq_merge(const float *cand_d, const uint32_t *cand_id, const uint32_t *probes, const BqWork &w,

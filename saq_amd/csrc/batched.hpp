@@ -5,10 +5,9 @@
 namespace saqgpu {
 
 constexpr int kMB = 16;     // vectors per MFMA column block (one mblock record)
-constexpr int kBWaves = 1;  // waves per tile workgroup (one: LDS is then allocated per wavefront)
-constexpr int kBThreads = 64 * kBWaves;
 // Vector ranges of a list's work tiles: at least kMinSplit vectors, at most about the launch's
-// work / kSplitWaves (the resident wavefronts of a 256-CU chip at 2 waves per SIMD).
+// work / (kSplitWaves / waves per workgroup) (kSplitWaves: the resident wavefronts of a 256-CU
+// chip at 2 waves per SIMD).
 constexpr int kMinSplit = 512;
 constexpr int kSplitWaves = 2048;
 // Lists the grouping scan holds in registers: 1024 threads x kScanRounds each (the nlist limit of the
@@ -42,6 +41,7 @@ struct BatchedView {
 struct BqWork {
     int32_t *counts, *slot, *cstart, *candstart, *nsplit, *ntiles, *npairs, *sv, *pairs;
     int32_t *tile_clu, *tile_first, *tile_cnt, *tile_v0, *tile_cand, *work;
+    uint32_t *qbound;  // per query: the distance bound its tiles share (nq entries)
 };
 // Upper bounds of the work tiles and candidate lists of a launch.
 inline int64_t bq_max_tiles(int64_t npairs, int K) { return npairs + K + kSplitWaves; }
@@ -51,11 +51,11 @@ inline int64_t bq_max_cands(int64_t npairs, int QT) { return npairs + (int64_t)k
 struct BqPairBufs {
     int8_t *qq1, *qq2;  // [pairs][kq]
     float *pc;          // [pairs][nqs][3]
-    int32_t *pcorr;     // [pairs][nqs][2]
     float *pq2;         // [pairs]
 };
 
-size_t bscan_smem_bytes(const BatchedView &bv, int topk, int qtm, int qs);
+// LDS of the tile kernel's workgroup for a query tile of qt (16, 32, 64 or 128) queries.
+size_t bscan_smem_bytes(const BatchedView &bv, int topk, int qt, int qs);
 hipError_t launch_bq_build(const IndexView &ix, const BatchedView &bv, const int32_t *mb_clu, int64_t total_mb,
                            const int32_t *chunk_seg, const int32_t *chunk_kc, int nchunks, uint8_t *out,
                            hipStream_t st);
@@ -70,9 +70,10 @@ hipError_t launch_bq_merge(const float *cand_d, const uint32_t *cand_id, const u
 hipError_t launch_bq_pairprep(const BatchedView &bv, const float *qrot, const uint32_t *probes, const BqWork &w,
                               const BqPairBufs &pb, int NP, int64_t max_pairs, int qs, hipStream_t st);
 
-// One launch of the tile kernel: qtm = query tiles of 16 per work tile (1, 2, 4), qs = terms
-// of the quantised query (1 = int8, 2 = int8x2).  grid is an upper bound on the workgroups.
-hipError_t launch_bscan_tiles(const BatchedView &bv, const BqWork &w, const BqPairBufs &pb, int topk,
-                              float *cand_d, uint32_t *cand_id, int qtm, int qs, int grid, hipStream_t st);
+// One launch of the tile kernel: qt = queries per work tile (16: one wave; 32, 64: 2 or 4 waves of
+// 16 queries; 128: 4 waves of 32), qs = terms of the quantised query (1 = int8, 2 = int8x2).  grid
+// is an upper bound on the workgroups.
+hipError_t launch_bscan_tiles(const BatchedView &bv, const BqWork &w, const BqPairBufs &pb, int NP, int topk,
+                              float *cand_d, uint32_t *cand_id, int qt, int qs, int grid, hipStream_t st);
 
 }  // namespace saqgpu

@@ -5,9 +5,9 @@ The batched path computes, for every vector of every probed list, the accurate S
 quantised per (query, list, segment) to int8 (or two int8 terms).  This model restates that
 in numpy from independent inputs: the oracle's rotated query (orc_gemv_rowvec), the oracle
 encoder's codes and factors (orc_caq_encode on the oracle's rotated residuals) and the
-rotated centroids.  The integer dot products are exact; the fp32 epilogue follows the
-kernel's operation order except the per-segment sum of r^2, whose order differs, so model
-distances match to about 1e-6 relative, not bitwise.
+rotated centroids.  The integer dot products are exact; the fp32 epilogue takes the same terms
+in the same order as the kernel but without its fused multiply-adds (and with its own order for
+the per-segment sum of r^2), so model distances match to about 1e-6 relative, not bitwise.
 """
 import ctypes as C
 
@@ -92,6 +92,12 @@ def f32(x):
     return np.float32(x)
 
 
+def fma32(a, b, c):
+    """float32 fused multiply-add: the product of two float32 values is exact in float64, so one
+    float64 add and one rounding to float32 (double rounding aside) give fmaf's result."""
+    return (np.asarray(a, np.float64) * np.asarray(b, np.float64) + np.asarray(c, np.float64)).astype(np.float32)
+
+
 def model_search(case, tables, qrot, probes, topk, qs=1, npq=None):
     """Per query: the topk smallest (distance, id) pairs of the batched estimator, and the
     distance of every probed vector (dict id -> distance)."""
@@ -127,8 +133,13 @@ def model_search(case, tables, qrot, probes, topk, qs=1, npq=None):
                     v2 = np.clip(np.rint((rem * inv2).astype(np.float32)), -127, 127)
                 sqd = f32(2.0 / (1 << b))
                 sr = f32(np.sum(r.astype(np.float64)))
-                consts.append((v1.astype(np.int64), v2.astype(np.int64), f32(sqd * s1), f32(sqd * s2),
-                               f32(f32(-1.0 + sqd / 2) * sr)))
+                a1, a2 = f32(sqd * s1), f32(sqd * s2)
+                cst = f32(f32(-1.0 + sqd / 2) * sr)
+                if b >= 8:  # the code shift's correction, folded into the constant as pair prep does
+                    cst = fma32(a1, f32(128 * int(v1.sum())), cst)
+                    if qs == 2:
+                        cst = fma32(a2, f32(128 * int(v2.sum())), cst)
+                consts.append((v1.astype(np.int64), v2.astype(np.int64), a1, a2, cst))
             o2 = np.zeros(n, np.float32)
             for s in range(len(case.plan)):
                 o2 = (o2 + segs[s][1] * segs[s][1]).astype(np.float32)
@@ -138,12 +149,18 @@ def model_search(case, tables, qrot, probes, topk, qs=1, npq=None):
                     continue
                 v1, v2, a1, a2, kq = consts[s]
                 codes, _, resc = segs[s]
-                i1 = (codes @ v1).astype(np.float32)
-                ip = (a1 * i1).astype(np.float32)
+                # the kernel's operands: codes of 8 bits shifted by -128; 9+ bits as a shifted low
+                # byte plane and a high plane
+                lo = (codes & 255) - 128 if b >= 9 else (codes - 128 if b == 8 else codes)
+                hi = codes >> 8
+                ip = fma32(a1, (lo @ v1).astype(np.float32), kq)
+                if b >= 9:
+                    ip = fma32(f32(256) * a1, (hi @ v1).astype(np.float32), ip)
                 if qs == 2:
-                    ip = (ip + a2 * (codes @ v2).astype(np.float32)).astype(np.float32)
-                ip = (resc * (ip + kq)).astype(np.float32)
-                dist = (dist - f32(2) * ip).astype(np.float32)
+                    ip = fma32(a2, (lo @ v2).astype(np.float32), ip)
+                    if b >= 9:
+                        ip = fma32(f32(256) * a2, (hi @ v2).astype(np.float32), ip)
+                dist = fma32((f32(-2) * resc).astype(np.float32), ip, dist)
             for u, d in zip(ids.tolist(), dist.tolist()):
                 cand[u] = d
         order = sorted(cand.items(), key=lambda t: (t[1], t[0]))[:topk]

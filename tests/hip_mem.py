@@ -15,6 +15,9 @@ def hip():
         L.hipFree.argtypes = [C.c_void_p]
         L.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
         L.hipDeviceSynchronize.argtypes = []
+        L.hipStreamCreate.argtypes = [C.POINTER(C.c_void_p)]
+        L.hipStreamSynchronize.argtypes = [C.c_void_p]
+        L.hipStreamDestroy.argtypes = [C.c_void_p]
         _hip = L
     return _hip
 

@@ -78,20 +78,22 @@ def test_topk_sizes_and_unfilled(topk):
 def test_query_tiles_give_identical_results():
     # 4 lists and 200 queries: several full tiles per list at every tile size
     case = F.make_case(n=3000, d=128, K=4, nq=200, plan=[(128, 5)], seed=57)
-    out = [run(case, 10, 2, PREC_INT8, query_tile=t) for t in (16, 32, 64)]
-    for ids, d in out[1:]:
-        assert np.array_equal(ids, out[0][0])
-        assert np.array_equal(d.view(np.uint32), out[0][1].view(np.uint32))
+    for prec in (PREC_INT8, PREC_INT8X2):
+        out = [run(case, 10, 2, prec, query_tile=t) for t in (16, 32, 64, 128)]
+        for ids, d in out[1:]:
+            assert np.array_equal(ids, out[0][0])
+            assert np.array_equal(d.view(np.uint32), out[0][1].view(np.uint32))
 
 
-@pytest.mark.parametrize("topk,tiles", [(10, (16, 64)), (100, (16, 32))])
+@pytest.mark.parametrize("topk,tiles", [(10, (16, 64, 128)), (100, (16, 32, 128))])
 def test_long_lists_split_into_vector_ranges(topk, tiles):
     """Lists of ~4000 vectors: each (list, query tile) becomes 16 work tiles of 256-vector
     ranges with candidate lists of their own, which the merge reads back per pair."""
     case = F.make_case(n=8000, d=128, K=2, nq=150, plan=[(64, 6), (64, 3)], seed=63)
     out = [run(case, topk, 2, PREC_INT8, query_tile=t) for t in tiles]
-    assert np.array_equal(out[0][0], out[1][0])
-    assert np.array_equal(out[0][1].view(np.uint32), out[1][1].view(np.uint32))
+    for ids, d in out[1:]:
+        assert np.array_equal(out[0][0], ids)
+        assert np.array_equal(out[0][1].view(np.uint32), d.view(np.uint32))
 
 
 def test_uneven_nprobe_per_query():
@@ -152,4 +154,40 @@ def test_recall_not_below_plain_scan():
     r_bat = np.mean([len(set(bat[i]) & set(gt[i])) / 10 for i in range(100)])
     print(f"recall@10 plain {r_plain:.3f} batched {r_bat:.3f}")
     assert r_bat >= r_plain - 0.005
+    ix.close()
+
+
+def test_graph_replays_equal_eager_calls():
+    """SAQGPU_USE_GRAPH: the first call captures the batched step into a HIP graph; later calls with
+    the same buffers replay it.  Each replay -- with new query contents written into the same device
+    buffer and eager work on the GPU in between -- equals an eager call on the same queries, for the
+    GEMM coarse quantiser (a library GEMM inside the graph) and the exact one."""
+    import ctypes as C
+    from hip_mem import DevArray, hip
+    case = F.make_case(n=4000, d=200, K=24, nq=64, plan=[(128, 5), (128, 3)], seed=71)
+    ix = F.gpu_index(case)
+    H = hip()
+    stream = C.c_void_p()
+    assert H.hipStreamCreate(C.byref(stream)) == 0
+    rng = np.random.default_rng(5)
+    for exact in (False, True):
+        qbuf = DevArray(case.queries)
+        ids = DevArray(shape=(64, 10), dtype=np.uint32)
+        dd = DevArray(shape=(64, 10), dtype=np.float32)
+        for rep in range(4):
+            q = case.queries[rng.permutation(64)].copy()
+            assert H.hipMemcpy(C.c_void_p(qbuf.data_ptr()), q.ctypes.data, q.nbytes, 1) == 0
+            print(f"graph replay exact={exact} rep={rep}", flush=True)
+            ix.search_batched(qbuf, 10, 11, exact_probe=exact, out_ids=ids, out_dists=dd, stream=stream.value,
+                              graph=True)
+            assert H.hipStreamSynchronize(stream) == 0
+            got_i, got_d = ids.numpy(), dd.numpy()
+            want_i, want_d = ix.search_batched(q, 10, 11, exact_probe=exact)
+            assert np.array_equal(got_i, want_i), f"exact={exact} rep={rep}"
+            assert np.array_equal(got_d.view(np.uint32), want_d.view(np.uint32))
+            ix.search(case.queries, 10, 4)  # other GPU work between replays
+        qbuf.free()
+        ids.free()
+        dd.free()
+    assert H.hipStreamDestroy(stream) == 0
     ix.close()
