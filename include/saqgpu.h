@@ -205,19 +205,22 @@ int saqgpu_make_plan(const float *variance, int32_t num_dim, float avg_bits, int
 /* ---- batched scan (no single counterpart in the reference) ----------------------------
  * Serves a batch of queries as one grouped job instead of nq independent IVF::search calls:
  * (query, list) pairs are grouped by list and each list's codes are contracted against a tile
- * of its queries on int8 MFMA.  Every vector of every probed list gets the accurate estimate
+ * of its queries on MFMA.  Every vector of every probed list gets the accurate estimate
  * of caq_estimator.hpp:190-217 (no fast-scan pruning), computed from the full codes with the
- * residual query quantised per (query, list, segment) to int8 (one term) or int8x2 (two
- * terms); the per-query top-k is exact for those estimates, ties broken by id.  IDs therefore
+ * residual query quantised per (query, list, segment) to int8 (one term), int8x2 (two
+ * terms) or bf16; the per-query top-k is exact for those estimates, ties broken by id.  IDs therefore
  * differ from IVF::search's only where the estimates' quantisation reorders near-ties. */
 #define SAQGPU_PREC_INT8 1
 #define SAQGPU_PREC_INT8X2 2
+/* BF16: the residual per (query, list, segment) as r / max|r| in bf16 (an fp32 scale), contracted
+ * with the codes on v_mfma_f32_16x16x32_bf16 with fp32 accumulation. */
+#define SAQGPU_PREC_BF16 3
 
 typedef struct {
     int32_t topk;                     /* 1..256 */
     int32_t nprobe;                   /* lists per query (the maximum when nprobe_per_query is set) */
     const int32_t *nprobe_per_query;  /* NULL, or nq entries in 1..nprobe, in the queries' memory kind */
-    int32_t precision;                /* SAQGPU_PREC_INT8 or SAQGPU_PREC_INT8X2 */
+    int32_t precision;                /* SAQGPU_PREC_INT8, SAQGPU_PREC_INT8X2 or SAQGPU_PREC_BF16 */
     int32_t query_tile;               /* queries per work tile: 16, 32, 64 or 128 (a workgroup of 1, 2, 4
                                          or 4 waves reading each code chunk once for them); 0 = the
                                          autotuned choice */

@@ -53,6 +53,7 @@ class BatchedParams(C.Structure):
 
 PREC_INT8 = 1
 PREC_INT8X2 = 2
+PREC_BF16 = 3
 SAQGPU_NOALLOC_SCRATCH_FLOATS = 1 << 24  # include/saqgpu.h
 
 _lib = None

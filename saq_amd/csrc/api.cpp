@@ -1345,8 +1345,8 @@ int prepare_batched_locked(saqgpu_index *ix) {
 int check_batched_params(const saqgpu_index *ix, const saqgpu_batched_params *p, int64_t nq, int32_t flags) {
     if (p->topk < 1 || p->topk > 256) return fail(SAQGPU_EINVAL, "batched topk must be in 1..256");
     if (p->nprobe < 1 || p->nprobe > ix->num_cen) return fail(SAQGPU_EINVAL, "nprobe must be in 1..K");
-    if (p->precision != SAQGPU_PREC_INT8 && p->precision != SAQGPU_PREC_INT8X2)
-        return fail(SAQGPU_EINVAL, "precision must be SAQGPU_PREC_INT8 or SAQGPU_PREC_INT8X2");
+    if (p->precision != SAQGPU_PREC_INT8 && p->precision != SAQGPU_PREC_INT8X2 && p->precision != SAQGPU_PREC_BF16)
+        return fail(SAQGPU_EINVAL, "precision must be SAQGPU_PREC_INT8, SAQGPU_PREC_INT8X2 or SAQGPU_PREC_BF16");
     if (p->query_tile != 0 && p->query_tile != 16 && p->query_tile != 32 && p->query_tile != 64 &&
         p->query_tile != 128)
         return fail(SAQGPU_EINVAL, "query_tile must be 0, 16, 32, 64 or 128");
@@ -1384,7 +1384,8 @@ int search_batched_locked(saqgpu_index *ix, const float *queries, int64_t nq, co
     if (graph && !dev) return fail(SAQGPU_EINVAL, "SAQGPU_USE_GRAPH needs SAQGPU_DEVICE_PTRS");
     if (graph && !st) return fail(SAQGPU_EINVAL, "SAQGPU_USE_GRAPH needs a stream other than the null stream");
     const int NP = p->nprobe, K = ix->num_cen, topk = p->topk;
-    const int qs = p->precision == SAQGPU_PREC_INT8X2 ? 2 : 1;
+    // query terms: 1 (int8), 2 (int8x2) or BF16 (3: one term, rows of 2 bytes per dimension)
+    const int qs = p->precision == SAQGPU_PREC_INT8X2 ? 2 : (p->precision == SAQGPU_PREC_BF16 ? 3 : 1);
     const BatchedView bv = batched_view(ix);
     int max_smem = 0;
     HIPCHK(hipDeviceGetAttribute(&max_smem, hipDeviceAttributeMaxSharedMemoryPerBlock, ix->device));
@@ -1399,7 +1400,7 @@ int search_batched_locked(saqgpu_index *ix, const float *queries, int64_t nq, co
     rc |= ix->ws_qrot.ensure(nq * ix->D_pad);
     rc |= ix->ws_bound.ensure(nq * ix->nseg);
     rc |= ix->bq_ws.ensure(4 * (size_t)K + 2 * npairs + 5 * max_tiles + 4 + nq);
-    rc |= ix->bq_qq.ensure((size_t)qs * npairs * ix->bq_kq);
+    rc |= ix->bq_qq.ensure((size_t)(qs == 1 ? 1 : 2) * npairs * ix->bq_kq);
     rc |= ix->bq_pc.ensure((size_t)npairs * (3 * ix->bq_nqs + 1));
     rc |= ix->bq_cand_d.ensure(max_cands * topk);
     rc |= ix->bq_cand_id.ensure(max_cands * topk);
@@ -1595,7 +1596,7 @@ int saqgpu_autotune_batched(saqgpu_index *ix, const float *queries, int64_t nq, 
     HIPCHK(hipDeviceGetAttribute(&max_smem, hipDeviceAttributeMaxSharedMemoryPerBlock, ix->device));
     if ((rc = prepare_batched_locked(ix))) return rc;
     const BatchedView bv = batched_view(ix);
-    const int qs = p->precision == SAQGPU_PREC_INT8X2 ? 2 : 1;
+    const int qs = p->precision == SAQGPU_PREC_INT8X2 ? 2 : (p->precision == SAQGPU_PREC_BF16 ? 3 : 1);
     float best = 1e30f;
     int best_qt = 16;
     for (int qt : {16, 32, 64, 128}) {

@@ -32,6 +32,8 @@
 namespace saqgpu {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 
 // ------------------------------------------------------------------ layout conversion
 // Low bits of the CodeHelper<k> compacted layouts (utils/code_helper.hpp:95-591) for
@@ -483,6 +485,42 @@ struct PairPrepArgs {
     float *pq2;         // [pairs]: |r|^2 over all segments
 };
 
+// BF16 form of a segment's residual (precision SAQGPU_PREC_BF16): r / amax rounded to bf16 (an fp32
+// scale s1 = amax), stored 2 bytes per dimension in the row; the tile kernel uses the codes unshifted,
+// so the constant term is (-1 + sqd / 2) * sum r alone.  r[k] holds the 4 dimensions from
+// base + 4 lane + 256 k of the padded vector; those outside the segment are skipped.
+__device__ __forceinline__ uint32_t bf16_pair(float x, float y) {
+    // round to nearest even (finite inputs)
+    auto rne = [](float v) {
+        const uint32_t u = __float_as_uint(v);
+        return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
+    };
+    return rne(x) | (rne(y) << 16);
+}
+
+template <int NK>
+__device__ __forceinline__ void bf16_segment(const PairPrepArgs &a, const BSeg &bs, int64_t g, const float4 (&r)[NK],
+                                             int base, float amax, float sr) {
+    const int lane = threadIdx.x & 63;
+    const float inv = amax > 0.0f ? 1.0f / amax : 0.0f;
+    uint16_t *row = reinterpret_cast<uint16_t *>(a.qq1) + g * a.bv.kq + bs.kq_off;
+#pragma unroll
+    for (int k = 0; k < NK; ++k) {
+        const int d = base + 4 * lane + 256 * k;  // absolute padded dimension of r[k].x
+        if (d < bs.doff || d >= bs.doff + bs.dim) continue;
+        uint2 v;
+        v.x = bf16_pair(r[k].x * inv, r[k].y * inv);
+        v.y = bf16_pair(r[k].z * inv, r[k].w * inv);
+        *reinterpret_cast<uint2 *>(row + (d - bs.doff)) = v;
+    }
+    if (lane == 0) {
+        float *pc = a.pc + (g * a.bv.nqs + bs.qs) * 3;
+        pc[0] = bs.sqd * amax;
+        pc[1] = 0.0f;
+        pc[2] = (-1.0f + bs.sqd / 2) * sr;
+    }
+}
+
 // The pair's constants of one quantised segment: the term scales a1 = sqd * s1, a2 = sqd * s2 and
 // the constant term of its inner product, (-1 + sqd / 2) * sum r plus the code shift's correction
 // a_t * 128 * sum q_t of each term (codes of 8 and more bits are stored shifted by -128), so that
@@ -543,6 +581,10 @@ __device__ __forceinline__ void pairprep_segment(const PairPrepArgs &a, const BS
     red3([](float v) { return xor_f<1>(v); });
     q2 += sq;
     if (bs.qs < 0) return;
+    if (a.qs == 3) {
+        bf16_segment<NK>(a, bs, g, r, bs.doff, amax, sr);
+        return;
+    }
     const float s1 = amax / 127.0f, inv1 = amax > 0.0f ? 127.0f / amax : 0.0f;
     const float s2 = s1 / 254.0f, inv2 = amax > 0.0f ? 254.0f * 127.0f / amax : 0.0f;
     int sum1 = 0, sum2 = 0;
@@ -621,6 +663,10 @@ __device__ __forceinline__ void pairprep_whole(const PairPrepArgs &a, int64_t g,
         sq += xor_f<1>(sq), sr += xor_f<1>(sr), amax = fmaxf(amax, xor_f<1>(amax));
         q2 += sq;
         if (bs.qs < 0) continue;
+        if (a.qs == 3) {
+            bf16_segment<NKT>(a, bs, g, r, 0, amax, sr);
+            continue;
+        }
         const float s1 = amax / 127.0f, inv1 = amax > 0.0f ? 127.0f / amax : 0.0f;
         const float s2 = s1 / 254.0f, inv2 = amax > 0.0f ? 254.0f * 127.0f / amax : 0.0f;
         int sum1 = 0, sum2 = 0;
@@ -747,16 +793,18 @@ struct WSmem {
 };
 
 // mblocks per wave chunk: 32 vectors per chunk for int8 (fewer registers: 3 waves per SIMD, 20%
-// faster on the 10M x 1024 workload than 64 at 2 waves), half a chunk's accumulators for int8x2
-// (whose second term doubles them)
-__host__ __device__ constexpr int nvw_of(int qtm, int qs) { return qs == 2 ? (qtm >= 2 ? 1 : 2) : (qtm <= 2 ? 2 : 1); }
+// faster on the 10M x 1024 workload than 64 at 2 waves); int8x2 (a second term) and BF16 (query
+// operands twice as wide) at larger query tiles take half that
+__host__ __device__ constexpr int nvw_of(int qtm, int qs) { return qs >= 2 ? (qtm >= 2 ? 1 : 2) : (qtm <= 2 ? 2 : 1); }
 
 // candidate buffer places per row: 64 for 16-query tiles, fewer for larger tiles (LDS)
 __host__ __device__ constexpr int cand_cap(int qtm) { return 64 / qtm; }
 
-// LDS of one wave (QTM query tiles of 16 rows).
+// LDS of one wave (QTM query tiles of 16 rows): the areas' places from base into s, and the bytes.
+// s is a reference, never tested against null: a null test of a stack object's address keeps
+// the compiler from holding its fields in registers.
 template <int QTM, int QS>
-__host__ __device__ inline size_t wave_smem(int nqs, int topk, WSmem *m, uint8_t *base) {
+__host__ __device__ inline size_t wave_layout(int nqs, int topk, uint8_t *base, WSmem &s) {
     constexpr int QT = 16 * QTM, CVW = 16 * nvw_of(QTM, QS);
     size_t off = 0;
     auto take = [&](size_t bytes) {
@@ -764,7 +812,6 @@ __host__ __device__ inline size_t wave_smem(int nqs, int topk, WSmem *m, uint8_t
         off += (bytes + 15) / 16 * 16;
         return p;
     };
-    WSmem s;
     s.dt = (float *)take((size_t)QT * CVW * 4);
     s.dids = (uint32_t *)take(CVW * 4);
     s.ka = (float *)take((size_t)QT * nqs * 4);
@@ -782,14 +829,19 @@ __host__ __device__ inline size_t wave_smem(int nqs, int topk, WSmem *m, uint8_t
     s.cbd = (float *)take((size_t)QT * cap * 4);
     s.cbi = (uint32_t *)take((size_t)QT * cap * 4);
     s.cbn = (int32_t *)take((size_t)QT * 4);
-    if (m) *m = s;
     return off;
+}
+
+template <int QTM, int QS>
+__host__ __device__ inline size_t wave_smem(int nqs, int topk) {
+    WSmem unused;
+    return wave_layout<QTM, QS>(nqs, topk, nullptr, unused);
 }
 
 // LDS of a workgroup of W waves: W waves' areas and the tile index it broadcasts.
 template <int W, int QTM, int QS>
 __host__ __device__ inline size_t wg_smem(int nqs, int topk) {
-    return W * wave_smem<QTM, QS>(nqs, topk, nullptr, nullptr) + 16;
+    return W * wave_smem<QTM, QS>(nqs, topk) + 16;
 }
 
 // LDS traffic between the lanes of one wave: keep the compiler from moving accesses across
@@ -801,7 +853,7 @@ __device__ __forceinline__ void wave_sync() {
 
 // The row's sorted list (k <= 64 places, padded) back to LDS with its k-th pair; while the list
 // is not full (its k-th place still +inf) the threshold stays the query's bound.
-__device__ __forceinline__ void store_row(const WSmem m, int i, int topk, float d, uint32_t id) {
+__device__ __forceinline__ void store_row(const WSmem &m, int i, int topk, float d, uint32_t id) {
     const int lane = threadIdx.x & 63;
     if (lane < topk) {
         m.tkd[i * topk + lane] = d;
@@ -825,7 +877,7 @@ __device__ __forceinline__ void store_row(const WSmem m, int i, int topk, float 
 // bitonic merge.
 constexpr int kSeqInsert = 8;
 
-__device__ __forceinline__ void flush_row(const WSmem m, int i, int topk, int nb, int cap) {
+__device__ __forceinline__ void flush_row(const WSmem &m, int i, int topk, int nb, int cap) {
     const int lane = threadIdx.x & 63;
     float d = lane < topk ? m.tkd[i * topk + lane] : INFINITY;
     uint32_t id = lane < topk ? m.tki[i * topk + lane] : 0xFFFFFFFFu;
@@ -934,6 +986,88 @@ __device__ __forceinline__ void seg_mma(const BSeg &bs, const uint8_t *const (&r
     }
 }
 
+__device__ __forceinline__ v4f mfma_bf16(v8bf a, v8bf b, v4f c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// A lane's 16 code bytes (dims 0..15 of its slice, 4 per dword) as two bf16 operands (dims 0..7
+// and 8..15), exactly: every code is an integer below 256.  flip: 0x80808080 undoes the -128 shift
+// of 8-bit codes and of the low plane of 9+-bit codes, 0 otherwise.
+__device__ __forceinline__ void codes_bf16(v4i c, uint32_t flip, v8bf &lo, v8bf &hi) {
+    uint32_t o[8];
+#pragma unroll
+    for (int d = 0; d < 4; ++d) {
+        const uint32_t x = (uint32_t)c[d] ^ flip;
+        const float f0 = (float)(x & 0xFFu), f1 = (float)((x >> 8) & 0xFFu);
+        const float f2 = (float)((x >> 16) & 0xFFu), f3 = (float)(x >> 24);
+        // the high halves of two floats side by side: their bf16 (exact here)
+        o[2 * d] = __builtin_amdgcn_perm(__float_as_uint(f1), __float_as_uint(f0), 0x07060302u);
+        o[2 * d + 1] = __builtin_amdgcn_perm(__float_as_uint(f3), __float_as_uint(f2), 0x07060302u);
+    }
+    const uint4 l = make_uint4(o[0], o[1], o[2], o[3]), h = make_uint4(o[4], o[5], o[6], o[7]);
+    lo = __builtin_bit_cast(v8bf, l);
+    hi = __builtin_bit_cast(v8bf, h);
+}
+
+// seg_mma for BF16 queries (SAQGPU_PREC_BF16): the same code loads, each lane's 16 codes of a
+// 64-dim chunk split into two v_mfma_f32_16x16x32_bf16 (dims 0..7 and 8..15 of the lane's slice,
+// paired with the same dims of the query row, so the contraction runs over the same dimensions),
+// fp32 accumulators; 9+-bit codes as a low and a high plane.
+template <int KIND, int QTM, int NVW>
+__device__ __forceinline__ void seg_mma_bf16(const BSeg &bs, const uint8_t *const (&rec)[NVW],
+                                             const uint16_t *const (&arow)[QTM], v4f (&acc)[QTM][NVW],
+                                             v4f (&ach)[QTM][NVW], int lane) {
+    constexpr int kKB = KIND == 0 ? 4 : 2;
+    const int nk = bs.dim / 64;
+    const uint32_t flip = (KIND == 2 || bs.bits == 8) ? 0x80808080u : 0u;
+    for (int kc0 = 0; kc0 < nk; kc0 += kKB) {
+        v8bf Al[kKB][QTM], Ah[kKB][QTM];
+        v4i B[kKB][NVW], BH[kKB][NVW];
+        uint2 BN[kKB][NVW];
+#pragma unroll
+        for (int u = 0; u < kKB; ++u) {
+            const int kc = min(kc0 + u, nk - 1);
+            const bool live = kc0 + u < nk;
+#pragma unroll
+            for (int t = 0; t < QTM; ++t) {
+                const v8bf *ap = reinterpret_cast<const v8bf *>(arow[t] + bs.kq_off + kc * 64);
+                const v8bf zero = __builtin_bit_cast(v8bf, make_uint4(0u, 0u, 0u, 0u));
+                Al[u][t] = live ? ap[0] : zero;
+                Ah[u][t] = live ? ap[1] : zero;
+            }
+#pragma unroll
+            for (int j = 0; j < NVW; ++j) {
+                const uint8_t *p = rec[j] + bs.code_off + (int64_t)kc * bs.chunk_bytes;
+                if (KIND == 0) {
+                    BN[u][j] = *reinterpret_cast<const uint2 *>(p + lane * 8);
+                } else {
+                    B[u][j] = *reinterpret_cast<const v4i *>(p + lane * 16);
+                    if (KIND == 2) BH[u][j] = *reinterpret_cast<const v4i *>(p + 1024 + lane * 16);
+                }
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < kKB; ++u)
+#pragma unroll
+            for (int j = 0; j < NVW; ++j) {
+                v8bf bl, bh;
+                codes_bf16(KIND == 0 ? unpack_nibbles(BN[u][j]) : B[u][j], flip, bl, bh);
+                v8bf hl, hh;
+                if (KIND == 2) codes_bf16(BH[u][j], 0u, hl, hh);
+#pragma unroll
+                for (int t = 0; t < QTM; ++t) {
+                    acc[t][j] = mfma_bf16(Al[u][t], bl, acc[t][j]);
+                    acc[t][j] = mfma_bf16(Ah[u][t], bh, acc[t][j]);
+                    if (KIND == 2) {
+                        ach[t][j] = mfma_bf16(Al[u][t], hl, ach[t][j]);
+                        ach[t][j] = mfma_bf16(Ah[u][t], hh, ach[t][j]);
+                    }
+                }
+            }
+    }
+}
+
 template <int W, int QTM, int QS, int KR>
 __global__ __launch_bounds__(64 * W, (W * QTM >= 8 || (QTM == 4 && QS == 2)) ? 1 : 2) void bscan_kernel(BScanArgs a) {
     constexpr int QT = 16 * QTM;          // query rows of a wave
@@ -941,10 +1075,12 @@ __global__ __launch_bounds__(64 * W, (W * QTM >= 8 || (QTM == 4 && QS == 2)) ? 1
     constexpr int CVW = 16 * NVW;         // vectors per chunk
     extern __shared__ __align__(16) uint8_t smem_raw[];
     const BatchedView &bv = a.bv;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const size_t per_wave = wave_smem<QTM, QS>(bv.nqs, a.topk, nullptr, nullptr);
+    // the wave index through readfirstlane: the compiler then keeps the LDS layout's pointers in
+    // scalar registers
+    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const size_t per_wave = wave_smem<QTM, QS>(bv.nqs, a.topk);
     WSmem m;
-    wave_smem<QTM, QS>(bv.nqs, a.topk, &m, smem_raw + w * per_wave);
+    wave_layout<QTM, QS>(bv.nqs, a.topk, smem_raw + w * per_wave, m);
     int32_t *tile_s = reinterpret_cast<int32_t *>(smem_raw + W * per_wave);
     const int ntiles = *a.ntiles, sv = *a.sv;
     for (int round = 0;; ++round) {
@@ -993,11 +1129,13 @@ __global__ __launch_bounds__(64 * W, (W * QTM >= 8 || (QTM == 4 && QS == 2)) ? 1
         wave_sync();
         // A operand rows of this lane: query row t*16 + lane%16 of the wave's rows
         const int8_t *arow1[QTM], *arow2[QTM];
+        const uint16_t *arowb[QTM];  // BF16 rows (QS == 3)
 #pragma unroll
         for (int t = 0; t < QTM; ++t) {
             const int64_t g = first + min(t * 16 + lane % 16, cnt - 1);
             arow1[t] = a.qq1 + g * bv.kq + 16 * (lane / 16);
             arow2[t] = QS == 2 ? a.qq2 + g * bv.kq + 16 * (lane / 16) : nullptr;
+            arowb[t] = reinterpret_cast<const uint16_t *>(a.qq1) + g * bv.kq + 16 * (lane / 16);
         }
         // the tile's vector range [v0, v0 + sv) of the list, in mblocks (v0 and sv are multiples of 64)
         const int n = bv.clu_n[c];
@@ -1045,17 +1183,29 @@ __global__ __launch_bounds__(64 * W, (W * QTM >= 8 || (QTM == 4 && QS == 2)) ? 1
                 const BSeg bs = bv.segs[s];
                 if (bs.qs < 0) continue;
                 v4i acc1[QTM][NVW], acc2[QTM][NVW], ach1[QTM][NVW], ach2[QTM][NVW];
+                v4f facc[QTM][NVW], fach[QTM][NVW];
 #pragma unroll
                 for (int t = 0; t < QTM; ++t)
 #pragma unroll
-                    for (int j = 0; j < NVW; ++j) acc1[t][j] = acc2[t][j] = ach1[t][j] = ach2[t][j] = v4i{0, 0, 0, 0};
+                    for (int j = 0; j < NVW; ++j) {
+                        acc1[t][j] = acc2[t][j] = ach1[t][j] = ach2[t][j] = v4i{0, 0, 0, 0};
+                        facc[t][j] = fach[t][j] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+                    }
                 const bool two = bs.bits > 8;
-                if (bs.bits <= 4)
+                if (QS == 3) {
+                    if (bs.bits <= 4)
+                        seg_mma_bf16<0, QTM, NVW>(bs, rec, arowb, facc, fach, lane);
+                    else if (!two)
+                        seg_mma_bf16<1, QTM, NVW>(bs, rec, arowb, facc, fach, lane);
+                    else
+                        seg_mma_bf16<2, QTM, NVW>(bs, rec, arowb, facc, fach, lane);
+                } else if (bs.bits <= 4) {
                     seg_mma<0, QTM, QS, NVW>(bs, rec, arow1, arow2, acc1, acc2, ach1, ach2, lane);
-                else if (!two)
+                } else if (!two) {
                     seg_mma<1, QTM, QS, NVW>(bs, rec, arow1, arow2, acc1, acc2, ach1, ach2, lane);
-                else
+                } else {
                     seg_mma<2, QTM, QS, NVW>(bs, rec, arow1, arow2, acc1, acc2, ach1, ach2, lane);
+                }
                 // epilogue of the segment: ip = a1 acc1 (+ 256 a1 acch1) (+ a2 acc2 ...) + c, then
                 // dist += (-2 rescale) ip, all fused multiply-adds
 #pragma unroll
@@ -1072,8 +1222,10 @@ __global__ __launch_bounds__(64 * W, (W * QTM >= 8 || (QTM == 4 && QS == 2)) ? 1
                         const float m2r = fac[j * fac_words + kMB * (1 + bs.qs) + lane % kMB];
 #pragma unroll
                         for (int r = 0; r < 4; ++r) {
-                            float ip = fmaf(av[r], (float)acc1[t][j][r], cv[r]);
-                            if (two) ip = fmaf(256.0f * av[r], (float)ach1[t][j][r], ip);
+                            const float x1 = QS == 3 ? facc[t][j][r] : (float)acc1[t][j][r];
+                            const float h1 = QS == 3 ? fach[t][j][r] : (float)ach1[t][j][r];
+                            float ip = fmaf(av[r], x1, cv[r]);
+                            if (two) ip = fmaf(256.0f * av[r], h1, ip);
                             if (QS == 2) {
                                 ip = fmaf(bv2[r], (float)acc2[t][j][r], ip);
                                 if (two) ip = fmaf(256.0f * bv2[r], (float)ach2[t][j][r], ip);
@@ -1380,12 +1532,16 @@ static hipError_t launch_bscan_t(const BScanArgs &a, int grid, hipStream_t st) {
 size_t bscan_smem_bytes(const BatchedView &bv, int topk, int qt, int qs) {
     int W, qtm;
     tile_shape(qt, W, qtm);
-    if (qs == 2) {
-        if (qtm == 1) return (size_t)W * wave_smem<1, 2>(bv.nqs, topk, nullptr, nullptr) + 16;
-        return (size_t)W * wave_smem<2, 2>(bv.nqs, topk, nullptr, nullptr) + 16;
+    if (qs == 3) {
+        if (qtm == 1) return (size_t)W * wave_smem<1, 3>(bv.nqs, topk) + 16;
+        return (size_t)W * wave_smem<2, 3>(bv.nqs, topk) + 16;
     }
-    if (qtm == 1) return (size_t)W * wave_smem<1, 1>(bv.nqs, topk, nullptr, nullptr) + 16;
-    return (size_t)W * wave_smem<2, 1>(bv.nqs, topk, nullptr, nullptr) + 16;
+    if (qs == 2) {
+        if (qtm == 1) return (size_t)W * wave_smem<1, 2>(bv.nqs, topk) + 16;
+        return (size_t)W * wave_smem<2, 2>(bv.nqs, topk) + 16;
+    }
+    if (qtm == 1) return (size_t)W * wave_smem<1, 1>(bv.nqs, topk) + 16;
+    return (size_t)W * wave_smem<2, 1>(bv.nqs, topk) + 16;
 }
 
 template <int W, int QTM, int QS>
@@ -1441,6 +1597,14 @@ hipError_t launch_bscan_tiles(const BatchedView &bv, const BqWork &w, const BqPa
     a.topk = topk;
     a.cand_d = cand_d;
     a.cand_id = cand_id;
+    if (qs == 3) {
+        switch (qt) {
+        case 16: return launch_bscan_q<1, 1, 3>(a, grid, st);
+        case 32: return launch_bscan_q<2, 1, 3>(a, grid, st);
+        case 64: return launch_bscan_q<4, 1, 3>(a, grid, st);
+        default: return launch_bscan_q<4, 2, 3>(a, grid, st);
+        }
+    }
     switch (qt) {
     case 16: return qs == 2 ? launch_bscan_q<1, 1, 2>(a, grid, st) : launch_bscan_q<1, 1, 1>(a, grid, st);
     case 32: return qs == 2 ? launch_bscan_q<2, 1, 2>(a, grid, st) : launch_bscan_q<2, 1, 1>(a, grid, st);

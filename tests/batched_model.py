@@ -2,12 +2,13 @@
 
 The batched path computes, for every vector of every probed list, the accurate SAQ estimate
 (caq_estimator.hpp:190-217) from the full codes, with the residual query r = q_rot - c_rot
-quantised per (query, list, segment) to int8 (or two int8 terms).  This model restates that
-in numpy from independent inputs: the oracle's rotated query (orc_gemv_rowvec), the oracle
-encoder's codes and factors (orc_caq_encode on the oracle's rotated residuals) and the
-rotated centroids.  The integer dot products are exact; the fp32 epilogue takes the same terms
-in the same order as the kernel but without its fused multiply-adds (and with its own order for
-the per-segment sum of r^2), so model distances match to about 1e-6 relative, not bitwise.
+quantised per (query, list, segment) to int8 (or two int8 terms, or bfloat16).  This model
+restates that in numpy from independent inputs: the oracle's rotated query (orc_gemv_rowvec),
+the oracle encoder's codes and factors (orc_caq_encode on the oracle's rotated residuals) and
+the rotated centroids.  The integer dot products are exact; the fp32 epilogue takes the same
+terms in the same order as the kernel, with its fused multiply-adds (fma32), but with its own
+order for the sums of r and r^2, so model distances match to about 1e-6 relative, not bitwise.
+BF16 dot products are summed by the matrix cores in float32: see model_search for their bound.
 """
 import ctypes as C
 
@@ -98,15 +99,28 @@ def fma32(a, b, c):
     return (np.asarray(a, np.float64) * np.asarray(b, np.float64) + np.asarray(c, np.float64)).astype(np.float32)
 
 
-def model_search(case, tables, qrot, probes, topk, qs=1, npq=None):
+def bf16_rne(x):
+    """float32 values rounded to bfloat16 (round to nearest, ties to even), as float32."""
+    u = np.asarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return r.astype(np.uint32).view(np.float32)
+
+
+def model_search(case, tables, qrot, probes, topk, qs=1, npq=None, with_err=False):
     """Per query: the topk smallest (distance, id) pairs of the batched estimator, and the
-    distance of every probed vector (dict id -> distance)."""
+    distance of every probed vector (dict id -> distance).
+
+    qs = 3 (BF16 queries): r / amax rounded to bfloat16 times the unshifted codes, summed in
+    float32 by the matrix cores in an order of their own.  The model sums exactly (float64) and,
+    with with_err, also returns per vector a bound on the distance's deviation from it: the
+    float32 summation bound 2 n u sum |v code| (n dimensions, u = 2^-24, a factor 2 for an
+    accumulation that truncates) carried through |a1| and |2 rescale|."""
     nq = qrot.shape[0]
-    results, alld = [], []
+    results, alld, allerr = [], [], []
     offs = np.cumsum([0] + [d for d, _ in case.plan])
     for qi in range(nq):
         lim = probes.shape[1] if npq is None else int(npq[qi])
-        cand = {}
+        cand, cerr = {}, {}
         for c in probes[qi, :lim]:
             ids, segs = tables[c]
             n = len(ids)
@@ -122,6 +136,13 @@ def model_search(case, tables, qrot, probes, topk, qs=1, npq=None):
                     consts.append(None)
                     continue
                 amax = f32(np.max(np.abs(r)))
+                sqd = f32(2.0 / (1 << b))
+                sr = f32(np.sum(r.astype(np.float64)))
+                if qs == 3:
+                    inv = f32(f32(1) / amax) if amax > 0 else f32(0)
+                    v = bf16_rne((r * inv).astype(np.float32))
+                    consts.append((v.astype(np.float64), None, f32(sqd * amax), f32(0), f32(f32(-1.0 + sqd / 2) * sr)))
+                    continue
                 s1 = f32(amax / f32(127))
                 inv1 = f32(f32(127) / amax) if amax > 0 else f32(0)
                 s2 = f32(s1 / f32(254))
@@ -131,8 +152,6 @@ def model_search(case, tables, qrot, probes, topk, qs=1, npq=None):
                 if qs == 2:
                     rem = (r - (v1.astype(np.float32) * s1).astype(np.float32)).astype(np.float32)
                     v2 = np.clip(np.rint((rem * inv2).astype(np.float32)), -127, 127)
-                sqd = f32(2.0 / (1 << b))
-                sr = f32(np.sum(r.astype(np.float64)))
                 a1, a2 = f32(sqd * s1), f32(sqd * s2)
                 cst = f32(f32(-1.0 + sqd / 2) * sr)
                 if b >= 8:  # the code shift's correction, folded into the constant as pair prep does
@@ -144,11 +163,25 @@ def model_search(case, tables, qrot, probes, topk, qs=1, npq=None):
             for s in range(len(case.plan)):
                 o2 = (o2 + segs[s][1] * segs[s][1]).astype(np.float32)
             dist = (o2 + q2).astype(np.float32)
+            err = np.zeros(n, np.float64)
             for s, (ds, b) in enumerate(case.plan):
                 if b == 0:
                     continue
                 v1, v2, a1, a2, kq = consts[s]
                 codes, _, resc = segs[s]
+                if qs == 3:
+                    # unshifted codes; 9+ bits as the low byte and the high part
+                    lo = codes & 255 if b >= 9 else codes
+                    hi = codes >> 8
+                    ip = fma32(a1, (lo @ v1).astype(np.float32), kq)
+                    mag = np.abs(lo) @ np.abs(v1)
+                    if b >= 9:
+                        ip = fma32(f32(256) * a1, (hi @ v1).astype(np.float32), ip)
+                        mag = mag + 256 * (np.abs(hi) @ np.abs(v1))
+                    m2r = (f32(-2) * resc).astype(np.float32)
+                    dist = fma32(m2r, ip, dist)
+                    err += np.abs(m2r.astype(np.float64)) * abs(float(a1)) * 2 * ds * 2.0 ** -24 * mag
+                    continue
                 # the kernel's operands: codes of 8 bits shifted by -128; 9+ bits as a shifted low
                 # byte plane and a high plane
                 lo = (codes & 255) - 128 if b >= 9 else (codes - 128 if b == 8 else codes)
@@ -161,9 +194,13 @@ def model_search(case, tables, qrot, probes, topk, qs=1, npq=None):
                     if b >= 9:
                         ip = fma32(f32(256) * a2, (hi @ v2).astype(np.float32), ip)
                 dist = fma32((f32(-2) * resc).astype(np.float32), ip, dist)
-            for u, d in zip(ids.tolist(), dist.tolist()):
+            for u, d, e in zip(ids.tolist(), dist.tolist(), err.tolist()):
                 cand[u] = d
+                cerr[u] = e
         order = sorted(cand.items(), key=lambda t: (t[1], t[0]))[:topk]
         results.append(order)
         alld.append(cand)
+        allerr.append(cerr)
+    if with_err:
+        return results, alld, allerr
     return results, alld
