@@ -305,6 +305,87 @@ def single_records(torch, ix, base, queries, dev, nrec, nqs, reps=3, cpu=None):
     return res
 
 
+def precision_report(torch, saq_amd, make_plan, sdata, base, queries, cent, sizes, var, cid32, npq_np, topk, gt,
+                     dev, stream, steps, log_fn, nq_exact=1000, nq_ref=16):
+    """The same base at B=8 (10M x 1024: codes of 8 bits, stored shifted to signed int8), searched by
+    the batched scan with int8, int8x2 and bf16 queries.  Per precision: the relative error
+    |d - d*| / d* of the returned top-k distances against (a) the FP64 exact distance |q - o|^2 of
+    the same vector (nq_exact queries) and (b) the reference's accurate estimate of the same vector
+    (compAccurateDist, fp32 with the unquantised query, saqgpu_estimate; nq_ref queries at k=256),
+    which isolates the query quantisation; recall@k and the step time on the whole query batch."""
+    D = base.shape[1]
+    plan = make_plan(var, D, 8.0)
+    rots = [sdata.random_rotation(d, 2000 + i) for i, (d, _) in enumerate(plan)]
+    ix = saq_amd.Index(D, plan, cent.cpu().numpy(), sizes, var, rotations=rots)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ix.encode(base, cid32)
+    torch.cuda.synchronize()
+    t_enc = time.perf_counter() - t0
+    ix.prepare_batched()
+    nq = queries.shape[0]
+    npmax = int(npq_np.max()) if npq_np is not None else 64
+    npq_all = (torch.from_numpy(npq_np).to(dev) if npq_np is not None
+               else torch.full((nq,), npmax, dtype=torch.int32, device=dev))
+    probes = ix.probe(queries, npmax)  # one probe list for every precision
+    ids = torch.zeros(nq, topk, dtype=torch.int32, device=dev)
+    dd = torch.zeros(nq, topk, dtype=torch.float32, device=dev)
+    qe = queries[:nq_exact].double()
+    out = {"bits": 8.0, "plan": plan, "encode_mvec_per_s": round(base.shape[0] / t_enc / 1e6, 3),
+           "nprobe": "per query as the headline" if npq_np is not None else npmax,
+           "vs_fp64_exact": {}, "vs_reference_estimate": {}, "recall_at_10": {}, "ms_per_step": {}}
+    names = {saq_amd.PREC_INT8: "int8", saq_amd.PREC_INT8X2: "int8x2", saq_amd.PREC_BF16: "bf16"}
+    qn = queries[:nq_ref].cpu().numpy()
+    pr_np = probes.cpu().numpy().view(np.uint32) if hasattr(probes, "cpu") else np.asarray(probes).view(np.uint32)
+    ref_est = []
+    for i in range(nq_ref):
+        npi = int(npq_np[i]) if npq_np is not None else npmax
+        e_ids, e_acc, _, _ = ix.estimate(qn[i], npi, probes=pr_np[i, :npi])
+        ref_est.append(dict(zip(e_ids.tolist(), e_acc.tolist())))
+    for prec, name in names.items():
+        def go():
+            ix.search_batched(queries, topk, npmax, probes=probes, nprobe_per_query=npq_all, precision=prec,
+                              out_ids=ids, out_dists=dd, stream=stream)
+        go()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            go()
+        torch.cuda.synchronize()
+        out["ms_per_step"][name] = round((time.perf_counter() - t0) / steps * 1e3, 3)
+        got_i = ids.cpu().numpy()
+        out["recall_at_10"][name] = round(recall(got_i, gt, topk), 4)
+        # (a) against FP64 exact distances
+        sel = ids[:nq_exact].long()
+        exact = ((base[sel].double() - qe[:, None, :]) ** 2).sum(2)
+        rel = ((dd[:nq_exact].double() - exact).abs() / exact.clamp_min(1e-30)).cpu().numpy().ravel()
+        out["vs_fp64_exact"][name] = {"max_rel": float(rel.max()), "mean_rel": float(rel.mean()),
+                                      "pairs": int(rel.size)}
+        # (b) against the reference's accurate estimate, top-256 of nq_ref queries
+        kk = 256
+        bi = torch.zeros(nq_ref, kk, dtype=torch.int32, device=dev)
+        bdv = torch.zeros(nq_ref, kk, dtype=torch.float32, device=dev)
+        ix.search_batched(queries[:nq_ref], kk, npmax, probes=probes[:nq_ref], nprobe_per_query=npq_all[:nq_ref],
+                          precision=prec, out_ids=bi, out_dists=bdv, stream=stream)
+        torch.cuda.synchronize()
+        rr, missing = [], 0
+        for i, (row_i, row_d) in enumerate(zip(bi.cpu().numpy().view(np.uint32), bdv.cpu().numpy())):
+            for u, dv in zip(row_i.tolist(), row_d.tolist()):
+                if u not in ref_est[i]:  # not one of the probed vectors
+                    missing += 1
+                    continue
+                want = ref_est[i][u]
+                rr.append(abs(dv - want) / max(abs(want), 1e-30))
+        rr = np.asarray(rr)
+        out["vs_reference_estimate"][name] = {"max_rel": float(rr.max()), "mean_rel": float(rr.mean()),
+                                              "pairs": int(rr.size), "ids_not_probed": missing}
+        log_fn(f"precision B=8 {name}: recall {out['recall_at_10'][name]}, step {out['ms_per_step'][name]} ms, "
+               f"vs FP64 exact max {rel.max():.3e} mean {rel.mean():.3e}, vs reference estimate max "
+               f"{rr.max():.3e} mean {rr.mean():.3e}")
+    ix.close()
+    return out
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -385,6 +466,8 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=3.0, help="duration of each serving-loop load")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving loop (kernel profiling runs)")
     ap.add_argument("--no-single", action="store_true", help="skip the single-vector record measurement")
+    ap.add_argument("--no-precision", action="store_true",
+                    help="skip the B=8 index and its int8 / int8x2 / bf16 distance errors")
     ap.add_argument("--exact-probe", action="store_true",
                     help="batched scan with the reference-order coarse quantiser instead of the fp32-MFMA one")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -649,6 +732,10 @@ def main():
     serving = [] if args.no_serve else serve(torch, ix, queries, loads, args.serve_seconds, 5, bprec,
                                              args.exact_probe, log)
 
+    precision = None if args.no_precision else precision_report(
+        torch, saq_amd, make_plan, sdata, base, queries, cent, sizes, var, cid32, npq_np, args.topk, gt, dev,
+        stream, steps, log)
+
     cpu = None
     if not args.no_cpu:
         # the host cores this process may use: the box's CPU share per GPU (OMP_NUM_THREADS, set
@@ -694,6 +781,7 @@ def main():
                                        f"{nlist} Zipf-sized lists" if W["npq_range"] else "")),
                    "serving": serving, "p99_ms": serving[-1]["p99_ms"] if serving else None,
                    "encode_mvec_per_s": round(n / t_enc / 1e6, 3), "single_vector": single,
+                   "precision_b8": precision,
                    "estimate": est},
         "roofline": {"bound": "mfma", "achieved": round(b_tops, 2), "peak": INT8_PEAK_TOPS, "unit": "TOPS",
                      "frac": round(b_tops / INT8_PEAK_TOPS, 5), "traffic": btraffic,
