@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libsaqgpu.so")
-SOURCES = ["api.cpp", "plan.cpp", "query_prep.hip", "probe.hip", "encode.hip", "batched.hip", "estimate.hip", "slice_scan.hip", "single.hip"]
+SOURCES = ["api.cpp", "plan.cpp", "query_prep.hip", "probe.hip", "encode.hip", "slots.hip", "batched.hip", "estimate.hip", "slice_scan.hip", "single.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",

@@ -41,6 +41,11 @@ hipError_t launch_encode_segment(const IndexView &ix, const float *data, const u
                                  const int64_t *gblk, const int32_t *lane, const uint32_t *clu, const float *cent,
                                  const float *rot, int seg, int seg_dim, int64_t nslots, int64_t ld, int rd_lmt,
                                  float eps, float *O, uint16_t *code, int *fail, hipStream_t st);
+size_t slot_sort_bytes(int64_t n);
+hipError_t launch_make_slots(const uint32_t *cid, int64_t n, int num_cen, const int64_t *clu_vec,
+                             const int64_t *clu_blk, const int32_t *clu_n, uint32_t *iota, uint32_t *src,
+                             uint32_t *clu, int64_t *gvec, int64_t *gblk, int32_t *lane, void *temp,
+                             size_t temp_bytes, int *bad, hipStream_t st);
 hipError_t launch_caq_only(const IndexView &ix, const int64_t *gvec, const int64_t *gblk, const int32_t *lane,
                            const uint32_t *clu, const float *cent, int seg, int64_t nslots, int64_t ld, int rd_lmt,
                            float eps, float *O, uint16_t *code, int *fail, hipStream_t st);
@@ -85,6 +90,10 @@ template <class T>
 struct DevBuf {
     T *p = nullptr;
     size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }  // a function's scratch buffers go when it returns
     int alloc(size_t cnt) {
         if (p) (void)hipFree(p);
         p = nullptr;
@@ -1105,40 +1114,6 @@ int saqgpu_estimate(saqgpu_index *ix, const float *query, const uint32_t *probes
 }  // extern "C"
 
 namespace {
-struct Slots {
-    std::vector<uint32_t> src, clu;
-    std::vector<int64_t> gvec, gblk;
-    std::vector<int32_t> lane;
-};
-
-// IVF::construct's bucketing (ivf.hpp:149-159): members of each list in increasing id order.
-int make_slots(const saqgpu_index *ix, const uint32_t *cid, int64_t n, Slots &sl) {
-    std::vector<int64_t> cnt(ix->num_cen, 0);
-    for (int64_t i = 0; i < n; ++i) {
-        if (cid[i] >= (uint32_t)ix->num_cen) return fail(SAQGPU_EINVAL, "cluster id out of range");
-        cnt[cid[i]]++;
-    }
-    for (int c = 0; c < ix->num_cen; ++c)
-        if (cnt[c] != ix->clu_n[c]) return fail(SAQGPU_EINVAL, "cluster ids disagree with the index's cluster sizes");
-    std::vector<int64_t> rank(ix->num_cen, 0);
-    sl.src.resize(n);
-    sl.clu.resize(n);
-    sl.gvec.resize(n);
-    sl.gblk.resize(n);
-    sl.lane.resize(n);
-    for (int64_t i = 0; i < n; ++i) {
-        const uint32_t c = cid[i];
-        const int64_t r = rank[c]++;
-        const int64_t slot = ix->clu_vec[c] + r;
-        sl.src[slot] = (uint32_t)i;
-        sl.clu[slot] = c;
-        sl.gvec[slot] = slot;
-        sl.gblk[slot] = ix->clu_blk[c] + r / kBlock;
-        sl.lane[slot] = (int32_t)(r % kBlock);
-    }
-    return SAQGPU_OK;
-}
-
 }  // namespace
 
 extern "C" {
@@ -1152,25 +1127,33 @@ int saqgpu_encode(saqgpu_index *ix, const float *data, const uint32_t *cluster_i
     if (int rc_ = call.begin()) return rc_;
     hipStream_t st = (hipStream_t)stream;
     const bool dev = flags & SAQGPU_DEVICE_PTRS;
-    std::vector<uint32_t> hcid;
-    const uint32_t *cid = cluster_ids;
-    if (dev) {
-        hcid.resize(n);
-        HIPCHK(hipMemcpy(hcid.data(), cluster_ids, n * 4, hipMemcpyDeviceToHost));
-        cid = hcid.data();
-    }
-    Slots sl;
-    if (int rc = make_slots(ix, cid, n, sl)) return rc;
-    DevBuf<uint32_t> d_src, d_clu;
+    // the lists' members in increasing id order (IVF::construct's bucketing), sorted on the GPU
+    DevBuf<uint32_t> d_cid, d_iota, d_src, d_clu;
     DevBuf<int64_t> d_gvec, d_gblk;
     DevBuf<int32_t> d_lane;
+    DevBuf<uint8_t> d_sort;
+    DevBuf<int> d_fail;
+    const size_t sort_bytes = slot_sort_bytes(n);
+    const size_t n1 = std::max<int64_t>(1, n);
+    if (d_iota.ensure(n1) || d_src.ensure(n1) || d_clu.ensure(n1) || d_gvec.ensure(n1) || d_gblk.ensure(n1) ||
+        d_lane.ensure(n1) || d_sort.ensure(std::max<size_t>(1, sort_bytes)) || d_fail.ensure(2))
+        return fail(SAQGPU_ENOMEM, "allocation failed");
+    const uint32_t *cid = cluster_ids;
+    if (!dev) {
+        if (d_cid.ensure(n1)) return fail(SAQGPU_ENOMEM, "allocation failed");
+        HIPCHK(hipMemcpyAsync(d_cid.p, cluster_ids, n * 4, hipMemcpyHostToDevice, st));
+        cid = d_cid.p;
+    }
+    HIPCHK(hipMemsetAsync(d_fail.p, 0, 2 * sizeof(int), st));
+    HIPCHK(launch_make_slots(cid, n, ix->num_cen, ix->d_clu_vec.p, ix->d_clu_blk.p, ix->d_clu_n.p, d_iota.p, d_src.p,
+                             d_clu.p, d_gvec.p, d_gblk.p, d_lane.p, d_sort.p, sort_bytes, d_fail.p + 1, st));
+    int bad = 0;
+    HIPCHK(hipMemcpyAsync(&bad, d_fail.p + 1, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (bad & 1) return fail(SAQGPU_EINVAL, "cluster id out of range");
+    if (bad & 2) return fail(SAQGPU_EINVAL, "cluster ids disagree with the index's cluster sizes");
     DevBuf<float> d_data, d_O, d_bound;
     DevBuf<uint16_t> d_code;
-    DevBuf<int> d_fail;
-    int rc = 0;
-    if ((rc = upload(d_src, sl.src)) || (rc = upload(d_clu, sl.clu)) || (rc = upload(d_gvec, sl.gvec)) ||
-        (rc = upload(d_gblk, sl.gblk)) || (rc = upload(d_lane, sl.lane)))
-        return rc;
     const float *ddata = data;
     if (!dev) {
         if (d_data.ensure(std::max<int64_t>(1, n) * ix->num_dim)) return fail(SAQGPU_ENOMEM, "allocation failed");
@@ -1183,9 +1166,8 @@ int saqgpu_encode(saqgpu_index *ix, const float *data, const uint32_t *cluster_i
     HIPCHK(launch_query_prep(v, ix->centroids.p, ix->num_cen, ix->variance.p, ix->has_rot ? ix->rot.p : nullptr,
                              ix->d_rot_off.p, 0.0f, ix->crot.p, d_bound.p, ix->max_dim, st));
     const int64_t chunk = std::min<int64_t>(std::max<int64_t>(1, n), 1 << 18);
-    if (d_O.ensure(chunk * ix->max_dim) || d_code.ensure(chunk * ix->max_dim) || d_fail.ensure(1))
+    if (d_O.ensure(chunk * ix->max_dim) || d_code.ensure(chunk * ix->max_dim))
         return fail(SAQGPU_ENOMEM, "allocation failed");
-    HIPCHK(hipMemsetAsync(d_fail.p, 0, sizeof(int), st));
     for (int64_t off = 0; off < n; off += chunk) {
         const int64_t cnt = std::min(chunk, n - off);
         for (int s = 0; s < ix->nseg; ++s)
@@ -1198,8 +1180,7 @@ int saqgpu_encode(saqgpu_index *ix, const float *data, const uint32_t *cluster_i
     HIPCHK(hipMemcpyAsync(&nf, d_fail.p, sizeof(int), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     if (n_check_failed) *n_check_failed = nf;
-    std::vector<uint32_t> ids(sl.src.begin(), sl.src.end());
-    if (n) HIPCHK(hipMemcpy(ix->ids.p, ids.data(), n * 4, hipMemcpyHostToDevice));
+    if (n) HIPCHK(hipMemcpy(ix->ids.p, d_src.p, n * 4, hipMemcpyDeviceToDevice));
     std::fill(ix->clu_set.begin(), ix->clu_set.end(), 1);
     ix->bq_ready = false;
     return SAQGPU_OK;
@@ -1220,15 +1201,13 @@ int saqgpu_encode_residuals(saqgpu_index *ix, int32_t cid, int32_t seg, const fl
     std::vector<float> cen(ix->num_dim, 0.0f);
     const int doff = ix->segs[seg].doff;
     for (int j = 0; j < D && doff + j < ix->num_dim; ++j) cen[doff + j] = centroid_seg ? centroid_seg[j] : 0.0f;
-    Slots sl;
-    sl.gvec.resize(n);
-    sl.gblk.resize(n);
-    sl.lane.resize(n);
-    sl.clu.assign(n, 0);
+    std::vector<int64_t> gvec(n), gblk(n);
+    std::vector<int32_t> lane(n);
+    std::vector<uint32_t> clu(n, 0);
     for (int64_t r = 0; r < n; ++r) {
-        sl.gvec[r] = ix->clu_vec[cid] + r;
-        sl.gblk[r] = ix->clu_blk[cid] + r / kBlock;
-        sl.lane[r] = (int32_t)(r % kBlock);
+        gvec[r] = ix->clu_vec[cid] + r;
+        gblk[r] = ix->clu_blk[cid] + r / kBlock;
+        lane[r] = (int32_t)(r % kBlock);
     }
     DevBuf<int64_t> d_gvec, d_gblk;
     DevBuf<int32_t> d_lane;
@@ -1238,8 +1217,8 @@ int saqgpu_encode_residuals(saqgpu_index *ix, int32_t cid, int32_t seg, const fl
     DevBuf<int> d_fail;
     int rc = 0;
     std::vector<float> otv(ot);
-    if ((rc = upload(d_gvec, sl.gvec)) || (rc = upload(d_gblk, sl.gblk)) || (rc = upload(d_lane, sl.lane)) ||
-        (rc = upload(d_clu, sl.clu)) || (rc = upload(d_O, otv)) || (rc = upload(d_cen, cen)))
+    if ((rc = upload(d_gvec, gvec)) || (rc = upload(d_gblk, gblk)) || (rc = upload(d_lane, lane)) ||
+        (rc = upload(d_clu, clu)) || (rc = upload(d_O, otv)) || (rc = upload(d_cen, cen)))
         return rc;
     if (d_code.ensure((size_t)D * n) || d_fail.ensure(1)) return fail(SAQGPU_ENOMEM, "allocation failed");
     HIPCHK(hipMemset(d_fail.p, 0, sizeof(int)));

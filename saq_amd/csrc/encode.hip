@@ -100,6 +100,94 @@ __global__ __launch_bounds__(256) void rotate_kernel(EncArgs a) {
     }
 }
 
+// The same rotation with one lane per vector: a workgroup takes 256 vectors (4 waves, lane =
+// vector) and walks the output columns in blocks of kRotJB; for each block it steps over the
+// input dims 16 at a time, staging that 16 x kRotJB tile of P in LDS (double-buffered, one
+// barrier per step) from where every wave reads it as broadcasts.  Every column is the FMA chain
+// of Eigen's GEMV over blocks of BC input dims (BC = 0: one block of all D dims, for D < 128), the
+// blocks' chains summed in order -- the sums rotate_kernel forms, bit for bit.  Requires D a
+// multiple of 64 (segment dims are).  grid: ceil(nslots / 256).
+constexpr int kRotJB = 32;
+
+template <int BC>
+__global__ __launch_bounds__(256) void rotate_rows_kernel(EncArgs a) {
+    __shared__ __align__(16) float tile[2][16][kRotJB];
+    const IndexView &ix = a.ix;
+    const SegDesc sd = ix.segs[a.seg];
+    const int D = sd.dim;
+    const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const bool live = r < a.nslots;
+    const int64_t row = live ? (a.src ? (int64_t)a.src[r] : r) : 0;
+    const float *x_row = a.data + row * ix.num_dim + sd.doff;
+    const int nd = min(D, ix.num_dim - sd.doff);  // dims of the segment inside the data rows
+    const bool vec4 = (ix.num_dim % 4 == 0) && (sd.doff % 4 == 0);
+    const float *P = a.rot;
+    // staging: thread t moves P[i0 + t / 16][j0 + 2 (t % 16) .. +1]
+    const int si = threadIdx.x / 16, sj = 2 * (threadIdx.x % 16);
+    const int nsteps = D / 16;
+    for (int j0 = 0; j0 < D; j0 += kRotJB) {
+        float acc[kRotJB], c[kRotJB];
+#pragma unroll
+        for (int j = 0; j < kRotJB; ++j) acc[j] = c[j] = 0.0f;
+        float2 nxt = *reinterpret_cast<const float2 *>(P + (int64_t)si * D + j0 + sj);
+        __syncthreads();  // the previous column block's last reads of tile[] are done
+        *reinterpret_cast<float2 *>(&tile[0][si][sj]) = nxt;
+        __syncthreads();
+        for (int k = 0; k < nsteps; ++k) {
+            const int i0 = 16 * k;
+            if (k + 1 < nsteps) nxt = *reinterpret_cast<const float2 *>(P + (int64_t)(i0 + 16 + si) * D + j0 + sj);
+            const float(*t)[kRotJB] = tile[k & 1];
+            // 4 input dims at a time (a runtime loop: the scheduler then holds at most 4 rows of
+            // the tile in registers)
+#pragma unroll 1
+            for (int g = 0; g < 4; ++g) {
+                const int i = i0 + 4 * g;
+                float4 x4;
+                if (live && vec4 && i + 4 <= nd) {
+                    x4 = *reinterpret_cast<const float4 *>(x_row + i);
+                } else {
+                    x4.x = (live && i < nd) ? x_row[i] : 0.0f;
+                    x4.y = (live && i + 1 < nd) ? x_row[i + 1] : 0.0f;
+                    x4.z = (live && i + 2 < nd) ? x_row[i + 2] : 0.0f;
+                    x4.w = (live && i + 3 < nd) ? x_row[i + 3] : 0.0f;
+                }
+                const float xv[4] = {x4.x, x4.y, x4.z, x4.w};
+#pragma unroll
+                for (int ii = 0; ii < 4; ++ii) {
+#pragma unroll
+                    for (int j4 = 0; j4 < kRotJB; j4 += 4) {
+                        const float4 p = *reinterpret_cast<const float4 *>(&t[4 * g + ii][j4]);
+                        c[j4] = __fmaf_rn(p.x, xv[ii], c[j4]);
+                        c[j4 + 1] = __fmaf_rn(p.y, xv[ii], c[j4 + 1]);
+                        c[j4 + 2] = __fmaf_rn(p.z, xv[ii], c[j4 + 2]);
+                        c[j4 + 3] = __fmaf_rn(p.w, xv[ii], c[j4 + 3]);
+                    }
+                }
+                if (BC == 4 || (BC == 16 && g == 3)) {
+#pragma unroll
+                    for (int j = 0; j < kRotJB; ++j) {
+                        acc[j] = acc[j] + c[j];
+                        c[j] = 0.0f;
+                    }
+                }
+            }
+            if (k + 1 < nsteps) {
+                *reinterpret_cast<float2 *>(&tile[(k + 1) & 1][si][sj]) = nxt;
+                __syncthreads();
+            }
+        }
+        if (BC == 0)
+#pragma unroll
+            for (int j = 0; j < kRotJB; ++j) acc[j] = acc[j] + c[j];
+        if (!live) continue;
+#pragma unroll
+        for (int j = 0; j < kRotJB; ++j) {
+            const int d = sd.doff + j0 + j;
+            a.O[(int64_t)(j0 + j) * a.ld + r] = a.srec ? acc[j] : acc[j] - ix.crot[(int64_t)a.clu[r] * ix.D_pad + d];
+        }
+    }
+}
+
 // Eigen redux (Redux.h:227-272) of n elements (n a multiple of 64) by one thread.
 template <class F>
 __device__ __forceinline__ float redux_thread64(int n, F elem) {
@@ -265,10 +353,22 @@ __global__ __launch_bounds__(256) void caq_kernel(EncArgs a) {
                     break;
                 }
                 adj_cnt = 0;
-                for (int j = 0; j < D; ++j) {
+                // 8 dims' residuals and codes are loaded together ahead of the sequential updates
+                // (D is a multiple of 64)
+                for (int j8 = 0; j8 < D; j8 += 8) {
+                  float ov[8];
+                  int cv[8];
+#pragma unroll
+                  for (int u = 0; u < 8; ++u) {
+                      ov[u] = O(j8 + u);
+                      cv[u] = code[(int64_t)(j8 + u) * ld];
+                  }
+#pragma unroll
+                  for (int u = 0; u < 8; ++u) {
+                    const int j = j8 + u;
                     int cur = 0;
-                    const float oj = O(j);
-                    const int c0 = code[(int64_t)j * ld];
+                    const float oj = ov[u];
+                    const int c0 = cv[u];
                     double oa = (c0 + 0.5) * delta + v_mi;
                     int c = c0;
                     const double tmp = oa_l2sqr - oa * oa;
@@ -295,13 +395,24 @@ __global__ __launch_bounds__(256) void caq_kernel(EncArgs a) {
                         code[(int64_t)j * ld] = (uint16_t)c;
                         adj_cnt += cur;
                     }
+                  }
                 }
                 if (a.rd_lmt && round >= a.rd_lmt) break;
                 double chk_oa = 0, chk_ip = 0;
-                for (int j = 0; j < D; ++j) {
-                    const double q = (code[(int64_t)j * ld] + 0.5) * delta + v_mi;
-                    chk_ip += q * (double)O(j);
-                    chk_oa += q * q;
+                for (int j8 = 0; j8 < D; j8 += 8) {
+                    float ov[8];
+                    int cv[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        ov[u] = O(j8 + u);
+                        cv[u] = code[(int64_t)(j8 + u) * ld];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        const double q = (cv[u] + 0.5) * delta + v_mi;
+                        chk_ip += q * (double)ov[u];
+                        chk_oa += q * q;
+                    }
                 }
                 oa_l2sqr = chk_oa;
                 ip = chk_ip;
@@ -389,13 +500,29 @@ __global__ __launch_bounds__(256) void caq_kernel(EncArgs a) {
     compact_code(bits - 1, D, lc, [&](int d) { return (uint32_t)code[(int64_t)d * ld] & (short_bit - 1u); });
 }
 
+// Residuals of a chunk: the lane-per-vector kernel where P is given (Eigen's blocking of the
+// input dims: all of them below 128, else 16, or 4 from 8000 on), the column kernel otherwise.
+static void launch_rotate(const EncArgs &a, int seg_dim, hipStream_t st) {
+    if (a.rot && seg_dim % 64 == 0) {
+        const dim3 g((unsigned)((a.nslots + 255) / 256));
+        if (seg_dim < 128)
+            hipLaunchKernelGGL(rotate_rows_kernel<0>, g, dim3(256), 0, st, a);
+        else if (seg_dim * 4 < 32000)
+            hipLaunchKernelGGL(rotate_rows_kernel<16>, g, dim3(256), 0, st, a);
+        else
+            hipLaunchKernelGGL(rotate_rows_kernel<4>, g, dim3(256), 0, st, a);
+        return;
+    }
+    const dim3 g1((unsigned)((a.nslots + kRotRows - 1) / kRotRows), (unsigned)((seg_dim + 255) / 256));
+    hipLaunchKernelGGL(rotate_kernel, g1, dim3(256), 0, st, a);
+}
+
 hipError_t launch_encode_segment(const IndexView &ix, const float *data, const uint32_t *src, const int64_t *gvec,
                                  const int64_t *gblk, const int32_t *lane, const uint32_t *clu, const float *cent,
                                  const float *rot, int seg, int seg_dim, int64_t nslots, int64_t ld, int rd_lmt,
                                  float eps, float *O, uint16_t *code, int *fail, hipStream_t st) {
     EncArgs a{ix, data, src, gvec, gblk, lane, clu, cent, rot, seg, nslots, ld, rd_lmt, eps, O, code, fail};
-    dim3 g1((unsigned)((nslots + kRotRows - 1) / kRotRows), (unsigned)((seg_dim + 255) / 256));
-    hipLaunchKernelGGL(rotate_kernel, g1, dim3(256), 0, st, a);
+    launch_rotate(a, seg_dim, st);
     hipLaunchKernelGGL(caq_kernel, dim3((unsigned)((nslots + 255) / 256)), dim3(256), 0, st, a);
     return hipGetLastError();
 }
@@ -406,8 +533,7 @@ hipError_t launch_single_quantize_segment(const IndexView &ix, const float *data
                                           int32_t code_off, int32_t long_off, int32_t exf_off, hipStream_t st) {
     EncArgs a{ix, data, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, rot, seg, nslots, ld, rd_lmt, eps,
               O, code, fail, srec, sstride, code_off, long_off, exf_off};
-    dim3 g1((unsigned)((nslots + kRotRows - 1) / kRotRows), (unsigned)((seg_dim + 255) / 256));
-    hipLaunchKernelGGL(rotate_kernel, g1, dim3(256), 0, st, a);
+    launch_rotate(a, seg_dim, st);
     hipLaunchKernelGGL(caq_kernel, dim3((unsigned)((nslots + 255) / 256)), dim3(256), 0, st, a);
     return hipGetLastError();
 }

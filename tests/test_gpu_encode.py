@@ -115,3 +115,33 @@ def test_degenerate_vectors():
     fresh.encode(case.base, case.cluster_ids)
     compare_clusters(case, fresh)
     fresh.close()
+
+
+def test_encode_rejects_bad_cluster_ids():
+    """IVF::construct's bucketing, done on the GPU, still refuses a cluster id past the last list
+    and ids whose lists' sizes differ from the index's; valid ids from host or device memory build
+    the same index."""
+    from hip_mem import DevArray
+    from saq_amd import SaqGpuError
+    case = F.make_case(n=700, d=64, K=5, nq=2, plan=[(64, 4)], seed=43)
+    mk = lambda: F.gpu_index(F.Case(case.num_dim, case.plan, case.base, case.queries, case.centroids,  # noqa: E731
+                                    case.cluster_ids, case.variance, case.rotations))
+    ix = mk()
+    bad = case.cluster_ids.copy()
+    bad[123] = 5
+    with pytest.raises(SaqGpuError, match="out of range"):
+        ix.encode(case.base, bad)
+    moved = case.cluster_ids.copy()
+    moved[7] = (moved[7] + 1) % 5
+    with pytest.raises(SaqGpuError, match="disagree"):
+        ix.encode(case.base, moved)
+    ix.encode(case.base, case.cluster_ids)
+    compare_clusters(case, ix)
+    ix.close()
+    ix = mk()
+    xb, cb = DevArray(case.base), DevArray(case.cluster_ids)
+    ix.encode(xb, cb)
+    compare_clusters(case, ix)
+    xb.free()
+    cb.free()
+    ix.close()
