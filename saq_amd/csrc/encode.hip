@@ -100,90 +100,142 @@ __global__ __launch_bounds__(256) void rotate_kernel(EncArgs a) {
     }
 }
 
-// The same rotation with one lane per vector: a workgroup takes 256 vectors (4 waves, lane =
-// vector) and walks the output columns in blocks of kRotJB; for each block it steps over the
-// input dims 16 at a time, staging that 16 x kRotJB tile of P in LDS (double-buffered, one
-// barrier per step) from where every wave reads it as broadcasts.  Every column is the FMA chain
-// of Eigen's GEMV over blocks of BC input dims (BC = 0: one block of all D dims, for D < 128), the
-// blocks' chains summed in order -- the sums rotate_kernel forms, bit for bit.  Requires D a
-// multiple of 64 (segment dims are).  grid: ceil(nslots / 256).
-constexpr int kRotJB = 32;
+// The same rotation with lanes over vectors: a workgroup takes 256 R vectors (4 waves; lane l
+// holds vectors l, l + 256, ...) and walks the output columns in blocks of JB; for each block it
+// steps over the input dims 16 at a time.  A step stages in LDS the 16 x JB tile of P, read back
+// as broadcasts so that each P value feeds R vectors' FMAs, and the 16 dims of the workgroup's
+// vectors, loaded 64-byte row pieces at a time (lanes owning whole rows would touch 64 rows per
+// load).  Every column is the FMA chain of Eigen's GEMV over blocks of BC input dims (BC = 0: one
+// block of all D dims, for D < 128), the blocks' chains summed in order -- the sums rotate_kernel
+// forms, bit for bit.  The 16 dims of a step go G at a time through a runtime loop, which bounds
+// the P values the scheduler holds in registers to G JB.  Requires D a multiple of 64 (segment
+// dims are).  grid: ceil(nslots / (256 R)).
+// (JB 64, R 1, G 4: the fastest of JB 16..64, R 1..2, G 2..4 on 10M x 1024 vectors)
+constexpr int kRotJB = 64, kRotR = 1, kRotG = 4;
+constexpr int kRotXS = 20;  // floats per staged row: 16 dims + 4 (conflict-free float4 reads)
 
 template <int BC>
 __global__ __launch_bounds__(256) void rotate_rows_kernel(EncArgs a) {
-    __shared__ __align__(16) float tile[2][16][kRotJB];
+    constexpr int JB = kRotJB, R = kRotR, G = kRotG;
+    constexpr int NR = 256 * R;  // vectors of the workgroup
+    __shared__ __align__(16) float tile[16][JB];
+    __shared__ __align__(16) float xs[NR][kRotXS];
     const IndexView &ix = a.ix;
     const SegDesc sd = ix.segs[a.seg];
     const int D = sd.dim;
-    const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    const bool live = r < a.nslots;
-    const int64_t row = live ? (a.src ? (int64_t)a.src[r] : r) : 0;
-    const float *x_row = a.data + row * ix.num_dim + sd.doff;
     const int nd = min(D, ix.num_dim - sd.doff);  // dims of the segment inside the data rows
     const bool vec4 = (ix.num_dim % 4 == 0) && (sd.doff % 4 == 0);
-    const float *P = a.rot;
-    // staging: thread t moves P[i0 + t / 16][j0 + 2 (t % 16) .. +1]
-    const int si = threadIdx.x / 16, sj = 2 * (threadIdx.x % 16);
-    const int nsteps = D / 16;
-    for (int j0 = 0; j0 < D; j0 += kRotJB) {
-        float acc[kRotJB], c[kRotJB];
+    const int64_t v0 = (int64_t)blockIdx.x * NR;
+    // staging of the vectors' dims: thread t moves the 4 dims 4 (t % 4) .. + 3 of rows t / 4 + 64 q
+    const float *srow[4 * R];
+    bool slive[4 * R];
 #pragma unroll
-        for (int j = 0; j < kRotJB; ++j) acc[j] = c[j] = 0.0f;
-        float2 nxt = *reinterpret_cast<const float2 *>(P + (int64_t)si * D + j0 + sj);
-        __syncthreads();  // the previous column block's last reads of tile[] are done
-        *reinterpret_cast<float2 *>(&tile[0][si][sj]) = nxt;
-        __syncthreads();
+    for (int q = 0; q < 4 * R; ++q) {
+        const int64_t r = v0 + threadIdx.x / 4 + 64 * q;
+        slive[q] = r < a.nslots;
+        const int64_t row = slive[q] ? (a.src ? (int64_t)a.src[r] : r) : 0;
+        srow[q] = a.data + row * ix.num_dim + sd.doff + 4 * (threadIdx.x % 4);
+    }
+    const int sc = 4 * (threadIdx.x % 4);
+    const float *P = a.rot;
+    constexpr int kStage = (16 * JB + 255) / 256;
+    const int nsteps = D / 16;
+    for (int j0 = 0; j0 < D; j0 += JB) {
+        float acc[R][JB], c[R][JB];
+#pragma unroll
+        for (int k = 0; k < R; ++k)
+#pragma unroll
+            for (int j = 0; j < JB; ++j) acc[k][j] = c[k][j] = 0.0f;
         for (int k = 0; k < nsteps; ++k) {
             const int i0 = 16 * k;
-            if (k + 1 < nsteps) nxt = *reinterpret_cast<const float2 *>(P + (int64_t)(i0 + 16 + si) * D + j0 + sj);
-            const float(*t)[kRotJB] = tile[k & 1];
-            // 4 input dims at a time (a runtime loop: the scheduler then holds at most 4 rows of
-            // the tile in registers)
-#pragma unroll 1
-            for (int g = 0; g < 4; ++g) {
-                const int i = i0 + 4 * g;
-                float4 x4;
-                if (live && vec4 && i + 4 <= nd) {
-                    x4 = *reinterpret_cast<const float4 *>(x_row + i);
+            // this step's P tile and vector dims, loaded before the barrier that ends the previous
+            // step's reads, stored after it
+            float pn[kStage];
+#pragma unroll
+            for (int q = 0; q < kStage; ++q) {
+                const int e = threadIdx.x + 256 * q;
+                if (e < 16 * JB) pn[q] = P[(int64_t)(i0 + e / JB) * D + j0 + e % JB];
+            }
+            float4 xn[4 * R];
+#pragma unroll
+            for (int q = 0; q < 4 * R; ++q) {
+                const int d = i0 + sc;
+                if (slive[q] && vec4 && d + 4 <= nd) {
+                    xn[q] = *reinterpret_cast<const float4 *>(srow[q] + i0);
                 } else {
-                    x4.x = (live && i < nd) ? x_row[i] : 0.0f;
-                    x4.y = (live && i + 1 < nd) ? x_row[i + 1] : 0.0f;
-                    x4.z = (live && i + 2 < nd) ? x_row[i + 2] : 0.0f;
-                    x4.w = (live && i + 3 < nd) ? x_row[i + 3] : 0.0f;
-                }
-                const float xv[4] = {x4.x, x4.y, x4.z, x4.w};
-#pragma unroll
-                for (int ii = 0; ii < 4; ++ii) {
-#pragma unroll
-                    for (int j4 = 0; j4 < kRotJB; j4 += 4) {
-                        const float4 p = *reinterpret_cast<const float4 *>(&t[4 * g + ii][j4]);
-                        c[j4] = __fmaf_rn(p.x, xv[ii], c[j4]);
-                        c[j4 + 1] = __fmaf_rn(p.y, xv[ii], c[j4 + 1]);
-                        c[j4 + 2] = __fmaf_rn(p.z, xv[ii], c[j4 + 2]);
-                        c[j4 + 3] = __fmaf_rn(p.w, xv[ii], c[j4 + 3]);
-                    }
-                }
-                if (BC == 4 || (BC == 16 && g == 3)) {
-#pragma unroll
-                    for (int j = 0; j < kRotJB; ++j) {
-                        acc[j] = acc[j] + c[j];
-                        c[j] = 0.0f;
-                    }
+                    xn[q].x = (slive[q] && d < nd) ? srow[q][i0] : 0.0f;
+                    xn[q].y = (slive[q] && d + 1 < nd) ? srow[q][i0 + 1] : 0.0f;
+                    xn[q].z = (slive[q] && d + 2 < nd) ? srow[q][i0 + 2] : 0.0f;
+                    xn[q].w = (slive[q] && d + 3 < nd) ? srow[q][i0 + 3] : 0.0f;
                 }
             }
-            if (k + 1 < nsteps) {
-                *reinterpret_cast<float2 *>(&tile[(k + 1) & 1][si][sj]) = nxt;
-                __syncthreads();
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < kStage; ++q) {
+                const int e = threadIdx.x + 256 * q;
+                if (e < 16 * JB) tile[e / JB][e % JB] = pn[q];
+            }
+#pragma unroll
+            for (int q = 0; q < 4 * R; ++q)
+                *reinterpret_cast<float4 *>(&xs[threadIdx.x / 4 + 64 * q][sc]) = xn[q];
+            __syncthreads();
+#pragma unroll 1
+            for (int g = 0; g < 16 / G; ++g) {
+                float xv[R][G];
+#pragma unroll
+                for (int q = 0; q < R; ++q) {
+                    const float *xr = &xs[threadIdx.x + 256 * q][G * g];
+                    if (G == 4) {
+                        const float4 v = *reinterpret_cast<const float4 *>(xr);
+                        xv[q][0] = v.x;
+                        xv[q][1 % G] = v.y;
+                        xv[q][2 % G] = v.z;
+                        xv[q][3 % G] = v.w;
+                    } else {
+#pragma unroll
+                        for (int u = 0; u < G; ++u) xv[q][u] = xr[u];
+                    }
+                }
+#pragma unroll
+                for (int ii = 0; ii < G; ++ii) {
+#pragma unroll
+                    for (int j4 = 0; j4 < JB; j4 += 4) {
+                        const float4 p = *reinterpret_cast<const float4 *>(&tile[G * g + ii][j4]);
+#pragma unroll
+                        for (int q = 0; q < R; ++q) {
+                            c[q][j4] = __fmaf_rn(p.x, xv[q][ii], c[q][j4]);
+                            c[q][j4 + 1] = __fmaf_rn(p.y, xv[q][ii], c[q][j4 + 1]);
+                            c[q][j4 + 2] = __fmaf_rn(p.z, xv[q][ii], c[q][j4 + 2]);
+                            c[q][j4 + 3] = __fmaf_rn(p.w, xv[q][ii], c[q][j4 + 3]);
+                        }
+                    }
+                    // Eigen's block ends: every 4 dims (BC 4) or 16 (BC 16)
+                    const int idx = G * g + ii;
+                    if ((BC == 4 && idx % 4 == 3) || (BC == 16 && idx == 15)) {
+#pragma unroll
+                        for (int q = 0; q < R; ++q)
+#pragma unroll
+                            for (int j = 0; j < JB; ++j) {
+                                acc[q][j] = acc[q][j] + c[q][j];
+                                c[q][j] = 0.0f;
+                            }
+                    }
+                }
             }
         }
-        if (BC == 0)
 #pragma unroll
-            for (int j = 0; j < kRotJB; ++j) acc[j] = acc[j] + c[j];
-        if (!live) continue;
+        for (int q = 0; q < R; ++q) {
+            if (BC == 0)
 #pragma unroll
-        for (int j = 0; j < kRotJB; ++j) {
-            const int d = sd.doff + j0 + j;
-            a.O[(int64_t)(j0 + j) * a.ld + r] = a.srec ? acc[j] : acc[j] - ix.crot[(int64_t)a.clu[r] * ix.D_pad + d];
+                for (int j = 0; j < JB; ++j) acc[q][j] = acc[q][j] + c[q][j];
+            const int64_t r = v0 + threadIdx.x + 256 * q;
+            if (r >= a.nslots) continue;
+#pragma unroll
+            for (int j = 0; j < JB; ++j) {
+                const int d = sd.doff + j0 + j;
+                a.O[(int64_t)(j0 + j) * a.ld + r] =
+                    a.srec ? acc[q][j] : acc[q][j] - ix.crot[(int64_t)a.clu[r] * ix.D_pad + d];
+            }
         }
     }
 }
@@ -504,7 +556,7 @@ __global__ __launch_bounds__(256) void caq_kernel(EncArgs a) {
 // input dims: all of them below 128, else 16, or 4 from 8000 on), the column kernel otherwise.
 static void launch_rotate(const EncArgs &a, int seg_dim, hipStream_t st) {
     if (a.rot && seg_dim % 64 == 0) {
-        const dim3 g((unsigned)((a.nslots + 255) / 256));
+        const dim3 g((unsigned)((a.nslots + 256 * kRotR - 1) / (256 * kRotR)));
         if (seg_dim < 128)
             hipLaunchKernelGGL(rotate_rows_kernel<0>, g, dim3(256), 0, st, a);
         else if (seg_dim * 4 < 32000)
