@@ -77,6 +77,9 @@ typedef struct {
 typedef struct {
     int32_t caq_adj_rd_lmt; /* adjustment round limit (0 disables adjustment) */
     float caq_adj_eps;      /* adjustment epsilon */
+    int32_t caq_ori_qB;     /* 0, or: codes found at this many bits (above every segment's, at most
+                               16) and shifted down to the segment's (CAQEncoder::downUpSample,
+                               caq_encoder.hpp:144-157, 165-169, 217-219) */
 } saqgpu_caq_cfg;
 
 /* Search parameters (IVF::search arguments + SearcherConfig, ivf.hpp:91-94, config.h:75-78). */
@@ -126,6 +129,14 @@ int saqgpu_index_get_cluster(const saqgpu_index *idx, int32_t cid, float *short_
  * reference's CHECK_GE(ip, 0) in *n_check_failed (may be NULL). */
 int saqgpu_encode(saqgpu_index *idx, const float *data, const uint32_t *cluster_ids, int64_t n,
                   const saqgpu_caq_cfg *cfg, int32_t flags, int64_t *n_check_failed, void *stream);
+
+/* IVF::construct with use_1_centroid (ivf.hpp:163-176): as saqgpu_encode, but every list is
+ * quantised against the one centroid `centroid` (num_dim floats, host memory; the reference
+ * passes data.colwise().mean(), see saqgpu_column_mean_var): every list's rotated centroid
+ * becomes that centroid's.  The lists' own centroids still select the probed lists. */
+int saqgpu_encode_one_centroid(saqgpu_index *idx, const float *data, const uint32_t *cluster_ids, int64_t n,
+                               const float *centroid, const saqgpu_caq_cfg *cfg, int32_t flags,
+                               int64_t *n_check_failed, void *stream);
 
 /* Encodes rotated residuals directly (the per-vector CAQEncoder::encode_and_fac + packing,
  * caq_encoder.hpp:222-247, cluster_packer.hpp:59-79) for one segment of one cluster whose
@@ -201,6 +212,14 @@ int saqgpu_single_estimate(saqgpu_index *idx, const float *queries, int64_t nq, 
  * per-dimension variance (num_dim floats).  dims/bits receive up to SAQGPU_MAX_SEG entries. */
 int saqgpu_make_plan(const float *variance, int32_t num_dim, float avg_bits, int32_t enable_segmentation,
                      int32_t seg_eqseg, int32_t *dims, int32_t *bits, int32_t *num_seg);
+
+/* Column means and population variances of an n x d row-major host matrix, in fp32 (host code):
+ * the data.colwise().mean() of IVF::construct with use_1_centroid (ivf.hpp:163-167) and the two
+ * column-wise means of SaqDataMaker::compute_variance (saq_data.hpp:96-101), summed in Eigen
+ * 3.4's order for a row-major matrix.  The means equal the reference's release build bit for
+ * bit; its -Ofast build reassociates the variance sums, which here keep Eigen's source order.
+ * mean or var may be NULL. */
+int saqgpu_column_mean_var(const float *data, int64_t n, int32_t d, float *mean, float *var);
 
 /* ---- batched scan (no single counterpart in the reference) ----------------------------
  * Serves a batch of queries as one grouped job instead of nq independent IVF::search calls:

@@ -117,6 +117,17 @@ float ref_eigen_var_q2(const float *var, int64_t nvar, int64_t off, const float 
     return (vv.segment(off, n).array() * qv.array().square()).sum();
 }
 
+// Column means and population variances of an n x d row-major matrix as Eigen 3.4 evaluates
+// data.colwise().mean() and ((data.rowwise() - mean).array().square()).colwise().mean()
+// (the expressions of SaqDataMaker::compute_variance and of IVF::construct's one-centroid mean).
+void ref_eigen_colwise_mean_var(const float *X, int64_t n, int64_t d, float *mean, float *var) {
+    FloatRowMat m = Eigen::Map<const FloatRowMat>(X, n, d);
+    FloatVec mu = m.colwise().mean();
+    FloatVec v = ((m.rowwise() - mu).array().square()).colwise().mean();
+    std::memcpy(mean, mu.data(), d * sizeof(float));
+    std::memcpy(var, v.data(), d * sizeof(float));
+}
+
 void ref_eigen_gemv(const float *q, const float *P, int64_t D, float *out) {
     FloatVec qv = Eigen::Map<const FloatVec>(q, D);
     FloatRowMat Pm = Eigen::Map<const FloatRowMat>(P, D, D);

@@ -700,10 +700,12 @@ void orc_compact_code(int k, const uint16_t *raw, int64_t D, uint8_t *out) {
 
 int orc_caq_encode(const float *o, int64_t D, int bits, const orc_caq_cfg_t *cfg, const float *centroid,
                    uint16_t *code, orc_code_info_t *info) {
-    // CAQEncoder::encode + code_adjustment + encode_and_fac (caq/caq_encoder.hpp:57-142, 172-247)
+    // CAQEncoder::encode + code_adjustment + encode_and_fac (caq/caq_encoder.hpp:57-142, 172-247);
+    // with caq_ori_qB the codes are found at ori_qb bits (the code maximum, :163-168) and shifted
+    // down to `bits` by downUpSample (:144-157)
     std::memset(info, 0, sizeof(*info));
     double v_mx = 0, v_mi = 0, delta = 0, ip = 0, oa_l2sqr = 0;
-    const int code_max = (1 << bits) - 1;
+    const int code_max = (1 << (cfg->ori_qb ? cfg->ori_qb : bits)) - 1;
     if (bits > 0) {
         float mx = o[0], mn = o[0];
         for (int64_t j = 1; j < D; ++j) {
@@ -783,6 +785,18 @@ int orc_caq_encode(const float *o, int64_t D, int bits, const orc_caq_cfg_t *cfg
                 ip = chk_ip;
             }
             info->rounds = round;
+        }
+        if (cfg->ori_qb) {  // downUpSample, :144-157
+            const int shift = cfg->ori_qb - bits;
+            delta *= 1 << shift;
+            ip = 0;
+            oa_l2sqr = 0;
+            for (int64_t j = 0; j < D; ++j) {
+                code[j] = (uint16_t)(code[j] >> shift);
+                const double q = (code[j] + 0.5) * delta + v_mi;
+                ip += q * o[j];
+                oa_l2sqr += q * q;
+            }
         }
     }
     const double o_l2sqr = orc_eigen_sqnorm(o, D);

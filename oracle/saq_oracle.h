@@ -95,6 +95,7 @@ typedef struct {
     int32_t rd_lmt;   /* caq_adj_rd_lmt */
     float eps;        /* caq_adj_eps */
     int32_t rotate;   /* random_rotation: P supplied */
+    int32_t ori_qb;   /* caq_ori_qB: 0, or encode at this many bits and shift down */
 } orc_caq_cfg_t;
 
 typedef struct {

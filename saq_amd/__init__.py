@@ -38,7 +38,7 @@ class ClusterSizes(C.Structure):
 
 
 class CaqCfg(C.Structure):
-    _fields_ = [("caq_adj_rd_lmt", C.c_int32), ("caq_adj_eps", C.c_float)]
+    _fields_ = [("caq_adj_rd_lmt", C.c_int32), ("caq_adj_eps", C.c_float), ("caq_ori_qB", C.c_int32)]
 
 
 class SearchParams(C.Structure):
@@ -92,6 +92,9 @@ _SIGS = {
     "saqgpu_timing": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
     "saqgpu_make_plan": (C.c_int, [C.c_void_p, C.c_int32, C.c_float, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
                                    C.c_void_p]),
+    "saqgpu_column_mean_var": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_void_p, C.c_void_p]),
+    "saqgpu_encode_one_centroid": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p,
+                                             C.POINTER(CaqCfg), C.c_int32, C.POINTER(C.c_int64), C.c_void_p]),
 }
 
 
@@ -112,6 +115,16 @@ def lib():
 
 def exported_symbols():
     return list(_SIGS)
+
+
+def column_mean_var(data):
+    """Column means and variances of a host n x d float32 matrix in the reference's Eigen order
+    (saqgpu_column_mean_var)."""
+    data = np.ascontiguousarray(data, np.float32)
+    n, d = data.shape
+    mean, var = np.zeros(d, np.float32), np.zeros(d, np.float32)
+    _check(lib().saqgpu_column_mean_var(data.ctypes.data, n, d, mean.ctypes.data, var.ctypes.data))
+    return mean, var
 
 
 def _check(rc):
@@ -199,17 +212,25 @@ class Index:
         out["ids"] = out["ids"][:n]
         return out
 
-    def encode(self, data, cluster_ids, rd_lmt=6, eps=1e-8, stream=None):
-        """IVF::construct's quantisation on the GPU (data: host ndarray or device tensor)."""
-        cfg = CaqCfg(rd_lmt, eps)
+    def encode(self, data, cluster_ids, rd_lmt=6, eps=1e-8, stream=None, ori_qb=0, centroid=None):
+        """IVF::construct's quantisation on the GPU (data: host ndarray or device tensor).
+        centroid: one centroid (num_dim floats, host) that every list is quantised against
+        (IVF::construct's use_1_centroid), else each list's own."""
+        cfg = CaqCfg(rd_lmt, eps, ori_qb)
         nfail = C.c_int64(0)
         dev = not isinstance(data, np.ndarray)
-        _check(lib().saqgpu_encode(self.h, _ptr(data), _ptr(cluster_ids), int(data.shape[0]), C.byref(cfg),
-                                   DEVICE_PTRS if dev else 0, C.byref(nfail), stream))
+        if centroid is None:
+            _check(lib().saqgpu_encode(self.h, _ptr(data), _ptr(cluster_ids), int(data.shape[0]), C.byref(cfg),
+                                       DEVICE_PTRS if dev else 0, C.byref(nfail), stream))
+        else:
+            cen = np.ascontiguousarray(centroid, np.float32).ravel()
+            _check(lib().saqgpu_encode_one_centroid(self.h, _ptr(data), _ptr(cluster_ids), int(data.shape[0]),
+                                                    _ptr(cen), C.byref(cfg), DEVICE_PTRS if dev else 0,
+                                                    C.byref(nfail), stream))
         return int(nfail.value)
 
-    def encode_residuals(self, cid, seg, o, centroid_seg, rd_lmt=6, eps=1e-8):
-        cfg = CaqCfg(rd_lmt, eps)
+    def encode_residuals(self, cid, seg, o, centroid_seg, rd_lmt=6, eps=1e-8, ori_qb=0):
+        cfg = CaqCfg(rd_lmt, eps, ori_qb)
         _check(lib().saqgpu_encode_residuals(self.h, cid, seg, _ptr(np.ascontiguousarray(o, np.float32)),
                                              _ptr(np.ascontiguousarray(centroid_seg, np.float32)), C.byref(cfg)))
 
@@ -330,7 +351,7 @@ class Index:
         return (int(nb.value), int(off[0]), int(off[1]), int(off[2]),
                 [(int(off[3 + 2 * s]), int(off[4 + 2 * s])) for s in range(len(self.plan))])
 
-    def single_quantize(self, data, rd_lmt=6, eps=1e-8, stride=None, out=None, stream=None):
+    def single_quantize(self, data, rd_lmt=6, eps=1e-8, stride=None, out=None, stream=None, ori_qb=0):
         """SAQuantizerSingle::quantize of every row of data into SaqSingleDataWrapper records
         `stride` bytes apart (default: the record size rounded up to 64).  Host ndarray in, uint8
         ndarray out; or device tensors with `out` given.  Returns (records, stride, n_check_failed)."""
@@ -341,7 +362,7 @@ class Index:
         if not dev:
             data = np.ascontiguousarray(data, np.float32)
             out = np.zeros(max(1, n) * stride, np.uint8) if out is None else out
-        cfg = CaqCfg(rd_lmt, eps)
+        cfg = CaqCfg(rd_lmt, eps, ori_qb)
         nfail = C.c_int64(0)
         _check(lib().saqgpu_single_quantize(self.h, _ptr(data), n, C.byref(cfg), _ptr(out), stride,
                                             DEVICE_PTRS if dev else 0, C.byref(nfail), stream))

@@ -56,7 +56,7 @@ def build_host_tools(verbose=False):
     inc = os.path.join(os.path.dirname(HERE), "include")
     for src in ("ivf_run.cpp",):
         out = os.path.join(host, src[:-4])
-        cmd = [os.environ.get("CXX", "g++"), "-std=c++17", "-O2", "-Wall", "-I" + inc, "-I" + host,
+        cmd = [os.environ.get("CXX", "g++"), "-std=c++17", "-O2", "-ffp-contract=off", "-Wall", "-I" + inc, "-I" + host,
                os.path.join(host, src), "-o", out + ".tmp", "-L" + HERE, "-lsaqgpu", "-Wl,-rpath,$ORIGIN/.."]
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:

@@ -40,7 +40,8 @@ hipError_t launch_probe_exact(float *ip, const float *cnorm, const float *centro
 hipError_t launch_encode_segment(const IndexView &ix, const float *data, const uint32_t *src, const int64_t *gvec,
                                  const int64_t *gblk, const int32_t *lane, const uint32_t *clu, const float *cent,
                                  const float *rot, int seg, int seg_dim, int64_t nslots, int64_t ld, int rd_lmt,
-                                 float eps, float *O, uint16_t *code, int *fail, hipStream_t st);
+                                 float eps, int ori_qb, bool shared_cent, float *O, uint16_t *code, int *fail,
+                                 hipStream_t st);
 size_t slot_sort_bytes(int64_t n);
 hipError_t launch_make_slots(const uint32_t *cid, int64_t n, int num_cen, const int64_t *clu_vec,
                              const int64_t *clu_blk, const int32_t *clu_n, uint32_t *iota, uint32_t *src,
@@ -48,10 +49,10 @@ hipError_t launch_make_slots(const uint32_t *cid, int64_t n, int num_cen, const 
                              size_t temp_bytes, int *bad, hipStream_t st);
 hipError_t launch_caq_only(const IndexView &ix, const int64_t *gvec, const int64_t *gblk, const int32_t *lane,
                            const uint32_t *clu, const float *cent, int seg, int64_t nslots, int64_t ld, int rd_lmt,
-                           float eps, float *O, uint16_t *code, int *fail, hipStream_t st);
+                           float eps, int ori_qb, float *O, uint16_t *code, int *fail, hipStream_t st);
 hipError_t launch_single_quantize_segment(const IndexView &ix, const float *data, const float *rot, int seg,
-                                          int seg_dim, int64_t nslots, int64_t ld, int rd_lmt, float eps, float *O,
-                                          uint16_t *code, int *fail, uint8_t *srec, int64_t sstride,
+                                          int seg_dim, int64_t nslots, int64_t ld, int rd_lmt, float eps, int ori_qb,
+                                          float *O, uint16_t *code, int *fail, uint8_t *srec, int64_t sstride,
                                           int32_t code_off, int32_t long_off, int32_t exf_off, hipStream_t st);
 struct SingleQ;
 size_t single_query_state_bytes();
@@ -1114,15 +1115,29 @@ int saqgpu_estimate(saqgpu_index *ix, const float *query, const uint32_t *probes
 }  // extern "C"
 
 namespace {
+// QuantSingleConfig checks of the encoder (CAQEncoder's constructor, caq_encoder.hpp:163-169: the
+// original bit count exceeds every segment's; codes are kept in 16 bits here).
+int check_caq_cfg(const saqgpu_index *ix, const saqgpu_caq_cfg *cfg) {
+    if (cfg->caq_adj_rd_lmt < 0) return fail(SAQGPU_EINVAL, "caq_adj_rd_lmt must be >= 0");
+    if (cfg->caq_ori_qB) {
+        int maxb = 0;
+        for (int b : ix->seg_bits) maxb = std::max(maxb, b);
+        if (cfg->caq_ori_qB <= maxb || cfg->caq_ori_qB > 16)
+            return fail(SAQGPU_EINVAL, "caq_ori_qB must exceed every segment's bits and be at most 16");
+    }
+    return SAQGPU_OK;
+}
+
 }  // namespace
 
 extern "C" {
 
-int saqgpu_encode(saqgpu_index *ix, const float *data, const uint32_t *cluster_ids, int64_t n,
-                  const saqgpu_caq_cfg *cfg, int32_t flags, int64_t *n_check_failed, void *stream) {
+static int encode_impl(saqgpu_index *ix, const float *data, const uint32_t *cluster_ids, int64_t n,
+                       const float *one_centroid, const saqgpu_caq_cfg *cfg, int32_t flags, int64_t *n_check_failed,
+                       void *stream) {
     if (!ix || !data || !cluster_ids || !cfg) return fail(SAQGPU_EINVAL, "null argument");
     if (n != ix->num_data) return fail(SAQGPU_EINVAL, "n must equal the index's num_data");
-    if (cfg->caq_adj_rd_lmt < 0) return fail(SAQGPU_EINVAL, "caq_adj_rd_lmt must be >= 0");
+    if (int rc_ = check_caq_cfg(ix, cfg)) return rc_;
     IndexCall call(ix, (hipStream_t)stream);
     if (int rc_ = call.begin()) return rc_;
     hipStream_t st = (hipStream_t)stream;
@@ -1161,10 +1176,26 @@ int saqgpu_encode(saqgpu_index *ix, const float *data, const uint32_t *cluster_i
         ddata = d_data.p;
     }
     const IndexView v = ix->view();
-    // rotated centroids c_s * P_s of every list (quantizer.hpp:46), Eigen GEMV order
+    // rotated centroids c_s * P_s of every list (quantizer.hpp:46), Eigen GEMV order; with one
+    // shared centroid, its rotation in every list's row
     if (d_bound.ensure((size_t)ix->num_cen * ix->nseg)) return fail(SAQGPU_ENOMEM, "allocation failed");
-    HIPCHK(launch_query_prep(v, ix->centroids.p, ix->num_cen, ix->variance.p, ix->has_rot ? ix->rot.p : nullptr,
-                             ix->d_rot_off.p, 0.0f, ix->crot.p, d_bound.p, ix->max_dim, st));
+    DevBuf<float> d_cen1;
+    const float *cent = ix->centroids.p;
+    if (one_centroid) {
+        if (d_cen1.ensure(ix->num_dim)) return fail(SAQGPU_ENOMEM, "allocation failed");
+        HIPCHK(hipMemcpyAsync(d_cen1.p, one_centroid, ix->num_dim * 4, hipMemcpyHostToDevice, st));
+        cent = d_cen1.p;
+        HIPCHK(launch_query_prep(v, cent, 1, ix->variance.p, ix->has_rot ? ix->rot.p : nullptr, ix->d_rot_off.p, 0.0f,
+                                 ix->crot.p, d_bound.p, ix->max_dim, st));
+        for (int64_t have = 1; have < ix->num_cen; have *= 2) {
+            const int64_t cp = std::min<int64_t>(have, ix->num_cen - have);
+            HIPCHK(hipMemcpyAsync(ix->crot.p + have * ix->D_pad, ix->crot.p, cp * ix->D_pad * 4,
+                                  hipMemcpyDeviceToDevice, st));
+        }
+    } else {
+        HIPCHK(launch_query_prep(v, cent, ix->num_cen, ix->variance.p, ix->has_rot ? ix->rot.p : nullptr,
+                                 ix->d_rot_off.p, 0.0f, ix->crot.p, d_bound.p, ix->max_dim, st));
+    }
     const int64_t chunk = std::min<int64_t>(std::max<int64_t>(1, n), 1 << 18);
     if (d_O.ensure(chunk * ix->max_dim) || d_code.ensure(chunk * ix->max_dim))
         return fail(SAQGPU_ENOMEM, "allocation failed");
@@ -1172,9 +1203,10 @@ int saqgpu_encode(saqgpu_index *ix, const float *data, const uint32_t *cluster_i
         const int64_t cnt = std::min(chunk, n - off);
         for (int s = 0; s < ix->nseg; ++s)
             HIPCHK(launch_encode_segment(v, ddata, d_src.p + off, d_gvec.p + off, d_gblk.p + off, d_lane.p + off,
-                                         d_clu.p + off, ix->centroids.p,
+                                         d_clu.p + off, cent,
                                          ix->has_rot ? ix->rot.p + ix->rot_off[s] : nullptr, s, ix->segs[s].dim, cnt,
-                                         chunk, cfg->caq_adj_rd_lmt, cfg->caq_adj_eps, d_O.p, d_code.p, d_fail.p, st));
+                                         chunk, cfg->caq_adj_rd_lmt, cfg->caq_adj_eps, cfg->caq_ori_qB,
+                                         one_centroid != nullptr, d_O.p, d_code.p, d_fail.p, st));
     }
     int nf = 0;
     HIPCHK(hipMemcpyAsync(&nf, d_fail.p, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -1186,10 +1218,23 @@ int saqgpu_encode(saqgpu_index *ix, const float *data, const uint32_t *cluster_i
     return SAQGPU_OK;
 }
 
+int saqgpu_encode(saqgpu_index *ix, const float *data, const uint32_t *cluster_ids, int64_t n,
+                  const saqgpu_caq_cfg *cfg, int32_t flags, int64_t *n_check_failed, void *stream) {
+    return encode_impl(ix, data, cluster_ids, n, nullptr, cfg, flags, n_check_failed, stream);
+}
+
+int saqgpu_encode_one_centroid(saqgpu_index *ix, const float *data, const uint32_t *cluster_ids, int64_t n,
+                               const float *centroid, const saqgpu_caq_cfg *cfg, int32_t flags,
+                               int64_t *n_check_failed, void *stream) {
+    if (!centroid) return fail(SAQGPU_EINVAL, "null centroid");
+    return encode_impl(ix, data, cluster_ids, n, centroid, cfg, flags, n_check_failed, stream);
+}
+
 int saqgpu_encode_residuals(saqgpu_index *ix, int32_t cid, int32_t seg, const float *o, const float *centroid_seg,
                             const saqgpu_caq_cfg *cfg) {
     if (!ix || !o || !cfg || cid < 0 || cid >= ix->num_cen || seg < 0 || seg >= ix->nseg)
         return fail(SAQGPU_EINVAL, "bad argument");
+    if (int rc_ = check_caq_cfg(ix, cfg)) return rc_;
     IndexCall call(ix, nullptr);
     if (int rc_ = call.begin()) return rc_;
     const int64_t n = ix->clu_n[cid];
@@ -1223,7 +1268,7 @@ int saqgpu_encode_residuals(saqgpu_index *ix, int32_t cid, int32_t seg, const fl
     if (d_code.ensure((size_t)D * n) || d_fail.ensure(1)) return fail(SAQGPU_ENOMEM, "allocation failed");
     HIPCHK(hipMemset(d_fail.p, 0, sizeof(int)));
     HIPCHK(launch_caq_only(ix->view(), d_gvec.p, d_gblk.p, d_lane.p, d_clu.p, d_cen.p, seg, n, n,
-                           cfg->caq_adj_rd_lmt, cfg->caq_adj_eps, d_O.p, d_code.p, d_fail.p, 0));
+                           cfg->caq_adj_rd_lmt, cfg->caq_adj_eps, cfg->caq_ori_qB, d_O.p, d_code.p, d_fail.p, 0));
     HIPCHK(hipDeviceSynchronize());
     ix->bq_ready = false;
     return SAQGPU_OK;
@@ -1665,7 +1710,7 @@ int saqgpu_single_quantize(saqgpu_index *ix, const float *data, int64_t n, const
                            uint8_t *records, int64_t record_stride, int32_t flags, int64_t *n_check_failed,
                            void *stream) {
     if (!ix || !cfg || n < 0 || (n > 0 && (!data || !records))) return fail(SAQGPU_EINVAL, "bad argument");
-    if (cfg->caq_adj_rd_lmt < 0) return fail(SAQGPU_EINVAL, "caq_adj_rd_lmt must be >= 0");
+    if (int rc_ = check_caq_cfg(ix, cfg)) return rc_;
     const SingleLayout L = single_layout(ix);
     if (int rc = check_stride(L, record_stride)) return rc;
     if (n_check_failed) *n_check_failed = 0;
@@ -1699,7 +1744,7 @@ int saqgpu_single_quantize(saqgpu_index *ix, const float *data, int64_t n, const
         for (int s = 0; s < ix->nseg; ++s)
             HIPCHK(launch_single_quantize_segment(
                 v, ddata + off * ix->num_dim, ix->has_rot ? ix->rot.p + ix->rot_off[s] : nullptr, s, ix->segs[s].dim,
-                cnt, chunk, cfg->caq_adj_rd_lmt, cfg->caq_adj_eps, d_O.p, d_code.p, d_fail.p,
+                cnt, chunk, cfg->caq_adj_rd_lmt, cfg->caq_adj_eps, cfg->caq_ori_qB, d_O.p, d_code.p, d_fail.p,
                 drec + off * record_stride, record_stride, (int32_t)L.seg_code[s], (int32_t)L.seg_long[s],
                 (int32_t)L.exf_off, st));
     }

@@ -35,6 +35,7 @@ struct EncArgs {
     int64_t ld;            // leading dimension (slots) of the transposed scratch
     int32_t rd_lmt;
     float eps;
+    int32_t ori_qb;        // caq_ori_qB: 0, or the bits the codes are found at before the shift down
     float *O;              // [D_s][ld]
     uint16_t *code;        // [D_s][ld]
     int *fail;
@@ -43,6 +44,7 @@ struct EncArgs {
     uint8_t *srec;
     int64_t sstride;
     int32_t s_code_off, s_long_off, s_exf_off;
+    int64_t cent_stride = -1;  // floats between lists' centroids in cent (0: one shared centroid)
 };
 
 constexpr int kRotRows = 8;
@@ -365,7 +367,7 @@ __global__ __launch_bounds__(256) void caq_kernel(EncArgs a) {
     auto O = [&](int j) { return o[(int64_t)j * ld]; };
 
     double v_mx = 0, v_mi = 0, delta = 0, ip = 0, oa_l2sqr = 0;
-    const int cm = (1 << bits) - 1;
+    const int cm = (1 << (a.ori_qb ? a.ori_qb : bits)) - 1;
     if (bits > 0) {
         float mx = O(0), mn = O(0);
         for (int j = 1; j < D; ++j) {
@@ -470,6 +472,19 @@ __global__ __launch_bounds__(256) void caq_kernel(EncArgs a) {
                 ip = chk_ip;
             }
         }
+        if (a.ori_qb) {  // CAQEncoder::downUpSample (caq_encoder.hpp:144-157)
+            const int shift = a.ori_qb - bits;
+            delta *= 1 << shift;
+            ip = 0;
+            oa_l2sqr = 0;
+            for (int j = 0; j < D; ++j) {
+                const int c = code[(int64_t)j * ld] >> shift;
+                code[(int64_t)j * ld] = (uint16_t)c;
+                const double q = (c + 0.5) * delta + v_mi;
+                ip += q * (double)O(j);
+                oa_l2sqr += q * q;
+            }
+        }
     }
     // encode_and_fac (caq_encoder.hpp:222-247)
     const double o_l2sqr = (double)redux_thread64(D, [&](int j) {
@@ -523,7 +538,7 @@ __global__ __launch_bounds__(256) void caq_kernel(EncArgs a) {
     if (bits == 0) return;
     // ip_cent_oa = centroid . get_oa() with the unrotated segment centroid (caq_encoder.hpp:42-45, 241)
     const float fd = (float)delta, fm = (float)v_mi;
-    const float *cen = a.cent + (int64_t)a.clu[t] * ix.num_dim;
+    const float *cen = a.cent + (int64_t)a.clu[t] * (a.cent_stride < 0 ? ix.num_dim : a.cent_stride);
     const float ipc = redux_thread64(D, [&](int j) {
         const int d = sd.doff + j;
         const float cj = d < ix.num_dim ? cen[d] : 0.0f;
@@ -572,19 +587,21 @@ static void launch_rotate(const EncArgs &a, int seg_dim, hipStream_t st) {
 hipError_t launch_encode_segment(const IndexView &ix, const float *data, const uint32_t *src, const int64_t *gvec,
                                  const int64_t *gblk, const int32_t *lane, const uint32_t *clu, const float *cent,
                                  const float *rot, int seg, int seg_dim, int64_t nslots, int64_t ld, int rd_lmt,
-                                 float eps, float *O, uint16_t *code, int *fail, hipStream_t st) {
-    EncArgs a{ix, data, src, gvec, gblk, lane, clu, cent, rot, seg, nslots, ld, rd_lmt, eps, O, code, fail};
+                                 float eps, int ori_qb, bool shared_cent, float *O, uint16_t *code, int *fail,
+                                 hipStream_t st) {
+    EncArgs a{ix, data, src, gvec, gblk, lane, clu, cent, rot, seg, nslots, ld, rd_lmt, eps, ori_qb, O, code, fail};
+    a.cent_stride = shared_cent ? 0 : -1;
     launch_rotate(a, seg_dim, st);
     hipLaunchKernelGGL(caq_kernel, dim3((unsigned)((nslots + 255) / 256)), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 
 hipError_t launch_single_quantize_segment(const IndexView &ix, const float *data, const float *rot, int seg,
-                                          int seg_dim, int64_t nslots, int64_t ld, int rd_lmt, float eps, float *O,
-                                          uint16_t *code, int *fail, uint8_t *srec, int64_t sstride,
+                                          int seg_dim, int64_t nslots, int64_t ld, int rd_lmt, float eps, int ori_qb,
+                                          float *O, uint16_t *code, int *fail, uint8_t *srec, int64_t sstride,
                                           int32_t code_off, int32_t long_off, int32_t exf_off, hipStream_t st) {
     EncArgs a{ix, data, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, rot, seg, nslots, ld, rd_lmt, eps,
-              O, code, fail, srec, sstride, code_off, long_off, exf_off};
+              ori_qb, O, code, fail, srec, sstride, code_off, long_off, exf_off};
     launch_rotate(a, seg_dim, st);
     hipLaunchKernelGGL(caq_kernel, dim3((unsigned)((nslots + 255) / 256)), dim3(256), 0, st, a);
     return hipGetLastError();
@@ -592,8 +609,9 @@ hipError_t launch_single_quantize_segment(const IndexView &ix, const float *data
 
 hipError_t launch_caq_only(const IndexView &ix, const int64_t *gvec, const int64_t *gblk, const int32_t *lane,
                            const uint32_t *clu, const float *cent, int seg, int64_t nslots, int64_t ld, int rd_lmt,
-                           float eps, float *O, uint16_t *code, int *fail, hipStream_t st) {
-    EncArgs a{ix, nullptr, nullptr, gvec, gblk, lane, clu, cent, nullptr, seg, nslots, ld, rd_lmt, eps, O, code, fail};
+                           float eps, int ori_qb, float *O, uint16_t *code, int *fail, hipStream_t st) {
+    EncArgs a{ix, nullptr, nullptr, gvec, gblk, lane, clu, cent, nullptr, seg, nslots, ld, rd_lmt, eps, ori_qb, O, code,
+              fail};
     hipLaunchKernelGGL(caq_kernel, dim3((unsigned)((nslots + 255) / 256)), dim3(256), 0, st, a);
     return hipGetLastError();
 }

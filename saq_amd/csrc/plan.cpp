@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <limits>
+#include <thread>
 #include <utility>
 #include <vector>
 
@@ -127,5 +128,55 @@ extern "C" int saqgpu_make_plan(const float *variance, int32_t num_dim, float av
         n = equal_plan(D_pad, avg_bits, 1, dims, bits);
     if (n < 1 || n > SAQGPU_MAX_SEG) return SAQGPU_EUNSUPPORTED;
     *num_seg = n;
+    return SAQGPU_OK;
+}
+
+namespace {
+
+// One column's sum in the order Eigen 3.4 reduces a row-major matrix column-wise: columns inside
+// the 16-float packets take packetwise_redux_impl's order (PartialReduxEvaluator.h:100-121: the
+// first row, then groups of 4 rows added as (r1 + r2) + (r3 + r4), then the rows left over);
+// the columns after the last whole packet take the scalar redux, row after row.
+template <class F>
+float column_sum(int64_t n, bool packet, F elem) {
+    float p = elem(0);
+    int64_t i = 1;
+    if (packet) {
+        const int64_t size4 = (n - 1) & ~int64_t(3);
+        for (; i < size4; i += 4) p = p + ((elem(i) + elem(i + 1)) + (elem(i + 2) + elem(i + 3)));
+    }
+    for (; i < n; ++i) p = p + elem(i);
+    return p;
+}
+
+}  // namespace
+
+extern "C" int saqgpu_column_mean_var(const float *data, int64_t n, int32_t d, float *mean, float *var) {
+    if (!data || n < 1 || d < 1 || (!mean && !var)) return SAQGPU_EINVAL;
+    const int32_t packet_cols = d / 16 * 16;
+    // the release build's -freciprocal-math multiplies by 1/n where the source divides
+    const float rn = 1.0f / (float)n;
+    std::vector<float> mu(d);
+    auto work = [&](int32_t j0, int32_t j1) {
+        for (int32_t j = j0; j < j1; ++j) {
+            const bool pk = j < packet_cols;
+            const float m = column_sum(n, pk, [&](int64_t i) { return data[i * d + j]; }) * rn;
+            mu[j] = m;
+            if (mean) mean[j] = m;
+            if (var)
+                var[j] = column_sum(n, pk, [&](int64_t i) {
+                             const float x = data[i * d + j] - m;
+                             return x * x;
+                         }) * rn;
+        }
+    };
+    const int nt = (int)std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(16, d), n * (int64_t)d >> 20));
+    std::vector<std::thread> th;
+    const int32_t per = (d + nt - 1) / nt;
+    for (int t = 0; t < nt; ++t) {
+        const int32_t j0 = t * per, j1 = std::min<int32_t>(d, j0 + per);
+        if (j0 < j1) th.emplace_back(work, j0, j1);
+    }
+    for (auto &t : th) t.join();
     return SAQGPU_OK;
 }

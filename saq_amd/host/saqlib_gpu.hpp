@@ -37,6 +37,7 @@ struct QuantSingleConfig {  // quantization/config.h:12-19 (CAQ only)
     bool random_rotation = true;
     int caq_adj_rd_lmt = 6;
     float caq_adj_eps = 1e-8f;
+    int caq_ori_qB = 0;  // codes found at this many bits, then shifted down (caq_encoder.hpp:144-169)
 };
 
 struct QuantizeConfig {  // quantization/config.h:21-27
@@ -113,8 +114,10 @@ class IVF {
     }
 
     // IVF::construct (ivf.hpp:130-184): data n x num_dim, centroids K x num_dim, row-major host
-    // arrays.  Variance is computed from the data unless set_variance() was called.
-    void construct(const float *data, const float *centroids, const PID *cluster_ids, uint64_t seed = 0) {
+    // arrays.  Variance is computed from the data unless set_variance() was called; with
+    // use_1_centroid every list is quantised against the data's column mean (ivf.hpp:163-176).
+    void construct(const float *data, const float *centroids, const PID *cluster_ids, uint64_t seed = 0,
+                   bool use_1_centroid = false) {
         if (variance_.empty()) compute_variance(data);
         if (plan_.empty()) make_plan();
         if (cfg_.single.random_rotation && rotations_.empty()) draw_rotations(seed);
@@ -124,9 +127,18 @@ class IVF {
             ++sizes[cluster_ids[i]];
         }
         create(centroids, sizes);
-        saqgpu_caq_cfg cc{cfg_.single.caq_adj_rd_lmt, cfg_.single.caq_adj_eps};
+        saqgpu_caq_cfg cc{cfg_.single.caq_adj_rd_lmt, cfg_.single.caq_adj_eps, cfg_.single.caq_ori_qB};
         int64_t nfail = 0;
-        check(saqgpu_encode(h_, data, cluster_ids, (int64_t)num_data_, &cc, 0, &nfail, nullptr), "construct");
+        if (use_1_centroid) {
+            std::vector<float> mean(num_dim_);
+            check(saqgpu_column_mean_var(data, (int64_t)num_data_, (int32_t)num_dim_, mean.data(), nullptr),
+                  "construct");
+            check(saqgpu_encode_one_centroid(h_, data, cluster_ids, (int64_t)num_data_, mean.data(), &cc, 0, &nfail,
+                                             nullptr),
+                  "construct");
+        } else {
+            check(saqgpu_encode(h_, data, cluster_ids, (int64_t)num_data_, &cc, 0, &nfail, nullptr), "construct");
+        }
         check_failures_ = nfail;
     }
 
@@ -356,7 +368,7 @@ class IVF {
 
     void encode_single_cfg(uint8_t *b) const {
         std::memset(b, 0, kSingleCfgBytes);
-        const int32_t caq = 0, ori_qb = 0;
+        const int32_t caq = 0, ori_qb = cfg_.single.caq_ori_qB;
         const int32_t rd = cfg_.single.caq_adj_rd_lmt;
         const float eps = cfg_.single.caq_adj_eps;
         std::memcpy(b + 0, &caq, 4);
@@ -384,6 +396,7 @@ class IVF {
         cfg_.single.random_rotation = b[kSingleOff + 4] != 0;
         std::memcpy(&cfg_.single.caq_adj_rd_lmt, b + kSingleOff + 8, 4);
         std::memcpy(&cfg_.single.caq_adj_eps, b + kSingleOff + 12, 4);
+        std::memcpy(&cfg_.single.caq_ori_qB, b + kSingleOff + 16, 4);
     }
     // SaqCluData::save/load write longb_code_bytes_ * num_vec_ long-code bytes
     // (cluster_data.hpp:226-229, 320, 330): the padded per-vector stride in the per-vector
@@ -415,21 +428,14 @@ class IVF {
         if (!h_) throw Error(SAQGPU_ESTATE, std::string(what) + ": index not constructed");
     }
 
-    // SaqDataMaker::compute_variance (saq_data.hpp:96-101): per-dimension population variance.
-    // Accumulated in double here; the reference's fp32 Eigen column means can differ in the last
-    // bit, which moves the plan only at exact DP ties -- pass set_variance() to pin it.
+    // SaqDataMaker::compute_variance (saq_data.hpp:96-101): per-dimension population variance in
+    // fp32, summed in Eigen's column-wise order (saqgpu_column_mean_var).
     void compute_variance(const float *data) {
-        std::vector<double> mean(num_dim_, 0.0), var(num_dim_, 0.0);
-        for (size_t i = 0; i < num_data_; ++i)
-            for (size_t j = 0; j < num_dim_; ++j) mean[j] += data[i * num_dim_ + j];
-        for (auto &m : mean) m /= (double)num_data_;
-        for (size_t i = 0; i < num_data_; ++i)
-            for (size_t j = 0; j < num_dim_; ++j) {
-                const double t = data[i * num_dim_ + j] - mean[j];
-                var[j] += t * t;
-            }
+        std::vector<float> var(num_dim_);
+        check(saqgpu_column_mean_var(data, (int64_t)num_data_, (int32_t)num_dim_, nullptr, var.data()),
+              "compute_variance");
         variance_.assign(num_dim_padded_, 0.0f);
-        for (size_t j = 0; j < num_dim_; ++j) variance_[j] = (float)(var[j] / (double)num_data_);
+        std::copy(var.begin(), var.end(), variance_.begin());
     }
 
     // Random orthonormal D_s x D_s matrices (Gaussian rows, modified Gram-Schmidt in double).

@@ -47,7 +47,7 @@ class LutScalars(C.Structure):
 
 
 class CaqCfg(C.Structure):
-    _fields_ = [("rd_lmt", C.c_int32), ("eps", C.c_float), ("rotate", C.c_int32)]
+    _fields_ = [("rd_lmt", C.c_int32), ("eps", C.c_float), ("rotate", C.c_int32), ("ori_qb", C.c_int32)]
 
 
 class CodeInfo(C.Structure):
@@ -197,6 +197,7 @@ def ref():
         L.ref_eigen_var_q2.restype = C.c_float
         L.ref_eigen_var_q2.argtypes = [f32p, i64, i64, f32p, i64]
         L.ref_eigen_gemv.argtypes = [f32p, f32p, i64, f32p]
+        L.ref_eigen_colwise_mean_var.argtypes = [f32p, i64, i64, f32p, f32p]
         L.ref_eigen_gemm.argtypes = [f32p, i64, i64, f32p, f32p]
         L.ref_lut_quantize.argtypes = [f32p, i64, C.c_float, C.c_float, u16p]
         L.ref_lut_delta.restype = C.c_float

@@ -70,11 +70,13 @@ def make_case(n=3000, d=128, K=12, nq=24, plan=None, avg_bits=4.0, seed=1, rotat
     return case
 
 
-def encode_oracle(case):
-    """IVF::construct with the oracle encoder: per cluster, per segment (saq_quantizer.hpp:34-60)."""
+def encode_oracle(case, ori_qb=0, one_centroid=None):
+    """IVF::construct with the oracle encoder: per cluster, per segment (saq_quantizer.hpp:34-60).
+    ori_qb: caq_ori_qB; one_centroid: the centroid every list is quantised against
+    (use_1_centroid), else each list's own."""
     O = oracle()
     plan = case.orc_plan()
-    cfg = CaqCfg(case.rd_lmt, case.eps, 1)
+    cfg = CaqCfg(case.rd_lmt, case.eps, 1, ori_qb)
     case.clusters = []
     for c in range(case.K):
         members = np.nonzero(case.cluster_ids == c)[0]
@@ -98,7 +100,7 @@ def encode_oracle(case):
             cp = max(0, min(ds, case.num_dim - off))
             if n:
                 X[:n, :cp] = case.base[members, off:off + cp]
-            cen[:cp] = case.centroids[c, off:off + cp]
+            cen[:cp] = (case.centroids[c] if one_centroid is None else one_centroid)[off:off + cp]
             co = _cluster_out(clu)
             P = case.rotations[s]
             O.orc_quantize_segment(C.byref(plan), s, X, n, cen, P.ctypes.data if P is not None else None,

@@ -145,3 +145,50 @@ def test_encode_rejects_bad_cluster_ids():
     xb.free()
     cb.free()
     ix.close()
+
+
+@pytest.mark.parametrize("plan,qb", [([(64, 4), (64, 3)], 7), ([(128, 1)], 4), ([(64, 5), (64, 0)], 9)])
+def test_ori_qb_encode_matches_oracle(plan, qb):
+    """caq_ori_qB: codes found at qb bits and shifted down (downUpSample) -- every byte equals the
+    oracle's with the same setting, and (above 1 bit, where the shifted sign codes can coincide
+    with the plain ones) differs from the plain encode."""
+    d = sum(x for x, _ in plan)
+    case = F.make_case(n=900, d=d, K=5, nq=2, plan=plan, seed=45, encode=False)
+    F.encode_oracle(case, ori_qb=qb)
+    ix = F.gpu_index(F.Case(case.num_dim, case.plan, case.base, case.queries, case.centroids, case.cluster_ids,
+                            case.variance, case.rotations))
+    ix.encode(case.base, case.cluster_ids, ori_qb=qb)
+    compare_clusters(case, ix)
+    if max(b for _, b in plan) > 1:
+        ix.encode(case.base, case.cluster_ids)
+        plain = ix.get_cluster(0)
+        assert not np.array_equal(plain["long_factors"], case.clusters[0]["long_factors"])
+    ix.close()
+
+
+def test_ori_qb_must_exceed_every_segment():
+    from saq_amd import SaqGpuError
+    case = F.make_case(n=300, d=128, K=3, nq=2, plan=[(64, 4), (64, 2)], seed=47)
+    ix = F.gpu_index(case)
+    for qb in (4, 3, 17):
+        with pytest.raises(SaqGpuError, match="caq_ori_qB"):
+            ix.encode(case.base, case.cluster_ids, ori_qb=qb)
+    ix.close()
+
+
+def test_one_centroid_encode_matches_oracle():
+    """use_1_centroid: every list quantised against data.colwise().mean() (computed in the
+    reference's order by saqgpu_column_mean_var) -- bytes equal the oracle's, every list's rotated
+    centroid is the shared one, and search still probes by the lists' own centroids."""
+    import saq_amd
+    case = F.make_case(n=1500, d=200, K=6, nq=8, plan=[(128, 4), (128, 2)], seed=49, encode=False)
+    mean, _ = saq_amd.column_mean_var(case.base)
+    F.encode_oracle(case, one_centroid=np.concatenate([mean, np.zeros(case.D_pad - case.num_dim, np.float32)]))
+    ix = F.gpu_index(F.Case(case.num_dim, case.plan, case.base, case.queries, case.centroids, case.cluster_ids,
+                            case.variance, case.rotations))
+    ix.encode(case.base, case.cluster_ids, centroid=mean)
+    compare_clusters(case, ix)
+    crot = [ix.get_cluster(c)["centroid_rot"] for c in range(case.K)]
+    assert all(np.array_equal(crot[0], x) for x in crot[1:])
+    assert np.array_equal(ix.probe(case.queries, 3), F.orc_probes(case, 3))
+    ix.close()
