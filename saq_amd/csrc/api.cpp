@@ -30,8 +30,8 @@ hipError_t launch_probe(const float *centroids, int K, int D, const float *queri
 hipError_t launch_sqnorm_rows(const float *x, int64_t rows, int D, float *out, hipStream_t st);
 size_t estimate_smem_bytes(const IndexView &ix);
 hipError_t launch_estimate(const IndexView &ix, const float *qrot, const float *bound, const uint32_t *probes,
-                           int nprobe, const int64_t *offs, int dist_type, uint32_t *out_ids, float *out_acc,
-                           float *out_fast, float *out_vars, hipStream_t st);
+                           const int64_t *offs, const int2 *pieces, int npieces, int dist_type, uint32_t *out_ids,
+                           float *out_acc, float *out_fast, float *out_vars, hipStream_t st);
 hipError_t launch_probe_select_ip(const float *ip, const float *cnorm, int K, int64_t nq, int nprobe, int dist_type,
                                   uint32_t *probes, hipStream_t st);
 hipError_t launch_probe_exact(float *ip, const float *cnorm, const float *centroids, int K, int D,
@@ -175,6 +175,7 @@ struct saqgpu_index {
     DevBuf<float> ws_q, ws_qrot, ws_bound, ws_dists, ws_cdist;
     DevBuf<int32_t> ws_ties;   // per query: its probe list holds exact distance ties
     DevBuf<int64_t> ws_eoffs;  // IVF::estimate: per probe its first output slot
+    DevBuf<int2> ws_epieces;   //   per workgroup its (probe, first block)
     DevBuf<uint32_t> ws_eids;  //   and its outputs
     DevBuf<float> ws_eout;
     DevBuf<uint32_t> ws_probes, ws_ids;
@@ -520,6 +521,7 @@ void saqgpu_index_destroy(saqgpu_index *ix) {
     ix->ws_met.release();
     ix->ws_ties.release();
     ix->ws_eoffs.release();
+    ix->ws_epieces.release();
     ix->ws_eids.release();
     ix->ws_eout.release();
     ix->bq_mb.release();
@@ -1086,22 +1088,25 @@ int saqgpu_estimate(saqgpu_index *ix, const float *query, const uint32_t *probes
         HIPCHK(hipMemcpy(pr.data(), ix->ws_probes.p, nprobe * 4, hipMemcpyDeviceToHost));
     }
     std::vector<int64_t> offs(nprobe);
+    std::vector<int2> pieces;  // 8-block pieces of the probed lists, one workgroup each
     int64_t tot = 0;
     for (int i = 0; i < nprobe; ++i) {
         offs[i] = tot;
         tot += ix->clu_n[pr[i]];
+        const int nb = (ix->clu_n[pr[i]] + kBlock - 1) / kBlock;
+        for (int st = 0; st < nb; st += 8) pieces.push_back(make_int2(i, st));
     }
     // the index's estimate workspace, grown to the largest call and kept
     DevBuf<int64_t> &d_offs = ix->ws_eoffs;
     DevBuf<uint32_t> &d_ids = ix->ws_eids;
     DevBuf<float> &d_out = ix->ws_eout;
-    if ((rc = upload(d_offs, offs))) return rc;
+    if ((rc = upload(d_offs, offs)) || (rc = upload(ix->ws_epieces, pieces))) return rc;
     if (d_ids.ensure(std::max<int64_t>(1, tot)) || d_out.ensure(3 * std::max<int64_t>(1, tot)))
         return fail(SAQGPU_ENOMEM, "allocation failed");
     HIPCHK(launch_query_prep(ix->view(), ix->ws_q.p, 1, ix->variance.p, ix->has_rot ? ix->rot.p : nullptr,
                              ix->d_rot_off.p, vars_bound_m, ix->ws_qrot.p, ix->ws_bound.p, ix->max_dim, 0));
-    HIPCHK(launch_estimate(ix->view(), ix->ws_qrot.p, ix->ws_bound.p, ix->ws_probes.p, nprobe, d_offs.p, dist_type,
-                           d_ids.p, d_out.p, d_out.p + tot, d_out.p + 2 * tot, 0));
+    HIPCHK(launch_estimate(ix->view(), ix->ws_qrot.p, ix->ws_bound.p, ix->ws_probes.p, d_offs.p, ix->ws_epieces.p,
+                           (int)pieces.size(), dist_type, d_ids.p, d_out.p, d_out.p + tot, d_out.p + 2 * tot, 0));
     if (tot) {
         HIPCHK(hipMemcpy(out_ids, d_ids.p, tot * 4, hipMemcpyDeviceToHost));
         HIPCHK(hipMemcpy(out_acc, d_out.p, tot * 4, hipMemcpyDeviceToHost));

@@ -2,9 +2,9 @@
 //
 // For one query and its probed lists: the accurate, fast and variance estimates of every
 // vector, unpruned, in probe order (the reference's test_relative_error path).  One workgroup
-// per probed list: prepare_list builds the residual, the per-segment scalars and the u16 LUTs
-// (CaqCluEstimator::prepare); the list then streams 8 blocks (256 vectors, one thread per
-// vector) at a time:
+// per 8-block piece (256 vectors, one thread per vector) of a probed list, so that a query's
+// long lists spread over the chip: prepare_list builds the list's residual, per-segment scalars
+// and u16 LUTs (CaqCluEstimator::prepare), then the piece's vectors get:
 //   fast  = sum over segments, in order from 0, of the segment's fast estimate (its variance
 //           estimate for 0-bit segments)          SaqCluEstimator::compFastDist (:148-158)
 //   vars  = sum of the variance estimates         SaqCluEstimator::varsEstDist  (:127-137)
@@ -23,6 +23,7 @@ struct EstArgs {
     const float *bound;     // nseg
     const uint32_t *probes; // nprobe
     const int64_t *offs;    // output start of each probed list
+    const int2 *pieces;     // per workgroup: (probe index, first block of its 8)
     uint32_t *out_ids;
     float *out_acc, *out_fast, *out_vars;
 };
@@ -70,7 +71,8 @@ __global__ __launch_bounds__(256) void estimate_kernel(EstArgs a) {
     const int S = ix.nseg, NG = ix.ngroups;
     EstSmem m;
     est_layout(smem, ix, &m);
-    const uint32_t c = a.probes[blockIdx.x];
+    const int2 piece = a.pieces[blockIdx.x];
+    const uint32_t c = a.probes[piece.x];
     const int n = ix.clu_n[c];
     if (n == 0) return;  // uniform per workgroup
     for (int i = tid; i < S; i += 256) m.segs[i] = ix.segs[i];
@@ -94,7 +96,8 @@ __global__ __launch_bounds__(256) void estimate_kernel(EstArgs a) {
     const int nb = (n + kBlock - 1) / kBlock;
     const int64_t vec0 = ix.clu_vec[c];
     const int v = tid, bl = v >> 5, ln = v & 31;
-    for (int st = 0; st < nb; st += 8) {
+    {
+        const int st = piece.y;
         const bool active = st + bl < nb;
         const int idx = (st + bl) * kBlock + ln;
         if (active) {
@@ -173,7 +176,7 @@ __global__ __launch_bounds__(256) void estimate_kernel(EstArgs a) {
                 fast = fast + fd;
                 accd = accd + ad;
             }
-            const int64_t o = a.offs[blockIdx.x] + idx;
+            const int64_t o = a.offs[piece.x] + idx;
             a.out_ids[o] = ix.ids[gv];
             a.out_acc[o] = accd;
             if (a.out_fast) a.out_fast[o] = fast;
@@ -186,18 +189,18 @@ __global__ __launch_bounds__(256) void estimate_kernel(EstArgs a) {
 size_t estimate_smem_bytes(const IndexView &ix) { return est_layout(nullptr, ix, nullptr); }
 
 hipError_t launch_estimate(const IndexView &ix, const float *qrot, const float *bound, const uint32_t *probes,
-                           int nprobe, const int64_t *offs, int dist_type, uint32_t *out_ids, float *out_acc,
-                           float *out_fast, float *out_vars, hipStream_t st) {
-    if (nprobe < 1) return hipSuccess;
-    EstArgs a{ix, qrot, bound, probes, offs, out_ids, out_acc, out_fast, out_vars};
+                           const int64_t *offs, const int2 *pieces, int npieces, int dist_type, uint32_t *out_ids,
+                           float *out_acc, float *out_fast, float *out_vars, hipStream_t st) {
+    if (npieces < 1) return hipSuccess;
+    EstArgs a{ix, qrot, bound, probes, offs, pieces, out_ids, out_acc, out_fast, out_vars};
     const size_t smem = estimate_smem_bytes(ix);
     const void *fn = dist_type == 1 ? (const void *)estimate_kernel<1> : (const void *)estimate_kernel<0>;
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
     if (e != hipSuccess) return e;
     if (dist_type == 1)
-        hipLaunchKernelGGL(estimate_kernel<1>, dim3(nprobe), dim3(256), smem, st, a);
+        hipLaunchKernelGGL(estimate_kernel<1>, dim3(npieces), dim3(256), smem, st, a);
     else
-        hipLaunchKernelGGL(estimate_kernel<0>, dim3(nprobe), dim3(256), smem, st, a);
+        hipLaunchKernelGGL(estimate_kernel<0>, dim3(npieces), dim3(256), smem, st, a);
     return hipGetLastError();
 }
 
