@@ -108,6 +108,16 @@ const char *saqgpu_last_error(void);
 int saqgpu_index_create(const saqgpu_index_desc *desc, saqgpu_index **out);
 void saqgpu_index_destroy(saqgpu_index *idx);
 
+/* A search handle for concurrent callers (IVF::search called from many host threads, as
+ * src/test_qps.cpp:77-84 does): a handle reads idx's device data but has its own workspace,
+ * lock, device-order event and library handles, so calls on different handles -- each on its own
+ * stream -- run concurrently, while calls on one handle serialise as on an index.  Every search,
+ * probe and estimate entry point accepts a handle in place of the index (the batched ones where
+ * idx built its batched layout before the handle was opened); entry points that change the index
+ * refuse a handle, and refuse idx while any handle is open.  Destroy handles with
+ * saqgpu_index_destroy before idx. */
+int saqgpu_index_search_handle(saqgpu_index *idx, saqgpu_index **out);
+
 int saqgpu_cluster_sizes_of(const saqgpu_index *idx, int32_t cid, saqgpu_cluster_sizes *out);
 
 /* Uploads one cluster given in the reference layout (SaqCluData::load, cluster_data.hpp:317-326);

@@ -62,6 +62,7 @@ _SIGS = {
     "saqgpu_last_error": (C.c_char_p, []),
     "saqgpu_index_create": (C.c_int, [C.POINTER(_Desc), C.POINTER(C.c_void_p)]),
     "saqgpu_index_destroy": (None, [C.c_void_p]),
+    "saqgpu_index_search_handle": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     "saqgpu_cluster_sizes_of": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(ClusterSizes)]),
     "saqgpu_index_set_cluster": (C.c_int, [C.c_void_p, C.c_int32] + [C.c_void_p] * 6),
     "saqgpu_index_get_cluster": (C.c_int, [C.c_void_p, C.c_int32] + [C.c_void_p] * 6),
@@ -168,6 +169,18 @@ class Index:
         h = C.c_void_p()
         _check(lib().saqgpu_index_create(C.byref(desc), C.byref(h)))
         self.h = h
+
+    def search_handle(self):
+        """A handle on this index's device data with its own workspace and lock
+        (saqgpu_index_search_handle): searches on different handles run concurrently.  Close
+        every handle before the index."""
+        h = C.c_void_p()
+        _check(lib().saqgpu_index_search_handle(self.h, C.byref(h)))
+        other = Index.__new__(Index)
+        other.__dict__.update({k: v for k, v in self.__dict__.items() if k != "h"})
+        other.h = h
+        other._parent = self  # the index outlives its handles
+        return other
 
     def close(self):
         if getattr(self, "h", None):

@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <atomic>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -91,10 +92,17 @@ template <class T>
 struct DevBuf {
     T *p = nullptr;
     size_t n = 0;
+    bool borrowed = false;  // another index's buffer (a search handle's view): never freed here
     DevBuf() = default;
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
     ~DevBuf() { release(); }  // a function's scratch buffers go when it returns
+    void borrow(const DevBuf &o) {
+        release();
+        p = o.p;
+        n = o.n;
+        borrowed = true;
+    }
     int alloc(size_t cnt) {
         if (p) (void)hipFree(p);
         p = nullptr;
@@ -108,9 +116,10 @@ struct DevBuf {
     }
     int ensure(size_t cnt) { return cnt <= n && p ? 0 : alloc(cnt); }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p && !borrowed) (void)hipFree(p);
         p = nullptr;
         n = 0;
+        borrowed = false;
     }
 };
 
@@ -153,6 +162,10 @@ struct BatchedGraphKey {
 };
 
 struct saqgpu_index {
+    // a search handle (saqgpu_index_search_handle) reads its parent's device data through
+    // borrowed buffers; the parent counts its open handles and refuses changes while any is open
+    saqgpu_index *parent = nullptr;
+    std::atomic<int> handles{0};
     int device = 0;
     int64_t num_data = 0;
     int num_dim = 0, num_cen = 0, nseg = 0, D_pad = 0, ngroups = 0, max_dim = 0;
@@ -305,6 +318,13 @@ struct IndexCall {
         if (begun) (void)hipEventRecord(ix->ws_free, st);
     }
 };
+
+// Entry points that change the index: refused on a search handle and while handles are open.
+int check_mutable(const saqgpu_index *ix) {
+    if (ix->parent) return fail(SAQGPU_ESTATE, "a search handle cannot change the index");
+    if (ix->handles.load() > 0) return fail(SAQGPU_ESTATE, "the index has open search handles");
+    return SAQGPU_OK;
+}
 }  // namespace
 
 namespace {
@@ -494,8 +514,76 @@ int saqgpu_index_create(const saqgpu_index_desc *d, saqgpu_index **out) {
     return SAQGPU_OK;
 }
 
+int saqgpu_index_search_handle(saqgpu_index *src, saqgpu_index **out) {
+    if (!src || !out) return fail(SAQGPU_EINVAL, "null argument");
+    if (src->parent) return fail(SAQGPU_EINVAL, "a search handle cannot be cloned: clone its index");
+    std::lock_guard<std::mutex> lk(src->mu);  // the index's state holds still while it is copied
+    if (hipSetDevice(src->device) != hipSuccess) return fail(SAQGPU_EDEVICE, "hipSetDevice failed");
+    auto *h = new saqgpu_index;
+    h->parent = src;
+    h->device = src->device;
+    h->num_data = src->num_data;
+    h->num_dim = src->num_dim;
+    h->num_cen = src->num_cen;
+    h->nseg = src->nseg;
+    h->D_pad = src->D_pad;
+    h->ngroups = src->ngroups;
+    h->max_dim = src->max_dim;
+    h->compact = src->compact;
+    h->has_rot = src->has_rot;
+    h->seg_dim = src->seg_dim;
+    h->seg_bits = src->seg_bits;
+    h->segs = src->segs;
+    h->clu_blk = src->clu_blk;
+    h->clu_vec = src->clu_vec;
+    h->rot_off = src->rot_off;
+    h->clu_n = src->clu_n;
+    h->clu_set = src->clu_set;
+    h->total_blocks = src->total_blocks;
+    h->block_stride = src->block_stride;
+    h->long_stride = src->long_stride;
+    h->blocks.borrow(src->blocks);
+    h->long_codes.borrow(src->long_codes);
+    h->exfac.borrow(src->exfac);
+    h->ipcent.borrow(src->ipcent);
+    h->crot.borrow(src->crot);
+    h->centroids.borrow(src->centroids);
+    h->rot.borrow(src->rot);
+    h->variance.borrow(src->variance);
+    h->ids.borrow(src->ids);
+    h->d_clu_blk.borrow(src->d_clu_blk);
+    h->d_clu_vec.borrow(src->d_clu_vec);
+    h->d_rot_off.borrow(src->d_rot_off);
+    h->d_clu_n.borrow(src->d_clu_n);
+    h->d_segs.borrow(src->d_segs);
+    if (src->bq_ready) {  // the batched layout, where the index has built it
+        h->bq_ready = true;
+        h->bq_tile = src->bq_tile;
+        h->bq_nqs = src->bq_nqs;
+        h->bq_kq = src->bq_kq;
+        h->bq_fac_off = src->bq_fac_off;
+        h->bq_stride = src->bq_stride;
+        h->bq_total_mb = src->bq_total_mb;
+        h->bsegs = src->bsegs;
+        h->clu_mb = src->clu_mb;
+        h->bq_mb.borrow(src->bq_mb);
+        h->d_clu_mb.borrow(src->d_clu_mb);
+        h->d_bq_order.borrow(src->d_bq_order);
+        h->d_bsegs.borrow(src->d_bsegs);
+    }
+    if (hipEventCreateWithFlags(&h->ws_free, hipEventDisableTiming) != hipSuccess) {
+        h->parent = nullptr;
+        saqgpu_index_destroy(h);
+        return fail(SAQGPU_EDEVICE, "event creation failed");
+    }
+    src->handles++;
+    *out = h;
+    return SAQGPU_OK;
+}
+
 void saqgpu_index_destroy(saqgpu_index *ix) {
     if (!ix) return;
+    if (ix->parent) ix->parent->handles--;
     hipSetDevice(ix->device);
     ix->blocks.release();
     ix->long_codes.release();
@@ -603,6 +691,7 @@ int saqgpu_index_set_cluster(saqgpu_index *ix, int32_t cid, const float *short_f
                              const uint8_t *long_code, const float *long_factors, const uint32_t *ids,
                              const float *centroid_rot) {
     if (!ix || cid < 0 || cid >= ix->num_cen) return fail(SAQGPU_EINVAL, "bad cluster id");
+    if (int rc_ = check_mutable(ix)) return rc_;
     IndexCall call(const_cast<saqgpu_index *>(ix), nullptr);
     if (int rc_ = call.begin()) return rc_;
     const int64_t n = ix->clu_n[cid];
@@ -1143,6 +1232,7 @@ static int encode_impl(saqgpu_index *ix, const float *data, const uint32_t *clus
     if (!ix || !data || !cluster_ids || !cfg) return fail(SAQGPU_EINVAL, "null argument");
     if (n != ix->num_data) return fail(SAQGPU_EINVAL, "n must equal the index's num_data");
     if (int rc_ = check_caq_cfg(ix, cfg)) return rc_;
+    if (int rc_ = check_mutable(ix)) return rc_;
     IndexCall call(ix, (hipStream_t)stream);
     if (int rc_ = call.begin()) return rc_;
     hipStream_t st = (hipStream_t)stream;
@@ -1240,6 +1330,7 @@ int saqgpu_encode_residuals(saqgpu_index *ix, int32_t cid, int32_t seg, const fl
     if (!ix || !o || !cfg || cid < 0 || cid >= ix->num_cen || seg < 0 || seg >= ix->nseg)
         return fail(SAQGPU_EINVAL, "bad argument");
     if (int rc_ = check_caq_cfg(ix, cfg)) return rc_;
+    if (int rc_ = check_mutable(ix)) return rc_;
     IndexCall call(ix, nullptr);
     if (int rc_ = call.begin()) return rc_;
     const int64_t n = ix->clu_n[cid];
@@ -1305,6 +1396,7 @@ BatchedView batched_view(const saqgpu_index *ix) {
 // |o|^2[16], rescale[nqs][16], id[16].
 int prepare_batched_locked(saqgpu_index *ix) {
     if (ix->bq_ready) return SAQGPU_OK;
+    if (ix->parent) return fail(SAQGPU_ESTATE, "build the batched layout on the index before opening the handle");
     for (int c = 0; c < ix->num_cen; ++c)
         if (!ix->clu_set[c] && ix->clu_n[c] > 0) return fail(SAQGPU_ESTATE, "index has clusters not yet set");
     for (int s = 0; s < ix->nseg; ++s)

@@ -244,3 +244,53 @@ def test_cluster_layout_round_trip():
                 assert np.array_equal(got[key], ref[key][:got[key].size]), (plan, c, key)
             assert np.array_equal(got["ids"], ref["ids"][:ref["num_vec"]])
         ix.close()
+
+
+def test_search_handles_run_concurrently_and_agree():
+    """Concurrent IVF::search callers (test_qps.cpp:77-84): four search handles, each searched from
+    its own host thread on its own stream, return what the index returns; a handle cannot change
+    the index, and the index refuses changes while handles are open."""
+    import ctypes as C
+    import threading
+    from hip_mem import hip
+    from saq_amd import SaqGpuError
+    case = F.make_case(n=6000, d=128, K=24, nq=64, plan=[(64, 5), (64, 3)], seed=91)
+    ix = F.gpu_index(case)
+    want = ix.search(case.queries, 10, 8)
+    H = hip()
+    hs, streams = [], []
+    for _ in range(4):
+        hs.append(ix.search_handle())
+        st = C.c_void_p()
+        assert H.hipStreamCreate(C.byref(st)) == 0
+        streams.append(st)
+    out, errs = [None] * 4, []
+
+    def run(k):
+        try:
+            out[k] = [hs[k].search(case.queries, 10, 8, stream=streams[k].value) for _ in range(3)]
+        except Exception as e:  # reported below
+            errs.append(e)
+
+    threads = [threading.Thread(target=run, args=(k,)) for k in range(4)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert not errs, errs
+    for k in range(4):
+        for ids, d, met in out[k]:
+            assert np.array_equal(ids, want[0])
+            assert np.array_equal(d.view(np.uint32), want[1].view(np.uint32))
+            assert np.array_equal(met, want[2])
+    with pytest.raises(SaqGpuError, match="search handle"):
+        hs[0].encode(case.base, case.cluster_ids)
+    with pytest.raises(SaqGpuError, match="open search handles"):
+        ix.encode(case.base, case.cluster_ids)
+    for h, st in zip(hs, streams):
+        h.close()
+        assert H.hipStreamDestroy(st) == 0
+    ix.encode(case.base, case.cluster_ids)
+    again = ix.search(case.queries, 10, 8)
+    assert np.array_equal(again[0], want[0])
+    ix.close()
