@@ -129,7 +129,9 @@ def read_traffic(name, n, bits, nprobe, **extra):
     try:
         t = json.load(open(path))
         if t.get("nprobe") == nprobe and t.get("bits") == bits and t.get("n") == n and \
-                all(t.get(k) == v for k, v in extra.items()):
+                all(t.get(k) == v for k, v in extra.items() if k != "query_tile"):
+            if "by_query_tile" in t:
+                return t["by_query_tile"].get(str(extra.get("query_tile")))
             return t.get("hbm_bytes_per_launch")
     except (OSError, ValueError):
         pass
@@ -466,6 +468,8 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=3.0, help="duration of each serving-loop load")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving loop (kernel profiling runs)")
     ap.add_argument("--no-single", action="store_true", help="skip the single-vector record measurement")
+    ap.add_argument("--traffic-run", action="store_true",
+                    help="only the launches the committed HBM-traffic summaries are measured on")
     ap.add_argument("--no-precision", action="store_true",
                     help="skip the B=8 index and its int8 / int8x2 / bf16 distance errors")
     ap.add_argument("--exact-probe", action="store_true",
@@ -555,6 +559,22 @@ def main():
         if r >= args.target_recall:
             break
     log(f"plan {plan}; index {ix.device_bytes() / 1e9:.2f} GB")
+    if args.traffic_run:
+        # the launches the HBM-traffic summaries under profiles/ count, in this order: one plain
+        # search (sliced scan) at the workload's nprobe, then one batched int8 search per query
+        # tile 16, 32, 64, 128 (one bscan_kernel each)
+        np_t = int(W["npq_range"][1]) if W["npq_range"] else (args.nprobe or 64)
+        search(ix, np_t)
+        ix.prepare_batched()
+        tb_i = torch.zeros(nq, args.topk, dtype=torch.int32, device=dev)
+        tb_d = torch.zeros(nq, args.topk, dtype=torch.float32, device=dev)
+        for qt in (16, 32, 64, 128):
+            ix.search_batched(queries, args.topk, np_t, precision=saq_amd.PREC_INT8, query_tile=qt, out_ids=tb_i,
+                              out_dists=tb_d, stream=stream, nprobe_per_query=npq)
+        torch.cuda.synchronize()
+        print(json.dumps({"traffic_run": args.workload, "n": n, "bits": bits, "nprobe": np_t,
+                          "query_tiles": [16, 32, 64, 128]}))
+        return
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     nfail = ix.encode(base, cid32)
