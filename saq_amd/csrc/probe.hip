@@ -790,6 +790,9 @@ __global__ __launch_bounds__(256) void probe_heap_kernel(float *__restrict__ dis
     for (int i = tid; i < nprobe; i += 256) probes[q * nprobe + i] = hi[i];
 }
 
+// queries up to which a call computes every (query, centroid) distance before the tie pass
+constexpr int64_t kHeapPrecomputeMax = 64;
+
 // The exact coarse quantiser over GEMM products ip (nq x K) and centroid norms cnorm: probe
 // lists equal to probe_tile + probe_select's.  nprobe <= kExactCand.
 // ip is overwritten with exact distances in the rows of queries with ties; ties: nq ints.
@@ -807,12 +810,23 @@ hipError_t launch_probe_exact(float *ip, const float *cnorm, const float *centro
         K <= 4096 ? go(probe_exact_kernel<1, 16>) : go(probe_exact_kernel<1, 0>);
     else
         K <= 4096 ? go(probe_exact_kernel<0, 16>) : go(probe_exact_kernel<0, 0>);
+    // a query with ties needs every centroid's exact distance for std::partial_sort's order: for
+    // a few queries they are computed by the whole chip beforehand (into ip, which the exact
+    // kernel has finished with), for many by each flagged query's own workgroup
+    const int recompute = nq > kHeapPrecomputeMax ? 1 : 0;
+    if (!recompute) {
+        const dim3 g1((unsigned)((K + 255) / 256), (unsigned)nq);
+        if (dist_type == 1)
+            hipLaunchKernelGGL(probe_dist_kernel<1>, g1, dim3(256), sm, st, centroids, K, D, queries, ip);
+        else
+            hipLaunchKernelGGL(probe_dist_kernel<0>, g1, dim3(256), sm, st, centroids, K, D, queries, ip);
+    }
     if (dist_type == 1)
-        hipLaunchKernelGGL(probe_heap_kernel<1>, dim3((unsigned)nq), dim3(256), 0, st, ip, 1, centroids, K, D, queries,
-                           nprobe, ties, probes);
+        hipLaunchKernelGGL(probe_heap_kernel<1>, dim3((unsigned)nq), dim3(256), 0, st, ip, recompute, centroids, K, D,
+                           queries, nprobe, ties, probes);
     else
-        hipLaunchKernelGGL(probe_heap_kernel<0>, dim3((unsigned)nq), dim3(256), 0, st, ip, 1, centroids, K, D, queries,
-                           nprobe, ties, probes);
+        hipLaunchKernelGGL(probe_heap_kernel<0>, dim3((unsigned)nq), dim3(256), 0, st, ip, recompute, centroids, K, D,
+                           queries, nprobe, ties, probes);
     return hipGetLastError();
 }
 

@@ -174,15 +174,17 @@ def test_gpu_probe_matches_oracle_across_dims(d, dist_type, path, monkeypatch):
     ix.close()
 
 
+@pytest.mark.parametrize("nq", [12, 80])
 @pytest.mark.parametrize("path", ["gemm", "tile"])
 @pytest.mark.parametrize("dist_type", [0, 1])
-def test_gpu_probe_exact_ties(dist_type, path, monkeypatch):
+def test_gpu_probe_exact_ties(dist_type, path, nq, monkeypatch):
     """Exact distance ties come out in std::partial_sort's order (the oracle calls the same
     libstdc++ algorithm the reference does): duplicated centroids tie in pairs, and a zero query
     ties every centroid under IP, so the radix select has to reach the id bytes and the heap
-    selection decides the order."""
+    selection decides the order.  Up to 64 queries the GEMM path computes every exact distance
+    on the whole chip before the tie pass, above it each flagged query's workgroup does."""
     monkeypatch.setenv("SAQGPU_PROBE", path)
-    case = F.make_case(n=1500, d=96, K=300, nq=12, plan=[(128, 2)], seed=23)
+    case = F.make_case(n=1500, d=96, K=300, nq=nq, plan=[(128, 2)], seed=23)
     case.centroids[150:] = case.centroids[:150]
     case.queries[0] = 0.0
     case.queries[5] = case.centroids[7]
