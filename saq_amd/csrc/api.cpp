@@ -1603,7 +1603,7 @@ int search_batched_locked(saqgpu_index *ix, const float *queries, int64_t nq, co
         }
         HIPCHK(launch_query_prep(ix->view(), dq, nq, ix->variance.p, ix->has_rot ? ix->rot.p : nullptr,
                                  ix->d_rot_off.p, 4.0f, ix->ws_qrot.p, ix->ws_bound.p, ix->max_dim, st));
-        HIPCHK(launch_bq_group(dp, dnpq, nq, NP, K, query_tile, ix->d_bq_order.p, ix->d_clu_n.p, w, st));
+        HIPCHK(launch_bq_group(dp, dnpq, nq, NP, K, query_tile, qs, ix->d_bq_order.p, ix->d_clu_n.p, w, st));
         HIPCHK(launch_bq_pairprep(bv, ix->ws_qrot.p, dp, w, pb, NP, npairs, qs, st));
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (timed) {

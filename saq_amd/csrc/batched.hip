@@ -1474,11 +1474,14 @@ __global__ __launch_bounds__(256) void bq_merge_kernel(const float *cand_d, cons
 
 // ------------------------------------------------------------------ host-side launchers
 // The (waves per workgroup, query tiles of 16 per wave) of a query tile of qt queries.
-static void tile_shape(int qt, int &W, int &qtm) {
+static void tile_shape(int qt, int qs, int &W, int &qtm) {
+    // (int8 64 as 2 waves x 2 row tiles: 11.9 ms against 15.7 ms as 4 waves x 1 on 10M x 1024; the
+    // wider query forms keep 4 x 1 there, 21 ms against 25-31 ms; 4 row tiles per wave hold 2 waves
+    // per SIMD and ran 19-21 ms)
     switch (qt) {
     case 16: W = 1, qtm = 1; break;
     case 32: W = 2, qtm = 1; break;
-    case 64: W = 4, qtm = 1; break;
+    case 64: W = qs == 1 ? 2 : 4, qtm = qs == 1 ? 2 : 1; break;
     default: W = 4, qtm = 2; break;  // 128
     }
 }
@@ -1495,10 +1498,10 @@ hipError_t launch_bq_build(const IndexView &ix, const BatchedView &bv, const int
     return hipGetLastError();
 }
 
-hipError_t launch_bq_group(const uint32_t *probes, const int32_t *npq, int64_t nq, int NP, int K, int QT,
+hipError_t launch_bq_group(const uint32_t *probes, const int32_t *npq, int64_t nq, int NP, int K, int QT, int qs,
                            const int32_t *order, const int32_t *clu_n, const BqWork &w, hipStream_t st) {
     int W, qtm;
-    tile_shape(QT, W, qtm);
+    tile_shape(QT, qs, W, qtm);
     const int workers = kSplitWaves / W;  // resident workgroups of the tile kernel
     hipLaunchKernelGGL(bq_zero_kernel, dim3(16), dim3(256), 0, st, w.counts, K, w.work, w.qbound, nq);
     const int64_t n = nq * NP;
@@ -1531,7 +1534,7 @@ static hipError_t launch_bscan_t(const BScanArgs &a, int grid, hipStream_t st) {
 
 size_t bscan_smem_bytes(const BatchedView &bv, int topk, int qt, int qs) {
     int W, qtm;
-    tile_shape(qt, W, qtm);
+    tile_shape(qt, qs, W, qtm);
     if (qs == 3) {
         if (qtm == 1) return (size_t)W * wave_smem<1, 3>(bv.nqs, topk) + 16;
         return (size_t)W * wave_smem<2, 3>(bv.nqs, topk) + 16;
@@ -1608,7 +1611,7 @@ hipError_t launch_bscan_tiles(const BatchedView &bv, const BqWork &w, const BqPa
     switch (qt) {
     case 16: return qs == 2 ? launch_bscan_q<1, 1, 2>(a, grid, st) : launch_bscan_q<1, 1, 1>(a, grid, st);
     case 32: return qs == 2 ? launch_bscan_q<2, 1, 2>(a, grid, st) : launch_bscan_q<2, 1, 1>(a, grid, st);
-    case 64: return qs == 2 ? launch_bscan_q<4, 1, 2>(a, grid, st) : launch_bscan_q<4, 1, 1>(a, grid, st);
+    case 64: return qs == 2 ? launch_bscan_q<4, 1, 2>(a, grid, st) : launch_bscan_q<2, 2, 1>(a, grid, st);
     default: return qs == 2 ? launch_bscan_q<4, 2, 2>(a, grid, st) : launch_bscan_q<4, 2, 1>(a, grid, st);
     }
 }

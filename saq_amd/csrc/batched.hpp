@@ -60,7 +60,7 @@ hipError_t launch_bq_build(const IndexView &ix, const BatchedView &bv, const int
                            const int32_t *chunk_seg, const int32_t *chunk_kc, int nchunks, uint8_t *out,
                            hipStream_t st);
 // order: the lists by decreasing size (the order work tiles are handed out in).
-hipError_t launch_bq_group(const uint32_t *probes, const int32_t *npq, int64_t nq, int NP, int K, int QT,
+hipError_t launch_bq_group(const uint32_t *probes, const int32_t *npq, int64_t nq, int NP, int K, int QT, int qs,
                            const int32_t *order, const int32_t *clu_n, const BqWork &w, hipStream_t st);
 hipError_t launch_bq_merge(const float *cand_d, const uint32_t *cand_id, const uint32_t *probes, const BqWork &w,
                            const int32_t *npq, int64_t nq, int NP, int topk, uint32_t *out_ids, float *out_d,
