@@ -157,7 +157,7 @@ def test_recall_not_below_plain_scan():
     r_plain = np.mean([len(set(plain[i]) & set(gt[i])) / 10 for i in range(100)])
     r_bat = np.mean([len(set(bat[i]) & set(gt[i])) / 10 for i in range(100)])
     print(f"recall@10 plain {r_plain:.3f} batched {r_bat:.3f}")
-    assert r_bat >= r_plain - 0.005
+    assert r_bat >= r_plain
     ix.close()
 
 
