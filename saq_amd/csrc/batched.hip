@@ -655,14 +655,14 @@ __device__ __forceinline__ void pairprep_whole(const PairPrepArgs &a, int64_t g,
                 sq += (r[k].x * r[k].x + r[k].y * r[k].y) + (r[k].z * r[k].z + r[k].w * r[k].w);
             }
         }
-        sq += xor_f<32>(sq), sr += xor_f<32>(sr), amax = fmaxf(amax, xor_f<32>(amax));
-        sq += xor_f<16>(sq), sr += xor_f<16>(sr), amax = fmaxf(amax, xor_f<16>(amax));
-        sq += xor_f<8>(sq), sr += xor_f<8>(sr), amax = fmaxf(amax, xor_f<8>(amax));
-        sq += xor_f<4>(sq), sr += xor_f<4>(sr), amax = fmaxf(amax, xor_f<4>(amax));
-        sq += xor_f<2>(sq), sr += xor_f<2>(sr), amax = fmaxf(amax, xor_f<2>(amax));
-        sq += xor_f<1>(sq), sr += xor_f<1>(sr), amax = fmaxf(amax, xor_f<1>(amax));
-        q2 += sq;
+        q2 += sq;  // |r|^2 is only needed whole: one reduction after the segments
         if (bs.qs < 0) continue;
+        sr += xor_f<32>(sr), amax = fmaxf(amax, xor_f<32>(amax));
+        sr += xor_f<16>(sr), amax = fmaxf(amax, xor_f<16>(amax));
+        sr += xor_f<8>(sr), amax = fmaxf(amax, xor_f<8>(amax));
+        sr += xor_f<4>(sr), amax = fmaxf(amax, xor_f<4>(amax));
+        sr += xor_f<2>(sr), amax = fmaxf(amax, xor_f<2>(amax));
+        sr += xor_f<1>(sr), amax = fmaxf(amax, xor_f<1>(amax));
         if (a.qs == 3) {
             bf16_segment<NKT>(a, bs, g, r, 0, amax, sr);
             continue;
@@ -694,14 +694,22 @@ __device__ __forceinline__ void pairprep_whole(const PairPrepArgs &a, int64_t g,
                     ((uint32_t)(uint8_t)v2[3] << 24);
             }
         }
-        sum1 += xor_lane<32>(sum1), sum2 += xor_lane<32>(sum2);
-        sum1 += xor_lane<16>(sum1), sum2 += xor_lane<16>(sum2);
-        sum1 += xor_lane<8>(sum1), sum2 += xor_lane<8>(sum2);
-        sum1 += xor_lane<4>(sum1), sum2 += xor_lane<4>(sum2);
-        sum1 += xor_lane<2>(sum1), sum2 += xor_lane<2>(sum2);
-        sum1 += xor_lane<1>(sum1), sum2 += xor_lane<1>(sum2);
+        if (bs.bits >= 8) {  // the code shift's correction: only 8 and more bits are stored shifted
+            sum1 += xor_lane<32>(sum1), sum2 += xor_lane<32>(sum2);
+            sum1 += xor_lane<16>(sum1), sum2 += xor_lane<16>(sum2);
+            sum1 += xor_lane<8>(sum1), sum2 += xor_lane<8>(sum2);
+            sum1 += xor_lane<4>(sum1), sum2 += xor_lane<4>(sum2);
+            sum1 += xor_lane<2>(sum1), sum2 += xor_lane<2>(sum2);
+            sum1 += xor_lane<1>(sum1), sum2 += xor_lane<1>(sum2);
+        }
         if (lane == 0) write_pair_consts(a, bs, g, s1, s2, sr, sum1, sum2);
     }
+    q2 += xor_f<32>(q2);
+    q2 += xor_f<16>(q2);
+    q2 += xor_f<8>(q2);
+    q2 += xor_f<4>(q2);
+    q2 += xor_f<2>(q2);
+    q2 += xor_f<1>(q2);
     if (lane == 0) a.pq2[g] = q2;
 }
 
