@@ -1510,6 +1510,9 @@ int search_batched_locked(saqgpu_index *ix, const float *queries, int64_t nq, co
     const BatchedView bv = batched_view(ix);
     int max_smem = 0;
     HIPCHK(hipDeviceGetAttribute(&max_smem, hipDeviceAttributeMaxSharedMemoryPerBlock, ix->device));
+    // a tile the caller left to the library shrinks until its per-query lists fit in LDS (large k)
+    if (p->query_tile == 0)
+        while (query_tile > 16 && (int64_t)bscan_smem_bytes(bv, topk, query_tile, qs) > max_smem) query_tile /= 2;
     if ((int64_t)bscan_smem_bytes(bv, topk, query_tile, qs) > max_smem)
         return fail(SAQGPU_EUNSUPPORTED, "query tile needs more LDS than a workgroup has; use a smaller query_tile");
     const int64_t npairs = nq * NP;

@@ -443,7 +443,8 @@ __device__ __forceinline__ void bitonic_steps(float &d, uint32_t &id, int lane) 
     }
 }
 
-__device__ __forceinline__ void merge_into64(float &d, uint32_t &id, float cd, uint32_t ci) {
+// ascending sort of one pair per lane (a full bitonic sort network)
+__device__ __forceinline__ void sort64(float &cd, uint32_t &ci) {
     const int lane = threadIdx.x & 63;
     bitonic_steps<2, 1>(cd, ci, lane);
     bitonic_steps<4, 2>(cd, ci, lane);
@@ -451,19 +452,56 @@ __device__ __forceinline__ void merge_into64(float &d, uint32_t &id, float cd, u
     bitonic_steps<16, 8>(cd, ci, lane);
     bitonic_steps<32, 16>(cd, ci, lane);
     bitonic_steps<64, 32>(cd, ci, lane);
-    const float rd = __shfl(cd, 63 - lane);
-    const uint32_t ri = __shfl(ci, 63 - lane);
-    if (dless(rd, ri, d, id)) {
-        d = rd;
-        id = ri;
-    }
-    // the 64 smallest of both, a bitonic sequence: sort ascending
+}
+
+// ascending order of a bitonic sequence of one pair per lane (a bitonic merge)
+__device__ __forceinline__ void merge64(float &d, uint32_t &id) {
+    const int lane = threadIdx.x & 63;
     cswap_lane<32>(d, id, (lane & 32) == 0);
     cswap_lane<16>(d, id, (lane & 16) == 0);
     cswap_lane<8>(d, id, (lane & 8) == 0);
     cswap_lane<4>(d, id, (lane & 4) == 0);
     cswap_lane<2>(d, id, (lane & 2) == 0);
     cswap_lane<1>(d, id, (lane & 1) == 0);
+}
+
+__device__ __forceinline__ void reverse64(float &d, uint32_t &id) {
+    const int lane = threadIdx.x & 63;
+    d = __shfl(d, 63 - lane);
+    id = (uint32_t)__shfl((int)id, 63 - lane);
+}
+
+// The pairwise-smaller of (a, ai) and (b, bi) into a, the larger into b.
+__device__ __forceinline__ void order_pair(float &a, uint32_t &ai, float &b, uint32_t &bi) {
+    if (dless(b, bi, a, ai)) {
+        const float t = a;
+        const uint32_t ti = ai;
+        a = b;
+        ai = bi;
+        b = t;
+        bi = ti;
+    }
+}
+
+// A sorted list of 64 KR pairs (entry 64 r + lane in d[r] of lane; KR = 1 or 2) takes up to 64
+// candidates (one per lane, (INF, 0xFFFFFFFF) for none) and keeps the 64 KR smallest of both,
+// sorted.  The candidates are sorted and reversed; against the list's last 64 entries L they form
+// a bitonic sequence whose pairwise-smaller half Y holds the 64 smallest of L and the candidates
+// (the larger half lies above all of them and above the list's first 64 entries, so it is
+// dropped).  SORTED: the candidates arrive sorted.  For KR = 2, Y is sorted, reversed and split against the first 64 entries the same
+// way into the new first and last 64 entries.
+template <int KR, bool SORTED = false>
+__device__ __forceinline__ void merge_into(float (&d)[KR], uint32_t (&id)[KR], float cd, uint32_t ci) {
+    if (!SORTED) sort64(cd, ci);
+    reverse64(cd, ci);
+    order_pair(d[KR - 1], id[KR - 1], cd, ci);
+    merge64(d[KR - 1], id[KR - 1]);
+    if constexpr (KR == 2) {
+        reverse64(d[1], id[1]);
+        order_pair(d[0], id[0], d[1], id[1]);
+        merge64(d[0], id[0]);
+        merge64(d[1], id[1]);
+    }
 }
 
 // ------------------------------------------------------------------ pair preparation
@@ -833,7 +871,7 @@ __host__ __device__ inline size_t wave_layout(int nqs, int topk, uint8_t *base, 
     s.tkd = (float *)take((size_t)QT * topk * 4);
     s.tki = (uint32_t *)take((size_t)QT * topk * 4);
     s.fac = (float *)take((size_t)2 * nvw_of(QTM, QS) * 64 * (2 + nqs));
-    const int cap = topk <= 64 ? cand_cap(QTM) : 0;
+    const int cap = topk <= 128 ? cand_cap(QTM) : 0;
     s.cbd = (float *)take((size_t)QT * cap * 4);
     s.cbi = (uint32_t *)take((size_t)QT * cap * 4);
     s.cbn = (int32_t *)take((size_t)QT * 4);
@@ -859,16 +897,46 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_wave_barrier();
 }
 
-// The row's sorted list (k <= 64 places, padded) back to LDS with its k-th pair; while the list
-// is not full (its k-th place still +inf) the threshold stays the query's bound.
-__device__ __forceinline__ void store_row(const WSmem &m, int i, int topk, float d, uint32_t id) {
+// Entry e of a list held as KR registers per lane (entry 64 r + lane in register r), read in
+// every lane; e is wave-uniform.
+template <int KR>
+__device__ __forceinline__ void list_entry(const float (&d)[KR], const uint32_t (&id)[KR], int e, float &x,
+                                           uint32_t &xi) {
+    const int r = KR == 1 ? 0 : e >> 6;
+    x = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r ? d[KR - 1] : d[0]), e & 63));
+    xi = (uint32_t)__builtin_amdgcn_readlane((int)(r ? id[KR - 1] : id[0]), e & 63);
+}
+
+// Row i's sorted list (topk <= 64 KR places) from LDS into registers; places past topk hold
+// (INF, 0xFFFFFFFF).
+template <int KR>
+__device__ __forceinline__ void load_row(const WSmem &m, int i, int topk, float (&d)[KR], uint32_t (&id)[KR]) {
     const int lane = threadIdx.x & 63;
-    if (lane < topk) {
-        m.tkd[i * topk + lane] = d;
-        m.tki[i * topk + lane] = id;
+#pragma unroll
+    for (int r = 0; r < KR; ++r) {
+        const int e = 64 * r + lane;
+        d[r] = e < topk ? m.tkd[i * topk + e] : INFINITY;
+        id[r] = e < topk ? m.tki[i * topk + e] : 0xFFFFFFFFu;
     }
-    float kd = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(d), topk - 1));
-    uint32_t ki = (uint32_t)__builtin_amdgcn_readlane((int)id, topk - 1);
+}
+
+// The row's sorted list back to LDS with its k-th pair; while the list is not full (its k-th
+// place still +inf) the threshold stays the query's bound.
+template <int KR>
+__device__ __forceinline__ void store_row(const WSmem &m, int i, int topk, const float (&d)[KR],
+                                          const uint32_t (&id)[KR]) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int r = 0; r < KR; ++r) {
+        const int e = 64 * r + lane;
+        if (e < topk) {
+            m.tkd[i * topk + e] = d[r];
+            m.tki[i * topk + e] = id[r];
+        }
+    }
+    float kd;
+    uint32_t ki;
+    list_entry<KR>(d, id, topk - 1, kd, ki);
     if (kd == INFINITY) {
         kd = m.rb[i];
         ki = 0xFFFFFFFFu;
@@ -881,14 +949,16 @@ __device__ __forceinline__ void store_row(const WSmem &m, int i, int topk, float
 }
 
 // Merges row i's nb buffered candidates into its list and empties the buffer: up to
-// kSeqInsert of them one by one (a position ballot and a one-lane shift each), more by the
+// kSeqInsert of them one by one (a position ballot and a one-place shift each), more by the
 // bitonic merge.
 constexpr int kSeqInsert = 8;
 
+template <int KR>
 __device__ __forceinline__ void flush_row(const WSmem &m, int i, int topk, int nb, int cap) {
     const int lane = threadIdx.x & 63;
-    float d = lane < topk ? m.tkd[i * topk + lane] : INFINITY;
-    uint32_t id = lane < topk ? m.tki[i * topk + lane] : 0xFFFFFFFFu;
+    float d[KR];
+    uint32_t id[KR];
+    load_row<KR>(m, i, topk, d, id);
     if (nb <= kSeqInsert) {
         float kd = m.kd[i];
         uint32_t ki = m.ki[i];
@@ -896,25 +966,41 @@ __device__ __forceinline__ void flush_row(const WSmem &m, int i, int topk, int n
             const float x = m.cbd[i * cap + e];
             const uint32_t xi = m.cbi[i * cap + e];
             if (!dless(x, xi, kd, ki)) continue;
-            const int pos = __popcll(__ballot(dless(d, id, x, xi)));
-            const float ud = __shfl_up(d, 1);
-            const uint32_t ui = __shfl_up(id, 1);
-            if (lane > pos) {
-                d = ud;
-                id = ui;
-            } else if (lane == pos) {
-                d = x;
-                id = xi;
+            int pos = 0;
+#pragma unroll
+            for (int r = 0; r < KR; ++r) pos += __popcll(__ballot(dless(d[r], id[r], x, xi)));
+            // entry 64 r + lane takes entry 64 r + lane - 1 above pos
+            float carry = 0.0f;
+            uint32_t carryi = 0u;
+#pragma unroll
+            for (int r = 0; r < KR; ++r) {
+                float ud = __shfl_up(d[r], 1);
+                uint32_t ui = (uint32_t)__shfl_up((int)id[r], 1);
+                const float top = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(d[r]), 63));
+                const uint32_t topi = (uint32_t)__builtin_amdgcn_readlane((int)id[r], 63);
+                if (lane == 0) {
+                    ud = carry;
+                    ui = carryi;
+                }
+                carry = top;
+                carryi = topi;
+                const int ent = 64 * r + lane;
+                if (ent > pos) {
+                    d[r] = ud;
+                    id[r] = ui;
+                } else if (ent == pos) {
+                    d[r] = x;
+                    id[r] = xi;
+                }
             }
-            kd = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(d), topk - 1));
-            ki = (uint32_t)__builtin_amdgcn_readlane((int)id, topk - 1);
+            list_entry<KR>(d, id, topk - 1, kd, ki);
         }
     } else {
         const float cd = lane < nb ? m.cbd[i * cap + lane] : INFINITY;
         const uint32_t ci = lane < nb ? m.cbi[i * cap + lane] : 0xFFFFFFFFu;
-        merge_into64(d, id, cd, ci);
+        merge_into<KR>(d, id, cd, ci);
     }
-    store_row(m, i, topk, d, id);
+    store_row<KR>(m, i, topk, d, id);
     if (lane == 0) m.cbn[i] = 0;
     wave_sync();
 }
@@ -1121,15 +1207,15 @@ __global__ __launch_bounds__(64 * W, (W * QTM >= 8 || (QTM == 4 && QS == 2)) ? 1
         for (int row = lane; row < QT; row += 64) {
             const int g = first + min(row, cnt - 1);
             m.q2[row] = a.pq2[g];
-            // top-k lists (k <= 64) start from the query's bound; longer ones from +inf
-            const float b = KR == 1 ? key_float(a.qbound[a.pairs[g] / a.NP]) : INFINITY;
+            // top-k lists (k <= 128) start from the query's bound; longer ones from +inf
+            const float b = KR <= 2 ? key_float(a.qbound[a.pairs[g] / a.NP]) : INFINITY;
             m.kd[row] = b;
             m.rb[row] = b;
             m.ki[row] = 0xFFFFFFFFu;
             m.tkn[row] = 0;
             m.cbn[row] = 0;
         }
-        if (KR == 1)
+        if (KR <= 2)
             for (int e = lane; e < QT * a.topk; e += 64) {
                 m.tkd[e] = INFINITY;
                 m.tki[e] = 0xFFFFFFFFu;
@@ -1276,7 +1362,7 @@ __global__ __launch_bounds__(64 * W, (W * QTM >= 8 || (QTM == 4 && QS == 2)) ? 1
             }
             wave_sync();
             const float x_id_pad = INFINITY;
-            while (KR == 1 && rows) {
+            while (KR <= 2 && rows) {
                 // buffered: candidates that beat the row's k-th pair are appended to its buffer;
                 // a full buffer is merged into the list (which lowers the k-th pair)
                 const int i = __builtin_ctzll(rows);
@@ -1289,15 +1375,16 @@ __global__ __launch_bounds__(64 * W, (W * QTM >= 8 || (QTM == 4 && QS == 2)) ? 1
                 constexpr int kCap = cand_cap(QTM);
                 int nb = m.cbn[i];
                 if (nb + __popcll(b) > kCap) {
-                    if (nb) flush_row(m, i, a.topk, nb, kCap);
+                    if (nb) flush_row<KR>(m, i, a.topk, nb, kCap);
                     nb = 0;
                     pass = dless(x, xi, m.kd[i], m.ki[i]);
                     b = __ballot(pass);
                     if (__popcll(b) > kCap) {  // more than a buffer holds: merge the chunk's row directly
-                        float d = lane < a.topk ? m.tkd[i * a.topk + lane] : INFINITY;
-                        uint32_t id = lane < a.topk ? m.tki[i * a.topk + lane] : 0xFFFFFFFFu;
-                        merge_into64(d, id, pass ? x : INFINITY, pass ? xi : 0xFFFFFFFFu);
-                        store_row(m, i, a.topk, d, id);
+                        float d[KR];
+                        uint32_t id[KR];
+                        load_row<KR>(m, i, a.topk, d, id);
+                        merge_into<KR>(d, id, pass ? x : INFINITY, pass ? xi : 0xFFFFFFFFu);
+                        store_row<KR>(m, i, a.topk, d, id);
                         b = 0;
                     }
                 }
@@ -1310,7 +1397,7 @@ __global__ __launch_bounds__(64 * W, (W * QTM >= 8 || (QTM == 4 && QS == 2)) ? 1
                 if (lane == 0) m.cbn[i] = nb + __popcll(b);
                 wave_sync();
             }
-            while (KR > 1 && rows) {
+            while (KR > 2 && rows) {
                 const int i = __builtin_ctzll(rows);
                 rows &= rows - 1;
                 const float x = lane < CVW ? m.dt[i * CVW + lane] : x_id_pad;
@@ -1331,14 +1418,14 @@ __global__ __launch_bounds__(64 * W, (W * QTM >= 8 || (QTM == 4 && QS == 2)) ? 1
             wave_sync();  // the next chunk rewrites dt and dids
         }
         // ---- the wave's per-query lists to global memory
-        if (KR == 1)
+        if (KR <= 2)
             for (int i = 0; i < cnt; ++i) {
                 const int nb = m.cbn[i];
-                if (nb) flush_row(m, i, a.topk, nb, cand_cap(QTM));
+                if (nb) flush_row<KR>(m, i, a.topk, nb, cand_cap(QTM));
             }
         for (int i = 0; i < cnt; ++i) {
             const int64_t pr = (int64_t)a.tile_cand[tile] + w * QT + i;
-            if (KR == 1) {  // padded lists: unused places already hold (INF, 0xFFFFFFFF)
+            if (KR <= 2) {  // padded lists: unused places already hold (INF, 0xFFFFFFFF)
                 for (int e = lane; e < a.topk; e += 64) {
                     a.cand_d[pr * a.topk + e] = m.tkd[i * a.topk + e];
                     a.cand_id[pr * a.topk + e] = m.tki[i * a.topk + e];
@@ -1420,7 +1507,11 @@ __global__ __launch_bounds__(256) void bq_merge_kernel(const float *cand_d, cons
                     const int64_t L = __shfl(li, src < 0 ? 0 : src);
                     const float x = src >= 0 ? cand_d[L * topk + e] : INFINITY;
                     const uint32_t xi = src >= 0 ? cand_id[L * topk + e] : 0xFFFFFFFFu;
-                    merge_into64(d, id, x, xi);
+                    float dd[1] = {d};
+                    uint32_t ii[1] = {id};
+                    merge_into<1>(dd, ii, x, xi);
+                    d = dd[0];
+                    id = ii[0];
                     kd = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(d), topk - 1));
                     ki = (uint32_t)__builtin_amdgcn_readlane((int)id, topk - 1);
                     m &= __ballot(has && dless(hd, hi, kd, ki));
@@ -1431,6 +1522,62 @@ __global__ __launch_bounds__(256) void bq_merge_kernel(const float *cand_d, cons
             const bool ok = id != 0xFFFFFFFFu || d != INFINITY;
             out_ids[q * topk + lane] = ok ? id : 0u;
             if (out_d) out_d[q * topk + lane] = ok ? d : 0.0f;
+        }
+        return;
+    }
+    if (KR == 2) {
+        // 64 < k <= 128: every list whose head beats the k-th pair is merged 64 entries at a time
+        // (a chunk only while its head beats it) by the two-register bitonic merge
+        float d[2] = {INFINITY, INFINITY};
+        uint32_t id[2] = {0xFFFFFFFFu, 0xFFFFFFFFu};
+        float kd = INFINITY;
+        uint32_t ki = 0xFFFFFFFFu;
+        for (int p0 = 0; p0 < lim; p0 += 64) {
+            const int p = p0 + lane;
+            int ns = 0, cn = 0;
+            int64_t base = 0;
+            if (p < lim) {
+                const int64_t i = q * NP + p;
+                const int c = (int)probes[i];
+                ns = nsplit[c];
+                cn = counts[c];
+                base = (int64_t)candstart[c] + slot_of[i];
+            }
+            int rmax = ns;
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) rmax = max(rmax, __shfl_xor(rmax, o));
+            for (int r = 0; r < rmax; ++r) {
+                const int64_t li = base + (int64_t)r * cn;  // this lane's r-th list
+                const bool has = r < ns;
+                const float hd = has ? cand_d[li * topk] : INFINITY;
+                const uint32_t hi = has ? cand_id[li * topk] : 0xFFFFFFFFu;
+                unsigned long long m = __ballot(has && dless(hd, hi, kd, ki));
+                while (m) {
+                    const int src = __builtin_ctzll(m);
+                    m &= m - 1;
+                    const int64_t L = __shfl(li, src);
+                    for (int e0 = 0; e0 < topk; e0 += 64) {
+                        const int e = e0 + lane;
+                        const float x = e < topk ? cand_d[L * topk + e] : INFINITY;
+                        const uint32_t xi = e < topk ? cand_id[L * topk + e] : 0xFFFFFFFFu;
+                        const float x0 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
+                        const uint32_t xi0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)xi);
+                        if (!dless(x0, xi0, kd, ki)) break;
+                        merge_into<2, true>(d, id, x, xi);  // a list chunk is sorted
+                        list_entry<2>(d, id, topk - 1, kd, ki);
+                    }
+                    m &= __ballot(has && dless(hd, hi, kd, ki));
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int e = j * 64 + lane;
+            if (e < topk) {
+                const bool ok = id[j] != 0xFFFFFFFFu || d[j] != INFINITY;
+                out_ids[q * topk + e] = ok ? id[j] : 0u;
+                if (out_d) out_d[q * topk + e] = ok ? d[j] : 0.0f;
+            }
         }
         return;
     }

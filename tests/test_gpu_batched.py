@@ -72,7 +72,7 @@ def test_dp_plan_and_padded_dim():
     run(case, 10, 5, PREC_INT8X2)
 
 
-@pytest.mark.parametrize("topk", [1, 100, 200])
+@pytest.mark.parametrize("topk", [1, 65, 100, 128, 200])
 def test_topk_sizes_and_unfilled(topk):
     case = F.make_case(n=1200, d=128, K=8, nq=12, plan=[(64, 6), (64, 3)], seed=55)
     run(case, topk, 2, PREC_INT8X2)
@@ -88,8 +88,8 @@ def test_query_tiles_give_identical_results():
             assert np.array_equal(d.view(np.uint32), out[0][1].view(np.uint32))
 
 
-@pytest.mark.parametrize("topk,tiles,prec", [(10, (16, 64, 128), PREC_INT8), (100, (16, 32, 128), PREC_INT8),
-                                             (10, (16, 128), PREC_BF16)])
+@pytest.mark.parametrize("topk,tiles,prec", [(10, (16, 64, 128), PREC_INT8), (100, (16, 32, 64), PREC_INT8),
+                                             (128, (16, 64), PREC_INT8X2), (10, (16, 128), PREC_BF16)])
 def test_long_lists_split_into_vector_ranges(topk, tiles, prec):
     """Lists of ~4000 vectors: each (list, query tile) becomes 16 work tiles of 256-vector
     ranges with candidate lists of their own, which the merge reads back per pair."""
