@@ -472,7 +472,7 @@ int saqgpu_index_create(const saqgpu_index_desc *d, saqgpu_index **out) {
     }
     ix->total_blocks = blk;
     int rc = 0;
-    rc |= ix->blocks.alloc(std::max<int64_t>(1, blk) * ix->block_stride);
+    rc |= ix->blocks.alloc(std::max<int64_t>(1, blk) * ix->block_stride + kBlockTailPad);
     rc |= ix->long_codes.alloc(std::max<int64_t>(1, vec) * std::max<int64_t>(16, ix->long_stride));
     rc |= ix->exfac.alloc(std::max<int64_t>(1, vec) * ix->nseg * 2);
     rc |= ix->ipcent.alloc(std::max<int64_t>(1, blk) * ix->nseg * kBlock);

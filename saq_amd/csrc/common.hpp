@@ -23,6 +23,10 @@
 namespace saqgpu {
 
 constexpr int kBlock = 32;
+// Bytes past the last block record the block array keeps readable: the list scan loads the
+// short codes of 8 groups at a time and one step ahead without clamping at the record's end,
+// so a load may reach up to 15 groups (3840 bytes) past a record; those values are never summed.
+constexpr int kBlockTailPad = 4096;
 constexpr int kMaxSeg = 64;
 
 struct SegDesc {

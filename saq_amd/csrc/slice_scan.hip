@@ -52,6 +52,23 @@ __device__ __forceinline__ uint32_t wave_shr1(uint32_t v, uint32_t fill) {
     return (uint32_t)__builtin_amdgcn_update_dpp((int)fill, (int)v, 0x138 /* wave_shr:1 */, 0xF, 0xF, false);
 }
 
+// The minimum over each 32-lane half of the wave, in every lane of it: DPP quad permutations
+// (lane ^ 1, lane ^ 2), the half-row and row mirrors (which pair the quads and then the half-rows
+// once each holds its minimum), then lane ^ 16 by ds_swizzle.  A compare and select per step:
+// the values are finite, and fminf would canonicalise every moved operand first.
+__device__ __forceinline__ float min32(float v) {
+    float t = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));  // [1,0,3,2]
+    v = t < v ? t : v;
+    t = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, false));  // [2,3,0,1]
+    v = t < v ? t : v;
+    t = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x141, 0xF, 0xF, false));  // row_half_mirror
+    v = t < v ? t : v;
+    t = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x140, 0xF, 0xF, false));  // row_mirror
+    v = t < v ? t : v;
+    t = __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x401F));  // lane ^ 16
+    return t < v ? t : v;
+}
+
 // ---------------------------------------------------------------- list scan
 
 struct LSmem {
@@ -80,7 +97,8 @@ __device__ __host__ inline size_t ls_layout(unsigned char *base, const IndexView
     m.sc = reinterpret_cast<float *>(take((size_t)S * SC_N * 4));
     m.cbmin = reinterpret_cast<float *>(take((size_t)ix.ngroups * 16 * 4));
     m.cbmax = reinterpret_cast<float *>(take((size_t)ix.ngroups * 16 * 4));
-    m.lut = reinterpret_cast<uint16_t *>(take((size_t)ix.ngroups * 256 * 2));
+    // code groups rounded up to 8, the extra groups' tables zero (the scan sums 8 groups at a time)
+    m.lut = reinterpret_cast<uint16_t *>(take((size_t)((ix.ngroups + 7) & ~7) * 256 * 2));
     m.xs = reinterpret_cast<float *>(take((size_t)S * kLT * 4));
     m.ipq = reinterpret_cast<float *>(take((size_t)S * kLT * 4));
     m.ctl = reinterpret_cast<int *>(take(16 * 4));
@@ -186,6 +204,10 @@ __global__ __launch_bounds__(kLT) void list_scan_kernel(SliceArgs a) {
         const float raw = size == K ? a.pool_d[q * K + K - 1] : FLT_MAX;
         T = DT == 1 ? -raw : raw;
     }
+    // zero tables for the code groups past NG up to a multiple of 8 (prepare_list's barriers order
+    // these stores before the scan)
+    const int NG8 = (NG + 7) & ~7;
+    for (int i = tid; i < (NG8 - NG) * 128; i += kLT) reinterpret_cast<uint32_t *>(m.lut + NG * 256)[i] = 0u;
     // CaqCluEstimator::prepare of every segment (caq_estimator.hpp:94-106)
     prepare_list<DT>(m.segs, S, m.gseg, NG, a.qrot + q * ix.D_pad, ix.crot + (int64_t)cid * ix.D_pad, a.bound + q * S,
                      m.r, m.sc, m.cbmin, m.cbmax, m.lut, ix.D_pad);
@@ -203,37 +225,20 @@ __global__ __launch_bounds__(kLT) void list_scan_kernel(SliceArgs a) {
         if (active) {
             const uint8_t *rec = blk0 + (int64_t)(st + bl) * ix.block_stride;
             const float *fac = reinterpret_cast<const float *>(rec);
-            {  // |o| of every segment: the loads first, then the stores
-                float xv[8];
-#pragma unroll
-                for (int u = 0; u < 8; ++u) xv[u] = u < S ? fac[u * kBlock + ln] : 0.0f;
-#pragma unroll
-                for (int u = 0; u < 8; ++u)
-                    if (u < S) m.xs[u * kLT + v] = xv[u];
-                for (int s = 8; s < S; ++s) m.xs[s * kLT + v] = fac[s * kBlock + ln];
-            }
-            const uint8_t *codes = rec + ix.fac_floats * 4 + ln * 8;
+            // this lane's 8-byte word of code group g at cw[32 g]
+            const uint2 *cw = reinterpret_cast<const uint2 *>(rec + ix.fac_floats * 4) + ln;
             uint32_t acc = 0;
             int bk = 0;
-            // the code words of 8 groups at a time, the next 8 loading while these are summed
-            // (indices past NG repeat the last group and are not summed)
-            uint2 w[8], wn[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u)  // NG == 0 (every segment at 0 bits): nothing to load
-                w[u] = NG > 0 ? *reinterpret_cast<const uint2 *>(codes + min(u, NG - 1) * 256) : uint2{0u, 0u};
-            for (int g0 = 0; g0 < NG; g0 += 8) {
-#pragma unroll
-                for (int u = 0; u < 8; ++u)  // unconditional, so the waits below count only w's loads
-                    wn[u] = *reinterpret_cast<const uint2 *>(codes + min(g0 + 8 + u, NG - 1) * 256);
+            // the sums of code groups g0 .. g0 + 7 from their words ww
+            auto sum8 = [&](const uint2 (&ww)[8], int g0) {
 #pragma unroll
                 for (int u = 0; u < 8; ++u) {
                     const int g = g0 + u;
-                    if (g >= NG) break;
                     // byte b of xh / xl is twice the high / low nibble of code byte b: one byte
                     // extract and add per LUT address (the integer sum is order-free)
                     const uint8_t *t = reinterpret_cast<const uint8_t *>(m.lut + g * 256);
-                    uint32_t xh = (w[u].x >> 3) & 0x1E1E1E1Eu, xl = (w[u].x << 1) & 0x1E1E1E1Eu;
-                    uint32_t yh = (w[u].y >> 3) & 0x1E1E1E1Eu, yl = (w[u].y << 1) & 0x1E1E1E1Eu;
+                    uint32_t xh = (ww[u].x >> 3) & 0x1E1E1E1Eu, xl = (ww[u].x << 1) & 0x1E1E1E1Eu;
+                    uint32_t yh = (ww[u].y >> 3) & 0x1E1E1E1Eu, yl = (ww[u].y << 1) & 0x1E1E1E1Eu;
                     // keep the four masked words as they are (no instruction): otherwise the mask is
                     // folded back into a shift and an and per nibble
                     asm("" : "+v"(xh), "+v"(xl), "+v"(yh), "+v"(yl));
@@ -256,20 +261,49 @@ __global__ __launch_bounds__(kLT) void list_scan_kernel(SliceArgs a) {
                         acc = 0;
                     }
                 }
+            };
+            // the code words of 8 groups at a time in two register sets, the next 8 loading while
+            // these are summed.  Groups past NG (up to NG8 + 7) read the next records or the array's
+            // tail padding (kBlockTailPad): their tables are zero, or they are never summed.
+            uint2 wa[8], wb[8];
+            const uint2 *cp = cw;
 #pragma unroll
-                for (int u = 0; u < 8; ++u) w[u] = wn[u];
+            for (int u = 0; u < 8; ++u) wa[u] = cp[32 * u];
+            {  // |o| of every segment, loaded after the first code words: one wait covers both
+                float xv[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) xv[u] = u < S ? fac[u * kBlock + ln] : 0.0f;
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    if (u < S) m.xs[u * kLT + v] = xv[u];
+                for (int s = 8; s < S; ++s) m.xs[s * kLT + v] = fac[s * kBlock + ln];
+            }
+            for (int g0 = 0; g0 < NG8; g0 += 16) {
+                cp += 256;
+#pragma unroll
+                for (int u = 0; u < 8; ++u) wb[u] = cp[32 * u];
+                sum8(wa, g0);
+                if (g0 + 8 >= NG8) break;
+                cp += 256;
+#pragma unroll
+                for (int u = 0; u < 8; ++u) wa[u] = cp[32 * u];
+                sum8(wb, g0 + 8);
             }
         }
-        // ---- per-vector estimates, block minima and the candidate test against T
+        // ---- per-vector estimates, block minima and the candidate test against T.  Lanes of
+        // blocks past the chunk's end read stale |o| and sums: their block's results are never
+        // stored or flagged, and the block minima never mix two blocks.
         bool flag = false;
         float cur = 0.0f;
-        if (single) {
+        if (2 * wave >= nbs) {
+            // both of this wave's blocks lie past the chunk's end: no estimates, no candidates
+        } else if (single) {
             const float *sc = m.sc;
-            const float x = active ? m.xs[v] : 0.0f;
+            const float x = m.xs[v];
             if (m.segs[0].bits == 0) {
                 cur = DT == 1 ? sc[SC_VFAC] : max0(x * x + sc[SC_VFAC]);
             } else {
-                const float ipq = active ? m.ipq[v] : 0.0f;
+                const float ipq = m.ipq[v];
                 float f = (ipq - sc[SC_SUMQCB]) * m.segs[0].rescale_sqrtD;
                 f = f * x;
                 cur = DT == 1 ? f * 0.5f + sc[SC_IPQC] : max0(x * x + (sc[SC_QL2] - f));
@@ -280,35 +314,39 @@ __global__ __launch_bounds__(kLT) void list_scan_kernel(SliceArgs a) {
         } else {
             // stage 1: sum of the variance estimates (saq_searcher.hpp:95-110)
             for (int s = 0; s < S; ++s) {
-                const float x = active ? m.xs[s * kLT + v] : 0.0f;
+                const float x = m.xs[s * kLT + v];
                 const float vd = DT == 1 ? m.sc[s * SC_N + SC_VFAC] : max0(x * x + m.sc[s * SC_N + SC_VFAC]);
                 cur = cur + vd;
             }
-            float m1 = cur;
-            for (int o = 16; o >= 1; o >>= 1) m1 = fminf(m1, __shfl_xor(m1, o, 32));
+            const float m1 = min32(cur);
             float M = m1;
+            // the block-info words 2 .. S + 3 (m1, M, the segments' minima), one per lane of the
+            // block and stored together; with more than 30 segments each minimum is stored alone
+            const bool wide = S > 30;
+            uint32_t bw = ln == 0 ? __float_as_uint(m1) : 0u;
             // stage 2: each quantised segment's variance estimate replaced by its fast
             // estimate, in segment order (saq_searcher.hpp:112-132)
             for (int s = 0; s < S; ++s) {
                 if (m.segs[s].bits == 0) continue;
                 const float *sc = m.sc + s * SC_N;
-                const float x = active ? m.xs[s * kLT + v] : 0.0f;
+                const float x = m.xs[s * kLT + v];
                 const float vd = DT == 1 ? sc[SC_VFAC] : max0(x * x + sc[SC_VFAC]);
-                const float ipq = active ? m.ipq[s * kLT + v] : 0.0f;
+                const float ipq = m.ipq[s * kLT + v];
                 float f = (ipq - sc[SC_SUMQCB]) * m.segs[s].rescale_sqrtD;
                 f = f * x;
                 const float fd = DT == 1 ? f * 0.5f + sc[SC_IPQC] : max0(x * x + (sc[SC_QL2] - f));
                 cur = cur - vd;
                 cur = cur + fd;
-                float ms = cur;
-                for (int o = 16; o >= 1; o >>= 1) ms = fminf(ms, __shfl_xor(ms, o, 32));
-                M = fmaxf(M, ms);
-                if (active && ln == 0) binfo[(st + bl) * bst + 4 + s] = __float_as_uint(ms);
+                const float ms = min32(cur);
+                M = ms > M ? ms : M;
+                if (wide) {
+                    if (active && ln == 0) binfo[(st + bl) * bst + 4 + s] = __float_as_uint(ms);
+                } else if (ln == 2 + s) {
+                    bw = __float_as_uint(ms);
+                }
             }
-            if (active && ln == 0) {
-                binfo[(st + bl) * bst + 2] = __float_as_uint(m1);
-                binfo[(st + bl) * bst + 3] = __float_as_uint(M);
-            }
+            if (ln == 1) bw = __float_as_uint(M);
+            if (active && (wide ? ln < 2 : ln < S + 2)) binfo[(st + bl) * bst + 2 + ln] = bw;
             // a block is refined only if m1 <= distk and M <= distk, a vector only if
             // cur < distk (saq_searcher.hpp:105-158); distk <= T throughout the slice
             flag = active && idx < n && m1 <= T && M <= T && cur < T;
@@ -325,7 +363,7 @@ __global__ __launch_bounds__(kLT) void list_scan_kernel(SliceArgs a) {
         }
         // the chunk's candidates: `total` contiguous records and one descriptor per cmax of them
         // (a refine wavefront takes one descriptor in one round)
-        const int ndesc = (total + a.cmax - 1) / a.cmax;
+        const int ndesc = (total + a.cmax - 1) >> __builtin_ctz(a.cmax);  // cmax: a power of two
         if (tid == 0 && total) {
             const unsigned long long old =
                 atomicAdd(cnt64, ((unsigned long long)ndesc << 32) | (unsigned long long)total);
@@ -343,25 +381,29 @@ __global__ __launch_bounds__(kLT) void list_scan_kernel(SliceArgs a) {
             // refine kernel's estimator state).
             const uint32_t dl = (uint32_t)m.ctl[2];
             const int per = kDescHead + kDescSc * S;
-            for (int i = tid; i < ndesc * per; i += kLT) {
-                const int k = i / per, w = i - k * per;
+            // wave w writes descriptors w, w + 4, ..., a lane per word
+            for (int k = wave; k < ndesc; k += 4) {
                 if (dl + k >= a.desc_cap) continue;
                 uint32_t *dd = a.desc + ((int64_t)slot * a.desc_cap + dl + k) * dstr;
-                uint32_t val;
-                if (w == 0) {
-                    val = (uint32_t)q;
-                } else if (w == 1) {
-                    val = cid;
-                } else if (w == 2) {
-                    val = cbase >= 0 ? (uint32_t)(cbase + k * a.cmax) : 0u;
-                } else if (w == 3) {
-                    val = cbase >= 0 ? (uint32_t)min(a.cmax, total - k * a.cmax) : 0u;
-                } else {
-                    constexpr int kSc[kDescSc] = {SC_SUMQCB, SC_QL2, SC_SUMQ, SC_VFAC, SC_IPQC};
-                    const int t = w - kDescHead, s = t / kDescSc, j = t - s * kDescSc;
-                    val = __float_as_uint(m.sc[s * SC_N + kSc[j]]);
+                for (int w = lane; w < per; w += 64) {
+                    uint32_t val;
+                    if (w == 0) {
+                        val = (uint32_t)q;
+                    } else if (w == 1) {
+                        val = cid;
+                    } else if (w == 2) {
+                        val = cbase >= 0 ? (uint32_t)(cbase + k * a.cmax) : 0u;
+                    } else if (w == 3) {
+                        val = cbase >= 0 ? (uint32_t)min(a.cmax, total - k * a.cmax) : 0u;
+                    } else {
+                        // SC_SUMQCB, SC_QL2, SC_SUMQ, SC_VFAC, SC_IPQC: consecutive scalars
+                        static_assert(SC_QL2 == SC_SUMQCB + 1 && SC_SUMQ == SC_SUMQCB + 2 &&
+                                      SC_VFAC == SC_SUMQCB + 3 && SC_IPQC == SC_SUMQCB + 4 && kDescSc == 5, "");
+                        const int t = w - kDescHead, s = t / kDescSc, j = t - s * kDescSc;
+                        val = __float_as_uint(m.sc[s * SC_N + SC_SUMQCB + j]);
+                    }
+                    dd[w] = val;
                 }
-                dd[w] = val;
             }
         }
         if (active && ln == 0) {
