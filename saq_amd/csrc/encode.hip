@@ -368,17 +368,61 @@ __global__ __launch_bounds__(256) void caq_kernel(EncArgs a) {
 
     double v_mx = 0, v_mi = 0, delta = 0, ip = 0, oa_l2sqr = 0;
     const int cm = (1 << (a.ori_qb ? a.ori_qb : bits)) - 1;
-    if (bits > 0) {
-        float mx = O(0), mn = O(0);
-        for (int j = 1; j < D; ++j) {
-            const float v = O(j);
-            mx = fmaxf(mx, v);
-            mn = fminf(mn, v);
+    // one pass over the residual: its max and min, and Eigen's redux (Redux.h:227-272) of o and
+    // of o * o (vec_sum, and |o|^2 for encode_and_fac) in redux_thread64's element order
+    float mx, mn, o_sum, o_sq;
+    {
+        float p0[16], p1[16], r0[16], r1[16];
+#pragma unroll
+        for (int l = 0; l < 16; ++l) {
+            const float x0 = O(l), x1 = O(16 + l);
+            p0[l] = x0;
+            p1[l] = x1;
+            r0[l] = x0 * x0;
+            r1[l] = x1 * x1;
         }
+        mx = mn = p0[0];
+#pragma unroll
+        for (int l = 0; l < 16; ++l) {
+            mx = fmaxf(fmaxf(mx, p0[l]), p1[l]);
+            mn = fminf(fminf(mn, p0[l]), p1[l]);
+        }
+        for (int idx = 32; idx < D; idx += 32) {
+#pragma unroll
+            for (int l = 0; l < 16; ++l) {
+                const float x0 = O(idx + l), x1 = O(idx + 16 + l);
+                mx = fmaxf(fmaxf(mx, x0), x1);
+                mn = fminf(fminf(mn, x0), x1);
+                p0[l] = p0[l] + x0;
+                p1[l] = p1[l] + x1;
+                r0[l] = r0[l] + x0 * x0;
+                r1[l] = r1[l] + x1 * x1;
+            }
+        }
+        float b[8], c[4], br[8], cr[4];
+#pragma unroll
+        for (int l = 0; l < 16; ++l) {
+            p0[l] = p0[l] + p1[l];
+            r0[l] = r0[l] + r1[l];
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            b[i] = p0[i] + p0[i + 8];
+            br[i] = r0[i] + r0[i + 8];
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            c[i] = b[i] + b[i + 4];
+            cr[i] = br[i] + br[i + 4];
+        }
+        o_sum = (c[0] + c[2]) + (c[1] + c[3]);
+        o_sq = (cr[0] + cr[2]) + (cr[1] + cr[3]);
+    }
+    if (bits > 0) {
         v_mx = (double)fmaxf(mx, -mn);
         v_mi = -v_mx;
         delta = (v_mx - v_mi) / (cm + 1);
-        const double vec_sum = (double)redux_thread64(D, O);
+        const double vec_sum = (double)o_sum;
         const float fvmi = (float)v_mi, finv = 1.0f / (float)delta, fcm = (float)cm;
         double ip_o_code = 0;
         unsigned long long code_l2sqr = 0;
@@ -407,6 +451,9 @@ __global__ __launch_bounds__(256) void caq_kernel(EncArgs a) {
                     break;
                 }
                 adj_cnt = 0;
+                // the exact recompute between rounds (caq_encoder.hpp:126-140) summed as each
+                // dim's code becomes final for the round: the same terms in the same order
+                double chk_oa = 0, chk_ip = 0;
                 // 8 dims' residuals and codes are loaded together ahead of the sequential updates
                 // (D is a multiple of 64)
                 for (int j8 = 0; j8 < D; j8 += 8) {
@@ -449,25 +496,12 @@ __global__ __launch_bounds__(256) void caq_kernel(EncArgs a) {
                         code[(int64_t)j * ld] = (uint16_t)c;
                         adj_cnt += cur;
                     }
+                    const double q = (c + 0.5) * delta + v_mi;
+                    chk_ip += q * (double)oj;
+                    chk_oa += q * q;
                   }
                 }
                 if (a.rd_lmt && round >= a.rd_lmt) break;
-                double chk_oa = 0, chk_ip = 0;
-                for (int j8 = 0; j8 < D; j8 += 8) {
-                    float ov[8];
-                    int cv[8];
-#pragma unroll
-                    for (int u = 0; u < 8; ++u) {
-                        ov[u] = O(j8 + u);
-                        cv[u] = code[(int64_t)(j8 + u) * ld];
-                    }
-#pragma unroll
-                    for (int u = 0; u < 8; ++u) {
-                        const double q = (cv[u] + 0.5) * delta + v_mi;
-                        chk_ip += q * (double)ov[u];
-                        chk_oa += q * q;
-                    }
-                }
                 oa_l2sqr = chk_oa;
                 ip = chk_ip;
             }
@@ -487,10 +521,7 @@ __global__ __launch_bounds__(256) void caq_kernel(EncArgs a) {
         }
     }
     // encode_and_fac (caq_encoder.hpp:222-247)
-    const double o_l2sqr = (double)redux_thread64(D, [&](int j) {
-        const float v = O(j);
-        return v * v;
-    });
+    const double o_l2sqr = (double)o_sq;
     const double o_l2norm = sqrt(o_l2sqr);
     double fac_rescale = ip != 0 ? o_l2sqr / ip : 0;
     const double fac_error = o_l2sqr * (double)1.9f * sqrt((((o_l2sqr * oa_l2sqr) / (ip * ip)) - 1) / (double)(D - 1));
