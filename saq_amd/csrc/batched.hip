@@ -168,7 +168,7 @@ __global__ void bq_zero_kernel(int32_t *counts, int K, int32_t *work, uint32_t *
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < K; i += gridDim.x * blockDim.x) counts[i] = 0;
     for (int64_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += gridDim.x * blockDim.x)
         qbound[i] = 0xFFFFFFFFu;
-    if (blockIdx.x == 0 && threadIdx.x == 0) *work = 0;
+    if (blockIdx.x == 0 && threadIdx.x < kXcds) work[threadIdx.x] = 0;
 }
 
 // (q, p) -> slot within its list's pair group
@@ -802,7 +802,7 @@ struct BScanArgs {
     const float *pq2;
     const int32_t *tile_clu, *tile_first, *tile_cnt, *tile_v0, *tile_cand;
     const int32_t *ntiles, *sv;
-    int32_t *work;  // tile counter, zero at launch
+    int32_t *work;  // kXcds tile counters, zero at launch
     const int32_t *pairs;  // grouped pair -> q * NP + p
     int NP;
     uint32_t *qbound;      // per query: an upper bound of its k-th distance (float_key), shared by its tiles
@@ -838,10 +838,14 @@ struct WSmem {
     int32_t *cbn;    // [QT]
 };
 
-// mblocks per wave chunk: 32 vectors per chunk for int8 (fewer registers: 3 waves per SIMD, 20%
-// faster on the 10M x 1024 workload than 64 at 2 waves); int8x2 (a second term) and BF16 (query
-// operands twice as wide) at larger query tiles take half that
-__host__ __device__ constexpr int nvw_of(int qtm, int qs) { return qs >= 2 ? (qtm >= 2 ? 1 : 2) : (qtm <= 2 ? 2 : 1); }
+// mblocks per wave chunk.  A wave re-reads its query operands (16 rows x sum D_s int8 per row
+// tile) from global memory for every chunk, so the longer the chunk the fewer of those reads:
+// int8 waves of one row tile take 64 vectors (196 VGPRs, 2 waves per SIMD; 32-query tiles 10.4
+// ms against 11.9 ms for the best 32-vector form on 10M x 1024), waves of two row tiles 32;
+// int8x2 (a second term) and BF16 (query operands twice as wide) take 32, or 16 at two row tiles
+__host__ __device__ constexpr int nvw_of(int qtm, int qs) {
+    return qs >= 2 ? (qtm >= 2 ? 1 : 2) : (qtm == 1 ? 4 : (qtm <= 2 ? 2 : 1));
+}
 
 // candidate buffer places per row: 64 for 16-query tiles, fewer for larger tiles (LDS)
 __host__ __device__ constexpr int cand_cap(int qtm) { return 64 / qtm; }
@@ -1177,17 +1181,18 @@ __global__ __launch_bounds__(64 * W, (W * QTM >= 8 || (QTM == 4 && QS == 2)) ? 1
     wave_layout<QTM, QS>(bv.nqs, a.topk, smem_raw + w * per_wave, m);
     int32_t *tile_s = reinterpret_cast<int32_t *>(smem_raw + W * per_wave);
     const int ntiles = *a.ntiles, sv = *a.sv;
+    // this workgroup's XCD and its tile stream: the k-th tile of stream x is run k / kTileRun of
+    // the stream's runs (global runs x, x + nx, x + 2 nx, ...), position k % kTileRun in it -- every
+    // tile index belongs to exactly one (stream, k)
+    const int nx = min(kXcds, (int)gridDim.x), xcd = (int)(blockIdx.x % nx);
+    auto stream_tile = [&](int k) { return ((k / kTileRun) * nx + xcd) * kTileRun + k % kTileRun; };
     for (int round = 0;; ++round) {
-        // the first tile of every workgroup is its own index; later ones come from the counter
         int tile;
         if (W == 1) {
-            tile = blockIdx.x;
-            if (round > 0) {
-                if (lane == 0) tile = gridDim.x + atomicAdd(a.work, 1);
-                tile = __shfl(tile, 0);
-            }
+            if (lane == 0) tile = stream_tile(atomicAdd(a.work + xcd, 1));
+            tile = __shfl(tile, 0);
         } else {
-            if (threadIdx.x == 0) *tile_s = round == 0 ? (int)blockIdx.x : (int)gridDim.x + atomicAdd(a.work, 1);
+            if (threadIdx.x == 0) *tile_s = stream_tile(atomicAdd(a.work + xcd, 1));
             __syncthreads();
             tile = *tile_s;
             __syncthreads();  // every wave has read it before the next round writes it
@@ -1630,13 +1635,14 @@ __global__ __launch_bounds__(256) void bq_merge_kernel(const float *cand_d, cons
 // ------------------------------------------------------------------ host-side launchers
 // The (waves per workgroup, query tiles of 16 per wave) of a query tile of qt queries.
 static void tile_shape(int qt, int qs, int &W, int &qtm) {
-    // (int8 64 as 2 waves x 2 row tiles: 11.9 ms against 15.7 ms as 4 waves x 1 on 10M x 1024; the
-    // wider query forms keep 4 x 1 there, 21 ms against 25-31 ms; 4 row tiles per wave hold 2 waves
-    // per SIMD and ran 19-21 ms)
+    // (one row tile per wave up to 64 queries: with 64-vector chunks the int8 tiles of 16 / 32 / 64
+    // queries ran 11.8 / 10.5 / 11.2 ms on 10M x 1024, 2 waves x 2 row tiles of 32-vector chunks
+    // 11.9 ms; the wider query forms 21 ms as 4 x 1 against 25-31 ms; 4 row tiles per wave hold 2
+    // waves per SIMD and ran 19-21 ms)
     switch (qt) {
     case 16: W = 1, qtm = 1; break;
     case 32: W = 2, qtm = 1; break;
-    case 64: W = qs == 1 ? 2 : 4, qtm = qs == 1 ? 2 : 1; break;
+    case 64: W = 4, qtm = 1; break;
     default: W = 4, qtm = 2; break;  // 128
     }
 }
@@ -1766,7 +1772,7 @@ hipError_t launch_bscan_tiles(const BatchedView &bv, const BqWork &w, const BqPa
     switch (qt) {
     case 16: return qs == 2 ? launch_bscan_q<1, 1, 2>(a, grid, st) : launch_bscan_q<1, 1, 1>(a, grid, st);
     case 32: return qs == 2 ? launch_bscan_q<2, 1, 2>(a, grid, st) : launch_bscan_q<2, 1, 1>(a, grid, st);
-    case 64: return qs == 2 ? launch_bscan_q<4, 1, 2>(a, grid, st) : launch_bscan_q<2, 2, 1>(a, grid, st);
+    case 64: return qs == 2 ? launch_bscan_q<4, 1, 2>(a, grid, st) : launch_bscan_q<4, 1, 1>(a, grid, st);
     default: return qs == 2 ? launch_bscan_q<4, 2, 2>(a, grid, st) : launch_bscan_q<4, 2, 1>(a, grid, st);
     }
 }
