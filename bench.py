@@ -738,8 +738,9 @@ def main():
     bdt, bkern_ms = timed(lambda: bsearch(bnp, bprec, bqt))
     bqps = nq * steps / bdt
     # algorithmic work of one tile-kernel launch: per work tile (a list and up to bqt of its
-    # queries) one pass over the list's 16-vector records; the residual queries (q_rot, c_rot)
-    # and the per-pair candidate lists; MFMA ops = 2 * K per (query, vector) pair and term
+    # queries) one pass over the list's 16-vector records; each pair's query operand row (int8:
+    # one byte per quantised dimension; int8x2 and BF16 two) and constants read once; the per-pair
+    # candidate lists; MFMA ops = 2 * K per (query, vector) pair and term
     pr = ix.probe(queries.cpu().numpy(), bnp).astype(np.int64)  # exact lists; the fast ones differ only at ties
     if npq_np is not None:
         pr = np.concatenate([pr[i, :npq_np[i]] for i in range(nq)])
@@ -749,7 +750,8 @@ def main():
     nqs = sum(1 for _, b in plan if b > 0)
     rstride = (kq_bytes + 64 * (2 + nqs) + 255) // 256 * 256
     tiles = (cnt + bqt - 1) // bqt
-    b_bytes = int((tiles * nmb).sum() * rstride + pr.size * (2 * d_pad * 4 + args.topk * 8))
+    a_row = sum(d for d, b in plan if b > 0) * (1 if bprec == saq_amd.PREC_INT8 else 2)
+    b_bytes = int((tiles * nmb).sum() * rstride + pr.size * (a_row + (3 * nqs + 1) * 4 + args.topk * 8))
     # the int8 MFMA work the estimator needs: 2 * sum D_s (quantised segments) per (query, probed
     # vector) pair; issued ops also count int8x2's second term and the 9+-bit codes' second plane
     pairs = float((cnt * sizes).sum())

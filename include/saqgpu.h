@@ -252,7 +252,9 @@ typedef struct {
     int32_t precision;                /* SAQGPU_PREC_INT8, SAQGPU_PREC_INT8X2 or SAQGPU_PREC_BF16 */
     int32_t query_tile;               /* queries per work tile: 16, 32, 64 or 128 (a workgroup of 1, 2, 4
                                          or 4 waves reading each code chunk once for them); 0 = the
-                                         autotuned choice */
+                                         autotuned choice, halved while the tile's per-query lists of
+                                         topk pairs do not fit in a workgroup's LDS (an explicit tile
+                                         that does not fit is SAQGPU_EUNSUPPORTED) */
     int32_t exact_probe;              /* with probes == NULL: 1 = the reference's coarse quantiser order
                                          (as saqgpu_probe), 0 = an fp32 GEMM's |c|^2 - 2 q.c (probe lists
                                          may differ where two centroid distances tie within fp32 rounding) */
