@@ -168,7 +168,7 @@ __global__ void bq_zero_kernel(int32_t *counts, int K, int32_t *work, uint32_t *
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < K; i += gridDim.x * blockDim.x) counts[i] = 0;
     for (int64_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += gridDim.x * blockDim.x)
         qbound[i] = 0xFFFFFFFFu;
-    if (blockIdx.x == 0 && threadIdx.x < kXcds) work[threadIdx.x] = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *work = 0;
 }
 
 // (q, p) -> slot within its list's pair group
@@ -802,7 +802,7 @@ struct BScanArgs {
     const float *pq2;
     const int32_t *tile_clu, *tile_first, *tile_cnt, *tile_v0, *tile_cand;
     const int32_t *ntiles, *sv;
-    int32_t *work;  // kXcds tile counters, zero at launch
+    int32_t *work;  // tile counter, zero at launch
     const int32_t *pairs;  // grouped pair -> q * NP + p
     int NP;
     uint32_t *qbound;      // per query: an upper bound of its k-th distance (float_key), shared by its tiles
@@ -1181,18 +1181,14 @@ __global__ __launch_bounds__(64 * W, (W * QTM >= 8 || (QTM == 4 && QS == 2)) ? 1
     wave_layout<QTM, QS>(bv.nqs, a.topk, smem_raw + w * per_wave, m);
     int32_t *tile_s = reinterpret_cast<int32_t *>(smem_raw + W * per_wave);
     const int ntiles = *a.ntiles, sv = *a.sv;
-    // this workgroup's XCD and its tile stream: the k-th tile of stream x is run k / kTileRun of
-    // the stream's runs (global runs x, x + nx, x + 2 nx, ...), position k % kTileRun in it -- every
-    // tile index belongs to exactly one (stream, k)
-    const int nx = min(kXcds, (int)gridDim.x), xcd = (int)(blockIdx.x % nx);
-    auto stream_tile = [&](int k) { return ((k / kTileRun) * nx + xcd) * kTileRun + k % kTileRun; };
-    for (int round = 0;; ++round) {
+    for (;;) {
         int tile;
+        // every tile comes from the launch's counter
         if (W == 1) {
-            if (lane == 0) tile = stream_tile(atomicAdd(a.work + xcd, 1));
+            if (lane == 0) tile = atomicAdd(a.work, 1);
             tile = __shfl(tile, 0);
         } else {
-            if (threadIdx.x == 0) *tile_s = stream_tile(atomicAdd(a.work + xcd, 1));
+            if (threadIdx.x == 0) *tile_s = atomicAdd(a.work, 1);
             __syncthreads();
             tile = *tile_s;
             __syncthreads();  // every wave has read it before the next round writes it

@@ -10,11 +10,6 @@ constexpr int kMB = 16;     // vectors per MFMA column block (one mblock record)
 // chip at 2 waves per SIMD).
 constexpr int kMinSplit = 512;
 constexpr int kSplitWaves = 2048;
-// The tile kernel's work counters: one per XCD (an MI355X has 8; a dispatch's consecutive
-// workgroups go to consecutive XCDs), each XCD taking runs of kTileRun consecutive work tiles --
-// the vector ranges of one list's query tile, which share the query operands -- so that the
-// tiles reading the same query rows meet in one XCD's L2.
-constexpr int kXcds = 8, kTileRun = 16;
 // Lists the grouping scan holds in registers: 1024 threads x kScanRounds each (the nlist limit of the
 // batched scan).
 constexpr int kScanRounds = 16;
