@@ -769,7 +769,7 @@ def main():
     prec_name = "int8x2" if bprec == saq_amd.PREC_INT8X2 else "int8"
     batched = {"qps": round(bqps, 1), "recall_at_10": round(brec, 4),
                "nprobe": bnp if npq is None else "U" + str(W["npq_range"]), "precision": prec_name,
-               "query_tile": bqt, "coarse_quantiser": "exact (reference order)" if args.exact_probe else "rocBLAS sgemm fp32",
+               "query_tile": bqt, "coarse_quantiser": "exact (reference order)" if args.exact_probe else "rocBLAS bf16 GEMM of hi/lo bf16 parts (hi.hi + hi.lo + lo.hi, fp32 accumulation)",
                "recall_below_plain": below,
                "ms_per_step": round(bdt / steps * 1e3, 4), "layout_build_ms": round(t_prep * 1e3, 1),
                "kernel_ms": round(bkern_ms, 4), "layout_bytes_per_launch": b_bytes,

@@ -29,6 +29,7 @@ hipError_t launch_query_prep(const IndexView &ix, const float *queries, int64_t 
 hipError_t launch_probe(const float *centroids, int K, int D, const float *queries, int64_t nq, int nprobe,
                         int dist_type, float *dist_ws, int32_t *ties, uint32_t *probes, hipStream_t st);
 hipError_t launch_sqnorm_rows(const float *x, int64_t rows, int D, float *out, hipStream_t st);
+hipError_t launch_split_bf16(const float *x, int64_t rows, int D, int second, uint16_t *out, hipStream_t st);
 size_t estimate_smem_bytes(const IndexView &ix);
 hipError_t launch_estimate(const IndexView &ix, const float *qrot, const float *bound, const uint32_t *probes,
                            const int64_t *offs, const int2 *pieces, int npieces, int dist_type, uint32_t *out_ids,
@@ -224,6 +225,9 @@ struct saqgpu_index {
     DevBuf<float> bq_pc;
     DevBuf<float> cnorm;  // |c|^2 per centroid (GEMM-based coarse quantisers), computed once
     bool cnorm_ready = false;
+    DevBuf<uint16_t> cen_bf3;  // centroid rows as bfloat16 hi | lo | hi (the batched coarse quantiser)
+    bool cen_bf3_ready = false;
+    DevBuf<uint16_t> ws_qbf3;  // query rows as bfloat16 hi | hi | lo
     DevBuf<float> bq_cand_d;
     DevBuf<uint32_t> bq_cand_id;
     // the captured batched step (SAQGPU_USE_GRAPH) and what it was captured with
@@ -337,15 +341,29 @@ int ensure_cnorm(saqgpu_index *ix, hipStream_t st) {
     return SAQGPU_OK;
 }
 
-// q.c for every (query, centroid) into ws_cdist (nq x K) by rocBLAS's fp32 GEMM: in column-major
-// terms ip^T (K x nq, ld K) = centroids (D x K, ld D)^T * queries (D x nq, ld D).
-int ip_gemm(saqgpu_index *ix, const float *dq, int64_t nq, hipStream_t st) {
-    const int K = ix->num_cen, D = ix->num_dim;
+// The centroid rows split into bfloat16 parts (hi | lo | hi), once per index.
+int ensure_cen_bf3(saqgpu_index *ix, hipStream_t st) {
+    if (ix->cen_bf3_ready) return SAQGPU_OK;
+    if (ix->cen_bf3.ensure((size_t)ix->num_cen * 3 * ix->num_dim)) return fail(SAQGPU_ENOMEM, "allocation failed");
+    HIPCHK(launch_split_bf16(ix->centroids.p, ix->num_cen, ix->num_dim, 0, ix->cen_bf3.p, st));
+    ix->cen_bf3_ready = true;
+    return SAQGPU_OK;
+}
+
+int blas_on(saqgpu_index *ix, hipStream_t st) {
     if (!ix->blas && rocblas_create_handle(&ix->blas) != rocblas_status_success) {
         ix->blas = nullptr;
         return fail(SAQGPU_EDEVICE, "rocblas_create_handle failed");
     }
     if (rocblas_set_stream(ix->blas, st) != rocblas_status_success) return fail(SAQGPU_EDEVICE, "rocblas_set_stream failed");
+    return SAQGPU_OK;
+}
+
+// q.c for every (query, centroid) into ws_cdist (nq x K) by rocBLAS's fp32 GEMM: in column-major
+// terms ip^T (K x nq, ld K) = centroids (D x K, ld D)^T * queries (D x nq, ld D).
+int ip_gemm(saqgpu_index *ix, const float *dq, int64_t nq, hipStream_t st) {
+    const int K = ix->num_cen, D = ix->num_dim;
+    if (int rc = blas_on(ix, st)) return rc;
     const float one = 1.0f, zero = 0.0f;
     if (rocblas_sgemm(ix->blas, rocblas_operation_transpose, rocblas_operation_none, K, (int)nq, D, &one,
                       ix->centroids.p, D, dq, D, &zero, ix->ws_cdist.p, K) != rocblas_status_success)
@@ -620,6 +638,8 @@ void saqgpu_index_destroy(saqgpu_index *ix) {
     ix->bq_qq.release();
     ix->bq_pc.release();
     ix->cnorm.release();
+    ix->cen_bf3.release();
+    ix->ws_qbf3.release();
     ix->bq_npq.release();
     ix->bq_cand_d.release();
     ix->bq_cand_id.release();
@@ -1484,14 +1504,28 @@ int default_query_tile(const saqgpu_index *ix, int64_t nq, int nprobe) {
     return per_list >= 96 ? 128 : (per_list >= 48 ? 64 : (per_list >= 24 ? 32 : 16));
 }
 
-// The batched scan's coarse quantiser: q.c for every (query, centroid) from rocBLAS's fp32 GEMM
-// into ws_cdist, then the exact top-nprobe selection over |c|^2 - 2 q.c (or q.c for IP).  Not FlatInitializer's reduction order: a probe list can differ from the reference's
-// where two centroid distances lie within fp32 rounding of each other.
+// The batched scan's coarse quantiser: q.c for every (query, centroid) on the matrix cores, as
+// one rocBLAS bfloat16 GEMM over K = 3 D of the rows split into bfloat16 parts (centroids
+// hi | lo | hi, queries hi | hi | lo: the products sum to hi.hi + lo.hi + hi.lo, fp32
+// accumulation), into ws_cdist; then the exact top-nprobe selection over |c|^2 - 2 q.c (or q.c
+// for IP).  The dropped lo.lo term and the parts' rounding are below 2^-16 of each |q_i c_i|,
+// about the accumulation error of an fp32 GEMM.  Not FlatInitializer's reduction order: a probe
+// list can differ from the reference's where two centroid distances lie within that error of
+// each other.  ws_qbf3 and cen_bf3 are allocated by the caller (cen_bf3 by ensure_cen_bf3).
 int probe_gemm(saqgpu_index *ix, const float *dq, int64_t nq, int nprobe, int dist_type, uint32_t *probes,
                hipStream_t st) {
+    const int K = ix->num_cen, D = ix->num_dim;
     int rc;
-    if ((rc = ensure_cnorm(ix, st)) || (rc = ip_gemm(ix, dq, nq, st))) return rc;
-    HIPCHK(launch_probe_select_ip(ix->ws_cdist.p, ix->cnorm.p, ix->num_cen, nq, nprobe, dist_type, probes, st));
+    if ((rc = ensure_cnorm(ix, st)) || (rc = ensure_cen_bf3(ix, st)) || (rc = blas_on(ix, st))) return rc;
+    HIPCHK(launch_split_bf16(dq, nq, D, 1, ix->ws_qbf3.p, st));
+    const float one = 1.0f, zero = 0.0f;
+    // column-major: ip^T (K x nq, ld K) = cen_bf3 (3D x K, ld 3D)^T * ws_qbf3 (3D x nq, ld 3D)
+    if (rocblas_gemm_ex(ix->blas, rocblas_operation_transpose, rocblas_operation_none, K, (int)nq, 3 * D, &one,
+                        ix->cen_bf3.p, rocblas_datatype_bf16_r, 3 * D, ix->ws_qbf3.p, rocblas_datatype_bf16_r, 3 * D,
+                        &zero, ix->ws_cdist.p, rocblas_datatype_f32_r, K, ix->ws_cdist.p, rocblas_datatype_f32_r, K,
+                        rocblas_datatype_f32_r, rocblas_gemm_algo_standard, 0, 0) != rocblas_status_success)
+        return fail(SAQGPU_EDEVICE, "rocblas_gemm_ex failed");
+    HIPCHK(launch_probe_select_ip(ix->ws_cdist.p, ix->cnorm.p, K, nq, nprobe, dist_type, probes, st));
     return SAQGPU_OK;
 }
 
@@ -1538,6 +1572,7 @@ int search_batched_locked(saqgpu_index *ix, const float *queries, int64_t nq, co
     if (!probes) {
         rc |= ix->ws_cdist.ensure((size_t)nq * K);
         if (p->exact_probe) rc |= ix->ws_ties.ensure(std::max<int64_t>(1, nq));
+        else rc |= ix->ws_qbf3.ensure((size_t)nq * 3 * ix->num_dim);
     }
     if (rc) return fail(SAQGPU_ENOMEM, "workspace allocation failed");
     // the replay of a captured graph: the same shape, parameters and buffers as the capture
@@ -1549,6 +1584,7 @@ int search_batched_locked(saqgpu_index *ix, const float *queries, int64_t nq, co
         return SAQGPU_OK;
     }
     if (!probes && (rc = ensure_cnorm(ix, st))) return rc;
+    if (!probes && !p->exact_probe && (rc = ensure_cen_bf3(ix, st))) return rc;
     const float *dq = queries;
     const uint32_t *dp = probes;
     const int32_t *dnpq = p->nprobe_per_query;

@@ -445,6 +445,39 @@ hipError_t launch_sqnorm_rows(const float *x, int64_t rows, int D, float *out, h
     return hipGetLastError();
 }
 
+// bfloat16 of x, round to nearest even (NaN stays NaN)
+__device__ __forceinline__ uint16_t bf16_rne(float x) {
+    uint32_t u = __float_as_uint(x);
+    if ((u & 0x7FFFFFFFu) > 0x7F800000u) return (uint16_t)((u >> 16) | 0x40u);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+// Rows of x (D floats) as three bfloat16 parts of 3 D each, for a GEMM over K = 3 D whose
+// products sum to hi.hi + hi.lo + lo.hi: hi = bf16(x), lo = bf16(x - hi).  Part order: centroid
+// rows (second = 0) hi | lo | hi, query rows (second = 1) hi | hi | lo.
+__global__ __launch_bounds__(256) void split_bf16_kernel(const float *__restrict__ x, int64_t rows, int D,
+                                                         int second, uint16_t *__restrict__ out) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= rows * D) return;
+    const int64_t r = e / D;
+    const int i = (int)(e - r * D);
+    const float v = x[e];
+    const uint16_t hi = bf16_rne(v);
+    const uint16_t lo = bf16_rne(v - __uint_as_float((uint32_t)hi << 16));
+    uint16_t *o = out + r * 3 * D + i;
+    o[0] = hi;
+    o[D] = second ? hi : lo;
+    o[2 * D] = second ? lo : hi;
+}
+
+hipError_t launch_split_bf16(const float *x, int64_t rows, int D, int second, uint16_t *out, hipStream_t st) {
+    if (rows == 0) return hipSuccess;
+    hipLaunchKernelGGL(split_bf16_kernel, dim3((unsigned)((rows * D + 255) / 256)), dim3(256), 0, st, x, rows, D,
+                       second, out);
+    return hipGetLastError();
+}
+
 // Top-nprobe selection over inner products ip (nq x K, from a library GEMM): keys are
 // cnorm - 2 ip for L2 (cnorm = |c|^2), ip for IP.
 hipError_t launch_probe_select_ip(const float *ip, const float *cnorm, int K, int64_t nq, int nprobe, int dist_type,
