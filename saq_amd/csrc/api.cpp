@@ -378,6 +378,9 @@ int ip_gemm(saqgpu_index *ix, const float *dq, int64_t nq, hipStream_t st) {
 // nprobe the window around the nprobe-th key holds too many centroids for the filter to pay
 // (at 16384 lists and nprobe 256, often more than its 1024 places).  ws_cdist holds nq x num_cen
 // floats.  SAQGPU_PROBE=tile / =gemm forces a form (gemm: any nprobe <= 1024).
+int probe_gemm(saqgpu_index *ix, const float *dq, int64_t nq, int nprobe, int dist_type, uint32_t *probes,
+               hipStream_t st);
+
 int exact_probe(saqgpu_index *ix, const float *dq, int64_t nq, int nprobe, int dist_type, uint32_t *probes,
                 hipStream_t st) {
     if (ix->ws_ties.ensure(std::max<int64_t>(1, nq))) return fail(SAQGPU_ENOMEM, "workspace allocation failed");
@@ -869,7 +872,14 @@ int saqgpu_probe(saqgpu_index *ix, const float *queries, int64_t nq, int32_t npr
         dp = ix->ws_probes.p;
     }
     if (ix->ws_cdist.ensure((size_t)nq * ix->num_cen)) return fail(SAQGPU_ENOMEM, "workspace allocation failed");
-    if (int rc = exact_probe(ix, dq, nq, nprobe, dist_type, dp, st)) return rc;
+    const char *form = getenv("SAQGPU_PROBE");
+    if (form && !strcmp(form, "bf16x3")) {  // the batched scan's coarse quantiser (not exact), for tests of its select
+        if (nprobe > 4096) return fail(SAQGPU_EUNSUPPORTED, "the bf16x3 coarse quantiser selects at most 4096 lists");
+        if (ix->ws_qbf3.ensure((size_t)nq * 3 * ix->num_dim)) return fail(SAQGPU_ENOMEM, "workspace allocation failed");
+        if (int rc = probe_gemm(ix, dq, nq, nprobe, dist_type, dp, st)) return rc;
+    } else if (int rc = exact_probe(ix, dq, nq, nprobe, dist_type, dp, st)) {
+        return rc;
+    }
     if (!dev) {
         HIPCHK(hipMemcpyAsync(probes, dp, nq * nprobe * 4, hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));

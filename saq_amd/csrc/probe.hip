@@ -294,12 +294,14 @@ struct RadixLds {
 // and only exact ties reach the id bytes.  A thread's
 // keys: kv[0 .. KPT) (centroids tid + 256 j below K) or, for KPT = 0, key(c) for c = tid,
 // tid + 256, ... below K.
+// pass0 > 0: every key shares its top 8 pass0 bits with pre0, and the search starts below them.
 template <int KPT, class KeyFn>
 __device__ __forceinline__ unsigned long long block_radix_kth(const unsigned long long (&kv)[KPT > 0 ? KPT : 1],
-                                                              KeyFn key, int K, int r, RadixLds &sh) {
+                                                              KeyFn key, int K, int r, RadixLds &sh, int pass0 = 0,
+                                                              unsigned long long pre0 = 0) {
     const int tid = threadIdx.x;
-    unsigned long long pre = 0;
-    for (int pass = 0; pass < 8; ++pass) {
+    unsigned long long pre = pass0 > 0 ? pre0 & (~0ull << (64 - 8 * pass0)) : 0ull;
+    for (int pass = pass0; pass < 8; ++pass) {
         const int shift = 56 - 8 * pass;
         sh.hist[tid] = 0;
         __syncthreads();
@@ -357,6 +359,8 @@ __global__ __launch_bounds__(256) void probe_select_kernel(const float *__restri
     __shared__ RadixLds rsh;
     __shared__ int cnt_s;
     __shared__ unsigned long long sel[kSelMax];
+    __shared__ unsigned long long small[KPT > 0 ? 1 : 256];
+    __shared__ unsigned long long thr_s, kmin_s;
     const int64_t q = blockIdx.x;
     const float *dq = dist + q * K;
     const int tid = threadIdx.x;
@@ -369,6 +373,80 @@ __global__ __launch_bounds__(256) void probe_select_kernel(const float *__restri
     if constexpr (KPT > 0) {
 #pragma unroll
         for (int j = 0; j < KPT; ++j) kv[j] = tid + 256 * j < K ? key(tid + 256 * j) : ~0ull;
+    }
+    if constexpr (KPT == 0) {
+        // Candidates first (nprobe <= 256, no tie report): T = the nprobe-th smallest of the 256
+        // threads' minimum keys has at least nprobe keys <= T, so the nprobe smallest keys all are
+        // <= T.  Those keys (a few times nprobe for the distance spreads of coarse quantisers) go
+        // to LDS in one more read, and the exact radix select runs over them there instead of
+        // over all K keys in global memory.  More than kSelMax of them: the full select below.
+        if (ties == nullptr && nprobe <= 256 && K >= 256) {
+            constexpr int U = 16;  // keys a thread loads together
+            unsigned long long m = ~0ull;
+            for (int c0 = tid; c0 < K; c0 += 256 * U) {
+                unsigned long long kk[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) kk[u] = c0 + 256 * u < K ? key(c0 + 256 * u) : ~0ull;
+#pragma unroll
+                for (int u = 0; u < U; ++u) m = min(m, kk[u]);
+            }
+            small[tid] = m;
+            if (tid == 0) cnt_s = 0;
+            __syncthreads();
+            int rank = 0;
+            for (int j = 0; j < 256; ++j) rank += small[j] < m;
+            if (rank == nprobe - 1) thr_s = m;  // keys are distinct (ids): one thread
+            if (rank == 0) kmin_s = m;
+            __syncthreads();
+            const unsigned long long T = thr_s, kmin = kmin_s;
+            const int lane = tid & 63;
+            for (int c0 = tid; c0 < K; c0 += 256 * U) {
+                unsigned long long kk[U];
+                int nt = 0;
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    kk[u] = c0 + 256 * u < K ? key(c0 + 256 * u) : ~0ull;
+                    nt += kk[u] <= T;
+                }
+                int incl = nt;  // one LDS atomic per wave and chunk: the wave's prefix sum of takes
+#pragma unroll
+                for (int o = 1; o < 64; o <<= 1) {
+                    const int v = __shfl_up(incl, o);
+                    if (lane >= o) incl += v;
+                }
+                int b = 0;
+                if (lane == 63 && incl) b = atomicAdd(&cnt_s, incl);
+                int p = __shfl(b, 63) + incl - nt;
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    if (kk[u] <= T) {
+                        if (p < kSelMax) sel[p] = kk[u];
+                        ++p;
+                    }
+            }
+            __syncthreads();
+            const int n = cnt_s;
+            if (n <= kSelMax) {
+                // the candidates lie in [kmin, T]: their common top bytes need no pass
+                const int pass0 = kmin == T ? 7 : __clzll(kmin ^ T) / 8;
+                const unsigned long long thr =
+                    block_radix_kth<0>(kv, [&](int i) { return sel[i]; }, n, nprobe, rsh, pass0, kmin);
+                if (tid == 0) cnt_s = 0;
+                __syncthreads();
+                for (int i = tid; i < n; i += 256) {
+                    const unsigned long long k = sel[i];
+                    if (k <= thr) small[atomicAdd(&cnt_s, 1)] = k;  // exactly nprobe of them
+                }
+                __syncthreads();
+                if (tid < nprobe) {
+                    const unsigned long long k = small[tid];
+                    int r = 0;
+                    for (int j = 0; j < nprobe; ++j) r += small[j] < k;
+                    probes[q * nprobe + r] = (uint32_t)(k & 0xFFFFFFFFu);
+                }
+                return;
+            }
+        }
     }
     const unsigned long long thr = block_radix_kth<KPT>(kv, key, K, nprobe, rsh);  // exactly nprobe keys <= thr
     if (tid == 0) cnt_s = 0;

@@ -296,3 +296,39 @@ def test_search_handles_run_concurrently_and_agree():
     again = ix.search(case.queries, 10, 8)
     assert np.array_equal(again[0], want[0])
     ix.close()
+
+
+@pytest.mark.parametrize("dist_type", [0, 1])
+def test_gpu_bf16x3_probe_select_more_than_4096_lists(dist_type, monkeypatch):
+    """The batched scan's coarse quantiser at K = 5000 (keys read from global memory): the lists
+    a query's threads' minimum keys bound, then the exact select over them in LDS.  Centroids
+    c_j = R (t_j, 0, ...) with t_j a permutation of a 0.1 grid and queries R (x, 0, ...), x 0.03
+    off the grid (R a random rotation), keep every two keys a query selects among at least 0.003
+    apart (L2: 0.07^2 - 0.03^2; IP: 0.1 |x|), ten times the bf16x3 GEMM's error bound
+    (2^-16 of 2 |x| |t|) and more, so the lists equal the float64 order.  Identical centroids (every key ties, more candidates
+    than LDS holds) take the full select and come out in id order."""
+    monkeypatch.setenv("SAQGPU_PROBE", "bf16x3")
+    from saq_amd import Index
+    rng = np.random.default_rng(37)
+    K, d, nq = 5000, 64, 6
+    R, _ = np.linalg.qr(rng.standard_normal((d, d)))
+    t = (rng.permutation(K) - K // 2) * 0.1
+    x = np.array([0.03, -0.47, 0.83, 0.13, -0.97, 0.53])
+    cent = np.zeros((K, d)); cent[:, 0] = t
+    qs = np.zeros((nq, d)); qs[:, 0] = x
+    cent = (cent @ R.T).astype(np.float32)
+    qs = (qs @ R.T).astype(np.float32)
+    var = np.ones(64, np.float32)
+    ix = Index(d, [(64, 2)], cent, np.zeros(K, np.int64), var)
+    for nprobe in (1, 37, 256):
+        got = ix.probe(qs, nprobe, dist_type=dist_type)
+        key = (t[None, :] - x[:, None]) ** 2 if dist_type == 0 else -(t[None, :] * x[:, None])
+        want = np.argsort(key, axis=1, kind="stable")[:, :nprobe].astype(np.uint32)
+        assert np.array_equal(want, got), f"nprobe={nprobe}"
+    ix.close()
+    same = np.repeat(cent[:1], K, axis=0)
+    ix = Index(d, [(64, 2)], same, np.zeros(K, np.int64), var)
+    for nprobe in (1, 100, 256):
+        got = ix.probe(qs, nprobe, dist_type=dist_type)
+        assert np.array_equal(got, np.tile(np.arange(nprobe, dtype=np.uint32), (nq, 1))), f"nprobe={nprobe}"
+    ix.close()
