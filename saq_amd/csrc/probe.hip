@@ -311,9 +311,13 @@ __device__ __forceinline__ unsigned long long block_radix_kth(const unsigned lon
             for (int j = 0; j < KPT; ++j)
                 if (tid + 256 * j < K && (kv[j] & hmask) == pre) atomicAdd(&sh.hist[(kv[j] >> shift) & 0xFF], 1u);
         } else {
-            for (int c = tid; c < K; c += 256) {
-                const unsigned long long k = key(c);
-                if ((k & hmask) == pre) atomicAdd(&sh.hist[(k >> shift) & 0xFF], 1u);
+            for (int c0 = tid; c0 < K; c0 += 256 * 8) {
+                unsigned long long kk[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) kk[u] = key(min(c0 + 256 * u, K - 1));  // loads issued together
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    if (c0 + 256 * u < K && (kk[u] & hmask) == pre) atomicAdd(&sh.hist[(kk[u] >> shift) & 0xFF], 1u);
             }
         }
         __syncthreads();
@@ -386,9 +390,9 @@ __global__ __launch_bounds__(256) void probe_select_kernel(const float *__restri
             for (int c0 = tid; c0 < K; c0 += 256 * U) {
                 unsigned long long kk[U];
 #pragma unroll
-                for (int u = 0; u < U; ++u) kk[u] = c0 + 256 * u < K ? key(c0 + 256 * u) : ~0ull;
+                for (int u = 0; u < U; ++u) kk[u] = key(min(c0 + 256 * u, K - 1));  // loads issued together
 #pragma unroll
-                for (int u = 0; u < U; ++u) m = min(m, kk[u]);
+                for (int u = 0; u < U; ++u) m = c0 + 256 * u < K ? min(m, kk[u]) : m;
             }
             small[tid] = m;
             if (tid == 0) cnt_s = 0;
@@ -404,8 +408,10 @@ __global__ __launch_bounds__(256) void probe_select_kernel(const float *__restri
                 unsigned long long kk[U];
                 int nt = 0;
 #pragma unroll
+                for (int u = 0; u < U; ++u) kk[u] = key(min(c0 + 256 * u, K - 1));
+#pragma unroll
                 for (int u = 0; u < U; ++u) {
-                    kk[u] = c0 + 256 * u < K ? key(c0 + 256 * u) : ~0ull;
+                    if (c0 + 256 * u >= K) kk[u] = ~0ull;  // above T: T is a key below K
                     nt += kk[u] <= T;
                 }
                 int incl = nt;  // one LDS atomic per wave and chunk: the wave's prefix sum of takes
@@ -459,12 +465,16 @@ __global__ __launch_bounds__(256) void probe_select_kernel(const float *__restri
                 if (p < kSelMax) sel[p] = kv[j];
             }
     } else {
-        for (int i = tid; i < K; i += 256) {
-            const unsigned long long k = key(i);
-            if (k <= thr) {
-                const int p = atomicAdd(&cnt_s, 1);
-                if (p < kSelMax) sel[p] = k;
-            }
+        for (int c0 = tid; c0 < K; c0 += 256 * 8) {
+            unsigned long long kk[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) kk[u] = key(min(c0 + 256 * u, K - 1));
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (c0 + 256 * u < K && kk[u] <= thr) {
+                    const int p = atomicAdd(&cnt_s, 1);
+                    if (p < kSelMax) sel[p] = kk[u];
+                }
         }
     }
     __syncthreads();
