@@ -1022,66 +1022,95 @@ __device__ __forceinline__ v4i mfma(v4i a, v4i b, v4i c) {
     return __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, c, 0, 0, 0);
 }
 
-// One quantised segment's contraction for a chunk of NVW mblocks: every 64-dim chunk of the
-// segment, the code operands of kKB chunks (4 for nibble codes, whose loads are half as wide)
-// loaded for all NVW mblocks together, then their MFMAs.  KIND: 0 = 4-bit codes stored as
-// nibbles, 1 = one int8 plane, 2 = a shifted low plane and a high plane (9+ bits).  The kind is
-// a template parameter so that no branch separates the loads of a step.
-template <int KIND, int QTM, int QS, int NVW>
-__device__ __forceinline__ void seg_mma(const BSeg &bs, const uint8_t *const (&rec)[NVW],
-                                        const int8_t *const (&arow1)[QTM], const int8_t *const (&arow2)[QTM],
-                                        v4i (&acc1)[QTM][NVW], v4i (&acc2)[QTM][NVW], v4i (&ach1)[QTM][NVW],
-                                        v4i (&ach2)[QTM][NVW], int lane) {
+// Raw buffer loads: the base and size are uniform (scalar registers) and a lane's offset is one
+// 32-bit register, so a load costs no vector address arithmetic.  Loads past the size read 0.
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+__device__ __forceinline__ Rsrc make_rsrc(const void *base, int bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, bytes, 0x00020000);
+}
+__device__ __forceinline__ v4i ld16(Rsrc r, int voff, int soff) {
+    return __builtin_bit_cast(v4i, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+}
+__device__ __forceinline__ uint2 ld8(Rsrc r, int voff, int soff) {
+    return __builtin_bit_cast(uint2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+
+// Operands of a chunk's MFMAs: rb covers the chunk's NVW mblock records, mblock j at byte
+// j * mb_stride; b16[j] / b8[j] are this lane's offsets of mblock j's 16-byte / 8-byte code
+// operands.  The mblock offset is in the per-lane offset, which the buffer's range check covers, so
+// mblocks past the list's end read 0.  ra1 / ra2: the wave's quantised query rows, aoff[t] this
+// lane's row offset in them.  Segment and chunk offsets go in the uniform offset.
+template <int QTM, int NVW>
+struct MmaOps {
+    Rsrc rb, ra1, ra2;
+    int b16[NVW], b8[NVW];
+    int aoff[QTM];
+};
+
+// One load-then-multiply step of a segment: the code operands of kKB 64-dim chunks (4 for
+// nibble codes, whose loads are half as wide) for all NVW mblocks and the matching query
+// operands, all in flight before the first MFMA waits.  A chunk past nk loads the last one again
+// and skips its MFMAs (a uniform branch).  FIRST: the step's first MFMAs start from zero instead
+// of the accumulators.  KIND: 0 = 4-bit codes stored as nibbles, 1 = one int8 plane, 2 = a
+// shifted low plane and a high plane (9+ bits).
+template <int KIND, int QTM, int QS, int NVW, bool FIRST>
+__device__ __forceinline__ void seg_step(const BSeg &bs, int kc0, int nk, const MmaOps<QTM, NVW> &o,
+                                         v4i (&acc1)[QTM][NVW], v4i (&acc2)[QTM][NVW], v4i (&ach1)[QTM][NVW],
+                                         v4i (&ach2)[QTM][NVW], int lane) {
     constexpr int kKB = KIND == 0 ? 4 : 2;
-    const int nk = bs.dim / 64;
-    for (int kc0 = 0; kc0 < nk; kc0 += kKB) {
-        v4i A1[kKB][QTM], A2[kKB][QTM], B[kKB][NVW], BH[kKB][NVW];
-        uint2 BN[kKB][NVW];
+    v4i A1[kKB][QTM], A2[kKB][QTM], B[kKB][NVW], BH[kKB][NVW];
+    uint2 BN[kKB][NVW];
 #pragma unroll
-        for (int u = 0; u < kKB; ++u) {
-            // a chunk past nk repeats the last one with a zero A operand: every load and MFMA of
-            // the step is unconditional, so the loads stay together
-            const int kc = min(kc0 + u, nk - 1);
-            const bool live = kc0 + u < nk;
+    for (int u = 0; u < kKB; ++u) {
+        const int kc = min(kc0 + u, nk - 1);
+        const int sa = bs.kq_off + kc * 64;
 #pragma unroll
-            for (int t = 0; t < QTM; ++t) {
-                const v4i a1 = *reinterpret_cast<const v4i *>(arow1[t] + bs.kq_off + kc * 64);
-                A1[u][t] = live ? a1 : v4i{0, 0, 0, 0};
-                if (QS == 2) {
-                    const v4i a2 = *reinterpret_cast<const v4i *>(arow2[t] + bs.kq_off + kc * 64);
-                    A2[u][t] = live ? a2 : v4i{0, 0, 0, 0};
-                }
-            }
-#pragma unroll
-            for (int j = 0; j < NVW; ++j) {
-                const uint8_t *p = rec[j] + bs.code_off + (int64_t)kc * bs.chunk_bytes;
-                if (KIND == 0) {
-                    BN[u][j] = *reinterpret_cast<const uint2 *>(p + lane * 8);
-                } else {
-                    B[u][j] = *reinterpret_cast<const v4i *>(p + lane * 16);
-                    if (KIND == 2) BH[u][j] = *reinterpret_cast<const v4i *>(p + 1024 + lane * 16);
-                }
-            }
+        for (int t = 0; t < QTM; ++t) {
+            A1[u][t] = ld16(o.ra1, o.aoff[t], sa);
+            if (QS == 2) A2[u][t] = ld16(o.ra2, o.aoff[t], sa);
         }
-        // keep the scheduler from interleaving the step's loads with its MFMAs: all of them are
-        // in flight before the first MFMA waits
-        __builtin_amdgcn_sched_barrier(0);
+        const int sb = bs.code_off + kc * bs.chunk_bytes;
 #pragma unroll
-        for (int u = 0; u < kKB; ++u) {
-#pragma unroll
-            for (int t = 0; t < QTM; ++t)
-#pragma unroll
-                for (int j = 0; j < NVW; ++j) {
-                    const v4i b = KIND == 0 ? unpack_nibbles(BN[u][j]) : B[u][j];
-                    acc1[t][j] = mfma(A1[u][t], b, acc1[t][j]);
-                    if (QS == 2) acc2[t][j] = mfma(A2[u][t], b, acc2[t][j]);
-                    if (KIND == 2) {
-                        ach1[t][j] = mfma(A1[u][t], BH[u][j], ach1[t][j]);
-                        if (QS == 2) ach2[t][j] = mfma(A2[u][t], BH[u][j], ach2[t][j]);
-                    }
-                }
+        for (int j = 0; j < NVW; ++j) {
+            if (KIND == 0) {
+                BN[u][j] = ld8(o.rb, o.b8[j], sb);
+            } else {
+                B[u][j] = ld16(o.rb, o.b16[j], sb);
+                if (KIND == 2) BH[u][j] = ld16(o.rb, o.b16[j] + 1024, sb);
+            }
         }
     }
+    __builtin_amdgcn_sched_barrier(0);
+    const v4i z = {0, 0, 0, 0};
+#pragma unroll
+    for (int u = 0; u < kKB; ++u) {
+        if (u > 0 && kc0 + u >= nk) break;
+        const bool zc = FIRST && u == 0;
+#pragma unroll
+        for (int t = 0; t < QTM; ++t)
+#pragma unroll
+            for (int j = 0; j < NVW; ++j) {
+                const v4i b = KIND == 0 ? unpack_nibbles(BN[u][j]) : B[u][j];
+                acc1[t][j] = mfma(A1[u][t], b, zc ? z : acc1[t][j]);
+                if (QS == 2) acc2[t][j] = mfma(A2[u][t], b, zc ? z : acc2[t][j]);
+                if (KIND == 2) {
+                    ach1[t][j] = mfma(A1[u][t], BH[u][j], zc ? z : ach1[t][j]);
+                    if (QS == 2) ach2[t][j] = mfma(A2[u][t], BH[u][j], zc ? z : ach2[t][j]);
+                }
+            }
+    }
+}
+
+// One quantised segment's contraction for a chunk of NVW mblocks, step by step; the accumulators
+// of the segment's plane(s) are set by its first step.
+template <int KIND, int QTM, int QS, int NVW>
+__device__ __forceinline__ void seg_mma(const BSeg &bs, const MmaOps<QTM, NVW> &o, v4i (&acc1)[QTM][NVW],
+                                        v4i (&acc2)[QTM][NVW], v4i (&ach1)[QTM][NVW], v4i (&ach2)[QTM][NVW],
+                                        int lane) {
+    constexpr int kKB = KIND == 0 ? 4 : 2;
+    const int nk = bs.dim / 64;
+    seg_step<KIND, QTM, QS, NVW, true>(bs, 0, nk, o, acc1, acc2, ach1, ach2, lane);
+    for (int kc0 = kKB; kc0 < nk; kc0 += kKB) seg_step<KIND, QTM, QS, NVW, false>(bs, kc0, nk, o, acc1, acc2, ach1, ach2, lane);
 }
 
 __device__ __forceinline__ v4f mfma_bf16(v8bf a, v8bf b, v4f c) {
@@ -1223,14 +1252,22 @@ __global__ __launch_bounds__(64 * W, (W * QTM >= 8 || (QTM == 4 && QS == 2)) ? 1
             }
         wave_sync();
         // A operand rows of this lane: query row t*16 + lane%16 of the wave's rows
-        const int8_t *arow1[QTM], *arow2[QTM];
         const uint16_t *arowb[QTM];  // BF16 rows (QS == 3)
 #pragma unroll
         for (int t = 0; t < QTM; ++t) {
             const int64_t g = first + min(t * 16 + lane % 16, cnt - 1);
-            arow1[t] = a.qq1 + g * bv.kq + 16 * (lane / 16);
-            arow2[t] = QS == 2 ? a.qq2 + g * bv.kq + 16 * (lane / 16) : nullptr;
             arowb[t] = reinterpret_cast<const uint16_t *>(a.qq1) + g * bv.kq + 16 * (lane / 16);
+        }
+        // int8 operands through buffer loads; the query rows of this wave: first .. first + QT - 1
+        MmaOps<QTM, NVW> ops;
+        ops.ra1 = make_rsrc(a.qq1 + (int64_t)first * bv.kq, QT * bv.kq);
+        ops.ra2 = make_rsrc(QS == 2 ? a.qq2 + (int64_t)first * bv.kq : a.qq1, QT * bv.kq);
+#pragma unroll
+        for (int t = 0; t < QTM; ++t) ops.aoff[t] = min(t * 16 + lane % 16, cnt - 1) * bv.kq + 16 * (lane / 16);
+#pragma unroll
+        for (int j = 0; j < NVW; ++j) {
+            ops.b16[j] = (int)(j * bv.mb_stride) + lane * 16;
+            ops.b8[j] = (int)(j * bv.mb_stride) + lane * 8;
         }
         // the tile's vector range [v0, v0 + sv) of the list, in mblocks (v0 and sv are multiples of 64)
         const int n = bv.clu_n[c];
@@ -1273,19 +1310,19 @@ __global__ __launch_bounds__(64 * W, (W * QTM >= 8 || (QTM == 4 && QS == 2)) ? 1
 #pragma unroll
             for (int j = 0; j < NVW; ++j)
                 rec[j] = bv.mb + (mb0 + min(cb + j, mb_end - 1)) * bv.mb_stride;  // clamped duplicates are discarded
+            ops.rb = make_rsrc(bv.mb + (mb0 + cb) * bv.mb_stride, (int)(min(NVW, mb_end - cb) * bv.mb_stride));
             if (cb + NVW < mb_end) fetch_fac(cb + NVW, slot ^ 1);
             for (int s = 0; s < bv.nseg; ++s) {
                 const BSeg bs = bv.segs[s];
                 if (bs.qs < 0) continue;
+                // int8 accumulators are set by their segment's first MFMAs
                 v4i acc1[QTM][NVW], acc2[QTM][NVW], ach1[QTM][NVW], ach2[QTM][NVW];
                 v4f facc[QTM][NVW], fach[QTM][NVW];
+                if (QS == 3)
 #pragma unroll
-                for (int t = 0; t < QTM; ++t)
+                    for (int t = 0; t < QTM; ++t)
 #pragma unroll
-                    for (int j = 0; j < NVW; ++j) {
-                        acc1[t][j] = acc2[t][j] = ach1[t][j] = ach2[t][j] = v4i{0, 0, 0, 0};
-                        facc[t][j] = fach[t][j] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
-                    }
+                        for (int j = 0; j < NVW; ++j) facc[t][j] = fach[t][j] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
                 const bool two = bs.bits > 8;
                 if (QS == 3) {
                     if (bs.bits <= 4)
@@ -1295,11 +1332,11 @@ __global__ __launch_bounds__(64 * W, (W * QTM >= 8 || (QTM == 4 && QS == 2)) ? 1
                     else
                         seg_mma_bf16<2, QTM, NVW>(bs, rec, arowb, facc, fach, lane);
                 } else if (bs.bits <= 4) {
-                    seg_mma<0, QTM, QS, NVW>(bs, rec, arow1, arow2, acc1, acc2, ach1, ach2, lane);
+                    seg_mma<0, QTM, QS, NVW>(bs, ops, acc1, acc2, ach1, ach2, lane);
                 } else if (!two) {
-                    seg_mma<1, QTM, QS, NVW>(bs, rec, arow1, arow2, acc1, acc2, ach1, ach2, lane);
+                    seg_mma<1, QTM, QS, NVW>(bs, ops, acc1, acc2, ach1, ach2, lane);
                 } else {
-                    seg_mma<2, QTM, QS, NVW>(bs, rec, arow1, arow2, acc1, acc2, ach1, ach2, lane);
+                    seg_mma<2, QTM, QS, NVW>(bs, ops, acc1, acc2, ach1, ach2, lane);
                 }
                 // epilogue of the segment: ip = a1 acc1 (+ 256 a1 acch1) (+ a2 acc2 ...) + c, then
                 // dist += (-2 rescale) ip, all fused multiply-adds
@@ -1318,9 +1355,8 @@ __global__ __launch_bounds__(64 * W, (W * QTM >= 8 || (QTM == 4 && QS == 2)) ? 1
 #pragma unroll
                         for (int r = 0; r < 4; ++r) {
                             const float x1 = QS == 3 ? facc[t][j][r] : (float)acc1[t][j][r];
-                            const float h1 = QS == 3 ? fach[t][j][r] : (float)ach1[t][j][r];
                             float ip = fmaf(av[r], x1, cv[r]);
-                            if (two) ip = fmaf(256.0f * av[r], h1, ip);
+                            if (two) ip = fmaf(256.0f * av[r], QS == 3 ? fach[t][j][r] : (float)ach1[t][j][r], ip);
                             if (QS == 2) {
                                 ip = fmaf(bv2[r], (float)acc2[t][j][r], ip);
                                 if (two) ip = fmaf(256.0f * bv2[r], (float)ach2[t][j][r], ip);
