@@ -888,10 +888,11 @@ __host__ __device__ inline size_t wave_smem(int nqs, int topk) {
     return wave_layout<QTM, QS>(nqs, topk, nullptr, unused);
 }
 
-// LDS of a workgroup of W waves: W waves' areas and the tile index it broadcasts.
+// LDS of a workgroup of W waves: W waves' areas, the tile index it broadcasts and the quantised
+// segments' descriptors (read once per chunk and segment: LDS instead of global memory).
 template <int W, int QTM, int QS>
 __host__ __device__ inline size_t wg_smem(int nqs, int topk) {
-    return W * wave_smem<QTM, QS>(nqs, topk) + 16;
+    return W * wave_smem<QTM, QS>(nqs, topk) + 16 + (size_t)nqs * sizeof(BSeg);
 }
 
 // LDS traffic between the lanes of one wave: keep the compiler from moving accesses across
@@ -1209,6 +1210,12 @@ __global__ __launch_bounds__(64 * W, (W * QTM >= 8 || (QTM == 4 && QS == 2)) ? 1
     WSmem m;
     wave_layout<QTM, QS>(bv.nqs, a.topk, smem_raw + w * per_wave, m);
     int32_t *tile_s = reinterpret_cast<int32_t *>(smem_raw + W * per_wave);
+    BSeg *qseg = reinterpret_cast<BSeg *>(smem_raw + W * per_wave + 16);  // quantised segments by qs
+    for (int s = threadIdx.x; s < bv.nseg; s += 64 * W) {
+        const BSeg b = bv.segs[s];
+        if (b.qs >= 0) qseg[b.qs] = b;
+    }
+    __syncthreads();
     const int ntiles = *a.ntiles, sv = *a.sv;
     for (;;) {
         int tile;
@@ -1312,9 +1319,8 @@ __global__ __launch_bounds__(64 * W, (W * QTM >= 8 || (QTM == 4 && QS == 2)) ? 1
                 rec[j] = bv.mb + (mb0 + min(cb + j, mb_end - 1)) * bv.mb_stride;  // clamped duplicates are discarded
             ops.rb = make_rsrc(bv.mb + (mb0 + cb) * bv.mb_stride, (int)(min(NVW, mb_end - cb) * bv.mb_stride));
             if (cb + NVW < mb_end) fetch_fac(cb + NVW, slot ^ 1);
-            for (int s = 0; s < bv.nseg; ++s) {
-                const BSeg bs = bv.segs[s];
-                if (bs.qs < 0) continue;
+            for (int k = 0; k < bv.nqs; ++k) {
+                const BSeg bs = qseg[k];
                 // int8 accumulators are set by their segment's first MFMAs
                 v4i acc1[QTM][NVW], acc2[QTM][NVW], ach1[QTM][NVW], ach2[QTM][NVW];
                 v4f facc[QTM][NVW], fach[QTM][NVW];
@@ -1729,15 +1735,15 @@ size_t bscan_smem_bytes(const BatchedView &bv, int topk, int qt, int qs) {
     int W, qtm;
     tile_shape(qt, qs, W, qtm);
     if (qs == 3) {
-        if (qtm == 1) return (size_t)W * wave_smem<1, 3>(bv.nqs, topk) + 16;
-        return (size_t)W * wave_smem<2, 3>(bv.nqs, topk) + 16;
+        if (qtm == 1) return (size_t)W * wave_smem<1, 3>(bv.nqs, topk) + 16 + (size_t)bv.nqs * sizeof(BSeg);
+        return (size_t)W * wave_smem<2, 3>(bv.nqs, topk) + 16 + (size_t)bv.nqs * sizeof(BSeg);
     }
     if (qs == 2) {
-        if (qtm == 1) return (size_t)W * wave_smem<1, 2>(bv.nqs, topk) + 16;
-        return (size_t)W * wave_smem<2, 2>(bv.nqs, topk) + 16;
+        if (qtm == 1) return (size_t)W * wave_smem<1, 2>(bv.nqs, topk) + 16 + (size_t)bv.nqs * sizeof(BSeg);
+        return (size_t)W * wave_smem<2, 2>(bv.nqs, topk) + 16 + (size_t)bv.nqs * sizeof(BSeg);
     }
-    if (qtm == 1) return (size_t)W * wave_smem<1, 1>(bv.nqs, topk) + 16;
-    return (size_t)W * wave_smem<2, 1>(bv.nqs, topk) + 16;
+    if (qtm == 1) return (size_t)W * wave_smem<1, 1>(bv.nqs, topk) + 16 + (size_t)bv.nqs * sizeof(BSeg);
+    return (size_t)W * wave_smem<2, 1>(bv.nqs, topk) + 16 + (size_t)bv.nqs * sizeof(BSeg);
 }
 
 template <int W, int QTM, int QS>
