@@ -665,7 +665,8 @@ __device__ __forceinline__ void pairprep_segment(const PairPrepArgs &a, const BS
 // elements inside it (segment offsets and dimensions are multiples of 64, so no float4 straddles
 // two segments).  Same quantisation and constants as pairprep_segment.
 template <int NKT>
-__device__ __forceinline__ void pairprep_whole(const PairPrepArgs &a, int64_t g, const float *qr, const float *cr) {
+__device__ __forceinline__ void pairprep_whole(const PairPrepArgs &a, const BSeg *segs, int64_t g, const float *qr,
+                                               const float *cr) {
     const int lane = threadIdx.x & 63;
     const BatchedView &bv = a.bv;
     float4 r[NKT];
@@ -682,7 +683,7 @@ __device__ __forceinline__ void pairprep_whole(const PairPrepArgs &a, int64_t g,
     }
     float q2 = 0.0f;
     for (int s = 0; s < bv.nseg; ++s) {
-        const BSeg bs = bv.segs[s];
+        const BSeg bs = segs[s];
         float amax = 0.0f, sr = 0.0f, sq = 0.0f;
 #pragma unroll
         for (int k = 0; k < NKT; ++k) {
@@ -758,15 +759,20 @@ template <int NKT>
 __global__ __launch_bounds__(256) void bq_pairprep_kernel(PairPrepArgs a) {
     const int lane = threadIdx.x & 63;
     const int64_t g = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    // the segment descriptors in LDS: each segment's step then reads them without a global round trip
+    __shared__ BSeg seg_s[kMaxSeg];
+    for (int s = threadIdx.x; s < a.bv.nseg; s += 256) seg_s[s] = a.bv.segs[s];
+    __syncthreads();
     if (g >= *a.npairs) return;
-    const BatchedView &bv = a.bv;
+    BatchedView bv = a.bv;
+    bv.segs = seg_s;
     const int pair = a.pairs[g];
     const int64_t q = pair / a.NP;
     const int c = (int)a.probes[pair];
     const float *qr = a.qrot + q * bv.D_pad;
     const float *cr = bv.crot + (int64_t)c * bv.D_pad;
     if constexpr (NKT > 0) {
-        pairprep_whole<NKT>(a, g, qr, cr);
+        pairprep_whole<NKT>(a, bv.segs, g, qr, cr);
         return;
     }
     float q2 = 0.0f;
