@@ -853,8 +853,10 @@ __host__ __device__ constexpr int nvw_of(int qtm, int qs) {
     return qs >= 2 ? (qtm >= 2 ? 1 : 2) : (qtm == 1 ? 4 : (qtm <= 2 ? 2 : 1));
 }
 
-// candidate buffer places per row: 64 for 16-query tiles, fewer for larger tiles (LDS)
-__host__ __device__ constexpr int cand_cap(int qtm) { return 64 / qtm; }
+// candidate buffer places per row: 28 for 16-query tiles, fewer for larger tiles (LDS: a wave of
+// one int8 row tile at k = 10 and 4 quantised segments then takes 13.2 KB, and 12 waves fit a
+// CU's 160 KB)
+__host__ __device__ constexpr int cand_cap(int qtm) { return 28 / qtm; }
 
 // LDS of one wave (QTM query tiles of 16 rows): the areas' places from base into s, and the bytes.
 // s is a reference, never tested against null: a null test of a stack object's address keeps
@@ -871,7 +873,7 @@ __host__ __device__ inline size_t wave_layout(int nqs, int topk, uint8_t *base, 
     s.dt = (float *)take((size_t)QT * CVW * 4);
     s.dids = (uint32_t *)take(CVW * 4);
     s.ka = (float *)take((size_t)QT * nqs * 4);
-    s.kb = (float *)take((size_t)QT * nqs * 4);
+    s.kb = (float *)take(QS == 2 ? (size_t)QT * nqs * 4 : 0);
     s.kc = (float *)take((size_t)QT * nqs * 4);
     s.q2 = (float *)take(QT * 4);
     s.kd = (float *)take(QT * 4);
@@ -1054,9 +1056,12 @@ struct MmaOps {
     int aoff[QTM];
 };
 
-// One load-then-multiply step of a segment: the code operands of kKB 64-dim chunks (4 for
-// nibble codes, whose loads are half as wide) for all NVW mblocks and the matching query
-// operands, all in flight before the first MFMA waits.  A chunk past nk loads the last one again
+// 64-dim chunks per load step: 4 for nibble codes (loads half as wide), 2 for one int8 plane, 1
+// for two planes (the step's code operands then take 32 registers at 4 mblocks, as the others do)
+__host__ __device__ constexpr int chunks_per_step(int kind) { return kind == 0 ? 4 : (kind == 1 ? 2 : 1); }
+
+// One load-then-multiply step of a segment: the code operands of kKB 64-dim chunks for all NVW
+// mblocks and the matching query operands, all in flight before the first MFMA waits.  A chunk past nk loads the last one again
 // and skips its MFMAs (a uniform branch).  FIRST: the step's first MFMAs start from zero instead
 // of the accumulators.  KIND: 0 = 4-bit codes stored as nibbles, 1 = one int8 plane, 2 = a
 // shifted low plane and a high plane (9+ bits).
@@ -1064,7 +1069,7 @@ template <int KIND, int QTM, int QS, int NVW, bool FIRST>
 __device__ __forceinline__ void seg_step(const BSeg &bs, int kc0, int nk, const MmaOps<QTM, NVW> &o,
                                          v4i (&acc1)[QTM][NVW], v4i (&acc2)[QTM][NVW], v4i (&ach1)[QTM][NVW],
                                          v4i (&ach2)[QTM][NVW], int lane) {
-    constexpr int kKB = KIND == 0 ? 4 : 2;
+    constexpr int kKB = chunks_per_step(KIND);
     v4i A1[kKB][QTM], A2[kKB][QTM], B[kKB][NVW], BH[kKB][NVW];
     uint2 BN[kKB][NVW];
 #pragma unroll
@@ -1114,7 +1119,7 @@ template <int KIND, int QTM, int QS, int NVW>
 __device__ __forceinline__ void seg_mma(const BSeg &bs, const MmaOps<QTM, NVW> &o, v4i (&acc1)[QTM][NVW],
                                         v4i (&acc2)[QTM][NVW], v4i (&ach1)[QTM][NVW], v4i (&ach2)[QTM][NVW],
                                         int lane) {
-    constexpr int kKB = KIND == 0 ? 4 : 2;
+    constexpr int kKB = chunks_per_step(KIND);
     const int nk = bs.dim / 64;
     seg_step<KIND, QTM, QS, NVW, true>(bs, 0, nk, o, acc1, acc2, ach1, ach2, lane);
     for (int kc0 = kKB; kc0 < nk; kc0 += kKB) seg_step<KIND, QTM, QS, NVW, false>(bs, kc0, nk, o, acc1, acc2, ach1, ach2, lane);
@@ -1203,7 +1208,8 @@ __device__ __forceinline__ void seg_mma_bf16(const BSeg &bs, const uint8_t *cons
 }
 
 template <int W, int QTM, int QS, int KR>
-__global__ __launch_bounds__(64 * W, (W * QTM >= 8 || (QTM == 4 && QS == 2)) ? 1 : 2) void bscan_kernel(BScanArgs a) {
+__global__ __launch_bounds__(64 * W, (W * QTM >= 8 || (QTM == 4 && QS == 2)) ? 1 : (QS == 1 && QTM == 1 && W >= 2 ? 3 : 2))
+void bscan_kernel(BScanArgs a) {
     constexpr int QT = 16 * QTM;          // query rows of a wave
     constexpr int NVW = nvw_of(QTM, QS);  // mblocks per chunk
     constexpr int CVW = 16 * NVW;         // vectors per chunk
@@ -1244,7 +1250,7 @@ __global__ __launch_bounds__(64 * W, (W * QTM >= 8 || (QTM == 4 && QS == 2)) ? 1
             const int row = e % QT, s = e / QT;
             const float *pc = a.pc + ((int64_t)(first + min(row, cnt - 1)) * bv.nqs + s) * 3;
             m.ka[e] = pc[0];
-            m.kb[e] = pc[1];
+            if (QS == 2) m.kb[e] = pc[1];
             m.kc[e] = pc[2];
         }
         for (int row = lane; row < QT; row += 64) {
