@@ -1567,7 +1567,7 @@ int search_batched_locked(saqgpu_index *ix, const float *queries, int64_t nq, co
     rc = 0;
     rc |= ix->ws_qrot.ensure(nq * ix->D_pad);
     rc |= ix->ws_bound.ensure(nq * ix->nseg);
-    rc |= ix->bq_ws.ensure(4 * (size_t)K + 2 * npairs + 5 * max_tiles + 4 + nq);
+    rc |= ix->bq_ws.ensure(5 * (size_t)K + 2 * npairs + 5 * max_tiles + 4 + nq);
     rc |= ix->bq_qq.ensure((size_t)(qs == 1 ? 1 : 2) * npairs * ix->bq_kq);
     rc |= ix->bq_pc.ensure((size_t)npairs * (3 * ix->bq_nqs + 1));
     rc |= ix->bq_cand_d.ensure(max_cands * topk);
@@ -1632,6 +1632,7 @@ int search_batched_locked(saqgpu_index *ix, const float *queries, int64_t nq, co
     w.work = w.npairs + 1;
     w.sv = w.work + 1;
     w.qbound = reinterpret_cast<uint32_t *>(w.sv + 1);
+    w.tfirst = w.sv + 1 + nq;
     BqPairBufs pb;
     pb.qq1 = ix->bq_qq.p;
     pb.qq2 = qs == 2 ? ix->bq_qq.p + npairs * ix->bq_kq : nullptr;

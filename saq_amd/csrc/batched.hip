@@ -193,15 +193,13 @@ __global__ void bq_count_kernel(const uint32_t *probes, const int32_t *npq, int6
 // list cannot set the kernel's tail.  Then exclusive
 // prefix sums, over the lists in the given order (largest first, so the longest tiles start
 // first), of the pair counts, of the work tiles (query tiles x ranges) and of the candidate
-// lists (pairs x ranges), and every list's work tiles.  The bounds sum_c tiles_c <= npairs / QT
+// lists (pairs x ranges); bq_tiles_kernel writes the tiles themselves.  The bounds sum_c tiles_c <= npairs / QT
 // + K + kSplitWaves and sum_c cands_c <= npairs + kSplitWaves * QT follow from
 // sv >= W / kSplitWaves.
 __global__ __launch_bounds__(1024) void bq_scan_kernel(const int32_t *counts, const int32_t *order,
                                                        const int32_t *clu_n, int K, int QT, int workers, int32_t *cstart,
                                                        int32_t *candstart, int32_t *nsplit, int32_t *ntiles,
-                                                       int32_t *npairs, int32_t *sv_out, int32_t *tile_clu,
-                                                       int32_t *tile_first, int32_t *tile_cnt, int32_t *tile_v0,
-                                                       int32_t *tile_cand) {
+                                                       int32_t *npairs, int32_t *sv_out, int32_t *tfirst) {
     // thread t holds lists order[t * per .. t * per + per - 1] (per <= kScanRounds)
     __shared__ int wsum[3][16];
     __shared__ long long wk_s[16];
@@ -277,16 +275,8 @@ __global__ __launch_bounds__(1024) void bq_scan_kernel(const int32_t *counts, co
         cstart[c] = rc;
         candstart[c] = rk;
         nsplit[c] = ns[j];
-        // the list's work tiles: every query tile of QT of its pairs times every vector range;
-        // the candidate list of (pair slot i, range r) is candstart[c] + r * n + i
-        for (int i = 0; i < n; i += QT)
-            for (int r = 0; r < ns[j]; ++r, ++rt) {
-                tile_clu[rt] = c;
-                tile_first[rt] = rc + i;
-                tile_cnt[rt] = min(QT, n - i);
-                tile_v0[rt] = r * sv;
-                tile_cand[rt] = rk + r * n + i;
-            }
+        tfirst[t * per + j] = rt;
+        rt += (n + QT - 1) / QT * ns[j];
         rc += n;
         rk += n * ns[j];
     }
@@ -294,6 +284,36 @@ __global__ __launch_bounds__(1024) void bq_scan_kernel(const int32_t *counts, co
         *ntiles = tt;
         *npairs = tc;
         *sv_out = sv;
+    }
+}
+
+// Every work tile, one thread each: every query tile of QT of a list's pairs times every vector
+// range of the list, in that order from the list's first tile; the candidate list of (pair slot i,
+// range r) is candstart[c] + r * n + i.  The list holding tile rt is the last one in the scan
+// order whose first tile is at or before rt (a list without tiles shares its successor's first
+// tile, and the search passes over it).
+__global__ void bq_tiles_kernel(const int32_t *order, const int32_t *tfirst, int K, int QT, const int32_t *counts,
+                                const int32_t *cstart, const int32_t *candstart, const int32_t *nsplit,
+                                const int32_t *ntiles, const int32_t *sv_p, int32_t *tile_clu, int32_t *tile_first,
+                                int32_t *tile_cnt, int32_t *tile_v0, int32_t *tile_cand) {
+    const int nt = *ntiles, sv = *sv_p;
+    for (int rt = blockIdx.x * blockDim.x + threadIdx.x; rt < nt; rt += gridDim.x * blockDim.x) {
+        int lo = 0, hi = K;  // the first list whose first tile is after rt
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (tfirst[mid] <= rt)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        const int jj = lo - 1;  // >= 0: tfirst[0] = 0
+        const int c = order[jj], n = counts[c], ns = nsplit[c];
+        const int local = rt - tfirst[jj], qi = local / ns, r = local - qi * ns, i = qi * QT;
+        tile_clu[rt] = c;
+        tile_first[rt] = cstart[c] + i;
+        tile_cnt[rt] = min(QT, n - i);
+        tile_v0[rt] = r * sv;
+        tile_cand[rt] = candstart[c] + r * n + i;
     }
 }
 
@@ -1719,8 +1739,10 @@ hipError_t launch_bq_group(const uint32_t *probes, const int32_t *npq, int64_t n
     const unsigned g = (unsigned)((n + 255) / 256);
     hipLaunchKernelGGL(bq_count_kernel, dim3(g), dim3(256), 0, st, probes, npq, nq, NP, w.counts, w.slot);
     hipLaunchKernelGGL(bq_scan_kernel, dim3(1), dim3(1024), 0, st, w.counts, order, clu_n, K, QT, workers, w.cstart,
-                       w.candstart, w.nsplit, w.ntiles, w.npairs, w.sv, w.tile_clu, w.tile_first, w.tile_cnt,
-                       w.tile_v0, w.tile_cand);
+                       w.candstart, w.nsplit, w.ntiles, w.npairs, w.sv, w.tfirst);
+    hipLaunchKernelGGL(bq_tiles_kernel, dim3(256), dim3(256), 0, st, order, w.tfirst, K, QT, w.counts, w.cstart,
+                       w.candstart, w.nsplit, w.ntiles, w.sv, w.tile_clu, w.tile_first, w.tile_cnt, w.tile_v0,
+                       w.tile_cand);
     hipLaunchKernelGGL(bq_scatter_kernel, dim3(g), dim3(256), 0, st, probes, w.slot, n, w.cstart, w.pairs);
     return hipGetLastError();
 }

@@ -42,6 +42,7 @@ struct BqWork {
     int32_t *counts, *slot, *cstart, *candstart, *nsplit, *ntiles, *npairs, *sv, *pairs;
     int32_t *tile_clu, *tile_first, *tile_cnt, *tile_v0, *tile_cand, *work;
     uint32_t *qbound;  // per query: the distance bound its tiles share (nq entries)
+    int32_t *tfirst;   // per list in the scan order: its first work tile
 };
 // Upper bounds of the work tiles and candidate lists of a launch.
 inline int64_t bq_max_tiles(int64_t npairs, int K) { return npairs + K + kSplitWaves; }
