@@ -7,9 +7,9 @@ namespace saqgpu {
 constexpr int kMB = 16;     // vectors per MFMA column block (one mblock record)
 // Vector ranges of a list's work tiles: at least kMinSplit vectors, at most about the launch's
 // work / (kSplitWaves / waves per workgroup) (kSplitWaves: the resident wavefronts of a 256-CU
-// chip at 2 waves per SIMD).
+// chip at 3 waves per SIMD, the int8 tile kernels' occupancy).
 constexpr int kMinSplit = 512;
-constexpr int kSplitWaves = 2048;
+constexpr int kSplitWaves = 3072;
 // Lists the grouping scan holds in registers: 1024 threads x kScanRounds each (the nlist limit of the
 // batched scan).
 constexpr int kScanRounds = 16;
