@@ -26,6 +26,7 @@
 // grouped into tiles); merge_kernel then merges each query's per-list lists.
 #include <algorithm>
 #include <cfloat>
+#include <type_traits>
 
 #include "batched.hpp"
 
@@ -1361,47 +1362,59 @@ void bscan_kernel(BScanArgs a) {
                     for (int t = 0; t < QTM; ++t)
 #pragma unroll
                         for (int j = 0; j < NVW; ++j) facc[t][j] = fach[t][j] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
-                const bool two = bs.bits > 8;
-                if (QS == 3) {
-                    if (bs.bits <= 4)
-                        seg_mma_bf16<0, QTM, NVW>(bs, rec, arowb, facc, fach, lane);
-                    else if (!two)
-                        seg_mma_bf16<1, QTM, NVW>(bs, rec, arowb, facc, fach, lane);
-                    else
-                        seg_mma_bf16<2, QTM, NVW>(bs, rec, arowb, facc, fach, lane);
-                } else if (bs.bits <= 4) {
-                    seg_mma<0, QTM, QS, NVW>(bs, ops, acc1, acc2, ach1, ach2, lane);
-                } else if (!two) {
-                    seg_mma<1, QTM, QS, NVW>(bs, ops, acc1, acc2, ach1, ach2, lane);
-                } else {
-                    seg_mma<2, QTM, QS, NVW>(bs, ops, acc1, acc2, ach1, ach2, lane);
-                }
                 // epilogue of the segment: ip = a1 acc1 (+ 256 a1 acch1) (+ a2 acc2 ...) + c, then
-                // dist += (-2 rescale) ip, all fused multiply-adds
+                // dist += (-2 rescale) ip, all fused multiply-adds.  TWO (a high plane) is a
+                // template constant: the one-plane form carries none of its instructions.
+                auto epilogue = [&](auto two_c) {
+                    constexpr bool TWO = decltype(two_c)::value;
 #pragma unroll
-                for (int t = 0; t < QTM; ++t) {
-                    const int rb = bs.qs * QT + t * 16 + 4 * (lane / 16);
-                    const float4 ka = *reinterpret_cast<const float4 *>(m.ka + rb);
-                    const float4 kc = *reinterpret_cast<const float4 *>(m.kc + rb);
-                    float4 kb = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                    if (QS == 2) kb = *reinterpret_cast<const float4 *>(m.kb + rb);
-                    const float av[4] = {ka.x, ka.y, ka.z, ka.w}, bv2[4] = {kb.x, kb.y, kb.z, kb.w};
-                    const float cv[4] = {kc.x, kc.y, kc.z, kc.w};
+                    for (int t = 0; t < QTM; ++t) {
+                        const int rb = bs.qs * QT + t * 16 + 4 * (lane / 16);
+                        const float4 ka = *reinterpret_cast<const float4 *>(m.ka + rb);
+                        const float4 kc = *reinterpret_cast<const float4 *>(m.kc + rb);
+                        float4 kb = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                        if (QS == 2) kb = *reinterpret_cast<const float4 *>(m.kb + rb);
+                        const float av[4] = {ka.x, ka.y, ka.z, ka.w}, bv2[4] = {kb.x, kb.y, kb.z, kb.w};
+                        const float cv[4] = {kc.x, kc.y, kc.z, kc.w};
 #pragma unroll
-                    for (int j = 0; j < NVW; ++j) {
-                        const float m2r = fac[j * fac_words + kMB * (1 + bs.qs) + lane % kMB];
+                        for (int j = 0; j < NVW; ++j) {
+                            const float m2r = fac[j * fac_words + kMB * (1 + bs.qs) + lane % kMB];
 #pragma unroll
-                        for (int r = 0; r < 4; ++r) {
-                            const float x1 = QS == 3 ? facc[t][j][r] : (float)acc1[t][j][r];
-                            float ip = fmaf(av[r], x1, cv[r]);
-                            if (two) ip = fmaf(256.0f * av[r], QS == 3 ? fach[t][j][r] : (float)ach1[t][j][r], ip);
-                            if (QS == 2) {
-                                ip = fmaf(bv2[r], (float)acc2[t][j][r], ip);
-                                if (two) ip = fmaf(256.0f * bv2[r], (float)ach2[t][j][r], ip);
+                            for (int r = 0; r < 4; ++r) {
+                                const float x1 = QS == 3 ? facc[t][j][r] : (float)acc1[t][j][r];
+                                float ip = fmaf(av[r], x1, cv[r]);
+                                if (TWO) ip = fmaf(256.0f * av[r], QS == 3 ? fach[t][j][r] : (float)ach1[t][j][r], ip);
+                                if (QS == 2) {
+                                    ip = fmaf(bv2[r], (float)acc2[t][j][r], ip);
+                                    if (TWO) ip = fmaf(256.0f * bv2[r], (float)ach2[t][j][r], ip);
+                                }
+                                dist[t][j][r] = fmaf(m2r, ip, dist[t][j][r]);
                             }
-                            dist[t][j][r] = fmaf(m2r, ip, dist[t][j][r]);
                         }
                     }
+                };
+                using One = std::integral_constant<bool, false>;
+                using Two = std::integral_constant<bool, true>;
+                if (QS == 3) {
+                    if (bs.bits <= 4) {
+                        seg_mma_bf16<0, QTM, NVW>(bs, rec, arowb, facc, fach, lane);
+                        epilogue(One{});
+                    } else if (bs.bits <= 8) {
+                        seg_mma_bf16<1, QTM, NVW>(bs, rec, arowb, facc, fach, lane);
+                        epilogue(One{});
+                    } else {
+                        seg_mma_bf16<2, QTM, NVW>(bs, rec, arowb, facc, fach, lane);
+                        epilogue(Two{});
+                    }
+                } else if (bs.bits <= 4) {
+                    seg_mma<0, QTM, QS, NVW>(bs, ops, acc1, acc2, ach1, ach2, lane);
+                    epilogue(One{});
+                } else if (bs.bits <= 8) {
+                    seg_mma<1, QTM, QS, NVW>(bs, ops, acc1, acc2, ach1, ach2, lane);
+                    epilogue(One{});
+                } else {
+                    seg_mma<2, QTM, QS, NVW>(bs, ops, acc1, acc2, ach1, ach2, lane);
+                    epilogue(Two{});
                 }
             }
             // ---- rows (queries) with a distance of this chunk at or below their k-th distance: the
