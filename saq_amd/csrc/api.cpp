@@ -1639,10 +1639,25 @@ int search_batched_locked(saqgpu_index *ix, const float *queries, int64_t nq, co
     pb.pc = ix->bq_pc.p;
     pb.pq2 = ix->bq_pc.p + npairs * 3 * ix->bq_nqs;
     const int grid = (int)std::min<int64_t>(max_tiles, 1 << 20);
-    // the step's device work (also what a graph captures): coarse quantiser, query rotation,
-    // grouping, pair prep, tile kernel, merge
+    // a second stream for the query rotation (the plain scan's first group stream)
+    if (ix->sl_groups.empty()) {
+        saqgpu_index::SliceGroup g{};
+        HIPCHK(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&g.done, hipEventDisableTiming));
+        ix->sl_groups.push_back(g);
+    }
+    if (!ix->sl_fork) HIPCHK(hipEventCreateWithFlags(&ix->sl_fork, hipEventDisableTiming));
+    const hipStream_t aux = ix->sl_groups[0].stream;
+    const hipEvent_t aux_done = ix->sl_groups[0].done;
+    // the step's device work (also what a graph captures): query rotation on the second stream
+    // beside the coarse quantiser and the grouping, then pair prep, tile kernel, merge
     auto enqueue = [&](bool timed) -> int {
         int rc2;
+        HIPCHK(hipEventRecord(ix->sl_fork, st));
+        HIPCHK(hipStreamWaitEvent(aux, ix->sl_fork, 0));
+        HIPCHK(launch_query_prep(ix->view(), dq, nq, ix->variance.p, ix->has_rot ? ix->rot.p : nullptr,
+                                 ix->d_rot_off.p, 4.0f, ix->ws_qrot.p, ix->ws_bound.p, ix->max_dim, aux));
+        HIPCHK(hipEventRecord(aux_done, aux));
         if (!probes) {
             if (p->exact_probe) {
                 if ((rc2 = exact_probe(ix, dq, nq, NP, 0, ix->ws_probes.p, st))) return rc2;
@@ -1651,9 +1666,8 @@ int search_batched_locked(saqgpu_index *ix, const float *queries, int64_t nq, co
             }
             dp = ix->ws_probes.p;
         }
-        HIPCHK(launch_query_prep(ix->view(), dq, nq, ix->variance.p, ix->has_rot ? ix->rot.p : nullptr,
-                                 ix->d_rot_off.p, 4.0f, ix->ws_qrot.p, ix->ws_bound.p, ix->max_dim, st));
         HIPCHK(launch_bq_group(dp, dnpq, nq, NP, K, query_tile, qs, ix->d_bq_order.p, ix->d_clu_n.p, w, st));
+        HIPCHK(hipStreamWaitEvent(st, aux_done, 0));
         HIPCHK(launch_bq_pairprep(bv, ix->ws_qrot.p, dp, w, pb, NP, npairs, qs, st));
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (timed) {
