@@ -837,14 +837,15 @@ __device__ void heap_adjust(float *hd, uint32_t *hi, int hole, int len, float vd
     hi[hole] = vi;
 }
 
-constexpr int kHeapChunk = 2048;  // distances staged in LDS per step of the selection
-
 // One workgroup per query; queries without a tie flag return at once.  recompute: dist holds GEMM
 // products, replaced first by the reference's distances (Eigen order, by half-waves) of every
 // centroid; otherwise dist already holds them.  Then one thread runs std::partial_sort (GCC 11
 // libstdc++: __heap_select = __make_heap of [0, nprobe) + __pop_heap of every later element that
 // compares below the top, then __sort_heap) over the candidates (id c, distance dist[c]) in id
-// order, the distances staged through LDS a chunk at a time by the whole workgroup.
+// order.  The top of the heap only moves one way (down for L2; up for IP, whose comparator is
+// !(a < b)), so a candidate that does not compare below the top at the start of its 256-candidate
+// chunk never would later: the workgroup keeps those out, in id order, and one thread runs
+// __pop_heap for the rest.
 template <int DT>
 __global__ __launch_bounds__(256) void probe_heap_kernel(float *__restrict__ dist, int recompute,
                                                          const float *__restrict__ cent, int K, int D,
@@ -853,7 +854,9 @@ __global__ __launch_bounds__(256) void probe_heap_kernel(float *__restrict__ dis
                                                          uint32_t *__restrict__ probes) {
     __shared__ float hd[kSelMax];
     __shared__ uint32_t hi[kSelMax];
-    __shared__ float chunk[kHeapChunk];
+    __shared__ float cd[256];
+    __shared__ uint32_t ci[256];
+    __shared__ int wcount[4];
     const int64_t q = blockIdx.x;
     if (!ties[q]) return;  // uniform per workgroup
     const int tid = threadIdx.x;
@@ -876,27 +879,39 @@ __global__ __launch_bounds__(256) void probe_heap_kernel(float *__restrict__ dis
         }
         __syncthreads();  // the row's writes are visible to the workgroup
     }
-    for (int c0 = 0; c0 < K; c0 += kHeapChunk) {
-        const int cn = min(kHeapChunk, K - c0);
-        for (int i = tid; i < cn; i += 256) chunk[i] = row[c0 + i];
-        __syncthreads();
-        if (tid == 0) {
-            for (int i = 0; i < cn; ++i) {
-                const int c = c0 + i;
-                const float d = chunk[i];
-                if (c < nprobe) {  // the first nprobe candidates in place
-                    hd[c] = d;
-                    hi[c] = (uint32_t)c;
-                    if (c == nprobe - 1 && nprobe >= 2)  // __make_heap
-                        for (int parent = (nprobe - 2) / 2;; --parent) {
-                            heap_adjust<DT>(hd, hi, parent, nprobe, hd[parent], hi[parent]);
-                            if (parent == 0) break;
-                        }
-                } else if (cand_less<DT>(d, hd[0])) {  // __pop_heap(first, middle, i)
-                    heap_adjust<DT>(hd, hi, 0, nprobe, d, (uint32_t)c);
-                }
-            }
+    // __make_heap of the first nprobe candidates
+    for (int c = tid; c < nprobe; c += 256) {
+        hd[c] = row[c];
+        hi[c] = (uint32_t)c;
+    }
+    __syncthreads();
+    if (tid == 0 && nprobe >= 2)
+        for (int parent = (nprobe - 2) / 2;; --parent) {
+            heap_adjust<DT>(hd, hi, parent, nprobe, hd[parent], hi[parent]);
+            if (parent == 0) break;
         }
+    __syncthreads();
+    // __pop_heap(first, middle, i) for every later candidate that compares below the top
+    const int lane = tid & 63, w = tid >> 6;
+    for (int c0 = nprobe; c0 < K; c0 += 256) {
+        const int c = c0 + tid;
+        const float d = c < K ? row[c] : 0.0f;
+        const bool pass = c < K && cand_less<DT>(d, hd[0]);
+        const unsigned long long b = __ballot(pass);
+        if (lane == 0) wcount[w] = __popcll(b);
+        __syncthreads();
+        int off = 0;
+        for (int i = 0; i < w; ++i) off += wcount[i];
+        const int n = wcount[0] + wcount[1] + wcount[2] + wcount[3];
+        if (pass) {
+            const int pos = off + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(b >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0));
+            cd[pos] = d;
+            ci[pos] = (uint32_t)c;
+        }
+        __syncthreads();
+        if (tid == 0)
+            for (int i = 0; i < n; ++i)
+                if (cand_less<DT>(cd[i], hd[0])) heap_adjust<DT>(hd, hi, 0, nprobe, cd[i], ci[i]);
         __syncthreads();
     }
     if (tid == 0)
