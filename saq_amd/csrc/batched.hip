@@ -817,7 +817,7 @@ __global__ __launch_bounds__(256) void bq_pairprep_kernel(PairPrepArgs a) {
 // rows 16 QTM w .. 16 QTM (w + 1) - 1 and walks the same vector range as its sibling waves, so
 // the W waves read every code chunk of the range at about the same time from the same CU: one
 // HBM read serves the workgroup's W * 16 * QTM queries, the other waves' reads hitting the caches.
-// Per chunk of 16*NVW vectors and per quantised segment, the B operands of 2-4 64-dim chunks for
+// Per chunk of 16*NVW vectors and per quantised segment, the B operands of 1-4 64-dim chunks for
 // all NVW mblocks are loaded together, then the MFMAs run; each segment's int32 sums fold into the
 // fp32 distances with three fused multiply-adds per (query, vector); after the segments the
 // chunk's distances go through the wave's LDS to the per-query top-k, which a query only touches
@@ -867,8 +867,8 @@ struct WSmem {
 
 // mblocks per wave chunk.  A wave re-reads its query operands (16 rows x sum D_s int8 per row
 // tile) from global memory for every chunk, so the longer the chunk the fewer of those reads:
-// int8 waves of one row tile take 64 vectors (196 VGPRs, 2 waves per SIMD; 32-query tiles 10.4
-// ms against 11.9 ms for the best 32-vector form on 10M x 1024), waves of two row tiles 32;
+// int8 waves of one row tile take 64 vectors (at 2 waves per SIMD, 32-query tiles 10.4 ms against
+// 11.9 ms for the best 32-vector form on 10M x 1024; now 139 VGPRs, 3 waves), two row tiles 32;
 // int8x2 (a second term) and BF16 (query operands twice as wide) take 32, or 16 at two row tiles
 __host__ __device__ constexpr int nvw_of(int qtm, int qs) {
     return qs >= 2 ? (qtm >= 2 ? 1 : 2) : (qtm == 1 ? 4 : (qtm <= 2 ? 2 : 1));
@@ -1082,9 +1082,9 @@ struct MmaOps {
 __host__ __device__ constexpr int chunks_per_step(int kind) { return kind == 0 ? 4 : (kind == 1 ? 2 : 1); }
 
 // One load-then-multiply step of a segment: the code operands of kKB 64-dim chunks for all NVW
-// mblocks and the matching query operands, all in flight before the first MFMA waits.  A chunk past nk loads the last one again
-// and skips its MFMAs (a uniform branch).  FIRST: the step's first MFMAs start from zero instead
-// of the accumulators.  KIND: 0 = 4-bit codes stored as nibbles, 1 = one int8 plane, 2 = a
+// mblocks and the matching query operands, all in flight before the first MFMA waits.  A chunk
+// past nk loads the last one again and skips its MFMAs (a uniform branch).  FIRST: the step's
+// first MFMAs start from zero instead of the accumulators.  KIND: 0 = 4-bit codes stored as nibbles, 1 = one int8 plane, 2 = a
 // shifted low plane and a high plane (9+ bits).
 template <int KIND, int QTM, int QS, int NVW, bool FIRST>
 __device__ __forceinline__ void seg_step(const BSeg &bs, int kc0, int nk, const MmaOps<QTM, NVW> &o,
@@ -1113,6 +1113,8 @@ __device__ __forceinline__ void seg_step(const BSeg &bs, int kc0, int nk, const 
             }
         }
     }
+    // keep the scheduler from interleaving the step's loads with its MFMAs: all of them are in
+    // flight before the first MFMA waits
     __builtin_amdgcn_sched_barrier(0);
     const v4i z = {0, 0, 0, 0};
 #pragma unroll
