@@ -342,14 +342,17 @@ class Index:
         """IVF::estimate for one query (host array of num_dim floats): every vector of the
         nprobe probed lists in probe order.  Returns (ids, acc, fast, vars)."""
         query = np.ascontiguousarray(query, np.float32).ravel()
-        if probes is None:
-            probes = self.probe(query[None, :], nprobe, dist_type=dist_type)[0]
-        probes = np.ascontiguousarray(probes, np.uint32)[:nprobe]
-        total = int(sum(int(self.cluster_sizes[c]) for c in probes))
-        ids = np.zeros(max(1, total), np.uint32)
-        acc, fast, var = (np.zeros(max(1, total), np.float32) for _ in range(3))
+        if probes is not None:
+            probes = np.ascontiguousarray(probes, np.uint32)[:nprobe]
+            total = int(sum(int(self.cluster_sizes[c]) for c in probes))
+        else:  # the library probes (FlatInitializer's lists); room for the nprobe largest lists
+            if getattr(self, "_sizes_desc", None) is None:
+                self._sizes_desc = np.cumsum(np.sort(np.asarray(self.cluster_sizes, np.int64))[::-1])
+            total = int(self._sizes_desc[min(nprobe, len(self._sizes_desc)) - 1])
+        ids = np.empty(max(1, total), np.uint32)
+        acc, fast, var = (np.empty(max(1, total), np.float32) for _ in range(3))
         n = C.c_int64(0)
-        _check(lib().saqgpu_estimate(self.h, query.ctypes.data, probes.ctypes.data, nprobe, m, dist_type,
+        _check(lib().saqgpu_estimate(self.h, query.ctypes.data, _ptr(probes), nprobe, m, dist_type,
                                      ids.ctypes.data, acc.ctypes.data, fast.ctypes.data, var.ctypes.data,
                                      C.byref(n)))
         k = n.value
